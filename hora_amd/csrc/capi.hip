@@ -1,0 +1,246 @@
+// capi.hip -- the extern "C" boundary (include/hora_mi355x.h).
+//
+// Each entry translates C++ exceptions into an hm_status plus a
+// thread-local message, so no error crosses the ABI as an exception or a
+// process abort (the reference panics in the same situations).
+#include <cstring>
+#include <vector>
+
+#include "index.h"
+
+using namespace hm;
+
+namespace {
+
+template <class F> int32_t guard(F &&f)
+{
+    try {
+        f();
+        return HM_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return HM_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return HM_ERR_INVALID_ARG;
+    }
+}
+
+void need(const void *p, const char *what)
+{
+    if (!p) fail(HM_ERR_INVALID_ARG, std::string(what) + " is null");
+}
+
+// Binds the calling thread to the index's device for the duration of a call.
+struct DeviceScope {
+    int prev = -1;
+    explicit DeviceScope(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) HM_HIP(hipSetDevice(dev));
+    }
+    ~DeviceScope()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+HM_API int32_t hm_params_default(int32_t kind, hm_params *out)
+{
+    return guard([&] {
+        need(out, "out");
+        std::memset(out, 0, sizeof(*out));
+        if (kind == HM_INDEX_IVFPQ) {
+            // IVFPQParams::default (pq_params.rs:80-91)
+            out->n_sub = 25;
+            out->sub_bits = 4;
+            out->n_kmeans_center = 256;
+            out->search_n_center = 8;
+            out->train_epoch = 100;
+            out->kmeans_accumulate = 0;
+            out->seed = 0x5EEDull;
+        } else if (kind != HM_INDEX_BRUTE_FORCE) {
+            fail(HM_ERR_INVALID_ARG, "unknown index kind");
+        }
+    });
+}
+
+HM_API int32_t hm_index_create(int32_t kind, uint64_t dimension, const hm_params *params,
+                               hm_index **out)
+{
+    return guard([&] {
+        need(out, "out");
+        *out = nullptr;
+        int dev = 0;
+        HM_HIP(hipGetDevice(&dev));
+        auto h = std::make_unique<hm_index>();
+        h->device = dev;
+        if (kind == HM_INDEX_BRUTE_FORCE) {
+            h->impl = make_bruteforce(dimension);
+        } else if (kind == HM_INDEX_IVFPQ) {
+            hm_params p;
+            if (params) {
+                p = *params;
+            } else {
+                hm_params_default(HM_INDEX_IVFPQ, &p);
+            }
+            h->impl = make_ivfpq(dimension, p);
+        } else {
+            fail(HM_ERR_INVALID_ARG, "unknown index kind");
+        }
+        *out = h.release();
+    });
+}
+
+HM_API int32_t hm_index_destroy(hm_index *h)
+{
+    return guard([&] {
+        if (!h) return;
+        {
+            DeviceScope ds(h->device);
+            delete h;
+        }
+    });
+}
+
+HM_API int32_t hm_index_add(hm_index *h, const float *rows, uint64_t n_rows)
+{
+    return guard([&] {
+        need(h, "index");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        h->impl->add(rows, n_rows);
+    });
+}
+
+HM_API int32_t hm_index_build(hm_index *h, int32_t metric)
+{
+    return guard([&] {
+        need(h, "index");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        h->impl->build(metric);
+    });
+}
+
+HM_API int32_t hm_index_built(const hm_index *h, int32_t *built)
+{
+    return guard([&] {
+        need(h, "index");
+        need(built, "built");
+        *built = h->impl->built() ? 1 : 0;
+    });
+}
+
+HM_API int32_t hm_index_search(hm_index *h, const float *queries, uint64_t nq, uint32_t k,
+                               uint64_t *idx, float *dist, uint32_t *count)
+{
+    return guard([&] {
+        need(h, "index");
+        if (nq == 0) return;
+        need(queries, "queries");
+        need(count, "count");
+        if (k) {
+            need(idx, "idx");
+            need(dist, "dist");
+        }
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        const uint64_t d = h->impl->dimension();
+        check_finite_host(queries, nq * d);
+        hipStream_t s = h->impl->stream();
+        h->q_stage.reserve(nq * d * sizeof(float));
+        h->idx_stage.reserve(nq * std::max<uint32_t>(k, 1) * sizeof(uint64_t));
+        h->dist_stage.reserve(nq * std::max<uint32_t>(k, 1) * sizeof(float));
+        h->count_stage.reserve(nq * sizeof(uint32_t));
+        HM_HIP(hipMemcpyAsync(h->q_stage.get(), queries, nq * d * sizeof(float),
+                              hipMemcpyHostToDevice, s));
+        h->impl->search_device(h->q_stage.as<float>(), nq, k, h->idx_stage.as<uint64_t>(),
+                               h->dist_stage.as<float>(), h->count_stage.as<uint32_t>(), s);
+        if (k) {
+            HM_HIP(hipMemcpyAsync(idx, h->idx_stage.get(), nq * k * sizeof(uint64_t),
+                                  hipMemcpyDeviceToHost, s));
+            HM_HIP(hipMemcpyAsync(dist, h->dist_stage.get(), nq * k * sizeof(float),
+                                  hipMemcpyDeviceToHost, s));
+        }
+        HM_HIP(hipMemcpyAsync(count, h->count_stage.get(), nq * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, s));
+        HM_HIP(hipStreamSynchronize(s));
+    });
+}
+
+HM_API int32_t hm_index_search_device(hm_index *h, const float *d_queries, uint64_t nq,
+                                      uint32_t k, uint64_t *d_idx, float *d_dist,
+                                      uint32_t *d_count, void *stream)
+{
+    return guard([&] {
+        need(h, "index");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        h->impl->search_device(d_queries, nq, k, d_idx, d_dist, d_count,
+                               static_cast<hipStream_t>(stream));
+    });
+}
+
+HM_API int32_t hm_index_set_search_mode(hm_index *h, int32_t mode, uint32_t shortlist)
+{
+    return guard([&] {
+        need(h, "index");
+        std::lock_guard<std::mutex> lk(h->mu);
+        h->impl->set_search_mode(mode, shortlist);
+    });
+}
+
+HM_API int32_t hm_index_nodes_size(const hm_index *h, uint64_t *out)
+{
+    return guard([&] {
+        need(h, "index");
+        need(out, "out");
+        *out = h->impl->nodes_size();
+    });
+}
+
+HM_API int32_t hm_index_dimension(const hm_index *h, uint64_t *out)
+{
+    return guard([&] {
+        need(h, "index");
+        need(out, "out");
+        *out = h->impl->dimension();
+    });
+}
+
+HM_API const char *hm_index_name(const hm_index *h) { return h ? h->impl->name() : ""; }
+
+HM_API int32_t hm_index_clear(hm_index *h)
+{
+    return guard([&] {
+        need(h, "index");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        h->impl->clear();
+    });
+}
+
+HM_API int32_t hm_index_get_rows(const hm_index *h, const uint64_t *idx, uint64_t n, float *out)
+{
+    return guard([&] {
+        need(h, "index");
+        if (n == 0) return;
+        need(idx, "idx");
+        need(out, "out");
+        DeviceScope ds(h->device);
+        h->impl->get_rows(idx, n, out);
+    });
+}
+
+HM_API const char *hm_last_error(void) { return last_error(); }
+
+}  // extern "C"

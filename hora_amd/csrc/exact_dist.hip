@@ -1,0 +1,234 @@
+// exact_dist.hip -- reference-order FP32 distance tiles for gfx950.
+//
+// Every (query, base) pair is accumulated by one lane, strictly in dimension
+// order, with no contraction into FMA (`fp contract(off)` below and the
+// build's -ffp-contract=off), so each value equals the reference's scalar
+// left fold bit for bit:
+//   Euclidean  acc = acc + (q_t - b_t)*(q_t - b_t)   simd_metrics.rs:96-102
+//   Manhattan  acc = acc + |q_t - b_t|                simd_metrics.rs:60-66
+//   Dot        acc = acc + q_t*b_t                    simd_metrics.rs:30-33
+// Cosine and Angular are formed in the epilogue from the dot sum and the
+// per-vector sequential sums s_qq, s_bb (metrics.rs:73-103).
+//
+// Tiling: a 256-thread workgroup owns a 128-query x 128-base tile; each lane
+// holds an 8x8 register micro-tile, and dimension slabs of 16 are staged
+// through LDS (transposed, double buffered) so every LDS read is a 16-byte
+// broadcast-friendly load.  The work is VALU-bound (no MFMA: the products
+// would be contracted and reordered), so the tile is sized for register
+// reuse: 16 LDS bytes feed 24-48 VALU operations.
+#include "kernels.h"
+#include "profiler.h"
+
+#pragma clang fp contract(off)
+
+namespace hm {
+
+namespace {
+
+constexpr int TQ = 128;   // queries per workgroup tile
+constexpr int TN = 128;   // base rows per workgroup tile
+constexpr int TK = 16;    // dimensions per LDS slab
+constexpr int LDS_PAD = 4;
+
+template <int ACC>
+__device__ __forceinline__ float acc_step(float acc, float q, float b)
+{
+    if constexpr (ACC == ACC_EUCLID) {
+        float d = q - b;
+        float s = d * d;
+        return acc + s;
+    } else if constexpr (ACC == ACC_MANHATTAN) {
+        float d = q - b;
+        return acc + fabsf(d);
+    } else {
+        float p = q * b;
+        return acc + p;
+    }
+}
+
+// metrics.rs:73-83 in the build's definition: -s_qb / (sqrt(s_qq)*sqrt(s_bb)).
+__device__ __forceinline__ float epi_cosine(float sqb, float sqq, float sbb)
+{
+    float den = sqrtf(sqq) * sqrtf(sbb);
+    return (-sqb) / den;
+}
+
+// metrics.rs:88-103 with float_two() == 1.0 (node.rs:54-56).
+__device__ __forceinline__ float epi_angular(float sqb, float sqq, float sbb)
+{
+    float rhd = -sqq, lhd = -sbb, rldot = -sqb;
+    float rlmul = rhd * lhd;
+    if (rlmul > 0.0f) {
+        float t = 1.0f * rldot;
+        t = t / sqrtf(rlmul);
+        return 1.0f - t;
+    }
+    return 1.0f;
+}
+
+template <int ACC, int EPI>
+__global__ __launch_bounds__(256, 2) void exact_tile_kernel(
+    const float *__restrict__ Q, uint32_t ldq, uint32_t nq, const float *__restrict__ B,
+    uint32_t ldb, uint32_t n, uint32_t dpad, const float *__restrict__ qnorm,
+    const float *__restrict__ bnorm, float *__restrict__ D, uint64_t ldd)
+{
+    __shared__ __align__(16) float sQ[2][TK][TQ + LDS_PAD];
+    __shared__ __align__(16) float sB[2][TK][TN + LDS_PAD];
+
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const uint32_t q0 = blockIdx.y * TQ, n0 = blockIdx.x * TN;
+
+    float acc[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = -0.0f;  // Rust's f32 Sum starts at -0.0
+
+    float4 rq[2], rb[2];
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+
+    auto gload = [&](uint32_t k0) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            int idx = tid + 256 * s;
+            int row = idx >> 2, c4 = idx & 3;
+            uint32_t qr = q0 + row, br = n0 + row;
+            rq[s] = qr < nq ? *reinterpret_cast<const float4 *>(Q + (size_t)qr * ldq + k0 + c4 * 4)
+                            : zero4;
+            rb[s] = br < n ? *reinterpret_cast<const float4 *>(B + (size_t)br * ldb + k0 + c4 * 4)
+                           : zero4;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            int idx = tid + 256 * s;
+            int row = idx >> 2, c4 = idx & 3;
+            sQ[buf][c4 * 4 + 0][row] = rq[s].x;
+            sQ[buf][c4 * 4 + 1][row] = rq[s].y;
+            sQ[buf][c4 * 4 + 2][row] = rq[s].z;
+            sQ[buf][c4 * 4 + 3][row] = rq[s].w;
+            sB[buf][c4 * 4 + 0][row] = rb[s].x;
+            sB[buf][c4 * 4 + 1][row] = rb[s].y;
+            sB[buf][c4 * 4 + 2][row] = rb[s].z;
+            sB[buf][c4 * 4 + 3][row] = rb[s].w;
+        }
+    };
+
+    const uint32_t nk = dpad / TK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (uint32_t kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * TK);
+#pragma unroll 2
+        for (int k = 0; k < TK; ++k) {
+            float4 qa = *reinterpret_cast<const float4 *>(&sQ[cur][k][ty * 8]);
+            float4 qb = *reinterpret_cast<const float4 *>(&sQ[cur][k][ty * 8 + 4]);
+            float4 ba = *reinterpret_cast<const float4 *>(&sB[cur][k][tx * 8]);
+            float4 bb = *reinterpret_cast<const float4 *>(&sB[cur][k][tx * 8 + 4]);
+            const float qv[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+            const float bv[8] = {ba.x, ba.y, ba.z, ba.w, bb.x, bb.y, bb.z, bb.w};
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) acc[i][j] = acc_step<ACC>(acc[i][j], qv[i], bv[j]);
+        }
+        if (kt + 1 < nk) sstore(cur ^ 1);
+        __syncthreads();
+    }
+
+    const uint32_t bj0 = n0 + tx * 8;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint32_t qi = q0 + ty * 8 + i;
+        if (qi >= nq) continue;
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            float a = acc[i][j];
+            if constexpr (EPI == EPI_COSINE) {
+                uint32_t bj = bj0 + j < n ? bj0 + j : n - 1;
+                a = epi_cosine(a, qnorm[qi], bnorm[bj]);
+            } else if constexpr (EPI == EPI_ANGULAR) {
+                uint32_t bj = bj0 + j < n ? bj0 + j : n - 1;
+                a = epi_angular(a, qnorm[qi], bnorm[bj]);
+            }
+            v[j] = a;
+        }
+        float *drow = D + (size_t)qi * ldd;
+        if (bj0 + 7 < n) {
+            *reinterpret_cast<float4 *>(drow + bj0) = make_float4(v[0], v[1], v[2], v[3]);
+            *reinterpret_cast<float4 *>(drow + bj0 + 4) = make_float4(v[4], v[5], v[6], v[7]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (bj0 + j < n) drow[bj0 + j] = v[j];
+        }
+    }
+}
+
+// s = sum_t x_t*x_t as a left fold from -0.0 (the reference's dot(x,x) up to
+// sign, calc.rs:13-18), one lane per row.
+__global__ void self_dot_kernel(const float *__restrict__ X, uint32_t ldx, uint32_t rows,
+                                uint32_t d, float *__restrict__ out)
+{
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const float *x = X + (size_t)r * ldx;
+    float acc = -0.0f;
+    for (uint32_t t = 0; t < d; ++t) {
+        float p = x[t] * x[t];
+        acc = acc + p;
+    }
+    out[r] = acc;
+}
+
+}  // namespace
+
+void launch_exact_tile(int metric, const float *Q, uint32_t ldq, uint32_t nq, const float *B,
+                       uint32_t ldb, uint32_t n, uint32_t dpad, const float *qnorm,
+                       const float *bnorm, float *D, uint64_t ldd, hipStream_t s)
+{
+    if (nq == 0 || n == 0) return;
+    dim3 grid(ceil_div(n, TN), ceil_div(nq, TQ));
+    dim3 block(256);
+    ProfScope prof("exact_tile", s);
+    switch (metric) {
+    case HM_METRIC_EUCLIDEAN:
+        exact_tile_kernel<ACC_EUCLID, EPI_NONE><<<grid, block, 0, s>>>(Q, ldq, nq, B, ldb, n, dpad,
+                                                                       qnorm, bnorm, D, ldd);
+        break;
+    case HM_METRIC_MANHATTAN:
+        exact_tile_kernel<ACC_MANHATTAN, EPI_NONE><<<grid, block, 0, s>>>(
+            Q, ldq, nq, B, ldb, n, dpad, qnorm, bnorm, D, ldd);
+        break;
+    case HM_METRIC_DOT_PRODUCT:
+        exact_tile_kernel<ACC_DOT, EPI_NONE><<<grid, block, 0, s>>>(Q, ldq, nq, B, ldb, n, dpad,
+                                                                    qnorm, bnorm, D, ldd);
+        break;
+    case HM_METRIC_COSINE_SIMILARITY:
+        exact_tile_kernel<ACC_DOT, EPI_COSINE><<<grid, block, 0, s>>>(Q, ldq, nq, B, ldb, n, dpad,
+                                                                      qnorm, bnorm, D, ldd);
+        break;
+    case HM_METRIC_ANGULAR:
+        exact_tile_kernel<ACC_DOT, EPI_ANGULAR><<<grid, block, 0, s>>>(
+            Q, ldq, nq, B, ldb, n, dpad, qnorm, bnorm, D, ldd);
+        break;
+    default:
+        fail(HM_ERR_UNKNOWN_METRIC, "unknown method");
+    }
+    HM_CHECK_LAUNCH();
+}
+
+void launch_self_dot(const float *X, uint32_t ldx, uint32_t rows, uint32_t d, float *out,
+                     hipStream_t s)
+{
+    if (rows == 0) return;
+    self_dot_kernel<<<ceil_div(rows, 256), 256, 0, s>>>(X, ldx, rows, d, out);
+    HM_CHECK_LAUNCH();
+}
+
+}  // namespace hm

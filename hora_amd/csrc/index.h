@@ -1,0 +1,63 @@
+// index.h -- host runtime objects behind the C ABI.
+#pragma once
+
+#include "hm_common.h"
+
+#include <memory>
+#include <mutex>
+
+namespace hm {
+
+// One ANNIndex implementation on one device (ann_index.rs:25-121).
+class Index {
+  public:
+    Index(int32_t kind, uint64_t dim) : kind_(kind), dim_(dim) {}
+    virtual ~Index() = default;
+
+    int32_t kind() const { return kind_; }
+    uint64_t dimension() const { return dim_; }
+
+    virtual void add(const float *rows, uint64_t n_rows) = 0;
+    virtual void build(int32_t metric) = 0;
+    virtual bool built() const = 0;
+    // Queries and outputs in device memory; outputs are [nq][k] / [nq].
+    virtual void search_device(const float *d_queries, uint64_t nq, uint32_t k, uint64_t *d_idx,
+                               float *d_dist, uint32_t *d_count, hipStream_t s) = 0;
+    virtual const char *name() const = 0;
+    virtual uint64_t nodes_size() const = 0;
+    virtual void clear() = 0;
+    virtual void get_rows(const uint64_t *idx, uint64_t n, float *out) const = 0;
+    virtual void set_search_mode(int32_t mode, uint32_t shortlist)
+    {
+        (void)mode;
+        (void)shortlist;
+        fail(HM_ERR_INVALID_ARG, "search mode not supported by this index");
+    }
+
+    hipStream_t stream() const { return stream_; }
+
+  protected:
+    int32_t kind_;
+    uint64_t dim_;
+    hipStream_t stream_ = nullptr;
+};
+
+std::unique_ptr<Index> make_bruteforce(uint64_t dim);
+std::unique_ptr<Index> make_ivfpq(uint64_t dim, const hm_params &p);
+
+// Host-side validation shared by the index types (node.rs:158-166).
+void check_finite_host(const float *x, uint64_t count);
+
+// Fills unused result slots (index UINT64_MAX, distance +inf, count 0).
+void launch_fill_results(uint64_t *idx, float *dist, uint32_t *count, uint64_t nq, uint32_t k,
+                         hipStream_t s);
+
+}  // namespace hm
+
+struct hm_index {
+    std::unique_ptr<hm::Index> impl;
+    std::mutex mu;
+    int device = 0;
+    // Host-entry staging buffers.
+    hm::DeviceBuffer q_stage, idx_stage, dist_stage, count_stage;
+};

@@ -1,0 +1,281 @@
+"""Python mirror of hora's ANNIndex surface over the C ABI.
+
+Names, argument meaning and error strings follow the reference:
+  trait ANNIndex            src/core/ann_index.rs:25-121
+  enum Metric               src/core/metrics.rs:5-13
+  Node                      src/core/node.rs:93-167
+  BruteForceIndex / Params  src/index/bruteforce_idx.rs, bruteforce_params.rs
+  IVFPQIndex / IVFPQParams  src/index/pq_idx.rs:267-523, pq_params.rs:43-91
+Where the reference returns Err(&'static str) or panics, this mirror raises
+HoraError carrying the status and the reference's message.
+
+Everything runs through libhora_mi355x.so on the GPU; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from enum import IntEnum
+from typing import Any, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from ._lib import HmParams, HoraError, check, lib
+
+HM_INDEX_BRUTE_FORCE = 1
+HM_INDEX_IVFPQ = 2
+
+HM_SEARCH_EXACT = 0
+HM_SEARCH_BF16_RERANK = 1
+HM_SEARCH_FP8_RERANK = 2
+
+
+class Metric(IntEnum):
+    """metrics.rs:5-13, same variant order."""
+    Unknown = 0
+    Manhattan = 1
+    DotProduct = 2
+    Euclidean = 3
+    CosineSimilarity = 4
+    Angular = 5
+
+
+class Node:
+    """node.rs:93-167: an owned f32 vector plus an optional id.  Non-finite
+    elements are rejected (node.rs:158-166 panics)."""
+
+    def __init__(self, vectors, idx: Any = None):
+        v = np.array(vectors, dtype=np.float32, copy=True).reshape(-1)
+        if not np.all(np.isfinite(v)):
+            raise HoraError(3, "invalid float element")
+        self._vectors = v
+        self._idx = idx
+
+    @classmethod
+    def new(cls, vectors) -> "Node":
+        return cls(vectors)
+
+    @classmethod
+    def new_with_idx(cls, vectors, idx) -> "Node":
+        return cls(vectors, idx)
+
+    def vectors(self) -> np.ndarray:
+        return self._vectors
+
+    def idx(self) -> Any:
+        return self._idx
+
+    def len(self) -> int:
+        return int(self._vectors.size)
+
+    def is_empty(self) -> bool:
+        return self._vectors.size == 0
+
+    def __repr__(self) -> str:
+        return f"Node(key: {self._idx!r}, vectors: {self._vectors!r})"
+
+
+class BruteForceParams:
+    """bruteforce_params.rs:5-14 (no fields)."""
+
+
+class IVFPQParams:
+    """pq_params.rs:43-91 builder; defaults of IVFPQParams::default().
+    seed / kmeans_accumulate are this build's additions (seeded split RNG,
+    f32-point-order or fp64 centroid sums)."""
+
+    def __init__(self):
+        p = HmParams()
+        check(lib().hm_params_default(HM_INDEX_IVFPQ, C.byref(p)))
+        self._p = p
+
+    def n_sub(self, v: int) -> "IVFPQParams":
+        self._p.n_sub = v
+        return self
+
+    def sub_bits(self, v: int) -> "IVFPQParams":
+        self._p.sub_bits = v
+        return self
+
+    def n_kmeans_center(self, v: int) -> "IVFPQParams":
+        self._p.n_kmeans_center = v
+        return self
+
+    def search_n_center(self, v: int) -> "IVFPQParams":
+        self._p.search_n_center = v
+        return self
+
+    def train_epoch(self, v: int) -> "IVFPQParams":
+        self._p.train_epoch = v
+        return self
+
+    def seed(self, v: int) -> "IVFPQParams":
+        self._p.seed = v
+        return self
+
+    def kmeans_accumulate(self, v: int) -> "IVFPQParams":
+        self._p.kmeans_accumulate = v
+        return self
+
+
+def _f32_rows(a, dim: int) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim == 1:
+        a = a.reshape(1, -1)
+    if a.shape[1] != dim:
+        raise HoraError(2, "different dimensions")
+    return a
+
+
+class _Index:
+    _kind = 0
+
+    def __init__(self, dimension: int, params: Optional[Any] = None):
+        h = C.c_void_p()
+        p = params._p if isinstance(params, IVFPQParams) else None
+        check(lib().hm_index_create(self._kind, dimension,
+                                    C.byref(p) if p is not None else None, C.byref(h)))
+        self._h = h
+        self._dim = int(dimension)
+        self._ids: List[Any] = []
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                lib().hm_index_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # ---- ANNIndex
+    def build(self, mt: Metric) -> None:
+        """ann_index.rs:30."""
+        check(lib().hm_index_build(self._h, int(mt)))
+
+    def add_node(self, item: Node) -> None:
+        """ann_index.rs:36."""
+        self._add_rows(item.vectors().reshape(1, -1), [item.idx()])
+
+    def add(self, vs, idx) -> None:
+        """ann_index.rs:41-43."""
+        self.add_node(Node.new_with_idx(vs, idx))
+
+    def madd(self, vss: Sequence, indices: Sequence) -> None:
+        """ann_index.rs:48-59."""
+        if len(vss) != len(indices):
+            raise HoraError(1, "vector's size is different with index")
+        rows = np.stack([np.asarray(v, dtype=np.float32) for v in vss]) if len(vss) else \
+            np.zeros((0, self._dim), np.float32)
+        self._add_rows(rows, list(indices))
+
+    def add_batch(self, rows, ids: Optional[Sequence] = None) -> None:
+        """Bulk form of madd over an [n, dim] array (ids default to positions)."""
+        rows = _f32_rows(rows, self._dim)
+        if ids is None:
+            ids = range(len(self._ids), len(self._ids) + rows.shape[0])
+        self._add_rows(rows, list(ids))
+
+    def _add_rows(self, rows, ids) -> None:
+        rows = _f32_rows(rows, self._dim)
+        check(lib().hm_index_add(self._h, rows.ctypes.data, rows.shape[0]))
+        self._ids.extend(ids)
+
+    def built(self) -> bool:
+        out = C.c_int32()
+        check(lib().hm_index_built(self._h, C.byref(out)))
+        return bool(out.value)
+
+    def rebuild(self, mt: Metric) -> None:
+        """ann_index.rs:69-71: the default implementation."""
+        raise HoraError(10, "not implement")
+
+    def search_batch(self, queries, k: int) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """node_search_k for a batch: (positions u64 [nq,k], dist f32 [nq,k],
+        count u32 [nq]); unused slots hold UINT64_MAX / +inf."""
+        q = _f32_rows(queries, self._dim)
+        nq = q.shape[0]
+        idx = np.empty((nq, max(k, 1)), np.uint64)
+        dist = np.empty((nq, max(k, 1)), np.float32)
+        cnt = np.empty(max(nq, 1), np.uint32)
+        check(lib().hm_index_search(self._h, q.ctypes.data, nq, k, idx.ctypes.data,
+                                    dist.ctypes.data, cnt.ctypes.data))
+        return idx[:, :k], dist[:, :k], cnt[:nq]
+
+    def node_search_k(self, item: Node, k: int) -> List[Tuple[Node, float]]:
+        """ann_index.rs:74: (Node, distance) ascending."""
+        idx, dist, cnt = self.search_batch(item.vectors().reshape(1, -1), k)
+        m = int(cnt[0])
+        pos = idx[0, :m]
+        rows = self.get_rows(pos)
+        return [(Node(rows[j], self._ids[int(pos[j])]), float(dist[0, j])) for j in range(m)]
+
+    def search_nodes(self, item, k: int) -> List[Tuple[Node, float]]:
+        """ann_index.rs:81-84."""
+        item = np.asarray(item, dtype=np.float32).reshape(-1)
+        if item.size != self._dim:
+            raise HoraError(2, "different dimensions")
+        return self.node_search_k(Node.new(item), k)
+
+    def search(self, item, k: int) -> List[Any]:
+        """ann_index.rs:91-97: ids of the k nearest."""
+        item = np.asarray(item, dtype=np.float32).reshape(-1)
+        if item.size != self._dim:
+            raise HoraError(2, "different dimensions")
+        Node.new(item)  # validation (node.rs:158-166)
+        idx, _, cnt = self.search_batch(item.reshape(1, -1), k)
+        return [self._ids[int(i)] for i in idx[0, : int(cnt[0])]]
+
+    def get_rows(self, positions) -> np.ndarray:
+        pos = np.ascontiguousarray(positions, dtype=np.uint64)
+        out = np.empty((pos.size, self._dim), np.float32)
+        check(lib().hm_index_get_rows(self._h, pos.ctypes.data, pos.size, out.ctypes.data))
+        return out
+
+    def name(self) -> str:
+        return lib().hm_index_name(self._h).decode()
+
+    def nodes_size(self) -> int:
+        out = C.c_uint64()
+        check(lib().hm_index_nodes_size(self._h, C.byref(out)))
+        return int(out.value)
+
+    def clear(self) -> None:
+        check(lib().hm_index_clear(self._h))
+        self._ids = []
+
+    def idx_info(self) -> str:
+        """ann_index.rs:113-115."""
+        return "not implement"
+
+    def dimension(self) -> int:
+        out = C.c_uint64()
+        check(lib().hm_index_dimension(self._h, C.byref(out)))
+        return int(out.value)
+
+    # ---- device-resident search (inputs/outputs already in HBM)
+    def search_device(self, q_ptr: int, nq: int, k: int, idx_ptr: int, dist_ptr: int,
+                      count_ptr: int, stream: int = 0) -> None:
+        check(lib().hm_index_search_device(self._h, q_ptr, nq, k, idx_ptr, dist_ptr, count_ptr,
+                                           stream or None))
+
+
+class BruteForceIndex(_Index):
+    """bruteforce_idx.rs:15-106."""
+    _kind = HM_INDEX_BRUTE_FORCE
+
+    def __init__(self, dimension: int, params: Optional[BruteForceParams] = None):
+        super().__init__(dimension, None)
+
+    def set_search_mode(self, mode: int, shortlist: int = 0) -> None:
+        check(lib().hm_index_set_search_mode(self._h, mode, shortlist))
+
+
+class IVFPQIndex(_Index):
+    """pq_idx.rs:267-523."""
+    _kind = HM_INDEX_IVFPQ
+
+    def __init__(self, dimension: int, params: Optional[IVFPQParams] = None):
+        super().__init__(dimension, params if params is not None else IVFPQParams())
+
+
+PQIVFIndex = IVFPQIndex  # README.md:57 name

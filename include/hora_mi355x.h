@@ -1,0 +1,166 @@
+/*
+ * hora_mi355x.h -- C ABI of the MI355X (gfx950) distance-and-scan engine.
+ *
+ * This is the drop-in boundary for hora's hot path: the entry points below
+ * are what a hora FFI crate binds in place of the Rust implementations of
+ * the ANNIndex trait (src/core/ann_index.rs:25-121) for BruteForceIndex
+ * (src/index/bruteforce_idx.rs) and IVFPQIndex / PQIVFIndex
+ * (src/index/pq_idx.rs).  Each declaration cites the reference item it
+ * replaces.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Host pointers unless the name ends in
+ *    _device (then device pointers, valid on the index's device).
+ *  - Every entry returns an int32 status (hm_status); on failure
+ *    hm_last_error() returns a thread-local message that reuses the
+ *    reference's error strings where the reference has one.
+ *  - Item indices are positional: the i-th row ever added has index i
+ *    (the reference's internal usize index, bruteforce_idx.rs:50-53).  The
+ *    caller keeps its own id table to map indices to its id type T, as the
+ *    reference's Node::idx does (node.rs:96).
+ *  - Result rows are k wide; row q holds count[q] results in ascending
+ *    distance order (node_search_k's drain+reverse, bruteforce_idx.rs:63-71);
+ *    unused slots hold index UINT64_MAX and distance +inf.
+ *  - Reference panics become error statuses (the reference has no status
+ *    for them); the reference's 100,001-item IVFPQ cap (pq_idx.rs:326) is
+ *    not reproduced.
+ */
+#ifndef HORA_MI355X_H
+#define HORA_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HM_API __attribute__((visibility("default")))
+
+typedef struct hm_index hm_index;
+
+/* Variant order of `enum Metric` (src/core/metrics.rs:5-13); also the
+ * bincode variant index. */
+typedef enum {
+    HM_METRIC_UNKNOWN = 0,
+    HM_METRIC_MANHATTAN = 1,
+    HM_METRIC_DOT_PRODUCT = 2,
+    HM_METRIC_EUCLIDEAN = 3,
+    HM_METRIC_COSINE_SIMILARITY = 4,
+    HM_METRIC_ANGULAR = 5
+} hm_metric;
+
+typedef enum {
+    HM_OK = 0,
+    HM_ERR_INVALID_ARG = 1,
+    HM_ERR_DIM_MISMATCH = 2,     /* "different dimensions" (calc.rs:25) / "dimension is different" (pq_idx.rs:342) */
+    HM_ERR_NONFINITE_INPUT = 3,  /* reference: panic!("invalid float element") (node.rs:162) */
+    HM_ERR_UNKNOWN_METRIC = 4,   /* "unknown method" (metrics.rs:32) */
+    HM_ERR_NOT_BUILT = 5,
+    HM_ERR_CAPACITY = 6,
+    HM_ERR_HIP = 7,
+    HM_ERR_NAN_DISTANCE = 8,     /* reference: partial_cmp().unwrap() panic (neighbor.rs:32) */
+    HM_ERR_EMPTY = 9,            /* "None to assigned impossible split center" (kmeans.rs:159) */
+    HM_ERR_NOT_IMPLEMENTED = 10, /* "not implement" (ann_index.rs:70) */
+    HM_ERR_OUT_OF_MEMORY = 11
+} hm_status;
+
+typedef enum {
+    HM_INDEX_BRUTE_FORCE = 1, /* BruteForceIndex (bruteforce_idx.rs:15-22) */
+    HM_INDEX_IVFPQ = 2        /* IVFPQIndex, exported as PQIVFIndex too (pq_idx.rs:267-292) */
+} hm_index_kind;
+
+/* BruteForceParams is empty (bruteforce_params.rs:5-14); IVFPQParams fields
+ * (pq_params.rs:43-91).  `seed` seeds the k-means split step, which the
+ * reference draws from an unseeded thread_rng (kmeans.rs:168). */
+typedef struct {
+    uint32_t n_sub;
+    uint32_t sub_bits;
+    uint32_t n_kmeans_center;
+    uint32_t search_n_center;
+    uint32_t train_epoch;
+    uint32_t kmeans_accumulate; /* 0: f32 in point order (reference kmeans.rs:102-113), 1: fp64 */
+    uint64_t seed;
+} hm_params;
+
+/* Search forms of BruteForceIndex.  EXACT runs every pair through the
+ * reference-order FP32 arithmetic; the RERANK forms build a shortlist with
+ * MFMA (BF16 or FP8-e4m3 with per-vector scale, FP32 accumulate) and rerank
+ * it with the exact arithmetic. */
+typedef enum {
+    HM_SEARCH_EXACT = 0,
+    HM_SEARCH_BF16_RERANK = 1,
+    HM_SEARCH_FP8_RERANK = 2
+} hm_search_mode;
+
+/* Fills the reference defaults: IVFPQParams::default() (pq_params.rs:80-91)
+ * for HM_INDEX_IVFPQ; zeros for HM_INDEX_BRUTE_FORCE. */
+HM_API int32_t hm_params_default(int32_t kind, hm_params *out);
+
+/* BruteForceIndex::new (bruteforce_idx.rs:25-32), IVFPQIndex::new
+ * (pq_idx.rs:295-330).  Binds to the current HIP device. */
+HM_API int32_t hm_index_create(int32_t kind, uint64_t dimension, const hm_params *params,
+                               hm_index **out);
+
+/* Drop of the Rust index value. */
+HM_API int32_t hm_index_destroy(hm_index *h);
+
+/* ANNIndex::add / madd / add_node (ann_index.rs:36-59; bruteforce_idx.rs:40-43;
+ * pq_idx.rs:340-355, 448-453): appends n_rows rows of `dimension` floats.
+ * Non-finite elements are rejected (node.rs:158-166) and nothing is added. */
+HM_API int32_t hm_index_add(hm_index *h, const float *rows, uint64_t n_rows);
+
+/* ANNIndex::build (ann_index.rs:30; bruteforce_idx.rs:36-39; pq_idx.rs:443-447). */
+HM_API int32_t hm_index_build(hm_index *h, int32_t metric);
+
+/* ANNIndex::built (ann_index.rs:64). */
+HM_API int32_t hm_index_built(const hm_index *h, int32_t *built);
+
+/* ANNIndex::node_search_k / search for a batch of nq queries
+ * (ann_index.rs:74-97; bruteforce_idx.rs:47-73; pq_idx.rs:408-477). */
+HM_API int32_t hm_index_search(hm_index *h, const float *queries, uint64_t nq, uint32_t k,
+                               uint64_t *idx, float *dist, uint32_t *count);
+
+/* Same, with queries and outputs in device memory, enqueued on `stream`
+ * (a hipStream_t; NULL = the index's own stream).  Returns after the work is
+ * enqueued and the stream has been synchronised. */
+HM_API int32_t hm_index_search_device(hm_index *h, const float *d_queries, uint64_t nq,
+                                      uint32_t k, uint64_t *d_idx, float *d_dist,
+                                      uint32_t *d_count, void *stream);
+
+/* Selects the BruteForceIndex search form and its shortlist depth
+ * (0 = automatic).  No reference counterpart (the reference has one form). */
+HM_API int32_t hm_index_set_search_mode(hm_index *h, int32_t mode, uint32_t shortlist);
+
+/* ANNIndex::nodes_size (ann_index.rs:105) -- number of rows added. */
+HM_API int32_t hm_index_nodes_size(const hm_index *h, uint64_t *out);
+
+/* ANNIndex::dimension (ann_index.rs:118). */
+HM_API int32_t hm_index_dimension(const hm_index *h, uint64_t *out);
+
+/* ANNIndex::name (ann_index.rs:102): "BruteForceIndex" / "IVFPQIndex". */
+HM_API const char *hm_index_name(const hm_index *h);
+
+/* ANNIndex::clear (ann_index.rs:110): drops all rows and built state. */
+HM_API int32_t hm_index_clear(hm_index *h);
+
+/* Copies rows by index into `out` (n x dimension floats): what
+ * search_nodes returns with each Node (ann_index.rs:81-84). */
+HM_API int32_t hm_index_get_rows(const hm_index *h, const uint64_t *idx, uint64_t n,
+                                 float *out);
+
+/* Per-kernel timing with HIP events recorded on the launch stream around
+ * each launch (off by default; instrumentation, no reference counterpart).
+ * hm_profile_query synchronises and returns the accumulated milliseconds
+ * and launch count of a kernel family ("exact_tile", "row_select", ...). */
+HM_API int32_t hm_profile_enable(int32_t on);
+HM_API int32_t hm_profile_reset(void);
+HM_API int32_t hm_profile_query(const char *kernel, double *total_ms, uint64_t *launches);
+
+/* Thread-local message of the last failed call on this thread. */
+HM_API const char *hm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HORA_MI355X_H */

@@ -1,0 +1,268 @@
+"""Pins the CPU oracle against the reference's known answers, hand traces,
+closed forms and brute force (CPU only)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+# ---------------------------------------------------------------- metrics
+def test_reference_known_answers(orc):
+    kats = _load("reference_kats.json")
+    assert orc.dot_neg(kats["dot"]["a"], kats["dot"]["b"]) == kats["dot"]["expect"]
+    m = kats["manhattan"]
+    assert orc.metric(m["a"], m["b"], orc.MANHATTAN) == m["expect"]
+
+
+def test_metric_closed_forms(orc):
+    a = [1.0, 2.0, 3.0]
+    b = [4.0, 0.0, -1.0]
+    # Euclidean is squared, no sqrt (simd_metrics.rs:96-102): 9+4+16
+    assert orc.metric(a, b, orc.EUCLIDEAN) == 29.0
+    # DotProduct is +sum(a*b) after the double negation (metrics.rs:56): 4+0-3
+    assert orc.metric(a, b, orc.DOT) == 1.0
+    assert orc.metric(a, b, orc.MANHATTAN) == 3 + 2 + 4
+    # Angular = 1 + cos (float_two() == 1.0): cos(a,a) = 1 -> 2; orthogonal -> 1
+    assert orc.metric(a, a, orc.ANGULAR) == pytest.approx(2.0, abs=1e-7)
+    assert orc.metric([1, 0], [0, 1], orc.ANGULAR) == 1.0
+    assert orc.metric([0, 0], [0, 1], orc.ANGULAR) == 1.0  # r <= 0 branch
+    # Cosine in this build: -cos; the reference's own formula is NaN
+    assert orc.metric([1, 0], [2, 0], orc.COSINE) == -1.0
+    assert orc.metric([1, 0], [0, 3], orc.COSINE) == 0.0
+    assert math.isnan(orc.cosine_reference([1, 2], [3, 4]))
+    with pytest.raises(orc.OracleError):
+        orc.metric(a, b, orc.UNKNOWN)
+
+
+def test_summation_is_a_left_fold(orc):
+    # products [1e8, 1, -1e8, 1]: left fold in f32 gives ((1e8+1)-1e8)+1 = 1
+    # (1e8+1 rounds to 1e8); any pairing of the two middle terms gives 0 or 2.
+    a = [1e8, 1.0, -1e8, 1.0]
+    assert orc.metric(a, [1, 1, 1, 1], orc.DOT) == 1.0
+    # squares [1e8(=1e4^2), 1, 1, ...]: adding 1 to 1e8 sixteen times by
+    # left fold leaves 1e8 (each +1 rounds away), pairwise would keep 16.
+    x = np.array([1e4] + [1.0] * 16, np.float32)
+    assert orc.metric(x, np.zeros_like(x), orc.EUCLIDEAN) == np.float32(1e8)
+
+
+def test_metrics_within_fp64_bound(orc):
+    rng = np.random.default_rng(3)
+    for d in (1, 7, 64, 784):
+        a = rng.normal(0, 10, d).astype(np.float32)
+        b = rng.normal(0, 10, d).astype(np.float32)
+        u = 2.0 ** -24
+        for mt in (orc.EUCLIDEAN, orc.MANHATTAN, orc.DOT):
+            got = orc.metric(a, b, mt)
+            exact = orc.metric_f64(a, b, mt)
+            if mt == orc.EUCLIDEAN:
+                mag = float(np.sum((a.astype(np.float64) - b) ** 2))
+            elif mt == orc.MANHATTAN:
+                mag = float(np.sum(np.abs(a.astype(np.float64) - b)))
+            else:
+                mag = float(np.sum(np.abs(a.astype(np.float64) * b)))
+            assert abs(got - exact) <= 2 * (d + 2) * u * mag + 1e-30
+
+
+# ------------------------------------------------------------------ heap
+def test_heap_hand_traces(orc):
+    fx = _load("heap_fixtures.json")
+    for c in fx["cases"]:
+        layout, popped = orc.heap_script(c["ops"], c["dists"], c["ids"])
+        assert [list(x) for x in layout] == c["layout"], c["name"]
+        assert [list(x) for x in popped] == c["popped"], c["name"]
+    for c in fx["bf_cases"]:
+        ids, dists = orc.replay_row(c["row"], c["k"])
+        assert ids.tolist() == c["ids"], c["name"]
+        assert dists.tolist() == c["dists"], c["name"]
+
+
+def test_heap_invariants_random(orc):
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        n = int(rng.integers(1, 60))
+        ops = (rng.random(n) < 0.3).astype(np.int32)
+        dists = rng.integers(0, 6, n).astype(np.float32)  # many ties
+        ids = np.arange(n, dtype=np.uint64)
+        layout, popped = orc.heap_script(ops, dists, ids)
+        # max-heap property on distance
+        for i in range(1, len(layout)):
+            assert layout[(i - 1) // 2][0] >= layout[i][0]
+        # every pop returns a maximum of the heap at that time
+        live = []
+        pi = 0
+        for i in range(n):
+            if ops[i] == 0:
+                live.append(float(dists[i]))
+            elif live:
+                m = max(live)
+                assert popped[pi][0] == m
+                live.remove(m)
+                pi += 1
+        assert sorted(live) == sorted(x[0] for x in layout)
+
+
+def test_bf_search_reduces_to_sort_without_ties(orc):
+    rng = np.random.default_rng(7)
+    base = rng.normal(0, 10, (500, 12)).astype(np.float32)
+    q = rng.normal(0, 10, (20, 12)).astype(np.float32)
+    for mt in (orc.EUCLIDEAN, orc.MANHATTAN, orc.DOT, orc.COSINE, orc.ANGULAR):
+        ids, dist, cnt = orc.bf_search(base, q, 10, mt)
+        rows = orc.distance_rows(base, q, mt)
+        for r in range(len(q)):
+            order = np.lexsort((np.arange(500), rows[r]))[:11]
+            assert len(set(rows[r][order].tolist())) == 11  # tie-free inputs
+            assert ids[r].tolist() == order[:10].tolist()
+            assert dist[r].tolist() == rows[r][order[:10]].tolist()
+        assert (cnt == 10).all()
+
+
+def test_bf_search_sequential_order_matches_numpy_fold(orc):
+    rng = np.random.default_rng(8)
+    base = rng.normal(0, 10, (50, 33)).astype(np.float32)
+    q = rng.normal(0, 10, (3, 33)).astype(np.float32)
+    rows = orc.distance_rows(base, q, orc.EUCLIDEAN)
+    diff = (q[:, None, :] - base[None, :, :]).astype(np.float32)
+    sq = (diff * diff).astype(np.float32)
+    fold = np.add.accumulate(sq, axis=2, dtype=np.float32)[:, :, -1]
+    assert np.array_equal(rows, fold)
+
+
+def test_bf_edge_cases(orc):
+    base = np.array([[0.0, 0.0], [1.0, 0.0], [2.0, 0.0]], np.float32)
+    ids, dist, cnt = orc.bf_search(base, np.zeros((1, 2), np.float32), 5, orc.EUCLIDEAN)
+    assert cnt[0] == 3 and ids[0, :3].tolist() == [0, 1, 2]
+    ids, dist, cnt = orc.bf_search(base, np.zeros((1, 2), np.float32), 0, orc.EUCLIDEAN)
+    assert cnt[0] == 0
+    with pytest.raises(orc.OracleError):
+        orc.bf_search(base, np.zeros((1, 2), np.float32), 2, orc.UNKNOWN)
+
+
+# ----------------------------------------------------------------- kmeans
+def test_kmeans_init_closed_form(orc):
+    rng = np.random.default_rng(9)
+    d = 130
+    x = rng.normal(0, 1, (40, d)).astype(np.float32)
+    c = orc.kmeans_init(x, 6)
+    mean = x.astype(np.float64).mean(axis=0)
+    plus = np.zeros(d, bool)
+    plus[[0, 64, 128]] = True  # (i & (1 << (j mod 64))) == 1 only for odd i, j%64==0
+    for i in range(6):
+        expect = mean + np.where(plus & (i % 2 == 1), 1.0, -1.0)
+        assert np.allclose(c[i], expect, atol=1e-5)
+    assert np.array_equal(c[0], c[2]) and np.array_equal(c[1], c[5])
+
+
+def test_kmeans_assign_lowest_index_argmin(orc):
+    rng = np.random.default_rng(10)
+    x = rng.normal(0, 1, (200, 8)).astype(np.float32)
+    cent = rng.normal(0, 1, (16, 8)).astype(np.float32)
+    cent[9] = cent[4]  # duplicate centre: the lower index must win
+    got = orc.kmeans_assign(x, cent)
+    rows = orc.distance_rows(cent, x, orc.EUCLIDEAN)
+    assert np.array_equal(got, np.argmin(rows, axis=1))
+    assert not (got == 9).any()
+
+
+def test_kmeans_update_closed_form(orc):
+    x = np.array([[1, 2], [3, 4], [10, 10], [5, 6]], np.float32)
+    assign = np.array([0, 0, 2, 0], np.uint32)
+    cent, cnt = orc.kmeans_update(x, assign, 3)
+    assert cnt.tolist() == [3, 0, 1]
+    assert cent[0].tolist() == [3.0, 4.0]
+    assert cent[1].tolist() == [0.0, 0.0]  # empty centre becomes zero (kmeans.rs:115-122)
+    assert cent[2].tolist() == [10.0, 10.0]
+    # residual variant: x - r
+    cent, _ = orc.kmeans_update(x, assign, 3, residual=np.array([1, 2], np.float32))
+    assert cent[0].tolist() == [2.0, 2.0]
+
+
+def test_kmeans_split_invariants(orc):
+    # one empty centre: it becomes src*(1-eps, 1+eps) and the source
+    # src*(1+eps, 1-eps) (kmeans.rs:177-187); counts halve (188-189)
+    eps = np.float32(1.0 / 1024.0)
+    cent = np.array([[2.0, 4.0], [0, 0], [8.0, 16.0]], np.float32)
+    cnt = np.array([10, 0, 6], np.uint64)
+    for seed in range(1, 30):
+        c2, n2, _ = orc.kmeans_split(cent, cnt, 16, seed)
+        j = [s for s in (0, 2) if np.array_equal(c2[1], cent[s] * np.array([1 - eps, 1 + eps], np.float32))]
+        assert len(j) == 1
+        j = j[0]
+        assert np.array_equal(c2[j], cent[j] * np.array([1 + eps, 1 - eps], np.float32))
+        assert n2[1] == cnt[j] // 2 and n2[j] == cnt[j] - cnt[j] // 2
+        assert int(n2.sum()) == 16
+    # several empties: total count conserved, every empty centre revived
+    cent = np.array([[2.0, 4.0], [0, 0], [8.0, 16.0], [0, 0]], np.float32)
+    c2, n2, _ = orc.kmeans_split(cent, np.array([10, 0, 6, 0], np.uint64), 16, 7)
+    assert int(n2.sum()) == 16 and (n2 > 0).all()
+
+
+def test_kmeans_train_runs_and_converges(orc):
+    x, centres, which = __import__("hora_amd.datasets", fromlist=["x"]).gaussian_mixture(
+        2000, 8, 8, seed=1, sigma=0.05)
+    cent, assign = orc.kmeans_train(x, 8, 20, seed=42)
+    assert cent.shape == (8, 8) and assign.shape == (2000,)
+    assert np.array_equal(assign, orc.kmeans_assign(x, cent))
+    # deterministic given the seed
+    cent2, assign2 = orc.kmeans_train(x, 8, 20, seed=42)
+    assert np.array_equal(cent, cent2) and np.array_equal(assign, assign2)
+
+
+# --------------------------------------------------------------------- PQ
+def test_pq_ranges_closed_form(orc):
+    assert orc.pq_ranges(10, 4) == [(0, 2), (2, 4), (4, 6), (6, 8)]  # dims 8,9 dropped
+    assert orc.pq_ranges(10, 3) == [(0, 4), (4, 7), (7, 10)]
+    assert orc.pq_ranges(128, 16) == [(8 * i, 8 * i + 8) for i in range(16)]
+    assert orc.pq_ranges(96, 12) == [(8 * i, 8 * i + 8) for i in range(12)]
+
+
+def test_pq_lut_closed_form(orc):
+    q = np.array([1, 2, 3, 4], np.float32)
+    res = np.array([1, 1, 1, 1], np.float32)
+    cb = np.zeros((2, 2, 2), np.float32)
+    cb[0, 1] = [1, 1]
+    cb[1, 0] = [2, 3]
+    lut = orc.pq_lut(q, res, 2, cb)
+    # residual z = [0,1,2,3]; sub0 = [0,1], sub1 = [2,3]
+    assert lut.tolist() == [[1.0, 1.0], [0.0, 13.0]]
+
+
+def test_ivfpq_search_matches_bruteforce_over_probed_lists(orc):
+    rng = np.random.default_rng(12)
+    d, nlist, n_sub, K = 8, 4, 4, 64
+    cent = rng.normal(0, 3, (nlist, d)).astype(np.float32)
+    x = rng.normal(0, 3, (300, d)).astype(np.float32)
+    assign = orc.kmeans_assign(x, cent)
+    order = np.argsort(assign, kind="stable")
+    offsets = np.concatenate([[0], np.cumsum(np.bincount(assign, minlength=nlist))])
+    cb = rng.normal(0, 1, (nlist, n_sub, K, d // n_sub)).astype(np.float32)
+    codes = np.zeros((300, n_sub), np.uint32)
+    for l in range(nlist):
+        mem = order[offsets[l]:offsets[l + 1]]
+        codes[offsets[l]:offsets[l + 1]] = orc.pq_encode(x[mem], n_sub, cb[l], cent[l])
+    q = rng.normal(0, 3, (5, d)).astype(np.float32)
+    ids, dist, cnt, probes = orc.ivfpq_search(q, orc.EUCLIDEAN, cent, cb, offsets, order, codes,
+                                              2, 7)
+    for r in range(5):
+        cd = orc.distance_rows(cent, q[r:r + 1], orc.EUCLIDEAN)[0]
+        assert probes[r].tolist() == np.argsort(cd, kind="stable")[:2].tolist()
+        cand = []
+        for l in probes[r]:
+            lut = orc.pq_lut(q[r], cent[l], n_sub, cb[l])
+            for row in range(offsets[l], offsets[l + 1]):
+                s = np.float32(0)
+                for j in range(n_sub):
+                    s = np.float32(s + lut[j, codes[row, j]])
+                cand.append((float(s), int(order[row])))
+        cand.sort()
+        assert len({c[0] for c in cand[:8]}) == 8  # tie-free case
+        assert ids[r].tolist() == [c[1] for c in cand[:7]]
+        assert dist[r].tolist() == [c[0] for c in cand[:7]]
