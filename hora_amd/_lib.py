@@ -76,12 +76,16 @@ def lib():
     L.hm_index_search.argtypes = [vp, vp, u64, u32, vp, vp, vp]
     L.hm_index_search_device.argtypes = [vp, vp, u64, u32, vp, vp, vp, vp]
     L.hm_index_set_search_mode.argtypes = [vp, i32, u32]
+    L.hm_index_search_stats.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
     L.hm_index_nodes_size.argtypes = [vp, C.POINTER(u64)]
     L.hm_index_dimension.argtypes = [vp, C.POINTER(u64)]
     L.hm_index_name.argtypes = [vp]
     L.hm_index_name.restype = C.c_char_p
     L.hm_index_clear.argtypes = [vp]
     L.hm_index_get_rows.argtypes = [vp, vp, u64, vp]
+    L.hm_index_set_nprobe.argtypes = [vp, u32]
+    L.hm_ivfpq_info.argtypes = [vp, vp]
+    L.hm_ivfpq_export.argtypes = [vp, vp, vp, vp, vp, vp]
     L.hm_profile_enable.argtypes = [i32]
     L.hm_profile_reset.argtypes = []
     L.hm_profile_query.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)]

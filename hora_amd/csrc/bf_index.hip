@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "index.h"
 #include "kernels.h"
@@ -74,6 +75,7 @@ class BruteForce final : public Index {
         HM_HIP(hipStreamSynchronize(stream_));
         n_ = need;
         bnorm_valid_ = false;
+        sl_valid_ = false;
     }
 
     void build(int32_t metric) override
@@ -94,6 +96,7 @@ class BruteForce final : public Index {
         bnorm_.release();
         n_ = cap_ = 0;
         bnorm_valid_ = false;
+        sl_valid_ = false;
     }
 
     void get_rows(const uint64_t *idx, uint64_t n, float *out) const override
@@ -108,8 +111,10 @@ class BruteForce final : public Index {
 
     void set_search_mode(int32_t mode, uint32_t shortlist) override
     {
-        if (mode != HM_SEARCH_EXACT)
-            fail(HM_ERR_NOT_IMPLEMENTED, "search mode not available in this build");
+        if (mode < HM_SEARCH_EXACT || mode > HM_SEARCH_FP8_RERANK)
+            fail(HM_ERR_INVALID_ARG, "unknown search mode");
+        if (shortlist > SELECT_MAX_K) fail(HM_ERR_INVALID_ARG, "shortlist above 8192");
+        if (mode != mode_) sl_valid_ = false;
         mode_ = mode;
         shortlist_ = shortlist;
     }
@@ -122,33 +127,53 @@ class BruteForce final : public Index {
         if (nq == 0) return;
         if (!dq || !d_count || (k && (!d_idx || !d_dist)))
             fail(HM_ERR_INVALID_ARG, "null pointer");
+        if (nq > 0x7FFFFFFFull) fail(HM_ERR_INVALID_ARG, "too many queries");
         if (k == 0 || n_ == 0) {
-            if (k) {
-                unsigned blocks = (unsigned)std::min<uint64_t>((nq * k + 255) / 256, 4096);
-                fill_results_kernel<<<blocks, 256, 0, s>>>(d_idx, d_dist, d_count, nq, k);
-            } else {
-                HM_HIP(hipMemsetAsync(d_count, 0, nq * sizeof(uint32_t), s));
-            }
+            launch_fill_results(d_idx, d_dist, d_count, nq, k, s);
             HM_HIP(hipStreamSynchronize(s));
             return;
         }
-        const uint32_t n = (uint32_t)n_;
-        const uint32_t K = (uint32_t)std::min<uint64_t>((uint64_t)k + 1, n);
-        if (K > SELECT_MAX_K) fail(HM_ERR_INVALID_ARG, "k too large (max 8191)");
-        if (nq > 0x7FFFFFFFull) fail(HM_ERR_INVALID_ARG, "too many queries");
-
         err_.reserve(sizeof(uint32_t));
         HM_HIP(hipMemsetAsync(err_.get(), 0, sizeof(uint32_t), s));
         launch_check_finite(dq, dim_, nq, (uint32_t)dim_, err_.as<uint32_t>(), s);
+        const bool shortlist_form = mode_ != HM_SEARCH_EXACT && metric_ != HM_METRIC_MANHATTAN &&
+                                    dpad_ <= 4096;
+        last_fallback_ = last_uncertified_ = 0;
+        if (shortlist_form)
+            search_shortlist(dq, nq, k, d_idx, d_dist, d_count, s);
+        else
+            search_exact(dq, dim_, nq, k, d_idx, d_dist, d_count, s);
+        uint32_t err = 0;
+        HM_HIP(hipMemcpyAsync(&err, err_.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HM_HIP(hipStreamSynchronize(s));
+        if (err & 2u) fail(HM_ERR_NONFINITE_INPUT, "invalid float element");
+        if (err & 1u) fail(HM_ERR_NAN_DISTANCE, "NaN distance (partial_cmp of NaN)");
+    }
 
-        const bool need_norm =
-            metric_ == HM_METRIC_COSINE_SIMILARITY || metric_ == HM_METRIC_ANGULAR;
-        if (need_norm && !bnorm_valid_) {
-            bnorm_.reserve(n_ * sizeof(float));
-            launch_self_dot(base_.as<float>(), dpad_, n, (uint32_t)dim_, bnorm_.as<float>(), s);
-            bnorm_valid_ = true;
-        }
+  private:
+    bool need_norm() const
+    {
+        return metric_ == HM_METRIC_COSINE_SIMILARITY || metric_ == HM_METRIC_ANGULAR;
+    }
 
+    void ensure_bnorm(hipStream_t s)
+    {
+        if (bnorm_valid_) return;
+        bnorm_.reserve(n_ * sizeof(float));
+        launch_self_dot(base_.as<float>(), dpad_, (uint32_t)n_, (uint32_t)dim_, bnorm_.as<float>(),
+                        s);
+        bnorm_valid_ = true;
+    }
+
+    // Full exact path: every pair in reference order, radix top-(k+1), heap
+    // replay for tied rows.  Queries: nq rows of dim floats, row stride ldq.
+    void search_exact(const float *dq, uint64_t ldq, uint64_t nq, uint32_t k, uint64_t *d_idx,
+                      float *d_dist, uint32_t *d_count, hipStream_t s)
+    {
+        const uint32_t n = (uint32_t)n_;
+        const uint32_t K = (uint32_t)std::min<uint64_t>((uint64_t)k + 1, n);
+        if (K > SELECT_MAX_K) fail(HM_ERR_INVALID_ARG, "k too large (max 8191)");
+        if (need_norm()) ensure_bnorm(s);
         const uint64_t ldd = round_up(n, 4);
         uint64_t qc = std::max<uint64_t>(1, d_budget_ / (ldd * sizeof(float)));
         qc = std::min<uint64_t>(qc, nq);
@@ -158,17 +183,16 @@ class BruteForce final : public Index {
         cand_d_.reserve(qc * K * sizeof(float));
         cand_i_.reserve(qc * K * sizeof(uint32_t));
         flags_.reserve(qc * sizeof(uint32_t));
-        if (need_norm) qnorm_.reserve(qc * sizeof(float));
+        qnorm_.reserve(qc * sizeof(float));
         if ((size_t)64 * (k + 1) * 8 > 64 * 1024) {
             heap_d_.reserve(qc * (k + 1) * sizeof(float));
             heap_i_.reserve(qc * (k + 1) * sizeof(uint32_t));
         }
-
         for (uint64_t q0 = 0; q0 < nq; q0 += qc) {
             const uint32_t rows = (uint32_t)std::min<uint64_t>(qc, nq - q0);
-            launch_pad_rows(dq + q0 * dim_, dim_, qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
+            launch_pad_rows(dq + q0 * ldq, ldq, qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
                             dpad_, s);
-            if (need_norm)
+            if (need_norm())
                 launch_self_dot(qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
                                 qnorm_.as<float>(), s);
             launch_exact_tile(metric_, qpad_.as<float>(), dpad_, rows, base_.as<float>(), dpad_,
@@ -182,11 +206,135 @@ class BruteForce final : public Index {
                                err_.as<uint32_t>(), heap_d_.as<float>(), heap_i_.as<uint32_t>(),
                                s);
         }
-        uint32_t err = 0;
-        HM_HIP(hipMemcpyAsync(&err, err_.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    }
+
+    // Derived base data of the shortlist forms (rebuilt after add/mode change).
+    void ensure_shortlist_data(hipStream_t s)
+    {
+        ensure_bnorm(s);
+        if (sl_valid_) return;
+        const uint32_t n = (uint32_t)n_;
+        inv_bnorm_.reserve(n_ * sizeof(float));
+        stat_.reserve(2 * sizeof(uint32_t));
+        HM_HIP(hipMemsetAsync(stat_.get(), 0, 2 * sizeof(uint32_t), s));
+        launch_norm_stats(bnorm_.as<float>(), n, inv_bnorm_.as<float>(), stat_.as<uint32_t>(), s);
+        if (mode_ == HM_SEARCH_BF16_RERANK) {
+            dk_ = (uint32_t)round_up(dim_, 32);
+            blow_.reserve(n_ * dk_ * 2);
+            launch_to_bf16(base_.as<float>(), dpad_, n_, (uint32_t)dim_, dk_, blow_.as<uint16_t>(),
+                           stat_.as<uint32_t>() + 1, s);
+        } else {
+            dk_ = (uint32_t)round_up(dim_, 64);
+            blow_.reserve(n_ * dk_);
+            bscale_.reserve(n_ * sizeof(float));
+            launch_to_fp8(base_.as<float>(), dpad_, n, (uint32_t)dim_, dk_, blow_.as<uint8_t>(),
+                          bscale_.as<float>(), s);
+        }
+        uint32_t st[2];
+        HM_HIP(hipMemcpyAsync(st, stat_.get(), sizeof(st), hipMemcpyDeviceToHost, s));
         HM_HIP(hipStreamSynchronize(s));
-        if (err & 2u) fail(HM_ERR_NONFINITE_INPUT, "invalid float element");
-        if (err & 1u) fail(HM_ERR_NAN_DISTANCE, "NaN distance (partial_cmp of NaN)");
+        float bm;
+        std::memcpy(&bm, &st[0], 4);
+        bmax_ = bm;
+        base_inexact_ = st[1] != 0;
+        sl_valid_ = true;
+    }
+
+    uint32_t shortlist_size(uint32_t k) const
+    {
+        uint64_t S = shortlist_ ? shortlist_ : std::max<uint64_t>(64, 4ull * (k + 1));
+        S = std::max<uint64_t>(S, (uint64_t)k + 1);
+        S = std::min<uint64_t>(S, SELECT_MAX_K);
+        return (uint32_t)std::min<uint64_t>(S, n_);
+    }
+
+    // MFMA shortlist (BF16 or FP8) + exact rerank; rows that are tied or not
+    // certified are rerun on the exact path.
+    void search_shortlist(const float *dq, uint64_t nq, uint32_t k, uint64_t *d_idx,
+                          float *d_dist, uint32_t *d_count, hipStream_t s)
+    {
+        ensure_shortlist_data(s);
+        const uint32_t n = (uint32_t)n_;
+        const uint32_t S = shortlist_size(k);
+        const bool bf16 = mode_ == HM_SEARCH_BF16_RERANK;
+        const uint32_t row_bytes = bf16 ? dk_ * 2 : dk_;
+        const uint64_t ldk = round_up(n, 4);
+        uint64_t qc = std::max<uint64_t>(1, d_budget_ / (ldk * sizeof(float)));
+        qc = std::min<uint64_t>(qc, nq);
+        qc = std::min<uint64_t>(qc, 65535ull * 128);
+        qpad_.reserve(qc * dpad_ * sizeof(float));
+        qnorm_.reserve(qc * sizeof(float));
+        qlow_.reserve(qc * row_bytes);
+        qscale_.reserve(qc * sizeof(float));
+        D_.reserve(qc * ldk * sizeof(float));
+        cand_d_.reserve(qc * S * sizeof(float));
+        cand_i_.reserve(qc * S * sizeof(uint32_t));
+        flags_.reserve(qc * sizeof(uint32_t));
+        rflags_.reserve(nq * sizeof(uint32_t));
+        qinexact_.reserve(sizeof(uint32_t));
+        HM_HIP(hipMemsetAsync(qinexact_.get(), 0, sizeof(uint32_t), s));
+
+        const float u = 5.9604644775390625e-8f;
+        const float gamma = 2.0f * (float)(std::max<uint32_t>(dk_, dpad_) + 8) * u;
+        const float c_bf16 = 1.0f / 128.0f + 1.0f / 65536.0f;
+        const float c_exact = base_inexact_ ? c_bf16 : 0.0f;
+        for (uint64_t q0 = 0; q0 < nq; q0 += qc) {
+            const uint32_t rows = (uint32_t)std::min<uint64_t>(qc, nq - q0);
+            launch_pad_rows(dq + q0 * dim_, dim_, qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
+                            dpad_, s);
+            launch_self_dot(qpad_.as<float>(), dpad_, rows, (uint32_t)dim_, qnorm_.as<float>(), s);
+            if (bf16)
+                launch_to_bf16(qpad_.as<float>(), dpad_, rows, (uint32_t)dim_, dk_,
+                               qlow_.as<uint16_t>(), qinexact_.as<uint32_t>(), s);
+            else
+                launch_to_fp8(qpad_.as<float>(), dpad_, rows, (uint32_t)dim_, dk_,
+                              qlow_.as<uint8_t>(), qscale_.as<float>(), s);
+            launch_score_gemm(bf16 ? 2 : 1, qlow_.get(), rows, blow_.get(), n, row_bytes,
+                              qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
+                              inv_bnorm_.as<float>(), metric_, D_.as<float>(), ldk, s);
+            launch_row_select(D_.as<float>(), ldk, rows, n, S, cand_d_.as<float>(),
+                              cand_i_.as<uint32_t>(), S, flags_.as<uint32_t>(), s);
+            launch_rerank(metric_, qpad_.as<float>(), dpad_, base_.as<float>(),
+                          cand_i_.as<uint32_t>(), cand_d_.as<float>(), S, S, qnorm_.as<float>(),
+                          bnorm_.as<float>(), rows, n, k, bmax_, gamma, c_exact, c_bf16,
+                          qinexact_.as<uint32_t>(), bf16 ? certify_ : 0, d_idx + q0 * k,
+                          d_dist + q0 * k, d_count + q0, rflags_.as<uint32_t>() + q0, s);
+        }
+        // Rows needing the exact path (ties: heap replay; uncertified).
+        std::vector<uint32_t> fl(nq);
+        HM_HIP(hipMemcpyAsync(fl.data(), rflags_.get(), nq * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, s));
+        HM_HIP(hipStreamSynchronize(s));
+        std::vector<uint32_t> redo;
+        for (uint64_t i = 0; i < nq; ++i)
+            if (fl[i] & (ROW_TIE | ROW_UNCERT)) redo.push_back((uint32_t)i);
+        last_fallback_ = redo.size();
+        last_uncertified_ = 0;
+        for (uint64_t i = 0; i < nq; ++i) last_uncertified_ += (fl[i] & ROW_UNCERT) ? 1 : 0;
+        if (redo.empty()) return;
+        const uint32_t m = (uint32_t)redo.size();
+        redo_rows_.reserve(m * sizeof(uint32_t));
+        redo_q_.reserve((size_t)m * dim_ * sizeof(float));
+        redo_i_.reserve((size_t)m * k * sizeof(uint64_t));
+        redo_d_.reserve((size_t)m * k * sizeof(float));
+        redo_c_.reserve((size_t)m * sizeof(uint32_t));
+        HM_HIP(hipMemcpyAsync(redo_rows_.get(), redo.data(), m * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, s));
+        launch_gather_rows(dq, dim_, redo_rows_.as<uint32_t>(), m, (uint32_t)dim_,
+                           redo_q_.as<float>(), dim_, s);
+        search_exact(redo_q_.as<float>(), dim_, m, k, redo_i_.as<uint64_t>(), redo_d_.as<float>(),
+                     redo_c_.as<uint32_t>(), s);
+        launch_scatter_results(redo_i_.as<uint64_t>(), redo_d_.as<float>(),
+                               redo_c_.as<uint32_t>(), redo_rows_.as<uint32_t>(), m, k, d_idx,
+                               d_dist, d_count, s);
+        HM_HIP(hipStreamSynchronize(s));
+    }
+
+  public:
+    void search_stats(uint64_t *fallback, uint64_t *uncertified) const override
+    {
+        *fallback = last_fallback_;
+        *uncertified = last_uncertified_;
     }
 
   private:
@@ -199,6 +347,14 @@ class BruteForce final : public Index {
     DeviceBuffer base_, bnorm_;
     bool bnorm_valid_ = false;
     DeviceBuffer qpad_, qnorm_, D_, cand_d_, cand_i_, flags_, err_, heap_d_, heap_i_;
+    // shortlist forms
+    bool sl_valid_ = false, base_inexact_ = true;
+    int certify_ = 1;
+    uint32_t dk_ = 0;
+    float bmax_ = 0;
+    DeviceBuffer inv_bnorm_, stat_, blow_, bscale_, qlow_, qscale_, qinexact_, rflags_;
+    DeviceBuffer redo_rows_, redo_q_, redo_i_, redo_d_, redo_c_;
+    uint64_t last_fallback_ = 0, last_uncertified_ = 0;
 };
 
 }  // namespace

@@ -199,6 +199,17 @@ HM_API int32_t hm_index_set_search_mode(hm_index *h, int32_t mode, uint32_t shor
     });
 }
 
+HM_API int32_t hm_index_search_stats(const hm_index *h, uint64_t *fallback,
+                                     uint64_t *uncertified)
+{
+    return guard([&] {
+        need(h, "index");
+        need(fallback, "fallback");
+        need(uncertified, "uncertified");
+        h->impl->search_stats(fallback, uncertified);
+    });
+}
+
 HM_API int32_t hm_index_nodes_size(const hm_index *h, uint64_t *out)
 {
     return guard([&] {
@@ -238,6 +249,36 @@ HM_API int32_t hm_index_get_rows(const hm_index *h, const uint64_t *idx, uint64_
         need(out, "out");
         DeviceScope ds(h->device);
         h->impl->get_rows(idx, n, out);
+    });
+}
+
+HM_API int32_t hm_index_set_nprobe(hm_index *h, uint32_t nprobe)
+{
+    return guard([&] {
+        need(h, "index");
+        std::lock_guard<std::mutex> lk(h->mu);
+        ivfpq_set_nprobe(h->impl.get(), nprobe);
+    });
+}
+
+HM_API int32_t hm_ivfpq_info(hm_index *h, uint32_t *out8)
+{
+    return guard([&] {
+        need(h, "index");
+        need(out8, "out");
+        std::lock_guard<std::mutex> lk(h->mu);
+        ivfpq_info(h->impl.get(), out8);
+    });
+}
+
+HM_API int32_t hm_ivfpq_export(hm_index *h, float *centers, uint64_t *list_offsets,
+                               uint64_t *list_ids, float *codebooks, uint8_t *codes)
+{
+    return guard([&] {
+        need(h, "index");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        ivfpq_export(h->impl.get(), centers, list_offsets, list_ids, codebooks, codes);
     });
 }
 

@@ -66,11 +66,22 @@ __device__ __forceinline__ float epi_angular(float sqb, float sqq, float sbb)
     return 1.0f;
 }
 
-template <int ACC, int EPI>
+__device__ __forceinline__ uint32_t fkey(float f)
+{
+    f = f + 0.0f;
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+// ARGMIN: instead of writing D, fold each row's (distance, column) minimum
+// into best[q] with a 64-bit atomicMin (lowest column among equal distances,
+// the strict '<' scan of kmeans.rs:136-145).
+template <int ACC, int EPI, bool ARGMIN>
 __global__ __launch_bounds__(256, 2) void exact_tile_kernel(
     const float *__restrict__ Q, uint32_t ldq, uint32_t nq, const float *__restrict__ B,
     uint32_t ldb, uint32_t n, uint32_t dpad, const float *__restrict__ qnorm,
-    const float *__restrict__ bnorm, float *__restrict__ D, uint64_t ldd)
+    const float *__restrict__ bnorm, float *__restrict__ D, uint64_t ldd,
+    unsigned long long *__restrict__ best)
 {
     __shared__ __align__(16) float sQ[2][TK][TQ + LDS_PAD];
     __shared__ __align__(16) float sB[2][TK][TN + LDS_PAD];
@@ -144,28 +155,44 @@ __global__ __launch_bounds__(256, 2) void exact_tile_kernel(
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const uint32_t qi = q0 + ty * 8 + i;
-        if (qi >= nq) continue;
+        const uint32_t qc = qi < nq ? qi : nq - 1;
         float v[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             float a = acc[i][j];
             if constexpr (EPI == EPI_COSINE) {
                 uint32_t bj = bj0 + j < n ? bj0 + j : n - 1;
-                a = epi_cosine(a, qnorm[qi], bnorm[bj]);
+                a = epi_cosine(a, qnorm[qc], bnorm[bj]);
             } else if constexpr (EPI == EPI_ANGULAR) {
                 uint32_t bj = bj0 + j < n ? bj0 + j : n - 1;
-                a = epi_angular(a, qnorm[qi], bnorm[bj]);
+                a = epi_angular(a, qnorm[qc], bnorm[bj]);
             }
             v[j] = a;
         }
-        float *drow = D + (size_t)qi * ldd;
-        if (bj0 + 7 < n) {
-            *reinterpret_cast<float4 *>(drow + bj0) = make_float4(v[0], v[1], v[2], v[3]);
-            *reinterpret_cast<float4 *>(drow + bj0 + 4) = make_float4(v[4], v[5], v[6], v[7]);
-        } else {
+        if constexpr (ARGMIN) {
+            unsigned long long m = ~0ull;
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if (bj0 + j < n) drow[bj0 + j] = v[j];
+            for (int j = 0; j < 8; ++j) {
+                unsigned long long key = ((unsigned long long)fkey(v[j]) << 32) | (bj0 + j);
+                if (bj0 + j < n && key < m) m = key;
+            }
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                unsigned long long other = __shfl_xor(m, o, 64);
+                m = other < m ? other : m;
+            }
+            if (tx == 0 && qi < nq) atomicMin(best + qi, m);
+        } else {
+            if (qi >= nq) continue;
+            float *drow = D + (size_t)qi * ldd;
+            if (bj0 + 7 < n) {
+                *reinterpret_cast<float4 *>(drow + bj0) = make_float4(v[0], v[1], v[2], v[3]);
+                *reinterpret_cast<float4 *>(drow + bj0 + 4) = make_float4(v[4], v[5], v[6], v[7]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (bj0 + j < n) drow[bj0 + j] = v[j];
+            }
         }
     }
 }
@@ -188,39 +215,57 @@ __global__ void self_dot_kernel(const float *__restrict__ X, uint32_t ldx, uint3
 
 }  // namespace
 
+template <bool ARGMIN>
+static void launch_exact_tile_impl(int metric, const float *Q, uint32_t ldq, uint32_t nq,
+                                   const float *B, uint32_t ldb, uint32_t n, uint32_t dpad,
+                                   const float *qnorm, const float *bnorm, float *D, uint64_t ldd,
+                                   unsigned long long *best, hipStream_t s)
+{
+    if (nq == 0 || n == 0) return;
+    // gridDim.y is at most 65535: longer query ranges run in slices
+    constexpr uint32_t QMAX = 65535u * TQ;
+    if (nq > QMAX) {
+        for (uint32_t q0 = 0; q0 < nq; q0 += QMAX) {
+            const uint32_t m = std::min(QMAX, nq - q0);
+            launch_exact_tile_impl<ARGMIN>(metric, Q + (size_t)q0 * ldq, ldq, m, B, ldb, n, dpad,
+                                           qnorm ? qnorm + q0 : nullptr, bnorm,
+                                           D ? D + (size_t)q0 * ldd : nullptr, ldd,
+                                           best ? best + q0 : nullptr, s);
+        }
+        return;
+    }
+    dim3 grid(ceil_div(n, TN), ceil_div(nq, TQ));
+    dim3 block(256);
+    ProfScope prof(ARGMIN ? "exact_argmin" : "exact_tile", s);
+#define HM_EXACT(ACC, EPI)                                                                \
+    exact_tile_kernel<ACC, EPI, ARGMIN><<<grid, block, 0, s>>>(Q, ldq, nq, B, ldb, n, dpad, \
+                                                              qnorm, bnorm, D, ldd, best)
+    switch (metric) {
+    case HM_METRIC_EUCLIDEAN: HM_EXACT(ACC_EUCLID, EPI_NONE); break;
+    case HM_METRIC_MANHATTAN: HM_EXACT(ACC_MANHATTAN, EPI_NONE); break;
+    case HM_METRIC_DOT_PRODUCT: HM_EXACT(ACC_DOT, EPI_NONE); break;
+    case HM_METRIC_COSINE_SIMILARITY: HM_EXACT(ACC_DOT, EPI_COSINE); break;
+    case HM_METRIC_ANGULAR: HM_EXACT(ACC_DOT, EPI_ANGULAR); break;
+    default: fail(HM_ERR_UNKNOWN_METRIC, "unknown method");
+    }
+#undef HM_EXACT
+    HM_CHECK_LAUNCH();
+}
+
 void launch_exact_tile(int metric, const float *Q, uint32_t ldq, uint32_t nq, const float *B,
                        uint32_t ldb, uint32_t n, uint32_t dpad, const float *qnorm,
                        const float *bnorm, float *D, uint64_t ldd, hipStream_t s)
 {
-    if (nq == 0 || n == 0) return;
-    dim3 grid(ceil_div(n, TN), ceil_div(nq, TQ));
-    dim3 block(256);
-    ProfScope prof("exact_tile", s);
-    switch (metric) {
-    case HM_METRIC_EUCLIDEAN:
-        exact_tile_kernel<ACC_EUCLID, EPI_NONE><<<grid, block, 0, s>>>(Q, ldq, nq, B, ldb, n, dpad,
-                                                                       qnorm, bnorm, D, ldd);
-        break;
-    case HM_METRIC_MANHATTAN:
-        exact_tile_kernel<ACC_MANHATTAN, EPI_NONE><<<grid, block, 0, s>>>(
-            Q, ldq, nq, B, ldb, n, dpad, qnorm, bnorm, D, ldd);
-        break;
-    case HM_METRIC_DOT_PRODUCT:
-        exact_tile_kernel<ACC_DOT, EPI_NONE><<<grid, block, 0, s>>>(Q, ldq, nq, B, ldb, n, dpad,
-                                                                    qnorm, bnorm, D, ldd);
-        break;
-    case HM_METRIC_COSINE_SIMILARITY:
-        exact_tile_kernel<ACC_DOT, EPI_COSINE><<<grid, block, 0, s>>>(Q, ldq, nq, B, ldb, n, dpad,
-                                                                      qnorm, bnorm, D, ldd);
-        break;
-    case HM_METRIC_ANGULAR:
-        exact_tile_kernel<ACC_DOT, EPI_ANGULAR><<<grid, block, 0, s>>>(
-            Q, ldq, nq, B, ldb, n, dpad, qnorm, bnorm, D, ldd);
-        break;
-    default:
-        fail(HM_ERR_UNKNOWN_METRIC, "unknown method");
-    }
-    HM_CHECK_LAUNCH();
+    launch_exact_tile_impl<false>(metric, Q, ldq, nq, B, ldb, n, dpad, qnorm, bnorm, D, ldd,
+                                  nullptr, s);
+}
+
+void launch_exact_argmin(int metric, const float *Q, uint32_t ldq, uint32_t nq, const float *B,
+                         uint32_t ldb, uint32_t n, uint32_t dpad, const float *qnorm,
+                         const float *bnorm, unsigned long long *best, hipStream_t s)
+{
+    launch_exact_tile_impl<true>(metric, Q, ldq, nq, B, ldb, n, dpad, qnorm, bnorm, nullptr, 0,
+                                 best, s);
 }
 
 void launch_self_dot(const float *X, uint32_t ldx, uint32_t rows, uint32_t d, float *out,

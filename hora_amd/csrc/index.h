@@ -34,6 +34,14 @@ class Index {
         fail(HM_ERR_INVALID_ARG, "search mode not supported by this index");
     }
 
+    // Rows of the last search that were rerun on the exact path, and how
+    // many of those failed shortlist certification (shortlist forms only).
+    virtual void search_stats(uint64_t *fallback, uint64_t *uncertified) const
+    {
+        *fallback = 0;
+        *uncertified = 0;
+    }
+
     hipStream_t stream() const { return stream_; }
 
   protected:
@@ -44,6 +52,12 @@ class Index {
 
 std::unique_ptr<Index> make_bruteforce(uint64_t dim);
 std::unique_ptr<Index> make_ivfpq(uint64_t dim, const hm_params &p);
+
+// IVFPQIndex extras (fail with INVALID_ARG on another index kind).
+int32_t ivfpq_info(Index *ix, uint32_t *out8);
+int32_t ivfpq_export(Index *ix, float *centers, uint64_t *offsets, uint64_t *ids,
+                     float *codebooks, uint8_t *codes);
+int32_t ivfpq_set_nprobe(Index *ix, uint32_t nprobe);
 
 // Host-side validation shared by the index types (node.rs:158-166).
 void check_finite_host(const float *x, uint64_t count);

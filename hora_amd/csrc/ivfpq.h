@@ -1,0 +1,57 @@
+// ivfpq.h -- launch wrappers of the IVFPQIndex kernels (ivfpq.hip).
+#pragma once
+
+#include "kernels.h"
+
+namespace hm {
+
+// kmeans.rs:50-86 for the coarse quantizer: column means of X (f32 left fold
+// in row order, or fp64) and the +-1 initial centres, written [nc][ldc].
+void launch_coarse_init(bool fp64, const float *X, uint32_t ldx, uint64_t n, uint32_t d,
+                        uint32_t nc, float *C, uint32_t ldc, hipStream_t s);
+void launch_unpack_assign(const unsigned long long *best, uint64_t n, uint32_t *assign,
+                          hipStream_t s);
+// Stable grouping of rows by list: ids = row indices sorted by (assign, row),
+// sorted_keys = assign in that order, off[0..nlist] = list offsets.
+void build_lists(const uint32_t *assign, uint64_t n, uint32_t nlist, uint32_t *sorted_keys,
+                 uint32_t *ids, uint32_t *off, DeviceBuffer &iota, DeviceBuffer &tmp,
+                 hipStream_t s);
+// kmeans.rs:88-125: centre = mean of members (point order), empty -> 0.
+void launch_update(bool fp64, const float *X, uint32_t ldx, const uint32_t *off,
+                   const uint32_t *ids, uint32_t nlist, uint32_t d, float *C, uint32_t ldc,
+                   hipStream_t s);
+// R[r] = X[ids[r]] - C[lists[r]] for rows in list order.
+void launch_residuals(const float *X, uint32_t ldx, const uint32_t *ids, const uint32_t *lists,
+                      const float *C, uint64_t n, uint32_t dpad, float *R, hipStream_t s);
+// Per (list, sub-space) k-means on the residuals: codebooks
+// [nlist][M][KS][sdm] and codes [n][M] (rows in list order).
+void launch_pq_train(bool fp64, const float *R, uint32_t ldr, const uint32_t *off,
+                     uint32_t nlist, uint32_t M, const uint32_t *sub_b, const uint32_t *sub_e,
+                     uint32_t KS, uint32_t sd_max, uint32_t sdm, uint32_t epochs, uint64_t seed,
+                     uint32_t max_list, uint64_t n, uint16_t *asg_scratch, float *codebooks,
+                     uint8_t *codes, hipStream_t s);
+// Candidate slots per (query, probe) workgroup.
+uint32_t ivf_scan_pool(uint32_t k1);
+// out[(q*nprobe + p)*k1 ..] = the k1 smallest (fkey(adc) << 32 | id) of
+// the p-th probed list of query q.
+void launch_ivf_scan(const float *Qp, uint32_t dpad, const float *C, const uint32_t *probes,
+                     uint32_t ldp, uint32_t nprobe, uint32_t rows, const uint32_t *off,
+                     const uint32_t *ids, const uint8_t *codes, const float *codebooks,
+                     const uint32_t *sub_b, const uint32_t *sub_e, uint32_t M, uint32_t KS,
+                     uint32_t sdm, uint32_t k1, unsigned long long *out, hipStream_t s);
+void launch_ivf_merge(const unsigned long long *cand, uint32_t per_query, uint32_t rows,
+                      uint32_t k, const uint32_t *coarse_flags, uint64_t *out_idx,
+                      float *out_dist, uint32_t *out_count, uint32_t *flags, hipStream_t s);
+void launch_ivf_replay(const uint32_t *rows, uint32_t nrows, const float *Dc, uint32_t ldd,
+                       uint32_t nlist, uint32_t nprobe, const float *Qp, uint32_t dpad,
+                       const float *C, const uint32_t *off, const uint32_t *ids,
+                       const uint8_t *codes, const float *codebooks, const uint32_t *sub_b,
+                       const uint32_t *sub_e, uint32_t M, uint32_t KS, uint32_t sdm, uint32_t k,
+                       uint32_t lmax, float *scratch_f, uint32_t *scratch_u, size_t per_thread,
+                       uint64_t *out_idx, float *out_dist, uint32_t *out_count, hipStream_t s);
+// kmeans.rs:150-193 on host memory, seeded SplitMix64 (state *rng).
+void split_centers_host(float *C, uint32_t ldc, uint32_t nc, uint32_t dim, uint64_t n,
+                        uint64_t *counts, uint64_t *rng);
+uint64_t host_stream_seed(uint64_t seed, uint64_t instance);
+
+}  // namespace hm

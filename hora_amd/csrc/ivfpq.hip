@@ -1,0 +1,775 @@
+// ivfpq.hip -- IVFPQIndex kernels: k-means steps, residual PQ training,
+// LUT + ADC inverted-list scan, per-query merge and the tie replay.
+//
+// Reference: src/core/kmeans.rs (Kmeans), src/index/pq_idx.rs (PQIndex,
+// IVFPQIndex).  Every distance that decides an assignment, a code or a
+// search result is computed with the reference's f32 left fold (no FMA), so
+// assignments, codes and ADC distances equal the reference's given the same
+// centroids and codebooks.
+#include <hipcub/hipcub.hpp>
+
+#include "ivfpq.h"
+#include "profiler.h"
+
+#pragma clang fp contract(off)
+
+namespace hm {
+
+namespace {
+
+__device__ __forceinline__ uint32_t fkey(float f)
+{
+    f = f + 0.0f;
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ __forceinline__ float unkey(uint32_t k)
+{
+    uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __uint_as_float(b);
+}
+
+// SplitMix64, the seeded stand-in for the reference's thread_rng
+// (kmeans.rs:168-171); the oracle uses the same generator.
+__device__ __host__ __forceinline__ uint64_t splitmix64(uint64_t *s)
+{
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __host__ __forceinline__ double splitmix_unit(uint64_t *s)
+{
+    return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__device__ __host__ __forceinline__ uint64_t stream_seed(uint64_t seed, uint64_t instance)
+{
+    uint64_t s = seed ^ (instance * 0xD1B54A32D192ED03ull);
+    return splitmix64(&s);
+}
+
+// kmeans.rs:76: (i & (1 << j)) == 1, shift amount masked to 6 bits.
+__device__ __host__ __forceinline__ float init_offset(uint64_t i, uint32_t j)
+{
+    return ((i & (1ull << (j & 63))) == 1ull) ? 1.0f : -1.0f;
+}
+
+// kmeans.rs:50-86: mean over the rows (left fold in row order, f32 or f64),
+// then centre i = mean + init_offset(i, j).  One lane per dimension.
+template <typename ACC_T>
+__global__ void coarse_init_kernel(const float *__restrict__ X, uint32_t ldx, uint64_t n,
+                                   uint32_t d, uint32_t nc, float *__restrict__ C, uint32_t ldc)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ldc) return;
+    if (j >= d) {
+        for (uint32_t c = 0; c < nc; ++c) C[(size_t)c * ldc + j] = 0.0f;
+        return;
+    }
+    ACC_T acc = 0;
+    uint64_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = X[(i + u) * ldx + j];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = acc + (ACC_T)v[u];
+    }
+    for (; i < n; ++i) acc = acc + (ACC_T)X[i * ldx + j];
+    float mean;
+    if constexpr (sizeof(ACC_T) == 4) mean = acc / (float)n;
+    else mean = (float)(acc / (double)n);
+    for (uint32_t c = 0; c < nc; ++c) {
+        float v = mean;
+        v = init_offset(c, j) > 0 ? v + 1.0f : v - 1.0f;
+        C[(size_t)c * ldc + j] = v;
+    }
+}
+
+__global__ void unpack_assign_kernel(const unsigned long long *__restrict__ best, uint64_t n,
+                                     uint32_t *__restrict__ assign)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        assign[i] = (uint32_t)(best[i] & 0xFFFFFFFFull);
+}
+
+__global__ void iota_kernel(uint32_t *__restrict__ v, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        v[i] = (uint32_t)i;
+}
+
+// off[c] = first position of key c in the sorted keys (off[nc] = n).
+__global__ void offsets_kernel(const uint32_t *__restrict__ sorted, uint64_t n, uint32_t nc,
+                               uint32_t *__restrict__ off)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > nc) return;
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) / 2;
+        if (sorted[mid] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    off[c] = (uint32_t)lo;
+}
+
+// kmeans.rs:88-125: per centre and dimension, the members' values summed in
+// point order (f32 or f64), divided by the count; an empty centre is zero.
+template <typename ACC_T>
+__global__ void update_kernel(const float *__restrict__ X, uint32_t ldx,
+                              const uint32_t *__restrict__ off, const uint32_t *__restrict__ ids,
+                              uint32_t d, float *__restrict__ C, uint32_t ldc)
+{
+    const uint32_t c = blockIdx.x;
+    const uint32_t j = blockIdx.y * blockDim.x + threadIdx.x;
+    if (j >= ldc) return;
+    const uint32_t a = off[c], b = off[c + 1];
+    if (j >= d || a == b) {
+        C[(size_t)c * ldc + j] = 0.0f;
+        return;
+    }
+    ACC_T acc = 0;
+    uint32_t r = a;
+    for (; r + 4 <= b; r += 4) {
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = X[(size_t)ids[r + u] * ldx + j];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = acc + (ACC_T)v[u];
+    }
+    for (; r < b; ++r) acc = acc + (ACC_T)X[(size_t)ids[r] * ldx + j];
+    const uint32_t cnt = b - a;
+    float v;
+    if constexpr (sizeof(ACC_T) == 4) v = acc / (float)cnt;
+    else v = (float)(acc / (double)cnt);
+    C[(size_t)c * ldc + j] = v;
+}
+
+// R[r] = X[ids[r]] - C[list(r)] (f32), rows in list order.
+__global__ void residual_kernel(const float *__restrict__ X, uint32_t ldx,
+                                const uint32_t *__restrict__ ids,
+                                const uint32_t *__restrict__ lists, const float *__restrict__ C,
+                                uint64_t n, uint32_t dpad, float *__restrict__ R)
+{
+    uint64_t total = n * dpad;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t r = e / dpad, j = e % dpad;
+        float x = X[(size_t)ids[r] * ldx + j];
+        float c = C[(size_t)lists[r] * dpad + j];
+        R[e] = x - c;
+    }
+}
+
+// One workgroup trains the k-means of sub-space s of list l on the list's
+// residuals (pq_idx.rs:122-149 -> kmeans.rs:195-207) and writes its
+// codebook and the final assignment (the codes, pq_idx.rs:143-146).
+constexpr int PQ_THREADS = 256;
+constexpr int PQ_MAXSD = 32;
+
+template <typename ACC_T>
+__global__ __launch_bounds__(PQ_THREADS) void pq_train_kernel(
+    const float *__restrict__ R, uint32_t ldr, const uint32_t *__restrict__ off, uint32_t M,
+    const uint32_t *__restrict__ sub_b, const uint32_t *__restrict__ sub_e, uint32_t KS,
+    uint32_t sdm, uint32_t epochs, uint64_t seed, int staged, uint64_t n_rows,
+    uint16_t *__restrict__ asg_g, float *__restrict__ codebooks, uint8_t *__restrict__ codes)
+{
+    extern __shared__ __align__(16) unsigned char pq_smem[];
+    const uint32_t inst = blockIdx.x;
+    const uint32_t l = inst / M, s = inst % M;
+    const uint32_t base = off[l], L = off[l + 1] - base;
+    const uint32_t b = sub_b[s], sd = sub_e[s] - sub_b[s];
+    const int tid = threadIdx.x;
+    float *cb = codebooks + ((size_t)l * M + s) * KS * sdm;
+    if (L == 0) {
+        for (uint32_t e = tid; e < KS * sdm; e += PQ_THREADS) cb[e] = 0.0f;
+        return;
+    }
+    float *cent = reinterpret_cast<float *>(pq_smem);          // KS * sd
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(cent + KS * sd);  // KS
+    float *zs = reinterpret_cast<float *>(cnt + KS);              // L * sd when staged
+    uint16_t *as = reinterpret_cast<uint16_t *>(zs + (staged ? (size_t)L * sd : 0));
+    // assignments: LDS when staged, else the global scratch laid out [M][n_rows]
+    uint16_t *asg = staged ? as : asg_g + (size_t)s * n_rows + base;
+
+    auto Z = [&](uint32_t i, uint32_t j) -> float {
+        return staged ? zs[i * sd + j] : R[(size_t)(base + i) * ldr + b + j];
+    };
+    if (staged) {
+        for (uint32_t e = tid; e < L * sd; e += PQ_THREADS) {
+            uint32_t i = e / sd, j = e % sd;
+            zs[e] = R[(size_t)(base + i) * ldr + b + j];
+        }
+        __syncthreads();
+    }
+
+    // init_center (kmeans.rs:50-86)
+    if (tid < (int)sd) {
+        ACC_T acc = 0;
+        for (uint32_t i = 0; i < L; ++i) acc = acc + (ACC_T)Z(i, tid);
+        float mean;
+        if constexpr (sizeof(ACC_T) == 4) mean = acc / (float)L;
+        else mean = (float)(acc / (double)L);
+        for (uint32_t c = 0; c < KS; ++c) {
+            float v = init_offset(c, tid) > 0 ? mean + 1.0f : mean - 1.0f;
+            cent[c * sd + tid] = v;
+        }
+    }
+    __syncthreads();
+
+    uint64_t rng = stream_seed(seed, 1ull + (uint64_t)l * M + s);
+    const float EPS = 1.0f / 1024.0f;
+    for (uint32_t ep = 0; ep <= epochs; ++ep) {
+        // search_data (kmeans.rs:127-148): argmin, strict '<'
+        for (uint32_t i = tid; i < L; i += PQ_THREADS) {
+            float z[PQ_MAXSD];
+            for (uint32_t j = 0; j < sd; ++j) z[j] = Z(i, j);
+            float bd = -0.0f;
+            for (uint32_t j = 0; j < sd; ++j) {
+                float df = z[j] - cent[j];
+                bd = bd + df * df;
+            }
+            uint32_t best = 0;
+            for (uint32_t c = 1; c < KS; ++c) {
+                float dc = -0.0f;
+                for (uint32_t j = 0; j < sd; ++j) {
+                    float df = z[j] - cent[c * sd + j];
+                    dc = dc + df * df;
+                }
+                if (dc < bd) {
+                    bd = dc;
+                    best = c;
+                }
+            }
+            if (ep == epochs) codes[(size_t)(base + i) * M + s] = (uint8_t)best;
+            else asg[i] = (uint16_t)best;
+        }
+        if (ep == epochs) break;  // the search after train (pq_idx.rs:143-144)
+        __syncthreads();
+        // update_center (kmeans.rs:88-125)
+        // KS <= 256 = PQ_THREADS: each lane owns at most one centre
+        float nv[PQ_MAXSD];
+        uint32_t mycnt = 0;
+        if ((uint32_t)tid < KS) {
+            const uint32_t c = tid;
+            ACC_T acc[PQ_MAXSD];
+            for (uint32_t j = 0; j < sd; ++j) acc[j] = 0;
+            uint32_t n_c = 0;
+            for (uint32_t i = 0; i < L; ++i) {
+                if (asg[i] == c) {
+                    ++n_c;
+                    for (uint32_t j = 0; j < sd; ++j) acc[j] = acc[j] + (ACC_T)Z(i, j);
+                }
+            }
+            for (uint32_t j = 0; j < sd; ++j) {
+                if (n_c == 0) nv[j] = 0.0f;
+                else if constexpr (sizeof(ACC_T) == 4) nv[j] = acc[j] / (float)n_c;
+                else nv[j] = (float)(acc[j] / (double)n_c);
+            }
+            mycnt = n_c;
+        }
+        __syncthreads();
+        if ((uint32_t)tid < KS) {
+            for (uint32_t j = 0; j < sd; ++j) cent[tid * sd + j] = nv[j];
+            cnt[tid] = mycnt;
+        }
+        __syncthreads();
+        // split_center (kmeans.rs:150-193), all epochs but the last
+        if (ep + 1 < epochs && tid == 0) {
+            for (uint32_t i = 0; i < KS; ++i) {
+                if (cnt[i] != 0) continue;
+                uint32_t sc = (i + 1) % KS;
+                for (;;) {
+                    double pick = (double)cnt[sc] / (double)L;
+                    if (splitmix_unit(&rng) < pick) break;
+                    sc = (sc + 1) % KS;
+                }
+                for (uint32_t j = 0; j < sd; ++j) {
+                    float src = cent[sc * sd + j];
+                    if (j % 2 == 0) {
+                        cent[i * sd + j] = src * (1.0f - EPS);
+                        cent[sc * sd + j] = src * (1.0f + EPS);
+                    } else {
+                        cent[i * sd + j] = src * (1.0f + EPS);
+                        cent[sc * sd + j] = src * (1.0f - EPS);
+                    }
+                }
+                cnt[i] = cnt[sc] / 2;
+                cnt[sc] -= cnt[i];
+            }
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    for (uint32_t e = tid; e < KS * sdm; e += PQ_THREADS) {
+        uint32_t c = e / sdm, j = e % sdm;
+        cb[e] = j < sd ? cent[c * sd + j] : 0.0f;
+    }
+}
+
+// One workgroup per (query, probe): residual, LUT (pq_idx.rs:170-179), ADC
+// over the list (181-188), top-k1 by (distance, global id) kept in LDS
+// across chunks of the list.
+__global__ __launch_bounds__(256) void ivf_scan_kernel(
+    const float *__restrict__ Qp, uint32_t dpad, const float *__restrict__ C,
+    const uint32_t *__restrict__ probes, uint32_t ldp, uint32_t nprobe,
+    const uint32_t *__restrict__ off, const uint32_t *__restrict__ ids,
+    const uint8_t *__restrict__ codes, const float *__restrict__ codebooks,
+    const uint32_t *__restrict__ sub_b, const uint32_t *__restrict__ sub_e, uint32_t M,
+    uint32_t KS, uint32_t sdm, uint32_t k1, uint32_t P, unsigned long long *__restrict__ out)
+{
+    extern __shared__ __align__(16) unsigned char sc_smem[];
+    unsigned long long *keys = reinterpret_cast<unsigned long long *>(sc_smem);  // P
+    float *lut = reinterpret_cast<float *>(keys + P);                            // M*KS
+    float *z = lut + (size_t)M * KS;                                             // dpad
+    const int tid = threadIdx.x;
+    const uint32_t q = blockIdx.y, p = blockIdx.x;
+    const uint32_t l = probes[(size_t)q * ldp + p];
+    for (uint32_t j = tid; j < dpad; j += 256)
+        z[j] = Qp[(size_t)q * dpad + j] - C[(size_t)l * dpad + j];
+    __syncthreads();
+    const float *cb = codebooks + (size_t)l * M * KS * sdm;
+    for (uint32_t e = tid; e < M * KS; e += 256) {
+        uint32_t s = e / KS, c = e % KS;
+        uint32_t b = sub_b[s], sd = sub_e[s] - b;
+        const float *w = cb + (size_t)e * sdm;
+        float acc = -0.0f;
+        for (uint32_t j = 0; j < sd; ++j) {
+            float df = z[b + j] - w[j];
+            acc = acc + df * df;
+        }
+        lut[e] = acc;
+    }
+    __syncthreads();
+    const uint32_t base = off[l], L = off[l + 1] - base;
+    const uint32_t CH = P - k1;
+    uint32_t kept = 0;
+    for (uint32_t c0 = 0; c0 < L || c0 == 0; c0 += CH) {
+        const uint32_t m = L > c0 ? (L - c0 < CH ? L - c0 : CH) : 0;
+        for (uint32_t i = tid; i < m; i += 256) {
+            const uint32_t row = base + c0 + i;
+            const uint8_t *cd = codes + (size_t)row * M;
+            float dist = 0.0f;
+            for (uint32_t j = 0; j < M; ++j) dist = dist + lut[j * KS + cd[j]];
+            keys[kept + i] = ((unsigned long long)fkey(dist) << 32) | ids[row];
+        }
+        const uint32_t total = kept + m;
+        uint32_t PP = 1;
+        while (PP < total) PP <<= 1;
+        for (uint32_t j = total + tid; j < PP; j += 256) keys[j] = ~0ull;
+        __syncthreads();
+        for (uint32_t size = 2; size <= PP; size <<= 1)
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t t = tid; t < PP / 2; t += 256) {
+                    uint32_t lo = 2 * stride * (t / stride) + (t % stride), hi = lo + stride;
+                    bool asc = (lo & size) == 0;
+                    unsigned long long a = keys[lo], bb = keys[hi];
+                    if ((a > bb) == asc) {
+                        keys[lo] = bb;
+                        keys[hi] = a;
+                    }
+                }
+                __syncthreads();
+            }
+        kept = total < k1 ? total : k1;
+        if (c0 + CH >= L) break;
+    }
+    unsigned long long *o = out + ((size_t)q * nprobe + p) * k1;
+    for (uint32_t j = tid; j < k1; j += 256) o[j] = j < kept ? keys[j] : ~0ull;
+}
+
+// Per query: the nprobe*k1 candidates -> first k ascending; ROW_TIE when
+// the first min(k+1, valid) distances are not pairwise distinct or the
+// coarse probe set was tied.
+__global__ __launch_bounds__(256) void ivf_merge_kernel(
+    const unsigned long long *__restrict__ cand, uint32_t per_query, uint32_t k,
+    const uint32_t *__restrict__ coarse_flags, uint64_t *__restrict__ out_idx,
+    float *__restrict__ out_dist, uint32_t *__restrict__ out_count, uint32_t *__restrict__ flags)
+{
+    extern __shared__ __align__(16) unsigned char mg_smem[];
+    unsigned long long *keys = reinterpret_cast<unsigned long long *>(mg_smem);
+    __shared__ uint32_t s_valid, s_tie;
+    const int tid = threadIdx.x;
+    const uint32_t q = blockIdx.x;
+    if (tid == 0) {
+        s_valid = 0;
+        s_tie = 0;
+    }
+    __syncthreads();
+    uint32_t P = 1;
+    while (P < per_query) P <<= 1;
+    const unsigned long long *c = cand + (size_t)q * per_query;
+    uint32_t valid = 0;
+    for (uint32_t j = tid; j < P; j += 256) {
+        unsigned long long v = j < per_query ? c[j] : ~0ull;
+        keys[j] = v;
+        valid += v != ~0ull;
+    }
+    atomicAdd(&s_valid, valid);
+    __syncthreads();
+    for (uint32_t size = 2; size <= P; size <<= 1)
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t t = tid; t < P / 2; t += 256) {
+                uint32_t lo = 2 * stride * (t / stride) + (t % stride), hi = lo + stride;
+                bool asc = (lo & size) == 0;
+                unsigned long long a = keys[lo], bb = keys[hi];
+                if ((a > bb) == asc) {
+                    keys[lo] = bb;
+                    keys[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    const uint32_t nv = s_valid;
+    const uint32_t m = k < nv ? k : nv;
+    const uint32_t K1 = (k + 1 < nv) ? k + 1 : nv;
+    const float inf = __int_as_float(0x7f800000);
+    for (uint32_t j = tid; j < k; j += 256) {
+        if (j < m) {
+            out_idx[(size_t)q * k + j] = (uint32_t)keys[j];
+            out_dist[(size_t)q * k + j] = unkey((uint32_t)(keys[j] >> 32));
+        } else {
+            out_idx[(size_t)q * k + j] = ~0ull;
+            out_dist[(size_t)q * k + j] = inf;
+        }
+    }
+    for (uint32_t j = tid; j + 1 < K1; j += 256)
+        if ((keys[j] >> 32) == (keys[j + 1] >> 32)) s_tie = 1;
+    __syncthreads();
+    if (tid == 0) {
+        out_count[q] = m;
+        flags[q] = (s_tie || (coarse_flags[q] & ROW_TIE)) ? ROW_TIE : 0u;
+    }
+}
+
+// ---- std BinaryHeap (max-heap on distance) over a per-thread array
+struct GHeap {
+    float *d;
+    uint32_t *ix;
+    uint32_t len;
+    __device__ void sift_up(uint32_t start, uint32_t pos)
+    {
+        float ed = d[pos];
+        uint32_t ei = ix[pos];
+        while (pos > start) {
+            uint32_t parent = (pos - 1) / 2;
+            if (ed <= d[parent]) break;
+            d[pos] = d[parent];
+            ix[pos] = ix[parent];
+            pos = parent;
+        }
+        d[pos] = ed;
+        ix[pos] = ei;
+    }
+    __device__ void sift_down_to_bottom(uint32_t pos)
+    {
+        uint32_t end = len, start = pos;
+        float ed = d[pos];
+        uint32_t ei = ix[pos];
+        uint32_t child = 2 * pos + 1;
+        while (end >= 2 && child < end - 1) {
+            child += (d[child] <= d[child + 1]) ? 1u : 0u;
+            d[pos] = d[child];
+            ix[pos] = ix[child];
+            pos = child;
+            child = 2 * pos + 1;
+        }
+        if (end >= 1 && child == end - 1) {
+            d[pos] = d[child];
+            ix[pos] = ix[child];
+            pos = child;
+        }
+        d[pos] = ed;
+        ix[pos] = ei;
+        sift_up(start, pos);
+    }
+    __device__ void push(float x, uint32_t i)
+    {
+        d[len] = x;
+        ix[len] = i;
+        len += 1;
+        sift_up(0, len - 1);
+    }
+    __device__ void pop(float *od, uint32_t *oi)
+    {
+        float ld = d[len - 1];
+        uint32_t li = ix[len - 1];
+        len -= 1;
+        if (len > 0) {
+            *od = d[0];
+            *oi = ix[0];
+            d[0] = ld;
+            ix[0] = li;
+            sift_down_to_bottom(0);
+        } else {
+            *od = ld;
+            *oi = li;
+        }
+    }
+};
+
+// Exact replay of IVFPQIndex::search_knn_adc + node_search_k
+// (pq_idx.rs:408-477) for flagged queries, one lane per query: coarse heap
+// of Neighbor(i, -dist) over all lists, nprobe pops, per list an unbounded
+// heap trimmed to k, popped into the global heap trimmed to k.
+__global__ void ivf_replay_kernel(const uint32_t *__restrict__ rows, uint32_t nrows,
+                                  const float *__restrict__ Dc, uint32_t ldd, uint32_t nlist,
+                                  uint32_t nprobe, const float *__restrict__ Qp, uint32_t dpad,
+                                  const float *__restrict__ C, const uint32_t *__restrict__ off,
+                                  const uint32_t *__restrict__ ids,
+                                  const uint8_t *__restrict__ codes,
+                                  const float *__restrict__ codebooks,
+                                  const uint32_t *__restrict__ sub_b,
+                                  const uint32_t *__restrict__ sub_e, uint32_t M, uint32_t KS,
+                                  uint32_t sdm, uint32_t k, uint32_t lmax, float *scratch_f,
+                                  uint32_t *scratch_u, size_t per_thread, uint64_t *out_idx,
+                                  float *out_dist, uint32_t *out_count)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrows) return;
+    const uint32_t q = rows[t];
+    float *fbase = scratch_f + (size_t)t * per_thread;
+    uint32_t *ubase = scratch_u + (size_t)t * per_thread;
+    // layout: coarse heap [nlist] | list heap [lmax] | global heap [k+1] | lut [M*KS]
+    GHeap coarse{fbase, ubase, 0};
+    GHeap lh{fbase + nlist, ubase + nlist, 0};
+    GHeap top{fbase + nlist + lmax, ubase + nlist + lmax, 0};
+    float *lut = fbase + nlist + lmax + (k + 1);
+    const float *drow = Dc + (size_t)q * ldd;
+    for (uint32_t i = 0; i < nlist; ++i) coarse.push(-drow[i], i);
+    const float *qv = Qp + (size_t)q * dpad;
+    for (uint32_t p = 0; p < nprobe && coarse.len > 0; ++p) {
+        float xd;
+        uint32_t l;
+        coarse.pop(&xd, &l);
+        const float *cb = codebooks + (size_t)l * M * KS * sdm;
+        for (uint32_t e = 0; e < M * KS; ++e) {
+            uint32_t s = e / KS, b = sub_b[s], sd = sub_e[s] - b;
+            float acc = -0.0f;
+            for (uint32_t j = 0; j < sd; ++j) {
+                float zj = qv[b + j] - C[(size_t)l * dpad + b + j];
+                float df = zj - cb[(size_t)e * sdm + j];
+                acc = acc + df * df;
+            }
+            lut[e] = acc;
+        }
+        const uint32_t base = off[l], L = off[l + 1] - base;
+        lh.len = 0;
+        for (uint32_t i = 0; i < L; ++i) {
+            const uint8_t *cd = codes + (size_t)(base + i) * M;
+            float dist = 0.0f;
+            for (uint32_t j = 0; j < M; ++j) dist = dist + lut[j * KS + cd[j]];
+            lh.push(dist, i);
+        }
+        while (lh.len > k) {
+            float a;
+            uint32_t b;
+            lh.pop(&a, &b);
+        }
+        while (lh.len > 0) {
+            float a;
+            uint32_t b;
+            lh.pop(&a, &b);
+            top.push(a, ids[base + b]);
+            if (top.len > k) {
+                float a2;
+                uint32_t b2;
+                top.pop(&a2, &b2);
+            }
+        }
+    }
+    const uint32_t cnt = top.len;
+    for (uint32_t p = cnt; p > 0; --p) {
+        float a;
+        uint32_t b;
+        top.pop(&a, &b);
+        out_idx[(size_t)q * k + p - 1] = b;
+        out_dist[(size_t)q * k + p - 1] = a;
+    }
+    const float inf = __int_as_float(0x7f800000);
+    for (uint32_t j = cnt; j < k; ++j) {
+        out_idx[(size_t)q * k + j] = ~0ull;
+        out_dist[(size_t)q * k + j] = inf;
+    }
+    out_count[q] = cnt;
+}
+
+inline unsigned grid_for(uint64_t work, unsigned cap = 8192)
+{
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((work + 255) / 256, cap));
+}
+
+}  // namespace
+
+void launch_coarse_init(bool fp64, const float *X, uint32_t ldx, uint64_t n, uint32_t d,
+                        uint32_t nc, float *C, uint32_t ldc, hipStream_t s)
+{
+    ProfScope prof("kmeans_init", s);
+    if (fp64)
+        coarse_init_kernel<double><<<ceil_div(ldc, 64), 64, 0, s>>>(X, ldx, n, d, nc, C, ldc);
+    else
+        coarse_init_kernel<float><<<ceil_div(ldc, 64), 64, 0, s>>>(X, ldx, n, d, nc, C, ldc);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_unpack_assign(const unsigned long long *best, uint64_t n, uint32_t *assign,
+                          hipStream_t s)
+{
+    unpack_assign_kernel<<<grid_for(n), 256, 0, s>>>(best, n, assign);
+    HM_CHECK_LAUNCH();
+}
+
+void build_lists(const uint32_t *assign, uint64_t n, uint32_t nlist, uint32_t *sorted_keys,
+                 uint32_t *ids, uint32_t *off, DeviceBuffer &iota, DeviceBuffer &tmp,
+                 hipStream_t s)
+{
+    ProfScope prof("build_lists", s);
+    iota.reserve(n * sizeof(uint32_t));
+    iota_kernel<<<grid_for(n), 256, 0, s>>>(iota.as<uint32_t>(), n);
+    HM_CHECK_LAUNCH();
+    size_t tb = 0;
+    int bits = 1;
+    while ((1u << bits) < nlist + 1 && bits < 32) ++bits;
+    HM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, assign, sorted_keys,
+                                              iota.as<uint32_t>(), ids, (int)n, 0, bits, s));
+    tmp.reserve(tb);
+    HM_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.get(), tb, assign, sorted_keys,
+                                              iota.as<uint32_t>(), ids, (int)n, 0, bits, s));
+    offsets_kernel<<<ceil_div(nlist + 1, 256), 256, 0, s>>>(sorted_keys, n, nlist, off);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_update(bool fp64, const float *X, uint32_t ldx, const uint32_t *off,
+                   const uint32_t *ids, uint32_t nlist, uint32_t d, float *C, uint32_t ldc,
+                   hipStream_t s)
+{
+    ProfScope prof("kmeans_update", s);
+    dim3 grid(nlist, ceil_div(ldc, 64));
+    if (fp64)
+        update_kernel<double><<<grid, 64, 0, s>>>(X, ldx, off, ids, d, C, ldc);
+    else
+        update_kernel<float><<<grid, 64, 0, s>>>(X, ldx, off, ids, d, C, ldc);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_residuals(const float *X, uint32_t ldx, const uint32_t *ids, const uint32_t *lists,
+                      const float *C, uint64_t n, uint32_t dpad, float *R, hipStream_t s)
+{
+    residual_kernel<<<grid_for(n * dpad), 256, 0, s>>>(X, ldx, ids, lists, C, n, dpad, R);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_pq_train(bool fp64, const float *R, uint32_t ldr, const uint32_t *off,
+                     uint32_t nlist, uint32_t M, const uint32_t *sub_b, const uint32_t *sub_e,
+                     uint32_t KS, uint32_t sd_max, uint32_t sdm, uint32_t epochs, uint64_t seed,
+                     uint32_t max_list, uint64_t n, uint16_t *asg_scratch, float *codebooks,
+                     uint8_t *codes, hipStream_t s)
+{
+    if (sd_max > (uint32_t)PQ_MAXSD) fail(HM_ERR_NOT_IMPLEMENTED, "sub-space dimension above 32");
+    if (KS > 256) fail(HM_ERR_NOT_IMPLEMENTED, "sub_bits above 8");
+    const size_t fixed = (size_t)KS * sd_max * 4 + (size_t)KS * 4;
+    const size_t stage = (size_t)max_list * sd_max * 4 + (size_t)max_list * 2 + 16;
+    const int staged = fixed + stage <= 56 * 1024 ? 1 : 0;
+    const size_t lds = fixed + (staged ? stage : 0);
+    if (n > 0xFFFFFFFFull) fail(HM_ERR_CAPACITY, "too many rows");
+    const dim3 grid(nlist * M);
+    ProfScope prof("pq_train", s);
+    if (fp64)
+        pq_train_kernel<double><<<grid, PQ_THREADS, lds, s>>>(R, ldr, off, M, sub_b, sub_e, KS,
+                                                              sdm, epochs, seed, staged, n,
+                                                              asg_scratch, codebooks, codes);
+    else
+        pq_train_kernel<float><<<grid, PQ_THREADS, lds, s>>>(R, ldr, off, M, sub_b, sub_e, KS,
+                                                             sdm, epochs, seed, staged, n,
+                                                             asg_scratch, codebooks, codes);
+    HM_CHECK_LAUNCH();
+}
+
+uint32_t ivf_scan_pool(uint32_t k1) { return k1 <= 1024 ? 4096u : 8192u; }
+
+void launch_ivf_scan(const float *Qp, uint32_t dpad, const float *C, const uint32_t *probes,
+                     uint32_t ldp, uint32_t nprobe, uint32_t rows, const uint32_t *off,
+                     const uint32_t *ids, const uint8_t *codes, const float *codebooks,
+                     const uint32_t *sub_b, const uint32_t *sub_e, uint32_t M, uint32_t KS,
+                     uint32_t sdm, uint32_t k1, unsigned long long *out, hipStream_t s)
+{
+    if (rows == 0 || nprobe == 0) return;
+    const uint32_t P = ivf_scan_pool(k1);
+    if (k1 * 2 > P) fail(HM_ERR_INVALID_ARG, "k too large for the list scan (max 4095)");
+    const size_t lds = (size_t)P * 8 + (size_t)M * KS * 4 + (size_t)dpad * 4;
+    if (lds > 160 * 1024) fail(HM_ERR_INVALID_ARG, "list scan does not fit in LDS");
+    ProfScope prof("ivf_scan", s);
+    ivf_scan_kernel<<<dim3(nprobe, rows), 256, lds, s>>>(Qp, dpad, C, probes, ldp, nprobe, off,
+                                                          ids, codes, codebooks, sub_b, sub_e, M,
+                                                          KS, sdm, k1, P, out);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_ivf_merge(const unsigned long long *cand, uint32_t per_query, uint32_t rows,
+                      uint32_t k, const uint32_t *coarse_flags, uint64_t *out_idx,
+                      float *out_dist, uint32_t *out_count, uint32_t *flags, hipStream_t s)
+{
+    if (rows == 0) return;
+    uint32_t P = 1;
+    while (P < per_query) P <<= 1;
+    if (P > 16384) fail(HM_ERR_INVALID_ARG, "nprobe*(k+1) above 16384");
+    ProfScope prof("ivf_merge", s);
+    ivf_merge_kernel<<<rows, 256, (size_t)P * 8, s>>>(cand, per_query, k, coarse_flags, out_idx,
+                                                      out_dist, out_count, flags);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_ivf_replay(const uint32_t *rows, uint32_t nrows, const float *Dc, uint32_t ldd,
+                       uint32_t nlist, uint32_t nprobe, const float *Qp, uint32_t dpad,
+                       const float *C, const uint32_t *off, const uint32_t *ids,
+                       const uint8_t *codes, const float *codebooks, const uint32_t *sub_b,
+                       const uint32_t *sub_e, uint32_t M, uint32_t KS, uint32_t sdm, uint32_t k,
+                       uint32_t lmax, float *scratch_f, uint32_t *scratch_u, size_t per_thread,
+                       uint64_t *out_idx, float *out_dist, uint32_t *out_count, hipStream_t s)
+{
+    if (nrows == 0) return;
+    ProfScope prof("ivf_replay", s);
+    ivf_replay_kernel<<<ceil_div(nrows, 64), 64, 0, s>>>(
+        rows, nrows, Dc, ldd, nlist, nprobe, Qp, dpad, C, off, ids, codes, codebooks, sub_b,
+        sub_e, M, KS, sdm, k, lmax, scratch_f, scratch_u, per_thread, out_idx, out_dist,
+        out_count);
+    HM_CHECK_LAUNCH();
+}
+
+void split_centers_host(float *C, uint32_t ldc, uint32_t nc, uint32_t dim, uint64_t n,
+                        uint64_t *counts, uint64_t *rng)
+{
+    if (n == 0) fail(HM_ERR_EMPTY, "None to assigned impossible split center");
+    const float EPS = 1.0f / 1024.0f;
+    for (uint32_t i = 0; i < nc; ++i) {
+        if (counts[i] != 0) continue;
+        uint32_t sc = (i + 1) % nc;
+        for (;;) {
+            double pick = (double)counts[sc] / (double)n;
+            if (splitmix_unit(rng) < pick) break;
+            sc = (sc + 1) % nc;
+        }
+        for (uint32_t j = 0; j < dim; ++j) {
+            float src = C[(size_t)sc * ldc + j];
+            if (j % 2 == 0) {
+                C[(size_t)i * ldc + j] = src * (1.0f - EPS);
+                C[(size_t)sc * ldc + j] = src * (1.0f + EPS);
+            } else {
+                C[(size_t)i * ldc + j] = src * (1.0f + EPS);
+                C[(size_t)sc * ldc + j] = src * (1.0f - EPS);
+            }
+        }
+        counts[i] = counts[sc] / 2;
+        counts[sc] -= counts[i];
+    }
+}
+
+uint64_t host_stream_seed(uint64_t seed, uint64_t instance) { return stream_seed(seed, instance); }
+
+}  // namespace hm

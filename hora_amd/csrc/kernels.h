@@ -11,6 +11,7 @@ enum EpilogueKind { EPI_NONE = 0, EPI_COSINE = 1, EPI_ANGULAR = 2 };
 // Row flags written by the selection kernels.
 constexpr uint32_t ROW_TIE = 1u;   // equal distances among the first min(k+1, n)
 constexpr uint32_t ROW_NAN = 2u;   // a NaN distance in the row
+constexpr uint32_t ROW_UNCERT = 4u;  // shortlist could not be certified
 
 // ---- exact_dist.hip
 // D[q][j] = metric(Q[q], B[j]) in the reference's summation order.
@@ -18,6 +19,11 @@ constexpr uint32_t ROW_NAN = 2u;   // a NaN distance in the row
 void launch_exact_tile(int metric, const float *Q, uint32_t ldq, uint32_t nq, const float *B,
                        uint32_t ldb, uint32_t n, uint32_t dpad, const float *qnorm,
                        const float *bnorm, float *D, uint64_t ldd, hipStream_t s);
+// best[q] = min over j of (fkey(metric(Q[q], B[j])) << 32 | j); best must be
+// initialised to ~0 by the caller.
+void launch_exact_argmin(int metric, const float *Q, uint32_t ldq, uint32_t nq, const float *B,
+                         uint32_t ldb, uint32_t n, uint32_t dpad, const float *qnorm,
+                         const float *bnorm, unsigned long long *best, hipStream_t s);
 // out[r] = sum_t X[r][t]^2 (left fold), t < d.
 void launch_self_dot(const float *X, uint32_t ldx, uint32_t rows, uint32_t d, float *out,
                      hipStream_t s);
@@ -49,5 +55,33 @@ void launch_check_finite(const float *X, uint64_t ld, uint64_t rows, uint32_t d,
 // dst[r][0..d) = src[r][0..d), dst[r][d..dpad) = 0.
 void launch_pad_rows(const float *src, uint64_t lds, float *dst, uint64_t ldd, uint64_t rows,
                      uint32_t d, uint32_t dpad, hipStream_t s);
+
+// ---- mfma_scores.hip
+// Kmat[q][j] = approximate ranking key of base row j for query q (see file
+// header); Q, B are bf16 (elem_bytes 2) or fp8 (1) rows of row_bytes bytes
+// (a multiple of 64, zero padded).
+void launch_score_gemm(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
+                       uint32_t row_bytes, const float *qscale, const float *bscale,
+                       const float *bnorm2, const float *inv_bnorm, int metric, float *Kmat,
+                       uint64_t ldk, hipStream_t s);
+void launch_to_bf16(const float *src, uint64_t lds, uint64_t rows, uint32_t d, uint32_t dk,
+                    uint16_t *dst, uint32_t *inexact, hipStream_t s);
+void launch_to_fp8(const float *src, uint64_t lds, uint32_t rows, uint32_t d, uint32_t dk,
+                   uint8_t *dst, float *dequant, hipStream_t s);
+void launch_norm_stats(const float *norm2, uint32_t rows, float *inv, uint32_t *maxbits,
+                       hipStream_t s);
+
+// ---- rerank.hip
+void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
+                   const uint32_t *cand_i, const float *cand_key, uint32_t ldc, uint32_t s,
+                   const float *qnorm2, const float *bnorm2, uint32_t rows, uint32_t n,
+                   uint32_t k, float bmax, float gamma, float c_in_exact, float c_in_inexact,
+                   const uint32_t *q_inexact, int certify, uint64_t *out_idx, float *out_dist,
+                   uint32_t *out_count, uint32_t *flags, hipStream_t st);
+void launch_gather_rows(const float *src, uint64_t lds, const uint32_t *rows, uint32_t nrows,
+                        uint32_t d, float *dst, uint64_t ldd, hipStream_t s);
+void launch_scatter_results(const uint64_t *si, const float *sd, const uint32_t *sc,
+                            const uint32_t *rows, uint32_t nrows, uint32_t k, uint64_t *di,
+                            float *dd, uint32_t *dc, hipStream_t s);
 
 }  // namespace hm

@@ -1,0 +1,251 @@
+// rerank.hip -- exact FP32 rerank of an MFMA shortlist, with certification.
+//
+// For each query the shortlist's candidates are rescored with the
+// reference-order arithmetic (the same left folds as exact_dist.hip, so each
+// value equals the reference's f32 result), sorted by (distance, position),
+// and the first min(k, n) are the answer when two conditions hold:
+//  * certified: every row outside the shortlist is provably farther than the
+//    (k+1)-th exact distance inside it.  Outside rows have approximate key
+//    >= key_s (the shortlist threshold), and |approx - exact| <= eps_q, an
+//    a-priori bound on the BF16/FP32 rounding (documented per metric below),
+//    so exact >= LB(key_s) - eps_q for all of them.
+//  * tie-free: the k+1 smallest exact distances are pairwise distinct (the
+//    reference's heap then returns them sorted, bruteforce_idx.rs:47-73).
+// Rows that fail either test are flagged and the caller reruns them on the
+// full exact path (which replays the heap for ties).
+#include "kernels.h"
+#include "profiler.h"
+
+#pragma clang fp contract(off)
+
+namespace hm {
+
+namespace {
+
+constexpr int RR_THREADS = 256;
+constexpr uint32_t RR_CAP = 8192;
+constexpr uint32_t RR_MAX_DIM = 4096;
+
+__device__ __forceinline__ uint32_t fkey(float f)
+{
+    f = f + 0.0f;
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+template <int ACC, int EPI>
+__global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
+    const float *__restrict__ Qp, uint32_t dpad, const float *__restrict__ Bp,
+    const uint32_t *__restrict__ cand_i, const float *__restrict__ cand_key, uint32_t ldc,
+    uint32_t s, const float *__restrict__ qnorm2, const float *__restrict__ bnorm2, uint32_t n,
+    uint32_t k, float bmax, float gamma, float c_in_exact, float c_in_inexact,
+    const uint32_t *__restrict__ q_inexact, int certify, uint64_t *__restrict__ out_idx,
+    float *__restrict__ out_dist, uint32_t *__restrict__ out_count, uint32_t *__restrict__ flags)
+{
+    // dynamic LDS: keys[P] (u64) | vals[P] | sq[dpad]
+    extern __shared__ __align__(16) unsigned char rr_smem[];
+    __shared__ uint32_t s_tie;
+    uint32_t P = 1;
+    while (P < s) P <<= 1;
+    unsigned long long *keys = reinterpret_cast<unsigned long long *>(rr_smem);
+    float *vals = reinterpret_cast<float *>(rr_smem + (size_t)P * 8);
+    float *sq = reinterpret_cast<float *>(rr_smem + (size_t)P * 12);
+
+    const int tid = threadIdx.x;
+    const uint32_t r = blockIdx.x;
+    if (tid == 0) s_tie = 0;
+    for (uint32_t t = tid; t < dpad; t += RR_THREADS) sq[t] = Qp[(size_t)r * dpad + t];
+    __syncthreads();
+
+    const uint32_t *ci = cand_i + (size_t)r * ldc;
+    const float qn2 = qnorm2[r];  // sum q_t^2 (left fold), needed by every bound
+    for (uint32_t j = tid; j < s; j += RR_THREADS) {
+        const uint32_t idx = ci[j];
+        const float4 *b4 = reinterpret_cast<const float4 *>(Bp + (size_t)idx * dpad);
+        float acc = -0.0f;
+        for (uint32_t t4 = 0; t4 < dpad / 4; ++t4) {
+            const float4 bv = b4[t4];
+            const float bb[4] = {bv.x, bv.y, bv.z, bv.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float q = sq[t4 * 4 + e];
+                if constexpr (ACC == ACC_EUCLID) {
+                    float df = q - bb[e];
+                    float sqr = df * df;
+                    acc = acc + sqr;
+                } else if constexpr (ACC == ACC_MANHATTAN) {
+                    float df = q - bb[e];
+                    acc = acc + fabsf(df);
+                } else {
+                    float p = q * bb[e];
+                    acc = acc + p;
+                }
+            }
+        }
+        float E = acc;
+        if constexpr (EPI == EPI_COSINE) {
+            float den = sqrtf(qn2) * sqrtf(bnorm2[idx]);
+            E = (-acc) / den;
+        } else if constexpr (EPI == EPI_ANGULAR) {
+            float rlmul = (-qn2) * (-bnorm2[idx]);
+            if (rlmul > 0.0f) {
+                float t = 1.0f * (-acc);
+                t = t / sqrtf(rlmul);
+                E = 1.0f - t;
+            } else {
+                E = 1.0f;
+            }
+        }
+        vals[j] = E;
+        keys[j] = ((unsigned long long)fkey(E) << 32) | j;
+    }
+    for (uint32_t j = s + tid; j < P; j += RR_THREADS) keys[j] = ~0ull;
+    __syncthreads();
+    for (uint32_t size = 2; size <= P; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t t = tid; t < P / 2; t += RR_THREADS) {
+                uint32_t lo = 2 * stride * (t / stride) + (t % stride);
+                uint32_t hi = lo + stride;
+                bool asc = (lo & size) == 0;
+                unsigned long long a = keys[lo], b = keys[hi];
+                if ((a > b) == asc) {
+                    keys[lo] = b;
+                    keys[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    const uint32_t m = k < n ? k : n;
+    const uint32_t K1 = (k + 1 < s) ? k + 1 : s;  // entries checked for ties
+    uint64_t *oi = out_idx + (size_t)r * k;
+    float *od = out_dist + (size_t)r * k;
+    const float inf = __int_as_float(0x7f800000);
+    for (uint32_t j = tid; j < k; j += RR_THREADS) {
+        if (j < m && j < s) {
+            const uint32_t pos = (uint32_t)keys[j];
+            oi[j] = ci[pos];
+            od[j] = vals[pos];
+        } else {
+            oi[j] = ~0ull;
+            od[j] = inf;
+        }
+        if (j + 1 < K1 && (uint32_t)(keys[j] >> 32) == (uint32_t)(keys[j + 1] >> 32)) s_tie = 1;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    out_count[r] = m < s ? m : s;
+    uint32_t f = s_tie ? ROW_TIE : 0u;
+    if (s < n && certify) {
+        bool ok = false;
+        if (k + 1 <= s) {
+            const float ek1 = vals[(uint32_t)keys[k]];
+            const float key_s = cand_key[(size_t)r * ldc + s - 1];
+            const double c_in = q_inexact && *q_inexact ? c_in_inexact : c_in_exact;
+            const double g = gamma, u = 5.9604644775390625e-8;  // 2^-24
+            const double nq = sqrt((double)qn2);
+            double lb;
+            if constexpr (ACC == ACC_EUCLID) {
+                // |-2(s~ - s)| <= 2(c_in + g)|q||b|; norms: g(|q|^2+|b|^2); the
+                // reference's own fold: g * E; the combination: 4u(|q|+|b|)^2.
+                double tot = (nq + bmax) * (nq + bmax);
+                double eps = 2.0 * (c_in + g) * nq * bmax + g * ((double)qn2 + (double)bmax * bmax) +
+                             (g + 4.0 * u) * tot;
+                lb = (double)qn2 + (double)key_s - 1.01 * eps - 1e-30;
+            } else if constexpr (EPI == EPI_NONE) {  // DotProduct
+                double eps = (c_in + 2.0 * g + 2.0 * u) * nq * bmax;
+                lb = (double)key_s - 1.01 * eps - 1e-30;
+            } else {  // Cosine / Angular in units of cos
+                double eps = c_in + 6.0 * g + 16.0 * u;
+                double base = (double)key_s / nq;
+                lb = (EPI == EPI_COSINE ? base : 1.0 + base) - 1.01 * eps;
+            }
+            ok = (double)ek1 < lb;
+        }
+        if (!ok) f |= ROW_UNCERT;
+    }
+    flags[r] = f;
+}
+
+__global__ void gather_rows_kernel(const float *__restrict__ src, uint64_t lds,
+                                   const uint32_t *__restrict__ rows, uint32_t nrows, uint32_t d,
+                                   float *__restrict__ dst, uint64_t ldd)
+{
+    uint64_t total = (uint64_t)nrows * d;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t r = e / d, c = e % d;
+        dst[r * ldd + c] = src[(uint64_t)rows[r] * lds + c];
+    }
+}
+
+__global__ void scatter_results_kernel(const uint64_t *__restrict__ si, const float *__restrict__ sd,
+                                       const uint32_t *__restrict__ sc,
+                                       const uint32_t *__restrict__ rows, uint32_t nrows,
+                                       uint32_t k, uint64_t *__restrict__ di,
+                                       float *__restrict__ dd, uint32_t *__restrict__ dc)
+{
+    uint64_t total = (uint64_t)nrows * k;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t r = e / k, j = e % k;
+        di[(uint64_t)rows[r] * k + j] = si[e];
+        dd[(uint64_t)rows[r] * k + j] = sd[e];
+    }
+    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < nrows;
+         r += (uint64_t)gridDim.x * blockDim.x)
+        dc[rows[r]] = sc[r];
+}
+
+}  // namespace
+
+void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
+                   const uint32_t *cand_i, const float *cand_key, uint32_t ldc, uint32_t s,
+                   const float *qnorm2, const float *bnorm2, uint32_t rows, uint32_t n,
+                   uint32_t k, float bmax, float gamma, float c_in_exact, float c_in_inexact,
+                   const uint32_t *q_inexact, int certify, uint64_t *out_idx, float *out_dist,
+                   uint32_t *out_count, uint32_t *flags, hipStream_t st)
+{
+    if (rows == 0) return;
+    if (s == 0 || s > RR_CAP) fail(HM_ERR_INVALID_ARG, "rerank: shortlist size out of range");
+    if (dpad > RR_MAX_DIM) fail(HM_ERR_INVALID_ARG, "rerank: dimension above 4096");
+    ProfScope prof("rerank", st);
+    uint32_t P = 1;
+    while (P < s) P <<= 1;
+    const size_t lds = (size_t)P * 12 + (size_t)dpad * 4;
+#define HM_RR(ACC, EPI)                                                                        \
+    rerank_kernel<ACC, EPI><<<rows, RR_THREADS, lds, st>>>(                                     \
+        Qp, dpad, Bp, cand_i, cand_key, ldc, s, qnorm2, bnorm2, n, k, bmax, gamma, c_in_exact, \
+        c_in_inexact, q_inexact, certify, out_idx, out_dist, out_count, flags)
+    switch (metric) {
+    case HM_METRIC_EUCLIDEAN: HM_RR(ACC_EUCLID, EPI_NONE); break;
+    case HM_METRIC_DOT_PRODUCT: HM_RR(ACC_DOT, EPI_NONE); break;
+    case HM_METRIC_COSINE_SIMILARITY: HM_RR(ACC_DOT, EPI_COSINE); break;
+    case HM_METRIC_ANGULAR: HM_RR(ACC_DOT, EPI_ANGULAR); break;
+    default: fail(HM_ERR_INVALID_ARG, "rerank: metric has no shortlist form");
+    }
+#undef HM_RR
+    HM_CHECK_LAUNCH();
+}
+
+void launch_gather_rows(const float *src, uint64_t lds, const uint32_t *rows, uint32_t nrows,
+                        uint32_t d, float *dst, uint64_t ldd, hipStream_t s)
+{
+    if (nrows == 0) return;
+    unsigned blocks = (unsigned)std::min<uint64_t>(((uint64_t)nrows * d + 255) / 256, 4096);
+    gather_rows_kernel<<<blocks, 256, 0, s>>>(src, lds, rows, nrows, d, dst, ldd);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_scatter_results(const uint64_t *si, const float *sd, const uint32_t *sc,
+                            const uint32_t *rows, uint32_t nrows, uint32_t k, uint64_t *di,
+                            float *dd, uint32_t *dc, hipStream_t s)
+{
+    if (nrows == 0) return;
+    unsigned blocks = (unsigned)std::min<uint64_t>(((uint64_t)nrows * std::max(k, 1u) + 255) / 256, 4096);
+    scatter_results_kernel<<<blocks, 256, 0, s>>>(si, sd, sc, rows, nrows, k, di, dd, dc);
+    HM_CHECK_LAUNCH();
+}
+
+}  // namespace hm

@@ -267,7 +267,14 @@ class BruteForceIndex(_Index):
         super().__init__(dimension, None)
 
     def set_search_mode(self, mode: int, shortlist: int = 0) -> None:
+        """EXACT, BF16_RERANK or FP8_RERANK; shortlist 0 = automatic."""
         check(lib().hm_index_set_search_mode(self._h, mode, shortlist))
+
+    def search_stats(self) -> Tuple[int, int]:
+        """(rows rerun on the exact path, rows uncertified) of the last search."""
+        a, b = C.c_uint64(), C.c_uint64()
+        check(lib().hm_index_search_stats(self._h, C.byref(a), C.byref(b)))
+        return int(a.value), int(b.value)
 
 
 class IVFPQIndex(_Index):
@@ -276,6 +283,32 @@ class IVFPQIndex(_Index):
 
     def __init__(self, dimension: int, params: Optional[IVFPQParams] = None):
         super().__init__(dimension, params if params is not None else IVFPQParams())
+
+    def set_search_n_center(self, nprobe: int) -> None:
+        check(lib().hm_index_set_nprobe(self._h, nprobe))
+
+    def info(self) -> dict:
+        out = np.zeros(8, np.uint32)
+        check(lib().hm_ivfpq_info(self._h, out.ctypes.data))
+        keys = ["nlist", "n_sub", "n_sub_center", "sub_dim_max", "n_trained", "trained",
+                "nprobe", "dpad"]
+        return dict(zip(keys, out.tolist()))
+
+    def export(self) -> dict:
+        """Trained state as arrays (centers, list_offsets, list_ids, codebooks,
+        codes); see hm_ivfpq_export."""
+        i = self.info()
+        nl, m, ks, sdm, n = i["nlist"], i["n_sub"], i["n_sub_center"], i["sub_dim_max"], \
+            i["n_trained"]
+        cen = np.zeros((nl, self._dim), np.float32)
+        off = np.zeros(nl + 1, np.uint64)
+        ids = np.zeros(max(n, 1), np.uint64)
+        cb = np.zeros((nl, m, ks, sdm), np.float32)
+        codes = np.zeros((max(n, 1), m), np.uint8)
+        check(lib().hm_ivfpq_export(self._h, cen.ctypes.data, off.ctypes.data, ids.ctypes.data,
+                                    cb.ctypes.data, codes.ctypes.data))
+        return {"centers": cen, "list_offsets": off, "list_ids": ids[:n], "codebooks": cb,
+                "codes": codes[:n]}
 
 
 PQIVFIndex = IVFPQIndex  # README.md:57 name

@@ -131,6 +131,11 @@ HM_API int32_t hm_index_search_device(hm_index *h, const float *d_queries, uint6
  * (0 = automatic).  No reference counterpart (the reference has one form). */
 HM_API int32_t hm_index_set_search_mode(hm_index *h, int32_t mode, uint32_t shortlist);
 
+/* Rows of the last search rerun on the exact path (tied or uncertified
+ * shortlist) and how many were uncertified.  No reference counterpart. */
+HM_API int32_t hm_index_search_stats(const hm_index *h, uint64_t *fallback,
+                                     uint64_t *uncertified);
+
 /* ANNIndex::nodes_size (ann_index.rs:105) -- number of rows added. */
 HM_API int32_t hm_index_nodes_size(const hm_index *h, uint64_t *out);
 
@@ -147,6 +152,24 @@ HM_API int32_t hm_index_clear(hm_index *h);
  * search_nodes returns with each Node (ann_index.rs:81-84). */
 HM_API int32_t hm_index_get_rows(const hm_index *h, const uint64_t *idx, uint64_t n,
                                  float *out);
+
+/* IVFPQIndex: number of lists probed per query, the search_n_center of
+ * IVFPQParams (pq_params.rs:53-55, 592-595) changed after construction. */
+HM_API int32_t hm_index_set_nprobe(hm_index *h, uint32_t nprobe);
+
+/* IVFPQIndex shape after build: out8 = {n_kmeans_center, n_sub, 2^sub_bits,
+ * sub_dim_max, rows trained, trained flag, nprobe, padded dimension}.  The
+ * reference keeps these in private fields (pq_idx.rs:268-292). */
+HM_API int32_t hm_ivfpq_info(hm_index *h, uint32_t *out8);
+
+/* IVFPQIndex trained state (_centers, _ivflist, per-list _centers and
+ * _assigned_center of pq_idx.rs:280-282, 33, 41) as arrays:
+ * centers [nlist][dim] f32; list_offsets [nlist+1]; list_ids [n] ascending
+ * within each list; codebooks [nlist][n_sub][2^sub_bits][sub_dim_max] f32
+ * (a sub-space with fewer dims is zero padded); codes [n][n_sub] u8 with row
+ * r belonging to list_ids[r].  Any output pointer may be NULL. */
+HM_API int32_t hm_ivfpq_export(hm_index *h, float *centers, uint64_t *list_offsets,
+                               uint64_t *list_ids, float *codebooks, uint8_t *codes);
 
 /* Per-kernel timing with HIP events recorded on the launch stream around
  * each launch (off by default; instrumentation, no reference counterpart).

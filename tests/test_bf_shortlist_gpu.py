@@ -1,0 +1,96 @@
+"""BruteForceIndex shortlist forms (BF16 / FP8 MFMA shortlist + exact FP32
+rerank) vs the oracle.
+
+BF16 is certified: every returned row set is provably the reference's, so
+ids and distances must be bit-exact.  FP8 is not certified: its recall@10 is
+measured against the oracle (tolerance written below) and every distance it
+returns must still be the exact reference value for that pair."""
+import numpy as np
+import pytest
+
+import hora_amd
+from hora_amd import BruteForceIndex, Metric
+from hora_amd.datasets import bf_normal, bf_sparse_u8, unit_normal
+
+pytestmark = pytest.mark.gpu
+
+GEMM_METRICS = [Metric.Euclidean, Metric.DotProduct, Metric.CosineSimilarity, Metric.Angular]
+
+
+def _index(base, mt, mode, shortlist=0):
+    idx = BruteForceIndex(base.shape[1])
+    idx.add_batch(base)
+    idx.build(mt)
+    idx.set_search_mode(mode, shortlist)
+    return idx
+
+
+def _exact_check(orc, idx, base, q, k, mt):
+    gi, gd, gc = idx.search_batch(q, k)
+    ri, rd, rc = orc.bf_search(base, q, k, int(mt))
+    assert np.array_equal(gc, rc)
+    assert np.array_equal(gi, ri)
+    assert np.array_equal(gd, rd)
+    return idx.search_stats()
+
+
+@pytest.mark.parametrize("mt", GEMM_METRICS, ids=lambda m: m.name)
+def test_bf16_exact_on_u8_data(orc, mt):
+    # integers 0..255 are exact in bf16: certification is tight
+    base = bf_sparse_u8(20_000, 784, seed=401)
+    q = bf_sparse_u8(256, 784, seed=402)
+    idx = _index(base, mt, hora_amd.HM_SEARCH_BF16_RERANK)
+    fallback, uncert = _exact_check(orc, idx, base, q, 10, mt)
+    assert uncert <= 256 // 20, (fallback, uncert)
+
+
+@pytest.mark.parametrize("mt", GEMM_METRICS, ids=lambda m: m.name)
+def test_bf16_exact_on_float_data(orc, mt):
+    base = bf_normal(10_000, 64, seed=411)
+    q = bf_normal(300, 64, seed=412)
+    idx = _index(base, mt, hora_amd.HM_SEARCH_BF16_RERANK, shortlist=256)
+    _exact_check(orc, idx, base, q, 10, mt)
+
+
+def test_bf16_small_shortlist_falls_back_exactly(orc):
+    # a shortlist of k+1 on rounded data rarely certifies: the exact path
+    # takes those rows and the answer is still the reference's
+    base = bf_normal(5_000, 48, seed=421)
+    q = bf_normal(128, 48, seed=422)
+    idx = _index(base, Metric.Euclidean, hora_amd.HM_SEARCH_BF16_RERANK, shortlist=11)
+    fallback, uncert = _exact_check(orc, idx, base, q, 10, Metric.Euclidean)
+    assert fallback >= uncert > 0
+
+
+def test_bf16_ties_and_manhattan(orc):
+    rng = np.random.default_rng(3)
+    uniq = rng.integers(0, 4, (500, 16)).astype(np.float32)
+    base = np.repeat(uniq, 2, axis=0)
+    q = rng.integers(0, 4, (64, 16)).astype(np.float32)
+    for mt in (Metric.Euclidean, Metric.DotProduct, Metric.Manhattan):
+        idx = _index(base, mt, hora_amd.HM_SEARCH_BF16_RERANK)
+        _exact_check(orc, idx, base, q, 5, mt)
+
+
+def test_bf16_large_k_and_small_n(orc):
+    base = bf_normal(300, 33, seed=431)
+    q = bf_normal(40, 33, seed=432)
+    for k in (1, 100, 299, 400):
+        idx = _index(base, Metric.Euclidean, hora_amd.HM_SEARCH_BF16_RERANK)
+        _exact_check(orc, idx, base, q, k, Metric.Euclidean)
+
+
+@pytest.mark.parametrize("mt", [Metric.DotProduct, Metric.CosineSimilarity],
+                         ids=lambda m: m.name)
+def test_fp8_recall_and_exact_distances(orc, mt):
+    base = unit_normal(20_000, 256, seed=441)
+    q = unit_normal(200, 256, seed=442)
+    idx = _index(base, mt, hora_amd.HM_SEARCH_FP8_RERANK, shortlist=512)
+    gi, gd, gc = idx.search_batch(q, 10)
+    ri, rd, rc = orc.bf_search(base, q, 10, int(mt))
+    recall = np.mean([len(set(gi[r].tolist()) & set(ri[r].tolist())) / 10 for r in range(200)])
+    assert recall >= 0.95, recall  # FP8 shortlist of 512, exact rerank
+    # each returned distance is the exact reference value of that pair
+    rows = orc.distance_rows(base, q, int(mt))
+    for r in range(200):
+        assert np.array_equal(gd[r], rows[r][gi[r].astype(np.int64)])

@@ -1,0 +1,141 @@
+"""IVFPQIndex on the GPU vs the oracle.
+
+f32 accumulation (the reference's point-order sums): training is bit-exact
+end to end given the seed (coarse centroids, inverted lists, per-list
+codebooks and codes), and search results are bit-exact.
+fp64 accumulation: parity is conditional on the trained state -- the oracle
+re-derives the list assignments, the codes and the search results from the
+exported centroids and codebooks, and they must be identical."""
+import numpy as np
+import pytest
+
+from hora_amd import HoraError, IVFPQIndex, IVFPQParams, Metric
+from hora_amd.datasets import gaussian_mixture
+
+pytestmark = pytest.mark.gpu
+
+
+def _build(x, mt, nlist, n_sub, sub_bits, epochs, seed, fp64=False, nprobe=4):
+    p = (IVFPQParams().n_sub(n_sub).sub_bits(sub_bits).n_kmeans_center(nlist)
+         .search_n_center(nprobe).train_epoch(epochs).seed(seed)
+         .kmeans_accumulate(1 if fp64 else 0))
+    idx = IVFPQIndex(x.shape[1], p)
+    idx.add_batch(x)
+    idx.build(mt)
+    return idx
+
+
+def _lists(ex):
+    off = ex["list_offsets"].astype(np.int64)
+    return [ex["list_ids"][off[l]:off[l + 1]].astype(np.int64) for l in range(len(off) - 1)]
+
+
+def _check_conditional(orc, idx, x, mt, q, nprobe, k):
+    ex = idx.export()
+    cen, cb, codes = ex["centers"], ex["codebooks"], ex["codes"]
+    n_sub = cb.shape[1]
+    # list assignments from the exported centroids (kmeans.rs:127-148)
+    assign = orc.kmeans_assign(x, cen, mt=int(mt))
+    lists = _lists(ex)
+    for l, mem in enumerate(lists):
+        assert np.array_equal(np.sort(mem), mem)
+        assert (assign[mem] == l).all()
+    assert sum(len(m) for m in lists) == x.shape[0]
+    # codes from the exported codebooks (pq_idx.rs:143-144)
+    off = ex["list_offsets"].astype(np.int64)
+    for l, mem in enumerate(lists):
+        if len(mem) == 0:
+            continue
+        ref = orc.pq_encode(x[mem], n_sub, cb[l], cen[l])
+        assert np.array_equal(ref, codes[off[l]:off[l + 1]]), f"codes of list {l}"
+    # search
+    idx.set_search_n_center(nprobe)
+    gi, gd, gc = idx.search_batch(q, k)
+    ri, rd, rc, _ = orc.ivfpq_search(q, int(mt), cen, cb, ex["list_offsets"], ex["list_ids"],
+                                     codes, nprobe, k)
+    assert np.array_equal(gc, rc)
+    assert np.array_equal(gi, ri)
+    assert np.array_equal(gd, rd)
+    return ex
+
+
+def test_f32_training_is_bit_exact_end_to_end(orc):
+    x, _, _ = gaussian_mixture(3000, 16, 12, seed=5, sigma=0.3)
+    nlist, n_sub, ks, epochs, seed = 16, 4, 16, 6, 77
+    idx = _build(x, Metric.Euclidean, nlist, n_sub, 4, epochs, seed)
+    ex = idx.export()
+    cen_ref, assign_ref = orc.kmeans_train(x, nlist, epochs, seed)
+    assert np.array_equal(ex["centers"], cen_ref)
+    lists = _lists(ex)
+    for l, mem in enumerate(lists):
+        assert np.array_equal(mem, np.nonzero(assign_ref == l)[0])
+    off = ex["list_offsets"].astype(np.int64)
+    for l, mem in enumerate(lists):
+        cb_ref, codes_ref = orc.pq_train(x[mem], n_sub, ks, cen_ref[l], epochs, seed,
+                                         1 + l * n_sub, ex["codebooks"].shape[3])
+        assert np.array_equal(ex["codebooks"][l], cb_ref), f"codebook {l}"
+        assert np.array_equal(ex["codes"][off[l]:off[l + 1]], codes_ref), f"codes {l}"
+    q, _, _ = gaussian_mixture(100, 16, 12, seed=6, sigma=0.3)
+    _check_conditional(orc, idx, x, Metric.Euclidean, q, 4, 10)
+
+
+def test_fp64_training_conditional_parity(orc):
+    x, _, _ = gaussian_mixture(5000, 32, 24, seed=8, sigma=0.3)
+    idx = _build(x, Metric.Euclidean, 32, 8, 8, 5, 9, fp64=True)
+    q, _, _ = gaussian_mixture(128, 32, 24, seed=10, sigma=0.3)
+    for nprobe, k in ((1, 10), (8, 10), (32, 50)):
+        _check_conditional(orc, idx, x, Metric.Euclidean, q, nprobe, k)
+
+
+def test_reference_default_params_uneven_subspaces(orc):
+    # IVFPQParams::default: n_sub=25, sub_bits=4 -> d=128 gives 3 subs of 6
+    # dims and 22 of 5 (pq_idx.rs:75-88)
+    x, _, _ = gaussian_mixture(2000, 128, 8, seed=11, sigma=0.3)
+    p = IVFPQParams().n_kmeans_center(8).train_epoch(3).seed(3)
+    idx = IVFPQIndex(128, p)
+    idx.add_batch(x)
+    idx.build(Metric.Euclidean)
+    info = idx.info()
+    assert (info["n_sub"], info["n_sub_center"], info["sub_dim_max"]) == (25, 16, 6)
+    q, _, _ = gaussian_mixture(50, 128, 8, seed=12, sigma=0.3)
+    _check_conditional(orc, idx, x, Metric.Euclidean, q, 8, 10)
+
+
+@pytest.mark.parametrize("mt", [Metric.DotProduct, Metric.Manhattan, Metric.CosineSimilarity],
+                         ids=lambda m: m.name)
+def test_coarse_metric_variants(orc, mt):
+    x, _, _ = gaussian_mixture(3000, 24, 10, seed=13, sigma=0.3)
+    idx = _build(x, mt, 12, 6, 5, 4, 21)
+    q, _, _ = gaussian_mixture(64, 24, 10, seed=14, sigma=0.3)
+    _check_conditional(orc, idx, x, mt, q, 3, 10)
+
+
+def test_ties_replay(orc):
+    # every row twice: equal codes give equal ADC distances in every list
+    x, _, _ = gaussian_mixture(800, 12, 6, seed=15, sigma=0.3)
+    x = np.repeat(x, 2, axis=0)
+    idx = _build(x, Metric.Euclidean, 8, 3, 4, 4, 5)
+    q, _, _ = gaussian_mixture(40, 12, 6, seed=16, sigma=0.3)
+    _check_conditional(orc, idx, x, Metric.Euclidean, q, 3, 7)
+    assert idx.search_stats()[0] > 0  # rows went through the replay
+
+
+def test_errors():
+    x, _, _ = gaussian_mixture(300, 8, 4, seed=1)
+    p = IVFPQParams().n_sub(2).sub_bits(4).n_kmeans_center(4).search_n_center(2).train_epoch(2)
+    idx = IVFPQIndex(8, p)
+    idx.add_batch(x)
+    with pytest.raises(HoraError) as e:
+        idx.search_batch(x[:2], 3)
+    assert e.value.status == 5
+    idx.build(Metric.Euclidean)
+    idx.set_search_n_center(5)  # more than n_kmeans_center
+    with pytest.raises(HoraError):
+        idx.search_batch(x[:2], 3)
+    empty = IVFPQIndex(8, p)
+    with pytest.raises(HoraError) as e:
+        empty.build(Metric.Euclidean)
+    assert e.value.status == 9
+    with pytest.raises(HoraError) as e:
+        IVFPQIndex(8, IVFPQParams().n_sub(2).sub_bits(9))
+    assert e.value.status == 10
