@@ -86,6 +86,12 @@ def lib():
     L.hm_index_set_nprobe.argtypes = [vp, u32]
     L.hm_ivfpq_info.argtypes = [vp, vp]
     L.hm_ivfpq_export.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.hm_ivfpq_train_probe.argtypes = [vp, vp, u64, u32, u32, u32, u32, C.c_float, u64,
+                                       C.POINTER(C.c_double)]
+    L.hm_ivfpq_set_probe_policy.argtypes = [vp, i32, C.c_float, u32, u32]
+    L.hm_ivfpq_mean_nprobe.argtypes = [vp, C.POINTER(C.c_double)]
+    L.hm_ivfpq_last_probes.argtypes = [vp, u64, u32, vp, vp]
+    L.hm_mlp_loss_grad.argtypes = [u32, u32, u32, u32] + [vp] * 12
     L.hm_profile_enable.argtypes = [i32]
     L.hm_profile_reset.argtypes = []
     L.hm_profile_query.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)]

@@ -78,6 +78,23 @@ class BruteForce final : public Index {
         sl_valid_ = false;
     }
 
+    void add_device(const float *d_rows, uint64_t n_rows, uint64_t ld) override
+    {
+        if (n_rows == 0) return;
+        if (n_ + n_rows > 0xFFFFFFF0ull) fail(HM_ERR_CAPACITY, "more than 2^32-16 rows");
+        const uint64_t need = n_ + n_rows;
+        if (need > cap_) {
+            base_.grow_keep(need * dpad_ * sizeof(float), n_ * dpad_ * sizeof(float), stream_);
+            cap_ = need;
+        }
+        float *dst = base_.as<float>() + n_ * dpad_;
+        launch_pad_rows(d_rows, ld, dst, dpad_, n_rows, (uint32_t)dim_, dpad_, stream_);
+        HM_HIP(hipStreamSynchronize(stream_));
+        n_ = need;
+        bnorm_valid_ = false;
+        sl_valid_ = false;
+    }
+
     void build(int32_t metric) override
     {
         if (metric < HM_METRIC_UNKNOWN || metric > HM_METRIC_ANGULAR)

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "index.h"
+#include "mlp.h"
 
 using namespace hm;
 
@@ -279,6 +280,92 @@ HM_API int32_t hm_ivfpq_export(hm_index *h, float *centers, uint64_t *list_offse
         std::lock_guard<std::mutex> lk(h->mu);
         DeviceScope ds(h->device);
         ivfpq_export(h->impl.get(), centers, list_offsets, list_ids, codebooks, codes);
+    });
+}
+
+HM_API int32_t hm_ivfpq_train_probe(hm_index *h, const float *train_queries, uint64_t n,
+                                    uint32_t label_k, uint32_t hidden, uint32_t epochs,
+                                    uint32_t batch, float lr, uint64_t seed, double *final_loss)
+{
+    return guard([&] {
+        need(h, "index");
+        need(train_queries, "train_queries");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        ivfpq_train_probe(h->impl.get(), train_queries, n, label_k, hidden, epochs, batch, lr,
+                          seed, final_loss);
+    });
+}
+
+HM_API int32_t hm_ivfpq_set_probe_policy(hm_index *h, int32_t mode, float mass, uint32_t pmin,
+                                         uint32_t pmax)
+{
+    return guard([&] {
+        need(h, "index");
+        std::lock_guard<std::mutex> lk(h->mu);
+        ivfpq_set_probe_policy(h->impl.get(), mode, mass, pmin, pmax);
+    });
+}
+
+HM_API int32_t hm_ivfpq_mean_nprobe(hm_index *h, double *out)
+{
+    return guard([&] {
+        need(h, "index");
+        need(out, "out");
+        std::lock_guard<std::mutex> lk(h->mu);
+        *out = ivfpq_mean_nprobe(h->impl.get());
+    });
+}
+
+HM_API int32_t hm_ivfpq_last_probes(hm_index *h, uint64_t nq, uint32_t ldp, uint32_t *probes,
+                                    uint32_t *counts)
+{
+    return guard([&] {
+        need(h, "index");
+        need(probes, "probes");
+        need(counts, "counts");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        ivfpq_last_probes(h->impl.get(), nq, ldp, probes, counts);
+    });
+}
+
+HM_API int32_t hm_mlp_loss_grad(uint32_t din, uint32_t hidden, uint32_t dout, uint32_t n,
+                                const float *W1, const float *b1, const float *W2,
+                                const float *b2, const float *X, const float *T, float *loss,
+                                float *gW1, float *gb1, float *gW2, float *gb2, float *logits)
+{
+    return guard([&] {
+        for (const void *p : {(const void *)W1, (const void *)b1, (const void *)W2,
+                              (const void *)b2, (const void *)X})
+            need(p, "input");
+        hipStream_t s = nullptr;
+        HM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } sg{s};
+        ProbeMLP net((int)din, (int)hidden, (int)dout, 1, s);
+        net.set_weights(W1, b1, W2, b2);
+        DeviceBuffer Xd, X16, Td, Zd;
+        Xd.reserve((size_t)n * din * 4);
+        X16.reserve((size_t)n * din * 2);
+        HM_HIP(hipMemcpyAsync(Xd.get(), X, (size_t)n * din * 4, hipMemcpyHostToDevice, s));
+        f32_to_bf16(Xd.as<float>(), X16.as<uint16_t>(), (long)n * din, s);
+        if (logits) {
+            Zd.reserve((size_t)n * dout * 4);
+            net.forward(X16.as<uint16_t>(), (int)n, Zd.as<float>());
+            HM_HIP(hipMemcpyAsync(logits, Zd.get(), (size_t)n * dout * 4, hipMemcpyDeviceToHost, s));
+        }
+        if (T) {
+            Td.reserve((size_t)n * dout * 4);
+            HM_HIP(hipMemcpyAsync(Td.get(), T, (size_t)n * dout * 4, hipMemcpyHostToDevice, s));
+            net.loss_grad(X16.as<uint16_t>(), Td.as<float>(), (int)n);
+            if (loss)
+                HM_HIP(hipMemcpyAsync(loss, net.loss_buf(), (size_t)n * 4, hipMemcpyDeviceToHost, s));
+            net.get_grads(gW1, gb1, gW2, gb2);
+        }
+        HM_HIP(hipStreamSynchronize(s));
     });
 }
 

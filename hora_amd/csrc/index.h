@@ -27,6 +27,15 @@ class Index {
     virtual uint64_t nodes_size() const = 0;
     virtual void clear() = 0;
     virtual void get_rows(const uint64_t *idx, uint64_t n, float *out) const = 0;
+    // Appends rows already in device memory (n rows of dimension floats,
+    // row stride ld floats).
+    virtual void add_device(const float *d_rows, uint64_t n_rows, uint64_t ld)
+    {
+        (void)d_rows;
+        (void)n_rows;
+        (void)ld;
+        fail(HM_ERR_NOT_IMPLEMENTED, "add_device not supported by this index");
+    }
     virtual void set_search_mode(int32_t mode, uint32_t shortlist)
     {
         (void)mode;
@@ -58,6 +67,13 @@ int32_t ivfpq_info(Index *ix, uint32_t *out8);
 int32_t ivfpq_export(Index *ix, float *centers, uint64_t *offsets, uint64_t *ids,
                      float *codebooks, uint8_t *codes);
 int32_t ivfpq_set_nprobe(Index *ix, uint32_t nprobe);
+int32_t ivfpq_set_probe_policy(Index *ix, int32_t mode, float mass, uint32_t pmin, uint32_t pmax);
+int32_t ivfpq_train_probe(Index *ix, const float *q, uint64_t n, uint32_t label_k,
+                          uint32_t hidden, uint32_t epochs, uint32_t batch, float lr,
+                          uint64_t seed, double *loss);
+double ivfpq_mean_nprobe(Index *ix);
+int32_t ivfpq_last_probes(Index *ix, uint64_t nq, uint32_t ldp, uint32_t *probes,
+                          uint32_t *counts);
 
 // Host-side validation shared by the index types (node.rs:158-166).
 void check_finite_host(const float *x, uint64_t count);

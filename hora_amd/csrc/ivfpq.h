@@ -38,7 +38,8 @@ void launch_ivf_scan(const float *Qp, uint32_t dpad, const float *C, const uint3
                      uint32_t ldp, uint32_t nprobe, uint32_t rows, const uint32_t *off,
                      const uint32_t *ids, const uint8_t *codes, const float *codebooks,
                      const uint32_t *sub_b, const uint32_t *sub_e, uint32_t M, uint32_t KS,
-                     uint32_t sdm, uint32_t k1, unsigned long long *out, hipStream_t s);
+                     uint32_t sdm, uint32_t k1, const uint32_t *nprobe_q,
+                     unsigned long long *out, hipStream_t s);
 void launch_ivf_merge(const unsigned long long *cand, uint32_t per_query, uint32_t rows,
                       uint32_t k, const uint32_t *coarse_flags, uint64_t *out_idx,
                       float *out_dist, uint32_t *out_count, uint32_t *flags, hipStream_t s);
@@ -48,7 +49,16 @@ void launch_ivf_replay(const uint32_t *rows, uint32_t nrows, const float *Dc, ui
                        const uint8_t *codes, const float *codebooks, const uint32_t *sub_b,
                        const uint32_t *sub_e, uint32_t M, uint32_t KS, uint32_t sdm, uint32_t k,
                        uint32_t lmax, float *scratch_f, uint32_t *scratch_u, size_t per_thread,
+                       const uint32_t *probes, uint32_t ldp, const uint32_t *nprobe_q,
                        uint64_t *out_idx, float *out_dist, uint32_t *out_count, hipStream_t s);
+// Learned-probe helpers.
+void launch_negate(const float *x, float *y, uint64_t n, hipStream_t s);
+void launch_probe_count(const float *negp, uint32_t ld, uint32_t rows, uint32_t pmax, float mass,
+                        uint32_t pmin, uint32_t *nprobe_q, hipStream_t s);
+void launch_dense_targets(const uint32_t *lab, uint32_t k, uint32_t rows, uint32_t nlist,
+                          float *T, hipStream_t s);
+void launch_ids_to_lists(const uint64_t *ids, uint64_t n, const uint32_t *assign, uint64_t n_items,
+                         uint32_t *out, hipStream_t s);
 // kmeans.rs:150-193 on host memory, seeded SplitMix64 (state *rng).
 void split_centers_host(float *C, uint32_t ldc, uint32_t nc, uint32_t dim, uint64_t n,
                         uint64_t *counts, uint64_t *rng);

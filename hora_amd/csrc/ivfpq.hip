@@ -322,7 +322,8 @@ __global__ __launch_bounds__(256) void ivf_scan_kernel(
     const uint32_t *__restrict__ off, const uint32_t *__restrict__ ids,
     const uint8_t *__restrict__ codes, const float *__restrict__ codebooks,
     const uint32_t *__restrict__ sub_b, const uint32_t *__restrict__ sub_e, uint32_t M,
-    uint32_t KS, uint32_t sdm, uint32_t k1, uint32_t P, unsigned long long *__restrict__ out)
+    uint32_t KS, uint32_t sdm, uint32_t k1, uint32_t P, const uint32_t *__restrict__ nprobe_q,
+    unsigned long long *__restrict__ out)
 {
     extern __shared__ __align__(16) unsigned char sc_smem[];
     unsigned long long *keys = reinterpret_cast<unsigned long long *>(sc_smem);  // P
@@ -330,6 +331,11 @@ __global__ __launch_bounds__(256) void ivf_scan_kernel(
     float *z = lut + (size_t)M * KS;                                             // dpad
     const int tid = threadIdx.x;
     const uint32_t q = blockIdx.y, p = blockIdx.x;
+    if (nprobe_q && p >= nprobe_q[q]) {  // this query probes fewer lists
+        unsigned long long *o = out + ((size_t)q * nprobe + p) * k1;
+        for (uint32_t j = tid; j < k1; j += 256) o[j] = ~0ull;
+        return;
+    }
     const uint32_t l = probes[(size_t)q * ldp + p];
     for (uint32_t j = tid; j < dpad; j += 256)
         z[j] = Qp[(size_t)q * dpad + j] - C[(size_t)l * dpad + j];
@@ -528,7 +534,9 @@ __global__ void ivf_replay_kernel(const uint32_t *__restrict__ rows, uint32_t nr
                                   const uint32_t *__restrict__ sub_b,
                                   const uint32_t *__restrict__ sub_e, uint32_t M, uint32_t KS,
                                   uint32_t sdm, uint32_t k, uint32_t lmax, float *scratch_f,
-                                  uint32_t *scratch_u, size_t per_thread, uint64_t *out_idx,
+                                  uint32_t *scratch_u, size_t per_thread,
+                                  const uint32_t *__restrict__ probes, uint32_t ldp,
+                                  const uint32_t *__restrict__ nprobe_q, uint64_t *out_idx,
                                   float *out_dist, uint32_t *out_count)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -541,13 +549,19 @@ __global__ void ivf_replay_kernel(const uint32_t *__restrict__ rows, uint32_t nr
     GHeap lh{fbase + nlist, ubase + nlist, 0};
     GHeap top{fbase + nlist + lmax, ubase + nlist + lmax, 0};
     float *lut = fbase + nlist + lmax + (k + 1);
-    const float *drow = Dc + (size_t)q * ldd;
-    for (uint32_t i = 0; i < nlist; ++i) coarse.push(-drow[i], i);
     const float *qv = Qp + (size_t)q * dpad;
-    for (uint32_t p = 0; p < nprobe && coarse.len > 0; ++p) {
+    uint32_t np_q = nprobe;
+    if (probes) {
+        np_q = nprobe_q[q];
+    } else {
+        const float *drow = Dc + (size_t)q * ldd;
+        for (uint32_t i = 0; i < nlist; ++i) coarse.push(-drow[i], i);
+    }
+    for (uint32_t p = 0; p < np_q && (probes || coarse.len > 0); ++p) {
         float xd;
         uint32_t l;
-        coarse.pop(&xd, &l);
+        if (probes) l = probes[(size_t)q * ldp + p];
+        else coarse.pop(&xd, &l);
         const float *cb = codebooks + (size_t)l * M * KS * sdm;
         for (uint32_t e = 0; e < M * KS; ++e) {
             uint32_t s = e / KS, b = sub_b[s], sd = sub_e[s] - b;
@@ -598,6 +612,58 @@ __global__ void ivf_replay_kernel(const uint32_t *__restrict__ rows, uint32_t nr
         out_dist[(size_t)q * k + j] = inf;
     }
     out_count[q] = cnt;
+}
+
+__global__ void negate_kernel(const float *__restrict__ x, float *__restrict__ y, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        y[i] = -x[i];
+}
+
+// nprobe_q[q] = smallest m with sum of the m largest probabilities >= mass,
+// clamped to [pmin, pmax]; the candidates hold -p sorted ascending.
+__global__ void probe_count_kernel(const float *__restrict__ negp, uint32_t ld, uint32_t rows,
+                                   uint32_t pmax, float mass, uint32_t pmin,
+                                   uint32_t *__restrict__ nprobe_q)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= rows) return;
+    float acc = 0.f;
+    uint32_t m = pmax;
+    for (uint32_t j = 0; j < pmax; ++j) {
+        acc += -negp[(size_t)q * ld + j];
+        if (acc >= mass) {
+            m = j + 1;
+            break;
+        }
+    }
+    nprobe_q[q] = m < pmin ? pmin : m;
+}
+
+// ids[i] -> list of the item; T[q][list] += 1/k (dense targets of a batch).
+__global__ void dense_targets_kernel(const uint32_t *__restrict__ lab, uint32_t k,
+                                     uint32_t rows, uint32_t nlist, float *__restrict__ T)
+{
+    const uint32_t q = blockIdx.x;
+    for (uint32_t j = threadIdx.x; j < nlist; j += blockDim.x) T[(size_t)q * nlist + j] = 0.f;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (uint32_t j = 0; j < k; ++j) {
+            uint32_t l = lab[(size_t)q * k + j];
+            if (l < nlist) T[(size_t)q * nlist + l] += 1.0f / (float)k;
+        }
+}
+
+__global__ void ids_to_lists_kernel(const uint64_t *__restrict__ ids, uint64_t n,
+                                    const uint32_t *__restrict__ assign, uint64_t n_items,
+                                    uint32_t *__restrict__ out)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t id = ids[i];
+        out[i] = id < n_items ? assign[id] : 0xFFFFFFFFu;
+    }
 }
 
 inline unsigned grid_for(uint64_t work, unsigned cap = 8192)
@@ -697,7 +763,8 @@ void launch_ivf_scan(const float *Qp, uint32_t dpad, const float *C, const uint3
                      uint32_t ldp, uint32_t nprobe, uint32_t rows, const uint32_t *off,
                      const uint32_t *ids, const uint8_t *codes, const float *codebooks,
                      const uint32_t *sub_b, const uint32_t *sub_e, uint32_t M, uint32_t KS,
-                     uint32_t sdm, uint32_t k1, unsigned long long *out, hipStream_t s)
+                     uint32_t sdm, uint32_t k1, const uint32_t *nprobe_q,
+                     unsigned long long *out, hipStream_t s)
 {
     if (rows == 0 || nprobe == 0) return;
     const uint32_t P = ivf_scan_pool(k1);
@@ -707,7 +774,7 @@ void launch_ivf_scan(const float *Qp, uint32_t dpad, const float *C, const uint3
     ProfScope prof("ivf_scan", s);
     ivf_scan_kernel<<<dim3(nprobe, rows), 256, lds, s>>>(Qp, dpad, C, probes, ldp, nprobe, off,
                                                           ids, codes, codebooks, sub_b, sub_e, M,
-                                                          KS, sdm, k1, P, out);
+                                                          KS, sdm, k1, P, nprobe_q, out);
     HM_CHECK_LAUNCH();
 }
 
@@ -731,14 +798,46 @@ void launch_ivf_replay(const uint32_t *rows, uint32_t nrows, const float *Dc, ui
                        const uint8_t *codes, const float *codebooks, const uint32_t *sub_b,
                        const uint32_t *sub_e, uint32_t M, uint32_t KS, uint32_t sdm, uint32_t k,
                        uint32_t lmax, float *scratch_f, uint32_t *scratch_u, size_t per_thread,
+                       const uint32_t *probes, uint32_t ldp, const uint32_t *nprobe_q,
                        uint64_t *out_idx, float *out_dist, uint32_t *out_count, hipStream_t s)
 {
     if (nrows == 0) return;
     ProfScope prof("ivf_replay", s);
     ivf_replay_kernel<<<ceil_div(nrows, 64), 64, 0, s>>>(
         rows, nrows, Dc, ldd, nlist, nprobe, Qp, dpad, C, off, ids, codes, codebooks, sub_b,
-        sub_e, M, KS, sdm, k, lmax, scratch_f, scratch_u, per_thread, out_idx, out_dist,
-        out_count);
+        sub_e, M, KS, sdm, k, lmax, scratch_f, scratch_u, per_thread, probes, ldp, nprobe_q,
+        out_idx, out_dist, out_count);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_negate(const float *x, float *y, uint64_t n, hipStream_t s)
+{
+    negate_kernel<<<grid_for(n), 256, 0, s>>>(x, y, n);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_probe_count(const float *negp, uint32_t ld, uint32_t rows, uint32_t pmax, float mass,
+                        uint32_t pmin, uint32_t *nprobe_q, hipStream_t s)
+{
+    if (rows == 0) return;
+    probe_count_kernel<<<ceil_div(rows, 256), 256, 0, s>>>(negp, ld, rows, pmax, mass, pmin,
+                                                            nprobe_q);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_dense_targets(const uint32_t *lab, uint32_t k, uint32_t rows, uint32_t nlist,
+                          float *T, hipStream_t s)
+{
+    if (rows == 0) return;
+    dense_targets_kernel<<<rows, 256, 0, s>>>(lab, k, rows, nlist, T);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_ids_to_lists(const uint64_t *ids, uint64_t n, const uint32_t *assign, uint64_t n_items,
+                         uint32_t *out, hipStream_t s)
+{
+    if (n == 0) return;
+    ids_to_lists_kernel<<<grid_for(n), 256, 0, s>>>(ids, n, assign, n_items, out);
     HM_CHECK_LAUNCH();
 }
 

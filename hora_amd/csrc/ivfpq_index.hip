@@ -19,6 +19,7 @@
 
 #include "index.h"
 #include "ivfpq.h"
+#include "mlp.h"
 
 namespace hm {
 
@@ -180,9 +181,11 @@ class IVFPQ final : public Index {
         launch_check_finite(dq, dim_, nq, (uint32_t)dim_, err_.as<uint32_t>(), s);
         const uint32_t k1 = k + 1;
         if (2 * k1 > ivf_scan_pool(k1)) fail(HM_ERR_INVALID_ARG, "k too large (max 4095)");
-        const uint32_t per_query = nprobe_ * k1;
+        const bool learned = probe_mode_ == 1 && mlp_;
+        const uint32_t np = learned ? pmax_ : nprobe_;
+        const uint32_t per_query = np * k1;
         if (per_query > 16384) fail(HM_ERR_INVALID_ARG, "nprobe*(k+1) above 16384");
-        const uint32_t Kc = std::min<uint32_t>(nprobe_ + 1, nlist_);
+        const uint32_t Kc = learned ? pmax_ : std::min<uint32_t>(nprobe_ + 1, nlist_);
         const bool norms = metric_ == HM_METRIC_COSINE_SIMILARITY || metric_ == HM_METRIC_ANGULAR;
         const uint64_t ldd = round_up(nlist_, 4);
         uint64_t qc = std::min<uint64_t>(nq, 65535);
@@ -195,41 +198,174 @@ class IVFPQ final : public Index {
         cflags_.reserve(qc * 4);
         mflags_.reserve(qc * 4);
         cand_.reserve(qc * per_query * 8);
+        if (learned) {
+            xq16_.reserve(qc * dim_ * 2);
+            Zq_.reserve(qc * nlist_ * 4);
+            Pq_.reserve(qc * nlist_ * 4);
+            nprobe_q_.reserve(qc * 4);
+        }
         std::vector<uint32_t> fl(qc);
+        last_replayed_ = 0;
+        uint64_t probes_used = 0;
+        std::vector<uint32_t> npq(learned ? qc : 0);
         for (uint64_t q0 = 0; q0 < nq; q0 += qc) {
             const uint32_t rows = (uint32_t)std::min<uint64_t>(qc, nq - q0);
             launch_pad_rows(dq + q0 * dim_, dim_, qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
                             dpad_, s);
-            if (norms)
-                launch_self_dot(qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
-                                qnorm_.as<float>(), s);
-            // pq_idx.rs:416-421: metric(q, c_i) with the build metric
-            launch_exact_tile(metric_, qpad_.as<float>(), dpad_, rows, centers_.as<float>(), dpad_,
-                              nlist_, dpad_, qnorm_.as<float>(), cnorm_.as<float>(),
-                              D_.as<float>(), ldd, s);
-            launch_row_select(D_.as<float>(), ldd, rows, nlist_, Kc, probe_d_.as<float>(),
-                              probe_i_.as<uint32_t>(), Kc, cflags_.as<uint32_t>(), s);
+            if (learned) {
+                // scores of every list from the probe network, then the
+                // smallest prefix of the best lists holding `mass` of them
+                f32_to_bf16(dq + q0 * dim_, xq16_.as<uint16_t>(), (long)rows * dim_, s);
+                mlp_->forward(xq16_.as<uint16_t>(), rows, Zq_.as<float>());
+                softmax_xent(Zq_.as<float>(), nullptr, rows, nlist_, 1.0f, nullptr, nullptr,
+                             nullptr, Pq_.as<float>(), s);
+                launch_negate(Pq_.as<float>(), D_.as<float>(), (uint64_t)rows * nlist_, s);
+                launch_row_select(D_.as<float>(), nlist_, rows, nlist_, Kc, probe_d_.as<float>(),
+                                  probe_i_.as<uint32_t>(), Kc, cflags_.as<uint32_t>(), s);
+                launch_probe_count(probe_d_.as<float>(), Kc, rows, pmax_, mass_, pmin_,
+                                   nprobe_q_.as<uint32_t>(), s);
+                HM_HIP(hipMemsetAsync(cflags_.get(), 0, rows * 4, s));
+            } else {
+                if (norms)
+                    launch_self_dot(qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
+                                    qnorm_.as<float>(), s);
+                // pq_idx.rs:416-421: metric(q, c_i) with the build metric
+                launch_exact_tile(metric_, qpad_.as<float>(), dpad_, rows, centers_.as<float>(),
+                                  dpad_, nlist_, dpad_, qnorm_.as<float>(), cnorm_.as<float>(),
+                                  D_.as<float>(), ldd, s);
+                launch_row_select(D_.as<float>(), ldd, rows, nlist_, Kc, probe_d_.as<float>(),
+                                  probe_i_.as<uint32_t>(), Kc, cflags_.as<uint32_t>(), s);
+            }
+            const uint32_t *npq_dev = learned ? nprobe_q_.as<uint32_t>() : nullptr;
             launch_ivf_scan(qpad_.as<float>(), dpad_, centers_.as<float>(),
-                            probe_i_.as<uint32_t>(), Kc, nprobe_, rows, off_.as<uint32_t>(),
+                            probe_i_.as<uint32_t>(), Kc, np, rows, off_.as<uint32_t>(),
                             ids_.as<uint32_t>(), codes_.as<uint8_t>(), codebooks_.as<float>(),
                             sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), M_, KS_, sdm_, k1,
-                            cand_.as<unsigned long long>(), s);
+                            npq_dev, cand_.as<unsigned long long>(), s);
             launch_ivf_merge(cand_.as<unsigned long long>(), per_query, rows, k,
                              cflags_.as<uint32_t>(), d_idx + q0 * k, d_dist + q0 * k,
                              d_count + q0, mflags_.as<uint32_t>(), s);
             HM_HIP(hipMemcpyAsync(fl.data(), mflags_.get(), rows * 4, hipMemcpyDeviceToHost, s));
+            if (learned)
+                HM_HIP(hipMemcpyAsync(npq.data(), nprobe_q_.get(), rows * 4,
+                                      hipMemcpyDeviceToHost, s));
             HM_HIP(hipStreamSynchronize(s));
+            if (learned)
+                for (uint32_t i = 0; i < rows; ++i) probes_used += npq[i];
+            else
+                probes_used += (uint64_t)rows * np;
             std::vector<uint32_t> tied;
             for (uint32_t i = 0; i < rows; ++i)
                 if (fl[i] & ROW_TIE) tied.push_back(i);
             last_replayed_ += tied.size();
-            if (!tied.empty()) replay(tied, rows, k, ldd, d_idx + q0 * k, d_dist + q0 * k,
-                                      d_count + q0, s);
+            if (!tied.empty())
+                replay(tied, k, ldd, learned, Kc, d_idx + q0 * k, d_dist + q0 * k, d_count + q0,
+                       s);
         }
+        last_mean_nprobe_ = (double)probes_used / (double)nq;
+        last_rows_ = nq;
+        last_chunks_ = (uint32_t)((nq + qc - 1) / qc);
+        last_kc_ = Kc;
+        last_learned_ = learned;
         uint32_t err = 0;
         HM_HIP(hipMemcpyAsync(&err, err_.get(), 4, hipMemcpyDeviceToHost, s));
         HM_HIP(hipStreamSynchronize(s));
         if (err & 2u) fail(HM_ERR_NONFINITE_INPUT, "invalid float element");
+    }
+
+    double last_mean_nprobe() const { return last_mean_nprobe_; }
+
+    // Lists scanned per query by the last search (one chunk of queries):
+    // probes [nq][ldp] in scan order, counts [nq].
+    void last_probes(uint64_t nq, uint32_t ldp, uint32_t *probes, uint32_t *counts) const
+    {
+        if (nq != last_rows_ || last_chunks_ != 1)
+            fail(HM_ERR_INVALID_ARG, "last search did not run as one chunk of this size");
+        std::vector<uint32_t> pr((size_t)nq * last_kc_), cn(nq);
+        HM_HIP(hipMemcpyAsync(pr.data(), probe_i_.get(), pr.size() * 4, hipMemcpyDeviceToHost,
+                              stream_));
+        if (last_learned_)
+            HM_HIP(hipMemcpyAsync(cn.data(), nprobe_q_.get(), nq * 4, hipMemcpyDeviceToHost,
+                                  stream_));
+        HM_HIP(hipStreamSynchronize(stream_));
+        for (uint64_t q = 0; q < nq; ++q) {
+            counts[q] = last_learned_ ? cn[q] : nprobe_;
+            for (uint32_t j = 0; j < ldp; ++j)
+                probes[q * ldp + j] = j < last_kc_ ? pr[q * last_kc_ + j] : 0xFFFFFFFFu;
+        }
+    }
+
+    void set_probe_policy(int32_t mode, float mass, uint32_t pmin, uint32_t pmax)
+    {
+        if (mode != 0 && mode != 1) fail(HM_ERR_INVALID_ARG, "probe mode must be 0 or 1");
+        if (mode == 1 && !mlp_) fail(HM_ERR_NOT_BUILT, "probe network not trained");
+        if (pmin == 0 || pmin > pmax || pmax > nlist_)
+            fail(HM_ERR_INVALID_ARG, "need 1 <= pmin <= pmax <= n_kmeans_center");
+        probe_mode_ = mode;
+        mass_ = mass;
+        pmin_ = pmin;
+        pmax_ = pmax;
+    }
+
+    // Trains the probe network: labels are the inverted lists holding each
+    // training query's exact top-label_k (GPU brute force over the indexed
+    // rows); the target of a query is the fraction of its top-k per list.
+    double train_probe(const float *h_queries, uint64_t n, uint32_t label_k, uint32_t hidden,
+                       uint32_t epochs, uint32_t batch, float lr, uint64_t seed)
+    {
+        if (!trained_) fail(HM_ERR_NOT_BUILT, "index not built");
+        if (n == 0 || label_k == 0 || hidden == 0 || batch == 0)
+            fail(HM_ERR_INVALID_ARG, "empty probe training set or shape");
+        check_finite_host(h_queries, n * dim_);
+        hipStream_t s = stream_;
+        DeviceBuffer Xtr, X16, lab, ids, dist, cnt, T;
+        Xtr.reserve(n * dim_ * 4);
+        HM_HIP(hipMemcpyAsync(Xtr.get(), h_queries, n * dim_ * 4, hipMemcpyHostToDevice, s));
+        X16.reserve(n * dim_ * 2);
+        f32_to_bf16(Xtr.as<float>(), X16.as<uint16_t>(), (long)(n * dim_), s);
+        HM_HIP(hipStreamSynchronize(s));
+        {
+            auto bf = make_bruteforce(dim_);
+            bf->add_device(base_.as<float>(), n_trained_, dpad_);
+            bf->build(metric_);
+            if (metric_ != HM_METRIC_MANHATTAN) bf->set_search_mode(HM_SEARCH_BF16_RERANK, 0);
+            const uint64_t chunk = 4096;
+            ids.reserve(chunk * label_k * 8);
+            dist.reserve(chunk * label_k * 4);
+            cnt.reserve(chunk * 4);
+            lab.reserve(n * label_k * 4);
+            for (uint64_t c0 = 0; c0 < n; c0 += chunk) {
+                const uint64_t rows = std::min<uint64_t>(chunk, n - c0);
+                bf->search_device(Xtr.as<float>() + c0 * dim_, rows, label_k, ids.as<uint64_t>(),
+                                  dist.as<float>(), cnt.as<uint32_t>(), s);
+                launch_ids_to_lists(ids.as<uint64_t>(), rows * label_k, assign_.as<uint32_t>(),
+                                    n_trained_, lab.as<uint32_t>() + c0 * label_k, s);
+            }
+            HM_HIP(hipStreamSynchronize(s));
+        }
+        mlp_ = std::make_unique<ProbeMLP>((int)dim_, (int)hidden, (int)nlist_, seed, s);
+        T.reserve((size_t)batch * nlist_ * 4);
+        std::vector<float> lossv(batch);
+        double last = 0.0;
+        for (uint32_t ep = 0; ep < epochs; ++ep) {
+            double tot = 0.0;
+            for (uint64_t b0 = 0; b0 < n; b0 += batch) {
+                const int rows = (int)std::min<uint64_t>(batch, n - b0);
+                launch_dense_targets(lab.as<uint32_t>() + b0 * label_k, label_k, rows, nlist_,
+                                     T.as<float>(), s);
+                mlp_->loss_grad(X16.as<uint16_t>() + b0 * dim_, T.as<float>(), rows);
+                mlp_->adam(lr, 0.0f);
+                if (ep + 1 == epochs) {
+                    HM_HIP(hipMemcpyAsync(lossv.data(), mlp_->loss_buf(), rows * 4,
+                                          hipMemcpyDeviceToHost, s));
+                    HM_HIP(hipStreamSynchronize(s));
+                    for (int i = 0; i < rows; ++i) tot += lossv[i];
+                }
+            }
+            if (ep + 1 == epochs) last = tot / (double)n;
+        }
+        HM_HIP(hipStreamSynchronize(s));
+        return last;
     }
 
     void search_stats(uint64_t *fallback, uint64_t *uncertified) const override
@@ -239,10 +375,9 @@ class IVFPQ final : public Index {
     }
 
   private:
-    void replay(const std::vector<uint32_t> &tied, uint32_t rows, uint32_t k, uint64_t ldd,
-                uint64_t *d_idx, float *d_dist, uint32_t *d_count, hipStream_t s)
+    void replay(const std::vector<uint32_t> &tied, uint32_t k, uint64_t ldd, bool learned,
+                uint32_t ldp, uint64_t *d_idx, float *d_dist, uint32_t *d_count, hipStream_t s)
     {
-        (void)rows;
         const uint32_t m = (uint32_t)tied.size();
         const size_t per = (size_t)nlist_ + max_list_ + (k + 1) + (size_t)M_ * KS_;
         rp_rows_.reserve(m * 4);
@@ -254,7 +389,8 @@ class IVFPQ final : public Index {
                           off_.as<uint32_t>(), ids_.as<uint32_t>(), codes_.as<uint8_t>(),
                           codebooks_.as<float>(), sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), M_,
                           KS_, sdm_, k, max_list_, rp_f_.as<float>(), rp_u_.as<uint32_t>(), per,
-                          d_idx, d_dist, d_count, s);
+                          learned ? probe_i_.as<uint32_t>() : nullptr, ldp,
+                          learned ? nprobe_q_.as<uint32_t>() : nullptr, d_idx, d_dist, d_count, s);
         HM_HIP(hipStreamSynchronize(s));
     }
 
@@ -358,6 +494,16 @@ class IVFPQ final : public Index {
         off_, codebooks_, codes_, iota_, sort_tmp_;
     DeviceBuffer qpad_, qnorm_, D_, probe_d_, probe_i_, cflags_, mflags_, cand_, err_;
     DeviceBuffer rp_rows_, rp_f_, rp_u_;
+    // learned probe
+    std::unique_ptr<ProbeMLP> mlp_;
+    int32_t probe_mode_ = 0;
+    float mass_ = 0.9f;
+    uint32_t pmin_ = 1, pmax_ = 1;
+    double last_mean_nprobe_ = 0.0;
+    uint64_t last_rows_ = 0;
+    uint32_t last_chunks_ = 0, last_kc_ = 0;
+    bool last_learned_ = false;
+    DeviceBuffer xq16_, Zq_, Pq_, nprobe_q_;
 };
 
 }  // namespace
@@ -382,6 +528,41 @@ int32_t ivfpq_export(Index *ix, float *centers, uint64_t *offsets, uint64_t *ids
     if (!p) fail(HM_ERR_INVALID_ARG, "not an IVFPQIndex");
     p->export_trained(centers, offsets, ids, codebooks, codes);
     return HM_OK;
+}
+
+int32_t ivfpq_set_probe_policy(Index *ix, int32_t mode, float mass, uint32_t pmin, uint32_t pmax)
+{
+    auto *p = dynamic_cast<IVFPQ *>(ix);
+    if (!p) fail(HM_ERR_INVALID_ARG, "not an IVFPQIndex");
+    p->set_probe_policy(mode, mass, pmin, pmax);
+    return HM_OK;
+}
+
+int32_t ivfpq_train_probe(Index *ix, const float *q, uint64_t n, uint32_t label_k,
+                          uint32_t hidden, uint32_t epochs, uint32_t batch, float lr,
+                          uint64_t seed, double *loss)
+{
+    auto *p = dynamic_cast<IVFPQ *>(ix);
+    if (!p) fail(HM_ERR_INVALID_ARG, "not an IVFPQIndex");
+    double l = p->train_probe(q, n, label_k, hidden, epochs, batch, lr, seed);
+    if (loss) *loss = l;
+    return HM_OK;
+}
+
+int32_t ivfpq_last_probes(Index *ix, uint64_t nq, uint32_t ldp, uint32_t *probes,
+                          uint32_t *counts)
+{
+    auto *p = dynamic_cast<IVFPQ *>(ix);
+    if (!p) fail(HM_ERR_INVALID_ARG, "not an IVFPQIndex");
+    p->last_probes(nq, ldp, probes, counts);
+    return HM_OK;
+}
+
+double ivfpq_mean_nprobe(Index *ix)
+{
+    auto *p = dynamic_cast<IVFPQ *>(ix);
+    if (!p) fail(HM_ERR_INVALID_ARG, "not an IVFPQIndex");
+    return p->last_mean_nprobe();
 }
 
 int32_t ivfpq_set_nprobe(Index *ix, uint32_t nprobe)

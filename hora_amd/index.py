@@ -287,6 +287,33 @@ class IVFPQIndex(_Index):
     def set_search_n_center(self, nprobe: int) -> None:
         check(lib().hm_index_set_nprobe(self._h, nprobe))
 
+    def train_probe(self, train_queries, label_k: int = 10, hidden: int = 512, epochs: int = 4,
+                    batch: int = 1024, lr: float = 1e-3, seed: int = 1) -> float:
+        """Train the learned-probe network on the GPU; returns the last
+        epoch's mean cross-entropy."""
+        q = _f32_rows(train_queries, self._dim)
+        loss = C.c_double()
+        check(lib().hm_ivfpq_train_probe(self._h, q.ctypes.data, q.shape[0], label_k, hidden,
+                                         epochs, batch, lr, seed, C.byref(loss)))
+        return float(loss.value)
+
+    def set_probe_policy(self, mode: int, mass: float = 0.9, pmin: int = 1,
+                         pmax: int = 64) -> None:
+        """0: fixed search_n_center (reference); 1: learned per-query nprobe."""
+        check(lib().hm_ivfpq_set_probe_policy(self._h, mode, mass, pmin, pmax))
+
+    def last_probes(self, nq: int, ldp: int):
+        probes = np.zeros((nq, ldp), np.uint32)
+        counts = np.zeros(nq, np.uint32)
+        check(lib().hm_ivfpq_last_probes(self._h, nq, ldp, probes.ctypes.data,
+                                         counts.ctypes.data))
+        return probes, counts
+
+    def mean_nprobe(self) -> float:
+        out = C.c_double()
+        check(lib().hm_ivfpq_mean_nprobe(self._h, C.byref(out)))
+        return float(out.value)
+
     def info(self) -> dict:
         out = np.zeros(8, np.uint32)
         check(lib().hm_ivfpq_info(self._h, out.ctypes.data))
@@ -312,3 +339,23 @@ class IVFPQIndex(_Index):
 
 
 PQIVFIndex = IVFPQIndex  # README.md:57 name
+
+
+def mlp_loss_grad(W1, b1, W2, b2, X, T=None):
+    """Probe-network forward/backward on the GPU for given weights (gradient
+    checks).  Returns dict(loss, gW1, gb1, gW2, gb2, logits)."""
+    W1, b1, W2, b2, X = (np.ascontiguousarray(a, dtype=np.float32) for a in (W1, b1, W2, b2, X))
+    din, hid = W1.shape
+    dout = W2.shape[1]
+    n = X.shape[0]
+    out = {"logits": np.zeros((n, dout), np.float32)}
+    ptr = lambda a: a.ctypes.data if a is not None else None  # noqa: E731
+    if T is not None:
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        out.update(loss=np.zeros(n, np.float32), gW1=np.zeros_like(W1), gb1=np.zeros_like(b1),
+                   gW2=np.zeros_like(W2), gb2=np.zeros_like(b2))
+    check(lib().hm_mlp_loss_grad(din, hid, dout, n, ptr(W1), ptr(b1), ptr(W2), ptr(b2), ptr(X),
+                                 ptr(T), ptr(out.get("loss")), ptr(out.get("gW1")),
+                                 ptr(out.get("gb1")), ptr(out.get("gW2")), ptr(out.get("gb2")),
+                                 ptr(out["logits"])))
+    return out

@@ -171,6 +171,43 @@ HM_API int32_t hm_ivfpq_info(hm_index *h, uint32_t *out8);
 HM_API int32_t hm_ivfpq_export(hm_index *h, float *centers, uint64_t *list_offsets,
                                uint64_t *list_ids, float *codebooks, uint8_t *codes);
 
+/* Learned probe (no reference counterpart: the reference probes a fixed
+ * search_n_center lists, pq_idx.rs:424).  Trains, on the GPU, a network
+ * dimension -> hidden (ReLU) -> n_kmeans_center whose softmax predicts
+ * which lists hold a query's exact top-label_k (labels from a GPU brute
+ * force over the indexed rows).  BF16 weights, FP32 accumulation, Adam.
+ * final_loss: mean cross-entropy of the last epoch. */
+HM_API int32_t hm_ivfpq_train_probe(hm_index *h, const float *train_queries, uint64_t n,
+                                    uint32_t label_k, uint32_t hidden, uint32_t epochs,
+                                    uint32_t batch, float lr, uint64_t seed, double *final_loss);
+
+/* mode 0: fixed search_n_center (the reference behaviour, and the
+ * fallback); mode 1: learned -- each query probes the smallest set of its
+ * highest-scored lists whose predicted mass reaches `mass`, between pmin and
+ * pmax lists, scanned in descending score with the reference's per-list
+ * scan and merge. */
+HM_API int32_t hm_ivfpq_set_probe_policy(hm_index *h, int32_t mode, float mass, uint32_t pmin,
+                                         uint32_t pmax);
+
+/* Mean number of lists probed per query by the last search. */
+HM_API int32_t hm_ivfpq_mean_nprobe(hm_index *h, double *out);
+
+/* The lists the last search scanned, in scan order: probes [nq][ldp]
+ * (unused slots UINT32_MAX), counts [nq].  Valid when the last search ran
+ * as a single chunk of nq queries.  For parity checks of the learned form. */
+HM_API int32_t hm_ivfpq_last_probes(hm_index *h, uint64_t nq, uint32_t ldp, uint32_t *probes,
+                                    uint32_t *counts);
+
+/* Probe network forward/backward on given weights, for gradient checks:
+ * X [n][din], T [n][dout] targets (NULL: forward only), W1 [din][hidden],
+ * W2 [hidden][dout]; outputs (any may be NULL) per-row loss, gradients of
+ * the mean loss, logits [n][dout].  Inputs are rounded to bf16 as in
+ * training. */
+HM_API int32_t hm_mlp_loss_grad(uint32_t din, uint32_t hidden, uint32_t dout, uint32_t n,
+                                const float *W1, const float *b1, const float *W2,
+                                const float *b2, const float *X, const float *T, float *loss,
+                                float *gW1, float *gb1, float *gW2, float *gb2, float *logits);
+
 /* Per-kernel timing with HIP events recorded on the launch stream around
  * each launch (off by default; instrumentation, no reference counterpart).
  * hm_profile_query synchronises and returns the accumulated milliseconds

@@ -80,7 +80,7 @@ def lib():
     L.orc_pq_encode.argtypes = [_f32p, _sz, _sz, _sz, _sz, C.c_void_p, _f32p, _sz, _u32p]
     L.orc_ivfpq_search.argtypes = [_f32p, _sz, _sz, C.c_int, _f32p, _sz, _f32p, _sz, _sz, _sz,
                                    _u64p, _u64p, _u32p, _sz, _sz, C.c_int, _u64p, _f32p,
-                                   _u32p, C.c_void_p]
+                                   _u32p, C.c_void_p, C.c_void_p, C.c_void_p]
     _lib = L
     return L
 
@@ -293,8 +293,10 @@ def pq_encode(items, n_sub: int, codebook, residual):
 
 
 def ivfpq_search(queries, mt: int, centers, codebooks, list_offsets, list_ids, codes,
-                 nprobe: int, k: int, nthreads: int = 0):
-    """IVFPQIndex::node_search_k on an index given as arrays (see C header)."""
+                 nprobe: int, k: int, nthreads: int = 0, given_probes=None, given_counts=None):
+    """IVFPQIndex::node_search_k on an index given as arrays (see C header).
+    given_probes [nq][nprobe] / given_counts [nq]: scan these lists instead
+    of the coarse top-nprobe."""
     q = _f32(queries)
     nq, d = q.shape
     cen = _f32(centers)
@@ -309,9 +311,13 @@ def ivfpq_search(queries, mt: int, centers, codebooks, list_offsets, list_ids, c
     dist = np.zeros((nq, kk), np.float32)
     cnt = np.zeros(nq, np.uint32)
     probes = np.zeros((nq, max(nprobe, 1)), np.uint32)
+    gp = gc = None
+    if given_probes is not None:
+        gp = np.ascontiguousarray(given_probes, dtype=np.uint32)
+        gc = np.ascontiguousarray(given_counts, dtype=np.uint32)
     st = lib().orc_ivfpq_search(q, nq, d, mt, cen, nlist, cb, n_sub, n_center, sdm, off, lid,
                                 cd, nprobe, k, nthreads, ids, dist, cnt,
-                                probes.ctypes.data_as(C.c_void_p))
+                                probes.ctypes.data_as(C.c_void_p), _ptr(gp), _ptr(gc))
     if st:
         raise OracleError(st, "ivfpq_search")
     return ids[:, :k], dist[:, :k], cnt, probes[:, :nprobe]

@@ -718,8 +718,11 @@ static void pq_adc_heap(const float *lut, const uint32_t *codes, size_t n, size_
     while (h->len > k) (void)heap_pop(h);
 }
 
-/* IVFPQIndex::search_knn_adc (pq_idx.rs:408-439) + node_search_k (458-477)
- * on an index given as arrays:
+/* IVFPQIndex::search_knn_adc (pq_idx.rs:408-439) + node_search_k (458-477).
+ * With given_probes ([nq][nprobe], given_counts[nq] used per query) the
+ * coarse heap is skipped and the given lists are scanned in that order
+ * (the learned-probe form, which has no reference counterpart; the per-list
+ * scan and the merge are the reference's).  The index is given as arrays:
  *   centers [nlist][d], codebooks [nlist][n_sub][n_center][sub_dim_max],
  *   list_offsets [nlist+1], list_ids [n] (global ids, ascending per list),
  *   codes [n][n_sub] in list order (row r belongs to list_ids[r]).
@@ -731,7 +734,8 @@ int orc_ivfpq_search(const float *queries, size_t nq, size_t d, int mt, const fl
                      size_t sub_dim_max, const uint64_t *list_offsets, const uint64_t *list_ids,
                      const uint32_t *codes, size_t nprobe, size_t k, int nthreads,
                      uint64_t *out_idx, float *out_dist, uint32_t *out_count,
-                     uint32_t *out_probes)
+                     uint32_t *out_probes, const uint32_t *given_probes,
+                     const uint32_t *given_counts)
 {
     uint32_t b[256], e[256];
     if (n_sub > 256) return ORC_ERR_ARG;
@@ -753,7 +757,7 @@ int orc_ivfpq_search(const float *queries, size_t nq, size_t d, int mt, const fl
             const float *q = queries + (size_t)qi * d;
             coarse.len = 0;
             top.len = 0;
-            for (size_t i = 0; i < nlist; ++i) {
+            for (size_t i = 0; i < nlist && !given_probes; ++i) {
                 float dist;
                 int s2 = orc_metric(q, centers + i * d, d, mt, &dist);
                 if (s2) {
@@ -766,8 +770,10 @@ int orc_ivfpq_search(const float *queries, size_t nq, size_t d, int mt, const fl
                 orc_nb x = {-dist, (uint64_t)i};
                 heap_push(&coarse, x);
             }
-            for (size_t p = 0; p < nprobe && coarse.len > 0; ++p) {
-                size_t c = (size_t)heap_pop(&coarse).idx;
+            size_t np_q = given_probes ? given_counts[qi] : nprobe;
+            for (size_t p = 0; p < np_q && (given_probes || coarse.len > 0); ++p) {
+                size_t c = given_probes ? given_probes[(size_t)qi * nprobe + p]
+                                        : (size_t)heap_pop(&coarse).idx;
                 if (out_probes) out_probes[(size_t)qi * nprobe + p] = (uint32_t)c;
                 size_t off = list_offsets[c], len = list_offsets[c + 1] - off;
                 pq_lut(q, centers + c * d, d, n_sub, b, e,

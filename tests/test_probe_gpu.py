@@ -1,0 +1,91 @@
+"""Learned probe: GPU forward/backward kernels vs the FP64 oracle, learned
+search parity with the reference's per-list scan over the chosen lists, and
+what the trained network gains over fixed nprobe."""
+import numpy as np
+import pytest
+
+from hora_amd import IVFPQIndex, IVFPQParams, Metric
+from hora_amd.datasets import gaussian_mixture
+from hora_amd.index import mlp_loss_grad
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_mlp_gradients_match_oracle():
+    rng = np.random.default_rng(1)
+    din, hid, dout, n = 24, 40, 72, 37  # ragged vs the 64x64 tiles
+    W1 = rng.normal(0, 0.3, (din, hid)).astype(np.float32)
+    b1 = rng.normal(0, 0.1, hid).astype(np.float32)
+    W2 = rng.normal(0, 0.3, (hid, dout)).astype(np.float32)
+    b2 = rng.normal(0, 0.1, dout).astype(np.float32)
+    X = rng.normal(0, 1, (n, din)).astype(np.float32)
+    T = rng.random((n, dout)).astype(np.float32)
+    T /= T.sum(axis=1, keepdims=True)
+    got = mlp_loss_grad(W1, b1, W2, b2, X, T)
+    ref = mlp.loss_grad(W1, b1, W2, b2, X, T, round_like_gpu=True)
+    # FP32 accumulation vs FP64 on the same bf16 operands; an intermediate
+    # that lands on a bf16 rounding boundary may round one ulp apart.
+    tol = 2e-2
+    assert _rel(got["logits"], ref["logits"]) < 1e-3
+    assert _rel(got["loss"], ref["loss"]) < 1e-3
+    for name in ("gW1", "gb1", "gW2", "gb2"):
+        assert _rel(got[name], ref[name]) < tol, name
+    # and against the unrounded FP64 mathematics (pinned by finite differences)
+    exact = mlp.loss_grad(W1, b1, W2, b2, X, T, round_like_gpu=False)
+    for name in ("gW1", "gW2"):
+        assert _rel(got[name], exact[name]) < 5e-2, name
+
+
+def _index(x, nlist, nprobe):
+    p = (IVFPQParams().n_sub(8).sub_bits(8).n_kmeans_center(nlist).search_n_center(nprobe)
+         .train_epoch(8).seed(5).kmeans_accumulate(1))
+    idx = IVFPQIndex(x.shape[1], p)
+    idx.add_batch(x)
+    idx.build(Metric.Euclidean)
+    return idx
+
+
+def _recall(got, truth):
+    k = truth.shape[1]
+    return float(np.mean([len(set(got[i].tolist()) & set(truth[i].tolist())) / k
+                          for i in range(len(truth))]))
+
+
+def test_learned_probe_parity_and_gain(orc):
+    x, cen, _ = gaussian_mixture(40_000, 32, 128, seed=21, aniso=(0.1, 0.5))
+    train, _, _ = gaussian_mixture(20_000, 32, 128, seed=22, aniso=(0.1, 0.5), centres=cen)
+    test, _, _ = gaussian_mixture(512, 32, 128, seed=23, aniso=(0.1, 0.5), centres=cen)
+    nlist, k = 64, 10
+    idx = _index(x, nlist, 4)
+    loss = idx.train_probe(train, label_k=k, hidden=128, epochs=6, batch=512, lr=2e-3, seed=3)
+    assert np.isfinite(loss)
+    truth, _, _ = orc.bf_search(x, test, k, orc.EUCLIDEAN)
+
+    # learned form: parity with the reference's scan over the chosen lists
+    idx.set_probe_policy(1, mass=0.9, pmin=1, pmax=16)
+    gi, gd, gc = idx.search_batch(test, k)
+    probes, counts = idx.last_probes(len(test), 16)
+    ex = idx.export()
+    ri, rd, rc, _ = orc.ivfpq_search(test, orc.EUCLIDEAN, ex["centers"], ex["codebooks"],
+                                     ex["list_offsets"], ex["list_ids"], ex["codes"], 16, k,
+                                     given_probes=probes, given_counts=counts)
+    assert np.array_equal(gc, rc)
+    assert np.array_equal(gi, ri)
+    assert np.array_equal(gd, rd)
+    learned_np = idx.mean_nprobe()
+    learned_recall = _recall(gi, truth)
+
+    # fixed nprobe at (at least) the same mean budget
+    idx.set_probe_policy(0, pmin=1, pmax=16)
+    p_fixed = int(np.ceil(learned_np))
+    idx.set_search_n_center(p_fixed)
+    fi, _, _ = idx.search_batch(test, k)
+    fixed_recall = _recall(fi, truth)
+    print(f"learned: recall@10 {learned_recall:.4f} at mean nprobe {learned_np:.2f}; "
+          f"fixed nprobe {p_fixed}: recall@10 {fixed_recall:.4f}")
+    assert learned_recall >= fixed_recall - 0.01
