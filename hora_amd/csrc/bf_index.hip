@@ -52,6 +52,8 @@ class BruteForce final : public Index {
     }
     ~BruteForce() override
     {
+        if (rg_exec_) (void)hipGraphExecDestroy(rg_exec_);
+        if (rg_stream_) (void)hipStreamDestroy(rg_stream_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
 
@@ -225,6 +227,112 @@ class BruteForce final : public Index {
         }
     }
 
+    static constexpr uint32_t RAG_CHUNK = 4096;
+
+    void capture_ragged_graph(uint32_t QC, uint32_t kmax, uint32_t S)
+    {
+        if (rg_exec_ && rg_qc_ == QC && rg_kmax_ == kmax && rg_s_ == S && rg_mode_ == mode_ &&
+            rg_metric_ == metric_ && rg_n_ == n_)
+            return;
+        if (rg_exec_) {
+            (void)hipGraphExecDestroy(rg_exec_);
+            rg_exec_ = nullptr;
+        }
+        if (!rg_stream_) HM_HIP(hipStreamCreateWithFlags(&rg_stream_, hipStreamNonBlocking));
+        const uint32_t n = (uint32_t)n_;
+        const bool bf16 = mode_ == HM_SEARCH_BF16_RERANK;
+        const uint32_t row_bytes = bf16 ? dk_ * 2 : dk_;
+        const uint64_t ldk = round_up(n, 4);
+        rg_rows_.reserve(4);
+        rg_q_.reserve((size_t)QC * dim_ * 4);
+        rg_ck_.reserve((size_t)QC * 4);
+        rg_coff_.reserve((size_t)QC * 8);
+        rg_oi_.reserve((size_t)QC * kmax * 8);
+        rg_od_.reserve((size_t)QC * kmax * 4);
+        rg_oc_.reserve((size_t)QC * 4);
+        rg_qpad_.reserve((size_t)QC * dpad_ * 4);
+        rg_qnorm_.reserve((size_t)QC * 4);
+        rg_qlow_.reserve((size_t)QC * row_bytes);
+        rg_qscale_.reserve((size_t)QC * 4);
+        rg_D_.reserve((size_t)QC * ldk * 4);
+        rg_cd_.reserve((size_t)QC * S * 4);
+        rg_ci_.reserve((size_t)QC * S * 4);
+        rg_sf_.reserve((size_t)QC * 4);
+        rg_rf_.reserve((size_t)QC * 4);
+        rg_qinex_.reserve(4);
+        const float u = 5.9604644775390625e-8f;
+        const float gamma = 2.0f * (float)(std::max<uint32_t>(dk_, dpad_) + 8) * u;
+        const float c_bf16 = 1.0f / 128.0f + 1.0f / 65536.0f;
+        const float c_exact = base_inexact_ ? c_bf16 : 0.0f;
+        const uint32_t *dr = rg_rows_.as<uint32_t>();
+        hipStream_t g = rg_stream_;
+        HM_HIP(hipStreamBeginCapture(g, hipStreamCaptureModeThreadLocal));
+        launch_fill_u32(rg_qinex_.as<uint32_t>(), 1, 0, g);
+        launch_pad_rows(rg_q_.as<float>(), dim_, rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_,
+                        dpad_, g, dr);
+        launch_self_dot(rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_, rg_qnorm_.as<float>(), g, dr);
+        if (bf16)
+            launch_to_bf16(rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_, dk_, rg_qlow_.as<uint16_t>(),
+                           rg_qinex_.as<uint32_t>(), g, dr);
+        else
+            launch_to_fp8(rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_, dk_, rg_qlow_.as<uint8_t>(),
+                          rg_qscale_.as<float>(), g, dr);
+        launch_score_gemm(bf16 ? 2 : 1, rg_qlow_.get(), QC, blow_.get(), n, row_bytes,
+                          rg_qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
+                          inv_bnorm_.as<float>(), metric_, rg_D_.as<float>(), ldk, g, dr);
+        launch_row_select(rg_D_.as<float>(), ldk, QC, n, S, rg_cd_.as<float>(),
+                          rg_ci_.as<uint32_t>(), S, rg_sf_.as<uint32_t>(), g, dr);
+        launch_rerank(metric_, rg_qpad_.as<float>(), dpad_, base_.as<float>(), rg_ci_.as<uint32_t>(),
+                      rg_cd_.as<float>(), S, S, rg_qnorm_.as<float>(), bnorm_.as<float>(), QC, n,
+                      kmax, bmax_, gamma, c_exact, c_bf16, rg_qinex_.as<uint32_t>(),
+                      bf16 ? certify_ : 0, rg_oi_.as<uint64_t>(), rg_od_.as<float>(),
+                      rg_oc_.as<uint32_t>(), rg_rf_.as<uint32_t>(), g, dr, rg_ck_.as<uint32_t>(),
+                      rg_coff_.as<uint64_t>());
+        hipGraph_t graph = nullptr;
+        HM_HIP(hipStreamEndCapture(g, &graph));
+        hipError_t e = hipGraphInstantiate(&rg_exec_, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HM_HIP(e);
+        rg_qc_ = QC;
+        rg_kmax_ = kmax;
+        rg_s_ = S;
+        rg_mode_ = mode_;
+        rg_metric_ = metric_;
+        rg_n_ = n_;
+    }
+
+    // Exact path for the listed queries with kmax, packed into the ragged
+    // layout (rg_kq_ / rg_off_ already on the device).
+    void ragged_exact(const float *dq, const std::vector<uint32_t> &rows, uint32_t kmax,
+                      uint64_t *d_idx, float *d_dist, uint32_t *d_count, hipStream_t s)
+    {
+        const uint32_t m = (uint32_t)rows.size();
+        if (m == 0) return;
+        if (kmax == 0 || n_ == 0) {
+            for (uint32_t r : rows) HM_HIP(hipMemsetAsync(d_count + r, 0, 4, s));
+            HM_HIP(hipStreamSynchronize(s));
+            return;
+        }
+        redo_rows_.reserve(m * 4);
+        redo_q_.reserve((size_t)m * dim_ * 4);
+        redo_i_.reserve((size_t)m * kmax * 8);
+        redo_d_.reserve((size_t)m * kmax * 4);
+        redo_c_.reserve((size_t)m * 4);
+        HM_HIP(hipMemcpyAsync(redo_rows_.get(), rows.data(), m * 4, hipMemcpyHostToDevice, s));
+        launch_gather_rows(dq, dim_, redo_rows_.as<uint32_t>(), m, (uint32_t)dim_,
+                           redo_q_.as<float>(), dim_, s);
+        if (!err_.get()) {
+            err_.reserve(4);
+            HM_HIP(hipMemsetAsync(err_.get(), 0, 4, s));
+        }
+        search_exact(redo_q_.as<float>(), dim_, m, kmax, redo_i_.as<uint64_t>(),
+                     redo_d_.as<float>(), redo_c_.as<uint32_t>(), s);
+        launch_scatter_ragged(redo_i_.as<uint64_t>(), redo_d_.as<float>(), redo_c_.as<uint32_t>(),
+                              redo_rows_.as<uint32_t>(), m, kmax, rg_kq_.as<uint32_t>(),
+                              rg_off_.as<uint64_t>(), d_idx, d_dist, d_count, s);
+        HM_HIP(hipStreamSynchronize(s));
+    }
+
     // Derived base data of the shortlist forms (rebuilt after add/mode change).
     void ensure_shortlist_data(hipStream_t s)
     {
@@ -348,6 +456,140 @@ class BruteForce final : public Index {
     }
 
   public:
+    // Ragged batches (batch b = queries [off[b], off[b+1]) with its own k):
+    // the shortlist pipeline for a chunk of up to RAG_CHUNK queries is one
+    // captured hipGraph whose kernels read the chunk's row count, per-query
+    // k and output offsets from device memory, so one graph serves every
+    // chunk and every call.  Results are packed batch after batch, k_b per
+    // query.  Tied / uncertified rows are rerun on the exact path.
+    void search_ragged_device(const float *dq, uint32_t nb, const uint32_t *h_off,
+                              const uint32_t *h_k, uint64_t *d_idx, float *d_dist,
+                              uint32_t *d_count, hipStream_t s) override
+    {
+        if (!s) s = stream_;
+        if (metric_ == HM_METRIC_UNKNOWN) fail(HM_ERR_UNKNOWN_METRIC, "unknown method");
+        if (nb == 0) return;
+        const uint64_t Q = h_off[nb];
+        if (Q == 0) return;
+        std::vector<uint32_t> kq(Q);
+        std::vector<uint64_t> off(Q + 1);
+        uint32_t kmax = 0;
+        for (uint32_t b = 0; b < nb; ++b) {
+            if (h_off[b + 1] < h_off[b]) fail(HM_ERR_INVALID_ARG, "batch offsets not ascending");
+            for (uint64_t q = h_off[b]; q < h_off[b + 1]; ++q) kq[q] = h_k[b];
+            kmax = std::max(kmax, h_k[b]);
+        }
+        off[0] = 0;
+        for (uint64_t q = 0; q < Q; ++q) off[q + 1] = off[q] + kq[q];
+        const bool shortlist_form = mode_ != HM_SEARCH_EXACT && metric_ != HM_METRIC_MANHATTAN &&
+                                    dpad_ <= 4096 && n_ > 0 && kmax > 0;
+        rg_kq_.reserve(Q * 4);
+        rg_off_.reserve((Q + 1) * 8);
+        HM_HIP(hipMemcpyAsync(rg_kq_.get(), kq.data(), Q * 4, hipMemcpyHostToDevice, s));
+        HM_HIP(hipMemcpyAsync(rg_off_.get(), off.data(), (Q + 1) * 8, hipMemcpyHostToDevice, s));
+        if (!shortlist_form) {
+            // exact form: one exact search with kmax, packed per query
+            std::vector<uint32_t> all(Q);
+            for (uint64_t q = 0; q < Q; ++q) all[q] = (uint32_t)q;
+            ragged_exact(dq, all, kmax, d_idx, d_dist, d_count, s);
+            return;
+        }
+        err_.reserve(4);
+        HM_HIP(hipMemsetAsync(err_.get(), 0, 4, s));
+        launch_check_finite(dq, dim_, Q, (uint32_t)dim_, err_.as<uint32_t>(), s);
+        ensure_shortlist_data(s);
+        const uint32_t S = shortlist_size(kmax);
+        const uint32_t QC = RAG_CHUNK;
+        capture_ragged_graph(QC, kmax, S);
+        rg_flags_all_.reserve(Q * 4);
+        uint32_t rows_h = 0;
+        for (uint64_t q0 = 0; q0 < Q; q0 += QC) {
+            rows_h = (uint32_t)std::min<uint64_t>(QC, Q - q0);
+            const uint64_t o0 = off[q0], o1 = off[q0 + rows_h];
+            HM_HIP(hipMemcpyAsync(rg_rows_.get(), &rows_h, 4, hipMemcpyHostToDevice, s));
+            HM_HIP(hipMemcpyAsync(rg_q_.get(), dq + q0 * dim_, (size_t)rows_h * dim_ * 4,
+                                  hipMemcpyDeviceToDevice, s));
+            HM_HIP(hipMemcpyAsync(rg_ck_.get(), kq.data() + q0, (size_t)rows_h * 4,
+                                  hipMemcpyHostToDevice, s));
+            std::vector<uint64_t> loc(rows_h);
+            for (uint32_t i = 0; i < rows_h; ++i) loc[i] = off[q0 + i] - o0;
+            HM_HIP(hipMemcpyAsync(rg_coff_.get(), loc.data(), (size_t)rows_h * 8,
+                                  hipMemcpyHostToDevice, s));
+            HM_HIP(hipStreamSynchronize(s));  // inputs staged before the graph's stream runs
+            HM_HIP(hipGraphLaunch(rg_exec_, rg_stream_));
+            HM_HIP(hipStreamSynchronize(rg_stream_));
+            if (o1 > o0) {
+                HM_HIP(hipMemcpyAsync(d_idx + o0, rg_oi_.get(), (o1 - o0) * 8,
+                                      hipMemcpyDeviceToDevice, s));
+                HM_HIP(hipMemcpyAsync(d_dist + o0, rg_od_.get(), (o1 - o0) * 4,
+                                      hipMemcpyDeviceToDevice, s));
+            }
+            HM_HIP(hipMemcpyAsync(d_count + q0, rg_oc_.get(), (size_t)rows_h * 4,
+                                  hipMemcpyDeviceToDevice, s));
+            HM_HIP(hipMemcpyAsync(rg_flags_all_.as<uint32_t>() + q0, rg_rf_.get(),
+                                  (size_t)rows_h * 4, hipMemcpyDeviceToDevice, s));
+            ++graph_launches_;
+        }
+        std::vector<uint32_t> fl(Q);
+        HM_HIP(hipMemcpyAsync(fl.data(), rg_flags_all_.get(), Q * 4, hipMemcpyDeviceToHost, s));
+        HM_HIP(hipStreamSynchronize(s));
+        std::vector<uint32_t> redo;
+        last_uncertified_ = 0;
+        for (uint64_t q = 0; q < Q; ++q) {
+            if (fl[q] & (ROW_TIE | ROW_UNCERT)) redo.push_back((uint32_t)q);
+            last_uncertified_ += (fl[q] & ROW_UNCERT) ? 1 : 0;
+        }
+        last_fallback_ = redo.size();
+        if (!redo.empty()) ragged_exact(dq, redo, kmax, d_idx, d_dist, d_count, s);
+        uint32_t err = 0;
+        HM_HIP(hipMemcpyAsync(&err, err_.get(), 4, hipMemcpyDeviceToHost, s));
+        HM_HIP(hipStreamSynchronize(s));
+        if (err & 2u) fail(HM_ERR_NONFINITE_INPUT, "invalid float element");
+        if (err & 1u) fail(HM_ERR_NAN_DISTANCE, "NaN distance (partial_cmp of NaN)");
+    }
+
+    // Picks the shortlist depth that minimises the search time of a query
+    // sample (k fixed), trying 64..4096; returns the choice.
+    uint32_t autotune(const float *h_queries, uint64_t nq, uint32_t k) override
+    {
+        if (mode_ == HM_SEARCH_EXACT || metric_ == HM_METRIC_MANHATTAN || n_ == 0 || nq == 0)
+            return 0;
+        hipStream_t s = stream_;
+        DeviceBuffer q, oi, od, oc;
+        q.reserve(nq * dim_ * 4);
+        oi.reserve(nq * std::max(k, 1u) * 8);
+        od.reserve(nq * std::max(k, 1u) * 4);
+        oc.reserve(nq * 4);
+        HM_HIP(hipMemcpyAsync(q.get(), h_queries, nq * dim_ * 4, hipMemcpyHostToDevice, s));
+        hipEvent_t e0, e1;
+        HM_HIP(hipEventCreate(&e0));
+        HM_HIP(hipEventCreate(&e1));
+        float best_ms = 1e30f;
+        uint32_t best = 0;
+        const uint32_t saved = shortlist_;
+        for (uint32_t S = 64; S <= 4096; S *= 2) {
+            if (S < k + 1 || S > n_) continue;
+            shortlist_ = S;
+            search_device(q.as<float>(), nq, k, oi.as<uint64_t>(), od.as<float>(),
+                          oc.as<uint32_t>(), s);  // warm
+            HM_HIP(hipEventRecord(e0, s));
+            search_device(q.as<float>(), nq, k, oi.as<uint64_t>(), od.as<float>(),
+                          oc.as<uint32_t>(), s);
+            HM_HIP(hipEventRecord(e1, s));
+            HM_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            HM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = S;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        shortlist_ = best ? best : saved;
+        return shortlist_;
+    }
+
     void search_stats(uint64_t *fallback, uint64_t *uncertified) const override
     {
         *fallback = last_fallback_;
@@ -372,6 +614,17 @@ class BruteForce final : public Index {
     DeviceBuffer inv_bnorm_, stat_, blow_, bscale_, qlow_, qscale_, qinexact_, rflags_;
     DeviceBuffer redo_rows_, redo_q_, redo_i_, redo_d_, redo_c_;
     uint64_t last_fallback_ = 0, last_uncertified_ = 0;
+    // ragged graph
+    hipStream_t rg_stream_ = nullptr;
+    hipGraphExec_t rg_exec_ = nullptr;
+    uint32_t rg_qc_ = 0, rg_kmax_ = 0, rg_s_ = 0;
+    int32_t rg_mode_ = -1, rg_metric_ = -1;
+    uint64_t rg_n_ = 0, graph_launches_ = 0;
+    DeviceBuffer rg_rows_, rg_q_, rg_ck_, rg_coff_, rg_oi_, rg_od_, rg_oc_, rg_kq_, rg_off_,
+        rg_flags_all_;
+    // buffers owned by the captured graph (never reallocated by other paths)
+    DeviceBuffer rg_qpad_, rg_qnorm_, rg_qlow_, rg_qscale_, rg_D_, rg_cd_, rg_ci_, rg_sf_, rg_rf_,
+        rg_qinex_;
 };
 
 }  // namespace

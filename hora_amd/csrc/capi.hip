@@ -191,6 +191,37 @@ HM_API int32_t hm_index_search_device(hm_index *h, const float *d_queries, uint6
     });
 }
 
+HM_API int32_t hm_index_search_ragged_device(hm_index *h, const float *d_queries,
+                                             uint32_t n_batches, const uint32_t *batch_offsets,
+                                             const uint32_t *batch_k, uint64_t *d_idx,
+                                             float *d_dist, uint32_t *d_count, void *stream)
+{
+    return guard([&] {
+        need(h, "index");
+        if (n_batches == 0) return;
+        need(batch_offsets, "batch_offsets");
+        need(batch_k, "batch_k");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        h->impl->search_ragged_device(d_queries, n_batches, batch_offsets, batch_k, d_idx, d_dist,
+                                      d_count, static_cast<hipStream_t>(stream));
+    });
+}
+
+HM_API int32_t hm_index_autotune(hm_index *h, const float *queries, uint64_t nq, uint32_t k,
+                                 uint32_t *shortlist)
+{
+    return guard([&] {
+        need(h, "index");
+        need(queries, "queries");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        check_finite_host(queries, nq * h->impl->dimension());
+        uint32_t sl = h->impl->autotune(queries, nq, k);
+        if (shortlist) *shortlist = sl;
+    });
+}
+
 HM_API int32_t hm_index_set_search_mode(hm_index *h, int32_t mode, uint32_t shortlist)
 {
     return guard([&] {

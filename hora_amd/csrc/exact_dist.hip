@@ -200,10 +200,11 @@ __global__ __launch_bounds__(256, 2) void exact_tile_kernel(
 // s = sum_t x_t*x_t as a left fold from -0.0 (the reference's dot(x,x) up to
 // sign, calc.rs:13-18), one lane per row.
 __global__ void self_dot_kernel(const float *__restrict__ X, uint32_t ldx, uint32_t rows,
-                                uint32_t d, float *__restrict__ out)
+                                uint32_t d, float *__restrict__ out,
+                                const uint32_t *__restrict__ dev_rows)
 {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
+    if (r >= rows || (dev_rows && r >= *dev_rows)) return;
     const float *x = X + (size_t)r * ldx;
     float acc = -0.0f;
     for (uint32_t t = 0; t < d; ++t) {
@@ -269,10 +270,10 @@ void launch_exact_argmin(int metric, const float *Q, uint32_t ldq, uint32_t nq, 
 }
 
 void launch_self_dot(const float *X, uint32_t ldx, uint32_t rows, uint32_t d, float *out,
-                     hipStream_t s)
+                     hipStream_t s, const uint32_t *dev_rows)
 {
     if (rows == 0) return;
-    self_dot_kernel<<<ceil_div(rows, 256), 256, 0, s>>>(X, ldx, rows, d, out);
+    self_dot_kernel<<<ceil_div(rows, 256), 256, 0, s>>>(X, ldx, rows, d, out, dev_rows);
     HM_CHECK_LAUNCH();
 }
 

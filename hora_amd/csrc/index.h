@@ -27,6 +27,22 @@ class Index {
     virtual uint64_t nodes_size() const = 0;
     virtual void clear() = 0;
     virtual void get_rows(const uint64_t *idx, uint64_t n, float *out) const = 0;
+    // Ragged batches: batch b = queries [off[b], off[b+1]) with k = ks[b];
+    // outputs packed batch after batch (k_b per query), counts per query.
+    virtual void search_ragged_device(const float *dq, uint32_t nb, const uint32_t *h_off,
+                                      const uint32_t *h_k, uint64_t *d_idx, float *d_dist,
+                                      uint32_t *d_count, hipStream_t s)
+    {
+        (void)dq, (void)nb, (void)h_off, (void)h_k, (void)d_idx, (void)d_dist, (void)d_count,
+            (void)s;
+        fail(HM_ERR_NOT_IMPLEMENTED, "ragged search not supported by this index");
+    }
+    // Chooses tuning parameters on a query sample; returns the shortlist.
+    virtual uint32_t autotune(const float *h_queries, uint64_t nq, uint32_t k)
+    {
+        (void)h_queries, (void)nq, (void)k;
+        return 0;
+    }
     // Appends rows already in device memory (n rows of dimension floats,
     // row stride ld floats).
     virtual void add_device(const float *d_rows, uint64_t n_rows, uint64_t ld)

@@ -25,8 +25,10 @@ void launch_exact_argmin(int metric, const float *Q, uint32_t ldq, uint32_t nq, 
                          uint32_t ldb, uint32_t n, uint32_t dpad, const float *qnorm,
                          const float *bnorm, unsigned long long *best, hipStream_t s);
 // out[r] = sum_t X[r][t]^2 (left fold), t < d.
+// dev_rows (optional, device memory): effective row count <= rows, read by
+// the kernels, so a captured graph can be replayed for fewer rows.
 void launch_self_dot(const float *X, uint32_t ldx, uint32_t rows, uint32_t d, float *out,
-                     hipStream_t s);
+                     hipStream_t s, const uint32_t *dev_rows = nullptr);
 
 // ---- select.hip
 // For each of `rows` rows of D (n values, row stride ldd): the K smallest by
@@ -35,7 +37,7 @@ void launch_self_dot(const float *X, uint32_t ldx, uint32_t rows, uint32_t d, fl
 constexpr uint32_t SELECT_MAX_K = 8192;
 void launch_row_select(const float *D, uint64_t ldd, uint32_t rows, uint32_t n, uint32_t K,
                        float *cand_d, uint32_t *cand_i, uint32_t ldc, uint32_t *flags,
-                       hipStream_t s);
+                       hipStream_t s, const uint32_t *dev_rows = nullptr);
 
 // Final result of node_search_k per row: rows without ROW_TIE copy the
 // first min(k, n) candidates; rows with ROW_TIE replay the reference's
@@ -54,7 +56,7 @@ void launch_check_finite(const float *X, uint64_t ld, uint64_t rows, uint32_t d,
 
 // dst[r][0..d) = src[r][0..d), dst[r][d..dpad) = 0.
 void launch_pad_rows(const float *src, uint64_t lds, float *dst, uint64_t ldd, uint64_t rows,
-                     uint32_t d, uint32_t dpad, hipStream_t s);
+                     uint32_t d, uint32_t dpad, hipStream_t s, const uint32_t *dev_rows = nullptr);
 
 // ---- mfma_scores.hip
 // Kmat[q][j] = approximate ranking key of base row j for query q (see file
@@ -63,11 +65,12 @@ void launch_pad_rows(const float *src, uint64_t lds, float *dst, uint64_t ldd, u
 void launch_score_gemm(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
                        uint32_t row_bytes, const float *qscale, const float *bscale,
                        const float *bnorm2, const float *inv_bnorm, int metric, float *Kmat,
-                       uint64_t ldk, hipStream_t s);
+                       uint64_t ldk, hipStream_t s, const uint32_t *dev_rows = nullptr);
 void launch_to_bf16(const float *src, uint64_t lds, uint64_t rows, uint32_t d, uint32_t dk,
-                    uint16_t *dst, uint32_t *inexact, hipStream_t s);
+                    uint16_t *dst, uint32_t *inexact, hipStream_t s,
+                    const uint32_t *dev_rows = nullptr);
 void launch_to_fp8(const float *src, uint64_t lds, uint32_t rows, uint32_t d, uint32_t dk,
-                   uint8_t *dst, float *dequant, hipStream_t s);
+                   uint8_t *dst, float *dequant, hipStream_t s, const uint32_t *dev_rows = nullptr);
 void launch_norm_stats(const float *norm2, uint32_t rows, float *inv, uint32_t *maxbits,
                        hipStream_t s);
 
@@ -77,7 +80,15 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
                    const float *qnorm2, const float *bnorm2, uint32_t rows, uint32_t n,
                    uint32_t k, float bmax, float gamma, float c_in_exact, float c_in_inexact,
                    const uint32_t *q_inexact, int certify, uint64_t *out_idx, float *out_dist,
-                   uint32_t *out_count, uint32_t *flags, hipStream_t st);
+                   uint32_t *out_count, uint32_t *flags, hipStream_t st,
+                   const uint32_t *dev_rows = nullptr, const uint32_t *kq = nullptr,
+                   const uint64_t *out_off = nullptr);
+void launch_scatter_ragged(const uint64_t *si, const float *sd, const uint32_t *sc,
+                           const uint32_t *rows, uint32_t nrows, uint32_t kmax, const uint32_t *kq,
+                           const uint64_t *out_off, uint64_t *di, float *dd, uint32_t *dc,
+                           hipStream_t s);
+// Fill kernel (graph-safe replacement of hipMemsetAsync).
+void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s);
 void launch_gather_rows(const float *src, uint64_t lds, const uint32_t *rows, uint32_t nrows,
                         uint32_t d, float *dst, uint64_t ldd, hipStream_t s);
 void launch_scatter_results(const uint64_t *si, const float *sd, const uint32_t *sc,

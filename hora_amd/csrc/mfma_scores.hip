@@ -33,8 +33,10 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
     const uint8_t *__restrict__ Q, uint32_t nq, const uint8_t *__restrict__ B, uint32_t n,
     uint32_t row_bytes, const float *__restrict__ qscale, const float *__restrict__ bscale,
     const float *__restrict__ bnorm2, const float *__restrict__ inv_bnorm, int metric,
-    float *__restrict__ Kmat, uint64_t ldk)
+    float *__restrict__ Kmat, uint64_t ldk, const uint32_t *__restrict__ dev_rows)
 {
+    if (dev_rows && *dev_rows < nq) nq = *dev_rows;
+    if (blockIdx.y * GM >= nq) return;
     __shared__ __align__(16) uint8_t sA[2][GM * LDS_ROW];
     __shared__ __align__(16) uint8_t sB[2][GN * LDS_ROW];
 
@@ -147,8 +149,9 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
 // Round-to-nearest-even f32 -> bf16; flags a value that does not survive.
 __global__ void to_bf16_kernel(const float *__restrict__ src, uint64_t lds, uint64_t rows,
                                uint32_t d, uint32_t dk, uint16_t *__restrict__ dst,
-                               uint32_t *inexact)
+                               uint32_t *inexact, const uint32_t *__restrict__ dev_rows)
 {
+    if (dev_rows && *dev_rows < rows) rows = *dev_rows;
     uint64_t total = rows * dk;
     uint32_t bad = 0;
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
@@ -167,11 +170,11 @@ __global__ void to_bf16_kernel(const float *__restrict__ src, uint64_t lds, uint
 // converter; dequant[r] = max|x| / 224.  One wave per row.
 __global__ void to_fp8_kernel(const float *__restrict__ src, uint64_t lds, uint32_t rows,
                               uint32_t d, uint32_t dk, uint8_t *__restrict__ dst,
-                              float *__restrict__ dequant)
+                              float *__restrict__ dequant, const uint32_t *__restrict__ dev_rows)
 {
     const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (r >= rows) return;
+    if (r >= rows || (dev_rows && r >= *dev_rows)) return;
     const float *x = src + (size_t)r * lds;
     float m = 0.0f;
     for (uint32_t c = lane; c < d; c += 64) m = fmaxf(m, fabsf(x[c]));
@@ -210,7 +213,7 @@ __global__ void norm_stats_kernel(const float *__restrict__ norm2, uint32_t rows
 void launch_score_gemm(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
                        uint32_t row_bytes, const float *qscale, const float *bscale,
                        const float *bnorm2, const float *inv_bnorm, int metric, float *Kmat,
-                       uint64_t ldk, hipStream_t s)
+                       uint64_t ldk, hipStream_t s, const uint32_t *dev_rows)
 {
     if (nq == 0 || n == 0) return;
     if (row_bytes % ROWB) fail(HM_ERR_INVALID_ARG, "score_gemm: row bytes not a multiple of 64");
@@ -219,28 +222,28 @@ void launch_score_gemm(int elem_bytes, const void *Q, uint32_t nq, const void *B
     if (elem_bytes == 2)
         score_gemm_kernel<2><<<grid, 256, 0, s>>>((const uint8_t *)Q, nq, (const uint8_t *)B, n,
                                                   row_bytes, qscale, bscale, bnorm2, inv_bnorm,
-                                                  metric, Kmat, ldk);
+                                                  metric, Kmat, ldk, dev_rows);
     else
         score_gemm_kernel<1><<<grid, 256, 0, s>>>((const uint8_t *)Q, nq, (const uint8_t *)B, n,
                                                   row_bytes, qscale, bscale, bnorm2, inv_bnorm,
-                                                  metric, Kmat, ldk);
+                                                  metric, Kmat, ldk, dev_rows);
     HM_CHECK_LAUNCH();
 }
 
 void launch_to_bf16(const float *src, uint64_t lds, uint64_t rows, uint32_t d, uint32_t dk,
-                    uint16_t *dst, uint32_t *inexact, hipStream_t s)
+                    uint16_t *dst, uint32_t *inexact, hipStream_t s, const uint32_t *dev_rows)
 {
     if (rows == 0) return;
     unsigned blocks = (unsigned)std::min<uint64_t>((rows * dk + 255) / 256, 8192);
-    to_bf16_kernel<<<blocks, 256, 0, s>>>(src, lds, rows, d, dk, dst, inexact);
+    to_bf16_kernel<<<blocks, 256, 0, s>>>(src, lds, rows, d, dk, dst, inexact, dev_rows);
     HM_CHECK_LAUNCH();
 }
 
 void launch_to_fp8(const float *src, uint64_t lds, uint32_t rows, uint32_t d, uint32_t dk,
-                   uint8_t *dst, float *dequant, hipStream_t s)
+                   uint8_t *dst, float *dequant, hipStream_t s, const uint32_t *dev_rows)
 {
     if (rows == 0) return;
-    to_fp8_kernel<<<ceil_div(rows, 4), 256, 0, s>>>(src, lds, rows, d, dk, dst, dequant);
+    to_fp8_kernel<<<ceil_div(rows, 4), 256, 0, s>>>(src, lds, rows, d, dk, dst, dequant, dev_rows);
     HM_CHECK_LAUNCH();
 }
 

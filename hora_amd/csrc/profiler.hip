@@ -44,6 +44,8 @@ bool profiling_enabled() { return g_on.load(std::memory_order_relaxed); }
 ProfScope::ProfScope(const char *name, hipStream_t s) : name_(name), s_(s)
 {
     if (!profiling_enabled()) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
     if (hipEventCreate(&start_) != hipSuccess || hipEventCreate(&stop_) != hipSuccess) {
         start_ = stop_ = nullptr;
         return;

@@ -40,8 +40,11 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
     uint32_t s, const float *__restrict__ qnorm2, const float *__restrict__ bnorm2, uint32_t n,
     uint32_t k, float bmax, float gamma, float c_in_exact, float c_in_inexact,
     const uint32_t *__restrict__ q_inexact, int certify, uint64_t *__restrict__ out_idx,
-    float *__restrict__ out_dist, uint32_t *__restrict__ out_count, uint32_t *__restrict__ flags)
+    float *__restrict__ out_dist, uint32_t *__restrict__ out_count, uint32_t *__restrict__ flags,
+    const uint32_t *__restrict__ dev_rows, const uint32_t *__restrict__ kq,
+    const uint64_t *__restrict__ out_off)
 {
+    if (dev_rows && blockIdx.x >= *dev_rows) return;  // graph replays with fewer rows
     // dynamic LDS: keys[P] (u64) | vals[P] | sq[dpad]
     extern __shared__ __align__(16) unsigned char rr_smem[];
     __shared__ uint32_t s_tie;
@@ -117,10 +120,13 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
         }
     }
 
+    // ragged batches: this query's own k and output offset
+    if (kq) k = kq[r];
+    const size_t obase = out_off ? out_off[r] : (size_t)r * k;
     const uint32_t m = k < n ? k : n;
     const uint32_t K1 = (k + 1 < s) ? k + 1 : s;  // entries checked for ties
-    uint64_t *oi = out_idx + (size_t)r * k;
-    float *od = out_dist + (size_t)r * k;
+    uint64_t *oi = out_idx + obase;
+    float *od = out_dist + obase;
     const float inf = __int_as_float(0x7f800000);
     for (uint32_t j = tid; j < k; j += RR_THREADS) {
         if (j < m && j < s) {
@@ -198,14 +204,59 @@ __global__ void scatter_results_kernel(const uint64_t *__restrict__ si, const fl
         dc[rows[r]] = sc[r];
 }
 
+// Exact-path results ([m][kmax]) of rows[] copied into a ragged output:
+// query q keeps its first kq[q] results at out_off[q].
+__global__ void scatter_ragged_kernel(const uint64_t *__restrict__ si, const float *__restrict__ sd,
+                                      const uint32_t *__restrict__ sc,
+                                      const uint32_t *__restrict__ rows, uint32_t nrows,
+                                      uint32_t kmax, const uint32_t *__restrict__ kq,
+                                      const uint64_t *__restrict__ out_off,
+                                      uint64_t *__restrict__ di, float *__restrict__ dd,
+                                      uint32_t *__restrict__ dc)
+{
+    const uint32_t r = blockIdx.x;
+    if (r >= nrows) return;
+    const uint32_t q = rows[r], kr = kq[q];
+    for (uint32_t j = threadIdx.x; j < kr; j += blockDim.x) {
+        di[out_off[q] + j] = si[(size_t)r * kmax + j];
+        dd[out_off[q] + j] = sd[(size_t)r * kmax + j];
+    }
+    if (threadIdx.x == 0) dc[q] = sc[r] < kr ? sc[r] : kr;
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t n, uint32_t v)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 }  // namespace
+
+void launch_scatter_ragged(const uint64_t *si, const float *sd, const uint32_t *sc,
+                           const uint32_t *rows, uint32_t nrows, uint32_t kmax, const uint32_t *kq,
+                           const uint64_t *out_off, uint64_t *di, float *dd, uint32_t *dc,
+                           hipStream_t s)
+{
+    if (nrows == 0) return;
+    scatter_ragged_kernel<<<nrows, 64, 0, s>>>(si, sd, sc, rows, nrows, kmax, kq, out_off, di, dd,
+                                               dc);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s)
+{
+    if (n == 0) return;
+    fill_u32_kernel<<<std::min<uint32_t>(ceil_div(n, 256), 1024), 256, 0, s>>>(p, n, v);
+    HM_CHECK_LAUNCH();
+}
 
 void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
                    const uint32_t *cand_i, const float *cand_key, uint32_t ldc, uint32_t s,
                    const float *qnorm2, const float *bnorm2, uint32_t rows, uint32_t n,
                    uint32_t k, float bmax, float gamma, float c_in_exact, float c_in_inexact,
                    const uint32_t *q_inexact, int certify, uint64_t *out_idx, float *out_dist,
-                   uint32_t *out_count, uint32_t *flags, hipStream_t st)
+                   uint32_t *out_count, uint32_t *flags, hipStream_t st,
+                   const uint32_t *dev_rows, const uint32_t *kq, const uint64_t *out_off)
 {
     if (rows == 0) return;
     if (s == 0 || s > RR_CAP) fail(HM_ERR_INVALID_ARG, "rerank: shortlist size out of range");
@@ -217,7 +268,8 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
 #define HM_RR(ACC, EPI)                                                                        \
     rerank_kernel<ACC, EPI><<<rows, RR_THREADS, lds, st>>>(                                     \
         Qp, dpad, Bp, cand_i, cand_key, ldc, s, qnorm2, bnorm2, n, k, bmax, gamma, c_in_exact, \
-        c_in_inexact, q_inexact, certify, out_idx, out_dist, out_count, flags)
+        c_in_inexact, q_inexact, certify, out_idx, out_dist, out_count, flags, dev_rows, kq,    \
+        out_off)
     switch (metric) {
     case HM_METRIC_EUCLIDEAN: HM_RR(ACC_EUCLID, EPI_NONE); break;
     case HM_METRIC_DOT_PRODUCT: HM_RR(ACC_DOT, EPI_NONE); break;

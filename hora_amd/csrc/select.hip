@@ -56,8 +56,10 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_w, u
 
 __global__ __launch_bounds__(SEL_THREADS) void row_select_kernel(
     const float *__restrict__ D, uint64_t ldd, uint32_t n, uint32_t K, float *__restrict__ cand_d,
-    uint32_t *__restrict__ cand_i, uint32_t ldc, uint32_t *__restrict__ flags)
+    uint32_t *__restrict__ cand_i, uint32_t ldc, uint32_t *__restrict__ flags,
+    const uint32_t *__restrict__ dev_rows)
 {
+    if (dev_rows && blockIdx.x >= *dev_rows) return;  // graph replays with fewer rows
     __shared__ uint32_t hist[2048];
     __shared__ unsigned long long cand[SEL_CAP];
     __shared__ uint32_t s_w[SEL_THREADS / 64];
@@ -329,8 +331,10 @@ __global__ void check_finite_kernel(const float *__restrict__ X, uint64_t ld, ui
 }
 
 __global__ void pad_rows_kernel(const float *__restrict__ src, uint64_t lds, float *__restrict__ dst,
-                                uint64_t ldd, uint64_t rows, uint32_t d, uint32_t dpad)
+                                uint64_t ldd, uint64_t rows, uint32_t d, uint32_t dpad,
+                                const uint32_t *__restrict__ dev_rows)
 {
+    if (dev_rows && *dev_rows < rows) rows = *dev_rows;
     uint64_t total = rows * dpad;
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
          e += (uint64_t)gridDim.x * blockDim.x) {
@@ -343,12 +347,13 @@ __global__ void pad_rows_kernel(const float *__restrict__ src, uint64_t lds, flo
 
 void launch_row_select(const float *D, uint64_t ldd, uint32_t rows, uint32_t n, uint32_t K,
                        float *cand_d, uint32_t *cand_i, uint32_t ldc, uint32_t *flags,
-                       hipStream_t s)
+                       hipStream_t s, const uint32_t *dev_rows)
 {
     if (rows == 0 || n == 0 || K == 0) return;
     if (K > SEL_CAP || K > n) fail(HM_ERR_INVALID_ARG, "row_select: K out of range");
     ProfScope prof("row_select", s);
-    row_select_kernel<<<rows, SEL_THREADS, 0, s>>>(D, ldd, n, K, cand_d, cand_i, ldc, flags);
+    row_select_kernel<<<rows, SEL_THREADS, 0, s>>>(D, ldd, n, K, cand_d, cand_i, ldc, flags,
+                                                   dev_rows);
     HM_CHECK_LAUNCH();
 }
 
@@ -385,12 +390,12 @@ void launch_check_finite(const float *X, uint64_t ld, uint64_t rows, uint32_t d,
 }
 
 void launch_pad_rows(const float *src, uint64_t lds, float *dst, uint64_t ldd, uint64_t rows,
-                     uint32_t d, uint32_t dpad, hipStream_t s)
+                     uint32_t d, uint32_t dpad, hipStream_t s, const uint32_t *dev_rows)
 {
     if (rows == 0) return;
     uint64_t total = rows * dpad;
     unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 8192);
-    pad_rows_kernel<<<blocks, 256, 0, s>>>(src, lds, dst, ldd, rows, d, dpad);
+    pad_rows_kernel<<<blocks, 256, 0, s>>>(src, lds, dst, ldd, rows, d, dpad, dev_rows);
     HM_CHECK_LAUNCH();
 }
 

@@ -270,6 +270,21 @@ class BruteForceIndex(_Index):
         """EXACT, BF16_RERANK or FP8_RERANK; shortlist 0 = automatic."""
         check(lib().hm_index_set_search_mode(self._h, mode, shortlist))
 
+    def search_ragged_device(self, q_ptr: int, batch_offsets, batch_k, idx_ptr: int,
+                             dist_ptr: int, count_ptr: int, stream: int = 0) -> None:
+        """Ragged batches with per-batch k on device buffers (see the C header)."""
+        off = np.ascontiguousarray(batch_offsets, dtype=np.uint32)
+        ks = np.ascontiguousarray(batch_k, dtype=np.uint32)
+        check(lib().hm_index_search_ragged_device(self._h, q_ptr, ks.size, off.ctypes.data,
+                                                  ks.ctypes.data, idx_ptr, dist_ptr, count_ptr,
+                                                  stream or None))
+
+    def autotune(self, queries, k: int) -> int:
+        q = _f32_rows(queries, self._dim)
+        out = C.c_uint32()
+        check(lib().hm_index_autotune(self._h, q.ctypes.data, q.shape[0], k, C.byref(out)))
+        return int(out.value)
+
     def search_stats(self) -> Tuple[int, int]:
         """(rows rerun on the exact path, rows uncertified) of the last search."""
         a, b = C.c_uint64(), C.c_uint64()

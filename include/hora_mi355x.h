@@ -127,6 +127,24 @@ HM_API int32_t hm_index_search_device(hm_index *h, const float *d_queries, uint6
                                       uint32_t k, uint64_t *d_idx, float *d_dist,
                                       uint32_t *d_count, void *stream);
 
+/* A set of query batches of uneven size, each with its own k: batch b is
+ * queries [batch_offsets[b], batch_offsets[b+1]) (host array, nb+1 entries)
+ * searched with k = batch_k[b] (host array).  d_queries holds all queries
+ * back to back; results are packed batch after batch, k_b entries per query
+ * (d_idx / d_dist hold sum_b n_b*k_b entries), d_count one per query.  The
+ * shortlist pipeline runs as one captured hipGraph per chunk of <= 4096
+ * queries.  No reference counterpart (the reference searches one query at a
+ * time, ann_index.rs:91-97). */
+HM_API int32_t hm_index_search_ragged_device(hm_index *h, const float *d_queries,
+                                             uint32_t n_batches, const uint32_t *batch_offsets,
+                                             const uint32_t *batch_k, uint64_t *d_idx,
+                                             float *d_dist, uint32_t *d_count, void *stream);
+
+/* Times the shortlist forms on a host query sample for shortlist depths
+ * 64..4096 and keeps the fastest (written to *shortlist). */
+HM_API int32_t hm_index_autotune(hm_index *h, const float *queries, uint64_t nq, uint32_t k,
+                                 uint32_t *shortlist);
+
 /* Selects the BruteForceIndex search form and its shortlist depth
  * (0 = automatic).  No reference counterpart (the reference has one form). */
 HM_API int32_t hm_index_set_search_mode(hm_index *h, int32_t mode, uint32_t shortlist);

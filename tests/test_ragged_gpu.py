@@ -1,0 +1,81 @@
+"""Ragged query batches (uneven sizes, per-batch k) through the captured
+hipGraph path, checked against the oracle; the autotuner's choice keeps
+results exact."""
+import numpy as np
+import pytest
+
+import hora_amd
+from hora_amd import BruteForceIndex, Metric
+from hora_amd.datasets import bf_normal, unit_normal
+
+pytestmark = pytest.mark.gpu
+
+
+def _ragged(idx, q, sizes, ks):
+    import torch
+
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint32)
+    tot = int(sum(s * k for s, k in zip(sizes, ks)))
+    dq = torch.from_numpy(q).cuda()
+    di = torch.empty(max(tot, 1), dtype=torch.int64, device="cuda")
+    dd = torch.empty(max(tot, 1), dtype=torch.float32, device="cuda")
+    dc = torch.empty(q.shape[0], dtype=torch.int32, device="cuda")
+    idx.search_ragged_device(dq.data_ptr(), off, ks, di.data_ptr(), dd.data_ptr(),
+                             dc.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    return (di.cpu().numpy().view(np.uint64)[:tot], dd.cpu().numpy()[:tot],
+            dc.cpu().numpy().view(np.uint32), off)
+
+
+def _check(orc, base, q, sizes, ks, mt, gi, gd, gc, off):
+    pos = 0
+    for b, (s, k) in enumerate(zip(sizes, ks)):
+        qb = q[off[b]:off[b + 1]]
+        ri, rd, rc = orc.bf_search(base, qb, k, int(mt))
+        assert np.array_equal(gc[off[b]:off[b + 1]], rc), f"batch {b} counts"
+        for r in range(s):
+            assert np.array_equal(gi[pos:pos + k], ri[r]), f"batch {b} row {r}"
+            assert np.array_equal(gd[pos:pos + k], rd[r])
+            pos += k
+
+
+@pytest.mark.parametrize("mode", [hora_amd.HM_SEARCH_BF16_RERANK, hora_amd.HM_SEARCH_EXACT])
+def test_ragged_batches_match_oracle(orc, mode):
+    rng = np.random.default_rng(9)
+    base = unit_normal(30_000, 96, seed=51)
+    idx = BruteForceIndex(96)
+    idx.add_batch(base)
+    idx.build(Metric.CosineSimilarity)
+    idx.set_search_mode(mode, 0)
+    for call in range(2):  # the second call replays the captured graph
+        sizes = rng.integers(1, 700, 9).tolist()
+        ks = rng.choice([1, 10, 100], 9).tolist()
+        q = unit_normal(int(sum(sizes)), 96, seed=60 + call)
+        gi, gd, gc, off = _ragged(idx, q, sizes, ks)
+        _check(orc, base, q, sizes, ks, Metric.CosineSimilarity, gi, gd, gc, off)
+
+
+def test_ragged_chunks_above_graph_capacity(orc):
+    base = bf_normal(8_000, 40, seed=71)
+    idx = BruteForceIndex(40)
+    idx.add_batch(base)
+    idx.build(Metric.Euclidean)
+    idx.set_search_mode(hora_amd.HM_SEARCH_BF16_RERANK, 128)
+    sizes, ks = [4096, 1, 3000], [10, 100, 1]  # 7097 queries: two graph chunks
+    q = bf_normal(sum(sizes), 40, seed=72)
+    gi, gd, gc, off = _ragged(idx, q, sizes, ks)
+    _check(orc, base, q, sizes, ks, Metric.Euclidean, gi, gd, gc, off)
+
+
+def test_autotune_keeps_results_exact(orc):
+    base = unit_normal(20_000, 64, seed=81)
+    q = unit_normal(256, 64, seed=82)
+    idx = BruteForceIndex(64)
+    idx.add_batch(base)
+    idx.build(Metric.DotProduct)
+    idx.set_search_mode(hora_amd.HM_SEARCH_BF16_RERANK, 0)
+    s = idx.autotune(q, 10)
+    assert 64 <= s <= 4096
+    gi, gd, gc = idx.search_batch(q, 10)
+    ri, rd, rc = orc.bf_search(base, q, 10, orc.DOT)
+    assert np.array_equal(gi, ri) and np.array_equal(gd, rd)
