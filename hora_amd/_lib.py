@@ -71,6 +71,7 @@ def lib():
     L.hm_index_create.argtypes = [i32, u64, C.POINTER(HmParams), C.POINTER(vp)]
     L.hm_index_destroy.argtypes = [vp]
     L.hm_index_add.argtypes = [vp, vp, u64]
+    L.hm_index_add_device.argtypes = [vp, vp, u64]
     L.hm_index_build.argtypes = [vp, i32]
     L.hm_index_built.argtypes = [vp, C.POINTER(i32)]
     L.hm_index_search.argtypes = [vp, vp, u64, u32, vp, vp, vp]

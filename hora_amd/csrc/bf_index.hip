@@ -166,6 +166,7 @@ class BruteForce final : public Index {
         HM_HIP(hipMemcpyAsync(&err, err_.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HM_HIP(hipStreamSynchronize(s));
         if (err & 2u) fail(HM_ERR_NONFINITE_INPUT, "invalid float element");
+        if (err & 4u) fail(HM_ERR_HIP, "internal: selection consistency check failed");
         if (err & 1u) fail(HM_ERR_NAN_DISTANCE, "NaN distance (partial_cmp of NaN)");
     }
 
@@ -432,7 +433,7 @@ class BruteForce final : public Index {
         HM_HIP(hipStreamSynchronize(s));
         std::vector<uint32_t> redo;
         for (uint64_t i = 0; i < nq; ++i)
-            if (fl[i] & (ROW_TIE | ROW_UNCERT)) redo.push_back((uint32_t)i);
+            if (fl[i] & (ROW_TIE | ROW_UNCERT | ROW_BAD)) redo.push_back((uint32_t)i);
         last_fallback_ = redo.size();
         last_uncertified_ = 0;
         for (uint64_t i = 0; i < nq; ++i) last_uncertified_ += (fl[i] & ROW_UNCERT) ? 1 : 0;
@@ -536,7 +537,7 @@ class BruteForce final : public Index {
         std::vector<uint32_t> redo;
         last_uncertified_ = 0;
         for (uint64_t q = 0; q < Q; ++q) {
-            if (fl[q] & (ROW_TIE | ROW_UNCERT)) redo.push_back((uint32_t)q);
+            if (fl[q] & (ROW_TIE | ROW_UNCERT | ROW_BAD)) redo.push_back((uint32_t)q);
             last_uncertified_ += (fl[q] & ROW_UNCERT) ? 1 : 0;
         }
         last_fallback_ = redo.size();

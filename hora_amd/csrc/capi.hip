@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "index.h"
+#include "kernels.h"
 #include "mlp.h"
 
 using namespace hm;
@@ -119,6 +120,27 @@ HM_API int32_t hm_index_add(hm_index *h, const float *rows, uint64_t n_rows)
         std::lock_guard<std::mutex> lk(h->mu);
         DeviceScope ds(h->device);
         h->impl->add(rows, n_rows);
+    });
+}
+
+HM_API int32_t hm_index_add_device(hm_index *h, const float *d_rows, uint64_t n_rows)
+{
+    return guard([&] {
+        need(h, "index");
+        if (n_rows == 0) return;
+        need(d_rows, "rows");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        hipStream_t s = h->impl->stream();
+        const uint64_t d = h->impl->dimension();
+        h->err_stage.reserve(4);
+        HM_HIP(hipMemsetAsync(h->err_stage.get(), 0, 4, s));
+        launch_check_finite(d_rows, d, n_rows, (uint32_t)d, h->err_stage.as<uint32_t>(), s);
+        uint32_t err = 0;
+        HM_HIP(hipMemcpyAsync(&err, h->err_stage.get(), 4, hipMemcpyDeviceToHost, s));
+        HM_HIP(hipStreamSynchronize(s));
+        if (err) fail(HM_ERR_NONFINITE_INPUT, "invalid float element");
+        h->impl->add_device(d_rows, n_rows, d);
     });
 }
 

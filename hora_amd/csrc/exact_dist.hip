@@ -96,35 +96,36 @@ __global__ __launch_bounds__(256, 2) void exact_tile_kernel(
 #pragma unroll
         for (int j = 0; j < 8; ++j) acc[i][j] = -0.0f;  // Rust's f32 Sum starts at -0.0
 
-    float4 rq[2], rb[2];
-    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-
+    // Register staging of the next 16-dim slab: two float4 per operand.  Rows
+    // past the end load from a clamped (valid) row and are zeroed afterwards.
+    float4 rq0, rq1, rb0, rb1;
+    const int row0 = tid >> 2, row1 = (tid + 256) >> 2, c4 = (tid & 3) * 4;
+    const bool qa0 = q0 + row0 < nq, qa1 = q0 + row1 < nq;
+    const bool ba0 = n0 + row0 < n, ba1 = n0 + row1 < n;
+    const float *qp0 = Q + (size_t)(qa0 ? q0 + row0 : nq - 1) * ldq + c4;
+    const float *qp1 = Q + (size_t)(qa1 ? q0 + row1 : nq - 1) * ldq + c4;
+    const float *bp0 = B + (size_t)(ba0 ? n0 + row0 : n - 1) * ldb + c4;
+    const float *bp1 = B + (size_t)(ba1 ? n0 + row1 : n - 1) * ldb + c4;
+    auto zero_if = [](float4 v, bool keep) {
+        return keep ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+    };
     auto gload = [&](uint32_t k0) {
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            int idx = tid + 256 * s;
-            int row = idx >> 2, c4 = idx & 3;
-            uint32_t qr = q0 + row, br = n0 + row;
-            rq[s] = qr < nq ? *reinterpret_cast<const float4 *>(Q + (size_t)qr * ldq + k0 + c4 * 4)
-                            : zero4;
-            rb[s] = br < n ? *reinterpret_cast<const float4 *>(B + (size_t)br * ldb + k0 + c4 * 4)
-                           : zero4;
-        }
+        rq0 = zero_if(*reinterpret_cast<const float4 *>(qp0 + k0), qa0);
+        rq1 = zero_if(*reinterpret_cast<const float4 *>(qp1 + k0), qa1);
+        rb0 = zero_if(*reinterpret_cast<const float4 *>(bp0 + k0), ba0);
+        rb1 = zero_if(*reinterpret_cast<const float4 *>(bp1 + k0), ba1);
+    };
+    auto put = [&](float (*S)[TQ + LDS_PAD], int row, float4 v) {
+        S[c4 + 0][row] = v.x;
+        S[c4 + 1][row] = v.y;
+        S[c4 + 2][row] = v.z;
+        S[c4 + 3][row] = v.w;
     };
     auto sstore = [&](int buf) {
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            int idx = tid + 256 * s;
-            int row = idx >> 2, c4 = idx & 3;
-            sQ[buf][c4 * 4 + 0][row] = rq[s].x;
-            sQ[buf][c4 * 4 + 1][row] = rq[s].y;
-            sQ[buf][c4 * 4 + 2][row] = rq[s].z;
-            sQ[buf][c4 * 4 + 3][row] = rq[s].w;
-            sB[buf][c4 * 4 + 0][row] = rb[s].x;
-            sB[buf][c4 * 4 + 1][row] = rb[s].y;
-            sB[buf][c4 * 4 + 2][row] = rb[s].z;
-            sB[buf][c4 * 4 + 3][row] = rb[s].w;
-        }
+        put(sQ[buf], row0, rq0);
+        put(sQ[buf], row1, rq1);
+        put(sB[buf], row0, rb0);
+        put(sB[buf], row1, rb1);
     };
 
     const uint32_t nk = dpad / TK;

@@ -105,5 +105,5 @@ struct hm_index {
     std::mutex mu;
     int device = 0;
     // Host-entry staging buffers.
-    hm::DeviceBuffer q_stage, idx_stage, dist_stage, count_stage;
+    hm::DeviceBuffer q_stage, idx_stage, dist_stage, count_stage, err_stage;
 };

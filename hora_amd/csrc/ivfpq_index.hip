@@ -89,6 +89,21 @@ class IVFPQ final : public Index {
         n_ = need;
     }
 
+    void add_device(const float *d_rows, uint64_t n_rows, uint64_t ld) override
+    {
+        if (n_rows == 0) return;
+        if (n_ + n_rows > 0xFFFFFFF0ull) fail(HM_ERR_CAPACITY, "more than 2^32-16 rows");
+        const uint64_t need = n_ + n_rows;
+        if (need > cap_) {
+            base_.grow_keep(need * dpad_ * 4, n_ * dpad_ * 4, stream_);
+            cap_ = need;
+        }
+        launch_pad_rows(d_rows, ld, base_.as<float>() + n_ * dpad_, dpad_, n_rows, (uint32_t)dim_,
+                        dpad_, stream_);
+        HM_HIP(hipStreamSynchronize(stream_));
+        n_ = need;
+    }
+
     void build(int32_t metric) override
     {
         if (metric < HM_METRIC_UNKNOWN || metric > HM_METRIC_ANGULAR)

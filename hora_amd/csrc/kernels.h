@@ -12,6 +12,7 @@ enum EpilogueKind { EPI_NONE = 0, EPI_COSINE = 1, EPI_ANGULAR = 2 };
 constexpr uint32_t ROW_TIE = 1u;   // equal distances among the first min(k+1, n)
 constexpr uint32_t ROW_NAN = 2u;   // a NaN distance in the row
 constexpr uint32_t ROW_UNCERT = 4u;  // shortlist could not be certified
+constexpr uint32_t ROW_BAD = 8u;     // internal consistency check failed
 
 // ---- exact_dist.hip
 // D[q][j] = metric(Q[q], B[j]) in the reference's summation order.

@@ -50,38 +50,47 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    uint4 ra[2], rb[2];
-    const uint4 z4 = make_uint4(0, 0, 0, 0);
-    auto gload = [&](uint32_t kb) {
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            int idx = tid + 256 * s;  // 512 chunks of 16 B per operand tile
-            int row = idx >> 2, c = idx & 3;
-            uint32_t qr = q0 + row, br = n0 + row;
-            ra[s] = qr < nq ? *reinterpret_cast<const uint4 *>(Q + (size_t)qr * row_bytes + kb + c * 16)
-                            : z4;
-            rb[s] = br < n ? *reinterpret_cast<const uint4 *>(B + (size_t)br * row_bytes + kb + c * 16)
-                           : z4;
-        }
-    };
-    auto sstore = [&](int buf) {
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            int idx = tid + 256 * s;
-            int row = idx >> 2, c = idx & 3;
-            *reinterpret_cast<uint4 *>(&sA[buf][row * LDS_ROW + c * 16]) = ra[s];
-            *reinterpret_cast<uint4 *>(&sB[buf][row * LDS_ROW + c * 16]) = rb[s];
-        }
-    };
+    // Register staging of the next K step: two 16-byte chunks per operand.
+    // Rows past the end load from a clamped (valid) row and are zeroed
+    // afterwards, so every load is a plain global load of a real row.
+    uint4 ra0, ra1, rb0, rb1;
+    const int row0 = tid >> 2, row1 = (tid + 256) >> 2, cc = (tid & 3) * 16;
+    const bool qa0 = q0 + row0 < nq, qa1 = q0 + row1 < nq;
+    const bool ba0 = n0 + row0 < n, ba1 = n0 + row1 < n;
+    const uint8_t *qp0 = Q + (size_t)(qa0 ? q0 + row0 : nq - 1) * row_bytes + cc;
+    const uint8_t *qp1 = Q + (size_t)(qa1 ? q0 + row1 : nq - 1) * row_bytes + cc;
+    const uint8_t *bp0 = B + (size_t)(ba0 ? n0 + row0 : n - 1) * row_bytes + cc;
+    const uint8_t *bp1 = B + (size_t)(ba1 ? n0 + row1 : n - 1) * row_bytes + cc;
+    const uint32_t mq0 = qa0 ? ~0u : 0u, mq1 = qa1 ? ~0u : 0u;
+    const uint32_t mb0 = ba0 ? ~0u : 0u, mb1 = ba1 ? ~0u : 0u;
+#define HM_MASK4(v, m) make_uint4((v).x & (m), (v).y & (m), (v).z & (m), (v).w & (m))
+#define HM_GLOAD(kb)                                                                   \
+    do {                                                                               \
+        const uint4 t0 = *reinterpret_cast<const uint4 *>(qp0 + (kb));                 \
+        const uint4 t1 = *reinterpret_cast<const uint4 *>(qp1 + (kb));                 \
+        const uint4 t2 = *reinterpret_cast<const uint4 *>(bp0 + (kb));                 \
+        const uint4 t3 = *reinterpret_cast<const uint4 *>(bp1 + (kb));                 \
+        ra0 = HM_MASK4(t0, mq0);                                                       \
+        ra1 = HM_MASK4(t1, mq1);                                                       \
+        rb0 = HM_MASK4(t2, mb0);                                                       \
+        rb1 = HM_MASK4(t3, mb1);                                                       \
+    } while (0)
+#define HM_SSTORE(buf)                                                           \
+    do {                                                                         \
+        *reinterpret_cast<uint4 *>(&sA[buf][row0 * LDS_ROW + cc]) = ra0;         \
+        *reinterpret_cast<uint4 *>(&sA[buf][row1 * LDS_ROW + cc]) = ra1;         \
+        *reinterpret_cast<uint4 *>(&sB[buf][row0 * LDS_ROW + cc]) = rb0;         \
+        *reinterpret_cast<uint4 *>(&sB[buf][row1 * LDS_ROW + cc]) = rb1;         \
+    } while (0)
 
     const uint32_t nk = row_bytes / ROWB;
-    gload(0);
-    sstore(0);
+    HM_GLOAD(0u);
+    HM_SSTORE(0);
     __syncthreads();
     const int lr = lane & 15, lk = lane >> 4;
     for (uint32_t kt = 0; kt < nk; ++kt) {
         const int cur = kt & 1;
-        if (kt + 1 < nk) gload((kt + 1) * ROWB);
+        if (kt + 1 < nk) HM_GLOAD((kt + 1) * ROWB);
         const uint8_t *As = sA[cur];
         const uint8_t *Bs = sB[cur];
         if constexpr (ELEM == 2) {
@@ -116,9 +125,12 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
                         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a[i], b[j], acc[i][j], 0, 0, 0);
             }
         }
-        if (kt + 1 < nk) sstore(cur ^ 1);
+        if (kt + 1 < nk) HM_SSTORE(cur ^ 1);
         __syncthreads();
     }
+#undef HM_GLOAD
+#undef HM_SSTORE
+#undef HM_MASK4
 
     // Epilogue: lane holds D[4*lk + e][lr] of each 16x16 tile.
 #pragma unroll

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
     if (dev_rows && blockIdx.x >= *dev_rows) return;  // graph replays with fewer rows
     // dynamic LDS: keys[P] (u64) | vals[P] | sq[dpad]
     extern __shared__ __align__(16) unsigned char rr_smem[];
-    __shared__ uint32_t s_tie;
+    __shared__ uint32_t s_tie, s_bad;
     uint32_t P = 1;
     while (P < s) P <<= 1;
     unsigned long long *keys = reinterpret_cast<unsigned long long *>(rr_smem);
@@ -56,7 +56,10 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
 
     const int tid = threadIdx.x;
     const uint32_t r = blockIdx.x;
-    if (tid == 0) s_tie = 0;
+    if (tid == 0) {
+        s_tie = 0;
+        s_bad = 0;
+    }
     for (uint32_t t = tid; t < dpad; t += RR_THREADS) sq[t] = Qp[(size_t)r * dpad + t];
     __syncthreads();
 
@@ -64,6 +67,12 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
     const float qn2 = qnorm2[r];  // sum q_t^2 (left fold), needed by every bound
     for (uint32_t j = tid; j < s; j += RR_THREADS) {
         const uint32_t idx = ci[j];
+        if (idx >= n) {  // never expected: keep the read in range and flag the row
+            s_bad = 1;
+            vals[j] = __int_as_float(0x7f800000);
+            keys[j] = ~0ull;
+            continue;
+        }
         const float4 *b4 = reinterpret_cast<const float4 *>(Bp + (size_t)idx * dpad);
         float acc = -0.0f;
         for (uint32_t t4 = 0; t4 < dpad / 4; ++t4) {
@@ -142,7 +151,7 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
     __syncthreads();
     if (tid != 0) return;
     out_count[r] = m < s ? m : s;
-    uint32_t f = s_tie ? ROW_TIE : 0u;
+    uint32_t f = (s_tie ? ROW_TIE : 0u) | (s_bad ? ROW_BAD : 0u);
     if (s < n && certify) {
         bool ok = false;
         if (k + 1 <= s) {

@@ -77,7 +77,10 @@ __global__ __launch_bounds__(SEL_THREADS) void row_select_kernel(
     }
 
     // Radix select of the K-th smallest key: digits of 11, 11 and 10 bits.
+    // After each pass, when every key up to the target bucket fits in LDS,
+    // they are collected and sorted directly (usually after the first pass).
     uint32_t mask = 0;
+    bool early = false;
 #pragma unroll 1
     for (int pass = 0; pass < 3; ++pass) {
         const int shift = pass == 0 ? 21 : (pass == 1 ? 10 : 0);
@@ -114,36 +117,62 @@ __global__ __launch_bounds__(SEL_THREADS) void row_select_kernel(
         }
         __syncthreads();
         mask |= (nb - 1) << shift;
+        if ((K - s_remaining) + s_bucket <= SEL_CAP) {
+            early = true;
+            break;
+        }
     }
 
-    // Collect: every key < T, then the `rem` lowest indices with key == T.
+    uint32_t total;  // candidates in LDS, sorted below; the first K are kept
+    bool bad = false;
     const uint32_t T = s_prefix, rem = s_remaining, ceq = s_bucket;
-    const uint32_t nlt = K - rem;
-    const bool all_eq = (ceq == rem);
-    for (uint32_t i = tid; i < n; i += SEL_THREADS) {
-        uint32_t key = fkey(row[i]);
-        if (key < T || (all_eq && key == T)) {
-            uint32_t pos = atomicAdd(&s_cnt, 1u);
-            cand[pos] = ((unsigned long long)key << 32) | i;
+    if (early) {
+        // every key whose top bits are <= the target prefix
+        total = (K - rem) + ceq;
+        for (uint32_t i = tid; i < n; i += SEL_THREADS) {
+            uint32_t key = fkey(row[i]);
+            if ((key & mask) <= T) {
+                uint32_t pos = atomicAdd(&s_cnt, 1u);
+                if (pos < SEL_CAP) cand[pos] = ((unsigned long long)key << 32) | i;
+            }
+        }
+        __syncthreads();
+        bad = s_cnt != total;
+    } else {
+        // Collect: every key < T, then the `rem` lowest indices with key == T.
+        total = K;
+        const uint32_t nlt = K - rem;
+        const bool all_eq = (ceq == rem);
+        for (uint32_t i = tid; i < n; i += SEL_THREADS) {
+            uint32_t key = fkey(row[i]);
+            if (key < T || (all_eq && key == T)) {
+                uint32_t pos = atomicAdd(&s_cnt, 1u);
+                if (pos < SEL_CAP) cand[pos] = ((unsigned long long)key << 32) | i;
+            }
+        }
+        __syncthreads();
+        bad = s_cnt != (all_eq ? K : nlt);
+        if (!all_eq && !bad) {
+            uint32_t taken = 0;  // uniform over the block
+            for (uint32_t base = 0; base < n && taken < rem; base += SEL_THREADS) {
+                uint32_t i = base + tid;
+                uint32_t p = (i < n && fkey(row[i]) == T) ? 1u : 0u;
+                uint32_t tot;
+                uint32_t ex = block_excl_scan(p, s_w, &tot);
+                if (p && taken + ex < rem) cand[nlt + taken + ex] = ((unsigned long long)T << 32) | i;
+                taken += tot;
+            }
         }
     }
-    __syncthreads();
-    if (!all_eq) {
-        uint32_t taken = 0;  // uniform over the block
-        for (uint32_t base = 0; base < n && taken < rem; base += SEL_THREADS) {
-            uint32_t i = base + tid;
-            uint32_t p = (i < n && fkey(row[i]) == T) ? 1u : 0u;
-            uint32_t tot;
-            uint32_t ex = block_excl_scan(p, s_w, &tot);
-            if (p && taken + ex < rem) cand[nlt + taken + ex] = ((unsigned long long)T << 32) | i;
-            taken += tot;
-        }
+    if (bad) {  // never expected: keep every index in range and flag the row
+        total = K;
+        for (uint32_t j = tid; j < K; j += SEL_THREADS) cand[j] = ~0ull << 32;
     }
 
-    // Bitonic sort of the K candidates (padded to a power of two).
+    // Bitonic sort of the candidates (padded to a power of two).
     uint32_t P = 1;
-    while (P < K) P <<= 1;
-    for (uint32_t j = K + tid; j < P; j += SEL_THREADS) cand[j] = ~0ull;
+    while (P < total) P <<= 1;
+    for (uint32_t j = total + tid; j < P; j += SEL_THREADS) cand[j] = ~0ull;
     __syncthreads();
     for (uint32_t size = 2; size <= P; size <<= 1) {
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
@@ -168,12 +197,14 @@ __global__ __launch_bounds__(SEL_THREADS) void row_select_kernel(
         unsigned long long c = cand[j];
         uint32_t idx = (uint32_t)c;
         oi[j] = idx;
-        od[j] = row[idx];
+        od[j] = idx < n ? row[idx] : __int_as_float(0x7f800000);
         if (j + 1 < K && (uint32_t)(c >> 32) == (uint32_t)(cand[j + 1] >> 32)) tie = 1;
     }
     if (tie) s_tie = 1;
     __syncthreads();
-    if (tid == 0) flags[blockIdx.x] = (s_tie ? ROW_TIE : 0u) | (s_nan ? ROW_NAN : 0u);
+    if (tid == 0)
+        flags[blockIdx.x] =
+            (s_tie ? ROW_TIE : 0u) | (s_nan ? ROW_NAN : 0u) | (bad ? ROW_BAD : 0u);
 }
 
 // ---- std BinaryHeap model (max-heap on distance only) on a per-lane array.
@@ -243,6 +274,170 @@ struct LaneHeap {
     }
 };
 
+// ---- std BinaryHeap held across the lanes of one wave: lane j owns slot j.
+// Every operation runs with uniform control flow; a slot is read with
+// v_readlane (uniform index) and written by the owning lane.
+struct WaveHeap {
+    float d;     // this lane's slot
+    uint32_t i;  // this lane's id
+    uint32_t len;
+    int lane;
+
+    __device__ float gd(uint32_t p) const
+    {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), (int)p));
+    }
+    __device__ uint32_t gi(uint32_t p) const
+    {
+        return (uint32_t)__builtin_amdgcn_readlane((int)i, (int)p);
+    }
+    __device__ void put(uint32_t p, float v, uint32_t id)
+    {
+        if ((uint32_t)lane == p) {
+            d = v;
+            i = id;
+        }
+    }
+    __device__ void sift_up(uint32_t start, uint32_t pos)
+    {
+        const float ed = gd(pos);
+        const uint32_t ei = gi(pos);
+        while (pos > start) {
+            const uint32_t parent = (pos - 1) / 2;
+            const float pd = gd(parent);
+            if (ed <= pd) break;
+            put(pos, pd, gi(parent));
+            pos = parent;
+        }
+        put(pos, ed, ei);
+    }
+    __device__ void sift_down_to_bottom(uint32_t pos)
+    {
+        const uint32_t end = len, start = pos;
+        const float ed = gd(pos);
+        const uint32_t ei = gi(pos);
+        uint32_t child = 2 * pos + 1;
+        while (end >= 2 && child < end - 1) {
+            child += (gd(child) <= gd(child + 1)) ? 1u : 0u;
+            put(pos, gd(child), gi(child));
+            pos = child;
+            child = 2 * pos + 1;
+        }
+        if (end >= 1 && child == end - 1) {
+            put(pos, gd(child), gi(child));
+            pos = child;
+        }
+        put(pos, ed, ei);
+        sift_up(start, pos);
+    }
+    __device__ void push(float x, uint32_t id)
+    {
+        put(len, x, id);
+        len += 1;
+        sift_up(0, len - 1);
+    }
+    __device__ void pop(float *od, uint32_t *oi)
+    {
+        const float ld = gd(len - 1);
+        const uint32_t li = gi(len - 1);
+        len -= 1;
+        if (len > 0) {
+            *od = gd(0);
+            *oi = gi(0);
+            put(0, ld, li);
+            sift_down_to_bottom(0);
+        } else {
+            *od = ld;
+            *oi = li;
+        }
+    }
+};
+
+// Replays bruteforce_idx.rs:47-73 over a tied row with the heap in the
+// lanes of one wave (k + 1 <= 64).  An element x greater than the heap's
+// maximum is pushed and popped again; that round trip permutes the layout
+// in a way that depends on the layout only, so when it is known to leave
+// the current layout unchanged (checked once after every change) such
+// elements are skipped.  One workgroup of 64 lanes per row; rows without a
+// tie copy their candidates.
+__global__ __launch_bounds__(64) void bf_finalize_wave_kernel(
+    const float *__restrict__ cand_d, const uint32_t *__restrict__ cand_i, uint32_t ldc,
+    const uint32_t *__restrict__ flags, const float *__restrict__ D, uint64_t ldd, uint32_t n,
+    uint32_t k, uint64_t *__restrict__ out_idx, float *__restrict__ out_dist,
+    uint32_t *__restrict__ out_count, uint32_t *err)
+{
+    const uint32_t r = blockIdx.x;
+    const int lane = threadIdx.x;
+    const float inf = __int_as_float(0x7f800000);
+    uint64_t *oi = out_idx + (size_t)r * k;
+    float *od = out_dist + (size_t)r * k;
+    const uint32_t m = k < n ? k : n;
+    const uint32_t f = flags[r];
+    if ((f & (ROW_NAN | ROW_BAD)) && n >= 2) {
+        if (lane == 0) atomicOr(err, (f & ROW_BAD) ? 4u : 1u);
+        for (uint32_t j = lane; j < k; j += 64) {
+            oi[j] = ~0ull;
+            od[j] = inf;
+        }
+        if (lane == 0) out_count[r] = 0;
+        return;
+    }
+    if (!(f & ROW_TIE)) {
+        const float *cd = cand_d + (size_t)r * ldc;
+        const uint32_t *ci = cand_i + (size_t)r * ldc;
+        for (uint32_t j = lane; j < k; j += 64) {
+            oi[j] = j < m ? (uint64_t)ci[j] : ~0ull;
+            od[j] = j < m ? cd[j] : inf;
+        }
+        if (lane == 0) out_count[r] = m;
+        return;
+    }
+    WaveHeap h{0.0f, 0u, 0u, lane};
+    const float *row = D + (size_t)r * ldd;
+    bool stable = false;
+    float root = inf;
+    for (uint32_t base = 0; base < n; base += 64) {
+        const float v = base + lane < n ? row[base + lane] : 0.0f;
+        const uint32_t cnt = n - base < 64 ? n - base : 64;
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const float x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)j));
+            if (h.len == k && stable && x > root) continue;  // a no-op round trip
+            h.push(x, base + j);
+            if (h.len > k) {
+                float a;
+                uint32_t b;
+                h.pop(&a, &b);
+            }
+            if (h.len == k) {
+                root = h.gd(0);
+                // would a larger element's push + pop leave this layout as is?
+                WaveHeap t = h;
+                t.push(inf, 0xFFFFFFFFu);
+                float a;
+                uint32_t b;
+                t.pop(&a, &b);
+                const bool same = (uint32_t)lane >= k || (t.d == h.d && t.i == h.i);
+                stable = root < inf && __all(same);
+            }
+        }
+    }
+    const uint32_t cntk = h.len;
+    for (uint32_t p = cntk; p > 0; --p) {
+        float a;
+        uint32_t b;
+        h.pop(&a, &b);
+        if (lane == 0) {
+            oi[p - 1] = b;
+            od[p - 1] = a;
+        }
+    }
+    for (uint32_t j = cntk + lane; j < k; j += 64) {
+        oi[j] = ~0ull;
+        od[j] = inf;
+    }
+    if (lane == 0) out_count[r] = cntk;
+}
+
 template <bool LDS_HEAP>
 __global__ __launch_bounds__(64) void bf_finalize_kernel(
     const float *__restrict__ cand_d, const uint32_t *__restrict__ cand_i, uint32_t ldc,
@@ -262,8 +457,8 @@ __global__ __launch_bounds__(64) void bf_finalize_kernel(
         out_count[r] = 0;
         return;
     }
-    if ((f & ROW_NAN) && n >= 2) {
-        atomicOr(err, 1u);
+    if ((f & (ROW_NAN | ROW_BAD)) && n >= 2) {
+        atomicOr(err, (f & ROW_BAD) ? 4u : 1u);
         for (uint32_t j = 0; j < k; ++j) {
             oi[j] = ~0ull;
             od[j] = inf;
@@ -365,6 +560,12 @@ void launch_bf_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ld
 {
     if (rows == 0) return;
     ProfScope prof("bf_finalize", s);
+    if (k >= 1 && k + 1 <= 64) {
+        bf_finalize_wave_kernel<<<rows, 64, 0, s>>>(cand_d, cand_i, ldc, flags, D, ldd, n, k,
+                                                     out_idx, out_dist, out_count, err);
+        HM_CHECK_LAUNCH();
+        return;
+    }
     size_t lds = (size_t)64 * (k + 1) * 8;
     if (lds <= 64 * 1024) {
         bf_finalize_kernel<true><<<ceil_div(rows, 64), 64, lds, s>>>(

@@ -175,6 +175,13 @@ class _Index:
             ids = range(len(self._ids), len(self._ids) + rows.shape[0])
         self._add_rows(rows, list(ids))
 
+    def add_device(self, ptr: int, n_rows: int, ids: Optional[Sequence] = None) -> None:
+        """Append n_rows rows already in device memory (contiguous f32)."""
+        check(lib().hm_index_add_device(self._h, ptr, n_rows))
+        if ids is None:
+            ids = range(len(self._ids), len(self._ids) + n_rows)
+        self._ids.extend(ids)
+
     def _add_rows(self, rows, ids) -> None:
         rows = _f32_rows(rows, self._dim)
         check(lib().hm_index_add(self._h, rows.ctypes.data, rows.shape[0]))
@@ -224,6 +231,13 @@ class _Index:
         Node.new(item)  # validation (node.rs:158-166)
         idx, _, cnt = self.search_batch(item.reshape(1, -1), k)
         return [self._ids[int(i)] for i in idx[0, : int(cnt[0])]]
+
+    def search_stats(self) -> Tuple[int, int]:
+        """(rows rerun on the exact path or replayed, rows uncertified) of the
+        last search."""
+        a, b = C.c_uint64(), C.c_uint64()
+        check(lib().hm_index_search_stats(self._h, C.byref(a), C.byref(b)))
+        return int(a.value), int(b.value)
 
     def get_rows(self, positions) -> np.ndarray:
         pos = np.ascontiguousarray(positions, dtype=np.uint64)
@@ -285,11 +299,7 @@ class BruteForceIndex(_Index):
         check(lib().hm_index_autotune(self._h, q.ctypes.data, q.shape[0], k, C.byref(out)))
         return int(out.value)
 
-    def search_stats(self) -> Tuple[int, int]:
-        """(rows rerun on the exact path, rows uncertified) of the last search."""
-        a, b = C.c_uint64(), C.c_uint64()
-        check(lib().hm_index_search_stats(self._h, C.byref(a), C.byref(b)))
-        return int(a.value), int(b.value)
+
 
 
 class IVFPQIndex(_Index):

@@ -109,6 +109,10 @@ HM_API int32_t hm_index_destroy(hm_index *h);
  * Non-finite elements are rejected (node.rs:158-166) and nothing is added. */
 HM_API int32_t hm_index_add(hm_index *h, const float *rows, uint64_t n_rows);
 
+/* Same as hm_index_add with the rows already in device memory (n_rows x
+ * dimension floats, contiguous); rejected if any element is non-finite. */
+HM_API int32_t hm_index_add_device(hm_index *h, const float *d_rows, uint64_t n_rows);
+
 /* ANNIndex::build (ann_index.rs:30; bruteforce_idx.rs:36-39; pq_idx.rs:443-447). */
 HM_API int32_t hm_index_build(hm_index *h, int32_t metric);
 

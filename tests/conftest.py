@@ -3,6 +3,10 @@ import sys
 
 import pytest
 
+# torch (used by the device-pointer tests) ships its own HIP runtime; load it
+# before libhora_mi355x.so so that one runtime serves the whole process.
+import torch  # noqa: F401
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
