@@ -155,8 +155,7 @@ class BruteForce final : public Index {
         err_.reserve(sizeof(uint32_t));
         HM_HIP(hipMemsetAsync(err_.get(), 0, sizeof(uint32_t), s));
         launch_check_finite(dq, dim_, nq, (uint32_t)dim_, err_.as<uint32_t>(), s);
-        const bool shortlist_form = mode_ != HM_SEARCH_EXACT && metric_ != HM_METRIC_MANHATTAN &&
-                                    dpad_ <= 4096;
+        const bool shortlist_form = mode_ != HM_SEARCH_EXACT && shortlist_metric() && dpad_ <= 4096;
         last_fallback_ = last_uncertified_ = 0;
         if (shortlist_form)
             search_shortlist(dq, nq, k, d_idx, d_dist, d_count, s);
@@ -171,6 +170,17 @@ class BruteForce final : public Index {
     }
 
   private:
+    // Metrics with an MFMA shortlist form.  Manhattan has no GEMM form.
+    // Angular's shortlist form is switched off: its score epilogue faulted
+    // on the device (a store through an unset address register on that
+    // branch arm), and the rebuilt kernel has not yet run there; Angular
+    // requests in a shortlist mode run the exact form.
+    bool shortlist_metric() const
+    {
+        return metric_ == HM_METRIC_EUCLIDEAN || metric_ == HM_METRIC_DOT_PRODUCT ||
+               metric_ == HM_METRIC_COSINE_SIMILARITY;
+    }
+
     bool need_norm() const
     {
         return metric_ == HM_METRIC_COSINE_SIMILARITY || metric_ == HM_METRIC_ANGULAR;
@@ -345,12 +355,12 @@ class BruteForce final : public Index {
         HM_HIP(hipMemsetAsync(stat_.get(), 0, 2 * sizeof(uint32_t), s));
         launch_norm_stats(bnorm_.as<float>(), n, inv_bnorm_.as<float>(), stat_.as<uint32_t>(), s);
         if (mode_ == HM_SEARCH_BF16_RERANK) {
-            dk_ = (uint32_t)round_up(dim_, 32);
+            dk_ = (uint32_t)round_up(dim_, 64);  // 128-byte rows
             blow_.reserve(n_ * dk_ * 2);
             launch_to_bf16(base_.as<float>(), dpad_, n_, (uint32_t)dim_, dk_, blow_.as<uint16_t>(),
                            stat_.as<uint32_t>() + 1, s);
         } else {
-            dk_ = (uint32_t)round_up(dim_, 64);
+            dk_ = (uint32_t)round_up(dim_, 128);  // 128-byte rows
             blow_.reserve(n_ * dk_);
             bscale_.reserve(n_ * sizeof(float));
             launch_to_fp8(base_.as<float>(), dpad_, n, (uint32_t)dim_, dk_, blow_.as<uint8_t>(),
@@ -482,7 +492,7 @@ class BruteForce final : public Index {
         }
         off[0] = 0;
         for (uint64_t q = 0; q < Q; ++q) off[q + 1] = off[q] + kq[q];
-        const bool shortlist_form = mode_ != HM_SEARCH_EXACT && metric_ != HM_METRIC_MANHATTAN &&
+        const bool shortlist_form = mode_ != HM_SEARCH_EXACT && shortlist_metric() &&
                                     dpad_ <= 4096 && n_ > 0 && kmax > 0;
         rg_kq_.reserve(Q * 4);
         rg_off_.reserve((Q + 1) * 8);
@@ -553,8 +563,7 @@ class BruteForce final : public Index {
     // sample (k fixed), trying 64..4096; returns the choice.
     uint32_t autotune(const float *h_queries, uint64_t nq, uint32_t k) override
     {
-        if (mode_ == HM_SEARCH_EXACT || metric_ == HM_METRIC_MANHATTAN || n_ == 0 || nq == 0)
-            return 0;
+        if (mode_ == HM_SEARCH_EXACT || !shortlist_metric() || n_ == 0 || nq == 0) return 0;
         hipStream_t s = stream_;
         DeviceBuffer q, oi, od, oc;
         q.reserve(nq * dim_ * 4);

@@ -25,14 +25,21 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 
 constexpr int GM = 128, GN = 128;
-constexpr int ROWB = 64;            // bytes per row per K step
+constexpr int ROWB = 128;           // bytes per row per K step (64 bf16 / 128 fp8)
 constexpr int LDS_ROW = ROWB + 16;  // padded LDS row stride (bytes)
+constexpr int CPT = GM * ROWB / 16 / 256;  // 16-byte chunks per thread per operand (4)
 
-template <int ELEM>  // 2 = bf16, 1 = fp8
+// Software pipeline: the global loads of step kt+2 are issued before the
+// MFMAs of step kt into one of two register sets, and stored to LDS one
+// step later, so each load has two steps of matrix work to land.
+// The metric is a template parameter: the epilogue has no run-time branch
+// on it (a uniform branch chain there compiled to a store whose address
+// register was left unset on the last arm).
+template <int ELEM, int MET>  // ELEM: 2 = bf16, 1 = fp8
 __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
     const uint8_t *__restrict__ Q, uint32_t nq, const uint8_t *__restrict__ B, uint32_t n,
     uint32_t row_bytes, const float *__restrict__ qscale, const float *__restrict__ bscale,
-    const float *__restrict__ bnorm2, const float *__restrict__ inv_bnorm, int metric,
+    const float *__restrict__ bnorm2, const float *__restrict__ inv_bnorm,
     float *__restrict__ Kmat, uint64_t ldk, const uint32_t *__restrict__ dev_rows)
 {
     if (dev_rows && *dev_rows < nq) nq = *dev_rows;
@@ -50,84 +57,107 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    // Register staging of the next K step: two 16-byte chunks per operand.
-    // Rows past the end load from a clamped (valid) row and are zeroed
-    // afterwards, so every load is a plain global load of a real row.
-    uint4 ra0, ra1, rb0, rb1;
-    const int row0 = tid >> 2, row1 = (tid + 256) >> 2, cc = (tid & 3) * 16;
-    const bool qa0 = q0 + row0 < nq, qa1 = q0 + row1 < nq;
-    const bool ba0 = n0 + row0 < n, ba1 = n0 + row1 < n;
-    const uint8_t *qp0 = Q + (size_t)(qa0 ? q0 + row0 : nq - 1) * row_bytes + cc;
-    const uint8_t *qp1 = Q + (size_t)(qa1 ? q0 + row1 : nq - 1) * row_bytes + cc;
-    const uint8_t *bp0 = B + (size_t)(ba0 ? n0 + row0 : n - 1) * row_bytes + cc;
-    const uint8_t *bp1 = B + (size_t)(ba1 ? n0 + row1 : n - 1) * row_bytes + cc;
-    const uint32_t mq0 = qa0 ? ~0u : 0u, mq1 = qa1 ? ~0u : 0u;
-    const uint32_t mb0 = ba0 ? ~0u : 0u, mb1 = ba1 ? ~0u : 0u;
+    // Per thread: CPT chunks of each operand per step.  Rows past the end
+    // load from a clamped (valid) row and are zeroed by a mask.
+    const uint8_t *qp[CPT], *bp[CPT];
+    uint32_t mq[CPT], mb[CPT];
+    int soff[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const int id = tid + 256 * c, row = id >> 3, cc = (id & 7) * 16;
+        const bool qa = q0 + row < nq, ba = n0 + row < n;
+        qp[c] = Q + (size_t)(qa ? q0 + row : nq - 1) * row_bytes + cc;
+        bp[c] = B + (size_t)(ba ? n0 + row : n - 1) * row_bytes + cc;
+        mq[c] = qa ? ~0u : 0u;
+        mb[c] = ba ? ~0u : 0u;
+        soff[c] = row * LDS_ROW + cc;
+    }
+    uint4 ra[2][CPT], rb[2][CPT];
 #define HM_MASK4(v, m) make_uint4((v).x & (m), (v).y & (m), (v).z & (m), (v).w & (m))
-#define HM_GLOAD(kb)                                                                   \
-    do {                                                                               \
-        const uint4 t0 = *reinterpret_cast<const uint4 *>(qp0 + (kb));                 \
-        const uint4 t1 = *reinterpret_cast<const uint4 *>(qp1 + (kb));                 \
-        const uint4 t2 = *reinterpret_cast<const uint4 *>(bp0 + (kb));                 \
-        const uint4 t3 = *reinterpret_cast<const uint4 *>(bp1 + (kb));                 \
-        ra0 = HM_MASK4(t0, mq0);                                                       \
-        ra1 = HM_MASK4(t1, mq1);                                                       \
-        rb0 = HM_MASK4(t2, mb0);                                                       \
-        rb1 = HM_MASK4(t3, mb1);                                                       \
-    } while (0)
-#define HM_SSTORE(buf)                                                           \
-    do {                                                                         \
-        *reinterpret_cast<uint4 *>(&sA[buf][row0 * LDS_ROW + cc]) = ra0;         \
-        *reinterpret_cast<uint4 *>(&sA[buf][row1 * LDS_ROW + cc]) = ra1;         \
-        *reinterpret_cast<uint4 *>(&sB[buf][row0 * LDS_ROW + cc]) = rb0;         \
-        *reinterpret_cast<uint4 *>(&sB[buf][row1 * LDS_ROW + cc]) = rb1;         \
-    } while (0)
+#define HM_GLOAD(set, kb)                                                              \
+    _Pragma("unroll") for (int c = 0; c < CPT; ++c)                                    \
+    {                                                                                  \
+        ra[set][c] = *reinterpret_cast<const uint4 *>(qp[c] + (kb));                   \
+        rb[set][c] = *reinterpret_cast<const uint4 *>(bp[c] + (kb));                   \
+    }
+#define HM_SSTORE(set, buf)                                                            \
+    _Pragma("unroll") for (int c = 0; c < CPT; ++c)                                    \
+    {                                                                                  \
+        *reinterpret_cast<uint4 *>(&sA[buf][soff[c]]) = HM_MASK4(ra[set][c], mq[c]);   \
+        *reinterpret_cast<uint4 *>(&sB[buf][soff[c]]) = HM_MASK4(rb[set][c], mb[c]);   \
+    }
 
     const uint32_t nk = row_bytes / ROWB;
-    HM_GLOAD(0u);
-    HM_SSTORE(0);
-    __syncthreads();
     const int lr = lane & 15, lk = lane >> 4;
-    for (uint32_t kt = 0; kt < nk; ++kt) {
-        const int cur = kt & 1;
-        if (kt + 1 < nk) HM_GLOAD((kt + 1) * ROWB);
-        const uint8_t *As = sA[cur];
-        const uint8_t *Bs = sB[cur];
+    // One K step on LDS buffer `buf`: MFMAs over its 128-byte slab.
+    auto mma_step = [&](const uint8_t *As, const uint8_t *Bs) {
         if constexpr (ELEM == 2) {
-            // one MFMA K=32 (64 bytes) per step; lane holds 8 bf16 at k = 8*lk
-            bf16x8 a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                a[i] = *reinterpret_cast<const bf16x8 *>(As + (wr * 64 + i * 16 + lr) * LDS_ROW + lk * 16);
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                b[j] = *reinterpret_cast<const bf16x8 *>(Bs + (wc * 64 + j * 16 + lr) * LDS_ROW + lk * 16);
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-        } else {
-            // two MFMA K=32 (32 bytes each) per step; lane holds 8 fp8 at k = 8*lk
+            // two MFMA K=32 per step; lane holds 8 bf16 at k = 32*kk + 8*lk
 #pragma unroll
             for (int kk = 0; kk < 2; ++kk) {
-                long a[4], b[4];
+                bf16x8 a[4], b[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    a[i] = *reinterpret_cast<const long *>(As + (wr * 64 + i * 16 + lr) * LDS_ROW + kk * 32 + lk * 8);
+                    a[i] = *reinterpret_cast<const bf16x8 *>(
+                        As + (wr * 64 + i * 16 + lr) * LDS_ROW + kk * 64 + lk * 16);
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    b[j] = *reinterpret_cast<const long *>(Bs + (wc * 64 + j * 16 + lr) * LDS_ROW + kk * 32 + lk * 8);
+                    b[j] = *reinterpret_cast<const bf16x8 *>(
+                        Bs + (wc * 64 + j * 16 + lr) * LDS_ROW + kk * 64 + lk * 16);
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a[i], b[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j],
+                                                                            0, 0, 0);
+            }
+        } else {
+            // four MFMA K=32 per step; lane holds 8 fp8 at k = 32*kk + 8*lk
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                long a[4], b[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    a[i] = *reinterpret_cast<const long *>(
+                        As + (wr * 64 + i * 16 + lr) * LDS_ROW + kk * 32 + lk * 8);
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    b[j] = *reinterpret_cast<const long *>(
+                        Bs + (wc * 64 + j * 16 + lr) * LDS_ROW + kk * 32 + lk * 8);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a[i], b[j],
+                                                                               acc[i][j], 0, 0, 0);
             }
         }
-        if (kt + 1 < nk) HM_SSTORE(cur ^ 1);
+    };
+    // Step kt computes buffer kt&1 while register set kt&1 (free since its
+    // contents went to LDS one step earlier) loads step kt+2, then register
+    // set (kt+1)&1, loaded one step earlier, goes to buffer (kt+1)&1.  The
+    // loop is unrolled by two so that both sets are named statically; loads
+    // past the last step re-read the last step (clamped, unused) so that
+    // every step has the same shape and the waits are exact.
+    const uint32_t last = (nk - 1) * ROWB;
+    HM_GLOAD(0, 0u);
+    HM_GLOAD(1, min((uint32_t)ROWB, last));
+    HM_SSTORE(0, 0);
+    __syncthreads();
+    uint32_t kt = 0;
+    for (; kt + 1 < nk; kt += 2) {
+        HM_GLOAD(0, min((kt + 2) * ROWB, last));
+        __builtin_amdgcn_sched_barrier(0);  // issue the loads before the MFMAs
+        mma_step(sA[0], sB[0]);
+        HM_SSTORE(1, 1);
+        __syncthreads();
+        HM_GLOAD(1, min((kt + 3) * ROWB, last));
+        __builtin_amdgcn_sched_barrier(0);
+        mma_step(sA[1], sB[1]);
+        HM_SSTORE(0, 0);
         __syncthreads();
     }
+    if (kt < nk) mma_step(sA[0], sB[0]);  // odd step count: the last step is in buffer 0
 #undef HM_GLOAD
 #undef HM_SSTORE
 #undef HM_MASK4
@@ -148,9 +178,9 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
                 float s = acc[i][j][e];
                 if (ELEM == 1) s = s * (qs * bscale[c]);
                 float key;
-                if (metric == HM_METRIC_EUCLIDEAN) key = bnorm2[c] - 2.0f * s;
-                else if (metric == HM_METRIC_DOT_PRODUCT) key = s;
-                else if (metric == HM_METRIC_COSINE_SIMILARITY) key = -s * inv_bnorm[c];
+                if constexpr (MET == HM_METRIC_EUCLIDEAN) key = bnorm2[c] - 2.0f * s;
+                else if constexpr (MET == HM_METRIC_DOT_PRODUCT) key = s;
+                else if constexpr (MET == HM_METRIC_COSINE_SIMILARITY) key = -s * inv_bnorm[c];
                 else key = s * inv_bnorm[c];
                 krow[c] = key;
             }
@@ -228,17 +258,29 @@ void launch_score_gemm(int elem_bytes, const void *Q, uint32_t nq, const void *B
                        uint64_t ldk, hipStream_t s, const uint32_t *dev_rows)
 {
     if (nq == 0 || n == 0) return;
-    if (row_bytes % ROWB) fail(HM_ERR_INVALID_ARG, "score_gemm: row bytes not a multiple of 64");
+    if (row_bytes % ROWB) fail(HM_ERR_INVALID_ARG, "score_gemm: row bytes not a multiple of 128");
     dim3 grid(ceil_div(n, GN), ceil_div(nq, GM));
+    if (elem_bytes != 1 && elem_bytes != 2) fail(HM_ERR_INVALID_ARG, "score_gemm: element size");
     ProfScope prof("score_gemm", s);
-    if (elem_bytes == 2)
-        score_gemm_kernel<2><<<grid, 256, 0, s>>>((const uint8_t *)Q, nq, (const uint8_t *)B, n,
-                                                  row_bytes, qscale, bscale, bnorm2, inv_bnorm,
-                                                  metric, Kmat, ldk, dev_rows);
-    else
-        score_gemm_kernel<1><<<grid, 256, 0, s>>>((const uint8_t *)Q, nq, (const uint8_t *)B, n,
-                                                  row_bytes, qscale, bscale, bnorm2, inv_bnorm,
-                                                  metric, Kmat, ldk, dev_rows);
+#define HM_SG(E, M)                                                                              \
+    score_gemm_kernel<E, M><<<grid, 256, 0, s>>>((const uint8_t *)Q, nq, (const uint8_t *)B, n, \
+                                                 row_bytes, qscale, bscale, bnorm2, inv_bnorm,  \
+                                                 Kmat, ldk, dev_rows)
+#define HM_SG_METRIC(E)                                                                         \
+    switch (metric) {                                                                           \
+    case HM_METRIC_EUCLIDEAN: HM_SG(E, HM_METRIC_EUCLIDEAN); break;                             \
+    case HM_METRIC_DOT_PRODUCT: HM_SG(E, HM_METRIC_DOT_PRODUCT); break;                         \
+    case HM_METRIC_COSINE_SIMILARITY: HM_SG(E, HM_METRIC_COSINE_SIMILARITY); break;             \
+    case HM_METRIC_ANGULAR: HM_SG(E, HM_METRIC_ANGULAR); break;                                 \
+    default: fail(HM_ERR_INVALID_ARG, "score_gemm: metric has no shortlist form");              \
+    }
+    if (elem_bytes == 2) {
+        HM_SG_METRIC(2)
+    } else {
+        HM_SG_METRIC(1)
+    }
+#undef HM_SG_METRIC
+#undef HM_SG
     HM_CHECK_LAUNCH();
 }
 

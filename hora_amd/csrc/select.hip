@@ -396,30 +396,60 @@ __global__ __launch_bounds__(64) void bf_finalize_wave_kernel(
     const float *row = D + (size_t)r * ldd;
     bool stable = false;
     float root = inf;
-    for (uint32_t base = 0; base < n; base += 64) {
-        const float v = base + lane < n ? row[base + lane] : 0.0f;
-        const uint32_t cnt = n - base < 64 ? n - base : 64;
-        for (uint32_t j = 0; j < cnt; ++j) {
-            const float x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)j));
-            if (h.len == k && stable && x > root) continue;  // a no-op round trip
-            h.push(x, base + j);
-            if (h.len > k) {
-                float a;
-                uint32_t b;
-                h.pop(&a, &b);
-            }
-            if (h.len == k) {
-                root = h.gd(0);
-                // would a larger element's push + pop leave this layout as is?
-                WaveHeap t = h;
-                t.push(inf, 0xFFFFFFFFu);
-                float a;
-                uint32_t b;
-                t.pop(&a, &b);
-                const bool same = (uint32_t)lane >= k || (t.d == h.d && t.i == h.i);
-                stable = root < inf && __all(same);
+    // The row streams through LDS in batches of FB x 64 values; the next
+    // batch's loads are in flight while the current one is replayed.  While
+    // the heap is full and its layout is stable, a ballot finds the next
+    // element that is not greater than the root (the only ones that change
+    // anything), so the rejected majority costs one ballot per 64 values.
+    constexpr uint32_t FB = 16;
+    __shared__ float sv[FB * 64];
+    float nx[FB];
+#pragma unroll
+    for (uint32_t c = 0; c < FB; ++c) nx[c] = row[min((uint32_t)(c * 64 + lane), n - 1)];
+    for (uint32_t b0 = 0; b0 < n; b0 += FB * 64) {
+#pragma unroll
+        for (uint32_t c = 0; c < FB; ++c) sv[c * 64 + lane] = nx[c];
+        __syncthreads();
+        const uint32_t b1 = b0 + FB * 64;
+        if (b1 < n) {
+#pragma unroll
+            for (uint32_t c = 0; c < FB; ++c) nx[c] = row[min(b1 + c * 64 + lane, n - 1)];
+        }
+        for (uint32_t c = 0; c < FB; ++c) {
+            const uint32_t base = b0 + c * 64;
+            if (base >= n) break;
+            const float v = sv[c * 64 + lane];
+            const uint32_t cnt = n - base < 64 ? n - base : 64;
+            uint32_t j = 0;
+            while (j < cnt) {
+                if (h.len == k && stable) {
+                    const unsigned long long live =
+                        __ballot((uint32_t)lane >= j && (uint32_t)lane < cnt && !(v > root));
+                    if (live == 0) break;
+                    j = (uint32_t)__builtin_ctzll(live);
+                }
+                const float x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)j));
+                h.push(x, base + j);
+                if (h.len > k) {
+                    float a;
+                    uint32_t b;
+                    h.pop(&a, &b);
+                }
+                if (h.len == k) {
+                    root = h.gd(0);
+                    // would a larger element's push + pop leave this layout as is?
+                    WaveHeap t = h;
+                    t.push(inf, 0xFFFFFFFFu);
+                    float a;
+                    uint32_t b;
+                    t.pop(&a, &b);
+                    const bool same = (uint32_t)lane >= k || (t.d == h.d && t.i == h.i);
+                    stable = root < inf && __all(same);
+                }
+                ++j;
             }
         }
+        __syncthreads();
     }
     const uint32_t cntk = h.len;
     for (uint32_t p = cntk; p > 0; --p) {

@@ -14,7 +14,15 @@ from hora_amd.datasets import bf_normal, bf_sparse_u8, unit_normal
 
 pytestmark = pytest.mark.gpu
 
-GEMM_METRICS = [Metric.Euclidean, Metric.DotProduct, Metric.CosineSimilarity, Metric.Angular]
+# Angular's shortlist form faulted the device twice (a store through an
+# address register left unset on the Angular arm of the score epilogue; the
+# metric is now a template parameter).  The rebuilt kernel has not run on a
+# device yet, so the form stays switched off (Angular runs the exact form)
+# and its cases are skipped until it is verified.
+_ANGULAR_SKIP = pytest.mark.skip(reason="Angular shortlist form switched off: device fault in "
+                                 "its score epilogue, fixed but not yet re-verified")
+GEMM_METRICS = [Metric.Euclidean, Metric.DotProduct, Metric.CosineSimilarity,
+                pytest.param(Metric.Angular, marks=_ANGULAR_SKIP)]
 
 
 def _index(base, mt, mode, shortlist=0):
@@ -34,7 +42,7 @@ def _exact_check(orc, idx, base, q, k, mt):
     return idx.search_stats()
 
 
-@pytest.mark.parametrize("mt", GEMM_METRICS, ids=lambda m: m.name)
+@pytest.mark.parametrize("mt", GEMM_METRICS, ids=lambda m: Metric(m).name)
 def test_bf16_exact_on_u8_data(orc, mt):
     # integers 0..255 are exact in bf16: certification is tight
     base = bf_sparse_u8(20_000, 784, seed=401)
@@ -44,7 +52,7 @@ def test_bf16_exact_on_u8_data(orc, mt):
     assert uncert <= 256 // 20, (fallback, uncert)
 
 
-@pytest.mark.parametrize("mt", GEMM_METRICS, ids=lambda m: m.name)
+@pytest.mark.parametrize("mt", GEMM_METRICS, ids=lambda m: Metric(m).name)
 def test_bf16_exact_on_float_data(orc, mt):
     base = bf_normal(10_000, 64, seed=411)
     q = bf_normal(300, 64, seed=412)
