@@ -127,7 +127,8 @@ def main_bf(args):
     for form in args.forms.split(","):
         ix = make(Metric.Euclidean, form)
         wall, ev = timer.run(step_fn(ix), args.steps, args.warmup, f"Euclidean/{form}")
-        km = kernel_ms(["exact_tile", "row_select", "bf_finalize", "score_gemm", "rerank"])
+        km = kernel_ms(["exact_tile", "row_select", "bf_finalize", "score_gemm", "score_filter",
+                        "list_prepare", "rerank"])
         fb = ix.search_stats()
         ids_by_form[form] = d_idx.cpu().numpy().view(np.uint64).copy()
         forms[form] = {"qps": round(args.steps * NQ / wall, 1),
@@ -175,16 +176,20 @@ def main_bf(args):
                     "achieved": round(ach, 3), "peak": FP32_PEAK, "unit": "TFLOP/s",
                     "frac": round(ach / FP32_PEAK, 4), "traffic": None,
                     "avg_launch_ms": round(avg * 1e3, 3)}
-    elif km["score_gemm"][1]:
-        avg = km["score_gemm"][0] / km["score_gemm"][1] / 1e3
-        dk = (D + 31) // 32 * 32 if best == "bf16" else (D + 63) // 64 * 64
-        work = 2.0 * (NQ * args.steps / km["score_gemm"][1]) * N * dk
-        peak = BF16_PEAK if best == "bf16" else FP8_PEAK
-        ach = work / avg / 1e12
-        roofline = {"bound": "mfma", "kernel": f"score_gemm_kernel<{best}>",
-                    "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
-                    "frac": round(ach / peak, 4), "traffic": None,
-                    "avg_launch_ms": round(avg * 1e3, 3)}
+    else:
+        # the dominant MFMA kernel: the filtered full scan when the filter
+        # form ran, else the full key-matrix GEMM; algorithmic work is
+        # 2 * queries * rows * D (the unpadded dimension)
+        kname = "score_filter" if km["score_filter"][1] else "score_gemm"
+        if km[kname][1]:
+            avg = km[kname][0] / km[kname][1] / 1e3
+            work = 2.0 * (NQ * args.steps / km[kname][1]) * N * D
+            peak = BF16_PEAK if best == "bf16" else FP8_PEAK
+            ach = work / avg / 1e12
+            roofline = {"bound": "mfma", "kernel": f"{kname}_kernel<{best}>",
+                        "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
+                        "frac": round(ach / peak, 4), "traffic": None,
+                        "avg_launch_ms": round(avg * 1e3, 3)}
 
     per_metric = {"Euclidean": b["qps"]}
     if not args.no_metrics:
@@ -335,7 +340,8 @@ def main_ragged(args):
             wall, _ = timer.run(lambda: ix.search_ragged_device(
                 xq.data_ptr(), off, ks, d_idx.data_ptr(), d_dist.data_ptr(), d_cnt.data_ptr(),
                 stream.cuda_stream), args.steps, args.warmup, f"{mt.name}/{form}")
-            km = kernel_ms(["score_gemm", "row_select", "rerank", "exact_tile"])
+            km = kernel_ms(["score_gemm", "score_filter", "list_prepare", "row_select", "rerank",
+                            "exact_tile"])
             fb = ix.search_stats()
             out[f"{mt.name}/{form}"] = {
                 "qps": round(args.steps * Q / wall, 1),

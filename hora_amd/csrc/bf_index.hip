@@ -78,6 +78,7 @@ class BruteForce final : public Index {
         n_ = need;
         bnorm_valid_ = false;
         sl_valid_ = false;
+        samp_ns_ = 0;
     }
 
     void add_device(const float *d_rows, uint64_t n_rows, uint64_t ld) override
@@ -95,6 +96,7 @@ class BruteForce final : public Index {
         n_ = need;
         bnorm_valid_ = false;
         sl_valid_ = false;
+        samp_ns_ = 0;
     }
 
     void build(int32_t metric) override
@@ -116,6 +118,7 @@ class BruteForce final : public Index {
         n_ = cap_ = 0;
         bnorm_valid_ = false;
         sl_valid_ = false;
+        samp_ns_ = 0;
     }
 
     void get_rows(const uint64_t *idx, uint64_t n, float *out) const override
@@ -133,7 +136,10 @@ class BruteForce final : public Index {
         if (mode < HM_SEARCH_EXACT || mode > HM_SEARCH_FP8_RERANK)
             fail(HM_ERR_INVALID_ARG, "unknown search mode");
         if (shortlist > SELECT_MAX_K) fail(HM_ERR_INVALID_ARG, "shortlist above 8192");
-        if (mode != mode_) sl_valid_ = false;
+        if (mode != mode_) {
+            sl_valid_ = false;
+            samp_ns_ = 0;
+        }
         mode_ = mode;
         shortlist_ = shortlist;
     }
@@ -242,18 +248,22 @@ class BruteForce final : public Index {
 
     void capture_ragged_graph(uint32_t QC, uint32_t kmax, uint32_t S)
     {
+        const FilterPlan fp = filter_plan(S);
+        if (fp.ns) ensure_sample(fp.ns, stream_);
         if (rg_exec_ && rg_qc_ == QC && rg_kmax_ == kmax && rg_s_ == S && rg_mode_ == mode_ &&
-            rg_metric_ == metric_ && rg_n_ == n_)
+            rg_metric_ == metric_ && rg_n_ == n_ && rg_ns_ == fp.ns)
             return;
         if (rg_exec_) {
             (void)hipGraphExecDestroy(rg_exec_);
             rg_exec_ = nullptr;
         }
+        HM_HIP(hipStreamSynchronize(stream_));  // sample data ready before capture
         if (!rg_stream_) HM_HIP(hipStreamCreateWithFlags(&rg_stream_, hipStreamNonBlocking));
         const uint32_t n = (uint32_t)n_;
         const bool bf16 = mode_ == HM_SEARCH_BF16_RERANK;
+        const bool filter = fp.ns != 0;
         const uint32_t row_bytes = bf16 ? dk_ * 2 : dk_;
-        const uint64_t ldk = round_up(n, 4);
+        const uint64_t ldk = round_up(filter ? fp.ns : n, 4);
         rg_rows_.reserve(4);
         rg_q_.reserve((size_t)QC * dim_ * 4);
         rg_ck_.reserve((size_t)QC * 4);
@@ -271,6 +281,15 @@ class BruteForce final : public Index {
         rg_sf_.reserve((size_t)QC * 4);
         rg_rf_.reserve((size_t)QC * 4);
         rg_qinex_.reserve(4);
+        if (filter) {
+            rg_seld_.reserve((size_t)QC * fp.m * 4);
+            rg_seli_.reserve((size_t)QC * fp.m * 4);
+            rg_list_.reserve((size_t)QC * fp.cap * 8);
+            rg_lcnt_.reserve((size_t)QC * 4);
+            rg_sq_.reserve((size_t)QC * 4);
+            rg_ksq_.reserve((size_t)QC * 4);
+            rg_pref_.reserve((size_t)QC * 4);
+        }
         const float u = 5.9604644775390625e-8f;
         const float gamma = 2.0f * (float)(std::max<uint32_t>(dk_, dpad_) + 8) * u;
         const float c_bf16 = 1.0f / 128.0f + 1.0f / 65536.0f;
@@ -288,17 +307,40 @@ class BruteForce final : public Index {
         else
             launch_to_fp8(rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_, dk_, rg_qlow_.as<uint8_t>(),
                           rg_qscale_.as<float>(), g, dr);
-        launch_score_gemm(bf16 ? 2 : 1, rg_qlow_.get(), QC, blow_.get(), n, row_bytes,
-                          rg_qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
-                          inv_bnorm_.as<float>(), metric_, rg_D_.as<float>(), ldk, g, dr);
-        launch_row_select(rg_D_.as<float>(), ldk, QC, n, S, rg_cd_.as<float>(),
-                          rg_ci_.as<uint32_t>(), S, rg_sf_.as<uint32_t>(), g, dr);
+        const uint32_t *s_q = nullptr, *pre = nullptr;
+        const float *key_s = nullptr;
+        if (filter) {
+            launch_score_gemm(bf16 ? 2 : 1, rg_qlow_.get(), QC, bsamp_.get(), fp.ns, row_bytes,
+                              rg_qscale_.as<float>(), bs_scale_.as<float>(), bs_norm2_.as<float>(),
+                              bs_inv_.as<float>(), metric_, rg_D_.as<float>(), ldk, g, dr);
+            launch_row_select(rg_D_.as<float>(), ldk, QC, fp.ns, fp.m, rg_seld_.as<float>(),
+                              rg_seli_.as<uint32_t>(), fp.m, rg_sf_.as<uint32_t>(), g, dr);
+            launch_fill_u32(rg_lcnt_.as<uint32_t>(), QC, 0, g);
+            const float *tau = rg_seld_.as<float>() + (fp.m - 1);
+            launch_score_filter(bf16 ? 2 : 1, rg_qlow_.get(), QC, blow_.get(), n, row_bytes,
+                                rg_qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
+                                inv_bnorm_.as<float>(), metric_, tau, fp.m, rg_list_.as<uint64_t>(),
+                                fp.cap, rg_lcnt_.as<uint32_t>(), g, dr);
+            launch_list_prepare(rg_list_.as<uint64_t>(), fp.cap, rg_lcnt_.as<uint32_t>(), tau, fp.m,
+                                QC, S, rg_ci_.as<uint32_t>(), rg_cd_.as<float>(), S,
+                                rg_sq_.as<uint32_t>(), rg_ksq_.as<float>(), rg_pref_.as<uint32_t>(),
+                                g, dr);
+            s_q = rg_sq_.as<uint32_t>();
+            key_s = rg_ksq_.as<float>();
+            pre = rg_pref_.as<uint32_t>();
+        } else {
+            launch_score_gemm(bf16 ? 2 : 1, rg_qlow_.get(), QC, blow_.get(), n, row_bytes,
+                              rg_qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
+                              inv_bnorm_.as<float>(), metric_, rg_D_.as<float>(), ldk, g, dr);
+            launch_row_select(rg_D_.as<float>(), ldk, QC, n, S, rg_cd_.as<float>(),
+                              rg_ci_.as<uint32_t>(), S, rg_sf_.as<uint32_t>(), g, dr);
+        }
         launch_rerank(metric_, rg_qpad_.as<float>(), dpad_, base_.as<float>(), rg_ci_.as<uint32_t>(),
                       rg_cd_.as<float>(), S, S, rg_qnorm_.as<float>(), bnorm_.as<float>(), QC, n,
                       kmax, bmax_, gamma, c_exact, c_bf16, rg_qinex_.as<uint32_t>(),
                       bf16 ? certify_ : 0, rg_oi_.as<uint64_t>(), rg_od_.as<float>(),
                       rg_oc_.as<uint32_t>(), rg_rf_.as<uint32_t>(), g, dr, rg_ck_.as<uint32_t>(),
-                      rg_coff_.as<uint64_t>());
+                      rg_coff_.as<uint64_t>(), s_q, key_s, pre);
         hipGraph_t graph = nullptr;
         HM_HIP(hipStreamEndCapture(g, &graph));
         hipError_t e = hipGraphInstantiate(&rg_exec_, graph, nullptr, nullptr, 0);
@@ -310,6 +352,7 @@ class BruteForce final : public Index {
         rg_mode_ = mode_;
         rg_metric_ = metric_;
         rg_n_ = n_;
+        rg_ns_ = fp.ns;
     }
 
     // Exact path for the listed queries with kmax, packed into the ragged
@@ -384,6 +427,56 @@ class BruteForce final : public Index {
         return (uint32_t)std::min<uint64_t>(S, n_);
     }
 
+    // Filter form of the shortlist: the S-deep shortlist is cut from a
+    // candidate list of every row whose key is below a per-query threshold
+    // tau, so no [queries x n] key matrix is written or scanned.  tau is the
+    // m-th smallest key over a strided sample of ns base rows, sized so that
+    // about 2S rows fall below it (m ~ 8 sample rows, a relative spread of
+    // about 1/sqrt(8) in the list length).  A list shorter than k+1 or
+    // longer than its capacity cannot be certified and its query reruns on
+    // the exact path, so the estimate affects speed only.
+    struct FilterPlan {
+        uint32_t ns = 0, m = 0, cap = 0;
+    };
+    FilterPlan filter_plan(uint32_t S) const
+    {
+        FilterPlan p;
+        if (!use_filter_ || n_ == 0) return p;
+        const uint64_t T = 2ull * S;
+        uint64_t ns = round_up((8 * n_ + T - 1) / T, 128);
+        ns = std::max<uint64_t>(ns, 1024);
+        if (ns * 4 > n_) return p;  // a sample this large does not pay
+        p.ns = (uint32_t)ns;
+        p.m = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((T * ns + n_ - 1) / n_, 1), ns);
+        p.cap = (uint32_t)std::min<uint64_t>(8192, std::max<uint64_t>(1024, round_up(8ull * S, 256)));
+        return p;
+    }
+
+    void ensure_sample(uint32_t ns, hipStream_t s)
+    {
+        if (samp_ns_ == ns) return;
+        const uint32_t n = (uint32_t)n_;
+        const uint32_t row_bytes = mode_ == HM_SEARCH_BF16_RERANK ? dk_ * 2 : dk_;
+        bsamp_.reserve((size_t)ns * row_bytes);
+        bs_norm2_.reserve((size_t)ns * 4);
+        bs_inv_.reserve((size_t)ns * 4);
+        launch_sample_rows(blow_.get(), row_bytes, n, ns, bsamp_.get(), s);
+        launch_sample_f32(bnorm_.as<float>(), n, ns, bs_norm2_.as<float>(), s);
+        launch_sample_f32(inv_bnorm_.as<float>(), n, ns, bs_inv_.as<float>(), s);
+        if (mode_ == HM_SEARCH_FP8_RERANK) {
+            bs_scale_.reserve((size_t)ns * 4);
+            launch_sample_f32(bscale_.as<float>(), n, ns, bs_scale_.as<float>(), s);
+        }
+        samp_ns_ = ns;
+    }
+
+    // Largest query chunk whose score grid fits the launch limit.
+    uint64_t grid_rows_limit(uint64_t ncols) const
+    {
+        const uint64_t tx = (ncols + 127) / 128;
+        return std::max<uint64_t>(1, 0x7FFFFFull / tx) * 128;
+    }
+
     // MFMA shortlist (BF16 or FP8) + exact rerank; rows that are tied or not
     // certified are rerun on the exact path.
     void search_shortlist(const float *dq, uint64_t nq, uint32_t k, uint64_t *d_idx,
@@ -394,10 +487,15 @@ class BruteForce final : public Index {
         const uint32_t S = shortlist_size(k);
         const bool bf16 = mode_ == HM_SEARCH_BF16_RERANK;
         const uint32_t row_bytes = bf16 ? dk_ * 2 : dk_;
-        const uint64_t ldk = round_up(n, 4);
-        uint64_t qc = std::max<uint64_t>(1, d_budget_ / (ldk * sizeof(float)));
+        const FilterPlan fp = filter_plan(S);
+        const bool filter = fp.ns != 0;
+        if (filter) ensure_sample(fp.ns, s);
+        const uint64_t ldk = round_up(filter ? fp.ns : n, 4);
+        const uint64_t per_q = ldk * sizeof(float) + (filter ? (uint64_t)fp.cap * 8 : 0);
+        uint64_t qc = std::max<uint64_t>(1, d_budget_ / per_q);
         qc = std::min<uint64_t>(qc, nq);
         qc = std::min<uint64_t>(qc, 65535ull * 128);
+        qc = std::min<uint64_t>(qc, grid_rows_limit(n));
         qpad_.reserve(qc * dpad_ * sizeof(float));
         qnorm_.reserve(qc * sizeof(float));
         qlow_.reserve(qc * row_bytes);
@@ -408,6 +506,16 @@ class BruteForce final : public Index {
         flags_.reserve(qc * sizeof(uint32_t));
         rflags_.reserve(nq * sizeof(uint32_t));
         qinexact_.reserve(sizeof(uint32_t));
+        if (filter) {
+            sel_d_.reserve(qc * fp.m * sizeof(float));
+            sel_i_.reserve(qc * fp.m * sizeof(uint32_t));
+            sel_f_.reserve(qc * sizeof(uint32_t));
+            list_.reserve(qc * fp.cap * sizeof(uint64_t));
+            lcnt_.reserve(qc * sizeof(uint32_t));
+            sq_.reserve(qc * sizeof(uint32_t));
+            ksq_.reserve(qc * sizeof(float));
+            pref_.reserve(qc * sizeof(uint32_t));
+        }
         HM_HIP(hipMemsetAsync(qinexact_.get(), 0, sizeof(uint32_t), s));
 
         const float u = 5.9604644775390625e-8f;
@@ -425,16 +533,41 @@ class BruteForce final : public Index {
             else
                 launch_to_fp8(qpad_.as<float>(), dpad_, rows, (uint32_t)dim_, dk_,
                               qlow_.as<uint8_t>(), qscale_.as<float>(), s);
-            launch_score_gemm(bf16 ? 2 : 1, qlow_.get(), rows, blow_.get(), n, row_bytes,
-                              qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
-                              inv_bnorm_.as<float>(), metric_, D_.as<float>(), ldk, s);
-            launch_row_select(D_.as<float>(), ldk, rows, n, S, cand_d_.as<float>(),
-                              cand_i_.as<uint32_t>(), S, flags_.as<uint32_t>(), s);
+            const uint32_t *s_q = nullptr, *pre = nullptr;
+            const float *key_s = nullptr;
+            if (filter) {
+                // threshold from the sample, then the filtered full scan
+                launch_score_gemm(bf16 ? 2 : 1, qlow_.get(), rows, bsamp_.get(), fp.ns, row_bytes,
+                                  qscale_.as<float>(), bs_scale_.as<float>(),
+                                  bs_norm2_.as<float>(), bs_inv_.as<float>(), metric_,
+                                  D_.as<float>(), ldk, s);
+                launch_row_select(D_.as<float>(), ldk, rows, fp.ns, fp.m, sel_d_.as<float>(),
+                                  sel_i_.as<uint32_t>(), fp.m, sel_f_.as<uint32_t>(), s);
+                launch_fill_u32(lcnt_.as<uint32_t>(), rows, 0, s);
+                const float *tau = sel_d_.as<float>() + (fp.m - 1);
+                launch_score_filter(bf16 ? 2 : 1, qlow_.get(), rows, blow_.get(), n, row_bytes,
+                                    qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
+                                    inv_bnorm_.as<float>(), metric_, tau, fp.m,
+                                    list_.as<uint64_t>(), fp.cap, lcnt_.as<uint32_t>(), s);
+                launch_list_prepare(list_.as<uint64_t>(), fp.cap, lcnt_.as<uint32_t>(), tau, fp.m,
+                                    rows, S, cand_i_.as<uint32_t>(), cand_d_.as<float>(), S,
+                                    sq_.as<uint32_t>(), ksq_.as<float>(), pref_.as<uint32_t>(), s);
+                s_q = sq_.as<uint32_t>();
+                key_s = ksq_.as<float>();
+                pre = pref_.as<uint32_t>();
+            } else {
+                launch_score_gemm(bf16 ? 2 : 1, qlow_.get(), rows, blow_.get(), n, row_bytes,
+                                  qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
+                                  inv_bnorm_.as<float>(), metric_, D_.as<float>(), ldk, s);
+                launch_row_select(D_.as<float>(), ldk, rows, n, S, cand_d_.as<float>(),
+                                  cand_i_.as<uint32_t>(), S, flags_.as<uint32_t>(), s);
+            }
             launch_rerank(metric_, qpad_.as<float>(), dpad_, base_.as<float>(),
                           cand_i_.as<uint32_t>(), cand_d_.as<float>(), S, S, qnorm_.as<float>(),
                           bnorm_.as<float>(), rows, n, k, bmax_, gamma, c_exact, c_bf16,
                           qinexact_.as<uint32_t>(), bf16 ? certify_ : 0, d_idx + q0 * k,
-                          d_dist + q0 * k, d_count + q0, rflags_.as<uint32_t>() + q0, s);
+                          d_dist + q0 * k, d_count + q0, rflags_.as<uint32_t>() + q0, s,
+                          nullptr, nullptr, nullptr, s_q, key_s, pre);
         }
         // Rows needing the exact path (ties: heap replay; uncertified).
         std::vector<uint32_t> fl(nq);
@@ -622,12 +755,18 @@ class BruteForce final : public Index {
     uint32_t dk_ = 0;
     float bmax_ = 0;
     DeviceBuffer inv_bnorm_, stat_, blow_, bscale_, qlow_, qscale_, qinexact_, rflags_;
+    // filter form: strided base sample (low-precision rows, norms, scales)
+    // and the per-chunk threshold / candidate-list buffers
+    uint32_t samp_ns_ = 0;
+    bool use_filter_ = true;
+    DeviceBuffer bsamp_, bs_norm2_, bs_inv_, bs_scale_;
+    DeviceBuffer Ds_, sel_d_, sel_i_, sel_f_, list_, lcnt_, sq_, ksq_, pref_;
     DeviceBuffer redo_rows_, redo_q_, redo_i_, redo_d_, redo_c_;
     uint64_t last_fallback_ = 0, last_uncertified_ = 0;
     // ragged graph
     hipStream_t rg_stream_ = nullptr;
     hipGraphExec_t rg_exec_ = nullptr;
-    uint32_t rg_qc_ = 0, rg_kmax_ = 0, rg_s_ = 0;
+    uint32_t rg_qc_ = 0, rg_kmax_ = 0, rg_s_ = 0, rg_ns_ = 0;
     int32_t rg_mode_ = -1, rg_metric_ = -1;
     uint64_t rg_n_ = 0, graph_launches_ = 0;
     DeviceBuffer rg_rows_, rg_q_, rg_ck_, rg_coff_, rg_oi_, rg_od_, rg_oc_, rg_kq_, rg_off_,
@@ -635,6 +774,7 @@ class BruteForce final : public Index {
     // buffers owned by the captured graph (never reallocated by other paths)
     DeviceBuffer rg_qpad_, rg_qnorm_, rg_qlow_, rg_qscale_, rg_D_, rg_cd_, rg_ci_, rg_sf_, rg_rf_,
         rg_qinex_;
+    DeviceBuffer rg_seld_, rg_seli_, rg_list_, rg_lcnt_, rg_sq_, rg_ksq_, rg_pref_;
 };
 
 }  // namespace

@@ -67,6 +67,18 @@ void launch_score_gemm(int elem_bytes, const void *Q, uint32_t nq, const void *B
                        uint32_t row_bytes, const float *qscale, const float *bscale,
                        const float *bnorm2, const float *inv_bnorm, int metric, float *Kmat,
                        uint64_t ldk, hipStream_t s, const uint32_t *dev_rows = nullptr);
+// Filter form: keys below tau[q * tau_ld] are appended to list[q][0..cap)
+// as fkey(key) << 32 | row, counted in cnt[q] (zeroed by the caller; the
+// count may exceed cap, the entries past cap are dropped).
+void launch_score_filter(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
+                         uint32_t row_bytes, const float *qscale, const float *bscale,
+                         const float *bnorm2, const float *inv_bnorm, int metric,
+                         const float *tau, uint32_t tau_ld, uint64_t *list, uint32_t cap,
+                         uint32_t *cnt, hipStream_t s, const uint32_t *dev_rows = nullptr);
+// Strided sample of the base: dst row i = src row floor(i * n / ns).
+void launch_sample_rows(const void *src, uint32_t row_bytes, uint32_t n, uint32_t ns, void *dst,
+                        hipStream_t s);
+void launch_sample_f32(const float *src, uint32_t n, uint32_t ns, float *dst, hipStream_t s);
 void launch_to_bf16(const float *src, uint64_t lds, uint64_t rows, uint32_t d, uint32_t dk,
                     uint16_t *dst, uint32_t *inexact, hipStream_t s,
                     const uint32_t *dev_rows = nullptr);
@@ -83,7 +95,18 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
                    const uint32_t *q_inexact, int certify, uint64_t *out_idx, float *out_dist,
                    uint32_t *out_count, uint32_t *flags, hipStream_t st,
                    const uint32_t *dev_rows = nullptr, const uint32_t *kq = nullptr,
-                   const uint64_t *out_off = nullptr);
+                   const uint64_t *out_off = nullptr, const uint32_t *s_q = nullptr,
+                   const float *key_s_q = nullptr, const uint32_t *pre_flags = nullptr);
+// Candidate lists of the filter form -> the shortlist format of rerank:
+// per query the first min(cnt, S) entries by (key, row) into cand_i /
+// cand_key (stride ldc), s_q = that count, key_s_q = a lower bound of the
+// key of every row left out (the smallest list key left out, else tau),
+// flags = ROW_UNCERT when the list overflowed.
+void launch_list_prepare(const uint64_t *list, uint32_t cap, const uint32_t *cnt,
+                         const float *tau, uint32_t tau_ld, uint32_t rows, uint32_t S,
+                         uint32_t *cand_i, float *cand_key, uint32_t ldc, uint32_t *s_q,
+                         float *key_s_q, uint32_t *flags, hipStream_t st,
+                         const uint32_t *dev_rows = nullptr);
 void launch_scatter_ragged(const uint64_t *si, const float *sd, const uint32_t *sc,
                            const uint32_t *rows, uint32_t nrows, uint32_t kmax, const uint32_t *kq,
                            const uint64_t *out_off, uint64_t *di, float *dd, uint32_t *dc,

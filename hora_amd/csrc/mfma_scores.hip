@@ -12,8 +12,10 @@
 //
 // Tiling: 256 threads = 4 waves in a 2x2 arrangement; the workgroup owns a
 // 128-query x 128-base tile, each wave 64x64 = 4x4 MFMA 16x16 tiles (64 fp32
-// accumulators per lane).  Each K step stages 64 bytes per row of both
-// operands in LDS (double buffered through registers).
+// accumulators per lane).  Each K step stages 128 bytes per row of both
+// operands in LDS (two LDS buffers, two register sets).  The filter form
+// writes no key matrix: keys below a per-query threshold are appended to
+// per-query candidate lists in the epilogue.
 #include "kernels.h"
 #include "profiler.h"
 
@@ -28,6 +30,43 @@ constexpr int GM = 128, GN = 128;
 constexpr int ROWB = 128;           // bytes per row per K step (64 bf16 / 128 fp8)
 constexpr int LDS_ROW = ROWB + 16;  // padded LDS row stride (bytes)
 constexpr int CPT = GM * ROWB / 16 / 256;  // 16-byte chunks per thread per operand (4)
+constexpr uint32_t GROUP_B = 8;     // base tiles per group of the tile order
+
+__device__ __forceinline__ uint32_t fkey(float f)
+{
+    f = f + 0.0f;
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+// Output of a score launch: either the full key matrix, or (FILTER) the
+// keys below a per-query threshold appended to per-query lists.
+struct ScoreOut {
+    float *Kmat;  // [nq][ldk]
+    uint64_t ldk;
+    const float *tau;  // FILTER: threshold of query q at tau[q * tau_ld]
+    uint32_t tau_ld;
+    uint64_t *list;  // FILTER: [nq][cap] of fkey(key) << 32 | row
+    uint32_t cap;
+    uint32_t *cnt;  // FILTER: appended per query (may exceed cap)
+};
+
+// Tile order.  The workgroups are dispatched round robin over the 8 XCDs
+// (workgroup L to XCD L % 8), each XCD with its own L2.  The linear id is
+// remapped so that every XCD walks a contiguous range of tiles, and tiles
+// are ordered in groups of GROUP_B base tiles with the query tiles inside a
+// group: the group's base tiles stay in that XCD's L2 while the (smaller)
+// query matrix streams past them, and the base is read from HBM once.
+__device__ __forceinline__ void tile_of(uint32_t L, uint32_t TX, uint32_t TY, uint32_t *bt,
+                                        uint32_t *qt)
+{
+    const uint32_t T = TX * TY, q = T / 8, r = T % 8, x = L % 8, i = L / 8;
+    const uint32_t t = x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+    const uint32_t per = GROUP_B * TY, g = t / per, w = t % per;
+    const uint32_t gb = min(GROUP_B, TX - g * GROUP_B);
+    *qt = w / gb;
+    *bt = g * GROUP_B + w % gb;
+}
 
 // Software pipeline: the global loads of step kt+2 are issued before the
 // MFMAs of step kt into one of two register sets, and stored to LDS one
@@ -35,21 +74,23 @@ constexpr int CPT = GM * ROWB / 16 / 256;  // 16-byte chunks per thread per oper
 // The metric is a template parameter: the epilogue has no run-time branch
 // on it (a uniform branch chain there compiled to a store whose address
 // register was left unset on the last arm).
-template <int ELEM, int MET>  // ELEM: 2 = bf16, 1 = fp8
+template <int ELEM, int MET, bool FILTER>  // ELEM: 2 = bf16, 1 = fp8
 __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
-    const uint8_t *__restrict__ Q, uint32_t nq, const uint8_t *__restrict__ B, uint32_t n,
-    uint32_t row_bytes, const float *__restrict__ qscale, const float *__restrict__ bscale,
-    const float *__restrict__ bnorm2, const float *__restrict__ inv_bnorm,
-    float *__restrict__ Kmat, uint64_t ldk, const uint32_t *__restrict__ dev_rows)
+    const uint8_t *__restrict__ Q, uint32_t nq, uint32_t TY, const uint8_t *__restrict__ B,
+    uint32_t n, uint32_t row_bytes, const float *__restrict__ qscale,
+    const float *__restrict__ bscale, const float *__restrict__ bnorm2,
+    const float *__restrict__ inv_bnorm, ScoreOut out, const uint32_t *__restrict__ dev_rows)
 {
+    uint32_t btile, qtile;
+    tile_of(blockIdx.x, (n + GN - 1) / GN, TY, &btile, &qtile);
     if (dev_rows && *dev_rows < nq) nq = *dev_rows;
-    if (blockIdx.y * GM >= nq) return;
+    if (qtile * GM >= nq) return;
     __shared__ __align__(16) uint8_t sA[2][GM * LDS_ROW];
     __shared__ __align__(16) uint8_t sB[2][GN * LDS_ROW];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
-    const uint32_t q0 = blockIdx.y * GM, n0 = blockIdx.x * GN;
+    const uint32_t q0 = qtile * GM, n0 = btile * GN;
 
     floatx4 acc[4][4];
 #pragma unroll
@@ -89,7 +130,7 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
 
     const uint32_t nk = row_bytes / ROWB;
     const int lr = lane & 15, lk = lane >> 4;
-    // One K step on LDS buffer `buf`: MFMAs over its 128-byte slab.
+    // One K step: MFMAs over a 128-byte slab in LDS.
     auto mma_step = [&](const uint8_t *As, const uint8_t *Bs) {
         if constexpr (ELEM == 2) {
             // two MFMA K=32 per step; lane holds 8 bf16 at k = 32*kk + 8*lk
@@ -170,7 +211,8 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
             const uint32_t q = q0 + wr * 64 + i * 16 + lk * 4 + e;
             if (q >= nq) continue;
             const float qs = (ELEM == 1) ? qscale[q] : 1.0f;
-            float *krow = Kmat + (size_t)q * ldk;
+            float thr = 0.0f;
+            if constexpr (FILTER) thr = out.tau[(size_t)q * out.tau_ld];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const uint32_t c = n0 + wc * 64 + j * 16 + lr;
@@ -182,10 +224,41 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
                 else if constexpr (MET == HM_METRIC_DOT_PRODUCT) key = s;
                 else if constexpr (MET == HM_METRIC_COSINE_SIMILARITY) key = -s * inv_bnorm[c];
                 else key = s * inv_bnorm[c];
-                krow[c] = key;
+                if constexpr (FILTER) {
+                    if (key < thr) {
+                        const uint32_t pos = atomicAdd(out.cnt + q, 1u);
+                        if (pos < out.cap)
+                            out.list[(size_t)q * out.cap + pos] =
+                                ((uint64_t)fkey(key) << 32) | c;
+                    }
+                } else {
+                    out.Kmat[(size_t)q * out.ldk + c] = key;
+                }
             }
         }
     }
+}
+
+// Rows of a strided sample of the base: dst row i = src row floor(i*n/ns).
+__global__ void sample_rows_kernel(const uint8_t *__restrict__ src, uint32_t row_bytes, uint32_t n,
+                                   uint32_t ns, uint8_t *__restrict__ dst)
+{
+    const uint32_t words = row_bytes / 4;
+    const uint64_t total = (uint64_t)ns * words;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = e / words, w = e % words;
+        const uint64_t r = i * n / ns;
+        reinterpret_cast<uint32_t *>(dst)[e] =
+            reinterpret_cast<const uint32_t *>(src)[r * words + w];
+    }
+}
+
+__global__ void sample_f32_kernel(const float *__restrict__ src, uint32_t n, uint32_t ns,
+                                  float *__restrict__ dst)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x)
+        dst[i] = src[(uint64_t)i * n / ns];
 }
 
 // Round-to-nearest-even f32 -> bf16; flags a value that does not survive.
@@ -252,27 +325,30 @@ __global__ void norm_stats_kernel(const float *__restrict__ norm2, uint32_t rows
 
 }  // namespace
 
-void launch_score_gemm(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
-                       uint32_t row_bytes, const float *qscale, const float *bscale,
-                       const float *bnorm2, const float *inv_bnorm, int metric, float *Kmat,
-                       uint64_t ldk, hipStream_t s, const uint32_t *dev_rows)
+template <bool FILTER>
+static void score_launch(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
+                         uint32_t row_bytes, const float *qscale, const float *bscale,
+                         const float *bnorm2, const float *inv_bnorm, int metric,
+                         const ScoreOut &out, hipStream_t s, const uint32_t *dev_rows)
 {
     if (nq == 0 || n == 0) return;
-    if (row_bytes % ROWB) fail(HM_ERR_INVALID_ARG, "score_gemm: row bytes not a multiple of 128");
-    dim3 grid(ceil_div(n, GN), ceil_div(nq, GM));
+    if (row_bytes == 0 || row_bytes % ROWB)
+        fail(HM_ERR_INVALID_ARG, "score_gemm: row bytes not a multiple of 128");
     if (elem_bytes != 1 && elem_bytes != 2) fail(HM_ERR_INVALID_ARG, "score_gemm: element size");
-    ProfScope prof("score_gemm", s);
-#define HM_SG(E, M)                                                                              \
-    score_gemm_kernel<E, M><<<grid, 256, 0, s>>>((const uint8_t *)Q, nq, (const uint8_t *)B, n, \
-                                                 row_bytes, qscale, bscale, bnorm2, inv_bnorm,  \
-                                                 Kmat, ldk, dev_rows)
-#define HM_SG_METRIC(E)                                                                         \
-    switch (metric) {                                                                           \
-    case HM_METRIC_EUCLIDEAN: HM_SG(E, HM_METRIC_EUCLIDEAN); break;                             \
-    case HM_METRIC_DOT_PRODUCT: HM_SG(E, HM_METRIC_DOT_PRODUCT); break;                         \
-    case HM_METRIC_COSINE_SIMILARITY: HM_SG(E, HM_METRIC_COSINE_SIMILARITY); break;             \
-    case HM_METRIC_ANGULAR: HM_SG(E, HM_METRIC_ANGULAR); break;                                 \
-    default: fail(HM_ERR_INVALID_ARG, "score_gemm: metric has no shortlist form");              \
+    const uint64_t TX = ceil_div(n, GN), TY = ceil_div(nq, GM);
+    if (TX * TY > 0x7FFFFFull) fail(HM_ERR_INVALID_ARG, "score_gemm: too many tiles");
+    const dim3 grid((unsigned)(TX * TY));
+#define HM_SG(E, M)                                                                           \
+    score_gemm_kernel<E, M, FILTER><<<grid, 256, 0, s>>>(                                     \
+        (const uint8_t *)Q, nq, (uint32_t)TY, (const uint8_t *)B, n, row_bytes, qscale, bscale, \
+        bnorm2, inv_bnorm, out, dev_rows)
+#define HM_SG_METRIC(E)                                                                       \
+    switch (metric) {                                                                         \
+    case HM_METRIC_EUCLIDEAN: HM_SG(E, HM_METRIC_EUCLIDEAN); break;                           \
+    case HM_METRIC_DOT_PRODUCT: HM_SG(E, HM_METRIC_DOT_PRODUCT); break;                       \
+    case HM_METRIC_COSINE_SIMILARITY: HM_SG(E, HM_METRIC_COSINE_SIMILARITY); break;           \
+    case HM_METRIC_ANGULAR: HM_SG(E, HM_METRIC_ANGULAR); break;                               \
+    default: fail(HM_ERR_INVALID_ARG, "score_gemm: metric has no shortlist form");            \
     }
     if (elem_bytes == 2) {
         HM_SG_METRIC(2)
@@ -281,6 +357,49 @@ void launch_score_gemm(int elem_bytes, const void *Q, uint32_t nq, const void *B
     }
 #undef HM_SG_METRIC
 #undef HM_SG
+    HM_CHECK_LAUNCH();
+}
+
+void launch_score_gemm(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
+                       uint32_t row_bytes, const float *qscale, const float *bscale,
+                       const float *bnorm2, const float *inv_bnorm, int metric, float *Kmat,
+                       uint64_t ldk, hipStream_t s, const uint32_t *dev_rows)
+{
+    ProfScope prof("score_gemm", s);
+    ScoreOut out{Kmat, ldk, nullptr, 0, nullptr, 0, nullptr};
+    score_launch<false>(elem_bytes, Q, nq, B, n, row_bytes, qscale, bscale, bnorm2, inv_bnorm,
+                        metric, out, s, dev_rows);
+}
+
+void launch_score_filter(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
+                         uint32_t row_bytes, const float *qscale, const float *bscale,
+                         const float *bnorm2, const float *inv_bnorm, int metric,
+                         const float *tau, uint32_t tau_ld, uint64_t *list, uint32_t cap,
+                         uint32_t *cnt, hipStream_t s, const uint32_t *dev_rows)
+{
+    if (!tau || !list || !cnt || cap == 0) fail(HM_ERR_INVALID_ARG, "score_filter: null output");
+    ProfScope prof("score_filter", s);
+    ScoreOut out{nullptr, 0, tau, tau_ld, list, cap, cnt};
+    score_launch<true>(elem_bytes, Q, nq, B, n, row_bytes, qscale, bscale, bnorm2, inv_bnorm,
+                       metric, out, s, dev_rows);
+}
+
+void launch_sample_rows(const void *src, uint32_t row_bytes, uint32_t n, uint32_t ns, void *dst,
+                        hipStream_t s)
+{
+    if (ns == 0) return;
+    if (row_bytes % 4 || ns > n) fail(HM_ERR_INVALID_ARG, "sample_rows: bad shape");
+    const unsigned blocks = (unsigned)std::min<uint64_t>(((uint64_t)ns * row_bytes / 4 + 255) / 256, 4096);
+    sample_rows_kernel<<<blocks, 256, 0, s>>>((const uint8_t *)src, row_bytes, n, ns,
+                                              (uint8_t *)dst);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_sample_f32(const float *src, uint32_t n, uint32_t ns, float *dst, hipStream_t s)
+{
+    if (ns == 0) return;
+    if (ns > n) fail(HM_ERR_INVALID_ARG, "sample_f32: bad shape");
+    sample_f32_kernel<<<std::min<unsigned>(ceil_div(ns, 256), 1024), 256, 0, s>>>(src, n, ns, dst);
     HM_CHECK_LAUNCH();
 }
 

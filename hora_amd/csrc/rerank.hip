@@ -42,14 +42,16 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
     const uint32_t *__restrict__ q_inexact, int certify, uint64_t *__restrict__ out_idx,
     float *__restrict__ out_dist, uint32_t *__restrict__ out_count, uint32_t *__restrict__ flags,
     const uint32_t *__restrict__ dev_rows, const uint32_t *__restrict__ kq,
-    const uint64_t *__restrict__ out_off)
+    const uint64_t *__restrict__ out_off, const uint32_t *__restrict__ s_q,
+    const float *__restrict__ key_s_q, const uint32_t *__restrict__ pre_flags)
 {
     if (dev_rows && blockIdx.x >= *dev_rows) return;  // graph replays with fewer rows
-    // dynamic LDS: keys[P] (u64) | vals[P] | sq[dpad]
+    // dynamic LDS: keys[P] (u64) | vals[P] | sq[dpad]; P covers the largest s
     extern __shared__ __align__(16) unsigned char rr_smem[];
     __shared__ uint32_t s_tie, s_bad;
     uint32_t P = 1;
     while (P < s) P <<= 1;
+    if (s_q) s = min(s, s_q[blockIdx.x]);  // this query's shortlist length
     unsigned long long *keys = reinterpret_cast<unsigned long long *>(rr_smem);
     float *vals = reinterpret_cast<float *>(rr_smem + (size_t)P * 8);
     float *sq = reinterpret_cast<float *>(rr_smem + (size_t)P * 12);
@@ -113,6 +115,11 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
     }
     for (uint32_t j = s + tid; j < P; j += RR_THREADS) keys[j] = ~0ull;
     __syncthreads();
+    {
+        uint32_t Pq = 1;  // this row's sort length
+        while (Pq < s) Pq <<= 1;
+        P = Pq;
+    }
     for (uint32_t size = 2; size <= P; size <<= 1) {
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
             for (uint32_t t = tid; t < P / 2; t += RR_THREADS) {
@@ -151,12 +158,12 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
     __syncthreads();
     if (tid != 0) return;
     out_count[r] = m < s ? m : s;
-    uint32_t f = (s_tie ? ROW_TIE : 0u) | (s_bad ? ROW_BAD : 0u);
+    uint32_t f = (s_tie ? ROW_TIE : 0u) | (s_bad ? ROW_BAD : 0u) | (pre_flags ? pre_flags[r] : 0u);
     if (s < n && certify) {
         bool ok = false;
         if (k + 1 <= s) {
             const float ek1 = vals[(uint32_t)keys[k]];
-            const float key_s = cand_key[(size_t)r * ldc + s - 1];
+            const float key_s = key_s_q ? key_s_q[r] : cand_key[(size_t)r * ldc + s - 1];
             const double c_in = q_inexact && *q_inexact ? c_in_inexact : c_in_exact;
             const double g = gamma, u = 5.9604644775390625e-8;  // 2^-24
             const double nq = sqrt((double)qn2);
@@ -181,6 +188,61 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
         if (!ok) f |= ROW_UNCERT;
     }
     flags[r] = f;
+}
+
+__device__ __forceinline__ float unfkey(uint32_t k)
+{
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+constexpr int LP_THREADS = 256;
+
+// One workgroup per query: sorts the query's appended (key, row) entries
+// and emits the first min(cnt, S).
+__global__ __launch_bounds__(LP_THREADS) void list_prepare_kernel(
+    const uint64_t *__restrict__ list, uint32_t cap, const uint32_t *__restrict__ cnt,
+    const float *__restrict__ tau, uint32_t tau_ld, uint32_t S, uint32_t *__restrict__ cand_i,
+    float *__restrict__ cand_key, uint32_t ldc, uint32_t *__restrict__ s_q,
+    float *__restrict__ key_s_q, uint32_t *__restrict__ flags, const uint32_t *__restrict__ dev_rows)
+{
+    const uint32_t r = blockIdx.x;
+    if (dev_rows && r >= *dev_rows) return;
+    extern __shared__ __align__(16) unsigned long long lp_keys[];
+    const int tid = threadIdx.x;
+    const uint32_t raw = cnt[r], c = raw < cap ? raw : cap;
+    uint32_t P = 1;
+    while (P < c) P <<= 1;
+    const uint64_t *src = list + (size_t)r * cap;
+    for (uint32_t j = tid; j < P; j += LP_THREADS) lp_keys[j] = j < c ? src[j] : ~0ull;
+    __syncthreads();
+    for (uint32_t size = 2; size <= P; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t t = tid; t < P / 2; t += LP_THREADS) {
+                const uint32_t lo = 2 * stride * (t / stride) + (t % stride), hi = lo + stride;
+                const bool asc = (lo & size) == 0;
+                const unsigned long long a = lp_keys[lo], b = lp_keys[hi];
+                if ((a > b) == asc) {
+                    lp_keys[lo] = b;
+                    lp_keys[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const uint32_t s = c < S ? c : S;
+    for (uint32_t j = tid; j < s; j += LP_THREADS) {
+        cand_i[(size_t)r * ldc + j] = (uint32_t)lp_keys[j];
+        cand_key[(size_t)r * ldc + j] = unfkey((uint32_t)(lp_keys[j] >> 32));
+    }
+    if (tid == 0) {
+        s_q[r] = s;
+        // every row left out has key >= the smallest key left out of the
+        // list (if any) >= ... and every row never appended has key >= tau
+        const float t = tau[(size_t)r * tau_ld];
+        const float left = c > S ? unfkey((uint32_t)(lp_keys[S] >> 32)) : t;
+        key_s_q[r] = left < t ? left : t;
+        flags[r] = raw > cap ? ROW_UNCERT : 0u;
+    }
 }
 
 __global__ void gather_rows_kernel(const float *__restrict__ src, uint64_t lds,
@@ -265,7 +327,8 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
                    uint32_t k, float bmax, float gamma, float c_in_exact, float c_in_inexact,
                    const uint32_t *q_inexact, int certify, uint64_t *out_idx, float *out_dist,
                    uint32_t *out_count, uint32_t *flags, hipStream_t st,
-                   const uint32_t *dev_rows, const uint32_t *kq, const uint64_t *out_off)
+                   const uint32_t *dev_rows, const uint32_t *kq, const uint64_t *out_off,
+                   const uint32_t *s_q, const float *key_s_q, const uint32_t *pre_flags)
 {
     if (rows == 0) return;
     if (s == 0 || s > RR_CAP) fail(HM_ERR_INVALID_ARG, "rerank: shortlist size out of range");
@@ -278,7 +341,7 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
     rerank_kernel<ACC, EPI><<<rows, RR_THREADS, lds, st>>>(                                     \
         Qp, dpad, Bp, cand_i, cand_key, ldc, s, qnorm2, bnorm2, n, k, bmax, gamma, c_in_exact, \
         c_in_inexact, q_inexact, certify, out_idx, out_dist, out_count, flags, dev_rows, kq,    \
-        out_off)
+        out_off, s_q, key_s_q, pre_flags)
     switch (metric) {
     case HM_METRIC_EUCLIDEAN: HM_RR(ACC_EUCLID, EPI_NONE); break;
     case HM_METRIC_DOT_PRODUCT: HM_RR(ACC_DOT, EPI_NONE); break;
@@ -287,6 +350,23 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
     default: fail(HM_ERR_INVALID_ARG, "rerank: metric has no shortlist form");
     }
 #undef HM_RR
+    HM_CHECK_LAUNCH();
+}
+
+void launch_list_prepare(const uint64_t *list, uint32_t cap, const uint32_t *cnt,
+                         const float *tau, uint32_t tau_ld, uint32_t rows, uint32_t S,
+                         uint32_t *cand_i, float *cand_key, uint32_t ldc, uint32_t *s_q,
+                         float *key_s_q, uint32_t *flags, hipStream_t st,
+                         const uint32_t *dev_rows)
+{
+    if (rows == 0) return;
+    if (cap == 0 || cap > RR_CAP || S == 0 || S > ldc)
+        fail(HM_ERR_INVALID_ARG, "list_prepare: capacity out of range");
+    ProfScope prof("list_prepare", st);
+    uint32_t P = 1;
+    while (P < cap) P <<= 1;
+    list_prepare_kernel<<<rows, LP_THREADS, (size_t)P * 8, st>>>(
+        list, cap, cnt, tau, tau_ld, S, cand_i, cand_key, ldc, s_q, key_s_q, flags, dev_rows);
     HM_CHECK_LAUNCH();
 }
 

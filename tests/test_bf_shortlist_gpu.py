@@ -37,8 +37,12 @@ def _exact_check(orc, idx, base, q, k, mt):
     gi, gd, gc = idx.search_batch(q, k)
     ri, rd, rc = orc.bf_search(base, q, k, int(mt))
     assert np.array_equal(gc, rc)
-    assert np.array_equal(gi, ri)
-    assert np.array_equal(gd, rd)
+    for r in range(q.shape[0]):
+        m = int(rc[r])  # the reference returns min(k, n) results; the rest is padding
+        assert np.array_equal(gi[r, :m], ri[r, :m]), f"row {r} ids"
+        assert np.array_equal(gd[r, :m], rd[r, :m]), f"row {r} distances"
+        assert (gi[r, m:] == np.iinfo(np.uint64).max).all()
+        assert np.isinf(gd[r, m:]).all()
     return idx.search_stats()
 
 
