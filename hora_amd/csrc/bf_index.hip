@@ -161,7 +161,7 @@ class BruteForce final : public Index {
         err_.reserve(sizeof(uint32_t));
         HM_HIP(hipMemsetAsync(err_.get(), 0, sizeof(uint32_t), s));
         launch_check_finite(dq, dim_, nq, (uint32_t)dim_, err_.as<uint32_t>(), s);
-        const bool shortlist_form = mode_ != HM_SEARCH_EXACT && shortlist_metric() && dpad_ <= 4096;
+        const bool shortlist_form = shortlist_usable(s);
         last_fallback_ = last_uncertified_ = 0;
         if (shortlist_form)
             search_shortlist(dq, nq, k, d_idx, d_dist, d_count, s);
@@ -187,6 +187,16 @@ class BruteForce final : public Index {
                metric_ == HM_METRIC_COSINE_SIMILARITY;
     }
 
+    // A zero-norm base row has no Cosine/Angular key (0 * inf), so the
+    // shortlist could not bound it; such bases search on the exact form
+    // (where Cosine reports the NaN distance as the reference would fail).
+    bool shortlist_usable(hipStream_t s)
+    {
+        if (mode_ == HM_SEARCH_EXACT || !shortlist_metric() || dpad_ > 4096 || n_ == 0) return false;
+        ensure_shortlist_data(s);
+        return !(zero_norm_ && need_norm());
+    }
+
     bool need_norm() const
     {
         return metric_ == HM_METRIC_COSINE_SIMILARITY || metric_ == HM_METRIC_ANGULAR;
@@ -203,8 +213,11 @@ class BruteForce final : public Index {
 
     // Full exact path: every pair in reference order, radix top-(k+1), heap
     // replay for tied rows.  Queries: nq rows of dim floats, row stride ldq.
+    // all_tied: every row is known to hold a tie among its k+1 smallest
+    // distances (and no NaN), so the select is skipped and each row goes
+    // straight to the heap replay over its exact distance row.
     void search_exact(const float *dq, uint64_t ldq, uint64_t nq, uint32_t k, uint64_t *d_idx,
-                      float *d_dist, uint32_t *d_count, hipStream_t s)
+                      float *d_dist, uint32_t *d_count, hipStream_t s, bool all_tied = false)
     {
         const uint32_t n = (uint32_t)n_;
         const uint32_t K = (uint32_t)std::min<uint64_t>((uint64_t)k + 1, n);
@@ -220,10 +233,6 @@ class BruteForce final : public Index {
         cand_i_.reserve(qc * K * sizeof(uint32_t));
         flags_.reserve(qc * sizeof(uint32_t));
         qnorm_.reserve(qc * sizeof(float));
-        if ((size_t)64 * (k + 1) * 8 > 64 * 1024) {
-            heap_d_.reserve(qc * (k + 1) * sizeof(float));
-            heap_i_.reserve(qc * (k + 1) * sizeof(uint32_t));
-        }
         for (uint64_t q0 = 0; q0 < nq; q0 += qc) {
             const uint32_t rows = (uint32_t)std::min<uint64_t>(qc, nq - q0);
             launch_pad_rows(dq + q0 * ldq, ldq, qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
@@ -234,13 +243,15 @@ class BruteForce final : public Index {
             launch_exact_tile(metric_, qpad_.as<float>(), dpad_, rows, base_.as<float>(), dpad_,
                               n, dpad_, qnorm_.as<float>(), bnorm_.as<float>(), D_.as<float>(),
                               ldd, s);
-            launch_row_select(D_.as<float>(), ldd, rows, n, K, cand_d_.as<float>(),
-                              cand_i_.as<uint32_t>(), K, flags_.as<uint32_t>(), s);
+            if (all_tied)
+                launch_fill_u32(flags_.as<uint32_t>(), rows, ROW_TIE, s);
+            else
+                launch_row_select(D_.as<float>(), ldd, rows, n, K, cand_d_.as<float>(),
+                                  cand_i_.as<uint32_t>(), K, flags_.as<uint32_t>(), s);
             launch_bf_finalize(cand_d_.as<float>(), cand_i_.as<uint32_t>(), K,
                                flags_.as<uint32_t>(), D_.as<float>(), ldd, rows, n, k,
                                d_idx + q0 * k, d_dist + q0 * k, d_count + q0,
-                               err_.as<uint32_t>(), heap_d_.as<float>(), heap_i_.as<uint32_t>(),
-                               s);
+                               err_.as<uint32_t>(), s);
         }
     }
 
@@ -358,7 +369,8 @@ class BruteForce final : public Index {
     // Exact path for the listed queries with kmax, packed into the ragged
     // layout (rg_kq_ / rg_off_ already on the device).
     void ragged_exact(const float *dq, const std::vector<uint32_t> &rows, uint32_t kmax,
-                      uint64_t *d_idx, float *d_dist, uint32_t *d_count, hipStream_t s)
+                      uint64_t *d_idx, float *d_dist, uint32_t *d_count, hipStream_t s,
+                      bool all_tied = false)
     {
         const uint32_t m = (uint32_t)rows.size();
         if (m == 0) return;
@@ -380,7 +392,7 @@ class BruteForce final : public Index {
             HM_HIP(hipMemsetAsync(err_.get(), 0, 4, s));
         }
         search_exact(redo_q_.as<float>(), dim_, m, kmax, redo_i_.as<uint64_t>(),
-                     redo_d_.as<float>(), redo_c_.as<uint32_t>(), s);
+                     redo_d_.as<float>(), redo_c_.as<uint32_t>(), s, all_tied);
         launch_scatter_ragged(redo_i_.as<uint64_t>(), redo_d_.as<float>(), redo_c_.as<uint32_t>(),
                               redo_rows_.as<uint32_t>(), m, kmax, rg_kq_.as<uint32_t>(),
                               rg_off_.as<uint64_t>(), d_idx, d_dist, d_count, s);
@@ -394,8 +406,8 @@ class BruteForce final : public Index {
         if (sl_valid_) return;
         const uint32_t n = (uint32_t)n_;
         inv_bnorm_.reserve(n_ * sizeof(float));
-        stat_.reserve(2 * sizeof(uint32_t));
-        HM_HIP(hipMemsetAsync(stat_.get(), 0, 2 * sizeof(uint32_t), s));
+        stat_.reserve(3 * sizeof(uint32_t));
+        HM_HIP(hipMemsetAsync(stat_.get(), 0, 3 * sizeof(uint32_t), s));
         launch_norm_stats(bnorm_.as<float>(), n, inv_bnorm_.as<float>(), stat_.as<uint32_t>(), s);
         if (mode_ == HM_SEARCH_BF16_RERANK) {
             dk_ = (uint32_t)round_up(dim_, 64);  // 128-byte rows
@@ -409,13 +421,14 @@ class BruteForce final : public Index {
             launch_to_fp8(base_.as<float>(), dpad_, n, (uint32_t)dim_, dk_, blow_.as<uint8_t>(),
                           bscale_.as<float>(), s);
         }
-        uint32_t st[2];
+        uint32_t st[3];
         HM_HIP(hipMemcpyAsync(st, stat_.get(), sizeof(st), hipMemcpyDeviceToHost, s));
         HM_HIP(hipStreamSynchronize(s));
         float bm;
         std::memcpy(&bm, &st[0], 4);
         bmax_ = bm;
         base_inexact_ = st[1] != 0;
+        zero_norm_ = st[2] != 0;
         sl_valid_ = true;
     }
 
@@ -574,13 +587,27 @@ class BruteForce final : public Index {
         HM_HIP(hipMemcpyAsync(fl.data(), rflags_.get(), nq * sizeof(uint32_t),
                               hipMemcpyDeviceToHost, s));
         HM_HIP(hipStreamSynchronize(s));
-        std::vector<uint32_t> redo;
-        for (uint64_t i = 0; i < nq; ++i)
-            if (fl[i] & (ROW_TIE | ROW_UNCERT | ROW_BAD)) redo.push_back((uint32_t)i);
-        last_fallback_ = redo.size();
+        // Certified rows with a tie only need the heap replay over their
+        // exact row; the others take the whole exact path.
+        std::vector<uint32_t> redo, tied;
         last_uncertified_ = 0;
-        for (uint64_t i = 0; i < nq; ++i) last_uncertified_ += (fl[i] & ROW_UNCERT) ? 1 : 0;
-        if (redo.empty()) return;
+        for (uint64_t i = 0; i < nq; ++i) {
+            if (fl[i] == ROW_TIE)
+                tied.push_back((uint32_t)i);
+            else if (fl[i] & (ROW_TIE | ROW_NAN | ROW_UNCERT | ROW_BAD))
+                redo.push_back((uint32_t)i);
+            last_uncertified_ += (fl[i] & ROW_UNCERT) ? 1 : 0;
+        }
+        last_fallback_ = redo.size() + tied.size();
+        if (!tied.empty()) rerun_rows(dq, tied, k, d_idx, d_dist, d_count, true, s);
+        if (!redo.empty()) rerun_rows(dq, redo, k, d_idx, d_dist, d_count, false, s);
+    }
+
+    // Exact path for the listed query rows; results scattered back.
+    void rerun_rows(const float *dq, const std::vector<uint32_t> &redo, uint32_t k,
+                    uint64_t *d_idx, float *d_dist, uint32_t *d_count, bool all_tied,
+                    hipStream_t s)
+    {
         const uint32_t m = (uint32_t)redo.size();
         redo_rows_.reserve(m * sizeof(uint32_t));
         redo_q_.reserve((size_t)m * dim_ * sizeof(float));
@@ -592,7 +619,7 @@ class BruteForce final : public Index {
         launch_gather_rows(dq, dim_, redo_rows_.as<uint32_t>(), m, (uint32_t)dim_,
                            redo_q_.as<float>(), dim_, s);
         search_exact(redo_q_.as<float>(), dim_, m, k, redo_i_.as<uint64_t>(), redo_d_.as<float>(),
-                     redo_c_.as<uint32_t>(), s);
+                     redo_c_.as<uint32_t>(), s, all_tied);
         launch_scatter_results(redo_i_.as<uint64_t>(), redo_d_.as<float>(),
                                redo_c_.as<uint32_t>(), redo_rows_.as<uint32_t>(), m, k, d_idx,
                                d_dist, d_count, s);
@@ -625,8 +652,7 @@ class BruteForce final : public Index {
         }
         off[0] = 0;
         for (uint64_t q = 0; q < Q; ++q) off[q + 1] = off[q] + kq[q];
-        const bool shortlist_form = mode_ != HM_SEARCH_EXACT && shortlist_metric() &&
-                                    dpad_ <= 4096 && n_ > 0 && kmax > 0;
+        const bool shortlist_form = kmax > 0 && shortlist_usable(s);
         rg_kq_.reserve(Q * 4);
         rg_off_.reserve((Q + 1) * 8);
         HM_HIP(hipMemcpyAsync(rg_kq_.get(), kq.data(), Q * 4, hipMemcpyHostToDevice, s));
@@ -677,13 +703,17 @@ class BruteForce final : public Index {
         std::vector<uint32_t> fl(Q);
         HM_HIP(hipMemcpyAsync(fl.data(), rg_flags_all_.get(), Q * 4, hipMemcpyDeviceToHost, s));
         HM_HIP(hipStreamSynchronize(s));
-        std::vector<uint32_t> redo;
+        std::vector<uint32_t> redo, tied;
         last_uncertified_ = 0;
         for (uint64_t q = 0; q < Q; ++q) {
-            if (fl[q] & (ROW_TIE | ROW_UNCERT | ROW_BAD)) redo.push_back((uint32_t)q);
+            if (fl[q] == ROW_TIE)
+                tied.push_back((uint32_t)q);
+            else if (fl[q] & (ROW_TIE | ROW_NAN | ROW_UNCERT | ROW_BAD))
+                redo.push_back((uint32_t)q);
             last_uncertified_ += (fl[q] & ROW_UNCERT) ? 1 : 0;
         }
-        last_fallback_ = redo.size();
+        last_fallback_ = redo.size() + tied.size();
+        if (!tied.empty()) ragged_exact(dq, tied, kmax, d_idx, d_dist, d_count, s, true);
         if (!redo.empty()) ragged_exact(dq, redo, kmax, d_idx, d_dist, d_count, s);
         uint32_t err = 0;
         HM_HIP(hipMemcpyAsync(&err, err_.get(), 4, hipMemcpyDeviceToHost, s));
@@ -748,9 +778,9 @@ class BruteForce final : public Index {
     size_t d_budget_ = (size_t)4 << 30;
     DeviceBuffer base_, bnorm_;
     bool bnorm_valid_ = false;
-    DeviceBuffer qpad_, qnorm_, D_, cand_d_, cand_i_, flags_, err_, heap_d_, heap_i_;
+    DeviceBuffer qpad_, qnorm_, D_, cand_d_, cand_i_, flags_, err_;
     // shortlist forms
-    bool sl_valid_ = false, base_inexact_ = true;
+    bool sl_valid_ = false, base_inexact_ = true, zero_norm_ = false;
     int certify_ = 1;
     uint32_t dk_ = 0;
     float bmax_ = 0;

@@ -198,6 +198,51 @@ __global__ __launch_bounds__(256, 2) void exact_tile_kernel(
     }
 }
 
+// Few queries (a fallback batch): one lane per base row, the RQ query rows
+// in LDS (broadcast reads), the base row streamed with 16-byte loads.  The
+// same left folds and epilogues as exact_tile_kernel, so the values are the
+// same bits; the 128-query tile would leave most of its work idle here.
+template <int ACC, int EPI, int RQ>
+__global__ __launch_bounds__(256) void exact_rows_kernel(
+    const float *__restrict__ Q, uint32_t ldq, uint32_t nq, const float *__restrict__ B,
+    uint32_t ldb, uint32_t n, uint32_t dpad, const float *__restrict__ qnorm,
+    const float *__restrict__ bnorm, float *__restrict__ D, uint64_t ldd)
+{
+    extern __shared__ __align__(16) float xr_q[];  // [RQ][dpad]
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t e = tid; e < RQ * dpad; e += 256) {
+        const uint32_t r = e / dpad, t = e % dpad;
+        xr_q[e] = r < nq ? Q[(size_t)r * ldq + t] : 0.0f;
+    }
+    __syncthreads();
+    const uint32_t j = blockIdx.x * 256 + tid;
+    const float *b = B + (size_t)(j < n ? j : n - 1) * ldb;
+    float acc[RQ];
+#pragma unroll
+    for (int r = 0; r < RQ; ++r) acc[r] = -0.0f;  // Rust's f32 Sum starts at -0.0
+#pragma unroll 4
+    for (uint32_t t = 0; t < dpad; t += 4) {
+        const float4 bv = *reinterpret_cast<const float4 *>(b + t);
+#pragma unroll
+        for (int r = 0; r < RQ; ++r) {
+            const float4 qv = *reinterpret_cast<const float4 *>(xr_q + r * dpad + t);
+            acc[r] = acc_step<ACC>(acc[r], qv.x, bv.x);
+            acc[r] = acc_step<ACC>(acc[r], qv.y, bv.y);
+            acc[r] = acc_step<ACC>(acc[r], qv.z, bv.z);
+            acc[r] = acc_step<ACC>(acc[r], qv.w, bv.w);
+        }
+    }
+    if (j >= n) return;
+#pragma unroll
+    for (int r = 0; r < RQ; ++r) {
+        if ((uint32_t)r >= nq) break;
+        float a = acc[r];
+        if constexpr (EPI == EPI_COSINE) a = epi_cosine(a, qnorm[r], bnorm[j]);
+        else if constexpr (EPI == EPI_ANGULAR) a = epi_angular(a, qnorm[r], bnorm[j]);
+        D[(size_t)r * ldd + j] = a;
+    }
+}
+
 // s = sum_t x_t*x_t as a left fold from -0.0 (the reference's dot(x,x) up to
 // sign, calc.rs:13-18), one lane per row.
 __global__ void self_dot_kernel(const float *__restrict__ X, uint32_t ldx, uint32_t rows,
@@ -234,6 +279,37 @@ static void launch_exact_tile_impl(int metric, const float *Q, uint32_t ldq, uin
                                            D ? D + (size_t)q0 * ldd : nullptr, ldd,
                                            best ? best + q0 : nullptr, s);
         }
+        return;
+    }
+    if (!ARGMIN && nq <= 8 && dpad % 4 == 0 && ldb % 4 == 0 && (size_t)8 * dpad * 4 <= 65536 &&
+        ((uintptr_t)B & 15) == 0) {
+        ProfScope prof("exact_rows", s);
+        const unsigned blocks = ceil_div(n, 256);
+#define HM_ROWS(ACC, EPI)                                                                      \
+    do {                                                                                       \
+        if (nq == 1)                                                                           \
+            exact_rows_kernel<ACC, EPI, 1><<<blocks, 256, (size_t)dpad * 4, s>>>(              \
+                Q, ldq, nq, B, ldb, n, dpad, qnorm, bnorm, D, ldd);                            \
+        else if (nq <= 2)                                                                      \
+            exact_rows_kernel<ACC, EPI, 2><<<blocks, 256, (size_t)2 * dpad * 4, s>>>(          \
+                Q, ldq, nq, B, ldb, n, dpad, qnorm, bnorm, D, ldd);                            \
+        else if (nq <= 4)                                                                      \
+            exact_rows_kernel<ACC, EPI, 4><<<blocks, 256, (size_t)4 * dpad * 4, s>>>(          \
+                Q, ldq, nq, B, ldb, n, dpad, qnorm, bnorm, D, ldd);                            \
+        else                                                                                   \
+            exact_rows_kernel<ACC, EPI, 8><<<blocks, 256, (size_t)8 * dpad * 4, s>>>(          \
+                Q, ldq, nq, B, ldb, n, dpad, qnorm, bnorm, D, ldd);                            \
+    } while (0)
+        switch (metric) {
+        case HM_METRIC_EUCLIDEAN: HM_ROWS(ACC_EUCLID, EPI_NONE); break;
+        case HM_METRIC_MANHATTAN: HM_ROWS(ACC_MANHATTAN, EPI_NONE); break;
+        case HM_METRIC_DOT_PRODUCT: HM_ROWS(ACC_DOT, EPI_NONE); break;
+        case HM_METRIC_COSINE_SIMILARITY: HM_ROWS(ACC_DOT, EPI_COSINE); break;
+        case HM_METRIC_ANGULAR: HM_ROWS(ACC_DOT, EPI_ANGULAR); break;
+        default: fail(HM_ERR_UNKNOWN_METRIC, "unknown method");
+        }
+#undef HM_ROWS
+        HM_CHECK_LAUNCH();
         return;
     }
     dim3 grid(ceil_div(n, TN), ceil_div(nq, TQ));

@@ -48,8 +48,7 @@ void launch_row_select(const float *D, uint64_t ldd, uint32_t rows, uint32_t n, 
 void launch_bf_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ldc,
                         const uint32_t *flags, const float *D, uint64_t ldd, uint32_t rows,
                         uint32_t n, uint32_t k, uint64_t *out_idx, float *out_dist,
-                        uint32_t *out_count, uint32_t *err, float *heap_d, uint32_t *heap_i,
-                        hipStream_t s);
+                        uint32_t *out_count, uint32_t *err, hipStream_t s);
 
 // Non-finite check of a device matrix (rows x d, stride ld): sets *err bit 1.
 void launch_check_finite(const float *X, uint64_t ld, uint64_t rows, uint32_t d, uint32_t *err,
@@ -84,7 +83,9 @@ void launch_to_bf16(const float *src, uint64_t lds, uint64_t rows, uint32_t d, u
                     const uint32_t *dev_rows = nullptr);
 void launch_to_fp8(const float *src, uint64_t lds, uint32_t rows, uint32_t d, uint32_t dk,
                    uint8_t *dst, float *dequant, hipStream_t s, const uint32_t *dev_rows = nullptr);
-void launch_norm_stats(const float *norm2, uint32_t rows, float *inv, uint32_t *maxbits,
+// inv[r] = 1/sqrt(norm2[r]); stats[0] |= max norm (float bits, atomicMax),
+// stats[2] |= 1 when a norm is zero.  stats[0..2] zeroed by the caller.
+void launch_norm_stats(const float *norm2, uint32_t rows, float *inv, uint32_t *stats,
                        hipStream_t s);
 
 // ---- rerank.hip

@@ -307,20 +307,24 @@ __global__ void to_fp8_kernel(const float *__restrict__ src, uint64_t lds, uint3
     }
 }
 
-// inv[r] = 1 / sqrt(norm2[r]); maxnorm = max_r sqrt(norm2[r]) (as float bits).
+// inv[r] = 1 / sqrt(norm2[r]); stats[0] = max_r sqrt(norm2[r]) (as float
+// bits); stats[2] = 1 if some row has norm zero.
 __global__ void norm_stats_kernel(const float *__restrict__ norm2, uint32_t rows,
-                                  float *__restrict__ inv, uint32_t *maxbits)
+                                  float *__restrict__ inv, uint32_t *stats)
 {
     float m = 0.0f;
+    uint32_t zero = 0;
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < rows;
          r += gridDim.x * blockDim.x) {
         float s = sqrtf(norm2[r]);
         if (inv) inv[r] = 1.0f / s;
         m = fmaxf(m, s);
+        zero |= s == 0.0f ? 1u : 0u;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(maxbits, __float_as_uint(m));
+    if ((threadIdx.x & 63) == 0) atomicMax(stats, __float_as_uint(m));
+    if (zero) atomicOr(stats + 2, 1u);
 }
 
 }  // namespace
@@ -420,12 +424,12 @@ void launch_to_fp8(const float *src, uint64_t lds, uint32_t rows, uint32_t d, ui
     HM_CHECK_LAUNCH();
 }
 
-void launch_norm_stats(const float *norm2, uint32_t rows, float *inv, uint32_t *maxbits,
+void launch_norm_stats(const float *norm2, uint32_t rows, float *inv, uint32_t *stats,
                        hipStream_t s)
 {
     if (rows == 0) return;
     unsigned blocks = std::min<unsigned>(ceil_div(rows, 256), 1024);
-    norm_stats_kernel<<<blocks, 256, 0, s>>>(norm2, rows, inv, maxbits);
+    norm_stats_kernel<<<blocks, 256, 0, s>>>(norm2, rows, inv, stats);
     HM_CHECK_LAUNCH();
 }
 

@@ -48,7 +48,7 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
     if (dev_rows && blockIdx.x >= *dev_rows) return;  // graph replays with fewer rows
     // dynamic LDS: keys[P] (u64) | vals[P] | sq[dpad]; P covers the largest s
     extern __shared__ __align__(16) unsigned char rr_smem[];
-    __shared__ uint32_t s_tie, s_bad;
+    __shared__ uint32_t s_tie, s_bad, s_nan;
     uint32_t P = 1;
     while (P < s) P <<= 1;
     if (s_q) s = min(s, s_q[blockIdx.x]);  // this query's shortlist length
@@ -61,6 +61,7 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
     if (tid == 0) {
         s_tie = 0;
         s_bad = 0;
+        s_nan = 0;
     }
     for (uint32_t t = tid; t < dpad; t += RR_THREADS) sq[t] = Qp[(size_t)r * dpad + t];
     __syncthreads();
@@ -110,6 +111,7 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
                 E = 1.0f;
             }
         }
+        if (E != E) s_nan = 1;  // the exact path reports it as the reference fails
         vals[j] = E;
         keys[j] = ((unsigned long long)fkey(E) << 32) | j;
     }
@@ -158,10 +160,13 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
     __syncthreads();
     if (tid != 0) return;
     out_count[r] = m < s ? m : s;
-    uint32_t f = (s_tie ? ROW_TIE : 0u) | (s_bad ? ROW_BAD : 0u) | (pre_flags ? pre_flags[r] : 0u);
-    if (s < n && certify) {
+    uint32_t f = (s_tie ? ROW_TIE : 0u) | (s_bad ? ROW_BAD : 0u) | (s_nan ? ROW_NAN : 0u) |
+                 (pre_flags ? pre_flags[r] : 0u);
+    if (s < n && k + 1 > s) {
+        f |= ROW_UNCERT;  // too few candidates to hold the k+1 nearest
+    } else if (s < n && certify) {
         bool ok = false;
-        if (k + 1 <= s) {
+        {
             const float ek1 = vals[(uint32_t)keys[k]];
             const float key_s = key_s_q ? key_s_q[r] : cand_key[(size_t)r * ldc + s - 1];
             const double c_in = q_inexact && *q_inexact ? c_in_inexact : c_in_exact;

@@ -54,6 +54,46 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_w, u
     return before + x - v;
 }
 
+// Visits every element of a row once, in no particular order: f(value,
+// index).  Several loads per thread are in flight before any is used; VEC
+// uses 16-byte loads (row 16-byte aligned, ldd a multiple of 4 and >= n, so
+// a load clamped to the row's last full vector stays inside the row).
+template <bool VEC, class F>
+__device__ __forceinline__ void row_for_each(const float *__restrict__ row, uint32_t n, uint64_t ldd,
+                                             int tid, F &&f)
+{
+    if constexpr (VEC) {
+        constexpr uint32_t U = 4, STEP = SEL_THREADS * 4 * U;
+        for (uint64_t b = 0; b < n; b += STEP) {
+            float4 v[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint64_t e = min(b + u * SEL_THREADS * 4 + 4 * (uint32_t)tid, ldd - 4);
+                v[u] = *reinterpret_cast<const float4 *>(row + e);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint64_t e = b + u * SEL_THREADS * 4 + 4 * (uint32_t)tid;
+                if (e < n) f(v[u].x, (uint32_t)e);
+                if (e + 1 < n) f(v[u].y, (uint32_t)e + 1);
+                if (e + 2 < n) f(v[u].z, (uint32_t)e + 2);
+                if (e + 3 < n) f(v[u].w, (uint32_t)e + 3);
+            }
+        }
+    } else {
+        constexpr uint32_t U = 8;
+        for (uint64_t b = tid; b < n; b += SEL_THREADS * U) {
+            float v[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) v[u] = row[min(b + u * SEL_THREADS, (uint64_t)n - 1)];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+                if (b + u * SEL_THREADS < n) f(v[u], (uint32_t)(b + u * SEL_THREADS));
+        }
+    }
+}
+
+template <bool VEC>
 __global__ __launch_bounds__(SEL_THREADS) void row_select_kernel(
     const float *__restrict__ D, uint64_t ldd, uint32_t n, uint32_t K, float *__restrict__ cand_d,
     uint32_t *__restrict__ cand_i, uint32_t ldc, uint32_t *__restrict__ flags,
@@ -89,12 +129,11 @@ __global__ __launch_bounds__(SEL_THREADS) void row_select_kernel(
         __syncthreads();
         const uint32_t prefix = s_prefix, remaining = s_remaining;
         uint32_t nan_local = 0;
-        for (uint32_t i = tid; i < n; i += SEL_THREADS) {
-            float f = row[i];
+        row_for_each<VEC>(row, n, ldd, tid, [&](float f, uint32_t) {
             if (pass == 0 && f != f) nan_local = 1;
-            uint32_t key = fkey(f);
+            const uint32_t key = fkey(f);
             if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & (nb - 1)], 1u);
-        }
+        });
         if (nan_local) s_nan = 1;
         __syncthreads();
         const uint32_t per = nb / SEL_THREADS;
@@ -129,13 +168,13 @@ __global__ __launch_bounds__(SEL_THREADS) void row_select_kernel(
     if (early) {
         // every key whose top bits are <= the target prefix
         total = (K - rem) + ceq;
-        for (uint32_t i = tid; i < n; i += SEL_THREADS) {
-            uint32_t key = fkey(row[i]);
+        row_for_each<VEC>(row, n, ldd, tid, [&](float f, uint32_t i) {
+            const uint32_t key = fkey(f);
             if ((key & mask) <= T) {
-                uint32_t pos = atomicAdd(&s_cnt, 1u);
+                const uint32_t pos = atomicAdd(&s_cnt, 1u);
                 if (pos < SEL_CAP) cand[pos] = ((unsigned long long)key << 32) | i;
             }
-        }
+        });
         __syncthreads();
         bad = s_cnt != total;
     } else {
@@ -143,13 +182,13 @@ __global__ __launch_bounds__(SEL_THREADS) void row_select_kernel(
         total = K;
         const uint32_t nlt = K - rem;
         const bool all_eq = (ceq == rem);
-        for (uint32_t i = tid; i < n; i += SEL_THREADS) {
-            uint32_t key = fkey(row[i]);
+        row_for_each<VEC>(row, n, ldd, tid, [&](float f, uint32_t i) {
+            const uint32_t key = fkey(f);
             if (key < T || (all_eq && key == T)) {
-                uint32_t pos = atomicAdd(&s_cnt, 1u);
+                const uint32_t pos = atomicAdd(&s_cnt, 1u);
                 if (pos < SEL_CAP) cand[pos] = ((unsigned long long)key << 32) | i;
             }
-        }
+        });
         __syncthreads();
         bad = s_cnt != (all_eq ? K : nlt);
         if (!all_eq && !bad) {
@@ -277,6 +316,48 @@ struct LaneHeap {
 // ---- std BinaryHeap held across the lanes of one wave: lane j owns slot j.
 // Every operation runs with uniform control flow; a slot is read with
 // v_readlane (uniform index) and written by the owning lane.
+// Streams a distance row through LDS in batches of 4096 values (16 float4
+// loads per lane, the next batch in flight while the current one is
+// processed) and hands it over in chunks of up to 64 consecutive values:
+// on_chunk(first_index, values_in_lds, count).  Rows are 16-byte aligned
+// with a stride ldd >= n that is a multiple of 4, so the vector loads
+// (clamped to the last full vector of the row) stay inside the row.
+template <class OnChunk>
+__device__ void stream_row(const float *__restrict__ row, uint32_t n, uint64_t ldd, int lane,
+                           float *sv, OnChunk &&on_chunk)
+{
+    constexpr uint32_t FB4 = 16, BATCH = FB4 * 256;
+    float nx[4 * FB4];  // scalar array: promoted to registers once the loops unroll
+#define HM_ROW_LOAD(b)                                                                     \
+    _Pragma("clang loop unroll(full)") for (uint32_t c4 = 0; c4 < FB4; ++c4)               \
+    {                                                                                      \
+        const uint64_t e = min((uint64_t)(b) + c4 * 256 + 4 * (uint32_t)lane, ldd - 4);    \
+        const float4 t = *reinterpret_cast<const float4 *>(row + e);                       \
+        nx[4 * c4] = t.x;                                                                  \
+        nx[4 * c4 + 1] = t.y;                                                              \
+        nx[4 * c4 + 2] = t.z;                                                              \
+        nx[4 * c4 + 3] = t.w;                                                              \
+    }
+    HM_ROW_LOAD(0u)
+    for (uint32_t b0 = 0; b0 < n; b0 += BATCH) {
+        __syncthreads();
+#pragma clang loop unroll(full)
+        for (uint32_t c4 = 0; c4 < FB4; ++c4)
+            reinterpret_cast<float4 *>(sv)[c4 * 64 + lane] =
+                make_float4(nx[4 * c4], nx[4 * c4 + 1], nx[4 * c4 + 2], nx[4 * c4 + 3]);
+        __syncthreads();
+        if (n - b0 > BATCH) {
+            HM_ROW_LOAD(b0 + BATCH)
+        }
+        for (uint32_t c = 0; c < BATCH / 64; ++c) {
+            const uint32_t base = b0 + c * 64;
+            if (base >= n) break;
+            on_chunk(base, sv + c * 64, n - base < 64 ? n - base : 64u);
+        }
+    }
+#undef HM_ROW_LOAD
+}
+
 struct WaveHeap {
     float d;     // this lane's slot
     uint32_t i;  // this lane's id
@@ -396,61 +477,41 @@ __global__ __launch_bounds__(64) void bf_finalize_wave_kernel(
     const float *row = D + (size_t)r * ldd;
     bool stable = false;
     float root = inf;
-    // The row streams through LDS in batches of FB x 64 values; the next
-    // batch's loads are in flight while the current one is replayed.  While
-    // the heap is full and its layout is stable, a ballot finds the next
-    // element that is not greater than the root (the only ones that change
-    // anything), so the rejected majority costs one ballot per 64 values.
-    constexpr uint32_t FB = 16;
-    __shared__ float sv[FB * 64];
-    float nx[FB];
-#pragma unroll
-    for (uint32_t c = 0; c < FB; ++c) nx[c] = row[min((uint32_t)(c * 64 + lane), n - 1)];
-    for (uint32_t b0 = 0; b0 < n; b0 += FB * 64) {
-#pragma unroll
-        for (uint32_t c = 0; c < FB; ++c) sv[c * 64 + lane] = nx[c];
-        __syncthreads();
-        const uint32_t b1 = b0 + FB * 64;
-        if (b1 < n) {
-#pragma unroll
-            for (uint32_t c = 0; c < FB; ++c) nx[c] = row[min(b1 + c * 64 + lane, n - 1)];
-        }
-        for (uint32_t c = 0; c < FB; ++c) {
-            const uint32_t base = b0 + c * 64;
-            if (base >= n) break;
-            const float v = sv[c * 64 + lane];
-            const uint32_t cnt = n - base < 64 ? n - base : 64;
-            uint32_t j = 0;
-            while (j < cnt) {
-                if (h.len == k && stable) {
-                    const unsigned long long live =
-                        __ballot((uint32_t)lane >= j && (uint32_t)lane < cnt && !(v > root));
-                    if (live == 0) break;
-                    j = (uint32_t)__builtin_ctzll(live);
-                }
-                const float x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)j));
-                h.push(x, base + j);
-                if (h.len > k) {
-                    float a;
-                    uint32_t b;
-                    h.pop(&a, &b);
-                }
-                if (h.len == k) {
-                    root = h.gd(0);
-                    // would a larger element's push + pop leave this layout as is?
-                    WaveHeap t = h;
-                    t.push(inf, 0xFFFFFFFFu);
-                    float a;
-                    uint32_t b;
-                    t.pop(&a, &b);
-                    const bool same = (uint32_t)lane >= k || (t.d == h.d && t.i == h.i);
-                    stable = root < inf && __all(same);
-                }
-                ++j;
+    // While the heap is full and its layout is stable, a ballot finds the
+    // next element that is not greater than the root (the only ones that
+    // change anything), so the rejected majority costs one ballot per 64.
+    __shared__ __align__(16) float sv[4096];
+    stream_row(row, n, ldd, lane, sv, [&](uint32_t base, const float *cv, uint32_t cnt) {
+        const float v = cv[lane];
+        uint32_t j = 0;
+        while (j < cnt) {
+            if (h.len == k && stable) {
+                const unsigned long long live =
+                    __ballot((uint32_t)lane >= j && (uint32_t)lane < cnt && !(v > root));
+                if (live == 0) break;
+                j = (uint32_t)__builtin_ctzll(live);
             }
+            const float x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)j));
+            h.push(x, base + j);
+            if (h.len > k) {
+                float a;
+                uint32_t b;
+                h.pop(&a, &b);
+            }
+            if (h.len == k) {
+                root = h.gd(0);
+                // would a larger element's push + pop leave this layout as is?
+                WaveHeap t = h;
+                t.push(inf, 0xFFFFFFFFu);
+                float a;
+                uint32_t b;
+                t.pop(&a, &b);
+                const bool same = (uint32_t)lane >= k || (t.d == h.d && t.i == h.i);
+                stable = root < inf && __all(same);
+            }
+            ++j;
         }
-        __syncthreads();
-    }
+    });
     const uint32_t cntk = h.len;
     for (uint32_t p = cntk; p > 0; --p) {
         float a;
@@ -468,77 +529,115 @@ __global__ __launch_bounds__(64) void bf_finalize_wave_kernel(
     if (lane == 0) out_count[r] = cntk;
 }
 
-template <bool LDS_HEAP>
-__global__ __launch_bounds__(64) void bf_finalize_kernel(
-    const float *__restrict__ cand_d, const uint32_t *__restrict__ cand_i, uint32_t ldc,
-    const uint32_t *__restrict__ flags, const float *__restrict__ D, uint64_t ldd, uint32_t rows,
-    uint32_t n, uint32_t k, uint64_t *__restrict__ out_idx, float *__restrict__ out_dist,
-    uint32_t *__restrict__ out_count, uint32_t *err, float *heap_d, uint32_t *heap_i)
+// Would pushing an element larger than every entry and popping it again
+// leave this (full) heap's layout as it is?  The push shifts the path from
+// the new leaf L to the root down one level; the pop's sift-down retraces
+// that path exactly when, at every level where the path runs through a
+// left child that has a right sibling, the parent is strictly greater than
+// the sibling (on equality the sift-down takes the right child), and, at
+// the bottom, when L is a right child, its parent is strictly greater than
+// the left sibling L-1.  Checked against a direct simulation of the two
+// operations on 200,000 random heaps with heavy ties.
+__device__ bool heap_noop_roundtrip(const float *d, uint32_t L)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const uint32_t r = blockIdx.x * 64 + threadIdx.x;
-    if (r >= rows) return;
+    if (L == 0) return true;
+    uint32_t a = L, p = (a - 1) / 2;
+    if (a == 2 * p + 2 && !(d[p] > d[L - 1])) return false;
+    a = p;
+    while (a > 0) {
+        p = (a - 1) / 2;
+        if (a == 2 * p + 1 && a + 1 < L && !(d[p] > d[a + 1])) return false;
+        a = p;
+    }
+    return true;
+}
+
+// Replays bruteforce_idx.rs:47-73 over a tied row for k + 1 > 64, with the
+// heap in LDS.  Every lane of the single wave executes the same heap
+// operations on the same LDS words (uniform control flow, broadcast reads),
+// and the lanes cooperate only to scan: while the heap is full and the
+// round trip of a larger element is a no-op, a ballot over 64 values finds
+// the next element that is not greater than the root.
+__global__ __launch_bounds__(64) void bf_replay_lds_kernel(
+    const float *__restrict__ cand_d, const uint32_t *__restrict__ cand_i, uint32_t ldc,
+    const uint32_t *__restrict__ flags, const float *__restrict__ D, uint64_t ldd, uint32_t n,
+    uint32_t k, uint64_t *__restrict__ out_idx, float *__restrict__ out_dist,
+    uint32_t *__restrict__ out_count, uint32_t *err)
+{
+    extern __shared__ __align__(16) unsigned char rp_smem[];
+    const uint32_t r = blockIdx.x;
+    const int lane = threadIdx.x;
     const float inf = __int_as_float(0x7f800000);
     uint64_t *oi = out_idx + (size_t)r * k;
     float *od = out_dist + (size_t)r * k;
     const uint32_t m = k < n ? k : n;
     const uint32_t f = flags[r];
-    if (k == 0) {
-        out_count[r] = 0;
-        return;
-    }
     if ((f & (ROW_NAN | ROW_BAD)) && n >= 2) {
-        atomicOr(err, (f & ROW_BAD) ? 4u : 1u);
-        for (uint32_t j = 0; j < k; ++j) {
+        if (lane == 0) atomicOr(err, (f & ROW_BAD) ? 4u : 1u);
+        for (uint32_t j = lane; j < k; j += 64) {
             oi[j] = ~0ull;
             od[j] = inf;
         }
-        out_count[r] = 0;
+        if (lane == 0) out_count[r] = 0;
         return;
     }
-    uint32_t cnt = m;
     if (!(f & ROW_TIE)) {
         const float *cd = cand_d + (size_t)r * ldc;
         const uint32_t *ci = cand_i + (size_t)r * ldc;
-        for (uint32_t j = 0; j < m; ++j) {
-            oi[j] = ci[j];
-            od[j] = cd[j];
+        for (uint32_t j = lane; j < k; j += 64) {
+            oi[j] = j < m ? (uint64_t)ci[j] : ~0ull;
+            od[j] = j < m ? cd[j] : inf;
         }
-    } else {
-        LaneHeap h;
-        if constexpr (LDS_HEAP) {
-            float *base_d = reinterpret_cast<float *>(smem);
-            uint32_t *base_i = reinterpret_cast<uint32_t *>(smem + (size_t)64 * (k + 1) * 4);
-            h.d = base_d + threadIdx.x * (k + 1);
-            h.ix = base_i + threadIdx.x * (k + 1);
-        } else {
-            h.d = heap_d + (size_t)r * (k + 1);
-            h.ix = heap_i + (size_t)r * (k + 1);
-        }
-        h.len = 0;
-        const float *row = D + (size_t)r * ldd;
-        for (uint32_t i = 0; i < n; ++i) {
-            h.push(row[i], i);
-            if (h.len > k) {
-                float xd;
-                uint32_t xi;
-                h.pop(&xd, &xi);
+        if (lane == 0) out_count[r] = m;
+        return;
+    }
+    LaneHeap h;
+    h.d = reinterpret_cast<float *>(rp_smem);
+    h.ix = reinterpret_cast<uint32_t *>(rp_smem + (size_t)(k + 1) * 4);
+    h.len = 0;
+    const float *row = D + (size_t)r * ldd;
+    bool stable = false;
+    float root = inf;
+    __shared__ __align__(16) float sv[4096];
+    stream_row(row, n, ldd, lane, sv, [&](uint32_t base, const float *cv, uint32_t cnt) {
+        const float v = cv[lane];
+        uint32_t j = 0;
+        while (j < cnt) {
+            if (h.len == k && stable) {
+                const unsigned long long live =
+                    __ballot((uint32_t)lane >= j && (uint32_t)lane < cnt && !(v > root));
+                if (live == 0) break;
+                j = (uint32_t)__builtin_ctzll(live);
             }
+            const float x = cv[j];
+            h.push(x, base + j);
+            if (h.len > k) {
+                float a;
+                uint32_t b;
+                h.pop(&a, &b);
+            }
+            if (h.len == k) {
+                root = h.d[0];
+                stable = root < inf && heap_noop_roundtrip(h.d, h.len);
+            }
+            ++j;
         }
-        cnt = h.len;
-        for (uint32_t p = cnt; p > 0; --p) {
-            float xd;
-            uint32_t xi;
-            h.pop(&xd, &xi);
-            oi[p - 1] = xi;
-            od[p - 1] = xd;
+    });
+    const uint32_t cntk = h.len;
+    for (uint32_t p = cntk; p > 0; --p) {
+        float a;
+        uint32_t b;
+        h.pop(&a, &b);
+        if (lane == 0) {
+            oi[p - 1] = b;
+            od[p - 1] = a;
         }
     }
-    for (uint32_t j = cnt; j < k; ++j) {
+    for (uint32_t j = cntk + lane; j < k; j += 64) {
         oi[j] = ~0ull;
         od[j] = inf;
     }
-    out_count[r] = cnt;
+    if (lane == 0) out_count[r] = cntk;
 }
 
 __global__ void check_finite_kernel(const float *__restrict__ X, uint64_t ld, uint64_t rows,
@@ -576,36 +675,35 @@ void launch_row_select(const float *D, uint64_t ldd, uint32_t rows, uint32_t n, 
 {
     if (rows == 0 || n == 0 || K == 0) return;
     if (K > SEL_CAP || K > n) fail(HM_ERR_INVALID_ARG, "row_select: K out of range");
+    if (ldd < n) fail(HM_ERR_INVALID_ARG, "row_select: row stride below row length");
     ProfScope prof("row_select", s);
-    row_select_kernel<<<rows, SEL_THREADS, 0, s>>>(D, ldd, n, K, cand_d, cand_i, ldc, flags,
-                                                   dev_rows);
+    if (ldd % 4 == 0 && ((uintptr_t)D & 15) == 0)
+        row_select_kernel<true><<<rows, SEL_THREADS, 0, s>>>(D, ldd, n, K, cand_d, cand_i, ldc,
+                                                             flags, dev_rows);
+    else
+        row_select_kernel<false><<<rows, SEL_THREADS, 0, s>>>(D, ldd, n, K, cand_d, cand_i, ldc,
+                                                              flags, dev_rows);
     HM_CHECK_LAUNCH();
 }
 
 void launch_bf_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ldc,
                         const uint32_t *flags, const float *D, uint64_t ldd, uint32_t rows,
                         uint32_t n, uint32_t k, uint64_t *out_idx, float *out_dist,
-                        uint32_t *out_count, uint32_t *err, float *heap_d, uint32_t *heap_i,
-                        hipStream_t s)
+                        uint32_t *out_count, uint32_t *err, hipStream_t s)
 {
     if (rows == 0) return;
+    if (k == 0) fail(HM_ERR_INVALID_ARG, "bf_finalize: k is zero");
+    if (k + 1 > SELECT_MAX_K) fail(HM_ERR_INVALID_ARG, "bf_finalize: k too large");
+    if (ldd % 4 || ldd < n || ((uintptr_t)D & 15))
+        fail(HM_ERR_INVALID_ARG, "bf_finalize: rows must be 16-byte aligned, stride a multiple of 4");
     ProfScope prof("bf_finalize", s);
-    if (k >= 1 && k + 1 <= 64) {
+    if (k + 1 <= 64) {
         bf_finalize_wave_kernel<<<rows, 64, 0, s>>>(cand_d, cand_i, ldc, flags, D, ldd, n, k,
                                                      out_idx, out_dist, out_count, err);
-        HM_CHECK_LAUNCH();
-        return;
-    }
-    size_t lds = (size_t)64 * (k + 1) * 8;
-    if (lds <= 64 * 1024) {
-        bf_finalize_kernel<true><<<ceil_div(rows, 64), 64, lds, s>>>(
-            cand_d, cand_i, ldc, flags, D, ldd, rows, n, k, out_idx, out_dist, out_count, err,
-            heap_d, heap_i);
     } else {
-        if (!heap_d || !heap_i) fail(HM_ERR_INVALID_ARG, "bf_finalize: heap scratch missing");
-        bf_finalize_kernel<false><<<ceil_div(rows, 64), 64, 0, s>>>(
-            cand_d, cand_i, ldc, flags, D, ldd, rows, n, k, out_idx, out_dist, out_count, err,
-            heap_d, heap_i);
+        const size_t lds = (size_t)(k + 1) * 8;  // <= 64 KiB
+        bf_replay_lds_kernel<<<rows, 64, lds, s>>>(cand_d, cand_i, ldc, flags, D, ldd, n, k,
+                                                   out_idx, out_dist, out_count, err);
     }
     HM_CHECK_LAUNCH();
 }

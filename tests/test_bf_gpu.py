@@ -69,6 +69,19 @@ def test_ties_replay_the_std_heap(orc, k):
         _check(orc, base, q, k, mt)
 
 
+@pytest.mark.parametrize("k", [63, 64, 200])
+def test_ties_replay_long_rows(orc, k):
+    # 30,000 rows in equal triples: the replay streams many batches of the
+    # row; k = 63 keeps the heap in the lanes of a wave, 64 and 200 in LDS
+    rng = np.random.default_rng(11)
+    uniq = rng.integers(0, 6, (10_000, 6)).astype(np.float32)
+    base = np.repeat(uniq, 3, axis=0)
+    rng.shuffle(base)
+    q = rng.integers(0, 6, (48, 6)).astype(np.float32)
+    for mt in (Metric.Euclidean, Metric.Manhattan):
+        _check(orc, base, q, k, mt)
+
+
 def test_hand_traced_tie_case(orc):
     # distances [1,1,1], k=2: the reference returns ids [2, 1]
     base = np.array([[1.0], [1.0], [1.0]], np.float32)

@@ -106,3 +106,25 @@ def test_fp8_recall_and_exact_distances(orc, mt):
     rows = orc.distance_rows(base, q, int(mt))
     for r in range(200):
         assert np.array_equal(gd[r], rows[r][gi[r].astype(np.int64)])
+
+
+def test_nan_distances_fail_like_the_exact_form():
+    # Cosine with a zero-norm row or query is NaN for that pair, which the
+    # reference cannot order (neighbor.rs:32): the shortlist forms report it
+    # exactly as the exact form does
+    from hora_amd import HoraError
+    base = bf_normal(6_000, 32, seed=451)
+    q = bf_normal(4, 32, seed=452)
+    for mode in (hora_amd.HM_SEARCH_BF16_RERANK, hora_amd.HM_SEARCH_FP8_RERANK):
+        idx = _index(base, Metric.CosineSimilarity, mode)
+        zq = q.copy()
+        zq[2] = 0.0
+        with pytest.raises(HoraError) as e:
+            idx.search_batch(zq, 5)
+        assert e.value.status == 8
+        zb = base.copy()
+        zb[1234] = 0.0
+        idx = _index(zb, Metric.CosineSimilarity, mode)
+        with pytest.raises(HoraError) as e:
+            idx.search_batch(q, 5)
+        assert e.value.status == 8
