@@ -40,7 +40,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="bf", choices=["bf", "ivfpq", "ragged", "probe"])
+    ap.add_argument("--workload", default="bf",
+                    choices=["bf", "ivfpq", "ragged", "probe", "copy"])
     ap.add_argument("--forms", default="exact,bf16,fp8")
     ap.add_argument("--cpu-sample", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -127,8 +128,8 @@ def main_bf(args):
     for form in args.forms.split(","):
         ix = make(Metric.Euclidean, form)
         wall, ev = timer.run(step_fn(ix), args.steps, args.warmup, f"Euclidean/{form}")
-        km = kernel_ms(["exact_tile", "row_select", "bf_finalize", "score_gemm", "score_filter",
-                        "list_prepare", "rerank"])
+        km = kernel_ms(["exact_tile", "exact_rows", "row_select", "bf_finalize", "score_gemm",
+                        "score_filter", "list_prepare", "rerank"])
         fb = ix.search_stats()
         ids_by_form[form] = d_idx.cpu().numpy().view(np.uint64).copy()
         forms[form] = {"qps": round(args.steps * NQ / wall, 1),
@@ -426,6 +427,23 @@ def main_probe(args):
                       "probe_loss": round(loss, 4), "results": res}), flush=True)
 
 
+def main_copy(args):
+    """Calibration for the HBM byte counters: a plain device copy of known
+    size (512 MiB read + 512 MiB written per launch)."""
+    import torch
+
+    n = 512 << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(max(args.steps, 1)):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy", "bytes_read_per_launch": n,
+                      "bytes_written_per_launch": n, "launches": max(args.steps, 1)}),
+          flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -434,8 +452,8 @@ def main():
         sys.exit(2)
     import torch  # noqa: F401  (load torch's HIP runtime before the engine's library)
 
-    {"bf": main_bf, "ivfpq": main_ivfpq, "ragged": main_ragged, "probe": main_probe}[
-        args.workload](args)
+    {"bf": main_bf, "ivfpq": main_ivfpq, "ragged": main_ragged, "probe": main_probe,
+     "copy": main_copy}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -8,7 +8,7 @@
 //   Angular     key =  s / ||b||             (E ~ 1 + key / ||q||)
 // and the shortlist is then reranked with the exact reference-order FP32
 // arithmetic (rerank.hip).  Operands are BF16 (v_mfma_f32_16x16x32_bf16) or
-// FP8-e4m3 with a per-vector scale (v_mfma_f32_16x16x32_fp8_fp8).
+// FP8-e4m3 with a per-vector scale (v_mfma_f32_16x16x128_f8f6f4).
 //
 // Tiling: 256 threads = 4 waves in a 2x2 arrangement; the workgroup owns a
 // 128-query x 128-base tile, each wave 64x64 = 4x4 MFMA 16x16 tiles (64 fp32
@@ -25,6 +25,7 @@ namespace {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+typedef int i32x8 __attribute__((ext_vector_type(8)));
 
 constexpr int GM = 128, GN = 128;
 constexpr int ROWB = 128;           // bytes per row per K step (64 bf16 / 128 fp8)
@@ -153,24 +154,26 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
                                                                             0, 0, 0);
             }
         } else {
-            // four MFMA K=32 per step; lane holds 8 fp8 at k = 32*kk + 8*lk
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                long a[4], b[4];
+            // one MFMA K=128 per step (v_mfma_f32_16x16x128_f8f6f4, e4m3 x
+            // e4m3, no block scale); lane holds 32 fp8 at k = 32*lk.  A and
+            // B use the same k packing, so the products pair up for any
+            // order the hardware assigns within a lane.
+            {
+                i32x8 a[4], b[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    a[i] = *reinterpret_cast<const long *>(
-                        As + (wr * 64 + i * 16 + lr) * LDS_ROW + kk * 32 + lk * 8);
+                    a[i] = *reinterpret_cast<const i32x8 *>(
+                        As + (wr * 64 + i * 16 + lr) * LDS_ROW + lk * 32);
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    b[j] = *reinterpret_cast<const long *>(
-                        Bs + (wc * 64 + j * 16 + lr) * LDS_ROW + kk * 32 + lk * 8);
+                    b[j] = *reinterpret_cast<const i32x8 *>(
+                        Bs + (wc * 64 + j * 16 + lr) * LDS_ROW + lk * 32);
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a[i], b[j],
-                                                                               acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                            a[i], b[j], acc[i][j], 0, 0, 0, 0, 0, 0);
             }
         }
     };
