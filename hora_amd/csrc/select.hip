@@ -319,14 +319,18 @@ struct LaneHeap {
 // Streams a distance row through LDS in batches of 4096 values (16 float4
 // loads per lane, the next batch in flight while the current one is
 // processed) and hands it over in chunks of up to 64 consecutive values:
-// on_chunk(first_index, values_in_lds, count).  Rows are 16-byte aligned
-// with a stride ldd >= n that is a multiple of 4, so the vector loads
-// (clamped to the last full vector of the row) stay inside the row.
-template <class OnChunk>
+// on_chunk(first_index, values_in_lds, count).  Before a batch is handed
+// over, skip_batch(batch_min) may drop it whole (the minimum is over the
+// batch's values with index < n; NaN is ignored by fminf, and rows with a
+// NaN never reach a replay).  Rows are 16-byte aligned with a stride ldd >=
+// n that is a multiple of 4, so the vector loads (clamped to the last full
+// vector of the row) stay inside the row.
+template <class SkipBatch, class OnChunk>
 __device__ void stream_row(const float *__restrict__ row, uint32_t n, uint64_t ldd, int lane,
-                           float *sv, OnChunk &&on_chunk)
+                           float *sv, SkipBatch &&skip_batch, OnChunk &&on_chunk)
 {
     constexpr uint32_t FB4 = 16, BATCH = FB4 * 256;
+    const float inf = __int_as_float(0x7f800000);
     float nx[4 * FB4];  // scalar array: promoted to registers once the loops unroll
 #define HM_ROW_LOAD(b)                                                                     \
     _Pragma("clang loop unroll(full)") for (uint32_t c4 = 0; c4 < FB4; ++c4)               \
@@ -340,15 +344,27 @@ __device__ void stream_row(const float *__restrict__ row, uint32_t n, uint64_t l
     }
     HM_ROW_LOAD(0u)
     for (uint32_t b0 = 0; b0 < n; b0 += BATCH) {
-        __syncthreads();
+        float m = inf;
 #pragma clang loop unroll(full)
-        for (uint32_t c4 = 0; c4 < FB4; ++c4)
-            reinterpret_cast<float4 *>(sv)[c4 * 64 + lane] =
-                make_float4(nx[4 * c4], nx[4 * c4 + 1], nx[4 * c4 + 2], nx[4 * c4 + 3]);
+        for (uint32_t c = 0; c < 4 * FB4; ++c) {
+            const uint64_t e = (uint64_t)b0 + (c / 4) * 256 + 4 * (uint32_t)lane + (c % 4);
+            if (e < n) m = fminf(m, nx[c]);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = fminf(m, __shfl_xor(m, o, 64));
+        const bool skip = skip_batch(m);
+        __syncthreads();
+        if (!skip) {
+#pragma clang loop unroll(full)
+            for (uint32_t c4 = 0; c4 < FB4; ++c4)
+                reinterpret_cast<float4 *>(sv)[c4 * 64 + lane] =
+                    make_float4(nx[4 * c4], nx[4 * c4 + 1], nx[4 * c4 + 2], nx[4 * c4 + 3]);
+        }
         __syncthreads();
         if (n - b0 > BATCH) {
             HM_ROW_LOAD(b0 + BATCH)
         }
+        if (skip) continue;
         for (uint32_t c = 0; c < BATCH / 64; ++c) {
             const uint32_t base = b0 + c * 64;
             if (base >= n) break;
@@ -481,7 +497,10 @@ __global__ __launch_bounds__(64) void bf_finalize_wave_kernel(
     // next element that is not greater than the root (the only ones that
     // change anything), so the rejected majority costs one ballot per 64.
     __shared__ __align__(16) float sv[4096];
-    stream_row(row, n, ldd, lane, sv, [&](uint32_t base, const float *cv, uint32_t cnt) {
+    // a batch whose values all exceed the root of a full, stable heap
+    // changes nothing
+    auto skip_batch = [&](float bmin) { return h.len == k && stable && bmin > root; };
+    stream_row(row, n, ldd, lane, sv, skip_batch, [&](uint32_t base, const float *cv, uint32_t cnt) {
         const float v = cv[lane];
         uint32_t j = 0;
         while (j < cnt) {
@@ -599,7 +618,8 @@ __global__ __launch_bounds__(64) void bf_replay_lds_kernel(
     bool stable = false;
     float root = inf;
     __shared__ __align__(16) float sv[4096];
-    stream_row(row, n, ldd, lane, sv, [&](uint32_t base, const float *cv, uint32_t cnt) {
+    auto skip_batch = [&](float bmin) { return h.len == k && stable && bmin > root; };
+    stream_row(row, n, ldd, lane, sv, skip_batch, [&](uint32_t base, const float *cv, uint32_t cnt) {
         const float v = cv[lane];
         uint32_t j = 0;
         while (j < cnt) {
