@@ -206,36 +206,58 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
 #undef HM_SSTORE
 #undef HM_MASK4
 
-    // Epilogue: lane holds D[4*lk + e][lr] of each 16x16 tile.
+    // Epilogue: lane holds D[4*lk + e][lr] of each 16x16 tile.  Every
+    // per-column and per-row operand is loaded up front (clamped indices,
+    // no branches), so the loads overlap instead of each waiting inside a
+    // branch; rows/columns past the end are masked afterwards.
+    float colv[4], cols[4];
+    uint32_t colc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t c = n0 + wc * 64 + j * 16 + lr;
+        const uint32_t cc = c < n ? c : n - 1;
+        colc[j] = c;
+        if constexpr (MET == HM_METRIC_EUCLIDEAN) colv[j] = bnorm2[cc];
+        else if constexpr (MET == HM_METRIC_DOT_PRODUCT) colv[j] = 0.0f;
+        else colv[j] = inv_bnorm[cc];
+        cols[j] = (ELEM == 1) ? bscale[cc] : 1.0f;
+    }
+    float rowt[4][4], rows_[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t q = q0 + wr * 64 + i * 16 + lk * 4 + e;
+            const uint32_t qc = q < nq ? q : nq - 1;
+            rows_[i][e] = (ELEM == 1) ? qscale[qc] : 1.0f;
+            if constexpr (FILTER) rowt[i][e] = out.tau[(size_t)qc * out.tau_ld];
+            else rowt[i][e] = 0.0f;
+        }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const uint32_t q = q0 + wr * 64 + i * 16 + lk * 4 + e;
-            if (q >= nq) continue;
-            const float qs = (ELEM == 1) ? qscale[q] : 1.0f;
-            float thr = 0.0f;
-            if constexpr (FILTER) thr = out.tau[(size_t)q * out.tau_ld];
+            const bool qok = q < nq;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const uint32_t c = n0 + wc * 64 + j * 16 + lr;
-                if (c >= n) continue;
                 float s = acc[i][j][e];
-                if (ELEM == 1) s = s * (qs * bscale[c]);
+                if (ELEM == 1) s = s * (rows_[i][e] * cols[j]);
                 float key;
-                if constexpr (MET == HM_METRIC_EUCLIDEAN) key = bnorm2[c] - 2.0f * s;
+                if constexpr (MET == HM_METRIC_EUCLIDEAN) key = colv[j] - 2.0f * s;
                 else if constexpr (MET == HM_METRIC_DOT_PRODUCT) key = s;
-                else if constexpr (MET == HM_METRIC_COSINE_SIMILARITY) key = -s * inv_bnorm[c];
-                else key = s * inv_bnorm[c];
+                else if constexpr (MET == HM_METRIC_COSINE_SIMILARITY) key = -s * colv[j];
+                else key = s * colv[j];
+                const uint32_t c = colc[j];
+                const bool ok = qok && c < n;
                 if constexpr (FILTER) {
-                    if (key < thr) {
+                    if (ok && key < rowt[i][e]) {  // rare: a few per query per tile row
                         const uint32_t pos = atomicAdd(out.cnt + q, 1u);
                         if (pos < out.cap)
-                            out.list[(size_t)q * out.cap + pos] =
-                                ((uint64_t)fkey(key) << 32) | c;
+                            out.list[(size_t)q * out.cap + pos] = ((uint64_t)fkey(key) << 32) | c;
                     }
                 } else {
-                    out.Kmat[(size_t)q * out.ldk + c] = key;
+                    if (ok) out.Kmat[(size_t)q * out.ldk + c] = key;
                 }
             }
         }
