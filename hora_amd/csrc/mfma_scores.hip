@@ -206,6 +206,19 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
 #undef HM_SSTORE
 #undef HM_MASK4
 
+    // Filter hits are staged in LDS (the operand buffers are free once every
+    // wave is past its last MFMA step) and then appended to the global
+    // lists with one atomic per hit issued by different threads, so their
+    // round trips overlap instead of serialising inside each wave.
+    constexpr uint32_t STAGE_CAP = sizeof(sA) / 12;
+    uint64_t *stage_v = reinterpret_cast<uint64_t *>(&sA[0][0]);
+    uint32_t *stage_q = reinterpret_cast<uint32_t *>(&sA[0][0] + (size_t)STAGE_CAP * 8);
+    __shared__ uint32_t s_nhit;
+    if constexpr (FILTER) {
+        __syncthreads();
+        if (tid == 0) s_nhit = 0;
+        __syncthreads();
+    }
     // Epilogue: lane holds D[4*lk + e][lr] of each 16x16 tile.  Every
     // per-column and per-row operand is loaded up front (clamped indices,
     // no branches), so the loads overlap instead of each waiting inside a
@@ -252,14 +265,27 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
                 const bool ok = qok && c < n;
                 if constexpr (FILTER) {
                     if (ok && key < rowt[i][e]) {  // rare: a few per query per tile row
-                        const uint32_t pos = atomicAdd(out.cnt + q, 1u);
-                        if (pos < out.cap)
-                            out.list[(size_t)q * out.cap + pos] = ((uint64_t)fkey(key) << 32) | c;
+                        const uint32_t pos = atomicAdd(&s_nhit, 1u);
+                        if (pos < STAGE_CAP) {
+                            stage_v[pos] = ((uint64_t)fkey(key) << 32) | c;
+                            stage_q[pos] = q;
+                        } else {
+                            atomicAdd(out.cnt + q, out.cap + 1);  // mark the list overflowed
+                        }
                     }
                 } else {
                     if (ok) out.Kmat[(size_t)q * out.ldk + c] = key;
                 }
             }
+        }
+    }
+    if constexpr (FILTER) {
+        __syncthreads();
+        const uint32_t nh = min(s_nhit, STAGE_CAP);
+        for (uint32_t h = tid; h < nh; h += 256) {
+            const uint32_t q = stage_q[h];
+            const uint32_t pos = atomicAdd(out.cnt + q, 1u);
+            if (pos < out.cap) out.list[(size_t)q * out.cap + pos] = stage_v[h];
         }
     }
 }
