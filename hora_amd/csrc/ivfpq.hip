@@ -390,6 +390,158 @@ __global__ __launch_bounds__(256) void ivf_scan_kernel(
     for (uint32_t j = tid; j < k1; j += 256) o[j] = j < kept ? keys[j] : ~0ull;
 }
 
+// ---- List-major scan.  The (query, probe) pairs are bucketed by list, and
+// one workgroup per list stages that list's codebook in LDS once, then
+// serves every query that probes the list: residual, LUT from the LDS
+// codebook, ADC over the list, top-k1.  The per-pair arithmetic is the same
+// as ivf_scan_kernel's (pq_idx.rs:151-194), so the candidates are the same.
+
+__global__ void fill_u32_small_kernel(uint32_t *p, uint32_t n, uint32_t v)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Pairs per list: counts, then positions (cnt zeroed by the caller).
+__global__ void pair_count_kernel(const uint32_t *__restrict__ probes, uint32_t ldp,
+                                  uint32_t nprobe, uint32_t rows,
+                                  const uint32_t *__restrict__ nprobe_q, uint32_t *cnt)
+{
+    const uint64_t total = (uint64_t)rows * nprobe;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t q = (uint32_t)(e / nprobe), p = (uint32_t)(e % nprobe);
+        if (nprobe_q && p >= nprobe_q[q]) continue;
+        atomicAdd(cnt + probes[(size_t)q * ldp + p], 1u);
+    }
+}
+
+// Exclusive scan of cnt[0..nl) into off[0..nl]; one workgroup of 1024.
+__global__ __launch_bounds__(1024) void pair_scan_kernel(const uint32_t *__restrict__ cnt,
+                                                          uint32_t nl, uint32_t *__restrict__ off)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t tid = threadIdx.x, per = (nl + 1023) / 1024, b = tid * per;
+    uint32_t sum = 0;
+    for (uint32_t i = 0; i < per && b + i < nl; ++i) sum += cnt[b + i];
+    part[tid] = sum;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {
+        const uint32_t v = tid >= o ? part[tid - o] : 0u;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[tid] - sum;
+    for (uint32_t i = 0; i < per && b + i < nl; ++i) {
+        off[b + i] = run;
+        run += cnt[b + i];
+    }
+    if (tid == 1023) off[nl] = part[1023];
+}
+
+__global__ void pair_fill_kernel(const uint32_t *__restrict__ probes, uint32_t ldp,
+                                 uint32_t nprobe, uint32_t rows,
+                                 const uint32_t *__restrict__ nprobe_q,
+                                 const uint32_t *__restrict__ off, uint32_t *fill,
+                                 uint32_t *__restrict__ pairs, unsigned long long *__restrict__ out,
+                                 uint32_t k1)
+{
+    const uint64_t total = (uint64_t)rows * nprobe;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t q = (uint32_t)(e / nprobe), p = (uint32_t)(e % nprobe);
+        if (nprobe_q && p >= nprobe_q[q]) {  // this query probes fewer lists
+            unsigned long long *o = out + e * k1;
+            for (uint32_t j = 0; j < k1; ++j) o[j] = ~0ull;
+            continue;
+        }
+        const uint32_t l = probes[(size_t)q * ldp + p];
+        pairs[off[l] + atomicAdd(fill + l, 1u)] = (uint32_t)e;  // e = q * nprobe + p
+    }
+}
+
+constexpr int LM_THREADS = 512;
+
+__global__ __launch_bounds__(LM_THREADS) void ivf_scan_lists_kernel(
+    const float *__restrict__ Qp, uint32_t dpad, const float *__restrict__ C,
+    const uint32_t *__restrict__ pair_off, const uint32_t *__restrict__ pairs, uint32_t nprobe,
+    const uint32_t *__restrict__ off, const uint32_t *__restrict__ ids,
+    const uint8_t *__restrict__ codes, const float *__restrict__ codebooks,
+    const uint32_t *__restrict__ sub_b, const uint32_t *__restrict__ sub_e, uint32_t M,
+    uint32_t KS, uint32_t sdm, uint32_t k1, uint32_t P, unsigned long long *__restrict__ out)
+{
+    const uint32_t l = blockIdx.x, p0 = pair_off[l], p1 = pair_off[l + 1];
+    if (p0 == p1) return;
+    extern __shared__ __align__(16) unsigned char lm_smem[];
+    const uint32_t ncb = M * KS * sdm;
+    float *cbs = reinterpret_cast<float *>(lm_smem);  // [M][KS][sdm]
+    float *lut = cbs + ncb;                            // [M][KS]
+    float *z = lut + (size_t)M * KS;                   // [dpad]
+    unsigned long long *keys = reinterpret_cast<unsigned long long *>(z + dpad);  // [P]
+    const int tid = threadIdx.x;
+    const float *cb = codebooks + (size_t)l * ncb;
+    if ((ncb & 3) == 0) {  // whole float4s per list, 16-byte aligned from the allocation base
+        for (uint32_t e = tid; e < ncb / 4; e += LM_THREADS)
+            reinterpret_cast<float4 *>(cbs)[e] = reinterpret_cast<const float4 *>(cb)[e];
+    } else {
+        for (uint32_t e = tid; e < ncb; e += LM_THREADS) cbs[e] = cb[e];
+    }
+    const uint32_t base = off[l], L = off[l + 1] - base;
+    const uint32_t CH = P - k1;
+    for (uint32_t pp = p0; pp < p1; ++pp) {
+        const uint32_t e = pairs[pp], q = e / nprobe;
+        __syncthreads();  // previous pair done with z / lut / keys
+        for (uint32_t j = tid; j < dpad; j += LM_THREADS)
+            z[j] = Qp[(size_t)q * dpad + j] - C[(size_t)l * dpad + j];
+        __syncthreads();
+        for (uint32_t t = tid; t < M * KS; t += LM_THREADS) {
+            const uint32_t sub = t / KS, b = sub_b[sub], sd = sub_e[sub] - b;
+            const float *w = cbs + (size_t)t * sdm;
+            float acc = -0.0f;
+            for (uint32_t j = 0; j < sd; ++j) {
+                float df = z[b + j] - w[j];
+                acc = acc + df * df;
+            }
+            lut[t] = acc;
+        }
+        __syncthreads();
+        uint32_t kept = 0;
+        for (uint32_t c0 = 0; c0 < L || c0 == 0; c0 += CH) {
+            const uint32_t m = L > c0 ? (L - c0 < CH ? L - c0 : CH) : 0;
+            for (uint32_t i = tid; i < m; i += LM_THREADS) {
+                const uint32_t row = base + c0 + i;
+                const uint8_t *cd = codes + (size_t)row * M;
+                float dist = 0.0f;
+                for (uint32_t j = 0; j < M; ++j) dist = dist + lut[j * KS + cd[j]];
+                keys[kept + i] = ((unsigned long long)fkey(dist) << 32) | ids[row];
+            }
+            const uint32_t total = kept + m;
+            uint32_t PP = 1;
+            while (PP < total) PP <<= 1;
+            for (uint32_t j = total + tid; j < PP; j += LM_THREADS) keys[j] = ~0ull;
+            __syncthreads();
+            for (uint32_t size = 2; size <= PP; size <<= 1)
+                for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                    for (uint32_t t = tid; t < PP / 2; t += LM_THREADS) {
+                        uint32_t lo = 2 * stride * (t / stride) + (t % stride), hi = lo + stride;
+                        bool asc = (lo & size) == 0;
+                        unsigned long long a = keys[lo], bb = keys[hi];
+                        if ((a > bb) == asc) {
+                            keys[lo] = bb;
+                            keys[hi] = a;
+                        }
+                    }
+                    __syncthreads();
+                }
+            kept = total < k1 ? total : k1;
+            if (c0 + CH >= L) break;
+        }
+        unsigned long long *o = out + (size_t)e * k1;
+        for (uint32_t j = tid; j < k1; j += LM_THREADS) o[j] = j < kept ? keys[j] : ~0ull;
+    }
+}
+
 // Per query: the nprobe*k1 candidates -> first k ascending; ROW_TIE when
 // the first min(k+1, valid) distances are not pairwise distinct or the
 // coarse probe set was tied.
@@ -775,6 +927,50 @@ void launch_ivf_scan(const float *Qp, uint32_t dpad, const float *C, const uint3
     ivf_scan_kernel<<<dim3(nprobe, rows), 256, lds, s>>>(Qp, dpad, C, probes, ldp, nprobe, off,
                                                           ids, codes, codebooks, sub_b, sub_e, M,
                                                           KS, sdm, k1, P, nprobe_q, out);
+    HM_CHECK_LAUNCH();
+}
+
+// LDS of the list-major scan for a pool of P candidates (0 if it does not
+// fit in 160 KiB with a pool of at least 2*k1, the chunked scan's minimum).
+static size_t lists_lds(uint32_t M, uint32_t KS, uint32_t sdm, uint32_t dpad, uint32_t P)
+{
+    return ((size_t)M * KS * sdm + (size_t)M * KS + dpad) * 4 + (size_t)P * 8;
+}
+
+uint32_t ivf_scan_lists_pool(uint32_t M, uint32_t KS, uint32_t sdm, uint32_t dpad, uint32_t k1)
+{
+    uint32_t P = 8192;
+    while (P >= 2 * k1 && P >= 512 && lists_lds(M, KS, sdm, dpad, P) > 160 * 1024) P >>= 1;
+    return (P >= 2 * k1 && P >= 512 && lists_lds(M, KS, sdm, dpad, P) <= 160 * 1024) ? P : 0;
+}
+
+void launch_ivf_scan_lists(const float *Qp, uint32_t dpad, const float *C,
+                           const uint32_t *probes, uint32_t ldp, uint32_t nprobe, uint32_t rows,
+                           uint32_t nlist, const uint32_t *off, const uint32_t *ids,
+                           const uint8_t *codes, const float *codebooks, const uint32_t *sub_b,
+                           const uint32_t *sub_e, uint32_t M, uint32_t KS, uint32_t sdm,
+                           uint32_t k1, const uint32_t *nprobe_q, uint32_t *pair_cnt,
+                           uint32_t *pair_off, uint32_t *pair_fill, uint32_t *pairs,
+                           unsigned long long *out, hipStream_t s)
+{
+    if (rows == 0 || nprobe == 0) return;
+    const uint32_t P = ivf_scan_lists_pool(M, KS, sdm, dpad, k1);
+    if (!P) fail(HM_ERR_INVALID_ARG, "list-major scan does not fit in LDS");
+    if ((uint64_t)rows * nprobe > 0xFFFFFFFFull) fail(HM_ERR_INVALID_ARG, "too many probe pairs");
+    ProfScope prof("ivf_scan", s);
+    const uint64_t total = (uint64_t)rows * nprobe;
+    const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 4096);
+    fill_u32_small_kernel<<<std::min<unsigned>(ceil_div(nlist, 256), 1024), 256, 0, s>>>(
+        pair_cnt, nlist, 0u);
+    fill_u32_small_kernel<<<std::min<unsigned>(ceil_div(nlist, 256), 1024), 256, 0, s>>>(
+        pair_fill, nlist, 0u);
+    pair_count_kernel<<<blocks, 256, 0, s>>>(probes, ldp, nprobe, rows, nprobe_q, pair_cnt);
+    pair_scan_kernel<<<1, 1024, 0, s>>>(pair_cnt, nlist, pair_off);
+    pair_fill_kernel<<<blocks, 256, 0, s>>>(probes, ldp, nprobe, rows, nprobe_q, pair_off,
+                                            pair_fill, pairs, out, k1);
+    ivf_scan_lists_kernel<<<nlist, LM_THREADS, lists_lds(M, KS, sdm, dpad, P), s>>>(
+        Qp, dpad, C, pair_off, pairs, nprobe, off, ids, codes, codebooks, sub_b, sub_e, M, KS,
+        sdm, k1, P, out);
     HM_CHECK_LAUNCH();
 }
 

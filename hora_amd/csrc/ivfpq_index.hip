@@ -213,6 +213,15 @@ class IVFPQ final : public Index {
         cflags_.reserve(qc * 4);
         mflags_.reserve(qc * 4);
         cand_.reserve(qc * per_query * 8);
+        // list-major scan when a list's codebook fits in LDS (one read of
+        // each probed codebook per chunk instead of one per query and probe)
+        const bool list_major = scan_mode_ != 1 && ivf_scan_lists_pool(M_, KS_, sdm_, dpad_, k1) != 0;
+        if (list_major) {
+            pair_cnt_.reserve((size_t)nlist_ * 4);
+            pair_fill_.reserve((size_t)nlist_ * 4);
+            pair_off_.reserve(((size_t)nlist_ + 1) * 4);
+            pairs_.reserve(qc * np * 4);
+        }
         if (learned) {
             xq16_.reserve(qc * dim_ * 2);
             Zq_.reserve(qc * nlist_ * 4);
@@ -252,11 +261,22 @@ class IVFPQ final : public Index {
                                   probe_i_.as<uint32_t>(), Kc, cflags_.as<uint32_t>(), s);
             }
             const uint32_t *npq_dev = learned ? nprobe_q_.as<uint32_t>() : nullptr;
-            launch_ivf_scan(qpad_.as<float>(), dpad_, centers_.as<float>(),
-                            probe_i_.as<uint32_t>(), Kc, np, rows, off_.as<uint32_t>(),
-                            ids_.as<uint32_t>(), codes_.as<uint8_t>(), codebooks_.as<float>(),
-                            sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), M_, KS_, sdm_, k1,
-                            npq_dev, cand_.as<unsigned long long>(), s);
+            if (list_major) {
+                launch_ivf_scan_lists(qpad_.as<float>(), dpad_, centers_.as<float>(),
+                                      probe_i_.as<uint32_t>(), Kc, np, rows, nlist_,
+                                      off_.as<uint32_t>(), ids_.as<uint32_t>(),
+                                      codes_.as<uint8_t>(), codebooks_.as<float>(),
+                                      sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), M_, KS_, sdm_,
+                                      k1, npq_dev, pair_cnt_.as<uint32_t>(),
+                                      pair_off_.as<uint32_t>(), pair_fill_.as<uint32_t>(),
+                                      pairs_.as<uint32_t>(), cand_.as<unsigned long long>(), s);
+            } else {
+                launch_ivf_scan(qpad_.as<float>(), dpad_, centers_.as<float>(),
+                                probe_i_.as<uint32_t>(), Kc, np, rows, off_.as<uint32_t>(),
+                                ids_.as<uint32_t>(), codes_.as<uint8_t>(), codebooks_.as<float>(),
+                                sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), M_, KS_, sdm_, k1,
+                                npq_dev, cand_.as<unsigned long long>(), s);
+            }
             launch_ivf_merge(cand_.as<unsigned long long>(), per_query, rows, k,
                              cflags_.as<uint32_t>(), d_idx + q0 * k, d_dist + q0 * k,
                              d_count + q0, mflags_.as<uint32_t>(), s);
@@ -508,6 +528,8 @@ class IVFPQ final : public Index {
     DeviceBuffer base_, sub_b_, sub_e_, centers_, cnorm_, xnorm_, best_, assign_, lists_, ids_,
         off_, codebooks_, codes_, iota_, sort_tmp_;
     DeviceBuffer qpad_, qnorm_, D_, probe_d_, probe_i_, cflags_, mflags_, cand_, err_;
+    DeviceBuffer pair_cnt_, pair_fill_, pair_off_, pairs_;
+    int scan_mode_ = 0;  // 0: list-major when it fits, 1: per (query, probe) workgroups
     DeviceBuffer rp_rows_, rp_f_, rp_u_;
     // learned probe
     std::unique_ptr<ProbeMLP> mlp_;

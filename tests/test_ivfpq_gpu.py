@@ -110,6 +110,16 @@ def test_coarse_metric_variants(orc, mt):
     _check_conditional(orc, idx, x, mt, q, 3, 10)
 
 
+def test_large_codebook_scan(orc):
+    # 8 x 256 codewords of 32 dims (256 KiB per list) do not fit in LDS: the
+    # per-(query, probe) scan runs instead of the list-major one
+    x, _, _ = gaussian_mixture(3000, 256, 8, seed=17, sigma=0.3)
+    idx = _build(x, Metric.Euclidean, 8, 8, 8, 3, 23)
+    assert idx.info()["sub_dim_max"] == 32
+    q, _, _ = gaussian_mixture(40, 256, 8, seed=18, sigma=0.3)
+    _check_conditional(orc, idx, x, Metric.Euclidean, q, 3, 10)
+
+
 def test_ties_replay(orc):
     # every row twice: equal codes give equal ADC distances in every list
     x, _, _ = gaussian_mixture(800, 12, 6, seed=15, sigma=0.3)
