@@ -44,15 +44,16 @@ void launch_ivf_scan(const float *Qp, uint32_t dpad, const float *C, const uint3
 // bucketed by list, one workgroup per list with its codebook in LDS.
 // Returns the candidate pool, or 0 when the codebook does not fit in LDS.
 uint32_t ivf_scan_lists_pool(uint32_t M, uint32_t KS, uint32_t sdm, uint32_t dpad, uint32_t k1);
-// Scratch: pair_cnt, pair_fill [nlist]; pair_off [nlist + 1]; pairs [rows * nprobe].
+// Scratch: pair_cnt, pair_fill [nlist]; pair_off, unit_off [nlist + 1];
+// pairs [rows * nprobe].
 void launch_ivf_scan_lists(const float *Qp, uint32_t dpad, const float *C,
                            const uint32_t *probes, uint32_t ldp, uint32_t nprobe, uint32_t rows,
                            uint32_t nlist, const uint32_t *off, const uint32_t *ids,
                            const uint8_t *codes, const float *codebooks, const uint32_t *sub_b,
                            const uint32_t *sub_e, uint32_t M, uint32_t KS, uint32_t sdm,
                            uint32_t k1, const uint32_t *nprobe_q, uint32_t *pair_cnt,
-                           uint32_t *pair_off, uint32_t *pair_fill, uint32_t *pairs,
-                           unsigned long long *out, hipStream_t s);
+                           uint32_t *pair_off, uint32_t *pair_fill, uint32_t *unit_off,
+                           uint32_t *pairs, unsigned long long *out, hipStream_t s);
 void launch_ivf_merge(const unsigned long long *cand, uint32_t per_query, uint32_t rows,
                       uint32_t k, const uint32_t *coarse_flags, uint64_t *out_idx,
                       float *out_dist, uint32_t *out_count, uint32_t *flags, hipStream_t s);

@@ -416,28 +416,42 @@ __global__ void pair_count_kernel(const uint32_t *__restrict__ probes, uint32_t 
     }
 }
 
-// Exclusive scan of cnt[0..nl) into off[0..nl]; one workgroup of 1024.
+// Exclusive scans of cnt[0..nl) into off[0..nl] and of ceil(cnt / G)
+// into uoff[0..nl]; one workgroup of 1024.
 __global__ __launch_bounds__(1024) void pair_scan_kernel(const uint32_t *__restrict__ cnt,
-                                                          uint32_t nl, uint32_t *__restrict__ off)
+                                                          uint32_t nl, uint32_t G,
+                                                          uint32_t *__restrict__ off,
+                                                          uint32_t *__restrict__ uoff)
 {
-    __shared__ uint32_t part[1024];
+    __shared__ uint32_t part[1024], upart[1024];
     const uint32_t tid = threadIdx.x, per = (nl + 1023) / 1024, b = tid * per;
-    uint32_t sum = 0;
-    for (uint32_t i = 0; i < per && b + i < nl; ++i) sum += cnt[b + i];
+    uint32_t sum = 0, usum = 0;
+    for (uint32_t i = 0; i < per && b + i < nl; ++i) {
+        sum += cnt[b + i];
+        usum += (cnt[b + i] + G - 1) / G;
+    }
     part[tid] = sum;
+    upart[tid] = usum;
     __syncthreads();
     for (uint32_t o = 1; o < 1024; o <<= 1) {
         const uint32_t v = tid >= o ? part[tid - o] : 0u;
+        const uint32_t w = tid >= o ? upart[tid - o] : 0u;
         __syncthreads();
         part[tid] += v;
+        upart[tid] += w;
         __syncthreads();
     }
-    uint32_t run = part[tid] - sum;
+    uint32_t run = part[tid] - sum, urun = upart[tid] - usum;
     for (uint32_t i = 0; i < per && b + i < nl; ++i) {
         off[b + i] = run;
+        uoff[b + i] = urun;
         run += cnt[b + i];
+        urun += (cnt[b + i] + G - 1) / G;
     }
-    if (tid == 1023) off[nl] = part[1023];
+    if (tid == 1023) {
+        off[nl] = part[1023];
+        uoff[nl] = upart[1023];
+    }
 }
 
 __global__ void pair_fill_kernel(const uint32_t *__restrict__ probes, uint32_t ldp,
@@ -462,23 +476,79 @@ __global__ void pair_fill_kernel(const uint32_t *__restrict__ probes, uint32_t l
 }
 
 constexpr int LM_THREADS = 512;
+constexpr uint32_t LM_BINS = 2048;
 
+// Block-wide: the bin of hist[0..LM_BINS) holding the need-th smallest
+// (1-based) counted element, and how many are still needed inside it.
+__device__ void lm_find_bin(const uint32_t *hist, uint32_t need, uint32_t *s_part,
+                            uint32_t *bin, uint32_t *rem)
+{
+    const uint32_t tid = threadIdx.x, per = LM_BINS / LM_THREADS, b0 = tid * per;
+    uint32_t sum = 0;
+    for (uint32_t i = 0; i < per; ++i) sum += hist[b0 + i];
+    s_part[tid] = sum;
+    __syncthreads();
+    for (uint32_t o = 1; o < LM_THREADS; o <<= 1) {
+        const uint32_t v = tid >= o ? s_part[tid - o] : 0u;
+        __syncthreads();
+        s_part[tid] += v;
+        __syncthreads();
+    }
+    const uint32_t incl = s_part[tid], excl = incl - sum;
+    if (excl < need && need <= incl) {
+        uint32_t c = excl;
+        for (uint32_t i = 0; i < per; ++i) {
+            const uint32_t h = hist[b0 + i];
+            if (c + h >= need) {
+                s_part[LM_THREADS] = b0 + i;
+                s_part[LM_THREADS + 1] = need - c;
+                break;
+            }
+            c += h;
+        }
+    }
+    __syncthreads();
+    *bin = s_part[LM_THREADS];
+    *rem = s_part[LM_THREADS + 1];
+    __syncthreads();
+}
+
+// A work unit is up to G pairs of one list (lists probed by many queries
+// are split over several workgroups).  The unit's workgroup stages the
+// list's codebook in LDS once; per pair it builds the LUT, and selects the
+// k1 smallest (fkey(adc) << 32 | id) of the list: lists up to P items are
+// sorted whole; longer ones take two 11-bit histogram passes over the ADC
+// keys to find a threshold, then collect the keys under it and sort those.
 __global__ __launch_bounds__(LM_THREADS) void ivf_scan_lists_kernel(
     const float *__restrict__ Qp, uint32_t dpad, const float *__restrict__ C,
-    const uint32_t *__restrict__ pair_off, const uint32_t *__restrict__ pairs, uint32_t nprobe,
+    const uint32_t *__restrict__ pair_off, const uint32_t *__restrict__ unit_off, uint32_t nlist,
+    uint32_t G, const uint32_t *__restrict__ pairs, uint32_t nprobe,
     const uint32_t *__restrict__ off, const uint32_t *__restrict__ ids,
     const uint8_t *__restrict__ codes, const float *__restrict__ codebooks,
     const uint32_t *__restrict__ sub_b, const uint32_t *__restrict__ sub_e, uint32_t M,
     uint32_t KS, uint32_t sdm, uint32_t k1, uint32_t P, unsigned long long *__restrict__ out)
 {
-    const uint32_t l = blockIdx.x, p0 = pair_off[l], p1 = pair_off[l + 1];
-    if (p0 == p1) return;
+    const uint32_t u = blockIdx.x;
+    if (u >= unit_off[nlist]) return;
+    uint32_t lo = 0, hi = nlist;  // the list l with unit_off[l] <= u < unit_off[l + 1]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (unit_off[mid] <= u) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t l = lo;
+    const uint32_t p0 = pair_off[l] + (u - unit_off[l]) * G;
+    const uint32_t p1 = min(p0 + G, pair_off[l + 1]);
     extern __shared__ __align__(16) unsigned char lm_smem[];
+    __shared__ uint32_t s_part[LM_THREADS + 2];
+    __shared__ uint32_t s_n;
     const uint32_t ncb = M * KS * sdm;
     float *cbs = reinterpret_cast<float *>(lm_smem);  // [M][KS][sdm]
     float *lut = cbs + ncb;                            // [M][KS]
-    float *z = lut + (size_t)M * KS;                   // [dpad]
-    unsigned long long *keys = reinterpret_cast<unsigned long long *>(z + dpad);  // [P]
+    unsigned char *region = reinterpret_cast<unsigned char *>(lut + (size_t)M * KS);
+    float *z = reinterpret_cast<float *>(region);                            // [dpad], LUT build
+    uint32_t *hist = reinterpret_cast<uint32_t *>(region);                   // [LM_BINS], select
+    unsigned long long *keys = reinterpret_cast<unsigned long long *>(region);  // [P], sort
     const int tid = threadIdx.x;
     const float *cb = codebooks + (size_t)l * ncb;
     if ((ncb & 3) == 0) {  // whole float4s per list, 16-byte aligned from the allocation base
@@ -488,10 +558,35 @@ __global__ __launch_bounds__(LM_THREADS) void ivf_scan_lists_kernel(
         for (uint32_t e = tid; e < ncb; e += LM_THREADS) cbs[e] = cb[e];
     }
     const uint32_t base = off[l], L = off[l + 1] - base;
-    const uint32_t CH = P - k1;
+    auto adc_key = [&](uint32_t i) -> unsigned long long {
+        const uint32_t row = base + i;
+        const uint8_t *cd = codes + (size_t)row * M;
+        float dist = 0.0f;
+        for (uint32_t j = 0; j < M; ++j) dist = dist + lut[j * KS + cd[j]];
+        return ((unsigned long long)fkey(dist) << 32) | ids[row];
+    };
+    auto sort_keys = [&](uint32_t total) {
+        uint32_t PP = 1;
+        while (PP < total) PP <<= 1;
+        for (uint32_t j = total + tid; j < PP; j += LM_THREADS) keys[j] = ~0ull;
+        __syncthreads();
+        for (uint32_t size = 2; size <= PP; size <<= 1)
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t t = tid; t < PP / 2; t += LM_THREADS) {
+                    const uint32_t a0 = 2 * stride * (t / stride) + (t % stride), a1 = a0 + stride;
+                    const bool asc = (a0 & size) == 0;
+                    const unsigned long long a = keys[a0], bb = keys[a1];
+                    if ((a > bb) == asc) {
+                        keys[a0] = bb;
+                        keys[a1] = a;
+                    }
+                }
+                __syncthreads();
+            }
+    };
     for (uint32_t pp = p0; pp < p1; ++pp) {
         const uint32_t e = pairs[pp], q = e / nprobe;
-        __syncthreads();  // previous pair done with z / lut / keys
+        __syncthreads();  // previous pair done with the region and the LUT
         for (uint32_t j = tid; j < dpad; j += LM_THREADS)
             z[j] = Qp[(size_t)q * dpad + j] - C[(size_t)l * dpad + j];
         __syncthreads();
@@ -507,35 +602,56 @@ __global__ __launch_bounds__(LM_THREADS) void ivf_scan_lists_kernel(
         }
         __syncthreads();
         uint32_t kept = 0;
-        for (uint32_t c0 = 0; c0 < L || c0 == 0; c0 += CH) {
-            const uint32_t m = L > c0 ? (L - c0 < CH ? L - c0 : CH) : 0;
-            for (uint32_t i = tid; i < m; i += LM_THREADS) {
-                const uint32_t row = base + c0 + i;
-                const uint8_t *cd = codes + (size_t)row * M;
-                float dist = 0.0f;
-                for (uint32_t j = 0; j < M; ++j) dist = dist + lut[j * KS + cd[j]];
-                keys[kept + i] = ((unsigned long long)fkey(dist) << 32) | ids[row];
-            }
-            const uint32_t total = kept + m;
-            uint32_t PP = 1;
-            while (PP < total) PP <<= 1;
-            for (uint32_t j = total + tid; j < PP; j += LM_THREADS) keys[j] = ~0ull;
+        if (L <= P) {
+            for (uint32_t i = tid; i < L; i += LM_THREADS) keys[i] = adc_key(i);
+            sort_keys(L);
+            kept = L < k1 ? L : k1;
+        } else {
+            // level 1: top 11 bits of the ADC key
+            for (uint32_t j = tid; j < LM_BINS; j += LM_THREADS) hist[j] = 0;
             __syncthreads();
-            for (uint32_t size = 2; size <= PP; size <<= 1)
-                for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                    for (uint32_t t = tid; t < PP / 2; t += LM_THREADS) {
-                        uint32_t lo = 2 * stride * (t / stride) + (t % stride), hi = lo + stride;
-                        bool asc = (lo & size) == 0;
-                        unsigned long long a = keys[lo], bb = keys[hi];
-                        if ((a > bb) == asc) {
-                            keys[lo] = bb;
-                            keys[hi] = a;
-                        }
-                    }
-                    __syncthreads();
+            for (uint32_t i = tid; i < L; i += LM_THREADS)
+                atomicAdd(&hist[(uint32_t)(adc_key(i) >> 53)], 1u);
+            __syncthreads();
+            uint32_t b1, r1;
+            lm_find_bin(hist, k1, s_part, &b1, &r1);
+            // level 2: the next 11 bits inside that bin
+            for (uint32_t j = tid; j < LM_BINS; j += LM_THREADS) hist[j] = 0;
+            __syncthreads();
+            for (uint32_t i = tid; i < L; i += LM_THREADS) {
+                const unsigned long long key = adc_key(i);
+                if ((uint32_t)(key >> 53) == b1) atomicAdd(&hist[(uint32_t)(key >> 42) & 2047u], 1u);
+            }
+            __syncthreads();
+            uint32_t b2, r2;
+            lm_find_bin(hist, r1, s_part, &b2, &r2);
+            const uint32_t thr = (b1 << 11) | b2;  // keep keys with key >> 42 <= thr
+            if (tid == 0) s_n = 0;
+            __syncthreads();
+            for (uint32_t i = tid; i < L; i += LM_THREADS) {
+                const unsigned long long key = adc_key(i);
+                if ((uint32_t)(key >> 42) <= thr) {
+                    const uint32_t pos = atomicAdd(&s_n, 1u);
+                    if (pos < P) keys[pos] = key;
                 }
-            kept = total < k1 ? total : k1;
-            if (c0 + CH >= L) break;
+            }
+            __syncthreads();
+            const uint32_t total = s_n;
+            if (total <= P) {
+                sort_keys(total);
+                kept = total < k1 ? total : k1;
+            } else {
+                // the threshold bin holds more than P keys: chunked sort of
+                // the whole list, keeping the k1 smallest of each chunk
+                __syncthreads();
+                const uint32_t CH = P - k1;
+                for (uint32_t c0 = 0; c0 < L; c0 += CH) {
+                    const uint32_t m = L - c0 < CH ? L - c0 : CH;
+                    for (uint32_t i = tid; i < m; i += LM_THREADS) keys[kept + i] = adc_key(c0 + i);
+                    sort_keys(kept + m);
+                    kept = kept + m < k1 ? kept + m : k1;
+                }
+            }
         }
         unsigned long long *o = out + (size_t)e * k1;
         for (uint32_t j = tid; j < k1; j += LM_THREADS) o[j] = j < kept ? keys[j] : ~0ull;
@@ -930,18 +1046,19 @@ void launch_ivf_scan(const float *Qp, uint32_t dpad, const float *C, const uint3
     HM_CHECK_LAUNCH();
 }
 
-// LDS of the list-major scan for a pool of P candidates (0 if it does not
-// fit in 160 KiB with a pool of at least 2*k1, the chunked scan's minimum).
+// LDS of the list-major scan: codebook, LUT, and one region shared by the
+// residual, the select histogram and the pool of P candidates.
 static size_t lists_lds(uint32_t M, uint32_t KS, uint32_t sdm, uint32_t dpad, uint32_t P)
 {
-    return ((size_t)M * KS * sdm + (size_t)M * KS + dpad) * 4 + (size_t)P * 8;
+    const size_t region = std::max<size_t>({(size_t)dpad * 4, (size_t)LM_BINS * 4, (size_t)P * 8});
+    return ((size_t)M * KS * sdm + (size_t)M * KS) * 4 + region;
 }
 
 uint32_t ivf_scan_lists_pool(uint32_t M, uint32_t KS, uint32_t sdm, uint32_t dpad, uint32_t k1)
 {
     uint32_t P = 8192;
-    while (P >= 2 * k1 && P >= 512 && lists_lds(M, KS, sdm, dpad, P) > 160 * 1024) P >>= 1;
-    return (P >= 2 * k1 && P >= 512 && lists_lds(M, KS, sdm, dpad, P) <= 160 * 1024) ? P : 0;
+    while (P > 512 && lists_lds(M, KS, sdm, dpad, P) > 160 * 1024) P >>= 1;
+    return (P >= 2 * k1 && lists_lds(M, KS, sdm, dpad, P) <= 160 * 1024) ? P : 0;
 }
 
 void launch_ivf_scan_lists(const float *Qp, uint32_t dpad, const float *C,
@@ -950,27 +1067,29 @@ void launch_ivf_scan_lists(const float *Qp, uint32_t dpad, const float *C,
                            const uint8_t *codes, const float *codebooks, const uint32_t *sub_b,
                            const uint32_t *sub_e, uint32_t M, uint32_t KS, uint32_t sdm,
                            uint32_t k1, const uint32_t *nprobe_q, uint32_t *pair_cnt,
-                           uint32_t *pair_off, uint32_t *pair_fill, uint32_t *pairs,
-                           unsigned long long *out, hipStream_t s)
+                           uint32_t *pair_off, uint32_t *pair_fill, uint32_t *unit_off,
+                           uint32_t *pairs, unsigned long long *out, hipStream_t s)
 {
     if (rows == 0 || nprobe == 0) return;
     const uint32_t P = ivf_scan_lists_pool(M, KS, sdm, dpad, k1);
     if (!P) fail(HM_ERR_INVALID_ARG, "list-major scan does not fit in LDS");
-    if ((uint64_t)rows * nprobe > 0xFFFFFFFFull) fail(HM_ERR_INVALID_ARG, "too many probe pairs");
-    ProfScope prof("ivf_scan", s);
     const uint64_t total = (uint64_t)rows * nprobe;
+    if (total > 0x7FFFFFFFull) fail(HM_ERR_INVALID_ARG, "too many probe pairs");
+    ProfScope prof("ivf_scan", s);
+    // about 2048 work units for the batch, whatever the list skew
+    const uint32_t G = (uint32_t)std::max<uint64_t>(8, (total + 2047) / 2048);
     const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 4096);
-    fill_u32_small_kernel<<<std::min<unsigned>(ceil_div(nlist, 256), 1024), 256, 0, s>>>(
-        pair_cnt, nlist, 0u);
-    fill_u32_small_kernel<<<std::min<unsigned>(ceil_div(nlist, 256), 1024), 256, 0, s>>>(
-        pair_fill, nlist, 0u);
+    const unsigned fb = std::min<unsigned>(ceil_div(nlist, 256), 1024);
+    fill_u32_small_kernel<<<fb, 256, 0, s>>>(pair_cnt, nlist, 0u);
+    fill_u32_small_kernel<<<fb, 256, 0, s>>>(pair_fill, nlist, 0u);
     pair_count_kernel<<<blocks, 256, 0, s>>>(probes, ldp, nprobe, rows, nprobe_q, pair_cnt);
-    pair_scan_kernel<<<1, 1024, 0, s>>>(pair_cnt, nlist, pair_off);
+    pair_scan_kernel<<<1, 1024, 0, s>>>(pair_cnt, nlist, G, pair_off, unit_off);
     pair_fill_kernel<<<blocks, 256, 0, s>>>(probes, ldp, nprobe, rows, nprobe_q, pair_off,
                                             pair_fill, pairs, out, k1);
-    ivf_scan_lists_kernel<<<nlist, LM_THREADS, lists_lds(M, KS, sdm, dpad, P), s>>>(
-        Qp, dpad, C, pair_off, pairs, nprobe, off, ids, codes, codebooks, sub_b, sub_e, M, KS,
-        sdm, k1, P, out);
+    const uint64_t units_max = (total + G - 1) / G + nlist;  // sum of ceil(cnt_l / G)
+    ivf_scan_lists_kernel<<<(unsigned)units_max, LM_THREADS, lists_lds(M, KS, sdm, dpad, P), s>>>(
+        Qp, dpad, C, pair_off, unit_off, nlist, G, pairs, nprobe, off, ids, codes, codebooks,
+        sub_b, sub_e, M, KS, sdm, k1, P, out);
     HM_CHECK_LAUNCH();
 }
 

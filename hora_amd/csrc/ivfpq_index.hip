@@ -220,6 +220,7 @@ class IVFPQ final : public Index {
             pair_cnt_.reserve((size_t)nlist_ * 4);
             pair_fill_.reserve((size_t)nlist_ * 4);
             pair_off_.reserve(((size_t)nlist_ + 1) * 4);
+            unit_off_.reserve(((size_t)nlist_ + 1) * 4);
             pairs_.reserve(qc * np * 4);
         }
         if (learned) {
@@ -269,7 +270,8 @@ class IVFPQ final : public Index {
                                       sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), M_, KS_, sdm_,
                                       k1, npq_dev, pair_cnt_.as<uint32_t>(),
                                       pair_off_.as<uint32_t>(), pair_fill_.as<uint32_t>(),
-                                      pairs_.as<uint32_t>(), cand_.as<unsigned long long>(), s);
+                                      unit_off_.as<uint32_t>(), pairs_.as<uint32_t>(),
+                                      cand_.as<unsigned long long>(), s);
             } else {
                 launch_ivf_scan(qpad_.as<float>(), dpad_, centers_.as<float>(),
                                 probe_i_.as<uint32_t>(), Kc, np, rows, off_.as<uint32_t>(),
@@ -528,7 +530,7 @@ class IVFPQ final : public Index {
     DeviceBuffer base_, sub_b_, sub_e_, centers_, cnorm_, xnorm_, best_, assign_, lists_, ids_,
         off_, codebooks_, codes_, iota_, sort_tmp_;
     DeviceBuffer qpad_, qnorm_, D_, probe_d_, probe_i_, cflags_, mflags_, cand_, err_;
-    DeviceBuffer pair_cnt_, pair_fill_, pair_off_, pairs_;
+    DeviceBuffer pair_cnt_, pair_fill_, pair_off_, unit_off_, pairs_;
     int scan_mode_ = 0;  // 0: list-major when it fits, 1: per (query, probe) workgroups
     DeviceBuffer rp_rows_, rp_f_, rp_u_;
     // learned probe
