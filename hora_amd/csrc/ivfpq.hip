@@ -1054,11 +1054,29 @@ static size_t lists_lds(uint32_t M, uint32_t KS, uint32_t sdm, uint32_t dpad, ui
     return ((size_t)M * KS * sdm + (size_t)M * KS) * 4 + region;
 }
 
+// Dynamic LDS available to the list-major scan: the per-workgroup limit
+// (160 KiB on gfx950) less the kernel's own static LDS.
+static size_t lists_lds_budget()
+{
+    static size_t budget = 0;
+    if (!budget) {
+        hipFuncAttributes attr{};
+        HM_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(ivf_scan_lists_kernel)));
+        int dev = 0, max_lds = 0;
+        HM_HIP(hipGetDevice(&dev));
+        HM_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        const size_t lim = std::min<size_t>((size_t)max_lds, 160 * 1024);
+        budget = lim > attr.sharedSizeBytes ? lim - attr.sharedSizeBytes : 1;
+    }
+    return budget;
+}
+
 uint32_t ivf_scan_lists_pool(uint32_t M, uint32_t KS, uint32_t sdm, uint32_t dpad, uint32_t k1)
 {
+    const size_t budget = lists_lds_budget();
     uint32_t P = 8192;
-    while (P > 512 && lists_lds(M, KS, sdm, dpad, P) > 160 * 1024) P >>= 1;
-    return (P >= 2 * k1 && lists_lds(M, KS, sdm, dpad, P) <= 160 * 1024) ? P : 0;
+    while (P > 512 && lists_lds(M, KS, sdm, dpad, P) > budget) P >>= 1;
+    return (P >= 2 * k1 && lists_lds(M, KS, sdm, dpad, P) <= budget) ? P : 0;
 }
 
 void launch_ivf_scan_lists(const float *Qp, uint32_t dpad, const float *C,
@@ -1072,7 +1090,8 @@ void launch_ivf_scan_lists(const float *Qp, uint32_t dpad, const float *C,
 {
     if (rows == 0 || nprobe == 0) return;
     const uint32_t P = ivf_scan_lists_pool(M, KS, sdm, dpad, k1);
-    if (!P) fail(HM_ERR_INVALID_ARG, "list-major scan does not fit in LDS");
+    if (!P || lists_lds(M, KS, sdm, dpad, P) > lists_lds_budget())
+        fail(HM_ERR_INVALID_ARG, "list-major scan does not fit in LDS");
     const uint64_t total = (uint64_t)rows * nprobe;
     if (total > 0x7FFFFFFFull) fail(HM_ERR_INVALID_ARG, "too many probe pairs");
     ProfScope prof("ivf_scan", s);

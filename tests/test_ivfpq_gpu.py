@@ -120,6 +120,26 @@ def test_large_codebook_scan(orc):
     _check_conditional(orc, idx, x, Metric.Euclidean, q, 3, 10)
 
 
+def test_long_lists_select(orc):
+    # lists of ~10,000 items exceed the list-major scan's sorting pool, so the
+    # k+1 smallest are found by the two histogram passes first
+    x, _, _ = gaussian_mixture(40_000, 16, 4, seed=19, sigma=0.3)
+    idx = _build(x, Metric.Euclidean, 4, 4, 4, 3, 29)
+    q, _, _ = gaussian_mixture(48, 16, 4, seed=20, sigma=0.3)
+    _check_conditional(orc, idx, x, Metric.Euclidean, q, 2, 10)
+
+
+def test_long_lists_all_tied(orc):
+    # 50 distinct points repeated 800 times: whole lists share a handful of
+    # ADC distances, so the threshold bin overflows the pool (chunked
+    # fallback) and every query goes through the heap replay
+    base, _, _ = gaussian_mixture(50, 8, 4, seed=21, sigma=0.3)
+    x = np.repeat(base, 800, axis=0)
+    idx = _build(x, Metric.Euclidean, 4, 2, 4, 3, 31)
+    q, _, _ = gaussian_mixture(16, 8, 4, seed=22, sigma=0.3)
+    _check_conditional(orc, idx, x, Metric.Euclidean, q, 2, 10)
+
+
 def test_ties_replay(orc):
     # every row twice: equal codes give equal ADC distances in every list
     x, _, _ = gaussian_mixture(800, 12, 6, seed=15, sigma=0.3)
