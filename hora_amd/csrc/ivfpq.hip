@@ -517,8 +517,9 @@ __device__ void lm_find_bin(const uint32_t *hist, uint32_t need, uint32_t *s_par
 // are split over several workgroups).  The unit's workgroup stages the
 // list's codebook in LDS once; per pair it builds the LUT, and selects the
 // k1 smallest (fkey(adc) << 32 | id) of the list: lists up to P items are
-// sorted whole; longer ones take two 11-bit histogram passes over the ADC
-// keys to find a threshold, then collect the keys under it and sort those.
+// sorted whole; longer ones keep their ADC keys in registers (up to 16 per
+// thread), take two 11-bit histogram passes over them to find a threshold,
+// and sort the keys under it; longer lists still are sorted in chunks.
 __global__ __launch_bounds__(LM_THREADS) void ivf_scan_lists_kernel(
     const float *__restrict__ Qp, uint32_t dpad, const float *__restrict__ C,
     const uint32_t *__restrict__ pair_off, const uint32_t *__restrict__ unit_off, uint32_t nlist,
@@ -558,11 +559,24 @@ __global__ __launch_bounds__(LM_THREADS) void ivf_scan_lists_kernel(
         for (uint32_t e = tid; e < ncb; e += LM_THREADS) cbs[e] = cb[e];
     }
     const uint32_t base = off[l], L = off[l + 1] - base;
+    // ADC of item i in the reference's order (a fold from +0.0 over the
+    // sub-spaces); codes are read a word at a time when M is a multiple of 4
     auto adc_key = [&](uint32_t i) -> unsigned long long {
         const uint32_t row = base + i;
         const uint8_t *cd = codes + (size_t)row * M;
         float dist = 0.0f;
-        for (uint32_t j = 0; j < M; ++j) dist = dist + lut[j * KS + cd[j]];
+        if ((M & 3) == 0) {
+            const uint32_t *cw = reinterpret_cast<const uint32_t *>(cd);
+            for (uint32_t w = 0; w < M / 4; ++w) {
+                const uint32_t v = cw[w];
+                dist = dist + lut[(4 * w) * KS + (v & 0xFFu)];
+                dist = dist + lut[(4 * w + 1) * KS + ((v >> 8) & 0xFFu)];
+                dist = dist + lut[(4 * w + 2) * KS + ((v >> 16) & 0xFFu)];
+                dist = dist + lut[(4 * w + 3) * KS + (v >> 24)];
+            }
+        } else {
+            for (uint32_t j = 0; j < M; ++j) dist = dist + lut[j * KS + cd[j]];
+        }
         return ((unsigned long long)fkey(dist) << 32) | ids[row];
     };
     auto sort_keys = [&](uint32_t total) {
@@ -584,6 +598,7 @@ __global__ __launch_bounds__(LM_THREADS) void ivf_scan_lists_kernel(
                 __syncthreads();
             }
     };
+    constexpr uint32_t RK = 16;  // keys held in registers per thread (lists up to 8192)
     for (uint32_t pp = p0; pp < p1; ++pp) {
         const uint32_t e = pairs[pp], q = e / nprobe;
         __syncthreads();  // previous pair done with the region and the LUT
@@ -606,43 +621,49 @@ __global__ __launch_bounds__(LM_THREADS) void ivf_scan_lists_kernel(
             for (uint32_t i = tid; i < L; i += LM_THREADS) keys[i] = adc_key(i);
             sort_keys(L);
             kept = L < k1 ? L : k1;
-        } else {
-            // level 1: top 11 bits of the ADC key
+        } else if (L <= RK * LM_THREADS) {
+            // every key of the list in registers: two 11-bit histogram
+            // passes find a threshold, the keys under it are sorted
+            unsigned long long kr[RK];
+#pragma unroll
+            for (uint32_t r = 0; r < RK; ++r) {
+                const uint32_t i = tid + r * LM_THREADS;
+                kr[r] = i < L ? adc_key(i) : ~0ull;
+            }
             for (uint32_t j = tid; j < LM_BINS; j += LM_THREADS) hist[j] = 0;
             __syncthreads();
-            for (uint32_t i = tid; i < L; i += LM_THREADS)
-                atomicAdd(&hist[(uint32_t)(adc_key(i) >> 53)], 1u);
+#pragma unroll
+            for (uint32_t r = 0; r < RK; ++r)
+                if (tid + r * LM_THREADS < L) atomicAdd(&hist[(uint32_t)(kr[r] >> 53)], 1u);
             __syncthreads();
             uint32_t b1, r1;
             lm_find_bin(hist, k1, s_part, &b1, &r1);
-            // level 2: the next 11 bits inside that bin
             for (uint32_t j = tid; j < LM_BINS; j += LM_THREADS) hist[j] = 0;
             __syncthreads();
-            for (uint32_t i = tid; i < L; i += LM_THREADS) {
-                const unsigned long long key = adc_key(i);
-                if ((uint32_t)(key >> 53) == b1) atomicAdd(&hist[(uint32_t)(key >> 42) & 2047u], 1u);
-            }
+#pragma unroll
+            for (uint32_t r = 0; r < RK; ++r)
+                if (tid + r * LM_THREADS < L && (uint32_t)(kr[r] >> 53) == b1)
+                    atomicAdd(&hist[(uint32_t)(kr[r] >> 42) & 2047u], 1u);
             __syncthreads();
             uint32_t b2, r2;
             lm_find_bin(hist, r1, s_part, &b2, &r2);
             const uint32_t thr = (b1 << 11) | b2;  // keep keys with key >> 42 <= thr
             if (tid == 0) s_n = 0;
             __syncthreads();
-            for (uint32_t i = tid; i < L; i += LM_THREADS) {
-                const unsigned long long key = adc_key(i);
-                if ((uint32_t)(key >> 42) <= thr) {
+#pragma unroll
+            for (uint32_t r = 0; r < RK; ++r)
+                if (tid + r * LM_THREADS < L && (uint32_t)(kr[r] >> 42) <= thr) {
                     const uint32_t pos = atomicAdd(&s_n, 1u);
-                    if (pos < P) keys[pos] = key;
+                    if (pos < P) keys[pos] = kr[r];
                 }
-            }
             __syncthreads();
             const uint32_t total = s_n;
             if (total <= P) {
                 sort_keys(total);
                 kept = total < k1 ? total : k1;
             } else {
-                // the threshold bin holds more than P keys: chunked sort of
-                // the whole list, keeping the k1 smallest of each chunk
+                // the threshold bin holds more than P keys (massive ties):
+                // chunked sort of the whole list keeping the k1 smallest
                 __syncthreads();
                 const uint32_t CH = P - k1;
                 for (uint32_t c0 = 0; c0 < L; c0 += CH) {
@@ -651,6 +672,15 @@ __global__ __launch_bounds__(LM_THREADS) void ivf_scan_lists_kernel(
                     sort_keys(kept + m);
                     kept = kept + m < k1 ? kept + m : k1;
                 }
+            }
+        } else {
+            // very long list: chunked sort keeping the k1 smallest
+            const uint32_t CH = P - k1;
+            for (uint32_t c0 = 0; c0 < L; c0 += CH) {
+                const uint32_t m = L - c0 < CH ? L - c0 : CH;
+                for (uint32_t i = tid; i < m; i += LM_THREADS) keys[kept + i] = adc_key(c0 + i);
+                sort_keys(kept + m);
+                kept = kept + m < k1 ? kept + m : k1;
             }
         }
         unsigned long long *o = out + (size_t)e * k1;

@@ -120,23 +120,25 @@ def test_large_codebook_scan(orc):
     _check_conditional(orc, idx, x, Metric.Euclidean, q, 3, 10)
 
 
-def test_long_lists_select(orc):
-    # lists of ~10,000 items exceed the list-major scan's sorting pool, so the
-    # k+1 smallest are found by the two histogram passes first
-    x, _, _ = gaussian_mixture(40_000, 16, 4, seed=19, sigma=0.3)
-    idx = _build(x, Metric.Euclidean, 4, 4, 4, 3, 29)
-    q, _, _ = gaussian_mixture(48, 16, 4, seed=20, sigma=0.3)
+@pytest.mark.parametrize("n", [12_000, 40_000], ids=["registers", "chunked"])
+def test_long_lists_select(orc, n):
+    # 16 x 256 codewords of 8 dims leave room for a 1,024-key sorting pool,
+    # so lists of ~3,000 take the histogram select over register-held keys
+    # and lists of ~10,000 the chunked sort
+    x, _, _ = gaussian_mixture(n, 128, 4, seed=19, sigma=0.3)
+    idx = _build(x, Metric.Euclidean, 4, 16, 8, 2, 29)
+    q, _, _ = gaussian_mixture(32, 128, 4, seed=20, sigma=0.3)
     _check_conditional(orc, idx, x, Metric.Euclidean, q, 2, 10)
 
 
 def test_long_lists_all_tied(orc):
-    # 50 distinct points repeated 800 times: whole lists share a handful of
-    # ADC distances, so the threshold bin overflows the pool (chunked
-    # fallback) and every query goes through the heap replay
-    base, _, _ = gaussian_mixture(50, 8, 4, seed=21, sigma=0.3)
-    x = np.repeat(base, 800, axis=0)
-    idx = _build(x, Metric.Euclidean, 4, 2, 4, 3, 31)
-    q, _, _ = gaussian_mixture(16, 8, 4, seed=22, sigma=0.3)
+    # 60 distinct points repeated 80 times: whole lists share a handful of ADC
+    # distances, so the threshold bin overflows the pool (chunked fallback)
+    # and every query goes through the heap replay
+    base, _, _ = gaussian_mixture(60, 128, 4, seed=21, sigma=0.3)
+    x = np.repeat(base, 80, axis=0)
+    idx = _build(x, Metric.Euclidean, 2, 16, 8, 2, 31)
+    q, _, _ = gaussian_mixture(16, 128, 4, seed=22, sigma=0.3)
     _check_conditional(orc, idx, x, Metric.Euclidean, q, 2, 10)
 
 
