@@ -132,14 +132,15 @@ def test_long_lists_select(orc, n):
 
 
 def test_long_lists_all_tied(orc):
-    # 60 distinct points repeated 80 times: whole lists share a handful of ADC
-    # distances, so the threshold bin overflows the pool (chunked fallback)
-    # and every query goes through the heap replay
-    base, _, _ = gaussian_mixture(60, 128, 4, seed=21, sigma=0.3)
-    x = np.repeat(base, 80, axis=0)
-    idx = _build(x, Metric.Euclidean, 2, 16, 8, 2, 31)
-    q, _, _ = gaussian_mixture(16, 128, 4, seed=22, sigma=0.3)
-    _check_conditional(orc, idx, x, Metric.Euclidean, q, 2, 10)
+    # 3 distinct points repeated 1,500 times in one list: its ADC keys take
+    # 3 values, the threshold bin holds 1,500 > 1,024 keys, so the select
+    # falls back to the chunked sort, and every query goes through the
+    # heap replay
+    base, _, _ = gaussian_mixture(3, 128, 3, seed=21, sigma=0.3)
+    x = np.repeat(base, 1500, axis=0)
+    idx = _build(x, Metric.Euclidean, 1, 16, 8, 2, 31, nprobe=1)
+    q, _, _ = gaussian_mixture(8, 128, 3, seed=22, sigma=0.3)
+    _check_conditional(orc, idx, x, Metric.Euclidean, q, 1, 10)
 
 
 def test_ties_replay(orc):
