@@ -822,6 +822,173 @@ struct GHeap {
 // (pq_idx.rs:408-477) for flagged queries, one lane per query: coarse heap
 // of Neighbor(i, -dist) over all lists, nprobe pops, per list an unbounded
 // heap trimmed to k, popped into the global heap trimmed to k.
+// Replay of pq_idx.rs:408-439 for one tied query per workgroup, with the
+// heaps in LDS.  The coarse heap and the global top-k heap are simulated by
+// wave 0 (all its lanes execute the same operations on the same LDS words);
+// per probed list the LUT and the ADC values are computed by the whole
+// workgroup.  A list's own push-all / pop-to-k phase (pq_idx.rs:165-194)
+// only needs simulating when equal values occur among its k+1 smallest:
+// otherwise its survivors are exactly its k smallest, popped in descending
+// order, which a sort gives directly.
+constexpr int RP_THREADS = 256;
+
+__global__ __launch_bounds__(RP_THREADS) void ivf_replay_lds_kernel(
+    const uint32_t *__restrict__ rows, const float *__restrict__ Dc, uint32_t ldd, uint32_t nlist,
+    uint32_t nprobe, const float *__restrict__ Qp, uint32_t dpad, const float *__restrict__ C,
+    const uint32_t *__restrict__ off, const uint32_t *__restrict__ ids,
+    const uint8_t *__restrict__ codes, const float *__restrict__ codebooks,
+    const uint32_t *__restrict__ sub_b, const uint32_t *__restrict__ sub_e, uint32_t M,
+    uint32_t KS, uint32_t sdm, uint32_t k, uint32_t Lcap, uint32_t Pcap,
+    const uint32_t *__restrict__ probes, uint32_t ldp, const uint32_t *__restrict__ nprobe_q,
+    uint64_t *out_idx, float *out_dist, uint32_t *out_count)
+{
+    extern __shared__ __align__(16) unsigned char rp_smem[];
+    __shared__ uint32_t s_tied, s_np;
+    const uint32_t q = rows[blockIdx.x];
+    const int tid = threadIdx.x;
+    const bool w0 = tid < 64;
+    unsigned long long *sk = reinterpret_cast<unsigned long long *>(rp_smem);  // [Pcap]
+    float *lut = reinterpret_cast<float *>(sk + Pcap);                         // [M*KS]
+    float *hd = lut + (size_t)M * KS;                                           // [Lcap]
+    uint32_t *hi = reinterpret_cast<uint32_t *>(hd + Lcap);                     // [Lcap]
+    float *cdd = reinterpret_cast<float *>(hi + Lcap);                          // [nlist]
+    uint32_t *cdi = reinterpret_cast<uint32_t *>(cdd + nlist);                  // [nlist]
+    float *gd = reinterpret_cast<float *>(cdi + nlist);                         // [k+1]
+    uint32_t *gi = reinterpret_cast<uint32_t *>(gd + (k + 1));                  // [k+1]
+    uint32_t *order = gi + (k + 1);                                             // [nprobe]
+    // probe order
+    if (w0) {
+        uint32_t np_q = nprobe;
+        if (probes) {
+            np_q = nprobe_q[q];
+            for (uint32_t p = tid; p < np_q; p += 64) order[p] = probes[(size_t)q * ldp + p];
+        } else {
+            GHeap coarse{cdd, cdi, 0};
+            const float *drow = Dc + (size_t)q * ldd;
+            for (uint32_t i = 0; i < nlist; ++i) coarse.push(-drow[i], i);
+            uint32_t p = 0;
+            for (; p < np_q && coarse.len > 0; ++p) {
+                float xd;
+                uint32_t l;
+                coarse.pop(&xd, &l);
+                order[p] = l;
+            }
+            np_q = p;
+        }
+        if (tid == 0) s_np = np_q;
+    }
+    __syncthreads();
+    const uint32_t np_q = s_np;
+    GHeap top{gd, gi, 0};  // used by wave 0 only
+    const float *qv = Qp + (size_t)q * dpad;
+    for (uint32_t p = 0; p < np_q; ++p) {
+        const uint32_t l = order[p];
+        const float *cb = codebooks + (size_t)l * M * KS * sdm;
+        for (uint32_t e = tid; e < M * KS; e += RP_THREADS) {
+            const uint32_t sub = e / KS, b = sub_b[sub], sd = sub_e[sub] - b;
+            float acc = -0.0f;
+            for (uint32_t j = 0; j < sd; ++j) {
+                float zj = qv[b + j] - C[(size_t)l * dpad + b + j];
+                float df = zj - cb[(size_t)e * sdm + j];
+                acc = acc + df * df;
+            }
+            lut[e] = acc;
+        }
+        if (tid == 0) s_tied = 0;
+        __syncthreads();
+        const uint32_t base = off[l], L = off[l + 1] - base;
+        uint32_t P = 1;
+        while (P < L) P <<= 1;
+        for (uint32_t i = tid; i < P; i += RP_THREADS) {
+            if (i < L) {
+                const uint8_t *cdp = codes + (size_t)(base + i) * M;
+                float dist = 0.0f;
+                for (uint32_t j = 0; j < M; ++j) dist = dist + lut[j * KS + cdp[j]];
+                hd[i] = dist;
+                hi[i] = i;
+                sk[i] = ((unsigned long long)fkey(dist) << 32) | i;
+            } else {
+                sk[i] = ~0ull;
+            }
+        }
+        __syncthreads();
+        for (uint32_t size = 2; size <= P; size <<= 1)
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t t = tid; t < P / 2; t += RP_THREADS) {
+                    const uint32_t a0 = 2 * stride * (t / stride) + (t % stride), a1 = a0 + stride;
+                    const bool asc = (a0 & size) == 0;
+                    const unsigned long long a = sk[a0], bb = sk[a1];
+                    if ((a > bb) == asc) {
+                        sk[a0] = bb;
+                        sk[a1] = a;
+                    }
+                }
+                __syncthreads();
+            }
+        const uint32_t K1 = k + 1 < L ? k + 1 : L;
+        for (uint32_t j = tid; j + 1 < K1; j += RP_THREADS)
+            if ((uint32_t)(sk[j] >> 32) == (uint32_t)(sk[j + 1] >> 32)) s_tied = 1;
+        __syncthreads();
+        if (w0) {
+            const uint32_t m = k < L ? k : L;
+            if (!s_tied) {
+                // survivors: the m smallest, popped in descending order
+                for (uint32_t j = m; j > 0; --j) {
+                    const unsigned long long v = sk[j - 1];
+                    top.push(unkey((uint32_t)(v >> 32)), ids[base + (uint32_t)v]);
+                    if (top.len > k) {
+                        float a;
+                        uint32_t b;
+                        top.pop(&a, &b);
+                    }
+                }
+            } else {
+                GHeap lh{hd, hi, 0};
+                for (uint32_t i = 0; i < L; ++i) {  // item i sits at position i until pushed
+                    const float x = hd[i];
+                    lh.len = i;
+                    lh.push(x, i);
+                }
+                while (lh.len > k) {
+                    float a;
+                    uint32_t b;
+                    lh.pop(&a, &b);
+                }
+                while (lh.len > 0) {
+                    float a;
+                    uint32_t b;
+                    lh.pop(&a, &b);
+                    top.push(a, ids[base + b]);
+                    if (top.len > k) {
+                        float a2;
+                        uint32_t b2;
+                        top.pop(&a2, &b2);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (w0) {
+        const uint32_t cnt = top.len;
+        for (uint32_t p = cnt; p > 0; --p) {
+            float a;
+            uint32_t b;
+            top.pop(&a, &b);
+            if (tid == 0) {
+                out_idx[(size_t)q * k + p - 1] = b;
+                out_dist[(size_t)q * k + p - 1] = a;
+            }
+        }
+        const float inf = __int_as_float(0x7f800000);
+        for (uint32_t j = cnt + tid; j < k; j += 64) {
+            out_idx[(size_t)q * k + j] = ~0ull;
+            out_dist[(size_t)q * k + j] = inf;
+        }
+        if (tid == 0) out_count[q] = cnt;
+    }
+}
+
 __global__ void ivf_replay_kernel(const uint32_t *__restrict__ rows, uint32_t nrows,
                                   const float *__restrict__ Dc, uint32_t ldd, uint32_t nlist,
                                   uint32_t nprobe, const float *__restrict__ Qp, uint32_t dpad,
@@ -1172,6 +1339,38 @@ void launch_ivf_replay(const uint32_t *rows, uint32_t nrows, const float *Dc, ui
         sub_e, M, KS, sdm, k, lmax, scratch_f, scratch_u, per_thread, probes, ldp, nprobe_q,
         out_idx, out_dist, out_count);
     HM_CHECK_LAUNCH();
+}
+
+static size_t replay_lds_bytes(uint32_t nlist, uint32_t nprobe, uint32_t M, uint32_t KS,
+                               uint32_t k, uint32_t Lcap, uint32_t Pcap)
+{
+    return (size_t)Pcap * 8 + (size_t)M * KS * 4 + (size_t)Lcap * 8 + (size_t)nlist * 8 +
+           (size_t)(k + 1) * 8 + (size_t)nprobe * 4;
+}
+
+bool launch_ivf_replay_lds(const uint32_t *rows, uint32_t nrows, const float *Dc, uint32_t ldd,
+                           uint32_t nlist, uint32_t nprobe, const float *Qp, uint32_t dpad,
+                           const float *C, const uint32_t *off, const uint32_t *ids,
+                           const uint8_t *codes, const float *codebooks, const uint32_t *sub_b,
+                           const uint32_t *sub_e, uint32_t M, uint32_t KS, uint32_t sdm,
+                           uint32_t k, uint32_t lmax, const uint32_t *probes, uint32_t ldp,
+                           const uint32_t *nprobe_q, uint64_t *out_idx, float *out_dist,
+                           uint32_t *out_count, hipStream_t s)
+{
+    if (nrows == 0) return true;
+    const uint32_t Lcap = std::max<uint32_t>(lmax, 1);
+    uint32_t Pcap = 1;
+    while (Pcap < Lcap) Pcap <<= 1;
+    hipFuncAttributes attr{};
+    HM_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(ivf_replay_lds_kernel)));
+    const size_t lds = replay_lds_bytes(nlist, nprobe, M, KS, k, Lcap, Pcap);
+    if (lds + attr.sharedSizeBytes > 160 * 1024) return false;  // the caller uses the global-memory form
+    ProfScope prof("ivf_replay", s);
+    ivf_replay_lds_kernel<<<nrows, RP_THREADS, lds, s>>>(
+        rows, Dc, ldd, nlist, nprobe, Qp, dpad, C, off, ids, codes, codebooks, sub_b, sub_e, M,
+        KS, sdm, k, Lcap, Pcap, probes, ldp, nprobe_q, out_idx, out_dist, out_count);
+    HM_CHECK_LAUNCH();
+    return true;
 }
 
 void launch_negate(const float *x, float *y, uint64_t n, hipStream_t s)

@@ -418,9 +418,21 @@ class IVFPQ final : public Index {
         const uint32_t m = (uint32_t)tied.size();
         const size_t per = (size_t)nlist_ + max_list_ + (k + 1) + (size_t)M_ * KS_;
         rp_rows_.reserve(m * 4);
+        HM_HIP(hipMemcpyAsync(rp_rows_.get(), tied.data(), m * 4, hipMemcpyHostToDevice, s));
+        const uint32_t np_max = learned ? ldp : nprobe_;
+        if (launch_ivf_replay_lds(rp_rows_.as<uint32_t>(), m, D_.as<float>(), (uint32_t)ldd,
+                                  nlist_, np_max, qpad_.as<float>(), dpad_, centers_.as<float>(),
+                                  off_.as<uint32_t>(), ids_.as<uint32_t>(), codes_.as<uint8_t>(),
+                                  codebooks_.as<float>(), sub_b_.as<uint32_t>(),
+                                  sub_e_.as<uint32_t>(), M_, KS_, sdm_, k, max_list_,
+                                  learned ? probe_i_.as<uint32_t>() : nullptr, ldp,
+                                  learned ? nprobe_q_.as<uint32_t>() : nullptr, d_idx, d_dist,
+                                  d_count, s)) {
+            HM_HIP(hipStreamSynchronize(s));
+            return;
+        }
         rp_f_.reserve(m * per * 4);
         rp_u_.reserve(m * per * 4);
-        HM_HIP(hipMemcpyAsync(rp_rows_.get(), tied.data(), m * 4, hipMemcpyHostToDevice, s));
         launch_ivf_replay(rp_rows_.as<uint32_t>(), m, D_.as<float>(), (uint32_t)ldd, nlist_,
                           nprobe_, qpad_.as<float>(), dpad_, centers_.as<float>(),
                           off_.as<uint32_t>(), ids_.as<uint32_t>(), codes_.as<uint8_t>(),
