@@ -31,7 +31,6 @@ constexpr int GM = 128, GN = 128;
 constexpr int ROWB = 128;           // bytes per row per K step (64 bf16 / 128 fp8)
 constexpr int LDS_ROW = ROWB + 16;  // padded LDS row stride (bytes)
 constexpr int CPT = GM * ROWB / 16 / 256;  // 16-byte chunks per thread per operand (4)
-constexpr uint32_t GROUP_B = 8;     // base tiles per group of the tile order
 
 __device__ __forceinline__ uint32_t fkey(float f)
 {
@@ -55,18 +54,19 @@ struct ScoreOut {
 // Tile order.  The workgroups are dispatched round robin over the 8 XCDs
 // (workgroup L to XCD L % 8), each XCD with its own L2.  The linear id is
 // remapped so that every XCD walks a contiguous range of tiles, and tiles
-// are ordered in groups of GROUP_B base tiles with the query tiles inside a
-// group: the group's base tiles stay in that XCD's L2 while the (smaller)
-// query matrix streams past them, and the base is read from HBM once.
-__device__ __forceinline__ void tile_of(uint32_t L, uint32_t TX, uint32_t TY, uint32_t *bt,
-                                        uint32_t *qt)
+// are ordered in groups of G base tiles with the query tiles inside a
+// group: the group's base tiles (about 2 MiB) stay in that XCD's L2 while
+// the query matrix streams past them, so the base is read once and the
+// queries once per group.
+__device__ __forceinline__ void tile_of(uint32_t L, uint32_t TX, uint32_t TY, uint32_t G,
+                                        uint32_t *bt, uint32_t *qt)
 {
     const uint32_t T = TX * TY, q = T / 8, r = T % 8, x = L % 8, i = L / 8;
     const uint32_t t = x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-    const uint32_t per = GROUP_B * TY, g = t / per, w = t % per;
-    const uint32_t gb = min(GROUP_B, TX - g * GROUP_B);
+    const uint32_t per = G * TY, g = t / per, w = t % per;
+    const uint32_t gb = min(G, TX - g * G);
     *qt = w / gb;
-    *bt = g * GROUP_B + w % gb;
+    *bt = g * G + w % gb;
 }
 
 // Software pipeline: the global loads of step kt+2 are issued before the
@@ -77,13 +77,13 @@ __device__ __forceinline__ void tile_of(uint32_t L, uint32_t TX, uint32_t TY, ui
 // register was left unset on the last arm).
 template <int ELEM, int MET, bool FILTER>  // ELEM: 2 = bf16, 1 = fp8
 __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
-    const uint8_t *__restrict__ Q, uint32_t nq, uint32_t TY, const uint8_t *__restrict__ B,
-    uint32_t n, uint32_t row_bytes, const float *__restrict__ qscale,
+    const uint8_t *__restrict__ Q, uint32_t nq, uint32_t TY, uint32_t G,
+    const uint8_t *__restrict__ B, uint32_t n, uint32_t row_bytes, const float *__restrict__ qscale,
     const float *__restrict__ bscale, const float *__restrict__ bnorm2,
     const float *__restrict__ inv_bnorm, ScoreOut out, const uint32_t *__restrict__ dev_rows)
 {
     uint32_t btile, qtile;
-    tile_of(blockIdx.x, (n + GN - 1) / GN, TY, &btile, &qtile);
+    tile_of(blockIdx.x, (n + GN - 1) / GN, TY, G, &btile, &qtile);
     if (dev_rows && *dev_rows < nq) nq = *dev_rows;
     if (qtile * GM >= nq) return;
     __shared__ __align__(16) uint8_t sA[2][GM * LDS_ROW];
@@ -393,10 +393,13 @@ static void score_launch(int elem_bytes, const void *Q, uint32_t nq, const void 
     const uint64_t TX = ceil_div(n, GN), TY = ceil_div(nq, GM);
     if (TX * TY > 0x7FFFFFull) fail(HM_ERR_INVALID_ARG, "score_gemm: too many tiles");
     const dim3 grid((unsigned)(TX * TY));
+    // base tiles per group: about 2 MiB of base rows per XCD's L2
+    const uint32_t G = (uint32_t)std::min<uint64_t>(
+        32, std::max<uint64_t>(2, (2u << 20) / ((uint64_t)GN * row_bytes)));
 #define HM_SG(E, M)                                                                           \
     score_gemm_kernel<E, M, FILTER><<<grid, 256, 0, s>>>(                                     \
-        (const uint8_t *)Q, nq, (uint32_t)TY, (const uint8_t *)B, n, row_bytes, qscale, bscale, \
-        bnorm2, inv_bnorm, out, dev_rows)
+        (const uint8_t *)Q, nq, (uint32_t)TY, G, (const uint8_t *)B, n, row_bytes, qscale,     \
+        bscale, bnorm2, inv_bnorm, out, dev_rows)
 #define HM_SG_METRIC(E)                                                                       \
     switch (metric) {                                                                         \
     case HM_METRIC_EUCLIDEAN: HM_SG(E, HM_METRIC_EUCLIDEAN); break;                           \
