@@ -80,6 +80,22 @@ class Timer:
         return wall, e0.elapsed_time(e1) / 1e3
 
 
+def measured_traffic(form):
+    """HBM bytes per launch of the filtered scan, from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs summarised in profiles/score_filter_hbm_traffic.md (None if
+    absent)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "score_filter_hbm_traffic.md")
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == form:
+                return int(f[2])
+    except OSError:
+        pass
+    return None
+
+
 def kernel_ms(names):
     from hora_amd._lib import profile_query
     out = {}
@@ -189,7 +205,8 @@ def main_bf(args):
             ach = work / avg / 1e12
             roofline = {"bound": "mfma", "kernel": f"{kname}_kernel<{best}>",
                         "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
-                        "frac": round(ach / peak, 4), "traffic": None,
+                        "frac": round(ach / peak, 4),
+                        "traffic": measured_traffic(best) if kname == "score_filter" else None,
                         "avg_launch_ms": round(avg * 1e3, 3)}
 
     per_metric = {"Euclidean": b["qps"]}
