@@ -335,7 +335,7 @@ class BruteForce final : public Index {
             launch_list_prepare(rg_list_.as<uint64_t>(), fp.cap, rg_lcnt_.as<uint32_t>(), tau, fp.m,
                                 QC, S, rg_ci_.as<uint32_t>(), rg_cd_.as<float>(), S,
                                 rg_sq_.as<uint32_t>(), rg_ksq_.as<float>(), rg_pref_.as<uint32_t>(),
-                                g, dr);
+                                g, dr, shortlist_ ? nullptr : rg_ck_.as<uint32_t>());
             s_q = rg_sq_.as<uint32_t>();
             key_s = rg_ksq_.as<float>();
             pre = rg_pref_.as<uint32_t>();

@@ -102,12 +102,13 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
 // per query the first min(cnt, S) entries by (key, row) into cand_i /
 // cand_key (stride ldc), s_q = that count, key_s_q = a lower bound of the
 // key of every row left out (the smallest list key left out, else tau),
-// flags = ROW_UNCERT when the list overflowed.
+// flags = ROW_UNCERT when the list overflowed.  kq (ragged batches, when the
+// depth is the automatic one): query r keeps min(S, max(64, 4(kq[r]+1))).
 void launch_list_prepare(const uint64_t *list, uint32_t cap, const uint32_t *cnt,
                          const float *tau, uint32_t tau_ld, uint32_t rows, uint32_t S,
                          uint32_t *cand_i, float *cand_key, uint32_t ldc, uint32_t *s_q,
                          float *key_s_q, uint32_t *flags, hipStream_t st,
-                         const uint32_t *dev_rows = nullptr);
+                         const uint32_t *dev_rows = nullptr, const uint32_t *kq = nullptr);
 void launch_scatter_ragged(const uint64_t *si, const float *sd, const uint32_t *sc,
                            const uint32_t *rows, uint32_t nrows, uint32_t kmax, const uint32_t *kq,
                            const uint64_t *out_off, uint64_t *di, float *dd, uint32_t *dc,

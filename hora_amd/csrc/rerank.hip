@@ -208,10 +208,16 @@ __global__ __launch_bounds__(LP_THREADS) void list_prepare_kernel(
     const uint64_t *__restrict__ list, uint32_t cap, const uint32_t *__restrict__ cnt,
     const float *__restrict__ tau, uint32_t tau_ld, uint32_t S, uint32_t *__restrict__ cand_i,
     float *__restrict__ cand_key, uint32_t ldc, uint32_t *__restrict__ s_q,
-    float *__restrict__ key_s_q, uint32_t *__restrict__ flags, const uint32_t *__restrict__ dev_rows)
+    float *__restrict__ key_s_q, uint32_t *__restrict__ flags, const uint32_t *__restrict__ dev_rows,
+    const uint32_t *__restrict__ kq)
 {
     const uint32_t r = blockIdx.x;
     if (dev_rows && r >= *dev_rows) return;
+    if (kq) {  // ragged batches: this query's own default depth max(64, 4(k+1)), at most S
+        const uint32_t kr = kq[r];
+        const uint64_t own = kr >= 15 ? 4ull * (kr + 1) : 64ull;
+        S = (uint32_t)(own < S ? own : S);
+    }
     extern __shared__ __align__(16) unsigned long long lp_keys[];
     const int tid = threadIdx.x;
     const uint32_t raw = cnt[r], c = raw < cap ? raw : cap;
@@ -362,7 +368,7 @@ void launch_list_prepare(const uint64_t *list, uint32_t cap, const uint32_t *cnt
                          const float *tau, uint32_t tau_ld, uint32_t rows, uint32_t S,
                          uint32_t *cand_i, float *cand_key, uint32_t ldc, uint32_t *s_q,
                          float *key_s_q, uint32_t *flags, hipStream_t st,
-                         const uint32_t *dev_rows)
+                         const uint32_t *dev_rows, const uint32_t *kq)
 {
     if (rows == 0) return;
     if (cap == 0 || cap > RR_CAP || S == 0 || S > ldc)
@@ -371,7 +377,7 @@ void launch_list_prepare(const uint64_t *list, uint32_t cap, const uint32_t *cnt
     uint32_t P = 1;
     while (P < cap) P <<= 1;
     list_prepare_kernel<<<rows, LP_THREADS, (size_t)P * 8, st>>>(
-        list, cap, cnt, tau, tau_ld, S, cand_i, cand_key, ldc, s_q, key_s_q, flags, dev_rows);
+        list, cap, cnt, tau, tau_ld, S, cand_i, cand_key, ldc, s_q, key_s_q, flags, dev_rows, kq);
     HM_CHECK_LAUNCH();
 }
 
