@@ -144,8 +144,8 @@ def main_bf(args):
     for form in args.forms.split(","):
         ix = make(Metric.Euclidean, form)
         wall, ev = timer.run(step_fn(ix), args.steps, args.warmup, f"Euclidean/{form}")
-        km = kernel_ms(["exact_tile", "exact_rows", "row_select", "bf_finalize", "score_gemm",
-                        "score_filter", "list_prepare", "rerank"])
+        km = kernel_ms(["exact_tile", "exact_rows", "row_select", "kth_smallest", "bf_finalize",
+                        "score_gemm", "score_filter", "list_prepare", "rerank"])
         fb = ix.search_stats()
         ids_by_form[form] = d_idx.cpu().numpy().view(np.uint64).copy()
         forms[form] = {"qps": round(args.steps * NQ / wall, 1),

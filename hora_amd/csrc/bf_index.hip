@@ -324,15 +324,25 @@ class BruteForce final : public Index {
             launch_score_gemm(bf16 ? 2 : 1, rg_qlow_.get(), QC, bsamp_.get(), fp.ns, row_bytes,
                               rg_qscale_.as<float>(), bs_scale_.as<float>(), bs_norm2_.as<float>(),
                               bs_inv_.as<float>(), metric_, rg_D_.as<float>(), ldk, g, dr);
-            launch_row_select(rg_D_.as<float>(), ldk, QC, fp.ns, fp.m, rg_seld_.as<float>(),
-                              rg_seli_.as<uint32_t>(), fp.m, rg_sf_.as<uint32_t>(), g, dr);
+            const float *tau;
+            uint32_t tau_ld;
+            if (fp.m <= 32) {
+                launch_kth_smallest(rg_D_.as<float>(), ldk, QC, fp.ns, fp.m, rg_seld_.as<float>(), g,
+                                    dr);
+                tau = rg_seld_.as<float>();
+                tau_ld = 1;
+            } else {
+                launch_row_select(rg_D_.as<float>(), ldk, QC, fp.ns, fp.m, rg_seld_.as<float>(),
+                                  rg_seli_.as<uint32_t>(), fp.m, rg_sf_.as<uint32_t>(), g, dr);
+                tau = rg_seld_.as<float>() + (fp.m - 1);
+                tau_ld = fp.m;
+            }
             launch_fill_u32(rg_lcnt_.as<uint32_t>(), QC, 0, g);
-            const float *tau = rg_seld_.as<float>() + (fp.m - 1);
             launch_score_filter(bf16 ? 2 : 1, rg_qlow_.get(), QC, blow_.get(), n, row_bytes,
                                 rg_qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
-                                inv_bnorm_.as<float>(), metric_, tau, fp.m, rg_list_.as<uint64_t>(),
+                                inv_bnorm_.as<float>(), metric_, tau, tau_ld, rg_list_.as<uint64_t>(),
                                 fp.cap, rg_lcnt_.as<uint32_t>(), g, dr);
-            launch_list_prepare(rg_list_.as<uint64_t>(), fp.cap, rg_lcnt_.as<uint32_t>(), tau, fp.m,
+            launch_list_prepare(rg_list_.as<uint64_t>(), fp.cap, rg_lcnt_.as<uint32_t>(), tau, tau_ld,
                                 QC, S, rg_ci_.as<uint32_t>(), rg_cd_.as<float>(), S,
                                 rg_sq_.as<uint32_t>(), rg_ksq_.as<float>(), rg_pref_.as<uint32_t>(),
                                 g, dr, shortlist_ ? nullptr : rg_ck_.as<uint32_t>());
@@ -554,15 +564,25 @@ class BruteForce final : public Index {
                                   qscale_.as<float>(), bs_scale_.as<float>(),
                                   bs_norm2_.as<float>(), bs_inv_.as<float>(), metric_,
                                   D_.as<float>(), ldk, s);
-                launch_row_select(D_.as<float>(), ldk, rows, fp.ns, fp.m, sel_d_.as<float>(),
-                                  sel_i_.as<uint32_t>(), fp.m, sel_f_.as<uint32_t>(), s);
+                const float *tau;
+                uint32_t tau_ld;
+                if (fp.m <= 32) {
+                    launch_kth_smallest(D_.as<float>(), ldk, rows, fp.ns, fp.m, sel_d_.as<float>(),
+                                        s);
+                    tau = sel_d_.as<float>();
+                    tau_ld = 1;
+                } else {
+                    launch_row_select(D_.as<float>(), ldk, rows, fp.ns, fp.m, sel_d_.as<float>(),
+                                      sel_i_.as<uint32_t>(), fp.m, sel_f_.as<uint32_t>(), s);
+                    tau = sel_d_.as<float>() + (fp.m - 1);
+                    tau_ld = fp.m;
+                }
                 launch_fill_u32(lcnt_.as<uint32_t>(), rows, 0, s);
-                const float *tau = sel_d_.as<float>() + (fp.m - 1);
                 launch_score_filter(bf16 ? 2 : 1, qlow_.get(), rows, blow_.get(), n, row_bytes,
                                     qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
-                                    inv_bnorm_.as<float>(), metric_, tau, fp.m,
+                                    inv_bnorm_.as<float>(), metric_, tau, tau_ld,
                                     list_.as<uint64_t>(), fp.cap, lcnt_.as<uint32_t>(), s);
-                launch_list_prepare(list_.as<uint64_t>(), fp.cap, lcnt_.as<uint32_t>(), tau, fp.m,
+                launch_list_prepare(list_.as<uint64_t>(), fp.cap, lcnt_.as<uint32_t>(), tau, tau_ld,
                                     rows, S, cand_i_.as<uint32_t>(), cand_d_.as<float>(), S,
                                     sq_.as<uint32_t>(), ksq_.as<float>(), pref_.as<uint32_t>(), s);
                 s_q = sq_.as<uint32_t>();

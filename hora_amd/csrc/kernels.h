@@ -40,6 +40,10 @@ void launch_row_select(const float *D, uint64_t ldd, uint32_t rows, uint32_t n, 
                        float *cand_d, uint32_t *cand_i, uint32_t ldc, uint32_t *flags,
                        hipStream_t s, const uint32_t *dev_rows = nullptr);
 
+// out[r] = the m-th smallest value of D's row r (1 <= m <= 32; NaN skipped).
+void launch_kth_smallest(const float *D, uint64_t ldd, uint32_t rows, uint32_t n, uint32_t m,
+                         float *out, hipStream_t s, const uint32_t *dev_rows = nullptr);
+
 // Final result of node_search_k per row: rows without ROW_TIE copy the
 // first min(k, n) candidates; rows with ROW_TIE replay the reference's
 // std-BinaryHeap push/pop stream over the whole distance row.  A NaN row

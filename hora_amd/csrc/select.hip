@@ -319,7 +319,7 @@ struct LaneHeap {
 // Streams a distance row through LDS in batches of 4096 values (16 float4
 // loads per lane, the next batch in flight while the current one is
 // processed) and hands it over in chunks of up to 64 consecutive values:
-// on_chunk(first_index, values_in_lds, count).  Before a batch is handed
+// on_chunk(first_index, this_lane's_value, values_in_lds, count).  Before a batch is handed
 // over, skip_batch(batch_min) may drop it whole (the minimum is over the
 // batch's values with index < n; NaN is ignored by fminf, and rows with a
 // NaN never reach a replay).  Rows are 16-byte aligned with a stride ldd >=
@@ -365,10 +365,14 @@ __device__ void stream_row(const float *__restrict__ row, uint32_t n, uint64_t l
             HM_ROW_LOAD(b0 + BATCH)
         }
         if (skip) continue;
+        // the next chunk's value is read from LDS while this one is examined
+        float vcur = sv[lane];
         for (uint32_t c = 0; c < BATCH / 64; ++c) {
             const uint32_t base = b0 + c * 64;
             if (base >= n) break;
-            on_chunk(base, sv + c * 64, n - base < 64 ? n - base : 64u);
+            const float vnext = sv[min(c + 1, BATCH / 64 - 1) * 64 + lane];
+            on_chunk(base, vcur, sv + c * 64, n - base < 64 ? n - base : 64u);
+            vcur = vnext;
         }
     }
 #undef HM_ROW_LOAD
@@ -500,8 +504,8 @@ __global__ __launch_bounds__(64) void bf_finalize_wave_kernel(
     // a batch whose values all exceed the root of a full, stable heap
     // changes nothing
     auto skip_batch = [&](float bmin) { return h.len == k && stable && bmin > root; };
-    stream_row(row, n, ldd, lane, sv, skip_batch, [&](uint32_t base, const float *cv, uint32_t cnt) {
-        const float v = cv[lane];
+    stream_row(row, n, ldd, lane, sv, skip_batch,
+               [&](uint32_t base, float v, const float *, uint32_t cnt) {
         uint32_t j = 0;
         while (j < cnt) {
             if (h.len == k && stable) {
@@ -619,8 +623,8 @@ __global__ __launch_bounds__(64) void bf_replay_lds_kernel(
     float root = inf;
     __shared__ __align__(16) float sv[4096];
     auto skip_batch = [&](float bmin) { return h.len == k && stable && bmin > root; };
-    stream_row(row, n, ldd, lane, sv, skip_batch, [&](uint32_t base, const float *cv, uint32_t cnt) {
-        const float v = cv[lane];
+    stream_row(row, n, ldd, lane, sv, skip_batch,
+               [&](uint32_t base, float v, const float *cv, uint32_t cnt) {
         uint32_t j = 0;
         while (j < cnt) {
             if (h.len == k && stable) {
@@ -658,6 +662,60 @@ __global__ __launch_bounds__(64) void bf_replay_lds_kernel(
         od[j] = inf;
     }
     if (lane == 0) out_count[r] = cntk;
+}
+
+// out[r] = the m-th smallest value of row r (m <= KTH_MAX), one wave per
+// row: each lane keeps its m smallest in registers (insertion into a
+// sorted run), then m rounds of a wave minimum take the m-th overall.
+// NaN values are never taken.
+constexpr uint32_t KTH_MAX = 32;
+
+__global__ __launch_bounds__(256) void kth_smallest_kernel(const float *__restrict__ D,
+                                                           uint64_t ldd, uint32_t rows, uint32_t n,
+                                                           uint32_t m, float *__restrict__ out,
+                                                           const uint32_t *__restrict__ dev_rows)
+{
+    const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= rows || (dev_rows && r >= *dev_rows)) return;
+    const float inf = __int_as_float(0x7f800000);
+    const float *row = D + (size_t)r * ldd;
+    float best[KTH_MAX];
+#pragma unroll
+    for (uint32_t i = 0; i < KTH_MAX; ++i) best[i] = inf;
+    for (uint32_t i = lane; i < n; i += 64) {
+        float x = row[i];
+        if (!(x < best[KTH_MAX - 1])) continue;  // also drops NaN
+        // insert x into the sorted run best[0..KTH_MAX)
+#pragma unroll
+        for (uint32_t t = 0; t < KTH_MAX; ++t) {
+            const float b = best[t];
+            const bool lt = x < b;
+            best[t] = lt ? x : b;
+            x = lt ? b : x;
+        }
+    }
+    // m rounds: the wave minimum of the lanes' heads; its lane drops its head
+    float head = best[0];
+    uint32_t used = 0;
+    float kth = inf;
+    for (uint32_t round = 0; round < m; ++round) {
+        float w = head;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) w = fminf(w, __shfl_xor(w, o, 64));
+        kth = w;
+        const unsigned long long owner = __ballot(head == w);
+        if ((uint32_t)lane == (uint32_t)__builtin_ctzll(owner)) {
+            ++used;
+            // shift: the lane's next smallest becomes its head
+            float nxt = inf;
+#pragma unroll
+            for (uint32_t t = 1; t < KTH_MAX; ++t)
+                if (t == used) nxt = best[t];
+            head = used < KTH_MAX ? nxt : inf;
+        }
+    }
+    if (lane == 0) out[r] = kth;
 }
 
 __global__ void check_finite_kernel(const float *__restrict__ X, uint64_t ld, uint64_t rows,
@@ -725,6 +783,16 @@ void launch_bf_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ld
         bf_replay_lds_kernel<<<rows, 64, lds, s>>>(cand_d, cand_i, ldc, flags, D, ldd, n, k,
                                                    out_idx, out_dist, out_count, err);
     }
+    HM_CHECK_LAUNCH();
+}
+
+void launch_kth_smallest(const float *D, uint64_t ldd, uint32_t rows, uint32_t n, uint32_t m,
+                         float *out, hipStream_t s, const uint32_t *dev_rows)
+{
+    if (rows == 0) return;
+    if (m == 0 || m > KTH_MAX || m > n) fail(HM_ERR_INVALID_ARG, "kth_smallest: m out of range");
+    ProfScope prof("kth_smallest", s);
+    kth_smallest_kernel<<<ceil_div(rows, 4), 256, 0, s>>>(D, ldd, rows, n, m, out, dev_rows);
     HM_CHECK_LAUNCH();
 }
 
