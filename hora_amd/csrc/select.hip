@@ -316,18 +316,17 @@ struct LaneHeap {
 // ---- std BinaryHeap held across the lanes of one wave: lane j owns slot j.
 // Every operation runs with uniform control flow; a slot is read with
 // v_readlane (uniform index) and written by the owning lane.
-// Streams a distance row through LDS in batches of 4096 values (16 float4
-// loads per lane, the next batch in flight while the current one is
-// processed) and hands it over in chunks of up to 64 consecutive values:
-// on_chunk(first_index, this_lane's_value, values_in_lds, count).  Before a batch is handed
-// over, skip_batch(batch_min) may drop it whole (the minimum is over the
-// batch's values with index < n; NaN is ignored by fminf, and rows with a
-// NaN never reach a replay).  Rows are 16-byte aligned with a stride ldd >=
-// n that is a multiple of 4, so the vector loads (clamped to the last full
-// vector of the row) stay inside the row.
-template <class SkipBatch, class OnChunk>
+// Streams a distance row in batches of 4096 values (16 float4 loads per
+// lane, the next batch in flight while the current one is processed) and
+// hands over, in index order, the 64-value chunks that may hold a value not
+// above threshold(): on_chunk(first_index, this_lane's_value,
+// values_in_lds, count).  Chunks are filtered by register ballots per group
+// of 256 values (rows with a NaN never reach a replay).  Rows are 16-byte
+// aligned with a stride ldd >= n that is a multiple of 4, so the vector
+// loads (clamped to the last full vector of the row) stay inside the row.
+template <class Threshold, class OnChunk>
 __device__ void stream_row(const float *__restrict__ row, uint32_t n, uint64_t ldd, int lane,
-                           float *sv, SkipBatch &&skip_batch, OnChunk &&on_chunk)
+                           float *sv, Threshold &&threshold, OnChunk &&on_chunk)
 {
     constexpr uint32_t FB4 = 16, BATCH = FB4 * 256;
     const float inf = __int_as_float(0x7f800000);
@@ -344,17 +343,24 @@ __device__ void stream_row(const float *__restrict__ row, uint32_t n, uint64_t l
     }
     HM_ROW_LOAD(0u)
     for (uint32_t b0 = 0; b0 < n; b0 += BATCH) {
-        float m = inf;
+        // Which 256-value groups hold a value not above the threshold (the
+        // root of a full, stable heap; +inf otherwise)?  The root only
+        // falls while the heap stays stable, so a group with none can be
+        // passed over later as long as the heap is still stable.
+        const float thr = threshold();
+        uint32_t gmask = 0;
 #pragma clang loop unroll(full)
-        for (uint32_t c = 0; c < 4 * FB4; ++c) {
-            const uint64_t e = (uint64_t)b0 + (c / 4) * 256 + 4 * (uint32_t)lane + (c % 4);
-            if (e < n) m = fminf(m, nx[c]);
+        for (uint32_t c4 = 0; c4 < FB4; ++c4) {
+            bool any = false;
+#pragma clang loop unroll(full)
+            for (uint32_t jj = 0; jj < 4; ++jj) {
+                const uint64_t e = (uint64_t)b0 + c4 * 256 + 4 * (uint32_t)lane + jj;
+                any = any || (e < n && !(nx[4 * c4 + jj] > thr));
+            }
+            if (__ballot(any)) gmask |= 1u << c4;
         }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) m = fminf(m, __shfl_xor(m, o, 64));
-        const bool skip = skip_batch(m);
         __syncthreads();
-        if (!skip) {
+        if (gmask) {
 #pragma clang loop unroll(full)
             for (uint32_t c4 = 0; c4 < FB4; ++c4)
                 reinterpret_cast<float4 *>(sv)[c4 * 64 + lane] =
@@ -364,15 +370,12 @@ __device__ void stream_row(const float *__restrict__ row, uint32_t n, uint64_t l
         if (n - b0 > BATCH) {
             HM_ROW_LOAD(b0 + BATCH)
         }
-        if (skip) continue;
-        // the next chunk's value is read from LDS while this one is examined
-        float vcur = sv[lane];
+        if (!gmask) continue;
         for (uint32_t c = 0; c < BATCH / 64; ++c) {
             const uint32_t base = b0 + c * 64;
             if (base >= n) break;
-            const float vnext = sv[min(c + 1, BATCH / 64 - 1) * 64 + lane];
-            on_chunk(base, vcur, sv + c * 64, n - base < 64 ? n - base : 64u);
-            vcur = vnext;
+            if (!((gmask >> (c >> 2)) & 1u) && threshold() < inf) continue;
+            on_chunk(base, sv[c * 64 + lane], sv + c * 64, n - base < 64 ? n - base : 64u);
         }
     }
 #undef HM_ROW_LOAD
@@ -437,6 +440,21 @@ struct WaveHeap {
         len += 1;
         sift_up(0, len - 1);
     }
+    // heap_noop_roundtrip (below) on the lane-resident heap
+    __device__ bool noop_roundtrip() const
+    {
+        const uint32_t L = len;
+        if (L == 0) return true;
+        uint32_t a = L, p = (a - 1) / 2;
+        if (a == 2 * p + 2 && !(gd(p) > gd(L - 1))) return false;
+        a = p;
+        while (a > 0) {
+            p = (a - 1) / 2;
+            if (a == 2 * p + 1 && a + 1 < L && !(gd(p) > gd(a + 1))) return false;
+            a = p;
+        }
+        return true;
+    }
     __device__ void pop(float *od, uint32_t *oi)
     {
         const float ld = gd(len - 1);
@@ -458,9 +476,9 @@ struct WaveHeap {
 // lanes of one wave (k + 1 <= 64).  An element x greater than the heap's
 // maximum is pushed and popped again; that round trip permutes the layout
 // in a way that depends on the layout only, so when it is known to leave
-// the current layout unchanged (checked once after every change) such
-// elements are skipped.  One workgroup of 64 lanes per row; rows without a
-// tie copy their candidates.
+// the current layout unchanged (heap_noop_roundtrip, checked after every
+// change) such elements are skipped.  One workgroup of 64 lanes per row;
+// rows without a tie copy their candidates.
 __global__ __launch_bounds__(64) void bf_finalize_wave_kernel(
     const float *__restrict__ cand_d, const uint32_t *__restrict__ cand_i, uint32_t ldc,
     const uint32_t *__restrict__ flags, const float *__restrict__ D, uint64_t ldd, uint32_t n,
@@ -501,10 +519,9 @@ __global__ __launch_bounds__(64) void bf_finalize_wave_kernel(
     // next element that is not greater than the root (the only ones that
     // change anything), so the rejected majority costs one ballot per 64.
     __shared__ __align__(16) float sv[4096];
-    // a batch whose values all exceed the root of a full, stable heap
-    // changes nothing
-    auto skip_batch = [&](float bmin) { return h.len == k && stable && bmin > root; };
-    stream_row(row, n, ldd, lane, sv, skip_batch,
+    // values above the root of a full, stable heap change nothing
+    auto threshold = [&]() { return (h.len == k && stable) ? root : inf; };
+    stream_row(row, n, ldd, lane, sv, threshold,
                [&](uint32_t base, float v, const float *, uint32_t cnt) {
         uint32_t j = 0;
         while (j < cnt) {
@@ -523,14 +540,7 @@ __global__ __launch_bounds__(64) void bf_finalize_wave_kernel(
             }
             if (h.len == k) {
                 root = h.gd(0);
-                // would a larger element's push + pop leave this layout as is?
-                WaveHeap t = h;
-                t.push(inf, 0xFFFFFFFFu);
-                float a;
-                uint32_t b;
-                t.pop(&a, &b);
-                const bool same = (uint32_t)lane >= k || (t.d == h.d && t.i == h.i);
-                stable = root < inf && __all(same);
+                stable = root < inf && h.noop_roundtrip();
             }
             ++j;
         }
@@ -622,8 +632,8 @@ __global__ __launch_bounds__(64) void bf_replay_lds_kernel(
     bool stable = false;
     float root = inf;
     __shared__ __align__(16) float sv[4096];
-    auto skip_batch = [&](float bmin) { return h.len == k && stable && bmin > root; };
-    stream_row(row, n, ldd, lane, sv, skip_batch,
+    auto threshold = [&]() { return (h.len == k && stable) ? root : inf; };
+    stream_row(row, n, ldd, lane, sv, threshold,
                [&](uint32_t base, float v, const float *cv, uint32_t cnt) {
         uint32_t j = 0;
         while (j < cnt) {
