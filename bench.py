@@ -10,8 +10,8 @@ rerank, FP8 shortlist + exact rerank -- and `value` is the fastest form whose
 recall@10 on the checked sample is >= 0.95.
 
 Other workloads (--workload): ivfpq (IVFPQIndex 1M x 128), ragged (ragged
-query batches over 1M x 768) and probe (learned nprobe over 5M x 96); each
-prints its own JSON line.
+query batches over 1M x 768), probe (learned nprobe over 5M x 96) and pq
+(standalone PQIndex 100k x 128); each prints its own JSON line.
 
 The roofline of the dominant kernel is timed live with HIP events that the
 library records on the launch stream; the CPU baseline is the oracle (a C
@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="bf",
-                    choices=["bf", "ivfpq", "ragged", "probe", "copy"])
+                    choices=["bf", "ivfpq", "ragged", "probe", "copy", "pq"])
     ap.add_argument("--forms", default="exact,bf16,fp8")
     ap.add_argument("--cpu-sample", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -331,6 +331,102 @@ def main_ivfpq(args):
           flush=True)
 
 
+def main_pq(args):
+    """Standalone PQIndex (pq_idx.rs:21-265) at the reference's item limit:
+    100,000 x 128 Gaussian mixture, 16 sub-spaces x 256 codewords, 10,000
+    queries, k=10; recall@10 against the exact brute force."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from hora_amd import BruteForceIndex, Metric, PQIndex, PQParams
+    import hora_amd
+    from hora_amd.datasets import gaussian_mixture
+
+    N, NQ, D, K = 100_000, 10_000, 128, 10
+    epochs = int(os.environ.get("HM_PQ_EPOCHS", "100"))
+    # sigma 1.0 by default; at sigma 0.3 the 98 items of a cluster share
+    # most of their codes, so most query heads are tied and go through the
+    # heap replay
+    sigma = float(os.environ.get("HM_PQ_SIGMA", "1.0"))
+    x, cen, _ = gaussian_mixture(N, D, 1024, seed=501, sigma=sigma)
+    q, _, _ = gaussian_mixture(NQ, D, 1024, seed=502, sigma=sigma, centres=cen)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    ix = PQIndex(D, PQParams().n_sub(16).sub_bits(8).train_epoch(epochs).seed(7))
+    ix.add_batch(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ix.build(Metric.Euclidean)
+    build_s = time.perf_counter() - t0
+    log(f"PQIndex build (16x256, {epochs} epochs): {build_s:.2f}s")
+    bf = BruteForceIndex(D)
+    bf.add_batch(x)
+    bf.build(Metric.Euclidean)
+    bf.set_search_mode(hora_amd.HM_SEARCH_BF16_RERANK, 0)
+    truth, _, _ = bf.search_batch(q, K)
+    del bf
+    dq = torch.from_numpy(q).to(dev)
+    d_idx = torch.empty((NQ, K), dtype=torch.int64, device=dev)
+    d_dist = torch.empty((NQ, K), dtype=torch.float32, device=dev)
+    d_cnt = torch.empty((NQ,), dtype=torch.int32, device=dev)
+    timer = Timer(torch, stream)
+    wall, ev = timer.run(lambda: ix.search_device(dq.data_ptr(), NQ, K, d_idx.data_ptr(),
+                                                  d_dist.data_ptr(), d_cnt.data_ptr(),
+                                                  stream.cuda_stream),
+                         args.steps, args.warmup, "pq")
+    km = kernel_ms(["pq_scan", "row_select", "pq_finalize", "pq_replay"])
+    replayed = int(ix.search_stats()[0])
+    got = d_idx.cpu().numpy().view(np.uint64)
+    r10 = float(np.mean([len(set(got[i].tolist()) & set(truth[i].tolist())) / K
+                         for i in range(NQ)]))
+    # parity on a sample: the oracle's search over the exported codebook/codes
+    ex = ix.export()
+    samp = 64
+    ri, rd, _ = oracle.pqindex_search(q[:samp], oracle.EUCLIDEAN, ex["codebook"], ex["codes"], K)
+    gd = d_dist.cpu().numpy()
+    parity = bool(np.array_equal(ri, got[:samp]) and np.array_equal(rd, gd[:samp]))
+    scan_ms, scan_n = km["pq_scan"]
+    roofline = None
+    if scan_n:
+        # bytes per scan launch: the codes once (N x 16 B) and the distance
+        # block written (rows x N x 4 B); the LUT lookups run in LDS
+        rows = NQ * args.steps / scan_n
+        bytes_ = N * 16 + rows * N * 4
+        t = scan_ms / scan_n / 1e3
+        ach = bytes_ / t / 1e9
+        roofline = {"bound": "hbm", "kernel": "pq_scan_kernel", "achieved": round(ach, 1),
+                    "peak": HBM_PEAK, "unit": "GB/s", "frac": round(ach / HBM_PEAK, 4),
+                    "traffic": None, "avg_launch_ms": round(scan_ms / scan_n, 3),
+                    "lds_lookups_per_s": round(rows * N * 16 / t, 1)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        ns = min(args.cpu_sample, NQ)
+        t0 = time.perf_counter()
+        oracle.pqindex_search(q[:ns], oracle.EUCLIDEAN, ex["codebook"], ex["codes"], K,
+                              nthreads=16)
+        t_all = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        oracle.pqindex_search(q[:10], oracle.EUCLIDEAN, ex["codebook"], ex["codes"], K,
+                              nthreads=1)
+        t_one = time.perf_counter() - t0
+        cpu = {"value": round(ns / t_all, 1), "unit": "QPS", "cores": 16, "kind": "port",
+               "sample": f"first {ns} queries on 16 threads (single thread: "
+                         f"{round(10 / t_one, 1)} QPS on 10 queries)"}
+    print(json.dumps({"metric": "QPS, 1 MI355X, PQIndex ADC", "value": round(args.steps * NQ / wall, 1),
+                      "unit": "QPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": round(ev * 1e3 / args.steps, 3), "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "f32 (u8 PQ codes)",
+                      "data": f"synthetic Gaussian mixture, 1,024 centres, sigma {sigma}",
+                      "config": {"workload": "PQIndex Euclidean 100,000x128, 16x256 PQ, "
+                                             "10,000 queries, k=10", "train_epoch": epochs},
+                      "recall_at_10": round(r10, 4), "parity_sample_bit_exact": parity,
+                      "build_s": round(build_s, 3), "tie_replayed_queries": replayed,
+                      "kernel_ms_per_step": {k: round(v[0] / args.steps, 3)
+                                             for k, v in km.items() if v[1]},
+                      "roofline": roofline, "cpu_baseline": cpu}), flush=True)
+
+
 def main_ragged(args):
     import numpy as np
     import torch
@@ -482,7 +578,7 @@ def main():
     import torch  # noqa: F401  (load torch's HIP runtime before the engine's library)
 
     {"bf": main_bf, "ivfpq": main_ivfpq, "ragged": main_ragged, "probe": main_probe,
-     "copy": main_copy}[args.workload](args)
+     "copy": main_copy, "pq": main_pq}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -15,5 +15,7 @@ from .index import (  # noqa: F401
     IVFPQParams,
     Metric,
     Node,
+    PQIndex,
     PQIVFIndex,
+    PQParams,
 )

@@ -89,6 +89,8 @@ def lib():
     L.hm_index_set_nprobe.argtypes = [vp, u32]
     L.hm_ivfpq_info.argtypes = [vp, vp]
     L.hm_ivfpq_export.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.hm_pq_info.argtypes = [vp, vp]
+    L.hm_pq_export.argtypes = [vp, vp, vp]
     L.hm_ivfpq_train_probe.argtypes = [vp, vp, u64, u32, u32, u32, u32, C.c_float, u64,
                                        C.POINTER(C.c_double)]
     L.hm_ivfpq_set_probe_policy.argtypes = [vp, i32, C.c_float, u32, u32]

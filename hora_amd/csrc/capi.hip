@@ -69,6 +69,13 @@ HM_API int32_t hm_params_default(int32_t kind, hm_params *out)
             out->train_epoch = 100;
             out->kmeans_accumulate = 0;
             out->seed = 0x5EEDull;
+        } else if (kind == HM_INDEX_PQ) {
+            // PQParams::default (pq_params.rs:31-40)
+            out->n_sub = 4;
+            out->sub_bits = 4;
+            out->train_epoch = 100;
+            out->kmeans_accumulate = 0;
+            out->seed = 0x5EEDull;
         } else if (kind != HM_INDEX_BRUTE_FORCE) {
             fail(HM_ERR_INVALID_ARG, "unknown index kind");
         }
@@ -95,6 +102,14 @@ HM_API int32_t hm_index_create(int32_t kind, uint64_t dimension, const hm_params
                 hm_params_default(HM_INDEX_IVFPQ, &p);
             }
             h->impl = make_ivfpq(dimension, p);
+        } else if (kind == HM_INDEX_PQ) {
+            hm_params p;
+            if (params) {
+                p = *params;
+            } else {
+                hm_params_default(HM_INDEX_PQ, &p);
+            }
+            h->impl = make_pq(dimension, p);
         } else {
             fail(HM_ERR_INVALID_ARG, "unknown index kind");
         }
@@ -333,6 +348,26 @@ HM_API int32_t hm_ivfpq_export(hm_index *h, float *centers, uint64_t *list_offse
         std::lock_guard<std::mutex> lk(h->mu);
         DeviceScope ds(h->device);
         ivfpq_export(h->impl.get(), centers, list_offsets, list_ids, codebooks, codes);
+    });
+}
+
+HM_API int32_t hm_pq_info(hm_index *h, uint32_t *out8)
+{
+    return guard([&] {
+        need(h, "index");
+        need(out8, "out8");
+        std::lock_guard<std::mutex> lk(h->mu);
+        pq_info(h->impl.get(), out8);
+    });
+}
+
+HM_API int32_t hm_pq_export(hm_index *h, float *codebook, uint8_t *codes)
+{
+    return guard([&] {
+        need(h, "index");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        pq_export(h->impl.get(), codebook, codes);
     });
 }
 

@@ -77,6 +77,11 @@ class Index {
 
 std::unique_ptr<Index> make_bruteforce(uint64_t dim);
 std::unique_ptr<Index> make_ivfpq(uint64_t dim, const hm_params &p);
+std::unique_ptr<Index> make_pq(uint64_t dim, const hm_params &p);
+
+// Standalone PQIndex extras (fail with INVALID_ARG on another index kind).
+int32_t pq_info(Index *ix, uint32_t *out8);
+int32_t pq_export(Index *ix, float *codebook, uint8_t *codes);
 
 // IVFPQIndex extras (fail with INVALID_ARG on another index kind).
 int32_t ivfpq_info(Index *ix, uint32_t *out8);

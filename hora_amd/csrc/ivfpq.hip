@@ -9,6 +9,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "ivfpq.h"
+#include "pq_common.h"
 #include "profiler.h"
 
 #pragma clang fp contract(off)
@@ -28,33 +29,6 @@ __device__ __forceinline__ float unkey(uint32_t k)
 {
     uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
     return __uint_as_float(b);
-}
-
-// SplitMix64, the seeded stand-in for the reference's thread_rng
-// (kmeans.rs:168-171); the oracle uses the same generator.
-__device__ __host__ __forceinline__ uint64_t splitmix64(uint64_t *s)
-{
-    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-__device__ __host__ __forceinline__ double splitmix_unit(uint64_t *s)
-{
-    return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0);
-}
-
-__device__ __host__ __forceinline__ uint64_t stream_seed(uint64_t seed, uint64_t instance)
-{
-    uint64_t s = seed ^ (instance * 0xD1B54A32D192ED03ull);
-    return splitmix64(&s);
-}
-
-// kmeans.rs:76: (i & (1 << j)) == 1, shift amount masked to 6 bits.
-__device__ __host__ __forceinline__ float init_offset(uint64_t i, uint32_t j)
-{
-    return ((i & (1ull << (j & 63))) == 1ull) ? 1.0f : -1.0f;
 }
 
 // kmeans.rs:50-86: mean over the rows (left fold in row order, f32 or f64),
@@ -342,7 +316,7 @@ __global__ __launch_bounds__(256) void ivf_scan_kernel(
     __syncthreads();
     const float *cb = codebooks + (size_t)l * M * KS * sdm;
     for (uint32_t e = tid; e < M * KS; e += 256) {
-        uint32_t s = e / KS, c = e % KS;
+        uint32_t s = e / KS;
         uint32_t b = sub_b[s], sd = sub_e[s] - b;
         const float *w = cb + (size_t)e * sdm;
         float acc = -0.0f;

@@ -1,0 +1,1043 @@
+// pq_index.hip -- standalone PQIndex on the device (pq_idx.rs:21-265).
+//
+// HBM layout after build:
+//   base       [cap][dpad] f32, rows in insertion order (item i = row i)
+//   codebook   [M][KS][sdm] f32 (a sub-space with fewer dims is zero padded)
+//   codes      [n][M] u8, item i's code in sub-space s at codes[i*M + s]
+// Build (PQIndex::train_center, pq_idx.rs:122-149): per sub-space the
+// reference's k-means with the build metric on the raw sub-vectors (no
+// residual): column means, then per epoch an assignment kernel (grid over
+// items x sub-spaces, the sub-space's codebook in LDS), a point-order update
+// kernel (one workgroup per sub-space, item chunks staged in LDS, one lane
+// per centre folding its members in item order) and the split step; a last
+// assignment writes the codes.
+// Search (search_knn_adc, pq_idx.rs:165-194): a workgroup builds the LUTs of
+// a group of queries in LDS with the build metric and scans the items' codes
+// (ADC: f32 sum over the sub-spaces in order), writing a distance block;
+// the radix select keeps the k+1 smallest by (distance, item); rows whose
+// k+1 smallest are pairwise distinct output them directly, the others
+// replay the reference's push-all / pop-to-k heap over the row.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "index.h"
+#include "ivfpq.h"
+#include "pq_common.h"
+#include "profiler.h"
+
+#pragma clang fp contract(off)
+
+namespace hm {
+
+namespace {
+
+enum { PM_EUCLID = 3, PM_MANHATTAN = 1, PM_DOT = 2, PM_COSINE = 4, PM_ANGULAR = 5 };
+
+// metrics.rs:73-83 as this build defines Cosine: -s_zy / (sqrt(s_zz)*sqrt(s_yy)).
+__device__ __forceinline__ float pm_cosine(float szy, float szz, float syy)
+{
+    float den = sqrtf(szz) * sqrtf(syy);
+    return (-szy) / den;
+}
+
+// metrics.rs:88-103 with float_two() == 1.0 (node.rs:54-56).
+__device__ __forceinline__ float pm_angular(float szy, float szz, float syy)
+{
+    float rhd = -szz, lhd = -syy, rldot = -szy;
+    float rlmul = rhd * lhd;
+    if (rlmul > 0.0f) {
+        float t = 1.0f * rldot;
+        t = t / sqrtf(rlmul);
+        return 1.0f - t;
+    }
+    return 1.0f;
+}
+
+// One fold step of the metric's accumulated sum (simd_metrics.rs scalar
+// branches): Euclid (z-y)^2, Manhattan |z-y|, the others z*y.
+template <int MET>
+__device__ __forceinline__ float pm_step(float acc, float z, float y)
+{
+    if constexpr (MET == PM_EUCLID) {
+        float d = z - y;
+        float sq = d * d;
+        return acc + sq;
+    } else if constexpr (MET == PM_MANHATTAN) {
+        float d = z - y;
+        return acc + fabsf(d);
+    } else {
+        float p = z * y;
+        return acc + p;
+    }
+}
+
+template <int MET>
+__device__ __forceinline__ float pm_finish(float acc, float szz, float syy)
+{
+    if constexpr (MET == PM_COSINE) return pm_cosine(acc, szz, syy);
+    else if constexpr (MET == PM_ANGULAR) return pm_angular(acc, szz, syy);
+    else return acc;
+}
+
+constexpr bool pm_norms(int met) { return met == PM_COSINE || met == PM_ANGULAR; }
+
+// kmeans.rs:50-86: mean of column t over the items (left fold in item
+// order from +0.0, f32 or f64), then centre c of the sub-space holding t
+// gets mean + 1 where init_offset(c, j) says so, else mean - 1 (j is the
+// dimension's index inside its sub-space).  One lane per used dimension.
+template <typename ACC_T>
+__global__ void pq_init_kernel(const float *__restrict__ X, uint32_t ldx, uint64_t n,
+                               const uint32_t *__restrict__ sub_b, const uint32_t *__restrict__ sub_e,
+                               uint32_t M, uint32_t KS, uint32_t sdm, float *__restrict__ cb)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= sub_e[M - 1]) return;
+    uint32_t s = 0;
+    while (t >= sub_e[s]) ++s;
+    const uint32_t j = t - sub_b[s];
+    ACC_T acc = 0;
+    for (uint64_t i = 0; i < n; ++i) acc = acc + (ACC_T)X[i * ldx + t];
+    float mean;
+    if constexpr (sizeof(ACC_T) == 4) mean = acc / (float)n;
+    else mean = (float)(acc / (double)n);
+    for (uint32_t c = 0; c < KS; ++c)
+        cb[((size_t)s * KS + c) * sdm + j] = init_offset(c, j) > 0 ? mean + 1.0f : mean - 1.0f;
+}
+
+// kmeans.rs:127-148 (search_data) under the build metric: item i of
+// sub-space s goes to the lowest-index centre with the smallest distance,
+// scanning with a strict '<' from centre 0 (a NaN distance never wins, and
+// a NaN at the current best keeps it).  Writes out[i*si + s*ss].
+constexpr int PA_THREADS = 256;
+constexpr int PA_REG = 32;  // sub-vector dims held in registers
+
+template <int MET, bool STAGED>
+__global__ __launch_bounds__(PA_THREADS) void pq_assign_kernel(
+    const float *__restrict__ X, uint32_t ldx, uint32_t n, const uint32_t *__restrict__ sub_b,
+    const uint32_t *__restrict__ sub_e, uint32_t KS, uint32_t sdm, const float *__restrict__ cb,
+    uint8_t *__restrict__ out, uint64_t si, uint64_t ss)
+{
+    extern __shared__ __align__(16) float pa_smem[];
+    const uint32_t s = blockIdx.y;
+    const uint32_t b = sub_b[s], sd = sub_e[s] - b;
+    const float *cbs = cb + (size_t)s * KS * sdm;
+    float *cn = pa_smem;           // [KS] codeword self-dots (Cosine / Angular)
+    float *cl = pa_smem + KS;      // [KS][sd] staged codebook (STAGED)
+    const float *C;
+    uint32_t ldc;
+    if constexpr (STAGED) {
+        for (uint32_t e = threadIdx.x; e < KS * sd; e += PA_THREADS)
+            cl[e] = cbs[(size_t)(e / sd) * sdm + e % sd];
+        C = cl;
+        ldc = sd;
+    } else {
+        C = cbs;
+        ldc = sdm;
+    }
+    __syncthreads();
+    if constexpr (pm_norms(MET)) {
+        for (uint32_t c = threadIdx.x; c < KS; c += PA_THREADS) {
+            float a = -0.0f;
+            for (uint32_t j = 0; j < sd; ++j) a = pm_step<PM_DOT>(a, C[c * ldc + j], C[c * ldc + j]);
+            cn[c] = a;
+        }
+        __syncthreads();
+    }
+    const uint32_t i = blockIdx.x * PA_THREADS + threadIdx.x;
+    if (i >= n) return;
+    const float *x = X + (size_t)i * ldx + b;
+    uint32_t best = 0;
+    float bd = 0.0f;
+    if (sd <= PA_REG) {
+        float z[PA_REG];
+#pragma unroll
+        for (int j = 0; j < PA_REG; ++j) {
+            // clamped address, masked value: every load stays inside the row
+            const float v = x[min((uint32_t)j, sd - 1)];
+            z[j] = (uint32_t)j < sd ? v : 0.0f;
+        }
+        float zz = -0.0f;
+        if constexpr (pm_norms(MET)) {
+#pragma unroll
+            for (int j = 0; j < PA_REG; ++j)
+                if ((uint32_t)j < sd) zz = pm_step<PM_DOT>(zz, z[j], z[j]);
+        }
+        for (uint32_t c = 0; c < KS; ++c) {
+            const float *y = C + (size_t)c * ldc;
+            float acc = -0.0f;
+#pragma unroll
+            for (int j = 0; j < PA_REG; ++j)
+                if ((uint32_t)j < sd) acc = pm_step<MET>(acc, z[j], y[j]);
+            float dc = pm_finish<MET>(acc, zz, pm_norms(MET) ? cn[c] : 0.0f);
+            if (c == 0) bd = dc;
+            else if (dc < bd) {
+                bd = dc;
+                best = c;
+            }
+        }
+    } else {
+        float zz = -0.0f;
+        if constexpr (pm_norms(MET))
+            for (uint32_t j = 0; j < sd; ++j) zz = pm_step<PM_DOT>(zz, x[j], x[j]);
+        for (uint32_t c = 0; c < KS; ++c) {
+            const float *y = C + (size_t)c * ldc;
+            float acc = -0.0f;
+            for (uint32_t j = 0; j < sd; ++j) acc = pm_step<MET>(acc, x[j], y[j]);
+            float dc = pm_finish<MET>(acc, zz, pm_norms(MET) ? cn[c] : 0.0f);
+            if (c == 0) bd = dc;
+            else if (dc < bd) {
+                bd = dc;
+                best = c;
+            }
+        }
+    }
+    out[(size_t)i * si + (size_t)s * ss] = (uint8_t)best;
+}
+
+// kmeans.rs:88-125 (update_center): centre c = the f32 (or f64) sum of its
+// members' sub-vectors in item order from +0.0, divided by their count; an
+// empty centre becomes zero.  One workgroup per sub-space; lane c owns
+// centre c; items are staged through LDS in chunks of PU_CHUNK, and the
+// dimensions in slabs of PU_W (each slab is an independent fold).
+constexpr int PU_THREADS = 256;
+constexpr int PU_CHUNK = 512;
+constexpr int PU_W = 32;
+
+template <typename ACC_T>
+__global__ __launch_bounds__(PU_THREADS) void pq_update_kernel(
+    const float *__restrict__ X, uint32_t ldx, uint32_t n, const uint32_t *__restrict__ sub_b,
+    const uint32_t *__restrict__ sub_e, uint32_t KS, uint32_t sdm,
+    const uint8_t *__restrict__ asg, float *__restrict__ cb, uint32_t *__restrict__ counts)
+{
+    __shared__ __align__(16) float zs[PU_CHUNK * PU_W];
+    __shared__ uint8_t as[PU_CHUNK];
+    const uint32_t s = blockIdx.x;
+    const uint32_t b = sub_b[s], sd = sub_e[s] - b;
+    const uint8_t *a = asg + (size_t)s * n;
+    const uint32_t c = threadIdx.x;
+    for (uint32_t t0 = 0; t0 < sd; t0 += PU_W) {
+        const uint32_t w = min((uint32_t)PU_W, sd - t0);
+        ACC_T acc[PU_W];
+#pragma unroll
+        for (int j = 0; j < PU_W; ++j) acc[j] = 0;
+        uint32_t n_c = 0;
+        for (uint32_t p0 = 0; p0 < n; p0 += PU_CHUNK) {
+            const uint32_t cnt = min((uint32_t)PU_CHUNK, n - p0);
+            for (uint32_t e = threadIdx.x; e < cnt * w; e += PU_THREADS) {
+                const uint32_t i = e / w, j = e % w;
+                zs[i * PU_W + j] = X[(size_t)(p0 + i) * ldx + b + t0 + j];
+            }
+            for (uint32_t e = threadIdx.x; e < cnt; e += PU_THREADS) as[e] = a[p0 + e];
+            __syncthreads();
+            if (c < KS) {
+                for (uint32_t i = 0; i < cnt; ++i) {
+                    if (as[i] == c) {
+                        ++n_c;
+#pragma unroll
+                        for (int j = 0; j < PU_W; ++j)
+                            if ((uint32_t)j < w) acc[j] = acc[j] + (ACC_T)zs[i * PU_W + j];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (c < KS) {
+            float *dst = cb + ((size_t)s * KS + c) * sdm + t0;
+#pragma unroll
+            for (int j = 0; j < PU_W; ++j) {
+                if ((uint32_t)j >= w) continue;
+                float v;
+                if (n_c == 0) v = 0.0f;
+                else if constexpr (sizeof(ACC_T) == 4) v = acc[j] / (float)n_c;
+                else v = (float)(acc[j] / (double)n_c);
+                dst[j] = v;
+            }
+            if (t0 == 0) counts[(size_t)s * KS + c] = n_c;
+        }
+    }
+}
+
+// kmeans.rs:150-193 (split_center) with the seeded generator: one lane per
+// sub-space walks the empty centres in order.
+__global__ void pq_split_kernel(float *__restrict__ cb, uint32_t M, uint32_t KS, uint32_t sdm,
+                                const uint32_t *__restrict__ sub_b,
+                                const uint32_t *__restrict__ sub_e, uint32_t *__restrict__ counts,
+                                uint64_t n, uint64_t *__restrict__ rng)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    const uint32_t sd = sub_e[s] - sub_b[s];
+    float *cent = cb + (size_t)s * KS * sdm;
+    uint32_t *cnt = counts + (size_t)s * KS;
+    uint64_t st = rng[s];
+    const float EPS = 1.0f / 1024.0f;
+    for (uint32_t i = 0; i < KS; ++i) {
+        if (cnt[i] != 0) continue;
+        uint32_t sc = (i + 1) % KS;
+        for (;;) {
+            double pick = (double)cnt[sc] / (double)n;
+            if (splitmix_unit(&st) < pick) break;
+            sc = (sc + 1) % KS;
+        }
+        for (uint32_t j = 0; j < sd; ++j) {
+            float src = cent[(size_t)sc * sdm + j];
+            if (j % 2 == 0) {
+                cent[(size_t)i * sdm + j] = src * (1.0f - EPS);
+                cent[(size_t)sc * sdm + j] = src * (1.0f + EPS);
+            } else {
+                cent[(size_t)i * sdm + j] = src * (1.0f + EPS);
+                cent[(size_t)sc * sdm + j] = src * (1.0f - EPS);
+            }
+        }
+        cnt[i] = cnt[sc] / 2;
+        cnt[sc] -= cnt[i];
+    }
+    rng[s] = st;
+}
+
+// out[s*KS + c] = sum_j cb[s][c][j]^2 over the sub-space's own dims
+// (left fold from -0.0).
+__global__ void pq_norms_kernel(const float *__restrict__ cb, uint32_t M, uint32_t KS,
+                                uint32_t sdm, const uint32_t *__restrict__ sub_b,
+                                const uint32_t *__restrict__ sub_e, float *__restrict__ out)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= M * KS) return;
+    const uint32_t s = e / KS, sd = sub_e[s] - sub_b[s];
+    const float *y = cb + (size_t)e * sdm;
+    float a = -0.0f;
+    for (uint32_t j = 0; j < sd; ++j) a = pm_step<PM_DOT>(a, y[j], y[j]);
+    out[e] = a;
+}
+
+// LUT entry (s, c) of query z (pq_idx.rs:170-179 with get_distance_from_vec_range
+// 151-163: the build metric on the raw sub-vector, no residual).
+template <int MET>
+__device__ __forceinline__ float pq_lut_entry(const float *__restrict__ z,
+                                              const float *__restrict__ y, uint32_t sd,
+                                              float syy)
+{
+    float acc = -0.0f, zz = -0.0f;
+    for (uint32_t j = 0; j < sd; ++j) {
+        acc = pm_step<MET>(acc, z[j], y[j]);
+        if constexpr (pm_norms(MET)) zz = pm_step<PM_DOT>(zz, z[j], z[j]);
+    }
+    return pm_finish<MET>(acc, zz, syy);
+}
+
+// Search: the LUTs of 4 queries in LDS as float4 lut4[s*KS + c] (one lane
+// of the float4 per query), then ADC over this workgroup's item range
+// (pq_idx.rs:181-188: dist = 0; dist += LUT[s][code_s] in sub-space order),
+// one item per lane: the item's codes are read once for the 4 queries and
+// each lookup is one 16-byte LDS read.  Rows of the distance block are
+// written by consecutive lanes.
+constexpr int PS_THREADS = 256;
+constexpr int PS_Q = 4;
+
+template <int MET>
+__global__ __launch_bounds__(PS_THREADS) void pq_scan_kernel(
+    const float *__restrict__ Qp, uint32_t dpad, uint32_t rows,
+    const uint32_t *__restrict__ sub_b, const uint32_t *__restrict__ sub_e, uint32_t M,
+    uint32_t KS, uint32_t sdm, const float *__restrict__ cb, const float *__restrict__ cbn,
+    const uint8_t *__restrict__ codes, uint32_t n, uint32_t per_split, float *__restrict__ D,
+    uint64_t ldd)
+{
+    extern __shared__ __align__(16) float4 lut4[];
+    float *lut = reinterpret_cast<float *>(lut4);
+    const uint32_t q0 = blockIdx.y * PS_Q;
+    const uint32_t i_begin = blockIdx.x * per_split;
+    const uint32_t i_end = min(n, i_begin + per_split);
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t e = tid; e < M * KS * PS_Q; e += PS_THREADS) {
+        const uint32_t qq = e % PS_Q, sc = e / PS_Q, s = sc / KS, b = sub_b[s], sd = sub_e[s] - b;
+        const uint32_t q = q0 + qq;
+        float v = 0.0f;
+        if (q < rows)
+            v = pq_lut_entry<MET>(Qp + (size_t)q * dpad + b, cb + (size_t)sc * sdm, sd,
+                                  pm_norms(MET) ? cbn[sc] : 0.0f);
+        lut[e] = v;
+    }
+    __syncthreads();
+    const uint32_t nq = min((uint32_t)PS_Q, rows - q0);
+    float *d0 = D + (size_t)q0 * ldd;
+    auto store = [&](uint32_t i, float a0, float a1, float a2, float a3) {
+        d0[i] = a0;
+        if (nq > 1) d0[ldd + i] = a1;
+        if (nq > 2) d0[2 * ldd + i] = a2;
+        if (nq > 3) d0[3 * ldd + i] = a3;
+    };
+    if ((M & 15u) == 0) {
+        // 16 codes per 16-byte load, two items in flight per lane
+        const uint32_t W = M / 16;
+        for (uint32_t i = i_begin + tid; i < i_end; i += 2 * PS_THREADS) {
+            const uint32_t i2 = i + PS_THREADS;
+            const bool two = i2 < i_end;
+            const uint4 *c1 = reinterpret_cast<const uint4 *>(codes + (size_t)i * M);
+            const uint4 *c2 = reinterpret_cast<const uint4 *>(codes + (size_t)(two ? i2 : i) * M);
+            float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+            float b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
+            for (uint32_t w = 0; w < W; ++w) {
+                const uint4 v1 = c1[w], v2 = c2[w];
+                const uint32_t x1[4] = {v1.x, v1.y, v1.z, v1.w};
+                const uint32_t x2[4] = {v2.x, v2.y, v2.z, v2.w};
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const uint32_t s = 16 * w + u;
+                    const float4 l1 = lut4[s * KS + ((x1[u >> 2] >> (8 * (u & 3))) & 0xFFu)];
+                    const float4 l2 = lut4[s * KS + ((x2[u >> 2] >> (8 * (u & 3))) & 0xFFu)];
+                    a0 = a0 + l1.x;
+                    a1 = a1 + l1.y;
+                    a2 = a2 + l1.z;
+                    a3 = a3 + l1.w;
+                    b0 = b0 + l2.x;
+                    b1 = b1 + l2.y;
+                    b2 = b2 + l2.z;
+                    b3 = b3 + l2.w;
+                }
+            }
+            store(i, a0, a1, a2, a3);
+            if (two) store(i2, b0, b1, b2, b3);
+        }
+    } else {
+        for (uint32_t i = i_begin + tid; i < i_end; i += PS_THREADS) {
+            const uint8_t *cd = codes + (size_t)i * M;
+            float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+            for (uint32_t s = 0; s < M; ++s) {
+                const float4 l = lut4[s * KS + cd[s]];
+                a0 = a0 + l.x;
+                a1 = a1 + l.y;
+                a2 = a2 + l.z;
+                a3 = a3 + l.w;
+            }
+            store(i, a0, a1, a2, a3);
+        }
+    }
+}
+
+// The LUTs of codebooks too large for LDS, in global memory:
+// lut_g[(q*M + s)*KS + c], one query per workgroup.
+template <int MET>
+__global__ __launch_bounds__(PS_THREADS) void pq_lut_global_kernel(
+    const float *__restrict__ Qp, uint32_t dpad, const uint32_t *__restrict__ sub_b,
+    const uint32_t *__restrict__ sub_e, uint32_t M, uint32_t KS, uint32_t sdm,
+    const float *__restrict__ cb, const float *__restrict__ cbn, float *__restrict__ lut_g)
+{
+    const uint32_t q = blockIdx.x;
+    float *lq = lut_g + (size_t)q * M * KS;
+    for (uint32_t e = threadIdx.x; e < M * KS; e += PS_THREADS) {
+        const uint32_t s = e / KS, b = sub_b[s], sd = sub_e[s] - b;
+        lq[e] = pq_lut_entry<MET>(Qp + (size_t)q * dpad + b, cb + (size_t)e * sdm, sd,
+                                  pm_norms(MET) ? cbn[e] : 0.0f);
+    }
+}
+
+// ADC over global LUTs (one query per workgroup row).
+__global__ __launch_bounds__(PS_THREADS) void pq_scan_glut_kernel(
+    uint32_t rows, uint32_t M, uint32_t KS, const uint8_t *__restrict__ codes, uint32_t n,
+    uint32_t per_split, const float *__restrict__ lut_g, float *__restrict__ D, uint64_t ldd)
+{
+    const uint32_t q = blockIdx.y;
+    if (q >= rows) return;
+    const uint32_t i_begin = blockIdx.x * per_split;
+    const uint32_t i_end = min(n, i_begin + per_split);
+    const float *lq = lut_g + (size_t)q * M * KS;
+    for (uint32_t i = i_begin + threadIdx.x; i < i_end; i += PS_THREADS) {
+        const uint8_t *cd = codes + (size_t)i * M;
+        float dist = 0.0f;
+        for (uint32_t s = 0; s < M; ++s) dist = dist + lq[s * KS + cd[s]];
+        D[(size_t)q * ldd + i] = dist;
+    }
+}
+
+// Rows whose k+1 smallest are pairwise distinct: the survivors of the
+// reference's pop-to-k are the k smallest, drained in ascending order
+// (pq_idx.rs:189-191, 212-233).  Copies them; tied rows are overwritten by
+// the replay.  A NaN among n >= 2 items sets err bit 0 (neighbor.rs:32).
+__global__ void pq_finalize_kernel(const float *__restrict__ cand_d,
+                                   const uint32_t *__restrict__ cand_i, uint32_t ldc,
+                                   const uint32_t *__restrict__ flags, uint32_t rows, uint32_t n,
+                                   uint32_t k, uint64_t *__restrict__ out_idx,
+                                   float *__restrict__ out_dist, uint32_t *__restrict__ out_count,
+                                   uint32_t *__restrict__ err)
+{
+    const uint32_t r = blockIdx.x;
+    if (r >= rows) return;
+    const uint32_t m = min(k, n);
+    const float inf = __int_as_float(0x7f800000);
+    for (uint32_t j = threadIdx.x; j < k; j += blockDim.x) {
+        if (j < m) {
+            out_idx[(size_t)r * k + j] = cand_i[(size_t)r * ldc + j];
+            out_dist[(size_t)r * k + j] = cand_d[(size_t)r * ldc + j];
+        } else {
+            out_idx[(size_t)r * k + j] = ~0ull;
+            out_dist[(size_t)r * k + j] = inf;
+        }
+    }
+    if (threadIdx.x == 0) {
+        out_count[r] = m;
+        if ((flags[r] & ROW_NAN) && n >= 2) atomicOr(err, 1u);
+    }
+}
+
+// std BinaryHeap (max-heap on distance) whose first H positions live in
+// LDS and the rest in a global buffer; every lane of the wave runs the same
+// operations on the same words.
+template <bool HYB> struct HybridHeap {
+    float *ld;
+    uint32_t *li;
+    float *gd;
+    uint32_t *gi;
+    uint32_t H, len;
+    __device__ float d(uint32_t p) const
+    {
+        if constexpr (HYB) return p < H ? ld[p] : gd[p - H];
+        else return ld[p];
+    }
+    __device__ uint32_t ix(uint32_t p) const
+    {
+        if constexpr (HYB) return p < H ? li[p] : gi[p - H];
+        else return li[p];
+    }
+    __device__ void put(uint32_t p, float x, uint32_t i)
+    {
+        if (!HYB || p < H) {
+            ld[p] = x;
+            li[p] = i;
+        } else {
+            gd[p - H] = x;
+            gi[p - H] = i;
+        }
+    }
+    // heap.rs:21-35: move up while the element is greater than its parent
+    __device__ void sift_up(uint32_t start, uint32_t pos, float ed, uint32_t ei)
+    {
+        while (pos > start) {
+            const uint32_t parent = (pos - 1) / 2;
+            const float pd = d(parent);
+            if (ed <= pd) break;
+            put(pos, pd, ix(parent));
+            pos = parent;
+        }
+        put(pos, ed, ei);
+    }
+    // heap.rs:54-68: the hole goes to the bottom along the larger child
+    // (the right one when left <= right), then the element sifts up
+    __device__ void sift_down_to_bottom(float ed, uint32_t ei)
+    {
+        const uint32_t end = len;
+        uint32_t pos = 0, child = 1;
+        while (end >= 2 && child < end - 1) {
+            const float a = d(child), bb = d(child + 1);
+            child += (a <= bb) ? 1u : 0u;
+            put(pos, child == pos * 2 + 1 ? a : bb, ix(child));
+            pos = child;
+            child = 2 * pos + 1;
+        }
+        if (end >= 1 && child == end - 1) {
+            put(pos, d(child), ix(child));
+            pos = child;
+        }
+        sift_up(0, pos, ed, ei);
+    }
+    __device__ void push(float x, uint32_t i)
+    {
+        len += 1;
+        sift_up(0, len - 1, x, i);
+    }
+    __device__ void pop(float *od, uint32_t *oi)
+    {
+        const float lx = d(len - 1);
+        const uint32_t lid = ix(len - 1);
+        len -= 1;
+        if (len > 0) {
+            *od = d(0);
+            *oi = ix(0);
+            sift_down_to_bottom(lx, lid);
+        } else {
+            *od = lx;
+            *oi = lid;
+        }
+    }
+};
+
+// Replay of pq_idx.rs:181-191 + 212-233 for tied rows: push every item of
+// the distance row in item order, pop until k remain, then drain (the pops
+// come out descending and are written from the back).  One wave per row;
+// rows are taken in a grid-stride loop so each workgroup reuses its slice
+// of the global heap buffer.
+constexpr int PR_THREADS = 64;
+
+template <bool HYB>
+__global__ __launch_bounds__(PR_THREADS) void pq_replay_kernel(
+    const uint32_t *__restrict__ rows, uint32_t nrows, const float *__restrict__ D, uint64_t ldd,
+    uint32_t n, uint32_t k, uint32_t H, float *__restrict__ gbuf_d, uint32_t *__restrict__ gbuf_i,
+    uint64_t *__restrict__ out_idx, float *__restrict__ out_dist)
+{
+    extern __shared__ __align__(16) unsigned char pr_smem[];
+    const int lane = threadIdx.x;
+    const size_t gslice = n > H ? (size_t)(n - H) : 0;
+    for (uint32_t t = blockIdx.x; t < nrows; t += gridDim.x) {
+        const uint32_t r = rows[t];
+        const float *drow = D + (size_t)r * ldd;
+        HybridHeap<HYB> h{reinterpret_cast<float *>(pr_smem),
+                     reinterpret_cast<uint32_t *>(pr_smem) + H,
+                     gbuf_d + blockIdx.x * gslice, gbuf_i + blockIdx.x * gslice, H, 0};
+        for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+            const float v = i0 + lane < n ? drow[i0 + lane] : 0.0f;
+            const uint32_t cnt = min(64u, n - i0);
+            for (uint32_t u = 0; u < cnt; ++u)
+                h.push(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), u)), i0 + u);
+        }
+        while (h.len > k) {
+            float a;
+            uint32_t b;
+            h.pop(&a, &b);
+        }
+        const uint32_t m = h.len;
+        for (uint32_t p = m; p > 0; --p) {
+            float a;
+            uint32_t b;
+            h.pop(&a, &b);
+            if (lane == 0) {
+                out_idx[(size_t)r * k + p - 1] = b;
+                out_dist[(size_t)r * k + p - 1] = a;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ host side
+
+namespace {
+
+size_t pq_lds_limit()
+{
+    static size_t lim = 0;
+    if (!lim) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
+                hipSuccess &&
+            v > 0)
+            lim = (size_t)v;
+        else
+            lim = 65536;
+    }
+    return lim;
+}
+
+class PQ final : public Index {
+  public:
+    PQ(uint64_t dim, const hm_params &p) : Index(HM_INDEX_PQ, dim), p_(p)
+    {
+        if (dim == 0 || dim > 4096) fail(HM_ERR_INVALID_ARG, "dimension out of range (1..4096)");
+        if (p.n_sub == 0 || p.n_sub > 256 || dim / p.n_sub == 0)
+            fail(HM_ERR_INVALID_ARG, "n_sub must be in 1..min(256, dimension)");
+        // pq_idx.rs:54: assert!(sub_bits <= 32)
+        if (p.sub_bits == 0 || p.sub_bits > 32) fail(HM_ERR_INVALID_ARG, "sub_bits out of range");
+        if (p.sub_bits > 8)
+            fail(HM_ERR_NOT_IMPLEMENTED, "sub_bits above 8 (codes are stored as u8)");
+        dpad_ = (uint32_t)round_up(dim, 16);
+        M_ = p.n_sub;
+        KS_ = 1u << p.sub_bits;
+        // pq_idx.rs:75-88
+        const uint32_t sd = (uint32_t)(dim / M_), r = (uint32_t)(dim % sd);
+        std::vector<uint32_t> b(M_), e(M_);
+        for (uint32_t i = 0; i < M_; ++i) {
+            if (i < r) {
+                b[i] = i * (sd + 1);
+                e[i] = (i + 1) * (sd + 1);
+            } else {
+                b[i] = r * (sd + 1) + (i - r) * sd;
+                e[i] = r * (sd + 1) + (i + 1 - r) * sd;
+            }
+        }
+        sdm_ = sd + (r ? 1 : 0);
+        HM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        sub_b_.reserve(M_ * 4);
+        sub_e_.reserve(M_ * 4);
+        HM_HIP(hipMemcpyAsync(sub_b_.get(), b.data(), M_ * 4, hipMemcpyHostToDevice, stream_));
+        HM_HIP(hipMemcpyAsync(sub_e_.get(), e.data(), M_ * 4, hipMemcpyHostToDevice, stream_));
+        HM_HIP(hipStreamSynchronize(stream_));
+    }
+    ~PQ() override
+    {
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+
+    // pq_idx.rs:101-115: add_item (the 100,000-item limit is not reproduced)
+    void add(const float *rows, uint64_t n_rows) override
+    {
+        if (n_rows == 0) return;
+        if (!rows) fail(HM_ERR_INVALID_ARG, "rows is null");
+        check_finite_host(rows, n_rows * dim_);
+        grow(n_rows);
+        float *dst = base_.as<float>() + n_ * dpad_;
+        if (dpad_ != dim_) HM_HIP(hipMemsetAsync(dst, 0, n_rows * dpad_ * 4, stream_));
+        HM_HIP(hipMemcpy2DAsync(dst, dpad_ * 4, rows, dim_ * 4, dim_ * 4, n_rows,
+                                hipMemcpyHostToDevice, stream_));
+        HM_HIP(hipStreamSynchronize(stream_));
+        n_ += n_rows;
+    }
+
+    void add_device(const float *d_rows, uint64_t n_rows, uint64_t ld) override
+    {
+        if (n_rows == 0) return;
+        grow(n_rows);
+        launch_pad_rows(d_rows, ld, base_.as<float>() + n_ * dpad_, dpad_, n_rows, (uint32_t)dim_,
+                        dpad_, stream_);
+        HM_HIP(hipStreamSynchronize(stream_));
+        n_ += n_rows;
+    }
+
+    // pq_idx.rs:198-202: build(mt) sets the metric and trains
+    void build(int32_t metric) override
+    {
+        if (metric < HM_METRIC_UNKNOWN || metric > HM_METRIC_ANGULAR)
+            fail(HM_ERR_INVALID_ARG, "metric out of range");
+        if (metric == HM_METRIC_UNKNOWN) fail(HM_ERR_UNKNOWN_METRIC, "unknown method");
+        metric_ = metric;
+        train();
+    }
+
+    bool built() const override { return true; }  // pq_idx.rs:209-211
+    const char *name() const override { return "PQIndex"; }
+    uint64_t nodes_size() const override { return n_; }
+
+    void clear() override
+    {
+        base_.release();
+        n_ = cap_ = 0;
+        trained_ = false;
+        n_trained_ = 0;
+    }
+
+    void get_rows(const uint64_t *idx, uint64_t n, float *out) const override
+    {
+        for (uint64_t i = 0; i < n; ++i) {
+            if (idx[i] >= n_) fail(HM_ERR_INVALID_ARG, "row index out of range");
+            HM_HIP(hipMemcpyAsync(out + i * dim_, base_.as<float>() + idx[i] * dpad_, dim_ * 4,
+                                  hipMemcpyDeviceToHost, stream_));
+        }
+        HM_HIP(hipStreamSynchronize(stream_));
+    }
+
+    void info(uint32_t *out) const
+    {
+        out[0] = M_;
+        out[1] = KS_;
+        out[2] = sdm_;
+        out[3] = (uint32_t)n_trained_;
+        out[4] = trained_ ? 1 : 0;
+        out[5] = dpad_;
+        out[6] = (uint32_t)metric_;
+        out[7] = 0;
+    }
+
+    void export_trained(float *codebook, uint8_t *codes) const
+    {
+        if (!trained_) fail(HM_ERR_NOT_BUILT, "index not built");
+        if (codebook)
+            HM_HIP(hipMemcpyAsync(codebook, cb_.get(), (size_t)M_ * KS_ * sdm_ * 4,
+                                  hipMemcpyDeviceToHost, stream_));
+        if (codes && n_trained_)
+            HM_HIP(hipMemcpyAsync(codes, codes_.get(), n_trained_ * M_, hipMemcpyDeviceToHost,
+                                  stream_));
+        HM_HIP(hipStreamSynchronize(stream_));
+    }
+
+    void search_device(const float *dq, uint64_t nq, uint32_t k, uint64_t *d_idx, float *d_dist,
+                       uint32_t *d_count, hipStream_t s) override
+    {
+        if (!s) s = stream_;
+        if (!trained_) fail(HM_ERR_NOT_BUILT, "index not built");
+        if (nq == 0) return;
+        if (!dq || !d_count || (k && (!d_idx || !d_dist))) fail(HM_ERR_INVALID_ARG, "null pointer");
+        const uint32_t n = (uint32_t)n_trained_;
+        if (k == 0 || n == 0) {
+            launch_fill_results(d_idx, d_dist, d_count, nq, k, s);
+            HM_HIP(hipStreamSynchronize(s));
+            return;
+        }
+        const uint32_t K = std::min<uint32_t>(k + 1, n);
+        if (K > SELECT_MAX_K) fail(HM_ERR_INVALID_ARG, "k too large (max 8191)");
+        err_.reserve(4);
+        HM_HIP(hipMemsetAsync(err_.get(), 0, 4, s));
+        launch_check_finite(dq, dim_, nq, (uint32_t)dim_, err_.as<uint32_t>(), s);
+        const uint64_t ldd = round_up(n, 4);
+        uint64_t qc = std::min<uint64_t>(nq, 65535);
+        qc = std::min<uint64_t>(qc, std::max<uint64_t>(1, ((size_t)1 << 30) / (ldd * 4)));
+        const int qb = lut_qb();
+        qpad_.reserve(qc * dpad_ * 4);
+        D_.reserve(qc * ldd * 4);
+        cand_d_.reserve(qc * K * 4);
+        cand_i_.reserve(qc * K * 4);
+        flags_.reserve(qc * 4);
+        if (qb == 0) lutg_.reserve(qc * M_ * KS_ * 4);
+        std::vector<uint32_t> fl(qc);
+        last_replayed_ = 0;
+        for (uint64_t q0 = 0; q0 < nq; q0 += qc) {
+            const uint32_t rows = (uint32_t)std::min<uint64_t>(qc, nq - q0);
+            launch_pad_rows(dq + q0 * dim_, dim_, qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
+                            dpad_, s);
+            scan(rows, qb, ldd, s);
+            launch_row_select(D_.as<float>(), ldd, rows, n, K, cand_d_.as<float>(),
+                              cand_i_.as<uint32_t>(), K, flags_.as<uint32_t>(), s);
+            {
+                ProfScope ps("pq_finalize", s);
+                pq_finalize_kernel<<<rows, 64, 0, s>>>(
+                    cand_d_.as<float>(), cand_i_.as<uint32_t>(), K, flags_.as<uint32_t>(), rows, n,
+                    k, d_idx + q0 * k, d_dist + q0 * k, d_count + q0, err_.as<uint32_t>());
+                HM_CHECK_LAUNCH();
+            }
+            HM_HIP(hipMemcpyAsync(fl.data(), flags_.get(), rows * 4, hipMemcpyDeviceToHost, s));
+            HM_HIP(hipStreamSynchronize(s));
+            std::vector<uint32_t> tied;
+            for (uint32_t i = 0; i < rows; ++i)
+                if ((fl[i] & ROW_TIE) && !(fl[i] & ROW_NAN)) tied.push_back(i);
+            last_replayed_ += tied.size();
+            if (!tied.empty()) replay(tied, n, k, ldd, d_idx + q0 * k, d_dist + q0 * k, s);
+        }
+        uint32_t err = 0;
+        HM_HIP(hipMemcpyAsync(&err, err_.get(), 4, hipMemcpyDeviceToHost, s));
+        HM_HIP(hipStreamSynchronize(s));
+        if (err & 2u) fail(HM_ERR_NONFINITE_INPUT, "invalid float element");
+        if (err & 1u) fail(HM_ERR_NAN_DISTANCE, "NaN distance (partial_cmp of NaN)");
+    }
+
+    void search_stats(uint64_t *fallback, uint64_t *uncertified) const override
+    {
+        *fallback = last_replayed_;
+        *uncertified = 0;
+    }
+
+  private:
+    void grow(uint64_t n_rows)
+    {
+        if (n_ + n_rows > 0xFFFFFFF0ull) fail(HM_ERR_CAPACITY, "more than 2^32-16 rows");
+        const uint64_t need = n_ + n_rows;
+        if (need > cap_) {
+            uint64_t nc = std::max<uint64_t>(need, cap_ + cap_ / 2);
+            nc = std::max<uint64_t>(nc, 1024);
+            base_.grow_keep(nc * dpad_ * 4, n_ * dpad_ * 4, stream_);
+            cap_ = nc;
+        }
+    }
+
+    // 1 when the LUTs of PS_Q queries fit in LDS (the scan kernel), else 0
+    // (LUTs in global memory).
+    int lut_qb() const
+    {
+        const size_t per = (size_t)M_ * KS_ * 4, lim = std::min<size_t>(pq_lds_limit(), 131072);
+        return per * PS_Q <= lim ? 1 : 0;
+    }
+
+    template <int MET> void scan_m(uint32_t rows, int qb, uint64_t ldd, hipStream_t s)
+    {
+        const uint32_t n = (uint32_t)n_trained_;
+        if (qb == 0) {
+            const uint32_t nsplit = std::max<uint32_t>(1, std::min<uint32_t>(
+                ceil_div(1024, rows), std::max<uint32_t>(1, n / 2048)));
+            const uint32_t per_split = (uint32_t)round_up(ceil_div(n, nsplit), 64);
+            pq_lut_global_kernel<MET><<<rows, PS_THREADS, 0, s>>>(
+                qpad_.as<float>(), dpad_, sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), M_, KS_,
+                sdm_, cb_.as<float>(), cbn_.as<float>(), lutg_.as<float>());
+            HM_CHECK_LAUNCH();
+            pq_scan_glut_kernel<<<dim3(ceil_div(n, per_split), rows), PS_THREADS, 0, s>>>(
+                rows, M_, KS_, codes_.as<uint8_t>(), n, per_split, lutg_.as<float>(),
+                D_.as<float>(), ldd);
+            HM_CHECK_LAUNCH();
+            return;
+        }
+        // about 2048 workgroups; each range a multiple of 2 x 256 items
+        const uint32_t groups = ceil_div(rows, PS_Q);
+        const uint32_t nsplit = std::max<uint32_t>(1, std::min<uint32_t>(
+            ceil_div(2048, groups), std::max<uint32_t>(1, n / 4096)));
+        const uint32_t per_split = (uint32_t)round_up(ceil_div(n, nsplit), 2 * PS_THREADS);
+        const uint32_t ns = ceil_div(n, per_split);
+        const size_t lds = (size_t)M_ * KS_ * PS_Q * 4;
+        pq_scan_kernel<MET><<<dim3(ns, groups), PS_THREADS, lds, s>>>(
+            qpad_.as<float>(), dpad_, rows, sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), M_, KS_,
+            sdm_, cb_.as<float>(), cbn_.as<float>(), codes_.as<uint8_t>(), n, per_split,
+            D_.as<float>(), ldd);
+        HM_CHECK_LAUNCH();
+    }
+
+    void scan(uint32_t rows, int qb, uint64_t ldd, hipStream_t s)
+    {
+        ProfScope ps("pq_scan", s);
+        switch (metric_) {
+        case HM_METRIC_EUCLIDEAN: scan_m<PM_EUCLID>(rows, qb, ldd, s); break;
+        case HM_METRIC_MANHATTAN: scan_m<PM_MANHATTAN>(rows, qb, ldd, s); break;
+        case HM_METRIC_DOT_PRODUCT: scan_m<PM_DOT>(rows, qb, ldd, s); break;
+        case HM_METRIC_COSINE_SIMILARITY: scan_m<PM_COSINE>(rows, qb, ldd, s); break;
+        default: scan_m<PM_ANGULAR>(rows, qb, ldd, s); break;
+        }
+    }
+
+    void replay(const std::vector<uint32_t> &tied, uint32_t n, uint32_t k, uint64_t ldd,
+                uint64_t *d_idx, float *d_dist, hipStream_t s)
+    {
+        ProfScope ps("pq_replay", s);
+        const uint32_t m = (uint32_t)tied.size();
+        rp_rows_.reserve(m * 4);
+        HM_HIP(hipMemcpyAsync(rp_rows_.get(), tied.data(), m * 4, hipMemcpyHostToDevice, s));
+        const size_t lds = std::min<size_t>(pq_lds_limit(), 131072) & ~(size_t)7;
+        const uint32_t H = (uint32_t)std::min<size_t>(lds / 8, n);
+        const uint32_t grid = std::min<uint32_t>(m, 256);
+        const size_t gslice = n > H ? (size_t)(n - H) : 0;
+        if (gslice) {
+            rp_gd_.reserve((size_t)grid * gslice * 4);
+            rp_gi_.reserve((size_t)grid * gslice * 4);
+        }
+        const size_t use = (size_t)H * 8;
+        if (gslice)
+            pq_replay_kernel<true><<<grid, PR_THREADS, use, s>>>(
+                rp_rows_.as<uint32_t>(), m, D_.as<float>(), ldd, n, k, H, rp_gd_.as<float>(),
+                rp_gi_.as<uint32_t>(), d_idx, d_dist);
+        else
+            pq_replay_kernel<false><<<grid, PR_THREADS, use, s>>>(
+                rp_rows_.as<uint32_t>(), m, D_.as<float>(), ldd, n, k, H, nullptr, nullptr, d_idx,
+                d_dist);
+        HM_CHECK_LAUNCH();
+        HM_HIP(hipStreamSynchronize(s));
+    }
+
+    template <int MET> void assign_t(uint8_t *out, uint64_t si, uint64_t ss, hipStream_t s)
+    {
+        const uint32_t n = (uint32_t)n_;
+        const size_t staged_bytes = ((size_t)KS_ + (size_t)KS_ * (sdm_)) * 4;
+        dim3 grid(ceil_div(n, PA_THREADS), M_);
+        if (staged_bytes <= 65536)
+            pq_assign_kernel<MET, true><<<grid, PA_THREADS, staged_bytes, s>>>(
+                base_.as<float>(), dpad_, n, sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), KS_,
+                sdm_, cb_.as<float>(), out, si, ss);
+        else
+            pq_assign_kernel<MET, false><<<grid, PA_THREADS, (size_t)KS_ * 4, s>>>(
+                base_.as<float>(), dpad_, n, sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), KS_,
+                sdm_, cb_.as<float>(), out, si, ss);
+        HM_CHECK_LAUNCH();
+    }
+
+    void assign(uint8_t *out, uint64_t si, uint64_t ss, hipStream_t s)
+    {
+        ProfScope ps("pq_assign", s);
+        switch (metric_) {
+        case HM_METRIC_EUCLIDEAN: assign_t<PM_EUCLID>(out, si, ss, s); break;
+        case HM_METRIC_MANHATTAN: assign_t<PM_MANHATTAN>(out, si, ss, s); break;
+        case HM_METRIC_DOT_PRODUCT: assign_t<PM_DOT>(out, si, ss, s); break;
+        case HM_METRIC_COSINE_SIMILARITY: assign_t<PM_COSINE>(out, si, ss, s); break;
+        default: assign_t<PM_ANGULAR>(out, si, ss, s); break;
+        }
+    }
+
+    // PQIndex::train_center (pq_idx.rs:122-149) -> Kmeans::train
+    // (kmeans.rs:195-207) per sub-space, then search_data for the codes.
+    void train()
+    {
+        hipStream_t s = stream_;
+        const uint64_t n = n_;
+        // kmeans.rs:158-160 + 203-204: a batch of 0 is an Err that train unwraps
+        if (n == 0) fail(HM_ERR_EMPTY, "None to assigned impossible split center");
+        const bool fp64 = p_.kmeans_accumulate != 0;
+        const uint32_t epochs = p_.train_epoch;
+        cb_.reserve((size_t)M_ * KS_ * sdm_ * 4);
+        cbn_.reserve((size_t)M_ * KS_ * 4);
+        asg_.reserve((size_t)M_ * n);
+        counts_.reserve((size_t)M_ * KS_ * 4);
+        codes_.reserve(n * M_ + 4);
+        rng_.reserve((size_t)M_ * 8);
+        std::vector<uint64_t> rs(M_);
+        for (uint32_t i = 0; i < M_; ++i) rs[i] = host_stream_seed(p_.seed, 1 + i);
+        HM_HIP(hipMemcpyAsync(rng_.get(), rs.data(), M_ * 8, hipMemcpyHostToDevice, s));
+        HM_HIP(hipMemsetAsync(cb_.get(), 0, (size_t)M_ * KS_ * sdm_ * 4, s));
+        {
+            ProfScope ps("pq_init", s);
+            const uint32_t used = (uint32_t)dim_;
+            if (fp64)
+                pq_init_kernel<double><<<ceil_div(used, 64), 64, 0, s>>>(
+                    base_.as<float>(), dpad_, n, sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), M_,
+                    KS_, sdm_, cb_.as<float>());
+            else
+                pq_init_kernel<float><<<ceil_div(used, 64), 64, 0, s>>>(
+                    base_.as<float>(), dpad_, n, sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), M_,
+                    KS_, sdm_, cb_.as<float>());
+            HM_CHECK_LAUNCH();
+        }
+        for (uint32_t ep = 0; ep < epochs; ++ep) {
+            assign(asg_.as<uint8_t>(), 1, n, s);
+            {
+                ProfScope ps("pq_update", s);
+                if (fp64)
+                    pq_update_kernel<double><<<M_, PU_THREADS, 0, s>>>(
+                        base_.as<float>(), dpad_, (uint32_t)n, sub_b_.as<uint32_t>(),
+                        sub_e_.as<uint32_t>(), KS_, sdm_, asg_.as<uint8_t>(), cb_.as<float>(),
+                        counts_.as<uint32_t>());
+                else
+                    pq_update_kernel<float><<<M_, PU_THREADS, 0, s>>>(
+                        base_.as<float>(), dpad_, (uint32_t)n, sub_b_.as<uint32_t>(),
+                        sub_e_.as<uint32_t>(), KS_, sdm_, asg_.as<uint8_t>(), cb_.as<float>(),
+                        counts_.as<uint32_t>());
+                HM_CHECK_LAUNCH();
+            }
+            if (ep + 1 < epochs) {
+                pq_split_kernel<<<ceil_div(M_, 64), 64, 0, s>>>(
+                    cb_.as<float>(), M_, KS_, sdm_, sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(),
+                    counts_.as<uint32_t>(), n, rng_.as<uint64_t>());
+                HM_CHECK_LAUNCH();
+            }
+        }
+        assign(codes_.as<uint8_t>(), M_, 1, s);
+        if (metric_ == HM_METRIC_COSINE_SIMILARITY || metric_ == HM_METRIC_ANGULAR)
+            codeword_norms(s);
+        HM_HIP(hipStreamSynchronize(s));
+        trained_ = true;
+        n_trained_ = n;
+    }
+
+    // cbn[s][c] = the s_yy of the LUT's Cosine / Angular entries.
+    void codeword_norms(hipStream_t s)
+    {
+        pq_norms_kernel<<<ceil_div((size_t)M_ * KS_, 256), 256, 0, s>>>(
+            cb_.as<float>(), M_, KS_, sdm_, sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(),
+            cbn_.as<float>());
+        HM_CHECK_LAUNCH();
+    }
+
+    hm_params p_;
+    uint32_t dpad_ = 0, M_ = 0, KS_ = 0, sdm_ = 0;
+    uint64_t n_ = 0, cap_ = 0, n_trained_ = 0, last_replayed_ = 0;
+    int32_t metric_ = HM_METRIC_UNKNOWN;
+    bool trained_ = false;
+    DeviceBuffer base_, sub_b_, sub_e_, cb_, cbn_, asg_, counts_, codes_, rng_;
+    DeviceBuffer qpad_, D_, cand_d_, cand_i_, flags_, err_, lutg_, rp_rows_, rp_gd_, rp_gi_;
+};
+
+}  // namespace
+
+std::unique_ptr<Index> make_pq(uint64_t dim, const hm_params &p)
+{
+    return std::make_unique<PQ>(dim, p);
+}
+
+int32_t pq_info(Index *ix, uint32_t *out)
+{
+    auto *p = dynamic_cast<PQ *>(ix);
+    if (!p) fail(HM_ERR_INVALID_ARG, "not a PQIndex");
+    p->info(out);
+    return HM_OK;
+}
+
+int32_t pq_export(Index *ix, float *codebook, uint8_t *codes)
+{
+    auto *p = dynamic_cast<PQ *>(ix);
+    if (!p) fail(HM_ERR_INVALID_ARG, "not a PQIndex");
+    p->export_trained(codebook, codes);
+    return HM_OK;
+}
+
+}  // namespace hm

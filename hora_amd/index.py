@@ -6,6 +6,7 @@ Names, argument meaning and error strings follow the reference:
   Node                      src/core/node.rs:93-167
   BruteForceIndex / Params  src/index/bruteforce_idx.rs, bruteforce_params.rs
   IVFPQIndex / IVFPQParams  src/index/pq_idx.rs:267-523, pq_params.rs:43-91
+  PQIndex / PQParams        src/index/pq_idx.rs:21-265, pq_params.rs:7-41
 Where the reference returns Err(&'static str) or panics, this mirror raises
 HoraError carrying the status and the reference's message.
 
@@ -23,6 +24,7 @@ from ._lib import HmParams, HoraError, check, lib
 
 HM_INDEX_BRUTE_FORCE = 1
 HM_INDEX_IVFPQ = 2
+HM_INDEX_PQ = 3
 
 HM_SEARCH_EXACT = 0
 HM_SEARCH_BF16_RERANK = 1
@@ -117,6 +119,37 @@ class IVFPQParams:
         return self
 
 
+class PQParams:
+    """pq_params.rs:7-41 builder; defaults of PQParams::default() (n_sub 4,
+    sub_bits 4, train_epoch 100).  seed / kmeans_accumulate as in
+    IVFPQParams."""
+
+    def __init__(self):
+        p = HmParams()
+        check(lib().hm_params_default(HM_INDEX_PQ, C.byref(p)))
+        self._p = p
+
+    def n_sub(self, v: int) -> "PQParams":
+        self._p.n_sub = v
+        return self
+
+    def sub_bits(self, v: int) -> "PQParams":
+        self._p.sub_bits = v
+        return self
+
+    def train_epoch(self, v: int) -> "PQParams":
+        self._p.train_epoch = v
+        return self
+
+    def seed(self, v: int) -> "PQParams":
+        self._p.seed = v
+        return self
+
+    def kmeans_accumulate(self, v: int) -> "PQParams":
+        self._p.kmeans_accumulate = v
+        return self
+
+
 def _f32_rows(a, dim: int) -> np.ndarray:
     a = np.ascontiguousarray(a, dtype=np.float32)
     if a.ndim == 1:
@@ -131,7 +164,7 @@ class _Index:
 
     def __init__(self, dimension: int, params: Optional[Any] = None):
         h = C.c_void_p()
-        p = params._p if isinstance(params, IVFPQParams) else None
+        p = params._p if isinstance(params, (IVFPQParams, PQParams)) else None
         check(lib().hm_index_create(self._kind, dimension,
                                     C.byref(p) if p is not None else None, C.byref(h)))
         self._h = h
@@ -364,6 +397,33 @@ class IVFPQIndex(_Index):
 
 
 PQIVFIndex = IVFPQIndex  # README.md:57 name
+
+
+class PQIndex(_Index):
+    """pq_idx.rs:21-265: product quantization over the whole base (no
+    coarse quantizer, no residual); the sub-space k-means and the LUT use
+    the build metric."""
+    _kind = HM_INDEX_PQ
+
+    def __init__(self, dimension: int, params: Optional[PQParams] = None):
+        super().__init__(dimension, params if params is not None else PQParams())
+
+    def info(self) -> dict:
+        out = np.zeros(8, np.uint32)
+        check(lib().hm_pq_info(self._h, out.ctypes.data))
+        keys = ["n_sub", "n_sub_center", "sub_dim_max", "n_trained", "trained", "dpad",
+                "metric", "reserved"]
+        return dict(zip(keys, out.tolist()))
+
+    def export(self) -> dict:
+        """Trained state: codebook [n_sub][2^sub_bits][sub_dim_max] and codes
+        [n][n_sub] (see hm_pq_export)."""
+        i = self.info()
+        m, ks, sdm, n = i["n_sub"], i["n_sub_center"], i["sub_dim_max"], i["n_trained"]
+        cb = np.zeros((m, ks, sdm), np.float32)
+        codes = np.zeros((max(n, 1), m), np.uint8)
+        check(lib().hm_pq_export(self._h, cb.ctypes.data, codes.ctypes.data))
+        return {"codebook": cb, "codes": codes[:n]}
 
 
 def mlp_loss_grad(W1, b1, W2, b2, X, T=None):

@@ -22,8 +22,8 @@
  *    distance order (node_search_k's drain+reverse, bruteforce_idx.rs:63-71);
  *    unused slots hold index UINT64_MAX and distance +inf.
  *  - Reference panics become error statuses (the reference has no status
- *    for them); the reference's 100,001-item IVFPQ cap (pq_idx.rs:326) is
- *    not reproduced.
+ *    for them); the reference's 100,001-item caps of IVFPQIndex and PQIndex
+ *    (pq_idx.rs:326, 66) are not reproduced.
  */
 #ifndef HORA_MI355X_H
 #define HORA_MI355X_H
@@ -66,11 +66,13 @@ typedef enum {
 
 typedef enum {
     HM_INDEX_BRUTE_FORCE = 1, /* BruteForceIndex (bruteforce_idx.rs:15-22) */
-    HM_INDEX_IVFPQ = 2        /* IVFPQIndex, exported as PQIVFIndex too (pq_idx.rs:267-292) */
+    HM_INDEX_IVFPQ = 2,       /* IVFPQIndex, exported as PQIVFIndex too (pq_idx.rs:267-292) */
+    HM_INDEX_PQ = 3           /* standalone PQIndex (pq_idx.rs:21-45) */
 } hm_index_kind;
 
 /* BruteForceParams is empty (bruteforce_params.rs:5-14); IVFPQParams fields
- * (pq_params.rs:43-91).  `seed` seeds the k-means split step, which the
+ * (pq_params.rs:43-91); PQParams (pq_params.rs:7-41) uses n_sub, sub_bits
+ * and train_epoch.  `seed` seeds the k-means split step, which the
  * reference draws from an unseeded thread_rng (kmeans.rs:168). */
 typedef struct {
     uint32_t n_sub;
@@ -93,11 +95,13 @@ typedef enum {
 } hm_search_mode;
 
 /* Fills the reference defaults: IVFPQParams::default() (pq_params.rs:80-91)
- * for HM_INDEX_IVFPQ; zeros for HM_INDEX_BRUTE_FORCE. */
+ * for HM_INDEX_IVFPQ, PQParams::default() (pq_params.rs:31-40) for
+ * HM_INDEX_PQ; zeros for HM_INDEX_BRUTE_FORCE. */
 HM_API int32_t hm_params_default(int32_t kind, hm_params *out);
 
 /* BruteForceIndex::new (bruteforce_idx.rs:25-32), IVFPQIndex::new
- * (pq_idx.rs:295-330).  Binds to the current HIP device. */
+ * (pq_idx.rs:295-330), PQIndex::new (pq_idx.rs:48-90).  Binds to the
+ * current HIP device. */
 HM_API int32_t hm_index_create(int32_t kind, uint64_t dimension, const hm_params *params,
                                hm_index **out);
 
@@ -164,7 +168,8 @@ HM_API int32_t hm_index_nodes_size(const hm_index *h, uint64_t *out);
 /* ANNIndex::dimension (ann_index.rs:118). */
 HM_API int32_t hm_index_dimension(const hm_index *h, uint64_t *out);
 
-/* ANNIndex::name (ann_index.rs:102): "BruteForceIndex" / "IVFPQIndex". */
+/* ANNIndex::name (ann_index.rs:102): "BruteForceIndex" / "IVFPQIndex" /
+ * "PQIndex". */
 HM_API const char *hm_index_name(const hm_index *h);
 
 /* ANNIndex::clear (ann_index.rs:110): drops all rows and built state. */
@@ -192,6 +197,17 @@ HM_API int32_t hm_ivfpq_info(hm_index *h, uint32_t *out8);
  * r belonging to list_ids[r].  Any output pointer may be NULL. */
 HM_API int32_t hm_ivfpq_export(hm_index *h, float *centers, uint64_t *list_offsets,
                                uint64_t *list_ids, float *codebooks, uint8_t *codes);
+
+/* Standalone PQIndex shape after build: out8 = {n_sub, 2^sub_bits,
+ * sub_dim_max, rows trained, trained flag, padded dimension, metric, 0}
+ * (private fields of pq_idx.rs:22-45). */
+HM_API int32_t hm_pq_info(hm_index *h, uint32_t *out8);
+
+/* Standalone PQIndex trained state (_centers and _assigned_center of
+ * pq_idx.rs:33, 41): codebook [n_sub][2^sub_bits][sub_dim_max] f32 (a
+ * sub-space with fewer dims is zero padded), codes [n][n_sub] u8 (row i is
+ * item i).  Either pointer may be NULL. */
+HM_API int32_t hm_pq_export(hm_index *h, float *codebook, uint8_t *codes);
 
 /* Learned probe (no reference counterpart: the reference probes a fixed
  * search_n_center lists, pq_idx.rs:424).  Trains, on the GPU, a network

@@ -81,6 +81,11 @@ def lib():
     L.orc_ivfpq_search.argtypes = [_f32p, _sz, _sz, C.c_int, _f32p, _sz, _f32p, _sz, _sz, _sz,
                                    _u64p, _u64p, _u32p, _sz, _sz, C.c_int, _u64p, _f32p,
                                    _u32p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_pqindex_train.argtypes = [_f32p, _sz, _sz, _sz, _sz, C.c_int, _sz, C.c_uint64,
+                                    _f32p, _sz, _u32p]
+    L.orc_pqindex_search.argtypes = [_f32p, _sz, _sz, C.c_int, _f32p, _sz, _sz, _sz, _u32p,
+                                     _sz, _sz, C.c_int, _u64p, _f32p, _u32p]
+    L.orc_pqindex_lut.argtypes = [_f32p, _sz, _sz, _f32p, _sz, _sz, C.c_int, _f32p]
     _lib = L
     return L
 
@@ -321,3 +326,53 @@ def ivfpq_search(queries, mt: int, centers, codebooks, list_offsets, list_ids, c
     if st:
         raise OracleError(st, "ivfpq_search")
     return ids[:, :k], dist[:, :k], cnt, probes[:, :nprobe]
+
+
+def pq_sub_dim_max(d: int, n_sub: int) -> int:
+    sd = d // n_sub
+    return sd + (1 if d % sd else 0)
+
+
+def pqindex_train(items, n_sub: int, n_center: int, mt: int, epochs: int, seed: int):
+    """Standalone PQIndex build (pq_idx.rs:122-149, 198-202): codebook
+    [n_sub][n_center][sub_dim_max], codes [n][n_sub]."""
+    items = _f32(items)
+    n, d = items.shape
+    sdm = pq_sub_dim_max(d, n_sub)
+    cb = np.zeros((n_sub, n_center, sdm), np.float32)
+    codes = np.zeros((max(n, 1), n_sub), np.uint32)
+    st = lib().orc_pqindex_train(items, n, d, n_sub, n_center, mt, epochs, seed, cb, sdm, codes)
+    if st:
+        raise OracleError(st, "pqindex_train")
+    return cb, codes[:n]
+
+
+def pqindex_search(queries, mt: int, codebook, codes, k: int, nthreads: int = 0):
+    """Standalone PQIndex::node_search_k (pq_idx.rs:165-194, 212-233) on a
+    given codebook and codes: (ids [nq,k], dist [nq,k], count [nq])."""
+    q = _f32(queries)
+    nq, d = q.shape
+    cb = _f32(codebook)
+    n_sub, n_center, sdm = cb.shape
+    cd = np.ascontiguousarray(codes, dtype=np.uint32)
+    n = cd.shape[0]
+    kk = max(k, 1)
+    ids = np.zeros((nq, kk), np.uint64)
+    dist = np.zeros((nq, kk), np.float32)
+    cnt = np.zeros(max(nq, 1), np.uint32)
+    st = lib().orc_pqindex_search(q, nq, d, mt, cb, n_sub, n_center, sdm, cd, n, k, nthreads,
+                                  ids, dist, cnt)
+    if st:
+        raise OracleError(st, "pqindex_search")
+    return ids[:, :k], dist[:, :k], cnt[:nq]
+
+
+def pqindex_lut(q, mt: int, codebook):
+    q = _f32(q).reshape(-1)
+    cb = _f32(codebook)
+    n_sub, n_center, sdm = cb.shape
+    lut = np.zeros(n_sub * n_center, np.float32)
+    st = lib().orc_pqindex_lut(q, q.size, n_sub, cb, n_center, sdm, mt, lut)
+    if st:
+        raise OracleError(st, "pqindex_lut")
+    return lut.reshape(n_sub, n_center)

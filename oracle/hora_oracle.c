@@ -634,9 +634,10 @@ int orc_pq_ranges(size_t d, size_t n_sub, uint32_t *begin, uint32_t *end)
  * fewer dims leaves its tail zero), codes [item][n_sub].  The nested PQ
  * always uses Euclidean (pq_idx.rs:71).  The RNG state runs through the subs
  * in order. */
-int orc_pq_train(const float *items, size_t n, size_t d, size_t n_sub, size_t n_center,
-                 const float *residual, size_t epochs, uint64_t seed, uint64_t first_instance,
-                 float *codebook, size_t sub_dim_max, uint32_t *codes)
+static int pq_train_mt(const float *items, size_t n, size_t d, size_t n_sub, size_t n_center,
+                       const float *residual, int mt, size_t epochs, uint64_t seed,
+                       uint64_t first_instance, float *codebook, size_t sub_dim_max,
+                       uint32_t *codes)
 {
     uint32_t b[256], e[256];
     if (n_sub > 256) return ORC_ERR_ARG;
@@ -653,7 +654,7 @@ int orc_pq_train(const float *items, size_t n, size_t d, size_t n_sub, size_t n_
     for (size_t s = 0; s < n_sub; ++s) {
         size_t sdim = e[s] - b[s];
         float *cent = (float *)calloc(n_center * sdim, sizeof(float));
-        orc_kmeans km = {sdim, n_center, b[s], residual, ORC_EUCLIDEAN, cent};
+        orc_kmeans km = {sdim, n_center, b[s], residual, mt, cent};
         uint64_t rng = orc_stream_seed(seed, first_instance + s);
         st = km_train(&km, n, items, d, epochs, &rng, assign);
         if (st) {
@@ -670,8 +671,48 @@ int orc_pq_train(const float *items, size_t n, size_t d, size_t n_sub, size_t n_
     return st;
 }
 
+int orc_pq_train(const float *items, size_t n, size_t d, size_t n_sub, size_t n_center,
+                 const float *residual, size_t epochs, uint64_t seed, uint64_t first_instance,
+                 float *codebook, size_t sub_dim_max, uint32_t *codes)
+{
+    return pq_train_mt(items, n, d, n_sub, n_center, residual, ORC_EUCLIDEAN, epochs, seed,
+                       first_instance, codebook, sub_dim_max, codes);
+}
+
+/* Standalone PQIndex::train_center (pq_idx.rs:122-149) after build(mt)
+ * (pq_idx.rs:198-202): no residual, and the sub-space k-means use the build
+ * metric (Kmeans::new(.., self.mt), pq_idx.rs:133).  The RNG streams of the
+ * subs are instances 1..n_sub of the seed. */
+int orc_pqindex_train(const float *items, size_t n, size_t d, size_t n_sub, size_t n_center,
+                      int mt, size_t epochs, uint64_t seed, float *codebook, size_t sub_dim_max,
+                      uint32_t *codes)
+{
+    if (mt == ORC_UNKNOWN || mt > ORC_ANGULAR || mt < 0) return ORC_ERR_UNKNOWN_METRIC;
+    if (n == 0) return ORC_ERR_EMPTY;
+    return pq_train_mt(items, n, d, n_sub, n_center, NULL, mt, epochs, seed, 1, codebook,
+                       sub_dim_max, codes);
+}
+
 /* pq_idx.rs:151-163, 170-179: LUT[s*K + c] = Euclid(fl(q[b..e] - res[b..e]),
  * codeword[s][c]); the residual difference is rounded to f32 first. */
+static void pq_lut_mt(const float *q, const float *residual, size_t n_sub, const uint32_t *b,
+                      const uint32_t *e, const float *codebook, size_t n_center,
+                      size_t sub_dim_max, int mt, float *lut)
+{
+    float z[4096];
+    for (size_t s = 0; s < n_sub; ++s) {
+        size_t sdim = e[s] - b[s];
+        for (size_t j = 0; j < sdim; ++j) {
+            float v = q[b[s] + j];
+            if (residual) v = v - residual[b[s] + j];
+            z[j] = v;
+        }
+        for (size_t c = 0; c < n_center; ++c)
+            orc_metric(z, codebook + (s * n_center + c) * sub_dim_max, sdim, mt,
+                       &lut[s * n_center + c]);
+    }
+}
+
 static void pq_lut(const float *q, const float *residual, size_t d, size_t n_sub,
                    const uint32_t *b, const uint32_t *e, const float *codebook, size_t n_center,
                    size_t sub_dim_max, float *lut)
@@ -716,6 +757,84 @@ static void pq_adc_heap(const float *lut, const uint32_t *codes, size_t n, size_
         heap_push(h, x);
     }
     while (h->len > k) (void)heap_pop(h);
+}
+
+/* Standalone PQIndex::search_knn_adc (pq_idx.rs:165-194) + node_search_k
+ * (212-233): LUT with the build metric on the raw query sub-vectors (no
+ * residual), ADC over every item in insertion order, all pushed into an
+ * unbounded heap, popped down to k, then drained (descending) and reversed.
+ * codebook [n_sub][n_center][sub_dim_max], codes [n][n_sub].  A NaN
+ * distance among n >= 2 items is the reference's partial_cmp().unwrap()
+ * panic (neighbor.rs:32): ORC_ERR_NAN. */
+int orc_pqindex_search(const float *queries, size_t nq, size_t d, int mt, const float *codebook,
+                       size_t n_sub, size_t n_center, size_t sub_dim_max, const uint32_t *codes,
+                       size_t n, size_t k, int nthreads, uint64_t *out_idx, float *out_dist,
+                       uint32_t *out_count)
+{
+    uint32_t b[256], e[256];
+    if (mt == ORC_UNKNOWN || mt > ORC_ANGULAR || mt < 0) return ORC_ERR_UNKNOWN_METRIC;
+    if (n_sub > 256) return ORC_ERR_ARG;
+    int st = orc_pq_ranges(d, n_sub, b, e);
+    if (st) return st;
+    int status = ORC_OK;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        orc_heap h = {0};
+        float *lut = (float *)malloc(n_sub * n_center * sizeof(float));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (long qi = 0; qi < (long)nq; ++qi) {
+            pq_lut_mt(queries + (size_t)qi * d, NULL, n_sub, b, e, codebook, n_center,
+                      sub_dim_max, mt, lut);
+            int nan_seen = 0;
+            h.len = 0;
+            for (size_t i = 0; i < n; ++i) {
+                float dist = 0.0f;
+                for (size_t j = 0; j < n_sub; ++j)
+                    dist = dist + lut[j * n_center + codes[i * n_sub + j]];
+                if (dist != dist) nan_seen = 1;
+                orc_nb x = {dist, (uint64_t)i};
+                heap_push(&h, x);
+            }
+            if (nan_seen && n >= 2) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+                status = ORC_ERR_NAN;
+            }
+            while (h.len > k) (void)heap_pop(&h);
+            size_t m = h.len;
+            for (size_t r = m; r > 0; --r) {
+                orc_nb x = heap_pop(&h);
+                out_idx[(size_t)qi * k + r - 1] = x.idx;
+                out_dist[(size_t)qi * k + r - 1] = x.dist;
+            }
+            for (size_t r = m; r < k; ++r) {
+                out_idx[(size_t)qi * k + r] = UINT64_MAX;
+                out_dist[(size_t)qi * k + r] = INFINITY;
+            }
+            out_count[qi] = (uint32_t)m;
+        }
+        free(lut);
+        free(h.data);
+    }
+    return status;
+}
+
+/* LUT of the standalone PQIndex (pq_idx.rs:170-179) with the build metric. */
+int orc_pqindex_lut(const float *q, size_t d, size_t n_sub, const float *codebook,
+                    size_t n_center, size_t sub_dim_max, int mt, float *lut)
+{
+    uint32_t b[256], e[256];
+    if (n_sub > 256) return ORC_ERR_ARG;
+    int st = orc_pq_ranges(d, n_sub, b, e);
+    if (st) return st;
+    pq_lut_mt(q, NULL, n_sub, b, e, codebook, n_center, sub_dim_max, mt, lut);
+    return ORC_OK;
 }
 
 /* IVFPQIndex::search_knn_adc (pq_idx.rs:408-439) + node_search_k (458-477).

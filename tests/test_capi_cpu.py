@@ -27,6 +27,9 @@ def test_params_default_matches_reference_defaults():
     # IVFPQParams::default (pq_params.rs:80-91)
     assert (p.n_sub, p.sub_bits, p.n_kmeans_center, p.search_n_center, p.train_epoch) == \
         (25, 4, 256, 8, 100)
+    # PQParams::default (pq_params.rs:31-40)
+    assert hora_amd.lib().hm_params_default(3, C.byref(p)) == 0
+    assert (p.n_sub, p.sub_bits, p.train_epoch) == (4, 4, 100)
     assert hora_amd.lib().hm_params_default(99, C.byref(p)) == 1
     assert b"unknown index kind" in hora_amd.lib().hm_last_error()
 

@@ -266,3 +266,99 @@ def test_ivfpq_search_matches_bruteforce_over_probed_lists(orc):
         assert len({c[0] for c in cand[:8]}) == 8  # tie-free case
         assert ids[r].tolist() == [c[1] for c in cand[:7]]
         assert dist[r].tolist() == [c[0] for c in cand[:7]]
+
+
+# ---------------------------------------------------------- standalone PQ
+def _adc_fold(lut, codes):
+    # sequential f32 sum over the sub-spaces, starting at +0.0 (pq_idx.rs:183-186)
+    d = np.zeros(codes.shape[0], np.float32)
+    for j in range(codes.shape[1]):
+        d = (d + lut[j][codes[:, j]]).astype(np.float32)
+    return d
+
+
+@pytest.mark.parametrize("mt", [1, 2, 3, 4, 5])
+def test_pqindex_lut_is_the_metric_per_codeword(orc, mt):
+    rng = np.random.default_rng(30 + mt)
+    d, n_sub, K = 10, 4, 8  # dims 8 and 9 are dropped (pq_idx.rs:75-88)
+    cb = rng.normal(0, 1, (n_sub, K, 2)).astype(np.float32)
+    q = rng.normal(0, 1, d).astype(np.float32)
+    lut = orc.pqindex_lut(q, mt, cb)
+    for s, (b, e) in enumerate(orc.pq_ranges(d, n_sub)):
+        for c in range(K):
+            assert lut[s, c] == orc.metric(q[b:e], cb[s, c, :e - b], mt)
+    # the dot LUT is the plain inner product (metrics.rs:50-57: double negation)
+    if mt == orc.DOT:
+        assert lut[0, 0] == np.float32(q[0] * cb[0, 0, 0]) + np.float32(q[1] * cb[0, 0, 1])
+
+
+def test_pqindex_search_without_ties_is_a_sort(orc):
+    rng = np.random.default_rng(40)
+    d, n_sub, K, n = 16, 4, 64, 700
+    cb = rng.normal(0, 1, (n_sub, K, 4)).astype(np.float32)
+    codes = rng.integers(0, K, (n, n_sub)).astype(np.uint32)
+    q = rng.normal(0, 1, (6, d)).astype(np.float32)
+    ids, dist, cnt = orc.pqindex_search(q, orc.EUCLIDEAN, cb, codes, 9)
+    for r in range(6):
+        adc = _adc_fold(orc.pqindex_lut(q[r], orc.EUCLIDEAN, cb), codes)
+        order = np.argsort(adc, kind="stable")
+        assert len(np.unique(adc[order[:10]])) == 10  # tie-free head
+        assert cnt[r] == 9
+        assert ids[r].tolist() == order[:9].tolist()
+        assert np.array_equal(dist[r], adc[order[:9]])
+
+
+def test_pqindex_search_ties_follow_the_std_heap(orc):
+    # 2 sub-spaces x 3 codewords: 9 distinct code rows over 80 items, so
+    # every query's head is tied; the expected result is the reference's
+    # push-all / pop-to-k / drain sequence run on the hand-traced heap model
+    rng = np.random.default_rng(41)
+    cb = rng.integers(0, 3, (2, 3, 1)).astype(np.float32)
+    codes = rng.integers(0, 3, (80, 2)).astype(np.uint32)
+    q = rng.integers(0, 3, (5, 2)).astype(np.float32)
+    for k in (1, 4, 13, 80, 100):
+        ids, dist, cnt = orc.pqindex_search(q, orc.EUCLIDEAN, cb, codes, k)
+        for r in range(5):
+            adc = _adc_fold(orc.pqindex_lut(q[r], orc.EUCLIDEAN, cb), codes)
+            n = adc.size
+            m = min(k, n)
+            ops = [0] * n + [1] * (n - m) + [1] * m
+            _, popped = orc.heap_script(ops, np.concatenate([adc, np.zeros(n)]),
+                                        np.concatenate([np.arange(n), np.zeros(n)]))
+            tail = popped[n - m:][::-1]
+            assert cnt[r] == m
+            assert ids[r, :m].tolist() == [i for _, i in tail]
+            assert dist[r, :m].tolist() == [x for x, _ in tail]
+        assert len(np.unique(adc)) < 10
+
+
+@pytest.mark.parametrize("mt", [1, 2, 3, 4])
+def test_pqindex_train_init_and_assign(orc, mt):
+    rng = np.random.default_rng(50 + mt)
+    x = rng.normal(1, 2, (300, 12)).astype(np.float32)
+    n_sub, K = 3, 8
+    # no epochs: the codebook is the initial +-1 pattern around the column
+    # means (kmeans.rs:50-86, j indexes the sub-space's own dims)
+    cb0, codes0 = orc.pqindex_train(x, n_sub, K, mt, 0, 1)
+    mean = np.zeros(12, np.float32)
+    for i in range(300):
+        mean = (mean + x[i]).astype(np.float32)
+    mean = (mean / np.float32(300)).astype(np.float32)
+    for s, (b, e) in enumerate(orc.pq_ranges(12, n_sub)):
+        for c in range(K):
+            off = np.array([1.0 if (c & (1 << j)) == 1 else -1.0 for j in range(e - b)],
+                           np.float32)
+            assert np.array_equal(cb0[s, c, :e - b], (mean[b:e] + off).astype(np.float32))
+    # after training, the codes are the strict-'<' argmin under the build
+    # metric against the final codebook (pq_idx.rs:143-144)
+    cb, codes = orc.pqindex_train(x, n_sub, K, mt, 4, 9)
+    for s, (b, e) in enumerate(orc.pq_ranges(12, n_sub)):
+        for i in range(0, 300, 7):
+            ds = [orc.metric(x[i, b:e], cb[s, c, :e - b], mt) for c in range(K)]
+            best = 0
+            for c in range(1, K):
+                if ds[c] < ds[best]:
+                    best = c
+            assert codes[i, s] == best
+    with pytest.raises(orc.OracleError):
+        orc.pqindex_train(x[:0], n_sub, K, mt, 3, 1)

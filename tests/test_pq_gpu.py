@@ -1,0 +1,140 @@
+"""Standalone PQIndex (pq_idx.rs:21-265) on the GPU vs the oracle.
+
+Training: with the same seed the device k-means (build metric, f32 sums in
+item order) gives the oracle's codebook and codes bit for bit.  Search: on
+the exported codebook and codes the oracle's LUT / ADC / push-all, pop-to-k
+heap gives the same ids, distances and counts, ties included."""
+import numpy as np
+import pytest
+
+from hora_amd import HoraError, Metric, PQIndex, PQParams
+from hora_amd.datasets import bf_normal, bf_sparse_u8, gaussian_mixture
+
+pytestmark = pytest.mark.gpu
+
+
+def _build(x, mt, n_sub, bits, epochs, seed=7):
+    idx = PQIndex(x.shape[1], PQParams().n_sub(n_sub).sub_bits(bits).train_epoch(epochs).seed(seed))
+    idx.add_batch(x)
+    idx.build(mt)
+    return idx
+
+
+def _search_check(orc, idx, q, k, mt):
+    ex = idx.export()
+    gi, gd, gc = idx.search_batch(q, k)
+    ri, rd, rc = orc.pqindex_search(q, int(mt), ex["codebook"], ex["codes"], k)
+    assert np.array_equal(gc, rc)
+    for r in range(q.shape[0]):
+        m = int(rc[r])
+        assert gi[r, :m].tolist() == ri[r, :m].tolist(), f"row {r} ids"
+        assert np.array_equal(gd[r, :m], rd[r, :m]), f"row {r} distances"
+        assert (gi[r, m:] == np.iinfo(np.uint64).max).all()
+    return gi, gd
+
+
+@pytest.mark.parametrize("mt", [Metric.Euclidean, Metric.Manhattan, Metric.DotProduct,
+                                Metric.CosineSimilarity, Metric.Angular],
+                         ids=lambda m: m.name)
+def test_training_is_bit_exact(orc, mt):
+    x = bf_normal(3000, 24, seed=61, sigma=2.0) + np.float32(0.5)
+    idx = _build(x, mt, 4, 4, 6, seed=11)
+    ex = idx.export()
+    cb, codes = orc.pqindex_train(x, 4, 16, int(mt), 6, 11)
+    assert np.array_equal(ex["codebook"], cb)
+    assert np.array_equal(ex["codes"], codes.astype(np.uint8))
+    info = idx.info()
+    assert (info["n_sub"], info["n_sub_center"], info["sub_dim_max"], info["n_trained"]) == \
+        (4, 16, 6, 3000)
+
+
+@pytest.mark.parametrize("mt", [Metric.Euclidean, Metric.Manhattan, Metric.DotProduct,
+                                Metric.CosineSimilarity, Metric.Angular],
+                         ids=lambda m: m.name)
+def test_search_default_params_ties(orc, mt):
+    # PQParams::default(): 4 sub-spaces x 16 codewords -- most heads are tied
+    x = bf_normal(4000, 32, seed=62, sigma=1.0) + np.float32(0.25)
+    q = bf_normal(64, 32, seed=63, sigma=1.0) + np.float32(0.25)
+    idx = _build(x, mt, 4, 4, 5)
+    for k in (1, 10, 100):
+        _search_check(orc, idx, q, k, mt)
+    assert idx.search_stats()[0] > 0  # rows went through the heap replay
+
+
+def test_search_tie_free_256_codewords(orc):
+    x, _, _ = gaussian_mixture(6000, 64, 64, seed=64)
+    q, _, _ = gaussian_mixture(100, 64, 64, seed=65)
+    idx = _build(x, Metric.Euclidean, 8, 8, 4)
+    _search_check(orc, idx, q, 10, Metric.Euclidean)
+
+
+def test_long_rows_hybrid_heap(orc):
+    # 40,000 items with 2 x 4 codewords: every row is tied and the replayed
+    # heap (40,000 entries) spills past the LDS part into global memory
+    x = bf_normal(40_000, 8, seed=66, sigma=1.0)
+    q = bf_normal(12, 8, seed=67, sigma=1.0)
+    idx = _build(x, Metric.Euclidean, 2, 2, 3)
+    for k in (5, 300):
+        _search_check(orc, idx, q, k, Metric.Euclidean)
+
+
+def test_wide_subspaces_and_large_codebooks(orc):
+    # sub-spaces of 100 dims (no register path), 256 codewords (codebook not
+    # staged in LDS), and 256 sub-spaces of 1 dim (LUTs in global memory)
+    x = bf_sparse_u8(700, 200, seed=68)
+    q = bf_sparse_u8(9, 200, seed=69)
+    idx = _build(x, Metric.Euclidean, 2, 8, 2)
+    cb, codes = orc.pqindex_train(x, 2, 256, int(Metric.Euclidean), 2, 7)
+    ex = idx.export()
+    assert np.array_equal(ex["codebook"], cb) and np.array_equal(ex["codes"], codes)
+    _search_check(orc, idx, q, 7, Metric.Euclidean)
+    x2 = bf_normal(600, 256, seed=70, sigma=1.0)
+    idx2 = _build(x2, Metric.Euclidean, 256, 8, 1)
+    cb2, codes2 = orc.pqindex_train(x2, 256, 256, int(Metric.Euclidean), 1, 7)
+    ex2 = idx2.export()
+    assert np.array_equal(ex2["codebook"], cb2) and np.array_equal(ex2["codes"], codes2)
+    _search_check(orc, idx2, bf_normal(5, 256, seed=71, sigma=1.0), 4, Metric.Euclidean)
+
+
+def test_dropped_trailing_dimensions(orc):
+    # d=10, n_sub=4: sub-spaces [0,2),[2,4),[4,6),[6,8), dims 8-9 ignored
+    # (pq_idx.rs:75-88)
+    x = bf_normal(500, 10, seed=72, sigma=1.0)
+    idx = _build(x, Metric.Euclidean, 4, 3, 3)
+    assert idx.info()["sub_dim_max"] == 2
+    q = bf_normal(6, 10, seed=73, sigma=1.0)
+    q2 = q.copy()
+    q2[:, 8:] += 100.0
+    a = idx.search_batch(q, 5)
+    b = idx.search_batch(q2, 5)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    _search_check(orc, idx, q, 5, Metric.Euclidean)
+
+
+def test_edges_and_errors(orc):
+    x = bf_normal(50, 8, seed=74, sigma=1.0)
+    idx = PQIndex(8, PQParams().n_sub(2).sub_bits(2).train_epoch(2))
+    assert idx.name() == "PQIndex" and idx.built()
+    with pytest.raises(HoraError) as e:  # search before build
+        idx.search_batch(x[:1], 3)
+    assert e.value.status == 5
+    with pytest.raises(HoraError) as e:  # empty batch: kmeans.rs:158-160 unwrapped
+        idx.build(Metric.Euclidean)
+    assert e.value.status == 9
+    with pytest.raises(HoraError) as e:
+        idx.add_batch(np.zeros((2, 7), np.float32))
+    assert e.value.status == 2
+    idx.add_batch(x)
+    with pytest.raises(HoraError) as e:
+        idx.build(Metric.Unknown)
+    assert e.value.status == 4
+    idx.build(Metric.Euclidean)
+    _search_check(orc, idx, x[:4], 80, Metric.Euclidean)  # k > n
+    gi, gd, gc = idx.search_batch(x[:3], 0)
+    assert gc.tolist() == [0, 0, 0]
+    one = _build(x[:1], Metric.Euclidean, 2, 2, 2)
+    _search_check(orc, one, x[:3], 4, Metric.Euclidean)
+    cos = _build(np.abs(x) + 1, Metric.CosineSimilarity, 2, 2, 2)
+    with pytest.raises(HoraError) as e:  # zero query: NaN LUT -> NaN distances
+        cos.search_batch(np.zeros((1, 8), np.float32), 3)
+    assert e.value.status == 8
