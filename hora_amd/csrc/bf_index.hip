@@ -99,6 +99,18 @@ class BruteForce final : public Index {
         samp_ns_ = 0;
     }
 
+    int32_t metric() const override { return metric_; }
+
+    void copy_rows(uint64_t first, uint64_t count, float *out) const override
+    {
+        if (first + count > n_) fail(HM_ERR_INVALID_ARG, "row range out of bounds");
+        if (count == 0) return;
+        HM_HIP(hipMemcpy2DAsync(out, dim_ * sizeof(float), base_.as<float>() + first * dpad_,
+                                dpad_ * sizeof(float), dim_ * sizeof(float), count,
+                                hipMemcpyDeviceToHost, stream_));
+        HM_HIP(hipStreamSynchronize(stream_));
+    }
+
     void build(int32_t metric) override
     {
         if (metric < HM_METRIC_UNKNOWN || metric > HM_METRIC_ANGULAR)

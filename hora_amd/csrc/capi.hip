@@ -279,6 +279,48 @@ HM_API int32_t hm_index_search_stats(const hm_index *h, uint64_t *fallback,
     });
 }
 
+HM_API int32_t hm_index_dump(hm_index *h, const char *path, const uint8_t *id_bytes,
+                             const uint64_t *id_offsets)
+{
+    return guard([&] {
+        need(h, "index");
+        need(path, "path");
+        if (id_bytes) need(id_offsets, "id_offsets");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        dump_index(h->impl.get(), path, id_bytes, id_offsets);
+    });
+}
+
+HM_API int32_t hm_index_load(int32_t kind, const char *path, int32_t id_width, hm_index **out)
+{
+    return guard([&] {
+        need(out, "out");
+        need(path, "path");
+        *out = nullptr;
+        int dev = 0;
+        HM_HIP(hipGetDevice(&dev));
+        auto h = std::make_unique<hm_index>();
+        h->device = dev;
+        h->impl = load_index(kind, path, id_width, h->loaded_ids, h->loaded_id_off);
+        *out = h.release();
+    });
+}
+
+HM_API int32_t hm_index_loaded_ids(const hm_index *h, uint8_t *id_bytes, uint64_t *id_offsets,
+                                   uint64_t *total_bytes)
+{
+    return guard([&] {
+        need(h, "index");
+        need(total_bytes, "total_bytes");
+        *total_bytes = h->loaded_ids.size();
+        if (id_bytes && !h->loaded_ids.empty())
+            std::memcpy(id_bytes, h->loaded_ids.data(), h->loaded_ids.size());
+        if (id_offsets)
+            std::memcpy(id_offsets, h->loaded_id_off.data(), h->loaded_id_off.size() * 8);
+    });
+}
+
 HM_API int32_t hm_index_nodes_size(const hm_index *h, uint64_t *out)
 {
     return guard([&] {

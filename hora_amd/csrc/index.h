@@ -5,6 +5,7 @@
 
 #include <memory>
 #include <mutex>
+#include <vector>
 
 namespace hm {
 
@@ -25,6 +26,10 @@ class Index {
                                float *d_dist, uint32_t *d_count, hipStream_t s) = 0;
     virtual const char *name() const = 0;
     virtual uint64_t nodes_size() const = 0;
+    // The metric of the last build (HM_METRIC_UNKNOWN before one).
+    virtual int32_t metric() const = 0;
+    // Rows [first, first + count) into out (count x dimension floats).
+    virtual void copy_rows(uint64_t first, uint64_t count, float *out) const = 0;
     virtual void clear() = 0;
     virtual void get_rows(const uint64_t *idx, uint64_t n, float *out) const = 0;
     // Ragged batches: batch b = queries [off[b], off[b+1]) with k = ks[b];
@@ -82,6 +87,17 @@ std::unique_ptr<Index> make_pq(uint64_t dim, const hm_params &p);
 // Standalone PQIndex extras (fail with INVALID_ARG on another index kind).
 int32_t pq_info(Index *ix, uint32_t *out8);
 int32_t pq_export(Index *ix, float *codebook, uint8_t *codes);
+const hm_params &pq_params(Index *ix);
+// Trained state from arrays (deserialization): codebook [M][KS][sdm],
+// codes [n_trained][M] for the first n_trained rows.
+void pq_import(Index *ix, int32_t metric, const float *codebook, const uint8_t *codes,
+               uint64_t n_trained);
+const hm_params &ivfpq_params(Index *ix);
+// Trained state from arrays: centers [nlist][dim], list offsets [nlist+1]
+// and ids (list order), codebooks [nlist][M][KS][sdm], codes [n][M] in list
+// order.
+void ivfpq_import(Index *ix, int32_t metric, const float *centers, const uint64_t *offsets,
+                  const uint64_t *ids, const float *codebooks, const uint8_t *codes);
 
 // IVFPQIndex extras (fail with INVALID_ARG on another index kind).
 int32_t ivfpq_info(Index *ix, uint32_t *out8);
@@ -95,6 +111,15 @@ int32_t ivfpq_train_probe(Index *ix, const float *q, uint64_t n, uint32_t label_
 double ivfpq_mean_nprobe(Index *ix);
 int32_t ivfpq_last_probes(Index *ix, uint64_t nq, uint32_t ldp, uint32_t *probes,
                           uint32_t *counts);
+
+// SerializableIndex (serialize.hip): the reference's bincode layout.
+// id_bytes/id_off: node i's Option<T> encoding is id_bytes[id_off[i] ..
+// id_off[i+1]); null writes Some(i as usize).  Load: id_width > 0 reads a
+// fixed-width T, 0 a u64-length-prefixed T; the encodings come back in
+// id_bytes / id_off (n + 1 offsets).
+void dump_index(Index *ix, const char *path, const uint8_t *id_bytes, const uint64_t *id_off);
+std::unique_ptr<Index> load_index(int32_t kind, const char *path, int32_t id_width,
+                                  std::vector<uint8_t> &id_bytes, std::vector<uint64_t> &id_off);
 
 // Host-side validation shared by the index types (node.rs:158-166).
 void check_finite_host(const float *x, uint64_t count);
@@ -111,4 +136,7 @@ struct hm_index {
     int device = 0;
     // Host-entry staging buffers.
     hm::DeviceBuffer q_stage, idx_stage, dist_stage, count_stage, err_stage;
+    // Option<T> encodings of the nodes read by hm_index_load.
+    std::vector<uint8_t> loaded_ids;
+    std::vector<uint64_t> loaded_id_off;
 };

@@ -114,6 +114,73 @@ class IVFPQ final : public Index {
     }
 
     bool built() const override { return true; }  // pq_idx.rs:454-456
+    int32_t metric() const override { return metric_; }
+    const hm_params &params() const { return p_; }
+
+    void copy_rows(uint64_t first, uint64_t count, float *out) const override
+    {
+        if (first + count > n_) fail(HM_ERR_INVALID_ARG, "row range out of bounds");
+        if (count == 0) return;
+        HM_HIP(hipMemcpy2DAsync(out, dim_ * 4, base_.as<float>() + first * dpad_, dpad_ * 4,
+                                dim_ * 4, count, hipMemcpyDeviceToHost, stream_));
+        HM_HIP(hipStreamSynchronize(stream_));
+    }
+
+    // Trained state from arrays (deserialization); the lists must cover
+    // rows 0..n-1 of this index exactly once.
+    void import_trained(int32_t metric, const float *centers, const uint64_t *offsets,
+                        const uint64_t *ids, const float *codebooks, const uint8_t *codes)
+    {
+        if (metric <= HM_METRIC_UNKNOWN || metric > HM_METRIC_ANGULAR)
+            fail(HM_ERR_UNKNOWN_METRIC, "unknown method");
+        const uint64_t n = offsets[nlist_];
+        if (n != n_) fail(HM_ERR_INVALID_ARG, "inverted lists do not cover the rows");
+        std::vector<uint32_t> off(nlist_ + 1), id(n), asg(n, 0xFFFFFFFFu), lst(n);
+        uint32_t maxl = 0;
+        for (uint32_t l = 0; l < nlist_; ++l) {
+            if (offsets[l + 1] < offsets[l]) fail(HM_ERR_INVALID_ARG, "list offsets decrease");
+            maxl = std::max<uint32_t>(maxl, (uint32_t)(offsets[l + 1] - offsets[l]));
+            for (uint64_t r = offsets[l]; r < offsets[l + 1]; ++r) {
+                if (ids[r] >= n || asg[ids[r]] != 0xFFFFFFFFu)
+                    fail(HM_ERR_INVALID_ARG, "inverted list ids are not a permutation");
+                asg[ids[r]] = l;
+                id[r] = (uint32_t)ids[r];
+                lst[r] = l;
+            }
+        }
+        for (uint32_t l = 0; l <= nlist_; ++l) off[l] = (uint32_t)offsets[l];
+        for (uint64_t i = 0; i < n * M_; ++i)
+            if (codes[i] >= KS_) fail(HM_ERR_INVALID_ARG, "code out of range");
+        hipStream_t s = stream_;
+        centers_.reserve((size_t)nlist_ * dpad_ * 4);
+        cnorm_.reserve((size_t)nlist_ * 4);
+        assign_.reserve(n * 4 + 4);
+        lists_.reserve(n * 4 + 4);
+        ids_.reserve(n * 4 + 4);
+        off_.reserve((nlist_ + 1) * 4);
+        codebooks_.reserve((size_t)nlist_ * M_ * KS_ * sdm_ * 4);
+        codes_.reserve(n * M_ + 4);
+        HM_HIP(hipMemsetAsync(centers_.get(), 0, (size_t)nlist_ * dpad_ * 4, s));
+        HM_HIP(hipMemcpy2DAsync(centers_.get(), dpad_ * 4, centers, dim_ * 4, dim_ * 4, nlist_,
+                                hipMemcpyHostToDevice, s));
+        HM_HIP(hipMemcpyAsync(off_.get(), off.data(), (nlist_ + 1) * 4, hipMemcpyHostToDevice, s));
+        if (n) {
+            HM_HIP(hipMemcpyAsync(ids_.get(), id.data(), n * 4, hipMemcpyHostToDevice, s));
+            HM_HIP(hipMemcpyAsync(assign_.get(), asg.data(), n * 4, hipMemcpyHostToDevice, s));
+            HM_HIP(hipMemcpyAsync(lists_.get(), lst.data(), n * 4, hipMemcpyHostToDevice, s));
+            HM_HIP(hipMemcpyAsync(codes_.get(), codes, n * M_, hipMemcpyHostToDevice, s));
+        }
+        HM_HIP(hipMemcpyAsync(codebooks_.get(), codebooks, (size_t)nlist_ * M_ * KS_ * sdm_ * 4,
+                              hipMemcpyHostToDevice, s));
+        metric_ = metric;
+        if (metric_ == HM_METRIC_COSINE_SIMILARITY || metric_ == HM_METRIC_ANGULAR)
+            launch_self_dot(centers_.as<float>(), dpad_, nlist_, (uint32_t)dim_, cnorm_.as<float>(),
+                            s);
+        HM_HIP(hipStreamSynchronize(s));
+        max_list_ = maxl;
+        trained_ = true;
+        n_trained_ = n;
+    }
     const char *name() const override { return "IVFPQIndex"; }
     uint64_t nodes_size() const override { return n_; }
 
@@ -562,6 +629,21 @@ class IVFPQ final : public Index {
 std::unique_ptr<Index> make_ivfpq(uint64_t dim, const hm_params &p)
 {
     return std::make_unique<IVFPQ>(dim, p);
+}
+
+const hm_params &ivfpq_params(Index *ix)
+{
+    auto *p = dynamic_cast<IVFPQ *>(ix);
+    if (!p) fail(HM_ERR_INVALID_ARG, "not an IVFPQIndex");
+    return p->params();
+}
+
+void ivfpq_import(Index *ix, int32_t metric, const float *centers, const uint64_t *offsets,
+                  const uint64_t *ids, const float *codebooks, const uint8_t *codes)
+{
+    auto *p = dynamic_cast<IVFPQ *>(ix);
+    if (!p) fail(HM_ERR_INVALID_ARG, "not an IVFPQIndex");
+    p->import_trained(metric, centers, offsets, ids, codebooks, codes);
 }
 
 int32_t ivfpq_info(Index *ix, uint32_t *out)

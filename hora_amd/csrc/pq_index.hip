@@ -705,6 +705,40 @@ class PQ final : public Index {
     }
 
     bool built() const override { return true; }  // pq_idx.rs:209-211
+    int32_t metric() const override { return metric_; }
+    const hm_params &params() const { return p_; }
+
+    void copy_rows(uint64_t first, uint64_t count, float *out) const override
+    {
+        if (first + count > n_) fail(HM_ERR_INVALID_ARG, "row range out of bounds");
+        if (count == 0) return;
+        HM_HIP(hipMemcpy2DAsync(out, dim_ * 4, base_.as<float>() + first * dpad_, dpad_ * 4,
+                                dim_ * 4, count, hipMemcpyDeviceToHost, stream_));
+        HM_HIP(hipStreamSynchronize(stream_));
+    }
+
+    void import_trained(int32_t metric, const float *codebook, const uint8_t *codes,
+                        uint64_t n)
+    {
+        if (metric <= HM_METRIC_UNKNOWN || metric > HM_METRIC_ANGULAR)
+            fail(HM_ERR_UNKNOWN_METRIC, "unknown method");
+        if (n > n_) fail(HM_ERR_INVALID_ARG, "more coded items than rows");
+        for (uint64_t i = 0; i < n * M_; ++i)
+            if (codes[i] >= KS_) fail(HM_ERR_INVALID_ARG, "code out of range");
+        hipStream_t s = stream_;
+        cb_.reserve((size_t)M_ * KS_ * sdm_ * 4);
+        cbn_.reserve((size_t)M_ * KS_ * 4);
+        codes_.reserve(n * M_ + 4);
+        HM_HIP(hipMemcpyAsync(cb_.get(), codebook, (size_t)M_ * KS_ * sdm_ * 4,
+                              hipMemcpyHostToDevice, s));
+        if (n) HM_HIP(hipMemcpyAsync(codes_.get(), codes, n * M_, hipMemcpyHostToDevice, s));
+        metric_ = metric;
+        if (metric_ == HM_METRIC_COSINE_SIMILARITY || metric_ == HM_METRIC_ANGULAR)
+            codeword_norms(s);
+        HM_HIP(hipStreamSynchronize(s));
+        trained_ = true;
+        n_trained_ = n;
+    }
     const char *name() const override { return "PQIndex"; }
     uint64_t nodes_size() const override { return n_; }
 
@@ -1030,6 +1064,21 @@ int32_t pq_info(Index *ix, uint32_t *out)
     if (!p) fail(HM_ERR_INVALID_ARG, "not a PQIndex");
     p->info(out);
     return HM_OK;
+}
+
+const hm_params &pq_params(Index *ix)
+{
+    auto *p = dynamic_cast<PQ *>(ix);
+    if (!p) fail(HM_ERR_INVALID_ARG, "not a PQIndex");
+    return p->params();
+}
+
+void pq_import(Index *ix, int32_t metric, const float *codebook, const uint8_t *codes,
+               uint64_t n_trained)
+{
+    auto *p = dynamic_cast<PQ *>(ix);
+    if (!p) fail(HM_ERR_INVALID_ARG, "not a PQIndex");
+    p->import_trained(metric, codebook, codes, n_trained);
 }
 
 int32_t pq_export(Index *ix, float *codebook, uint8_t *codes)

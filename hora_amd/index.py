@@ -150,6 +150,50 @@ class PQParams:
         return self
 
 
+# bincode encodings of the reference's IdxType implementations (node.rs:76-87):
+# (width in bytes, signed); String is u64-length-prefixed UTF-8 (width 0).
+_ID_TYPES = {"usize": (8, False), "u64": (8, False), "i64": (8, True), "u32": (4, False),
+             "i32": (4, True), "u16": (2, False), "i16": (2, True), "u128": (16, False),
+             "i128": (16, True), "String": (0, False)}
+
+
+def _id_codec(idx_type: str):
+    if idx_type not in _ID_TYPES:
+        raise HoraError(1, f"unsupported id type {idx_type!r}")
+    return _ID_TYPES[idx_type]
+
+
+def _encode_ids(ids, idx_type: str):
+    """Option<T> bincode bytes of each id (None -> tag 0) and n+1 offsets."""
+    width, signed = _id_codec(idx_type)
+    parts = []
+    for i in ids:
+        if i is None:
+            parts.append(b"\x00")
+        elif width:
+            parts.append(b"\x01" + int(i).to_bytes(width, "little", signed=signed))
+        else:
+            b = str(i).encode()
+            parts.append(b"\x01" + len(b).to_bytes(8, "little") + b)
+    off = np.zeros(len(parts) + 1, np.uint64)
+    off[1:] = np.cumsum([len(x) for x in parts]) if parts else []
+    return b"".join(parts), off
+
+
+def _decode_ids(blob: bytes, off, idx_type: str):
+    width, signed = _id_codec(idx_type)
+    out = []
+    for j in range(len(off) - 1):
+        e = blob[int(off[j]):int(off[j + 1])]
+        if e[0] == 0:
+            out.append(None)
+        elif width:
+            out.append(int.from_bytes(e[1:], "little", signed=signed))
+        else:
+            out.append(e[9:].decode())
+    return out
+
+
 def _f32_rows(a, dim: int) -> np.ndarray:
     a = np.ascontiguousarray(a, dtype=np.float32)
     if a.ndim == 1:
@@ -298,6 +342,36 @@ class _Index:
         out = C.c_uint64()
         check(lib().hm_index_dimension(self._h, C.byref(out)))
         return int(out.value)
+
+    # ---- SerializableIndex (ann_index.rs:141-158)
+    def dump(self, path: str, idx_type: str = "usize") -> None:
+        """Write the index in the reference's bincode layout; ids are
+        encoded as Option<idx_type>."""
+        blob, off = _encode_ids(self._ids, idx_type)
+        buf = np.frombuffer(blob, np.uint8) if blob else np.zeros(1, np.uint8)
+        check(lib().hm_index_dump(self._h, str(path).encode(), buf.ctypes.data,
+                                  off.ctypes.data))
+
+    @classmethod
+    def load(cls, path: str, idx_type: str = "usize"):
+        """Read an index written by dump (or by the reference's dump)."""
+        width, _ = _id_codec(idx_type)
+        h = C.c_void_p()
+        check(lib().hm_index_load(cls._kind, str(path).encode(), width, C.byref(h)))
+        obj = cls.__new__(cls)
+        obj._h = h
+        d = C.c_uint64()
+        check(lib().hm_index_dimension(h, C.byref(d)))
+        obj._dim = int(d.value)
+        n = C.c_uint64()
+        check(lib().hm_index_nodes_size(h, C.byref(n)))
+        total = C.c_uint64()
+        check(lib().hm_index_loaded_ids(h, None, None, C.byref(total)))
+        blob = np.zeros(max(int(total.value), 1), np.uint8)
+        off = np.zeros(int(n.value) + 1, np.uint64)
+        check(lib().hm_index_loaded_ids(h, blob.ctypes.data, off.ctypes.data, C.byref(total)))
+        obj._ids = _decode_ids(blob.tobytes(), off, idx_type)
+        return obj
 
     # ---- device-resident search (inputs/outputs already in HBM)
     def search_device(self, q_ptr: int, nq: int, k: int, idx_ptr: int, dist_ptr: int,

@@ -162,6 +162,29 @@ HM_API int32_t hm_index_set_search_mode(hm_index *h, int32_t mode, uint32_t shor
 HM_API int32_t hm_index_search_stats(const hm_index *h, uint64_t *fallback,
                                      uint64_t *uncertified);
 
+/* SerializableIndex::dump (ann_index.rs:152-157; bruteforce_idx.rs:98-105,
+ * pq_idx.rs:253-264, 507-522): writes the index to `path` in the reference's
+ * on-disk layout (serde derive over the index struct, bincode 1.x default
+ * encoding: little-endian, usize and lengths as u64, enum variants as u32,
+ * bool and Option tags as one byte).  Node i's `idx: Option<T>` is written
+ * from the caller's bincode bytes id_bytes[id_offsets[i] .. id_offsets[i+1])
+ * (n + 1 offsets); id_bytes NULL writes Some(i as usize). */
+HM_API int32_t hm_index_dump(hm_index *h, const char *path, const uint8_t *id_bytes,
+                             const uint64_t *id_offsets);
+
+/* SerializableIndex::load (ann_index.rs:145-150; bruteforce_idx.rs:87-96,
+ * pq_idx.rs:243-251, 490-505) for an index of `kind`: rows, metric and
+ * trained state come back without retraining.  id_width: the byte width of
+ * a fixed-width id type T (8 for usize), or 0 for a u64-length-prefixed T
+ * (String).  The ids are kept for hm_index_loaded_ids. */
+HM_API int32_t hm_index_load(int32_t kind, const char *path, int32_t id_width, hm_index **out);
+
+/* The Option<T> encodings of the nodes read by hm_index_load, node after
+ * node: *total_bytes is always set; id_bytes (total_bytes) and id_offsets
+ * (n + 1) are filled when not NULL. */
+HM_API int32_t hm_index_loaded_ids(const hm_index *h, uint8_t *id_bytes, uint64_t *id_offsets,
+                                   uint64_t *total_bytes);
+
 /* ANNIndex::nodes_size (ann_index.rs:105) -- number of rows added. */
 HM_API int32_t hm_index_nodes_size(const hm_index *h, uint64_t *out);
 

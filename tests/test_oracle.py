@@ -362,3 +362,40 @@ def test_pqindex_train_init_and_assign(orc, mt):
             assert codes[i, s] == best
     with pytest.raises(orc.OracleError):
         orc.pqindex_train(x[:0], n_sub, K, mt, 3, 1)
+
+
+# ------------------------------------------------------ on-disk layout
+def test_bincode_layout_hand_assembled():
+    """The oracle's restatement of the reference's file layout against bytes
+    assembled by hand from bincode 1.x's rules (little-endian, u64 lengths,
+    u32 enum variants, one-byte Option / bool tags) and the struct field
+    orders of bruteforce_idx.rs:15-22 and pq_idx.rs:21-45."""
+    import struct
+
+    from oracle import bincode as bc
+
+    q, i, f = (lambda v: struct.pack("<Q", v)), (lambda v: struct.pack("<I", v)), \
+        (lambda v: struct.pack("<f", v))
+    want = (q(2)                                   # tmp_nodes: 2 nodes
+            + q(2) + f(1.0) + f(2.0) + b"\x01" + q(7)  # Node{[1,2], Some(7)}
+            + q(2) + f(3.0) + f(-0.5) + b"\x00"        # Node{[3,-0.5], None}
+            + i(3)                                 # mt: Euclidean
+            + q(2))                                # dimension
+    got = bc.bruteforce([[1.0, 2.0], [3.0, -0.5]], [bc.option_usize(7), bc.option_usize(None)],
+                        3, 2)
+    assert got == want
+    assert want.hex().startswith("0200000000000000" "0200000000000000" "0000803f00000040" "01")
+    node = q(2) + f(1.0) + f(2.0) + b"\x01" + q(0)
+    want_pq = (q(2) + q(1) + q(2)                  # _dimension, _n_sub, _sub_dimension
+               + q(1) + q(2) + q(0) + q(2)         # _dimension_range [[0, 2]]
+               + q(1) + q(1) + q(2) + q(1) + q(5)  # _sub_bits .. _train_epoch
+               + q(1) + q(2) + q(2) + f(0.5) + f(1.5) + q(2) + f(2.0) + f(3.0)  # _centers
+               + b"\x01" + b"\x00" + q(0)          # _is_trained, _has_residual, _residual
+               + q(1) + q(100000)                  # _n_items, _max_item
+               + q(1) + node                       # _nodes
+               + q(1) + q(1) + q(1)                # _assigned_center [[1]]
+               + i(2)                              # mt: DotProduct
+               + q(1) + node)                      # _nodes_tmp
+    got_pq = bc.pqindex(2, 1, 1, 5, [[[0.5, 1.5], [2.0, 3.0]]], np.array([[1]]), [[1.0, 2.0]],
+                        [bc.option_usize(0)], 2)
+    assert got_pq == want_pq
