@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="bf",
-                    choices=["bf", "ivfpq", "ragged", "probe", "copy", "pq"])
+                    choices=["bf", "ivfpq", "ragged", "probe", "copy", "pq", "knn_graph"])
     ap.add_argument("--forms", default="exact,bf16,fp8")
     ap.add_argument("--cpu-sample", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -427,6 +427,78 @@ def main_pq(args):
                       "roofline": roofline, "cpu_baseline": cpu}), flush=True)
 
 
+def main_knn_graph(args):
+    """naive_build_knn_graph (knn.rs:14-41) over the headline base: the
+    60,000 x 784 integer-valued rows, Euclidean, k=10, every row a query
+    against all others (exact FP32, bit-exact to the reference)."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from hora_amd import BruteForceIndex, Metric
+    from hora_amd.datasets import bf_sparse_u8
+
+    N, D, K = 60_000, 784, 10
+    x = bf_sparse_u8(N, D, seed=1)
+    ix = BruteForceIndex(D)
+    ix.add_batch(x)
+    ix.build(Metric.Euclidean)
+    for _ in range(args.warmup):
+        ix.build_knn_graph(K)
+    torch.cuda.synchronize()
+    from hora_amd._lib import profile_enable, profile_reset
+    profile_reset()
+    profile_enable(True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        gi, gd, gc = ix.build_knn_graph(K)
+        log(f"knn_graph: step {i + 1}/{args.steps} done")
+    wall = (time.perf_counter() - t0) / args.steps
+    profile_enable(False)
+    km = kernel_ms(["exact_tile", "row_select", "graph_finalize", "bf_finalize"])
+    tied = int(ix.search_stats()[0])
+    # parity on a sample of rows: the oracle's graph over the same rows is
+    # the full O(n^2) loop, so check the sampled rows with bf_search(k+1)
+    # minus the row itself on tie-free rows, and count agreement
+    samp = list(range(0, N, N // 64))
+    ri, rd, _ = oracle.bf_search(x, x[samp], K + 1, oracle.EUCLIDEAN, nthreads=16)
+    agree = 0
+    for j, r in enumerate(samp):
+        row = [(int(i), float(d)) for i, d in zip(ri[j], rd[j]) if int(i) != r][:K]
+        agree += [i for i, _ in row][::-1] == gi[r].tolist()
+    et_ms, et_n = km["exact_tile"]
+    ops = 3.0 * N * N * D  # sub, mul, add per pair-dimension
+    roofline = None
+    if et_n:
+        ach = ops / (et_ms / args.steps / 1e3) / 1e12
+        roofline = {"bound": "valu", "kernel": "exact_tile_kernel", "achieved": round(ach, 1),
+                    "peak": FP32_PEAK, "unit": "T op/s", "frac": round(ach / FP32_PEAK, 4),
+                    "traffic": None, "avg_launch_ms": round(et_ms / et_n, 3)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        ns = 3000
+        t0 = time.perf_counter()
+        oracle.knn_graph(x[:ns], K, oracle.EUCLIDEAN, nthreads=16)
+        t = time.perf_counter() - t0
+        pairs_per_s = ns * (ns - 1) / t
+        cpu = {"value": round(pairs_per_s / (N - 1), 1), "unit": "graph rows/s", "cores": 16,
+               "kind": "port",
+               "sample": f"graph of the first {ns} rows on 16 threads ({pairs_per_s:.3g} "
+                         f"pairs/s), scaled to rows of {N - 1} pairs"}
+    print(json.dumps({"metric": "graph rows/s, 1 MI355X, naive_build_knn_graph",
+                      "value": round(N / wall, 1), "unit": "graph rows/s", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                      "data": "synthetic: integer values 0-255 as f32, half zeros",
+                      "config": {"workload": "kNN graph, 60,000x784, Euclidean, k=10"},
+                      "tie_replayed_rows": tied, "sample_rows_matching_oracle":
+                          f"{agree}/{len(samp)} (rows with ties may differ from a sort)",
+                      "kernel_ms_per_step": {k: round(v[0] / args.steps, 3)
+                                             for k, v in km.items() if v[1]},
+                      "roofline": roofline, "cpu_baseline": cpu}), flush=True)
+
+
 def main_ragged(args):
     import numpy as np
     import torch
@@ -578,7 +650,7 @@ def main():
     import torch  # noqa: F401  (load torch's HIP runtime before the engine's library)
 
     {"bf": main_bf, "ivfpq": main_ivfpq, "ragged": main_ragged, "probe": main_probe,
-     "copy": main_copy, "pq": main_pq}[args.workload](args)
+     "copy": main_copy, "pq": main_pq, "knn_graph": main_knn_graph}[args.workload](args)
 
 
 if __name__ == "__main__":

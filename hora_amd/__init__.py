@@ -18,4 +18,5 @@ from .index import (  # noqa: F401
     PQIndex,
     PQIVFIndex,
     PQParams,
+    naive_build_knn_graph,
 )

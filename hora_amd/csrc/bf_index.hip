@@ -267,6 +267,91 @@ class BruteForce final : public Index {
         }
     }
 
+    // naive_build_knn_graph (knn.rs:14-41): exact distances of a chunk of
+    // rows against all rows, the row's own column excluded.
+    void knn_graph(uint32_t k, uint64_t *h_idx, float *h_dist, uint32_t *h_count) override
+    {
+        if (metric_ == HM_METRIC_UNKNOWN) fail(HM_ERR_UNKNOWN_METRIC, "unknown method");
+        if (!h_count || (k && (!h_idx || !h_dist))) fail(HM_ERR_INVALID_ARG, "null pointer");
+        const uint32_t n = (uint32_t)n_;
+        if (n == 0) return;
+        if (k == 0 || n == 1) {
+            for (uint32_t i = 0; i < n; ++i) h_count[i] = 0;
+            for (uint64_t j = 0; j < (uint64_t)n * k; ++j) {
+                h_idx[j] = ~0ull;
+                h_dist[j] = __builtin_inff();
+            }
+            return;
+        }
+        if (k + 2 > SELECT_MAX_K) fail(HM_ERR_INVALID_ARG, "k too large (max 8190)");
+        hipStream_t s = stream_;
+        const uint32_t Ksel = std::min<uint32_t>(k + 2, n);
+        if (need_norm()) ensure_bnorm(s);
+        const uint64_t ldd = round_up(n, 4), ldc = round_up(n - 1, 4);
+        uint64_t qc = std::max<uint64_t>(1, d_budget_ / (ldd * sizeof(float) * 2));
+        qc = std::min<uint64_t>(std::min<uint64_t>(qc, n), 65535);
+        DeviceBuffer gi, gd, gc, tied, rl, tf, C, ti, td, tc;
+        gi.reserve((size_t)n * k * 8);
+        gd.reserve((size_t)n * k * 4);
+        gc.reserve((size_t)n * 4);
+        tied.reserve(qc * 4);
+        D_.reserve(qc * ldd * sizeof(float));
+        cand_d_.reserve(qc * Ksel * sizeof(float));
+        cand_i_.reserve(qc * Ksel * sizeof(uint32_t));
+        flags_.reserve(qc * sizeof(uint32_t));
+        err_.reserve(4);
+        HM_HIP(hipMemsetAsync(err_.get(), 0, 4, s));
+        std::vector<uint32_t> th(qc);
+        uint64_t replayed = 0;
+        for (uint32_t r0 = 0; r0 < n; r0 += (uint32_t)qc) {
+            const uint32_t rows = (uint32_t)std::min<uint64_t>(qc, n - r0);
+            launch_exact_tile(metric_, base_.as<float>() + (size_t)r0 * dpad_, dpad_, rows,
+                              base_.as<float>(), dpad_, n, dpad_,
+                              need_norm() ? bnorm_.as<float>() + r0 : nullptr, bnorm_.as<float>(),
+                              D_.as<float>(), ldd, s);
+            launch_graph_diag(D_.as<float>(), ldd, rows, r0, s);
+            launch_row_select(D_.as<float>(), ldd, rows, n, Ksel, cand_d_.as<float>(),
+                              cand_i_.as<uint32_t>(), Ksel, flags_.as<uint32_t>(), s);
+            launch_graph_finalize(cand_d_.as<float>(), cand_i_.as<uint32_t>(), Ksel, Ksel,
+                                  flags_.as<uint32_t>(), rows, r0, n, k, gi.as<uint64_t>(),
+                                  gd.as<float>(), gc.as<uint32_t>(), tied.as<uint32_t>(),
+                                  err_.as<uint32_t>(), s);
+            HM_HIP(hipMemcpyAsync(th.data(), tied.get(), rows * 4, hipMemcpyDeviceToHost, s));
+            HM_HIP(hipStreamSynchronize(s));
+            std::vector<uint32_t> list;
+            for (uint32_t r = 0; r < rows; ++r)
+                if (th[r]) list.push_back(r);
+            if (list.empty()) continue;
+            const uint32_t nt = (uint32_t)list.size();
+            replayed += nt;
+            rl.reserve(nt * 4);
+            tf.reserve(nt * 4);
+            C.reserve((size_t)nt * ldc * 4);
+            ti.reserve((size_t)nt * k * 8);
+            td.reserve((size_t)nt * k * 4);
+            tc.reserve((size_t)nt * 4);
+            HM_HIP(hipMemcpyAsync(rl.get(), list.data(), nt * 4, hipMemcpyHostToDevice, s));
+            launch_graph_compact(D_.as<float>(), ldd, rl.as<uint32_t>(), nt, r0, n, C.as<float>(),
+                                 ldc, s);
+            launch_fill_u32(tf.as<uint32_t>(), nt, ROW_TIE, s);
+            launch_bf_finalize(cand_d_.as<float>(), cand_i_.as<uint32_t>(), Ksel, tf.as<uint32_t>(),
+                               C.as<float>(), ldc, nt, n - 1, k, ti.as<uint64_t>(), td.as<float>(),
+                               tc.as<uint32_t>(), err_.as<uint32_t>(), s);
+            launch_graph_remap(ti.as<uint64_t>(), td.as<float>(), tc.as<uint32_t>(),
+                               rl.as<uint32_t>(), nt, r0, k, gi.as<uint64_t>(), gd.as<float>(), s);
+            HM_HIP(hipStreamSynchronize(s));
+        }
+        last_fallback_ = replayed;
+        last_uncertified_ = 0;
+        uint32_t err = 0;
+        HM_HIP(hipMemcpyAsync(&err, err_.get(), 4, hipMemcpyDeviceToHost, s));
+        HM_HIP(hipMemcpyAsync(h_idx, gi.get(), (size_t)n * k * 8, hipMemcpyDeviceToHost, s));
+        HM_HIP(hipMemcpyAsync(h_dist, gd.get(), (size_t)n * k * 4, hipMemcpyDeviceToHost, s));
+        HM_HIP(hipMemcpyAsync(h_count, gc.get(), (size_t)n * 4, hipMemcpyDeviceToHost, s));
+        HM_HIP(hipStreamSynchronize(s));
+        if (err & 1u) fail(HM_ERR_NAN_DISTANCE, "NaN distance (partial_cmp of NaN)");
+    }
+
     static constexpr uint32_t RAG_CHUNK = 4096;
 
     void capture_ragged_graph(uint32_t QC, uint32_t kmax, uint32_t S)

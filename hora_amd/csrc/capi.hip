@@ -321,6 +321,17 @@ HM_API int32_t hm_index_loaded_ids(const hm_index *h, uint8_t *id_bytes, uint64_
     });
 }
 
+HM_API int32_t hm_index_knn_graph(hm_index *h, uint32_t k, uint64_t *idx, float *dist,
+                                  uint32_t *count)
+{
+    return guard([&] {
+        need(h, "index");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        h->impl->knn_graph(k, idx, dist, count);
+    });
+}
+
 HM_API int32_t hm_index_nodes_size(const hm_index *h, uint64_t *out)
 {
     return guard([&] {

@@ -42,6 +42,13 @@ class Index {
             (void)s;
         fail(HM_ERR_NOT_IMPLEMENTED, "ragged search not supported by this index");
     }
+    // naive_build_knn_graph (knn.rs:14-41) over the indexed rows: row i of
+    // [n][k] holds its k nearest other rows, largest distance first.
+    virtual void knn_graph(uint32_t k, uint64_t *h_idx, float *h_dist, uint32_t *h_count)
+    {
+        (void)k, (void)h_idx, (void)h_dist, (void)h_count;
+        fail(HM_ERR_NOT_IMPLEMENTED, "kNN graph build not supported by this index");
+    }
     // Chooses tuning parameters on a query sample; returns the shortlist.
     virtual uint32_t autotune(const float *h_queries, uint64_t nq, uint32_t k)
     {

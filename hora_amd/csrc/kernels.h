@@ -125,4 +125,22 @@ void launch_scatter_results(const uint64_t *si, const float *sd, const uint32_t 
                             const uint32_t *rows, uint32_t nrows, uint32_t k, uint64_t *di,
                             float *dd, uint32_t *dc, hipStream_t s);
 
+// ---- knn_graph.hip (naive_build_knn_graph, knn.rs:14-41)
+// D[r][r0 + r] = +inf for the rows of a chunk (the node's own column).
+void launch_graph_diag(float *D, uint64_t ldd, uint32_t rows, uint32_t r0, hipStream_t s);
+// Graph rows r0.. from the Ksel smallest candidates: rows whose k+1
+// nearest others are distinct get those others largest first; the others
+// get tied[r] = 1.  NaN rows set *err bit 0.
+void launch_graph_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ldc,
+                           uint32_t Ksel, const uint32_t *flags, uint32_t rows, uint32_t r0,
+                           uint32_t n, uint32_t k, uint64_t *out_idx, float *out_dist,
+                           uint32_t *out_count, uint32_t *tied, uint32_t *err, hipStream_t s);
+// C[t] = row rows[t] of D without the node's own column (n - 1 values).
+void launch_graph_compact(const float *D, uint64_t ldd, const uint32_t *rows, uint32_t nt,
+                          uint32_t r0, uint32_t n, float *C, uint64_t ldc, hipStream_t s);
+// Replay output (ascending, compacted columns) -> graph rows, largest first.
+void launch_graph_remap(const uint64_t *ti, const float *td, const uint32_t *tc,
+                        const uint32_t *rows, uint32_t nt, uint32_t r0, uint32_t k,
+                        uint64_t *out_idx, float *out_dist, hipStream_t s);
+
 }  // namespace hm

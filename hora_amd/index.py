@@ -400,6 +400,18 @@ class BruteForceIndex(_Index):
                                                   ks.ctypes.data, idx_ptr, dist_ptr, count_ptr,
                                                   stream or None))
 
+    def build_knn_graph(self, k: int):
+        """naive_build_knn_graph (knn.rs:14-41) over the indexed rows with the
+        build metric: (idx u64 [n,k], dist f32 [n,k], count u32 [n]); row i
+        holds its k nearest other rows, largest distance first."""
+        n = self.nodes_size()
+        idx = np.empty((max(n, 1), max(k, 1)), np.uint64)
+        dist = np.empty((max(n, 1), max(k, 1)), np.float32)
+        cnt = np.empty(max(n, 1), np.uint32)
+        check(lib().hm_index_knn_graph(self._h, k, idx.ctypes.data, dist.ctypes.data,
+                                       cnt.ctypes.data))
+        return idx[:n, :k], dist[:n, :k], cnt[:n]
+
     def autotune(self, queries, k: int) -> int:
         q = _f32_rows(queries, self._dim)
         out = C.c_uint32()
@@ -498,6 +510,19 @@ class PQIndex(_Index):
         codes = np.zeros((max(n, 1), m), np.uint8)
         check(lib().hm_pq_export(self._h, cb.ctypes.data, codes.ctypes.data))
         return {"codebook": cb, "codes": codes[:n]}
+
+
+def naive_build_knn_graph(rows, mt: Metric, k: int) -> List[List[Tuple[int, float]]]:
+    """knn.rs:14-41: graph[n] = the k nearest other rows of row n as
+    (index, distance), in the order the reference's heap drains them
+    (largest distance first)."""
+    rows = np.ascontiguousarray(rows, dtype=np.float32)
+    ix = BruteForceIndex(rows.shape[1])
+    ix.add_batch(rows)
+    ix.build(mt)
+    idx, dist, cnt = ix.build_knn_graph(k)
+    return [[(int(idx[r, j]), float(dist[r, j])) for j in range(int(cnt[r]))]
+            for r in range(rows.shape[0])]
 
 
 def mlp_loss_grad(W1, b1, W2, b2, X, T=None):

@@ -185,6 +185,15 @@ HM_API int32_t hm_index_load(int32_t kind, const char *path, int32_t id_width, h
 HM_API int32_t hm_index_loaded_ids(const hm_index *h, uint8_t *id_bytes, uint64_t *id_offsets,
                                    uint64_t *total_bytes);
 
+/* naive_build_knn_graph (knn.rs:14-41; the same loop as SSGIndex's
+ * build_knn_graph, ssg_idx.rs:69-98) over the rows of a built
+ * BruteForceIndex, with its metric: row i of idx/dist ([n][k], host memory)
+ * holds the k nearest other rows in the order the reference's heap drains
+ * them -- largest distance first -- and count[i] = min(k, n - 1).  Unused
+ * slots hold UINT64_MAX / +inf. */
+HM_API int32_t hm_index_knn_graph(hm_index *h, uint32_t k, uint64_t *idx, float *dist,
+                                  uint32_t *count);
+
 /* ANNIndex::nodes_size (ann_index.rs:105) -- number of rows added. */
 HM_API int32_t hm_index_nodes_size(const hm_index *h, uint64_t *out);
 

@@ -85,6 +85,7 @@ def lib():
                                     _f32p, _sz, _u32p]
     L.orc_pqindex_search.argtypes = [_f32p, _sz, _sz, C.c_int, _f32p, _sz, _sz, _sz, _u32p,
                                      _sz, _sz, C.c_int, _u64p, _f32p, _u32p]
+    L.orc_knn_graph.argtypes = [_f32p, _sz, _sz, _sz, C.c_int, C.c_int, _u64p, _f32p, _u32p]
     L.orc_pqindex_lut.argtypes = [_f32p, _sz, _sz, _f32p, _sz, _sz, C.c_int, _f32p]
     _lib = L
     return L
@@ -376,3 +377,18 @@ def pqindex_lut(q, mt: int, codebook):
     if st:
         raise OracleError(st, "pqindex_lut")
     return lut.reshape(n_sub, n_center)
+
+
+def knn_graph(base, k: int, mt: int, nthreads: int = 0):
+    """naive_build_knn_graph (knn.rs:14-41): (idx [n,k], dist [n,k], count
+    [n]), rows largest distance first."""
+    base = _f32(base)
+    n, d = base.shape
+    kk = max(k, 1)
+    ids = np.zeros((max(n, 1), kk), np.uint64)
+    dist = np.zeros((max(n, 1), kk), np.float32)
+    cnt = np.zeros(max(n, 1), np.uint32)
+    st = lib().orc_knn_graph(base, n, d, k, mt, nthreads, ids, dist, cnt)
+    if st:
+        raise OracleError(st, "knn_graph")
+    return ids[:n, :k], dist[:n, :k], cnt[:n]

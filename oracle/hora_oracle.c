@@ -410,6 +410,62 @@ uint64_t orc_stream_seed(uint64_t seed, uint64_t instance)
     return splitmix64(&s);
 }
 
+/* naive_build_knn_graph (knn.rs:14-41): for node n, every other node i in
+ * index order is pushed as Neighbor(i, metric(node_n, node_i)) and the heap
+ * popped while len > k; the heap is drained by pop into the row (largest
+ * distance first, no reverse).  Node n itself is skipped, not pushed.
+ * out_idx/out_dist [n][k] (unused slots UINT64_MAX / +inf), out_count [n].
+ * A NaN compared in the heap (k >= 1 and at least 2 other nodes) is the
+ * reference's partial_cmp().unwrap() panic: ORC_ERR_NAN. */
+int orc_knn_graph(const float *base, size_t n, size_t d, size_t k, int mt, int nthreads,
+                  uint64_t *out_idx, float *out_dist, uint32_t *out_count)
+{
+    if (mt == ORC_UNKNOWN || mt > ORC_ANGULAR || mt < 0) return ORC_ERR_UNKNOWN_METRIC;
+    int status = ORC_OK;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        orc_heap h = {0};
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+        for (long r = 0; r < (long)n; ++r) {
+            int nan_seen = 0;
+            h.len = 0;
+            for (size_t i = 0; i < n; ++i) {
+                if (i == (size_t)r) continue;
+                float dist;
+                orc_metric(base + (size_t)r * d, base + i * d, d, mt, &dist);
+                if (dist != dist) nan_seen = 1;
+                orc_nb x = {dist, (uint64_t)i};
+                heap_push(&h, x);
+                if (h.len > k) (void)heap_pop(&h);
+            }
+            if (nan_seen && k >= 1 && n >= 3) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+                status = ORC_ERR_NAN;
+            }
+            size_t m = h.len;
+            for (size_t j = 0; j < m; ++j) {
+                orc_nb x = heap_pop(&h);
+                out_idx[(size_t)r * k + j] = x.idx;
+                out_dist[(size_t)r * k + j] = x.dist;
+            }
+            for (size_t j = m; j < k; ++j) {
+                out_idx[(size_t)r * k + j] = UINT64_MAX;
+                out_dist[(size_t)r * k + j] = INFINITY;
+            }
+            out_count[r] = (uint32_t)m;
+        }
+        free(h.data);
+    }
+    return status;
+}
+
 /* ------------------------------------------------------------ Kmeans     */
 /* kmeans.rs:8-214.  Data is addressed as rows of `stride` floats; the
  * k-means works on columns [begin, begin+dim) with an optional residual of

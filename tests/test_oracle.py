@@ -399,3 +399,45 @@ def test_bincode_layout_hand_assembled():
     got_pq = bc.pqindex(2, 1, 1, 5, [[[0.5, 1.5], [2.0, 3.0]]], np.array([[1]]), [[1.0, 2.0]],
                         [bc.option_usize(0)], 2)
     assert got_pq == want_pq
+
+
+# ------------------------------------------------------------- kNN graph
+def test_knn_graph_without_ties_is_a_descending_sort(orc):
+    rng = np.random.default_rng(60)
+    x = rng.normal(0, 1, (120, 7)).astype(np.float32)
+    ids, dist, cnt = orc.knn_graph(x, 5, orc.EUCLIDEAN)
+    full = orc.distance_rows(x, x, orc.EUCLIDEAN)
+    for r in range(120):
+        row = full[r].copy()
+        order = [i for i in np.argsort(row, kind="stable") if i != r][:5]
+        assert cnt[r] == 5
+        assert ids[r].tolist() == order[::-1]  # drained by pop: largest first
+        assert dist[r].tolist() == row[order[::-1]].tolist()
+
+
+def test_knn_graph_ties_follow_the_std_heap_and_skip_self(orc):
+    rng = np.random.default_rng(61)
+    x = rng.integers(0, 3, (40, 2)).astype(np.float32)
+    for k in (1, 3, 39, 50):
+        ids, dist, cnt = orc.knn_graph(x, k, orc.MANHATTAN)
+        for r in range(40):
+            others = [i for i in range(40) if i != r]
+            d = [orc.metric(x[r], x[i], orc.MANHATTAN) for i in others]
+            # push each other node, pop while len > k, then drain
+            layout_ops, ld, li = [], [], []
+            size = 0
+            for i, v in zip(others, d):
+                layout_ops.append(0), ld.append(v), li.append(i)
+                size += 1
+                if size > k:
+                    layout_ops.append(1), ld.append(0.0), li.append(0)
+                    size -= 1
+            m = min(k, 39)
+            layout_ops += [1] * m
+            ld += [0.0] * m
+            li += [0] * m
+            _, popped = orc.heap_script(layout_ops, ld, li)
+            tail = popped[len(popped) - m:]
+            assert cnt[r] == m
+            assert ids[r, :m].tolist() == [i for _, i in tail]
+            assert dist[r, :m].tolist() == [v for v, _ in tail]
