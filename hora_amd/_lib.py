@@ -93,6 +93,8 @@ def lib():
     L.hm_index_load.argtypes = [i32, C.c_char_p, i32, C.POINTER(vp)]
     L.hm_index_loaded_ids.argtypes = [vp, vp, vp, C.POINTER(u64)]
     L.hm_index_knn_graph.argtypes = [vp, u32, vp, vp, vp]
+    L.hm_index_distances.argtypes = [vp, vp, u64, vp, vp, vp]
+    L.hm_index_distances_device.argtypes = [vp, vp, u64, vp, vp, u64, vp, vp]
     L.hm_pq_info.argtypes = [vp, vp]
     L.hm_pq_export.argtypes = [vp, vp, vp]
     L.hm_ivfpq_train_probe.argtypes = [vp, vp, u64, u32, u32, u32, u32, C.c_float, u64,

@@ -352,6 +352,36 @@ class BruteForce final : public Index {
         if (err & 1u) fail(HM_ERR_NAN_DISTANCE, "NaN distance (partial_cmp of NaN)");
     }
 
+    // The distance evaluations of hnsw_idx.rs:243-254 / ssg_idx.rs:402-456,
+    // batched: metric(node, query) for every (query, node) pair.
+    void distances_device(const float *dq, uint64_t nq, const uint32_t *ids, const uint64_t *off,
+                          uint64_t total, float *out, hipStream_t s) override
+    {
+        if (!s) s = stream_;
+        if (metric_ == HM_METRIC_UNKNOWN) fail(HM_ERR_UNKNOWN_METRIC, "unknown method");
+        if (nq == 0 || total == 0) return;
+        if (!dq || !ids || !off || !out) fail(HM_ERR_INVALID_ARG, "null pointer");
+        if (nq > 0xFFFFFFFFull) fail(HM_ERR_INVALID_ARG, "too many queries");
+        if (need_norm()) ensure_bnorm(s);
+        err_.reserve(4);
+        HM_HIP(hipMemsetAsync(err_.get(), 0, 4, s));
+        launch_check_finite(dq, dim_, nq, (uint32_t)dim_, err_.as<uint32_t>(), s);
+        sv_q_.reserve(nq * dpad_ * 4);
+        sv_qn_.reserve(nq * 4);
+        launch_pad_rows(dq, dim_, sv_q_.as<float>(), dpad_, nq, (uint32_t)dim_, dpad_, s);
+        if (need_norm())
+            launch_self_dot(sv_q_.as<float>(), dpad_, (uint32_t)nq, (uint32_t)dim_,
+                            sv_qn_.as<float>(), s);
+        launch_pair_dist(metric_, sv_q_.as<float>(), dpad_, (uint32_t)nq, base_.as<float>(), dpad_,
+                         (uint32_t)n_, ids, off, total, sv_qn_.as<float>(), bnorm_.as<float>(), out,
+                         err_.as<uint32_t>(), s);
+        uint32_t err = 0;
+        HM_HIP(hipMemcpyAsync(&err, err_.get(), 4, hipMemcpyDeviceToHost, s));
+        HM_HIP(hipStreamSynchronize(s));
+        if (err & 2u) fail(HM_ERR_NONFINITE_INPUT, "invalid float element");
+        if (err & 4u) fail(HM_ERR_INVALID_ARG, "node index out of range");
+    }
+
     static constexpr uint32_t RAG_CHUNK = 4096;
 
     void capture_ragged_graph(uint32_t QC, uint32_t kmax, uint32_t S)
@@ -890,6 +920,7 @@ class BruteForce final : public Index {
     uint64_t n_ = 0, cap_ = 0;
     uint32_t dpad_ = 0;
     int32_t metric_ = HM_METRIC_UNKNOWN;
+    DeviceBuffer sv_q_, sv_qn_;  // distance service: padded queries, their norms
     int32_t mode_ = HM_SEARCH_EXACT;
     uint32_t shortlist_ = 0;
     size_t d_budget_ = (size_t)4 << 30;

@@ -332,6 +332,52 @@ HM_API int32_t hm_index_knn_graph(hm_index *h, uint32_t k, uint64_t *idx, float 
     });
 }
 
+HM_API int32_t hm_index_distances(hm_index *h, const float *queries, uint64_t nq,
+                                  const uint32_t *ids, const uint64_t *offsets, float *out)
+{
+    return guard([&] {
+        need(h, "index");
+        if (nq == 0) return;
+        need(queries, "queries");
+        need(offsets, "offsets");
+        const uint64_t total = offsets[nq];
+        for (uint64_t q = 0; q < nq; ++q)
+            if (offsets[q + 1] < offsets[q]) fail(HM_ERR_INVALID_ARG, "offsets decrease");
+        if (total == 0) return;
+        need(ids, "ids");
+        need(out, "out");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        hipStream_t s = h->impl->stream();
+        const uint64_t d = h->impl->dimension();
+        hm::DeviceBuffer dq, di, doff, dout;
+        dq.reserve(nq * d * 4);
+        di.reserve(total * 4);
+        doff.reserve((nq + 1) * 8);
+        dout.reserve(total * 4);
+        HM_HIP(hipMemcpyAsync(dq.get(), queries, nq * d * 4, hipMemcpyHostToDevice, s));
+        HM_HIP(hipMemcpyAsync(di.get(), ids, total * 4, hipMemcpyHostToDevice, s));
+        HM_HIP(hipMemcpyAsync(doff.get(), offsets, (nq + 1) * 8, hipMemcpyHostToDevice, s));
+        h->impl->distances_device(dq.as<float>(), nq, di.as<uint32_t>(), doff.as<uint64_t>(),
+                                  total, dout.as<float>(), s);
+        HM_HIP(hipMemcpyAsync(out, dout.get(), total * 4, hipMemcpyDeviceToHost, s));
+        HM_HIP(hipStreamSynchronize(s));
+    });
+}
+
+HM_API int32_t hm_index_distances_device(hm_index *h, const float *d_queries, uint64_t nq,
+                                         const uint32_t *d_ids, const uint64_t *d_offsets,
+                                         uint64_t total, float *d_out, void *stream)
+{
+    return guard([&] {
+        need(h, "index");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        h->impl->distances_device(d_queries, nq, d_ids, d_offsets, total, d_out,
+                                  static_cast<hipStream_t>(stream));
+    });
+}
+
 HM_API int32_t hm_index_nodes_size(const hm_index *h, uint64_t *out)
 {
     return guard([&] {

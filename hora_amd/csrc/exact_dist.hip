@@ -260,6 +260,47 @@ __global__ void self_dot_kernel(const float *__restrict__ X, uint32_t ldx, uint3
     out[r] = acc;
 }
 
+// Batched distance service of the graph searches: one lane per (query,
+// node) pair, metric(node, query) in the reference's fold order -- the
+// operand order of hnsw_idx.rs:243-254 and ssg_idx.rs:402-456.  Pairs p in
+// [off[q], off[q+1]) belong to query q.  A node index out of range writes
+// NaN and sets *err bit 2.
+template <int ACC, int EPI>
+__global__ __launch_bounds__(256) void pair_dist_kernel(
+    const float *__restrict__ Qp, uint32_t dpad, uint32_t nq, const float *__restrict__ B,
+    uint32_t ldb, uint32_t n, const uint32_t *__restrict__ ids, const uint64_t *__restrict__ off,
+    uint64_t total, const float *__restrict__ qnorm, const float *__restrict__ bnorm,
+    float *__restrict__ out, uint32_t *__restrict__ err)
+{
+    const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= total) return;
+    uint32_t lo = 0, hi = nq;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (off[mid] <= p) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t q = lo, id = ids[p];
+    if (id >= n) {
+        out[p] = __int_as_float(0x7fc00000);
+        atomicOr(err, 4u);
+        return;
+    }
+    const float4 *a = reinterpret_cast<const float4 *>(B + (size_t)id * ldb);
+    const float4 *b = reinterpret_cast<const float4 *>(Qp + (size_t)q * dpad);
+    float acc = -0.0f;
+    for (uint32_t t = 0; t < dpad / 4; ++t) {
+        const float4 x = a[t], y = b[t];
+        acc = acc_step<ACC>(acc, x.x, y.x);
+        acc = acc_step<ACC>(acc, x.y, y.y);
+        acc = acc_step<ACC>(acc, x.z, y.z);
+        acc = acc_step<ACC>(acc, x.w, y.w);
+    }
+    if constexpr (EPI == EPI_COSINE) acc = epi_cosine(acc, bnorm[id], qnorm[q]);
+    else if constexpr (EPI == EPI_ANGULAR) acc = epi_angular(acc, bnorm[id], qnorm[q]);
+    out[p] = acc;
+}
+
 }  // namespace
 
 template <bool ARGMIN>
@@ -344,6 +385,30 @@ void launch_exact_argmin(int metric, const float *Q, uint32_t ldq, uint32_t nq, 
 {
     launch_exact_tile_impl<true>(metric, Q, ldq, nq, B, ldb, n, dpad, qnorm, bnorm, nullptr, 0,
                                  best, s);
+}
+
+void launch_pair_dist(int metric, const float *Qp, uint32_t dpad, uint32_t nq, const float *B,
+                      uint32_t ldb, uint32_t n, const uint32_t *ids, const uint64_t *off,
+                      uint64_t total, const float *qnorm, const float *bnorm, float *out,
+                      uint32_t *err, hipStream_t s)
+{
+    if (total == 0 || nq == 0) return;
+    if (dpad % 4 || ldb % 4) fail(HM_ERR_INVALID_ARG, "pair_dist: rows must be padded to 4");
+    ProfScope prof("pair_dist", s);
+    const unsigned blocks = (unsigned)((total + 255) / 256);
+#define HM_PD(ACC, EPI)                                                                        \
+    pair_dist_kernel<ACC, EPI><<<blocks, 256, 0, s>>>(Qp, dpad, nq, B, ldb, n, ids, off, total,  \
+                                                      qnorm, bnorm, out, err)
+    switch (metric) {
+    case HM_METRIC_EUCLIDEAN: HM_PD(ACC_EUCLID, EPI_NONE); break;
+    case HM_METRIC_MANHATTAN: HM_PD(ACC_MANHATTAN, EPI_NONE); break;
+    case HM_METRIC_DOT_PRODUCT: HM_PD(ACC_DOT, EPI_NONE); break;
+    case HM_METRIC_COSINE_SIMILARITY: HM_PD(ACC_DOT, EPI_COSINE); break;
+    case HM_METRIC_ANGULAR: HM_PD(ACC_DOT, EPI_ANGULAR); break;
+    default: fail(HM_ERR_UNKNOWN_METRIC, "unknown method");
+    }
+#undef HM_PD
+    HM_CHECK_LAUNCH();
 }
 
 void launch_self_dot(const float *X, uint32_t ldx, uint32_t rows, uint32_t d, float *out,

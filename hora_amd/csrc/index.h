@@ -49,6 +49,15 @@ class Index {
         (void)k, (void)h_idx, (void)h_dist, (void)h_count;
         fail(HM_ERR_NOT_IMPLEMENTED, "kNN graph build not supported by this index");
     }
+    // Batched distance service of the graph searches: out[p] =
+    // metric(row ids[p], query q) for p in [off[q], off[q+1]), total pairs;
+    // everything in device memory.
+    virtual void distances_device(const float *dq, uint64_t nq, const uint32_t *ids,
+                                  const uint64_t *off, uint64_t total, float *out, hipStream_t s)
+    {
+        (void)dq, (void)nq, (void)ids, (void)off, (void)total, (void)out, (void)s;
+        fail(HM_ERR_NOT_IMPLEMENTED, "distance service not supported by this index");
+    }
     // Chooses tuning parameters on a query sample; returns the shortlist.
     virtual uint32_t autotune(const float *h_queries, uint64_t nq, uint32_t k)
     {

@@ -31,6 +31,13 @@ void launch_exact_argmin(int metric, const float *Q, uint32_t ldq, uint32_t nq, 
 void launch_self_dot(const float *X, uint32_t ldx, uint32_t rows, uint32_t d, float *out,
                      hipStream_t s, const uint32_t *dev_rows = nullptr);
 
+// out[p] = metric(B[ids[p]], Q[q]) for the pairs p in [off[q], off[q+1])
+// (total pairs); ids >= n give NaN and set *err bit 2.
+void launch_pair_dist(int metric, const float *Qp, uint32_t dpad, uint32_t nq, const float *B,
+                      uint32_t ldb, uint32_t n, const uint32_t *ids, const uint64_t *off,
+                      uint64_t total, const float *qnorm, const float *bnorm, float *out,
+                      uint32_t *err, hipStream_t s);
+
 // ---- select.hip
 // For each of `rows` rows of D (n values, row stride ldd): the K smallest by
 // (distance, index), ascending, into cand_d/cand_i (row stride ldc), and

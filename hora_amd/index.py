@@ -400,6 +400,30 @@ class BruteForceIndex(_Index):
                                                   ks.ctypes.data, idx_ptr, dist_ptr, count_ptr,
                                                   stream or None))
 
+    def distances(self, queries, id_lists) -> List[np.ndarray]:
+        """Batched distance service of the graph searches (hnsw_idx.rs:243-254,
+        ssg_idx.rs:402-456): for query q, metric(node, q) of every node
+        index in id_lists[q]."""
+        q = _f32_rows(queries, self._dim)
+        if len(id_lists) != q.shape[0]:
+            raise HoraError(1, "one id list per query")
+        lens = [len(x) for x in id_lists]
+        off = np.zeros(q.shape[0] + 1, np.uint64)
+        off[1:] = np.cumsum(lens) if lens else []
+        ids = np.ascontiguousarray(np.concatenate([np.asarray(x, np.uint32).reshape(-1)
+                                                   for x in id_lists])
+                                   if lens and sum(lens) else np.zeros(1, np.uint32),
+                                   dtype=np.uint32)
+        out = np.empty(max(int(off[-1]), 1), np.float32)
+        check(lib().hm_index_distances(self._h, q.ctypes.data, q.shape[0], ids.ctypes.data,
+                                       off.ctypes.data, out.ctypes.data))
+        return [out[int(off[i]):int(off[i + 1])] for i in range(q.shape[0])]
+
+    def distances_device(self, q_ptr: int, nq: int, ids_ptr: int, off_ptr: int, total: int,
+                         out_ptr: int, stream: int = 0) -> None:
+        check(lib().hm_index_distances_device(self._h, q_ptr, nq, ids_ptr, off_ptr, total,
+                                              out_ptr, stream or None))
+
     def build_knn_graph(self, k: int):
         """naive_build_knn_graph (knn.rs:14-41) over the indexed rows with the
         build metric: (idx u64 [n,k], dist f32 [n,k], count u32 [n]); row i

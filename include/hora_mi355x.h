@@ -194,6 +194,21 @@ HM_API int32_t hm_index_loaded_ids(const hm_index *h, uint8_t *id_bytes, uint64_
 HM_API int32_t hm_index_knn_graph(hm_index *h, uint32_t k, uint64_t *idx, float *dist,
                                   uint32_t *count);
 
+/* Batched distance service for graph searches: the distance evaluations of
+ * HNSWIndex::search_layer (get_distance_from_vec, hnsw_idx.rs:243-254) and
+ * SSGIndex::search (ssg_idx.rs:402-456), metric(node, query) with the
+ * index's metric and the reference's arithmetic, for many (query, node)
+ * pairs per call: out[p] for p in [offsets[q], offsets[q+1]) is the distance
+ * from node ids[p] to query q (offsets has nq + 1 entries, host memory in
+ * hm_index_distances).  A BruteForceIndex holds the nodes. */
+HM_API int32_t hm_index_distances(hm_index *h, const float *queries, uint64_t nq,
+                                  const uint32_t *ids, const uint64_t *offsets, float *out);
+
+/* Same with every buffer in device memory; `total` = offsets[nq]. */
+HM_API int32_t hm_index_distances_device(hm_index *h, const float *d_queries, uint64_t nq,
+                                         const uint32_t *d_ids, const uint64_t *d_offsets,
+                                         uint64_t total, float *d_out, void *stream);
+
 /* ANNIndex::nodes_size (ann_index.rs:105) -- number of rows added. */
 HM_API int32_t hm_index_nodes_size(const hm_index *h, uint64_t *out);
 
