@@ -480,128 +480,156 @@ __global__ void pq_finalize_kernel(const float *__restrict__ cand_d,
     }
 }
 
-// std BinaryHeap (max-heap on distance) whose first H positions live in
-// LDS and the rest in a global buffer; every lane of the wave runs the same
-// operations on the same words.
-template <bool HYB> struct HybridHeap {
-    float *ld;
-    uint32_t *li;
-    float *gd;
-    uint32_t *gi;
-    uint32_t H, len;
-    __device__ float d(uint32_t p) const
-    {
-        if constexpr (HYB) return p < H ? ld[p] : gd[p - H];
-        else return ld[p];
-    }
-    __device__ uint32_t ix(uint32_t p) const
-    {
-        if constexpr (HYB) return p < H ? li[p] : gi[p - H];
-        else return li[p];
-    }
-    __device__ void put(uint32_t p, float x, uint32_t i)
-    {
-        if (!HYB || p < H) {
-            ld[p] = x;
-            li[p] = i;
-        } else {
-            gd[p - H] = x;
-            gi[p - H] = i;
-        }
-    }
-    // heap.rs:21-35: move up while the element is greater than its parent
-    __device__ void sift_up(uint32_t start, uint32_t pos, float ed, uint32_t ei)
-    {
-        while (pos > start) {
-            const uint32_t parent = (pos - 1) / 2;
-            const float pd = d(parent);
-            if (ed <= pd) break;
-            put(pos, pd, ix(parent));
-            pos = parent;
-        }
-        put(pos, ed, ei);
-    }
-    // heap.rs:54-68: the hole goes to the bottom along the larger child
-    // (the right one when left <= right), then the element sifts up
-    __device__ void sift_down_to_bottom(float ed, uint32_t ei)
-    {
-        const uint32_t end = len;
-        uint32_t pos = 0, child = 1;
-        while (end >= 2 && child < end - 1) {
-            const float a = d(child), bb = d(child + 1);
-            child += (a <= bb) ? 1u : 0u;
-            put(pos, child == pos * 2 + 1 ? a : bb, ix(child));
-            pos = child;
-            child = 2 * pos + 1;
-        }
-        if (end >= 1 && child == end - 1) {
-            put(pos, d(child), ix(child));
-            pos = child;
-        }
-        sift_up(0, pos, ed, ei);
-    }
-    __device__ void push(float x, uint32_t i)
-    {
-        len += 1;
-        sift_up(0, len - 1, x, i);
-    }
-    __device__ void pop(float *od, uint32_t *oi)
-    {
-        const float lx = d(len - 1);
-        const uint32_t lid = ix(len - 1);
-        len -= 1;
-        if (len > 0) {
-            *od = d(0);
-            *oi = ix(0);
-            sift_down_to_bottom(lx, lid);
-        } else {
-            *od = lx;
-            *oi = lid;
-        }
-    }
-};
-
 // Replay of pq_idx.rs:181-191 + 212-233 for tied rows: push every item of
 // the distance row in item order, pop until k remain, then drain (the pops
 // come out descending and are written from the back).  One wave per row;
-// rows are taken in a grid-stride loop so each workgroup reuses its slice
-// of the global heap buffer.
+// rows are taken in a grid-stride loop so each workgroup reuses its slice of
+// the global heap buffer.
+//
+// The heap is the reference's std BinaryHeap (heap.rs:21-68) run by the
+// whole wave.  Entries are (distance bits, item) pairs; positions below H
+// live in LDS, the rest (HYB) in HBM.
+//   push: the new element climbs while it is greater than its parent, so
+//     with every ancestor loaded at once (one lane each; their values rise
+//     toward the root) the number of levels climbed is the count of leading
+//     lanes whose ancestor is smaller, and the moves are one parallel store.
+//   pop: the last element replaces the root; the hole descends to the bottom
+//     along the larger child (the right one when left <= right), and the
+//     element then climbs back while greater than its parent.  The descent
+//     is resolved PR_R levels per round: lane l loads one sibling pair of
+//     the PR_R-level subtree below the round's start, and the path is walked
+//     through register reads.  Along the path the moved values fall from the
+//     root down, so the climb stops below the first path value the element
+//     does not exceed: only path positions above that point are written.
 constexpr int PR_THREADS = 64;
+constexpr uint32_t PR_R = 6;  // 2^6 - 1 = 63 sibling pairs per round
 
 template <bool HYB>
 __global__ __launch_bounds__(PR_THREADS) void pq_replay_kernel(
     const uint32_t *__restrict__ rows, uint32_t nrows, const float *__restrict__ D, uint64_t ldd,
-    uint32_t n, uint32_t k, uint32_t H, float *__restrict__ gbuf_d, uint32_t *__restrict__ gbuf_i,
-    uint64_t *__restrict__ out_idx, float *__restrict__ out_dist)
+    uint32_t n, uint32_t k, uint32_t H, uint2 *__restrict__ gbuf, uint64_t *__restrict__ out_idx,
+    float *__restrict__ out_dist)
 {
-    extern __shared__ __align__(16) unsigned char pr_smem[];
-    const int lane = threadIdx.x;
+    extern __shared__ __align__(16) uint2 pr_heap[];
+    const uint32_t lane = threadIdx.x;
     const size_t gslice = n > H ? (size_t)(n - H) : 0;
+    uint2 *hg = HYB ? gbuf + (size_t)blockIdx.x * gslice : nullptr;
+    auto ld = [&](uint32_t p) -> uint2 {
+        if constexpr (HYB) {
+            if (p >= H) return hg[p - H];
+        }
+        return pr_heap[p];
+    };
+    auto st = [&](uint32_t p, uint2 v) {
+        if constexpr (HYB) {
+            if (p >= H) {
+                hg[p - H] = v;
+                return;
+            }
+        }
+        pr_heap[p] = v;
+    };
+    auto fv = [](uint2 v) { return __uint_as_float(v.x); };
+    auto rl = [](uint2 v, uint32_t l) -> uint2 {
+        return make_uint2((uint32_t)__builtin_amdgcn_readlane((int)v.x, (int)l),
+                          (uint32_t)__builtin_amdgcn_readlane((int)v.y, (int)l));
+    };
+    // heap.rs:21-35 (sift_up from the new last position p)
+    auto push = [&](uint2 e, uint32_t p) {
+        const uint32_t A = 31u - (uint32_t)__clz(p + 1);  // ancestors of p
+        uint2 u = make_uint2(0u, 0u);
+        bool mv = false;
+        if (lane < A) {
+            u = ld(((p + 1) >> (lane + 1)) - 1);
+            mv = fv(e) > fv(u);
+        }
+        const uint64_t m = __ballot(mv);
+        const uint32_t t = (uint32_t)__builtin_ctzll(~m);
+        if (lane < t) st(lane == 0 ? p : ((p + 1) >> lane) - 1, u);
+        if (lane == 0) st(t == 0 ? p : ((p + 1) >> t) - 1, e);
+    };
+    // heap.rs:54-68 + 95-112 (pop: sift_down_to_bottom(0), then sift_up)
+    auto pop = [&](uint32_t &len) -> uint2 {
+        len -= 1;
+        const uint2 last = ld(len);
+        if (len == 0) return last;
+        const uint2 root = ld(0);
+        const uint64_t end = len;
+        uint32_t cur = 0, depth = 0, pp = 0;
+        uint2 pw = make_uint2(0u, 0u);
+        bool done = false;
+        while (!done) {
+            const uint64_t r0 = cur;
+            uint2 a = make_uint2(0u, 0u), b = make_uint2(0u, 0u);
+            if (lane < 63) {
+                const uint32_t j = 32u - (uint32_t)__clz(lane + 1);
+                const uint64_t o = lane + 1 - (1u << (j - 1));
+                const uint64_t first = ((r0 + 1) << j) - 1 + 2 * o;
+                if (first < end) a = ld((uint32_t)first);
+                if (first + 1 < end) b = ld((uint32_t)first + 1);
+            }
+            for (uint32_t step = 1; step <= PR_R; ++step) {
+                const uint64_t c1 = 2 * (uint64_t)cur + 1;
+                if (c1 >= end) {
+                    done = true;
+                    break;
+                }
+                const uint32_t l =
+                    (1u << (step - 1)) - 1 + (uint32_t)((c1 - (((r0 + 1) << step) - 1)) / 2);
+                const uint2 av = rl(a, l);
+                uint32_t c;
+                uint2 w;
+                if (c1 + 1 < end) {
+                    const uint2 bv = rl(b, l);
+                    if (fv(av) <= fv(bv)) {
+                        c = (uint32_t)c1 + 1;
+                        w = bv;
+                    } else {
+                        c = (uint32_t)c1;
+                        w = av;
+                    }
+                } else {
+                    c = (uint32_t)c1;
+                    w = av;
+                    done = true;
+                }
+                if (lane == depth) {
+                    pp = cur;
+                    pw = w;
+                }
+                ++depth;
+                cur = c;
+                if (done) break;
+            }
+        }
+        const uint64_t m = __ballot(lane < depth && fv(last) > fv(pw));
+        const uint32_t sdepth = depth - (uint32_t)__popcll(m);
+        if (lane < sdepth) st(pp, pw);
+        if (lane == 0)
+            st(sdepth == depth ? cur : (uint32_t)__builtin_amdgcn_readlane((int)pp, (int)sdepth),
+               last);
+        return root;
+    };
     for (uint32_t t = blockIdx.x; t < nrows; t += gridDim.x) {
         const uint32_t r = rows[t];
         const float *drow = D + (size_t)r * ldd;
-        HybridHeap<HYB> h{reinterpret_cast<float *>(pr_smem),
-                     reinterpret_cast<uint32_t *>(pr_smem) + H,
-                     gbuf_d + blockIdx.x * gslice, gbuf_i + blockIdx.x * gslice, H, 0};
+        uint32_t len = 0;
         for (uint32_t i0 = 0; i0 < n; i0 += 64) {
             const float v = i0 + lane < n ? drow[i0 + lane] : 0.0f;
             const uint32_t cnt = min(64u, n - i0);
-            for (uint32_t u = 0; u < cnt; ++u)
-                h.push(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), u)), i0 + u);
+            for (uint32_t u = 0; u < cnt; ++u) {
+                const uint32_t bits = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(v), (int)u);
+                push(make_uint2(bits, i0 + u), len);
+                ++len;
+            }
         }
-        while (h.len > k) {
-            float a;
-            uint32_t b;
-            h.pop(&a, &b);
-        }
-        const uint32_t m = h.len;
+        while (len > k) (void)pop(len);
+        const uint32_t m = len;
         for (uint32_t p = m; p > 0; --p) {
-            float a;
-            uint32_t b;
-            h.pop(&a, &b);
+            const uint2 x = pop(len);
             if (lane == 0) {
-                out_idx[(size_t)r * k + p - 1] = b;
-                out_dist[(size_t)r * k + p - 1] = a;
+                out_idx[(size_t)r * k + p - 1] = x.y;
+                out_dist[(size_t)r * k + p - 1] = __uint_as_float(x.x);
             }
         }
         __syncthreads();
@@ -920,23 +948,22 @@ class PQ final : public Index {
         const uint32_t m = (uint32_t)tied.size();
         rp_rows_.reserve(m * 4);
         HM_HIP(hipMemcpyAsync(rp_rows_.get(), tied.data(), m * 4, hipMemcpyHostToDevice, s));
-        const size_t lds = std::min<size_t>(pq_lds_limit(), 131072) & ~(size_t)7;
-        const uint32_t H = (uint32_t)std::min<size_t>(lds / 8, n);
-        const uint32_t grid = std::min<uint32_t>(m, 256);
-        const size_t gslice = n > H ? (size_t)(n - H) : 0;
-        if (gslice) {
-            rp_gd_.reserve((size_t)grid * gslice * 4);
-            rp_gi_.reserve((size_t)grid * gslice * 4);
-        }
+        // rows up to 8,192 items keep the whole heap in LDS (<= 64 KiB);
+        // longer rows keep the top 11 levels (16 KiB, 8 rows per CU) and
+        // the rest in HBM
+        const bool hyb = n > 8192;
+        const uint32_t H = hyb ? 2047u : n;
+        const uint32_t grid = std::min<uint32_t>(m, hyb ? 2048u : 512u);
+        const size_t gslice = hyb ? (size_t)(n - H) : 0;
+        if (hyb) rp_g_.reserve((size_t)grid * gslice * 8);
         const size_t use = (size_t)H * 8;
-        if (gslice)
+        if (hyb)
             pq_replay_kernel<true><<<grid, PR_THREADS, use, s>>>(
-                rp_rows_.as<uint32_t>(), m, D_.as<float>(), ldd, n, k, H, rp_gd_.as<float>(),
-                rp_gi_.as<uint32_t>(), d_idx, d_dist);
+                rp_rows_.as<uint32_t>(), m, D_.as<float>(), ldd, n, k, H, rp_g_.as<uint2>(), d_idx,
+                d_dist);
         else
             pq_replay_kernel<false><<<grid, PR_THREADS, use, s>>>(
-                rp_rows_.as<uint32_t>(), m, D_.as<float>(), ldd, n, k, H, nullptr, nullptr, d_idx,
-                d_dist);
+                rp_rows_.as<uint32_t>(), m, D_.as<float>(), ldd, n, k, H, nullptr, d_idx, d_dist);
         HM_CHECK_LAUNCH();
         HM_HIP(hipStreamSynchronize(s));
     }
@@ -1048,7 +1075,7 @@ class PQ final : public Index {
     int32_t metric_ = HM_METRIC_UNKNOWN;
     bool trained_ = false;
     DeviceBuffer base_, sub_b_, sub_e_, cb_, cbn_, asg_, counts_, codes_, rng_;
-    DeviceBuffer qpad_, D_, cand_d_, cand_i_, flags_, err_, lutg_, rp_rows_, rp_gd_, rp_gi_;
+    DeviceBuffer qpad_, D_, cand_d_, cand_i_, flags_, err_, lutg_, rp_rows_, rp_g_;
 };
 
 }  // namespace
