@@ -25,6 +25,7 @@
 #include "ivfpq.h"
 #include "pq_common.h"
 #include "profiler.h"
+#include "wave_heap.h"
 
 #pragma clang fp contract(off)
 
@@ -482,27 +483,34 @@ __global__ void pq_finalize_kernel(const float *__restrict__ cand_d,
 
 // Replay of pq_idx.rs:181-191 + 212-233 for tied rows: push every item of
 // the distance row in item order, pop until k remain, then drain (the pops
-// come out descending and are written from the back).  One wave per row;
-// rows are taken in a grid-stride loop so each workgroup reuses its slice of
-// the global heap buffer.
-//
-// The heap is the reference's std BinaryHeap (heap.rs:21-68) run by the
-// whole wave.  Entries are (distance bits, item) pairs; positions below H
-// live in LDS, the rest (HYB) in HBM.
-//   push: the new element climbs while it is greater than its parent, so
-//     with every ancestor loaded at once (one lane each; their values rise
-//     toward the root) the number of levels climbed is the count of leading
-//     lanes whose ancestor is smaller, and the moves are one parallel store.
-//   pop: the last element replaces the root; the hole descends to the bottom
-//     along the larger child (the right one when left <= right), and the
-//     element then climbs back while greater than its parent.  The descent
-//     is resolved PR_R levels per round: lane l loads one sibling pair of
-//     the PR_R-level subtree below the round's start, and the path is walked
-//     through register reads.  Along the path the moved values fall from the
-//     root down, so the climb stops below the first path value the element
-//     does not exceed: only path positions above that point are written.
+// come out descending and are written from the back).  One wave per row
+// runs the std heap (wave_heap.h); rows are taken in a grid-stride loop so
+// each workgroup reuses its slice of the global heap buffer.  Heap positions
+// below H live in LDS, the rest (HYB) in HBM.
 constexpr int PR_THREADS = 64;
-constexpr uint32_t PR_R = 6;  // 2^6 - 1 = 63 sibling pairs per round
+
+template <bool HYB> struct WhHybrid {
+    uint2 *lds;
+    uint2 *g;
+    uint32_t H;
+    __device__ uint2 ld(uint32_t p) const
+    {
+        if constexpr (HYB) {
+            if (p >= H) return g[p - H];
+        }
+        return lds[p];
+    }
+    __device__ void st(uint32_t p, uint2 v) const
+    {
+        if constexpr (HYB) {
+            if (p >= H) {
+                g[p - H] = v;
+                return;
+            }
+        }
+        lds[p] = v;
+    }
+};
 
 template <bool HYB>
 __global__ __launch_bounds__(PR_THREADS) void pq_replay_kernel(
@@ -514,119 +522,22 @@ __global__ __launch_bounds__(PR_THREADS) void pq_replay_kernel(
     const uint32_t lane = threadIdx.x;
     const size_t gslice = n > H ? (size_t)(n - H) : 0;
     uint2 *hg = HYB ? gbuf + (size_t)blockIdx.x * gslice : nullptr;
-    auto ld = [&](uint32_t p) -> uint2 {
-        if constexpr (HYB) {
-            if (p >= H) return hg[p - H];
-        }
-        return pr_heap[p];
-    };
-    auto st = [&](uint32_t p, uint2 v) {
-        if constexpr (HYB) {
-            if (p >= H) {
-                hg[p - H] = v;
-                return;
-            }
-        }
-        pr_heap[p] = v;
-    };
-    auto fv = [](uint2 v) { return __uint_as_float(v.x); };
-    auto rl = [](uint2 v, uint32_t l) -> uint2 {
-        return make_uint2((uint32_t)__builtin_amdgcn_readlane((int)v.x, (int)l),
-                          (uint32_t)__builtin_amdgcn_readlane((int)v.y, (int)l));
-    };
-    // heap.rs:21-35 (sift_up from the new last position p)
-    auto push = [&](uint2 e, uint32_t p) {
-        const uint32_t A = 31u - (uint32_t)__clz(p + 1);  // ancestors of p
-        uint2 u = make_uint2(0u, 0u);
-        bool mv = false;
-        if (lane < A) {
-            u = ld(((p + 1) >> (lane + 1)) - 1);
-            mv = fv(e) > fv(u);
-        }
-        const uint64_t m = __ballot(mv);
-        const uint32_t t = (uint32_t)__builtin_ctzll(~m);
-        if (lane < t) st(lane == 0 ? p : ((p + 1) >> lane) - 1, u);
-        if (lane == 0) st(t == 0 ? p : ((p + 1) >> t) - 1, e);
-    };
-    // heap.rs:54-68 + 95-112 (pop: sift_down_to_bottom(0), then sift_up)
-    auto pop = [&](uint32_t &len) -> uint2 {
-        len -= 1;
-        const uint2 last = ld(len);
-        if (len == 0) return last;
-        const uint2 root = ld(0);
-        const uint64_t end = len;
-        uint32_t cur = 0, depth = 0, pp = 0;
-        uint2 pw = make_uint2(0u, 0u);
-        bool done = false;
-        while (!done) {
-            const uint64_t r0 = cur;
-            uint2 a = make_uint2(0u, 0u), b = make_uint2(0u, 0u);
-            if (lane < 63) {
-                const uint32_t j = 32u - (uint32_t)__clz(lane + 1);
-                const uint64_t o = lane + 1 - (1u << (j - 1));
-                const uint64_t first = ((r0 + 1) << j) - 1 + 2 * o;
-                if (first < end) a = ld((uint32_t)first);
-                if (first + 1 < end) b = ld((uint32_t)first + 1);
-            }
-            for (uint32_t step = 1; step <= PR_R; ++step) {
-                const uint64_t c1 = 2 * (uint64_t)cur + 1;
-                if (c1 >= end) {
-                    done = true;
-                    break;
-                }
-                const uint32_t l =
-                    (1u << (step - 1)) - 1 + (uint32_t)((c1 - (((r0 + 1) << step) - 1)) / 2);
-                const uint2 av = rl(a, l);
-                uint32_t c;
-                uint2 w;
-                if (c1 + 1 < end) {
-                    const uint2 bv = rl(b, l);
-                    if (fv(av) <= fv(bv)) {
-                        c = (uint32_t)c1 + 1;
-                        w = bv;
-                    } else {
-                        c = (uint32_t)c1;
-                        w = av;
-                    }
-                } else {
-                    c = (uint32_t)c1;
-                    w = av;
-                    done = true;
-                }
-                if (lane == depth) {
-                    pp = cur;
-                    pw = w;
-                }
-                ++depth;
-                cur = c;
-                if (done) break;
-            }
-        }
-        const uint64_t m = __ballot(lane < depth && fv(last) > fv(pw));
-        const uint32_t sdepth = depth - (uint32_t)__popcll(m);
-        if (lane < sdepth) st(pp, pw);
-        if (lane == 0)
-            st(sdepth == depth ? cur : (uint32_t)__builtin_amdgcn_readlane((int)pp, (int)sdepth),
-               last);
-        return root;
-    };
     for (uint32_t t = blockIdx.x; t < nrows; t += gridDim.x) {
         const uint32_t r = rows[t];
         const float *drow = D + (size_t)r * ldd;
-        uint32_t len = 0;
+        WaveHeapT<WhHybrid<HYB>> h{{pr_heap, hg, H}, 0u, lane};
         for (uint32_t i0 = 0; i0 < n; i0 += 64) {
             const float v = i0 + lane < n ? drow[i0 + lane] : 0.0f;
             const uint32_t cnt = min(64u, n - i0);
             for (uint32_t u = 0; u < cnt; ++u) {
                 const uint32_t bits = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(v), (int)u);
-                push(make_uint2(bits, i0 + u), len);
-                ++len;
+                h.push(make_uint2(bits, i0 + u));
             }
         }
-        while (len > k) (void)pop(len);
-        const uint32_t m = len;
+        while (h.len > k) (void)h.pop();
+        const uint32_t m = h.len;
         for (uint32_t p = m; p > 0; --p) {
-            const uint2 x = pop(len);
+            const uint2 x = h.pop();
             if (lane == 0) {
                 out_idx[(size_t)r * k + p - 1] = x.y;
                 out_dist[(size_t)r * k + p - 1] = __uint_as_float(x.x);

@@ -1,0 +1,135 @@
+// wave_heap.h -- the reference's std BinaryHeap (heap.rs:21-68, 95-112)
+// run by one whole wave, with the wave's lanes loading many heap slots at
+// once.  Entries are (distance bits, item) pairs; the storage policy S
+// provides `uint2 ld(uint32_t)` and `void st(uint32_t, uint2)`.  Every lane
+// of the wave must call push / pop together (the control flow is uniform).
+//
+//   push: the new element climbs while it is greater than its parent.  All
+//     ancestors are loaded at once (one lane each); their values rise toward
+//     the root, so the number of levels climbed is the count of leading
+//     lanes whose ancestor is smaller, and the moves are one parallel store.
+//   pop: the last element replaces the root; the hole descends to the bottom
+//     along the larger child (the right one when left <= right), and the
+//     element then climbs back while greater than its parent.  The descent
+//     is resolved WH_R levels per round: lane l loads one sibling pair of
+//     the WH_R-level subtree below the round's start and the path is walked
+//     through register reads.  The moved values fall along the path from the
+//     root down, so the climb stops below the first path value the element
+//     does not exceed: only path positions above that point are written.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hm {
+
+constexpr uint32_t WH_R = 6;  // 2^6 - 1 = 63 sibling pairs per round
+
+__device__ __forceinline__ float wh_val(uint2 v) { return __uint_as_float(v.x); }
+
+__device__ __forceinline__ uint2 wh_lane(uint2 v, uint32_t l)
+{
+    return make_uint2((uint32_t)__builtin_amdgcn_readlane((int)v.x, (int)l),
+                      (uint32_t)__builtin_amdgcn_readlane((int)v.y, (int)l));
+}
+
+template <class S> struct WaveHeapT {
+    S s;
+    uint32_t len;
+    uint32_t lane;
+
+    // heap.rs:21-35: append e at position len and sift it up
+    __device__ void push(uint2 e)
+    {
+        const uint32_t p = len++;
+        const uint32_t A = 31u - (uint32_t)__clz(p + 1);  // ancestors of p
+        uint2 u = make_uint2(0u, 0u);
+        bool mv = false;
+        if (lane < A) {
+            u = s.ld(((p + 1) >> (lane + 1)) - 1);
+            mv = wh_val(e) > wh_val(u);
+        }
+        const uint64_t m = __ballot(mv);
+        const uint32_t t = (uint32_t)__builtin_ctzll(~m);
+        if (lane < t) s.st(lane == 0 ? p : ((p + 1) >> lane) - 1, u);
+        if (lane == 0) s.st(t == 0 ? p : ((p + 1) >> t) - 1, e);
+    }
+
+    // heap.rs:54-68, 95-112: remove and return the root (len >= 1)
+    __device__ uint2 pop()
+    {
+        len -= 1;
+        const uint2 last = s.ld(len);
+        if (len == 0) return last;
+        const uint2 root = s.ld(0);
+        const uint64_t end = len;
+        uint32_t cur = 0, depth = 0, pp = 0;
+        uint2 pw = make_uint2(0u, 0u);
+        bool done = false;
+        while (!done) {
+            const uint64_t r0 = cur;
+            uint2 a = make_uint2(0u, 0u), b = make_uint2(0u, 0u);
+            if (lane < 63) {
+                const uint32_t j = 32u - (uint32_t)__clz(lane + 1);
+                const uint64_t o = lane + 1 - (1u << (j - 1));
+                const uint64_t first = ((r0 + 1) << j) - 1 + 2 * o;
+                if (first < end) a = s.ld((uint32_t)first);
+                if (first + 1 < end) b = s.ld((uint32_t)first + 1);
+            }
+            for (uint32_t step = 1; step <= WH_R; ++step) {
+                const uint64_t c1 = 2 * (uint64_t)cur + 1;
+                if (c1 >= end) {
+                    done = true;
+                    break;
+                }
+                const uint32_t l =
+                    (1u << (step - 1)) - 1 + (uint32_t)((c1 - (((r0 + 1) << step) - 1)) / 2);
+                const uint2 av = wh_lane(a, l);
+                uint32_t c;
+                uint2 w;
+                if (c1 + 1 < end) {
+                    const uint2 bv = wh_lane(b, l);
+                    if (wh_val(av) <= wh_val(bv)) {
+                        c = (uint32_t)c1 + 1;
+                        w = bv;
+                    } else {
+                        c = (uint32_t)c1;
+                        w = av;
+                    }
+                } else {
+                    c = (uint32_t)c1;
+                    w = av;
+                    done = true;
+                }
+                if (lane == depth) {
+                    pp = cur;
+                    pw = w;
+                }
+                ++depth;
+                cur = c;
+                if (done) break;
+            }
+        }
+        const uint64_t m = __ballot(lane < depth && wh_val(last) > wh_val(pw));
+        const uint32_t sd = depth - (uint32_t)__popcll(m);
+        if (lane < sd) s.st(pp, pw);
+        if (lane == 0)
+            s.st(sd == depth ? cur : (uint32_t)__builtin_amdgcn_readlane((int)pp, (int)sd), last);
+        return root;
+    }
+};
+
+// Storage: separate distance / item arrays (LDS or global).
+struct WhSplit {
+    float *d;
+    uint32_t *i;
+    __device__ uint2 ld(uint32_t p) const { return make_uint2(__float_as_uint(d[p]), i[p]); }
+    __device__ void st(uint32_t p, uint2 v) const
+    {
+        d[p] = __uint_as_float(v.x);
+        i[p] = v.y;
+    }
+};
+
+}  // namespace hm
