@@ -65,15 +65,19 @@ void launch_ivf_replay(const uint32_t *rows, uint32_t nrows, const float *Dc, ui
                        uint32_t lmax, float *scratch_f, uint32_t *scratch_u, size_t per_thread,
                        const uint32_t *probes, uint32_t ldp, const uint32_t *nprobe_q,
                        uint64_t *out_idx, float *out_dist, uint32_t *out_count, hipStream_t s);
-// The same replay with one workgroup per tied query and the heaps in LDS;
-// returns false (launching nothing) when the lists are too long for LDS.
+// The same replay with the heaps in LDS and each tied query's probed lists
+// simulated in parallel: order [nrows][nprobe] and np [nrows] hold the probe
+// order, seq [nrows][nprobe][k] / seq_cnt [nrows][nprobe] each list's
+// drained survivors.  Returns false (launching nothing) when the lists are
+// too long for LDS.
 bool launch_ivf_replay_lds(const uint32_t *rows, uint32_t nrows, const float *Dc, uint32_t ldd,
                            uint32_t nlist, uint32_t nprobe, const float *Qp, uint32_t dpad,
                            const float *C, const uint32_t *off, const uint32_t *ids,
                            const uint8_t *codes, const float *codebooks, const uint32_t *sub_b,
                            const uint32_t *sub_e, uint32_t M, uint32_t KS, uint32_t sdm,
                            uint32_t k, uint32_t lmax, const uint32_t *probes, uint32_t ldp,
-                           const uint32_t *nprobe_q, uint64_t *out_idx, float *out_dist,
+                           const uint32_t *nprobe_q, uint32_t *order, uint32_t *np,
+                           uint2 *seq, uint32_t *seq_cnt, uint64_t *out_idx, float *out_dist,
                            uint32_t *out_count, hipStream_t s);
 // Learned-probe helpers.
 void launch_negate(const float *x, float *y, uint64_t n, hipStream_t s);

@@ -792,177 +792,257 @@ struct GHeap {
     }
 };
 
-// Exact replay of IVFPQIndex::search_knn_adc + node_search_k
-// (pq_idx.rs:408-477) for flagged queries, one lane per query: coarse heap
-// of Neighbor(i, -dist) over all lists, nprobe pops, per list an unbounded
-// heap trimmed to k, popped into the global heap trimmed to k.
-// Replay of pq_idx.rs:408-439 for one tied query per workgroup, with the
-// heaps in LDS.  The coarse heap and the global top-k heap are simulated by
-// wave 0 (all its lanes execute the same operations on the same LDS words);
-// per probed list the LUT and the ADC values are computed by the whole
-// workgroup.  A list's own push-all / pop-to-k phase (pq_idx.rs:165-194)
-// only needs simulating when equal values occur among its k+1 smallest:
-// otherwise its survivors are exactly its k smallest, popped in descending
-// order, which a sort gives directly.
 constexpr int RP_THREADS = 256;
 
-__global__ __launch_bounds__(RP_THREADS) void ivf_replay_lds_kernel(
+// std BinaryHeap (heap.rs:21-68, 95-112) over packed (distance bits, item)
+// entries, position p stored at e[p + 1]: a sibling pair (2p+1, 2p+2) is one
+// aligned 16-byte read, so a level of sift_down_to_bottom is one LDS round
+// trip.  Every lane of the wave runs the same operations on the same words.
+struct PackedHeap {
+    uint2 *e;  // 16-byte aligned, capacity + 1 entries
+    uint32_t len;
+    __device__ static float val(uint2 x) { return __uint_as_float(x.y); }
+    __device__ void sift_up(uint32_t pos, uint2 x)
+    {
+        while (pos > 0) {
+            const uint32_t parent = (pos - 1) / 2;
+            const uint2 pe = e[parent + 1];
+            if (val(x) <= val(pe)) break;
+            e[pos + 1] = pe;
+            pos = parent;
+        }
+        e[pos + 1] = x;
+    }
+    __device__ void push(float d, uint32_t i)
+    {
+        const uint32_t pos = len++;
+        sift_up(pos, make_uint2(i, __float_as_uint(d)));
+    }
+    __device__ void pop(float *od, uint32_t *oi)
+    {
+        len -= 1;
+        const uint2 last = e[len + 1];
+        if (len == 0) {
+            *od = val(last);
+            *oi = last.x;
+            return;
+        }
+        const uint2 root = e[1];
+        *od = val(root);
+        *oi = root.x;
+        const uint32_t end = len;
+        uint32_t pos = 0, child = 1;
+        while (end >= 2 && child < end - 1) {
+            const uint4 pr = *reinterpret_cast<const uint4 *>(&e[child + 1]);
+            const uint2 a = make_uint2(pr.x, pr.y), b = make_uint2(pr.z, pr.w);
+            if (val(a) <= val(b)) {
+                child += 1;
+                e[pos + 1] = b;
+            } else {
+                e[pos + 1] = a;
+            }
+            pos = child;
+            child = 2 * pos + 1;
+        }
+        if (end >= 1 && child == end - 1) {
+            e[pos + 1] = e[child + 1];
+            pos = child;
+        }
+        sift_up(pos, last);
+    }
+};
+
+
+// Exact replay of IVFPQIndex::search_knn_adc + node_search_k
+// (pq_idx.rs:408-477) for tied queries, with the heaps in LDS, split three
+// ways so that a tied query's probed lists are simulated in parallel:
+//   ivf_order_kernel: the coarse heap of pq_idx.rs:416-425 (or the given
+//     probes) -> order[t][p], np[t]; one wave per tied query;
+//   ivf_list_replay_kernel: one workgroup per (tied query, probed list):
+//     LUT, ADC, and the list's push-all / pop-to-k heap (pq_idx.rs:181-191)
+//     drained in pop order into seq[t][p] (a tie-free list by a sort);
+//   ivf_merge_replay_kernel: one wave per tied query pushes every list's
+//     sequence, in probe order, into the top-k heap (pq_idx.rs:426-437) and
+//     drains it (node_search_k, pq_idx.rs:458-477).
+__global__ __launch_bounds__(64) void ivf_order_kernel(
     const uint32_t *__restrict__ rows, const float *__restrict__ Dc, uint32_t ldd, uint32_t nlist,
-    uint32_t nprobe, const float *__restrict__ Qp, uint32_t dpad, const float *__restrict__ C,
-    const uint32_t *__restrict__ off, const uint32_t *__restrict__ ids,
+    uint32_t nprobe, const uint32_t *__restrict__ probes, uint32_t ldp,
+    const uint32_t *__restrict__ nprobe_q, uint32_t ldo, uint32_t *__restrict__ order,
+    uint32_t *__restrict__ np_out)
+{
+    extern __shared__ __align__(16) unsigned char io_smem[];
+    const uint32_t t = blockIdx.x, q = rows[t];
+    const int tid = threadIdx.x;
+    uint32_t *ord = order + (size_t)t * ldo;
+    if (probes) {
+        const uint32_t np_q = nprobe_q[q];
+        for (uint32_t p = tid; p < np_q; p += 64) ord[p] = probes[(size_t)q * ldp + p];
+        if (tid == 0) np_out[t] = np_q;
+        return;
+    }
+    PackedHeap coarse{reinterpret_cast<uint2 *>(io_smem), 0};  // [nlist + 1]
+    const float *drow = Dc + (size_t)q * ldd;
+    for (uint32_t i = 0; i < nlist; ++i) coarse.push(-drow[i], i);
+    uint32_t p = 0;
+    for (; p < nprobe && coarse.len > 0; ++p) {
+        float xd;
+        uint32_t l;
+        coarse.pop(&xd, &l);
+        if (tid == 0) ord[p] = l;
+    }
+    if (tid == 0) np_out[t] = p;
+}
+
+__global__ __launch_bounds__(RP_THREADS) void ivf_list_replay_kernel(
+    const uint32_t *__restrict__ rows, const float *__restrict__ Qp, uint32_t dpad,
+    const float *__restrict__ C, const uint32_t *__restrict__ off, const uint32_t *__restrict__ ids,
     const uint8_t *__restrict__ codes, const float *__restrict__ codebooks,
     const uint32_t *__restrict__ sub_b, const uint32_t *__restrict__ sub_e, uint32_t M,
     uint32_t KS, uint32_t sdm, uint32_t k, uint32_t Lcap, uint32_t Pcap,
-    const uint32_t *__restrict__ probes, uint32_t ldp, const uint32_t *__restrict__ nprobe_q,
-    uint64_t *out_idx, float *out_dist, uint32_t *out_count)
+    const uint32_t *__restrict__ order, uint32_t ldo, const uint32_t *__restrict__ np,
+    uint2 *__restrict__ seq, uint32_t *__restrict__ seq_cnt)
 {
-    extern __shared__ __align__(16) unsigned char rp_smem[];
-    __shared__ uint32_t s_tied, s_np;
-    const uint32_t q = rows[blockIdx.x];
+    extern __shared__ __align__(16) unsigned char lr_smem[];
+    __shared__ uint32_t s_tied;
+    const uint32_t p = blockIdx.x, t = blockIdx.y;
     const int tid = threadIdx.x;
-    const bool w0 = tid < 64;
-    unsigned long long *sk = reinterpret_cast<unsigned long long *>(rp_smem);  // [Pcap]
-    float *lut = reinterpret_cast<float *>(sk + Pcap);                         // [M*KS]
-    float *hd = lut + (size_t)M * KS;                                           // [Lcap]
-    uint32_t *hi = reinterpret_cast<uint32_t *>(hd + Lcap);                     // [Lcap]
-    float *cdd = reinterpret_cast<float *>(hi + Lcap);                          // [nlist]
-    uint32_t *cdi = reinterpret_cast<uint32_t *>(cdd + nlist);                  // [nlist]
-    float *gd = reinterpret_cast<float *>(cdi + nlist);                         // [k+1]
-    uint32_t *gi = reinterpret_cast<uint32_t *>(gd + (k + 1));                  // [k+1]
-    uint32_t *order = gi + (k + 1);                                             // [nprobe]
-    // probe order
-    if (w0) {
-        uint32_t np_q = nprobe;
-        if (probes) {
-            np_q = nprobe_q[q];
-            for (uint32_t p = tid; p < np_q; p += 64) order[p] = probes[(size_t)q * ldp + p];
-        } else {
-            GHeap coarse{cdd, cdi, 0};
-            const float *drow = Dc + (size_t)q * ldd;
-            for (uint32_t i = 0; i < nlist; ++i) coarse.push(-drow[i], i);
-            uint32_t p = 0;
-            for (; p < np_q && coarse.len > 0; ++p) {
-                float xd;
-                uint32_t l;
-                coarse.pop(&xd, &l);
-                order[p] = l;
-            }
-            np_q = p;
+    uint2 *out = seq + ((size_t)t * ldo + p) * k;
+    if (p >= np[t]) {
+        if (tid == 0) seq_cnt[(size_t)t * ldo + p] = 0;
+        return;
+    }
+    const uint32_t q = rows[t], l = order[(size_t)t * ldo + p];
+    unsigned long long *sk = reinterpret_cast<unsigned long long *>(lr_smem);  // [Pcap]
+    uint2 *he = reinterpret_cast<uint2 *>(sk + ((Pcap + 1) & ~1u));            // [Lcap + 1], 16-byte aligned
+    float *lut = reinterpret_cast<float *>(he + Lcap + 2);                     // [M*KS]
+    const float *qv = Qp + (size_t)q * dpad;
+    const float *cb = codebooks + (size_t)l * M * KS * sdm;
+    for (uint32_t e = tid; e < M * KS; e += RP_THREADS) {
+        const uint32_t sub = e / KS, b = sub_b[sub], sd = sub_e[sub] - b;
+        float acc = -0.0f;
+        for (uint32_t j = 0; j < sd; ++j) {
+            float zj = qv[b + j] - C[(size_t)l * dpad + b + j];
+            float df = zj - cb[(size_t)e * sdm + j];
+            acc = acc + df * df;
         }
-        if (tid == 0) s_np = np_q;
+        lut[e] = acc;
+    }
+    if (tid == 0) s_tied = 0;
+    __syncthreads();
+    const uint32_t base = off[l], L = off[l + 1] - base;
+    uint32_t P = 1;
+    while (P < L) P <<= 1;
+    for (uint32_t i = tid; i < P; i += RP_THREADS) {
+        if (i < L) {
+            const uint8_t *cdp = codes + (size_t)(base + i) * M;
+            float dist = 0.0f;
+            for (uint32_t j = 0; j < M; ++j) dist = dist + lut[j * KS + cdp[j]];
+            he[i + 1] = make_uint2(i, __float_as_uint(dist));
+            sk[i] = ((unsigned long long)fkey(dist) << 32) | i;
+        } else {
+            sk[i] = ~0ull;
+        }
     }
     __syncthreads();
-    const uint32_t np_q = s_np;
-    GHeap top{gd, gi, 0};  // used by wave 0 only
-    const float *qv = Qp + (size_t)q * dpad;
-    for (uint32_t p = 0; p < np_q; ++p) {
-        const uint32_t l = order[p];
-        const float *cb = codebooks + (size_t)l * M * KS * sdm;
-        for (uint32_t e = tid; e < M * KS; e += RP_THREADS) {
-            const uint32_t sub = e / KS, b = sub_b[sub], sd = sub_e[sub] - b;
-            float acc = -0.0f;
-            for (uint32_t j = 0; j < sd; ++j) {
-                float zj = qv[b + j] - C[(size_t)l * dpad + b + j];
-                float df = zj - cb[(size_t)e * sdm + j];
-                acc = acc + df * df;
+    for (uint32_t size = 2; size <= P; size <<= 1)
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t u = tid; u < P / 2; u += RP_THREADS) {
+                const uint32_t a0 = 2 * stride * (u / stride) + (u % stride), a1 = a0 + stride;
+                const bool asc = (a0 & size) == 0;
+                const unsigned long long a = sk[a0], bb = sk[a1];
+                if ((a > bb) == asc) {
+                    sk[a0] = bb;
+                    sk[a1] = a;
+                }
             }
-            lut[e] = acc;
+            __syncthreads();
         }
-        if (tid == 0) s_tied = 0;
-        __syncthreads();
-        const uint32_t base = off[l], L = off[l + 1] - base;
-        uint32_t P = 1;
-        while (P < L) P <<= 1;
-        for (uint32_t i = tid; i < P; i += RP_THREADS) {
-            if (i < L) {
-                const uint8_t *cdp = codes + (size_t)(base + i) * M;
-                float dist = 0.0f;
-                for (uint32_t j = 0; j < M; ++j) dist = dist + lut[j * KS + cdp[j]];
-                hd[i] = dist;
-                hi[i] = i;
-                sk[i] = ((unsigned long long)fkey(dist) << 32) | i;
-            } else {
-                sk[i] = ~0ull;
-            }
+    const uint32_t K1 = k + 1 < L ? k + 1 : L;
+    for (uint32_t j = tid; j + 1 < K1; j += RP_THREADS)
+        if ((uint32_t)(sk[j] >> 32) == (uint32_t)(sk[j + 1] >> 32)) s_tied = 1;
+    __syncthreads();
+    if (tid >= 64) return;
+    const uint32_t m = k < L ? k : L;
+    if (!s_tied) {
+        // survivors: the m smallest, popped in descending order
+        for (uint32_t j = tid; j < m; j += 64) {
+            const unsigned long long v = sk[m - 1 - j];
+            out[j] = make_uint2(__float_as_uint(unkey((uint32_t)(v >> 32))),
+                                ids[base + (uint32_t)v]);
         }
-        __syncthreads();
-        for (uint32_t size = 2; size <= P; size <<= 1)
-            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                for (uint32_t t = tid; t < P / 2; t += RP_THREADS) {
-                    const uint32_t a0 = 2 * stride * (t / stride) + (t % stride), a1 = a0 + stride;
-                    const bool asc = (a0 & size) == 0;
-                    const unsigned long long a = sk[a0], bb = sk[a1];
-                    if ((a > bb) == asc) {
-                        sk[a0] = bb;
-                        sk[a1] = a;
-                    }
-                }
-                __syncthreads();
-            }
-        const uint32_t K1 = k + 1 < L ? k + 1 : L;
-        for (uint32_t j = tid; j + 1 < K1; j += RP_THREADS)
-            if ((uint32_t)(sk[j] >> 32) == (uint32_t)(sk[j + 1] >> 32)) s_tied = 1;
-        __syncthreads();
-        if (w0) {
-            const uint32_t m = k < L ? k : L;
-            if (!s_tied) {
-                // survivors: the m smallest, popped in descending order
-                for (uint32_t j = m; j > 0; --j) {
-                    const unsigned long long v = sk[j - 1];
-                    top.push(unkey((uint32_t)(v >> 32)), ids[base + (uint32_t)v]);
-                    if (top.len > k) {
-                        float a;
-                        uint32_t b;
-                        top.pop(&a, &b);
-                    }
-                }
-            } else {
-                GHeap lh{hd, hi, 0};
-                for (uint32_t i = 0; i < L; ++i) {  // item i sits at position i until pushed
-                    const float x = hd[i];
-                    lh.len = i;
-                    lh.push(x, i);
-                }
-                while (lh.len > k) {
-                    float a;
-                    uint32_t b;
-                    lh.pop(&a, &b);
-                }
-                while (lh.len > 0) {
-                    float a;
-                    uint32_t b;
-                    lh.pop(&a, &b);
-                    top.push(a, ids[base + b]);
-                    if (top.len > k) {
-                        float a2;
-                        uint32_t b2;
-                        top.pop(&a2, &b2);
-                    }
-                }
-            }
+    } else {
+        PackedHeap lh{he, 0};
+        for (uint32_t i = 0; i < L; ++i) {  // item i sits at position i until pushed
+            const uint2 x = he[i + 1];
+            lh.len = i + 1;
+            lh.sift_up(i, x);
         }
-        __syncthreads();
-    }
-    if (w0) {
-        const uint32_t cnt = top.len;
-        for (uint32_t p = cnt; p > 0; --p) {
+        while (lh.len > k) {
             float a;
             uint32_t b;
-            top.pop(&a, &b);
-            if (tid == 0) {
-                out_idx[(size_t)q * k + p - 1] = b;
-                out_dist[(size_t)q * k + p - 1] = a;
-            }
+            lh.pop(&a, &b);
         }
-        const float inf = __int_as_float(0x7f800000);
-        for (uint32_t j = cnt + tid; j < k; j += 64) {
-            out_idx[(size_t)q * k + j] = ~0ull;
-            out_dist[(size_t)q * k + j] = inf;
+        for (uint32_t j = 0; lh.len > 0; ++j) {
+            float a;
+            uint32_t b;
+            lh.pop(&a, &b);
+            if (tid == 0) out[j] = make_uint2(__float_as_uint(a), ids[base + b]);
         }
-        if (tid == 0) out_count[q] = cnt;
     }
+    if (tid == 0) seq_cnt[(size_t)t * ldo + p] = m;
 }
 
+__global__ __launch_bounds__(64) void ivf_merge_replay_kernel(
+    const uint32_t *__restrict__ rows, uint32_t k, uint32_t ldo, const uint32_t *__restrict__ np,
+    const uint2 *__restrict__ seq, const uint32_t *__restrict__ seq_cnt,
+    uint64_t *__restrict__ out_idx, float *__restrict__ out_dist, uint32_t *__restrict__ out_count)
+{
+    extern __shared__ __align__(16) unsigned char mr_smem[];
+    const uint32_t t = blockIdx.x, q = rows[t];
+    const int tid = threadIdx.x;
+    PackedHeap top{reinterpret_cast<uint2 *>(mr_smem), 0};  // [k + 2]
+    const uint32_t np_q = np[t];
+    for (uint32_t p = 0; p < np_q; ++p) {
+        const uint2 *sq = seq + ((size_t)t * ldo + p) * k;
+        const uint32_t m = seq_cnt[(size_t)t * ldo + p];
+        // 64 entries per load round, handed out through register reads
+        for (uint32_t j0 = 0; j0 < m; j0 += 64) {
+            const uint2 v = j0 + tid < m ? sq[j0 + tid] : make_uint2(0u, 0u);
+            const uint32_t cnt = min(64u, m - j0);
+            for (uint32_t u = 0; u < cnt; ++u) {
+                const uint32_t xd = (uint32_t)__builtin_amdgcn_readlane((int)v.x, (int)u);
+                const uint32_t xi = (uint32_t)__builtin_amdgcn_readlane((int)v.y, (int)u);
+                top.push(__uint_as_float(xd), xi);
+                if (top.len > k) {
+                    float a;
+                    uint32_t b;
+                    top.pop(&a, &b);
+                }
+            }
+        }
+    }
+    const uint32_t cnt = top.len;
+    for (uint32_t p = cnt; p > 0; --p) {
+        float a;
+        uint32_t b;
+        top.pop(&a, &b);
+        if (tid == 0) {
+            out_idx[(size_t)q * k + p - 1] = b;
+            out_dist[(size_t)q * k + p - 1] = a;
+        }
+    }
+    const float inf = __int_as_float(0x7f800000);
+    for (uint32_t j = cnt + tid; j < k; j += 64) {
+        out_idx[(size_t)q * k + j] = ~0ull;
+        out_dist[(size_t)q * k + j] = inf;
+    }
+    if (tid == 0) out_count[q] = cnt;
+}
+
+// The same replay in global memory for lists too long for LDS, one lane per
+// query: coarse heap of Neighbor(i, -dist) over all lists, nprobe pops, per
+// list an unbounded heap trimmed to k, popped into the global heap trimmed
+// to k.
 __global__ void ivf_replay_kernel(const uint32_t *__restrict__ rows, uint32_t nrows,
                                   const float *__restrict__ Dc, uint32_t ldd, uint32_t nlist,
                                   uint32_t nprobe, const float *__restrict__ Qp, uint32_t dpad,
@@ -1315,12 +1395,6 @@ void launch_ivf_replay(const uint32_t *rows, uint32_t nrows, const float *Dc, ui
     HM_CHECK_LAUNCH();
 }
 
-static size_t replay_lds_bytes(uint32_t nlist, uint32_t nprobe, uint32_t M, uint32_t KS,
-                               uint32_t k, uint32_t Lcap, uint32_t Pcap)
-{
-    return (size_t)Pcap * 8 + (size_t)M * KS * 4 + (size_t)Lcap * 8 + (size_t)nlist * 8 +
-           (size_t)(k + 1) * 8 + (size_t)nprobe * 4;
-}
 
 bool launch_ivf_replay_lds(const uint32_t *rows, uint32_t nrows, const float *Dc, uint32_t ldd,
                            uint32_t nlist, uint32_t nprobe, const float *Qp, uint32_t dpad,
@@ -1328,21 +1402,33 @@ bool launch_ivf_replay_lds(const uint32_t *rows, uint32_t nrows, const float *Dc
                            const uint8_t *codes, const float *codebooks, const uint32_t *sub_b,
                            const uint32_t *sub_e, uint32_t M, uint32_t KS, uint32_t sdm,
                            uint32_t k, uint32_t lmax, const uint32_t *probes, uint32_t ldp,
-                           const uint32_t *nprobe_q, uint64_t *out_idx, float *out_dist,
+                           const uint32_t *nprobe_q, uint32_t *order, uint32_t *np,
+                           uint2 *seq, uint32_t *seq_cnt, uint64_t *out_idx, float *out_dist,
                            uint32_t *out_count, hipStream_t s)
 {
     if (nrows == 0) return true;
+    if (nrows > 65535) return false;
     const uint32_t Lcap = std::max<uint32_t>(lmax, 1);
     uint32_t Pcap = 1;
     while (Pcap < Lcap) Pcap <<= 1;
     hipFuncAttributes attr{};
-    HM_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(ivf_replay_lds_kernel)));
-    const size_t lds = replay_lds_bytes(nlist, nprobe, M, KS, k, Lcap, Pcap);
-    if (lds + attr.sharedSizeBytes > 160 * 1024) return false;  // the caller uses the global-memory form
+    HM_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(ivf_list_replay_kernel)));
+    const size_t lds = (size_t)((Pcap + 1) & ~1u) * 8 + ((size_t)Lcap + 2) * 8 + (size_t)M * KS * 4;
+    const size_t lds_order = probes ? 0 : ((size_t)nlist + 2) * 8;
+    if (lds + attr.sharedSizeBytes > 160 * 1024 || lds_order > 160 * 1024 ||
+        (size_t)(k + 3) * 8 > 160 * 1024)
+        return false;  // the caller uses the global-memory form
     ProfScope prof("ivf_replay", s);
-    ivf_replay_lds_kernel<<<nrows, RP_THREADS, lds, s>>>(
-        rows, Dc, ldd, nlist, nprobe, Qp, dpad, C, off, ids, codes, codebooks, sub_b, sub_e, M,
-        KS, sdm, k, Lcap, Pcap, probes, ldp, nprobe_q, out_idx, out_dist, out_count);
+    const uint32_t ldo = nprobe;
+    ivf_order_kernel<<<nrows, 64, lds_order, s>>>(rows, Dc, ldd, nlist, nprobe, probes, ldp,
+                                                  nprobe_q, ldo, order, np);
+    HM_CHECK_LAUNCH();
+    ivf_list_replay_kernel<<<dim3(nprobe, nrows), RP_THREADS, lds, s>>>(
+        rows, Qp, dpad, C, off, ids, codes, codebooks, sub_b, sub_e, M, KS, sdm, k, Lcap, Pcap,
+        order, ldo, np, seq, seq_cnt);
+    HM_CHECK_LAUNCH();
+    ivf_merge_replay_kernel<<<nrows, 64, (size_t)(k + 3) * 8, s>>>(rows, k, ldo, np, seq, seq_cnt,
+                                                                  out_idx, out_dist, out_count);
     HM_CHECK_LAUNCH();
     return true;
 }

@@ -487,13 +487,19 @@ class IVFPQ final : public Index {
         rp_rows_.reserve(m * 4);
         HM_HIP(hipMemcpyAsync(rp_rows_.get(), tied.data(), m * 4, hipMemcpyHostToDevice, s));
         const uint32_t np_max = learned ? ldp : nprobe_;
+        rp_order_.reserve((size_t)m * np_max * 4);
+        rp_np_.reserve((size_t)m * 4);
+        rp_seq_.reserve((size_t)m * np_max * k * 8);
+        rp_scnt_.reserve((size_t)m * np_max * 4);
         if (launch_ivf_replay_lds(rp_rows_.as<uint32_t>(), m, D_.as<float>(), (uint32_t)ldd,
                                   nlist_, np_max, qpad_.as<float>(), dpad_, centers_.as<float>(),
                                   off_.as<uint32_t>(), ids_.as<uint32_t>(), codes_.as<uint8_t>(),
                                   codebooks_.as<float>(), sub_b_.as<uint32_t>(),
                                   sub_e_.as<uint32_t>(), M_, KS_, sdm_, k, max_list_,
                                   learned ? probe_i_.as<uint32_t>() : nullptr, ldp,
-                                  learned ? nprobe_q_.as<uint32_t>() : nullptr, d_idx, d_dist,
+                                  learned ? nprobe_q_.as<uint32_t>() : nullptr,
+                                  rp_order_.as<uint32_t>(), rp_np_.as<uint32_t>(),
+                                  rp_seq_.as<uint2>(), rp_scnt_.as<uint32_t>(), d_idx, d_dist,
                                   d_count, s)) {
             HM_HIP(hipStreamSynchronize(s));
             return;
@@ -611,7 +617,7 @@ class IVFPQ final : public Index {
     DeviceBuffer qpad_, qnorm_, D_, probe_d_, probe_i_, cflags_, mflags_, cand_, err_;
     DeviceBuffer pair_cnt_, pair_fill_, pair_off_, unit_off_, pairs_;
     int scan_mode_ = 0;  // 0: list-major when it fits, 1: per (query, probe) workgroups
-    DeviceBuffer rp_rows_, rp_f_, rp_u_;
+    DeviceBuffer rp_rows_, rp_f_, rp_u_, rp_order_, rp_np_, rp_seq_, rp_scnt_;
     // learned probe
     std::unique_ptr<ProbeMLP> mlp_;
     int32_t probe_mode_ = 0;
