@@ -196,67 +196,129 @@ __global__ __launch_bounds__(PA_THREADS) void pq_assign_kernel(
     out[(size_t)i * si + (size_t)s * ss] = (uint8_t)best;
 }
 
-// kmeans.rs:88-125 (update_center): centre c = the f32 (or f64) sum of its
-// members' sub-vectors in item order from +0.0, divided by their count; an
-// empty centre becomes zero.  One workgroup per sub-space; lane c owns
-// centre c; items are staged through LDS in chunks of PU_CHUNK, and the
-// dimensions in slabs of PU_W (each slab is an independent fold).
-constexpr int PU_THREADS = 256;
-constexpr int PU_CHUNK = 512;
-constexpr int PU_W = 32;
+// kmeans.rs:88-125 (update_center) in four parallel steps over all
+// sub-spaces.  The sums must run over each centre's members in item order,
+// so the items are first grouped by centre, stably:
+//   pq_chunk_hist: centre counts per chunk of PU_CH items;
+//   pq_chunk_scan: per centre, the exclusive prefix of those counts over
+//     the chunks, the centre's size, and the centres' base offsets;
+//   pq_scatter: item i goes to base[c] + prefix[chunk][c] + (items before
+//     it in its chunk with the same centre);
+//   pq_fold: one lane per (sub-space, centre, dimension) adds its members'
+//     values in item order from +0.0 (f32 or f64) and divides by the count;
+//     an empty centre becomes zero.
+constexpr int PU_CH = 256;
+
+__global__ __launch_bounds__(PU_CH) void pq_chunk_hist_kernel(const uint8_t *__restrict__ asg,
+                                                              uint32_t n, uint32_t KS,
+                                                              uint32_t nch,
+                                                              uint32_t *__restrict__ ccnt)
+{
+    __shared__ uint32_t hist[256];
+    const uint32_t ch = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
+    for (uint32_t c = tid; c < KS; c += PU_CH) hist[c] = 0;
+    __syncthreads();
+    const uint32_t i = ch * PU_CH + tid;
+    if (i < n) atomicAdd(&hist[asg[(size_t)s * n + i]], 1u);
+    __syncthreads();
+    for (uint32_t c = tid; c < KS; c += PU_CH) ccnt[((size_t)s * nch + ch) * KS + c] = hist[c];
+}
+
+__global__ __launch_bounds__(256) void pq_chunk_scan_kernel(const uint32_t *__restrict__ ccnt,
+                                                            uint32_t nch, uint32_t KS,
+                                                            uint32_t *__restrict__ coff,
+                                                            uint32_t *__restrict__ cbase,
+                                                            uint32_t *__restrict__ counts)
+{
+    __shared__ uint32_t tot[256];
+    const uint32_t s = blockIdx.x, c = threadIdx.x;
+    uint32_t run = 0;
+    if (c < KS) {
+        const uint32_t *src = ccnt + (size_t)s * nch * KS + c;
+        uint32_t *dst = coff + (size_t)s * nch * KS + c;
+        uint32_t ch = 0;
+        for (; ch + 8 <= nch; ch += 8) {
+            uint32_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = src[(size_t)(ch + u) * KS];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                dst[(size_t)(ch + u) * KS] = run;
+                run += v[u];
+            }
+        }
+        for (; ch < nch; ++ch) {
+            const uint32_t v = src[(size_t)ch * KS];
+            dst[(size_t)ch * KS] = run;
+            run += v;
+        }
+        counts[(size_t)s * KS + c] = run;
+    }
+    tot[c] = c < KS ? run : 0u;
+    __syncthreads();
+    if (c == 0) {
+        uint32_t acc = 0;
+        for (uint32_t j = 0; j < KS; ++j) {
+            cbase[(size_t)s * (KS + 1) + j] = acc;
+            acc += tot[j];
+        }
+        cbase[(size_t)s * (KS + 1) + KS] = acc;
+    }
+}
+
+__global__ __launch_bounds__(PU_CH) void pq_scatter_kernel(const uint8_t *__restrict__ asg,
+                                                           uint32_t n, uint32_t KS, uint32_t nch,
+                                                           const uint32_t *__restrict__ coff,
+                                                           const uint32_t *__restrict__ cbase,
+                                                           uint32_t *__restrict__ members)
+{
+    __shared__ uint32_t a[PU_CH];
+    const uint32_t ch = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
+    const uint32_t i = ch * PU_CH + tid;
+    a[tid] = i < n ? asg[(size_t)s * n + i] : 0xFFFFFFFFu;
+    __syncthreads();
+    if (i >= n) return;
+    const uint32_t c = a[tid];
+    uint32_t rank = 0;
+    for (uint32_t t = 0; t < tid; ++t) rank += a[t] == c ? 1u : 0u;
+    const uint32_t pos = cbase[(size_t)s * (KS + 1) + c] + coff[((size_t)s * nch + ch) * KS + c] + rank;
+    members[(size_t)s * n + pos] = i;
+}
 
 template <typename ACC_T>
-__global__ __launch_bounds__(PU_THREADS) void pq_update_kernel(
+__global__ __launch_bounds__(256) void pq_fold_kernel(
     const float *__restrict__ X, uint32_t ldx, uint32_t n, const uint32_t *__restrict__ sub_b,
-    const uint32_t *__restrict__ sub_e, uint32_t KS, uint32_t sdm,
-    const uint8_t *__restrict__ asg, float *__restrict__ cb, uint32_t *__restrict__ counts)
+    const uint32_t *__restrict__ sub_e, uint32_t M, uint32_t KS, uint32_t sdm,
+    const uint32_t *__restrict__ cbase, const uint32_t *__restrict__ members,
+    float *__restrict__ cb)
 {
-    __shared__ __align__(16) float zs[PU_CHUNK * PU_W];
-    __shared__ uint8_t as[PU_CHUNK];
-    const uint32_t s = blockIdx.x;
+    const uint32_t e = blockIdx.x * 256 + threadIdx.x;  // (s, c, j) with j < sdm
+    if (e >= M * KS * sdm) return;
+    const uint32_t j = e % sdm, sc = e / sdm, c = sc % KS, s = sc / KS;
     const uint32_t b = sub_b[s], sd = sub_e[s] - b;
-    const uint8_t *a = asg + (size_t)s * n;
-    const uint32_t c = threadIdx.x;
-    for (uint32_t t0 = 0; t0 < sd; t0 += PU_W) {
-        const uint32_t w = min((uint32_t)PU_W, sd - t0);
-        ACC_T acc[PU_W];
+    if (j >= sd) return;
+    const uint32_t lo = cbase[(size_t)s * (KS + 1) + c], hi = cbase[(size_t)s * (KS + 1) + c + 1];
+    const uint32_t *mem = members + (size_t)s * n;
+    const float *col = X + b + j;
+    ACC_T acc = 0;
+    uint32_t m = lo;
+    for (; m + 8 <= hi; m += 8) {
+        uint32_t id[8];
+        float v[8];
 #pragma unroll
-        for (int j = 0; j < PU_W; ++j) acc[j] = 0;
-        uint32_t n_c = 0;
-        for (uint32_t p0 = 0; p0 < n; p0 += PU_CHUNK) {
-            const uint32_t cnt = min((uint32_t)PU_CHUNK, n - p0);
-            for (uint32_t e = threadIdx.x; e < cnt * w; e += PU_THREADS) {
-                const uint32_t i = e / w, j = e % w;
-                zs[i * PU_W + j] = X[(size_t)(p0 + i) * ldx + b + t0 + j];
-            }
-            for (uint32_t e = threadIdx.x; e < cnt; e += PU_THREADS) as[e] = a[p0 + e];
-            __syncthreads();
-            if (c < KS) {
-                for (uint32_t i = 0; i < cnt; ++i) {
-                    if (as[i] == c) {
-                        ++n_c;
+        for (int u = 0; u < 8; ++u) id[u] = mem[m + u];
 #pragma unroll
-                        for (int j = 0; j < PU_W; ++j)
-                            if ((uint32_t)j < w) acc[j] = acc[j] + (ACC_T)zs[i * PU_W + j];
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        if (c < KS) {
-            float *dst = cb + ((size_t)s * KS + c) * sdm + t0;
+        for (int u = 0; u < 8; ++u) v[u] = col[(size_t)id[u] * ldx];
 #pragma unroll
-            for (int j = 0; j < PU_W; ++j) {
-                if ((uint32_t)j >= w) continue;
-                float v;
-                if (n_c == 0) v = 0.0f;
-                else if constexpr (sizeof(ACC_T) == 4) v = acc[j] / (float)n_c;
-                else v = (float)(acc[j] / (double)n_c);
-                dst[j] = v;
-            }
-            if (t0 == 0) counts[(size_t)s * KS + c] = n_c;
-        }
+        for (int u = 0; u < 8; ++u) acc = acc + (ACC_T)v[u];
     }
+    for (; m < hi; ++m) acc = acc + (ACC_T)col[(size_t)mem[m] * ldx];
+    const uint32_t cnt = hi - lo;
+    float r;
+    if (cnt == 0) r = 0.0f;
+    else if constexpr (sizeof(ACC_T) == 4) r = acc / (float)cnt;
+    else r = (float)(acc / (double)cnt);
+    cb[((size_t)s * KS + c) * sdm + j] = r;
 }
 
 // kmeans.rs:150-193 (split_center) with the seeded generator: one lane per
@@ -921,6 +983,13 @@ class PQ final : public Index {
         cbn_.reserve((size_t)M_ * KS_ * 4);
         asg_.reserve((size_t)M_ * n);
         counts_.reserve((size_t)M_ * KS_ * 4);
+        {
+            const size_t nch = ceil_div(n, PU_CH);
+            ccnt_.reserve((size_t)M_ * nch * KS_ * 4);
+            coff_.reserve((size_t)M_ * nch * KS_ * 4);
+            cbase_.reserve((size_t)M_ * (KS_ + 1) * 4);
+            members_.reserve((size_t)M_ * n * 4);
+        }
         codes_.reserve(n * M_ + 4);
         rng_.reserve((size_t)M_ * 8);
         std::vector<uint64_t> rs(M_);
@@ -944,16 +1013,26 @@ class PQ final : public Index {
             assign(asg_.as<uint8_t>(), 1, n, s);
             {
                 ProfScope ps("pq_update", s);
+                const uint32_t nn = (uint32_t)n, nch = ceil_div(nn, PU_CH);
+                pq_chunk_hist_kernel<<<dim3(nch, M_), PU_CH, 0, s>>>(asg_.as<uint8_t>(), nn, KS_,
+                                                                     nch, ccnt_.as<uint32_t>());
+                pq_chunk_scan_kernel<<<M_, 256, 0, s>>>(ccnt_.as<uint32_t>(), nch, KS_,
+                                                        coff_.as<uint32_t>(), cbase_.as<uint32_t>(),
+                                                        counts_.as<uint32_t>());
+                pq_scatter_kernel<<<dim3(nch, M_), PU_CH, 0, s>>>(
+                    asg_.as<uint8_t>(), nn, KS_, nch, coff_.as<uint32_t>(), cbase_.as<uint32_t>(),
+                    members_.as<uint32_t>());
+                const unsigned fb = ceil_div((size_t)M_ * KS_ * sdm_, 256);
                 if (fp64)
-                    pq_update_kernel<double><<<M_, PU_THREADS, 0, s>>>(
-                        base_.as<float>(), dpad_, (uint32_t)n, sub_b_.as<uint32_t>(),
-                        sub_e_.as<uint32_t>(), KS_, sdm_, asg_.as<uint8_t>(), cb_.as<float>(),
-                        counts_.as<uint32_t>());
+                    pq_fold_kernel<double><<<fb, 256, 0, s>>>(
+                        base_.as<float>(), dpad_, nn, sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(),
+                        M_, KS_, sdm_, cbase_.as<uint32_t>(), members_.as<uint32_t>(),
+                        cb_.as<float>());
                 else
-                    pq_update_kernel<float><<<M_, PU_THREADS, 0, s>>>(
-                        base_.as<float>(), dpad_, (uint32_t)n, sub_b_.as<uint32_t>(),
-                        sub_e_.as<uint32_t>(), KS_, sdm_, asg_.as<uint8_t>(), cb_.as<float>(),
-                        counts_.as<uint32_t>());
+                    pq_fold_kernel<float><<<fb, 256, 0, s>>>(
+                        base_.as<float>(), dpad_, nn, sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(),
+                        M_, KS_, sdm_, cbase_.as<uint32_t>(), members_.as<uint32_t>(),
+                        cb_.as<float>());
                 HM_CHECK_LAUNCH();
             }
             if (ep + 1 < epochs) {
@@ -986,6 +1065,7 @@ class PQ final : public Index {
     int32_t metric_ = HM_METRIC_UNKNOWN;
     bool trained_ = false;
     DeviceBuffer base_, sub_b_, sub_e_, cb_, cbn_, asg_, counts_, codes_, rng_;
+    DeviceBuffer ccnt_, coff_, cbase_, members_;  // update: items grouped by centre
     DeviceBuffer qpad_, D_, cand_d_, cand_i_, flags_, err_, lutg_, rp_rows_, rp_g_;
 };
 
