@@ -307,15 +307,15 @@ def main_ivfpq(args):
     if scan_n:
         info = ix.export()
         sizes = np.diff(info["list_offsets"].astype(np.int64))
-        # algorithmic bytes per scan launch: the codes (16 B) and ids (4 B) of
-        # every probed list of every query, plus each distinct probed list's
-        # codebook (16 x 256 x 8 f32) once
+        # algorithmic bytes per scan launch: the codes (16 B) of every item of
+        # every probed list of every query and its ADC distance written (4 B),
+        # plus each distinct probed list's codebook (16 x 256 x 8 f32) once
         pr = res[pick]["_probes"]
         per_launch = NQ * args.steps / scan_n / NQ
         bytes_ = per_launch * (float(sizes[pr].sum()) * (16 + 4) +
                                len(np.unique(pr)) * 16 * 256 * 8 * 4)
         ach = bytes_ / (scan_ms / scan_n / 1e3) / 1e9
-        roofline = {"bound": "hbm", "kernel": "ivf_scan_kernel", "achieved": round(ach, 1),
+        roofline = {"bound": "hbm", "kernel": "ivf_adc_rows_kernel", "achieved": round(ach, 1),
                     "peak": HBM_PEAK, "unit": "GB/s", "frac": round(ach / HBM_PEAK, 4),
                     "traffic": None, "avg_launch_ms": round(scan_ms / scan_n, 3)}
     for v in res.values():

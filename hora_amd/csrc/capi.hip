@@ -494,6 +494,15 @@ HM_API int32_t hm_ivfpq_set_probe_policy(hm_index *h, int32_t mode, float mass, 
     });
 }
 
+HM_API int32_t hm_ivfpq_set_scan_form(hm_index *h, int32_t form)
+{
+    return guard([&] {
+        need(h, "index");
+        std::lock_guard<std::mutex> lk(h->mu);
+        ivfpq_set_scan_form(h->impl.get(), form);
+    });
+}
+
 HM_API int32_t hm_ivfpq_mean_nprobe(hm_index *h, double *out)
 {
     return guard([&] {

@@ -54,6 +54,28 @@ void launch_ivf_scan_lists(const float *Qp, uint32_t dpad, const float *C,
                            uint32_t k1, const uint32_t *nprobe_q, uint32_t *pair_cnt,
                            uint32_t *pair_off, uint32_t *pair_fill, uint32_t *unit_off,
                            uint32_t *pairs, unsigned long long *out, hipStream_t s);
+// Row form of the scan: D[q][seg[q*nprobe + p] + i] = ADC of item i of the
+// p-th probed list of query q (pq_idx.rs:151-194 arithmetic), row_n[q] =
+// the row's length.  Needs a codebook that fits in registers:
+// ivf_adc_rows_variant != 0.  D has stride ldd >= nprobe * (longest list).
+int ivf_adc_rows_variant(uint32_t M, uint32_t KS, uint32_t sdm);
+void launch_ivf_adc_rows(const float *Qp, uint32_t dpad, const float *C, const uint32_t *probes,
+                         uint32_t ldp, uint32_t nprobe, uint32_t rows, uint32_t nlist,
+                         const uint32_t *off, const uint8_t *codes, const float *codebooks,
+                         const uint32_t *sub_b, const uint32_t *sub_e, uint32_t M, uint32_t KS,
+                         uint32_t sdm, const uint32_t *nprobe_q, uint32_t *pair_cnt,
+                         uint32_t *pair_off, uint32_t *pair_fill, uint32_t *unit_off,
+                         uint32_t *pairs, uint32_t *seg, uint32_t *row_n, float *D, uint64_t ldd,
+                         hipStream_t s);
+// The row select's candidates (columns) -> ids, counts and ROW_TIE flags,
+// as launch_ivf_merge.
+void launch_ivf_rows_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ldc,
+                              const uint32_t *sel_flags, const uint32_t *coarse_flags,
+                              const uint32_t *probes, uint32_t ldp, uint32_t nprobe,
+                              const uint32_t *nprobe_q, const uint32_t *seg,
+                              const uint32_t *row_n, const uint32_t *off, const uint32_t *ids,
+                              uint32_t rows, uint32_t k, uint64_t *out_idx, float *out_dist,
+                              uint32_t *out_count, uint32_t *flags, hipStream_t s);
 void launch_ivf_merge(const unsigned long long *cand, uint32_t per_query, uint32_t rows,
                       uint32_t k, const uint32_t *coarse_flags, uint64_t *out_idx,
                       float *out_dist, uint32_t *out_count, uint32_t *flags, hipStream_t s);

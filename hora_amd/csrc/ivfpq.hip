@@ -440,8 +440,10 @@ __global__ void pair_fill_kernel(const uint32_t *__restrict__ probes, uint32_t l
          e += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t q = (uint32_t)(e / nprobe), p = (uint32_t)(e % nprobe);
         if (nprobe_q && p >= nprobe_q[q]) {  // this query probes fewer lists
-            unsigned long long *o = out + e * k1;
-            for (uint32_t j = 0; j < k1; ++j) o[j] = ~0ull;
+            if (out) {
+                unsigned long long *o = out + e * k1;
+                for (uint32_t j = 0; j < k1; ++j) o[j] = ~0ull;
+            }
             continue;
         }
         const uint32_t l = probes[(size_t)q * ldp + p];
@@ -659,6 +661,174 @@ __global__ __launch_bounds__(LM_THREADS) void ivf_scan_lists_kernel(
         }
         unsigned long long *o = out + (size_t)e * k1;
         for (uint32_t j = tid; j < k1; j += LM_THREADS) o[j] = j < kept ? keys[j] : ~0ull;
+    }
+}
+
+// ---- Row form of the list scan: the ADC distance of every item of every
+// probed list is written to the query's row of D (the lists in probe order,
+// each list's items in list order), and the k+1 smallest of each row come
+// from the row select.  seg[q * nprobe + p] = the column of the first item
+// of the query's p-th probed list, row_n[q] = the row's length.
+__global__ void ivf_seg_kernel(const uint32_t *__restrict__ probes, uint32_t ldp, uint32_t nprobe,
+                               uint32_t rows, const uint32_t *__restrict__ nprobe_q,
+                               const uint32_t *__restrict__ off, uint32_t *__restrict__ seg,
+                               uint32_t *__restrict__ row_n)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= rows) return;
+    const uint32_t np_q = nprobe_q ? nprobe_q[q] : nprobe;
+    uint32_t acc = 0;
+    for (uint32_t p = 0; p < np_q; ++p) {
+        const uint32_t l = probes[(size_t)q * ldp + p];
+        seg[(size_t)q * nprobe + p] = acc;
+        acc += off[l + 1] - off[l];
+    }
+    row_n[q] = acc;
+}
+
+// Work units as in ivf_scan_lists_kernel (up to G pairs of one list).  The
+// list's codebook is held in registers: thread t owns the LUT entries
+// t + r * LM_THREADS (r < R), each with its sub-vector of up to SD floats.
+// Per pair: the residual z = q - c_l in LDS, the LUT entries (the same fold
+// as ivf_scan_kernel: -0.0 plus (z - w)^2 per dimension in order), then the
+// ADC of every item (0.0 plus the LUT values in sub-space order) into D.
+template <int R, int SD>
+__global__ __launch_bounds__(LM_THREADS, 2) void ivf_adc_rows_kernel(
+    const float *__restrict__ Qp, uint32_t dpad, const float *__restrict__ C,
+    const uint32_t *__restrict__ pair_off, const uint32_t *__restrict__ unit_off, uint32_t nlist,
+    uint32_t G, const uint32_t *__restrict__ pairs, uint32_t nprobe,
+    const uint32_t *__restrict__ off, const uint8_t *__restrict__ codes,
+    const float *__restrict__ codebooks, const uint32_t *__restrict__ sub_b,
+    const uint32_t *__restrict__ sub_e, uint32_t M, uint32_t KS, uint32_t sdm,
+    const uint32_t *__restrict__ seg, float *__restrict__ D, uint64_t ldd)
+{
+    const uint32_t u = blockIdx.x;
+    if (u >= unit_off[nlist]) return;
+    uint32_t lo = 0, hi = nlist;  // the list l with unit_off[l] <= u < unit_off[l + 1]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (unit_off[mid] <= u) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t l = lo;
+    const uint32_t p0 = pair_off[l] + (u - unit_off[l]) * G;
+    const uint32_t p1 = min(p0 + G, pair_off[l + 1]);
+    extern __shared__ __align__(16) unsigned char ar_smem[];
+    const uint32_t nent = M * KS;
+    float *lut = reinterpret_cast<float *>(ar_smem);  // [nent]
+    float *z = lut + ((nent + 3) & ~3u);               // [dpad]
+    const uint32_t tid = threadIdx.x;
+    const float *cb = codebooks + (size_t)l * nent * sdm;
+    float w[R][SD];
+    uint32_t eb[R], esd[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t t = tid + (uint32_t)r * LM_THREADS;
+        eb[r] = 0;
+        esd[r] = 0;
+        if (t < nent) {
+            const uint32_t sub = t / KS;
+            eb[r] = sub_b[sub];
+            esd[r] = sub_e[sub] - eb[r];
+        }
+#pragma unroll
+        for (int j = 0; j < SD; ++j) w[r][j] = (t < nent && (uint32_t)j < sdm) ? cb[(size_t)t * sdm + j] : 0.0f;
+    }
+    const uint32_t base = off[l], L = off[l + 1] - base;
+    for (uint32_t pp = p0; pp < p1; ++pp) {
+        const uint32_t e = pairs[pp], q = e / nprobe;
+        for (uint32_t j = tid; j < dpad; j += LM_THREADS)
+            z[j] = Qp[(size_t)q * dpad + j] - C[(size_t)l * dpad + j];
+        __syncthreads();  // z written; every thread done reading the last pair's LUT
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t t = tid + (uint32_t)r * LM_THREADS;
+            float acc = -0.0f;
+#pragma unroll
+            for (int j = 0; j < SD; ++j) {
+                if ((uint32_t)j < esd[r]) {
+                    const float df = z[eb[r] + j] - w[r][j];
+                    acc = acc + df * df;
+                }
+            }
+            if (t < nent) lut[t] = acc;
+        }
+        __syncthreads();  // LUT complete; z free for the next pair
+        float *drow = D + (size_t)q * ldd + seg[e];
+        for (uint32_t i = tid; i < L; i += LM_THREADS) {
+            const size_t row = (size_t)base + i;
+            float dist = 0.0f;
+            if ((M & 15) == 0) {
+                const uint4 *cv = reinterpret_cast<const uint4 *>(codes + row * M);
+                for (uint32_t v4 = 0; v4 < M / 16; ++v4) {
+                    const uint4 c = cv[v4];
+                    const uint32_t wd[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) {
+                        const uint32_t s0 = 16 * v4 + 4 * h;
+                        dist = dist + lut[s0 * KS + (wd[h] & 0xFFu)];
+                        dist = dist + lut[(s0 + 1) * KS + ((wd[h] >> 8) & 0xFFu)];
+                        dist = dist + lut[(s0 + 2) * KS + ((wd[h] >> 16) & 0xFFu)];
+                        dist = dist + lut[(s0 + 3) * KS + (wd[h] >> 24)];
+                    }
+                }
+            } else if ((M & 3) == 0) {
+                const uint32_t *cw = reinterpret_cast<const uint32_t *>(codes + row * M);
+                for (uint32_t v = 0; v < M / 4; ++v) {
+                    const uint32_t c = cw[v];
+                    dist = dist + lut[(4 * v) * KS + (c & 0xFFu)];
+                    dist = dist + lut[(4 * v + 1) * KS + ((c >> 8) & 0xFFu)];
+                    dist = dist + lut[(4 * v + 2) * KS + ((c >> 16) & 0xFFu)];
+                    dist = dist + lut[(4 * v + 3) * KS + (c >> 24)];
+                }
+            } else {
+                const uint8_t *cd = codes + row * M;
+                for (uint32_t j = 0; j < M; ++j) dist = dist + lut[j * KS + cd[j]];
+            }
+            drow[i] = dist;
+        }
+    }
+}
+
+// Per query, from the row select's first K = min(k+1, row_n) candidates
+// (columns, ascending by (distance, column)): the first min(k, row_n) as
+// item ids, ROW_TIE when the K are not pairwise distinct (the select's flag)
+// or the coarse probe set was tied.  One wave per query.
+__global__ __launch_bounds__(64) void ivf_rows_finalize_kernel(
+    const float *__restrict__ cand_d, const uint32_t *__restrict__ cand_i, uint32_t ldc,
+    const uint32_t *__restrict__ sel_flags, const uint32_t *__restrict__ coarse_flags,
+    const uint32_t *__restrict__ probes, uint32_t ldp, uint32_t nprobe,
+    const uint32_t *__restrict__ nprobe_q, const uint32_t *__restrict__ seg,
+    const uint32_t *__restrict__ row_n, const uint32_t *__restrict__ off,
+    const uint32_t *__restrict__ ids, uint32_t k, uint64_t *__restrict__ out_idx,
+    float *__restrict__ out_dist, uint32_t *__restrict__ out_count, uint32_t *__restrict__ flags)
+{
+    const uint32_t q = blockIdx.x, tid = threadIdx.x;
+    const uint32_t n = row_n[q], m = k < n ? k : n;
+    const uint32_t np_q = nprobe_q ? nprobe_q[q] : nprobe;
+    const uint32_t *sg = seg + (size_t)q * nprobe;
+    const float inf = __int_as_float(0x7f800000);
+    for (uint32_t j = tid; j < k; j += 64) {
+        if (j < m) {
+            const uint32_t col = cand_i[(size_t)q * ldc + j];
+            uint32_t a = 0, b = np_q;  // the last probe p with sg[p] <= col
+            while (b - a > 1) {
+                const uint32_t mid = (a + b) / 2;
+                if (sg[mid] <= col) a = mid;
+                else b = mid;
+            }
+            const uint32_t l = probes[(size_t)q * ldp + a];
+            out_idx[(size_t)q * k + j] = ids[off[l] + (col - sg[a])];
+            out_dist[(size_t)q * k + j] = cand_d[(size_t)q * ldc + j];
+        } else {
+            out_idx[(size_t)q * k + j] = ~0ull;
+            out_dist[(size_t)q * k + j] = inf;
+        }
+    }
+    if (tid == 0) {
+        out_count[q] = m;
+        const uint32_t f = sel_flags[q];
+        flags[q] = ((f & (ROW_TIE | ROW_BAD)) || (coarse_flags[q] & ROW_TIE)) ? ROW_TIE : 0u;
     }
 }
 
@@ -1330,6 +1500,90 @@ uint32_t ivf_scan_lists_pool(uint32_t M, uint32_t KS, uint32_t sdm, uint32_t dpa
     return (P >= 2 * k1 && lists_lds(M, KS, sdm, dpad, P) <= budget) ? P : 0;
 }
 
+// (query, probe) pairs bucketed by list: pairs[pair_off[l] ..] = q * nprobe
+// + p, and work units of up to G pairs of one list (unit_off).  Returns G.
+static uint32_t build_pairs(const uint32_t *probes, uint32_t ldp, uint32_t nprobe, uint32_t rows,
+                            uint32_t nlist, const uint32_t *nprobe_q, uint32_t *pair_cnt,
+                            uint32_t *pair_off, uint32_t *pair_fill, uint32_t *unit_off,
+                            uint32_t *pairs, unsigned long long *out, uint32_t k1, hipStream_t s)
+{
+    const uint64_t total = (uint64_t)rows * nprobe;
+    // about 2048 work units for the batch, whatever the list skew
+    const uint32_t G = (uint32_t)std::max<uint64_t>(8, (total + 2047) / 2048);
+    const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 4096);
+    const unsigned fb = std::min<unsigned>(ceil_div(nlist, 256), 1024);
+    fill_u32_small_kernel<<<fb, 256, 0, s>>>(pair_cnt, nlist, 0u);
+    fill_u32_small_kernel<<<fb, 256, 0, s>>>(pair_fill, nlist, 0u);
+    pair_count_kernel<<<blocks, 256, 0, s>>>(probes, ldp, nprobe, rows, nprobe_q, pair_cnt);
+    pair_scan_kernel<<<1, 1024, 0, s>>>(pair_cnt, nlist, G, pair_off, unit_off);
+    pair_fill_kernel<<<blocks, 256, 0, s>>>(probes, ldp, nprobe, rows, nprobe_q, pair_off,
+                                            pair_fill, pairs, out, k1);
+    HM_CHECK_LAUNCH();
+    return G;
+}
+
+int ivf_adc_rows_variant(uint32_t M, uint32_t KS, uint32_t sdm)
+{
+    const uint64_t nent = (uint64_t)M * KS;
+    if (sdm <= 4 && nent <= 16u * LM_THREADS) return 16;
+    if (sdm <= 8 && nent <= 8u * LM_THREADS) return 8;
+    if (sdm <= 16 && nent <= 4u * LM_THREADS) return 4;
+    if (sdm <= 32 && nent <= 2u * LM_THREADS) return 2;
+    return 0;
+}
+
+void launch_ivf_adc_rows(const float *Qp, uint32_t dpad, const float *C, const uint32_t *probes,
+                         uint32_t ldp, uint32_t nprobe, uint32_t rows, uint32_t nlist,
+                         const uint32_t *off, const uint8_t *codes, const float *codebooks,
+                         const uint32_t *sub_b, const uint32_t *sub_e, uint32_t M, uint32_t KS,
+                         uint32_t sdm, const uint32_t *nprobe_q, uint32_t *pair_cnt,
+                         uint32_t *pair_off, uint32_t *pair_fill, uint32_t *unit_off,
+                         uint32_t *pairs, uint32_t *seg, uint32_t *row_n, float *D, uint64_t ldd,
+                         hipStream_t s)
+{
+    if (rows == 0 || nprobe == 0) return;
+    const int R = ivf_adc_rows_variant(M, KS, sdm);
+    if (!R) fail(HM_ERR_INVALID_ARG, "row-form scan: codebook does not fit in registers");
+    const uint64_t total = (uint64_t)rows * nprobe;
+    if (total > 0x7FFFFFFFull) fail(HM_ERR_INVALID_ARG, "too many probe pairs");
+    const size_t lds = (((size_t)M * KS + 3) & ~(size_t)3) * 4 + (size_t)dpad * 4;
+    if (lds > 64 * 1024) fail(HM_ERR_INVALID_ARG, "row-form scan: LUT does not fit in LDS");
+    ProfScope prof("ivf_scan", s);
+    ivf_seg_kernel<<<ceil_div(rows, 256), 256, 0, s>>>(probes, ldp, nprobe, rows, nprobe_q, off,
+                                                       seg, row_n);
+    const uint32_t G = build_pairs(probes, ldp, nprobe, rows, nlist, nprobe_q, pair_cnt, pair_off,
+                                   pair_fill, unit_off, pairs, nullptr, 0, s);
+    const unsigned units = (unsigned)((total + G - 1) / G + nlist);  // >= sum of ceil(cnt_l / G)
+#define HM_ADC_ROWS(RR, SDD)                                                                    \
+    ivf_adc_rows_kernel<RR, SDD><<<units, LM_THREADS, lds, s>>>(                                \
+        Qp, dpad, C, pair_off, unit_off, nlist, G, pairs, nprobe, off, codes, codebooks, sub_b, \
+        sub_e, M, KS, sdm, seg, D, ldd)
+    switch (R) {
+    case 16: HM_ADC_ROWS(16, 4); break;
+    case 8: HM_ADC_ROWS(8, 8); break;
+    case 4: HM_ADC_ROWS(4, 16); break;
+    default: HM_ADC_ROWS(2, 32); break;
+    }
+#undef HM_ADC_ROWS
+    HM_CHECK_LAUNCH();
+}
+
+void launch_ivf_rows_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ldc,
+                              const uint32_t *sel_flags, const uint32_t *coarse_flags,
+                              const uint32_t *probes, uint32_t ldp, uint32_t nprobe,
+                              const uint32_t *nprobe_q, const uint32_t *seg,
+                              const uint32_t *row_n, const uint32_t *off, const uint32_t *ids,
+                              uint32_t rows, uint32_t k, uint64_t *out_idx, float *out_dist,
+                              uint32_t *out_count, uint32_t *flags, hipStream_t s)
+{
+    if (rows == 0) return;
+    ProfScope prof("ivf_merge", s);
+    ivf_rows_finalize_kernel<<<rows, 64, 0, s>>>(cand_d, cand_i, ldc, sel_flags, coarse_flags,
+                                                 probes, ldp, nprobe, nprobe_q, seg, row_n, off,
+                                                 ids, k, out_idx, out_dist, out_count, flags);
+    HM_CHECK_LAUNCH();
+}
+
 void launch_ivf_scan_lists(const float *Qp, uint32_t dpad, const float *C,
                            const uint32_t *probes, uint32_t ldp, uint32_t nprobe, uint32_t rows,
                            uint32_t nlist, const uint32_t *off, const uint32_t *ids,
@@ -1346,16 +1600,8 @@ void launch_ivf_scan_lists(const float *Qp, uint32_t dpad, const float *C,
     const uint64_t total = (uint64_t)rows * nprobe;
     if (total > 0x7FFFFFFFull) fail(HM_ERR_INVALID_ARG, "too many probe pairs");
     ProfScope prof("ivf_scan", s);
-    // about 2048 work units for the batch, whatever the list skew
-    const uint32_t G = (uint32_t)std::max<uint64_t>(8, (total + 2047) / 2048);
-    const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 4096);
-    const unsigned fb = std::min<unsigned>(ceil_div(nlist, 256), 1024);
-    fill_u32_small_kernel<<<fb, 256, 0, s>>>(pair_cnt, nlist, 0u);
-    fill_u32_small_kernel<<<fb, 256, 0, s>>>(pair_fill, nlist, 0u);
-    pair_count_kernel<<<blocks, 256, 0, s>>>(probes, ldp, nprobe, rows, nprobe_q, pair_cnt);
-    pair_scan_kernel<<<1, 1024, 0, s>>>(pair_cnt, nlist, G, pair_off, unit_off);
-    pair_fill_kernel<<<blocks, 256, 0, s>>>(probes, ldp, nprobe, rows, nprobe_q, pair_off,
-                                            pair_fill, pairs, out, k1);
+    const uint32_t G = build_pairs(probes, ldp, nprobe, rows, nlist, nprobe_q, pair_cnt, pair_off,
+                                   pair_fill, unit_off, pairs, out, k1, s);
     const uint64_t units_max = (total + G - 1) / G + nlist;  // sum of ceil(cnt_l / G)
     ivf_scan_lists_kernel<<<(unsigned)units_max, LM_THREADS, lists_lds(M, KS, sdm, dpad, P), s>>>(
         Qp, dpad, C, pair_off, unit_off, nlist, G, pairs, nprobe, off, ids, codes, codebooks,

@@ -279,11 +279,30 @@ class IVFPQ final : public Index {
         probe_i_.reserve(qc * Kc * 4);
         cflags_.reserve(qc * 4);
         mflags_.reserve(qc * 4);
-        cand_.reserve(qc * per_query * 8);
-        // list-major scan when a list's codebook fits in LDS (one read of
-        // each probed codebook per chunk instead of one per query and probe)
-        const bool list_major = scan_mode_ != 1 && ivf_scan_lists_pool(M_, KS_, sdm_, dpad_, k1) != 0;
-        if (list_major) {
+
+        // Scan forms, fastest first: rows (list-major, codebook in registers,
+        // ADC distances into D rows, radix row select), list-major with the
+        // codebook in LDS and a per-list top-k1, and one workgroup per
+        // (query, probe).
+        const bool rows_ok = ivf_adc_rows_variant(M_, KS_, sdm_) != 0 && k1 <= SELECT_MAX_K;
+        const bool lists_ok = ivf_scan_lists_pool(M_, KS_, sdm_, dpad_, k1) != 0;
+        if ((scan_mode_ == 3 && !rows_ok) || (scan_mode_ == 2 && !lists_ok))
+            fail(HM_ERR_INVALID_ARG, "the chosen scan form does not fit this index");
+        const bool rows_form = (scan_mode_ == 0 || scan_mode_ == 3) && rows_ok;
+        const bool list_major = !rows_form && (scan_mode_ == 0 || scan_mode_ == 2) && lists_ok;
+        const uint64_t ldr = round_up(std::max<uint64_t>((uint64_t)np * max_list_, 1), 4);
+        if (rows_form) {
+            // rows of up to nprobe x (longest list) distances: at most 16 GiB
+            qc = std::min<uint64_t>(qc, std::max<uint64_t>(1, ((size_t)1 << 34) / (ldr * 4)));
+            Dr_.reserve(qc * ldr * 4);
+            seg_.reserve(qc * np * 4);
+            row_n_.reserve(qc * 4);
+            rcand_d_.reserve(qc * k1 * 4);
+            rcand_i_.reserve(qc * k1 * 4);
+            rflags_.reserve(qc * 4);
+        }
+        if (!rows_form) cand_.reserve(qc * per_query * 8);
+        if (list_major || rows_form) {
             pair_cnt_.reserve((size_t)nlist_ * 4);
             pair_fill_.reserve((size_t)nlist_ * 4);
             pair_off_.reserve(((size_t)nlist_ + 1) * 4);
@@ -329,7 +348,27 @@ class IVFPQ final : public Index {
                                   probe_i_.as<uint32_t>(), Kc, cflags_.as<uint32_t>(), s);
             }
             const uint32_t *npq_dev = learned ? nprobe_q_.as<uint32_t>() : nullptr;
-            if (list_major) {
+            if (rows_form) {
+                launch_ivf_adc_rows(qpad_.as<float>(), dpad_, centers_.as<float>(),
+                                    probe_i_.as<uint32_t>(), Kc, np, rows, nlist_,
+                                    off_.as<uint32_t>(), codes_.as<uint8_t>(),
+                                    codebooks_.as<float>(), sub_b_.as<uint32_t>(),
+                                    sub_e_.as<uint32_t>(), M_, KS_, sdm_, npq_dev,
+                                    pair_cnt_.as<uint32_t>(), pair_off_.as<uint32_t>(),
+                                    pair_fill_.as<uint32_t>(), unit_off_.as<uint32_t>(),
+                                    pairs_.as<uint32_t>(), seg_.as<uint32_t>(),
+                                    row_n_.as<uint32_t>(), Dr_.as<float>(), ldr, s);
+                launch_row_select(Dr_.as<float>(), ldr, rows, (uint32_t)ldr, k1,
+                                  rcand_d_.as<float>(), rcand_i_.as<uint32_t>(), k1,
+                                  rflags_.as<uint32_t>(), s, nullptr, row_n_.as<uint32_t>());
+                launch_ivf_rows_finalize(rcand_d_.as<float>(), rcand_i_.as<uint32_t>(), k1,
+                                         rflags_.as<uint32_t>(), cflags_.as<uint32_t>(),
+                                         probe_i_.as<uint32_t>(), Kc, np, npq_dev,
+                                         seg_.as<uint32_t>(), row_n_.as<uint32_t>(),
+                                         off_.as<uint32_t>(), ids_.as<uint32_t>(), rows, k,
+                                         d_idx + q0 * k, d_dist + q0 * k, d_count + q0,
+                                         mflags_.as<uint32_t>(), s);
+            } else if (list_major) {
                 launch_ivf_scan_lists(qpad_.as<float>(), dpad_, centers_.as<float>(),
                                       probe_i_.as<uint32_t>(), Kc, np, rows, nlist_,
                                       off_.as<uint32_t>(), ids_.as<uint32_t>(),
@@ -346,9 +385,10 @@ class IVFPQ final : public Index {
                                 sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), M_, KS_, sdm_, k1,
                                 npq_dev, cand_.as<unsigned long long>(), s);
             }
-            launch_ivf_merge(cand_.as<unsigned long long>(), per_query, rows, k,
-                             cflags_.as<uint32_t>(), d_idx + q0 * k, d_dist + q0 * k,
-                             d_count + q0, mflags_.as<uint32_t>(), s);
+            if (!rows_form)
+                launch_ivf_merge(cand_.as<unsigned long long>(), per_query, rows, k,
+                                 cflags_.as<uint32_t>(), d_idx + q0 * k, d_dist + q0 * k,
+                                 d_count + q0, mflags_.as<uint32_t>(), s);
             HM_HIP(hipMemcpyAsync(fl.data(), mflags_.get(), rows * 4, hipMemcpyDeviceToHost, s));
             if (learned)
                 HM_HIP(hipMemcpyAsync(npq.data(), nprobe_q_.get(), rows * 4,
@@ -397,6 +437,12 @@ class IVFPQ final : public Index {
             for (uint32_t j = 0; j < ldp; ++j)
                 probes[q * ldp + j] = j < last_kc_ ? pr[q * last_kc_ + j] : 0xFFFFFFFFu;
         }
+    }
+
+    void set_scan_form(int32_t form)
+    {
+        if (form < 0 || form > 3) fail(HM_ERR_INVALID_ARG, "scan form must be 0..3");
+        scan_mode_ = form;
     }
 
     void set_probe_policy(int32_t mode, float mass, uint32_t pmin, uint32_t pmax)
@@ -616,7 +662,10 @@ class IVFPQ final : public Index {
         off_, codebooks_, codes_, iota_, sort_tmp_;
     DeviceBuffer qpad_, qnorm_, D_, probe_d_, probe_i_, cflags_, mflags_, cand_, err_;
     DeviceBuffer pair_cnt_, pair_fill_, pair_off_, unit_off_, pairs_;
-    int scan_mode_ = 0;  // 0: list-major when it fits, 1: per (query, probe) workgroups
+    // 0: fastest form that fits; 1: per (query, probe); 2: list-major, top-k1
+    // per list; 3: rows (ADC distances into D rows, row select)
+    int scan_mode_ = 0;
+    DeviceBuffer Dr_, seg_, row_n_, rcand_d_, rcand_i_, rflags_;  // row-form scan
     DeviceBuffer rp_rows_, rp_f_, rp_u_, rp_order_, rp_np_, rp_seq_, rp_scnt_;
     // learned probe
     std::unique_ptr<ProbeMLP> mlp_;
@@ -666,6 +715,14 @@ int32_t ivfpq_export(Index *ix, float *centers, uint64_t *offsets, uint64_t *ids
     auto *p = dynamic_cast<IVFPQ *>(ix);
     if (!p) fail(HM_ERR_INVALID_ARG, "not an IVFPQIndex");
     p->export_trained(centers, offsets, ids, codebooks, codes);
+    return HM_OK;
+}
+
+int32_t ivfpq_set_scan_form(Index *ix, int32_t form)
+{
+    auto *p = dynamic_cast<IVFPQ *>(ix);
+    if (!p) fail(HM_ERR_INVALID_ARG, "not an IVFPQIndex");
+    p->set_scan_form(form);
     return HM_OK;
 }
 

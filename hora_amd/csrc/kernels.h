@@ -41,11 +41,14 @@ void launch_pair_dist(int metric, const float *Qp, uint32_t dpad, uint32_t nq, c
 // ---- select.hip
 // For each of `rows` rows of D (n values, row stride ldd): the K smallest by
 // (distance, index), ascending, into cand_d/cand_i (row stride ldc), and
-// ROW_TIE / ROW_NAN into flags[row].  K <= SELECT_MAX_K.
+// ROW_TIE / ROW_NAN into flags[row].  K <= SELECT_MAX_K.  row_n (optional,
+// device): per-row lengths <= n; a row shorter than K gives row_n[r]
+// candidates and fills the rest with (+inf, ~0).
 constexpr uint32_t SELECT_MAX_K = 8192;
 void launch_row_select(const float *D, uint64_t ldd, uint32_t rows, uint32_t n, uint32_t K,
                        float *cand_d, uint32_t *cand_i, uint32_t ldc, uint32_t *flags,
-                       hipStream_t s, const uint32_t *dev_rows = nullptr);
+                       hipStream_t s, const uint32_t *dev_rows = nullptr,
+                       const uint32_t *row_n = nullptr);
 
 // out[r] = the m-th smallest value of D's row r (1 <= m <= 32; NaN skipped).
 void launch_kth_smallest(const float *D, uint64_t ldd, uint32_t rows, uint32_t n, uint32_t m,

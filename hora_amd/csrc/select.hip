@@ -95,11 +95,15 @@ __device__ __forceinline__ void row_for_each(const float *__restrict__ row, uint
 
 template <bool VEC>
 __global__ __launch_bounds__(SEL_THREADS) void row_select_kernel(
-    const float *__restrict__ D, uint64_t ldd, uint32_t n, uint32_t K, float *__restrict__ cand_d,
-    uint32_t *__restrict__ cand_i, uint32_t ldc, uint32_t *__restrict__ flags,
-    const uint32_t *__restrict__ dev_rows)
+    const float *__restrict__ D, uint64_t ldd, uint32_t n_max, uint32_t K_max,
+    float *__restrict__ cand_d, uint32_t *__restrict__ cand_i, uint32_t ldc,
+    uint32_t *__restrict__ flags, const uint32_t *__restrict__ dev_rows,
+    const uint32_t *__restrict__ row_n)
 {
     if (dev_rows && blockIdx.x >= *dev_rows) return;  // graph replays with fewer rows
+    // ragged rows: this row's length and the candidates it can give
+    const uint32_t n = row_n ? min(row_n[blockIdx.x], n_max) : n_max;
+    const uint32_t K = min(K_max, n);
     __shared__ uint32_t hist[2048];
     __shared__ unsigned long long cand[SEL_CAP];
     __shared__ uint32_t s_w[SEL_THREADS / 64];
@@ -232,6 +236,10 @@ __global__ __launch_bounds__(SEL_THREADS) void row_select_kernel(
     float *od = cand_d + (size_t)blockIdx.x * ldc;
     uint32_t *oi = cand_i + (size_t)blockIdx.x * ldc;
     uint32_t tie = 0;
+    for (uint32_t j = K + tid; j < K_max; j += SEL_THREADS) {  // short ragged rows
+        oi[j] = ~0u;
+        od[j] = __int_as_float(0x7f800000);
+    }
     for (uint32_t j = tid; j < K; j += SEL_THREADS) {
         unsigned long long c = cand[j];
         uint32_t idx = (uint32_t)c;
@@ -759,18 +767,18 @@ __global__ void pad_rows_kernel(const float *__restrict__ src, uint64_t lds, flo
 
 void launch_row_select(const float *D, uint64_t ldd, uint32_t rows, uint32_t n, uint32_t K,
                        float *cand_d, uint32_t *cand_i, uint32_t ldc, uint32_t *flags,
-                       hipStream_t s, const uint32_t *dev_rows)
+                       hipStream_t s, const uint32_t *dev_rows, const uint32_t *row_n)
 {
     if (rows == 0 || n == 0 || K == 0) return;
-    if (K > SEL_CAP || K > n) fail(HM_ERR_INVALID_ARG, "row_select: K out of range");
+    if (K > SEL_CAP || (!row_n && K > n)) fail(HM_ERR_INVALID_ARG, "row_select: K out of range");
     if (ldd < n) fail(HM_ERR_INVALID_ARG, "row_select: row stride below row length");
     ProfScope prof("row_select", s);
     if (ldd % 4 == 0 && ((uintptr_t)D & 15) == 0)
         row_select_kernel<true><<<rows, SEL_THREADS, 0, s>>>(D, ldd, n, K, cand_d, cand_i, ldc,
-                                                             flags, dev_rows);
+                                                             flags, dev_rows, row_n);
     else
         row_select_kernel<false><<<rows, SEL_THREADS, 0, s>>>(D, ldd, n, K, cand_d, cand_i, ldc,
-                                                              flags, dev_rows);
+                                                              flags, dev_rows, row_n);
     HM_CHECK_LAUNCH();
 }
 

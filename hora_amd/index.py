@@ -465,6 +465,13 @@ class IVFPQIndex(_Index):
                                          epochs, batch, lr, seed, C.byref(loss)))
         return float(loss.value)
 
+    SCAN_FORMS = {"auto": 0, "per_probe": 1, "lists": 2, "rows": 3}
+
+    def set_scan_form(self, form: str) -> None:
+        """How search scans the probed lists (same results): "auto",
+        "per_probe", "lists" or "rows"."""
+        check(lib().hm_ivfpq_set_scan_form(self._h, self.SCAN_FORMS[form]))
+
     def set_probe_policy(self, mode: int, mass: float = 0.9, pmin: int = 1,
                          pmax: int = 64) -> None:
         """0: fixed search_n_center (reference); 1: learned per-query nprobe."""

@@ -274,6 +274,16 @@ HM_API int32_t hm_ivfpq_train_probe(hm_index *h, const float *train_queries, uin
 HM_API int32_t hm_ivfpq_set_probe_policy(hm_index *h, int32_t mode, float mass, uint32_t pmin,
                                          uint32_t pmax);
 
+/* How IVFPQ search scans the probed lists (the results are the same):
+ * 0 (default) the fastest form that fits; 1 one workgroup per (query,
+ * probed list); 2 list-major, the list's codebook in LDS and a top-(k+1)
+ * per list; 3 list-major with the codebook in registers, every ADC
+ * distance into the query's row and one radix select per row.  Forms 2 and 3
+ * fail at search time when the index's PQ shape does not fit them.  No
+ * counterpart in the reference (pq_idx.rs:408-455 has one scan loop); for
+ * tuning and for testing every form. */
+HM_API int32_t hm_ivfpq_set_scan_form(hm_index *h, int32_t form);
+
 /* Mean number of lists probed per query by the last search. */
 HM_API int32_t hm_ivfpq_mean_nprobe(hm_index *h, double *out);
 

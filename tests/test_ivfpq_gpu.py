@@ -30,6 +30,13 @@ def _lists(ex):
     return [ex["list_ids"][off[l]:off[l + 1]].astype(np.int64) for l in range(len(off) - 1)]
 
 
+def _same_valid(gi, gd, ri, rd, counts):
+    # entries past a row's count are unspecified fill in the oracle
+    for r, c in enumerate(counts):
+        assert np.array_equal(gi[r, :c], ri[r, :c]), f"ids of row {r}"
+        assert np.array_equal(gd[r, :c], rd[r, :c]), f"distances of row {r}"
+
+
 def _check_conditional(orc, idx, x, mt, q, nprobe, k):
     ex = idx.export()
     cen, cb, codes = ex["centers"], ex["codebooks"], ex["codes"]
@@ -54,8 +61,7 @@ def _check_conditional(orc, idx, x, mt, q, nprobe, k):
     ri, rd, rc, _ = orc.ivfpq_search(q, int(mt), cen, cb, ex["list_offsets"], ex["list_ids"],
                                      codes, nprobe, k)
     assert np.array_equal(gc, rc)
-    assert np.array_equal(gi, ri)
-    assert np.array_equal(gd, rd)
+    _same_valid(gi, gd, ri, rd, rc)
     return ex
 
 
@@ -120,37 +126,79 @@ def test_large_codebook_scan(orc):
     _check_conditional(orc, idx, x, Metric.Euclidean, q, 3, 10)
 
 
+@pytest.mark.parametrize("form", ["lists", "rows"])
 @pytest.mark.parametrize("n", [12_000, 40_000], ids=["registers", "chunked"])
-def test_long_lists_select(orc, n):
-    # 16 x 256 codewords of 8 dims leave room for a 1,024-key sorting pool,
-    # so lists of ~3,000 take the histogram select over register-held keys
-    # and lists of ~10,000 the chunked sort
+def test_long_lists_select(orc, n, form):
+    # lists form: 16 x 256 codewords of 8 dims leave room for a 1,024-key
+    # sorting pool, so lists of ~3,000 take the histogram select over
+    # register-held keys and lists of ~10,000 the chunked sort; rows form:
+    # rows of ~6,000 and ~20,000 distances through the radix row select
     x, _, _ = gaussian_mixture(n, 128, 4, seed=19, sigma=0.3)
     idx = _build(x, Metric.Euclidean, 4, 16, 8, 2, 29)
+    idx.set_scan_form(form)
     q, _, _ = gaussian_mixture(32, 128, 4, seed=20, sigma=0.3)
     _check_conditional(orc, idx, x, Metric.Euclidean, q, 2, 10)
 
 
-def test_long_lists_all_tied(orc):
+@pytest.mark.parametrize("form", ["lists", "rows"])
+def test_long_lists_all_tied(orc, form):
     # 3 distinct points repeated 1,500 times in one list: its ADC keys take
-    # 3 values, the threshold bin holds 1,500 > 1,024 keys, so the select
-    # falls back to the chunked sort, and every query goes through the
-    # heap replay
+    # 3 values (lists form: the threshold bin holds 1,500 > 1,024 keys, so
+    # the select falls back to the chunked sort), and every query goes
+    # through the heap replay
     base, _, _ = gaussian_mixture(3, 128, 3, seed=21, sigma=0.3)
     x = np.repeat(base, 1500, axis=0)
     idx = _build(x, Metric.Euclidean, 1, 16, 8, 2, 31, nprobe=1)
+    idx.set_scan_form(form)
     q, _, _ = gaussian_mixture(8, 128, 3, seed=22, sigma=0.3)
     _check_conditional(orc, idx, x, Metric.Euclidean, q, 1, 10)
+    assert idx.search_stats()[0] == 8
 
 
-def test_ties_replay(orc):
+@pytest.mark.parametrize("form", ["per_probe", "lists", "rows"])
+def test_ties_replay(orc, form):
     # every row twice: equal codes give equal ADC distances in every list
     x, _, _ = gaussian_mixture(800, 12, 6, seed=15, sigma=0.3)
     x = np.repeat(x, 2, axis=0)
     idx = _build(x, Metric.Euclidean, 8, 3, 4, 4, 5)
+    idx.set_scan_form(form)
     q, _, _ = gaussian_mixture(40, 12, 6, seed=16, sigma=0.3)
     _check_conditional(orc, idx, x, Metric.Euclidean, q, 3, 7)
     assert idx.search_stats()[0] > 0  # rows went through the replay
+
+
+@pytest.mark.parametrize("shape", [(64, 16, 8, 24), (32, 4, 4, 40), (48, 6, 5, 12),
+                                   (96, 3, 8, 6)],
+                         ids=["16x256x4", "4x16x8", "6x32x8", "3x256x32"])
+def test_scan_forms_agree(orc, shape):
+    # every scan form that fits gives the same results, tie flags included;
+    # k larger than some probed lists' total, and a k of 1
+    d, n_sub, bits, nlist = shape
+    x, cen, _ = gaussian_mixture(4000, d, nlist, seed=41, sigma=0.3)
+    idx = _build(x, Metric.Euclidean, nlist, n_sub, bits, 3, 43, nprobe=3)
+    q, _, _ = gaussian_mixture(70, d, nlist, seed=42, sigma=0.3, centres=cen)
+    ex = _check_conditional(orc, idx, x, Metric.Euclidean, q, 3, 10)
+    for k, nprobe in ((10, 3), (1, 1), (400, 2)):
+        idx.set_search_n_center(nprobe)
+        got = {}
+        for form in ("per_probe", "lists", "rows"):
+            idx.set_scan_form(form)
+            try:
+                got[form] = idx.search_batch(q, k) + (idx.search_stats()[0],)
+            except HoraError:
+                assert form != "per_probe"  # the codebook does not fit that form
+        assert "rows" in got or n_sub * (1 << bits) > 2048
+        ref = got["per_probe"]
+        for form, g in got.items():
+            for a, b in zip(ref, g):
+                assert np.array_equal(a, b), (form, k, nprobe)
+        ri, rd, rc, _ = orc.ivfpq_search(q, orc.EUCLIDEAN, ex["centers"], ex["codebooks"],
+                                         ex["list_offsets"], ex["list_ids"], ex["codes"],
+                                         nprobe, k)
+        assert np.array_equal(ref[2], rc)
+        _same_valid(ref[0], ref[1], ri, rd, rc)
+        assert (ref[2] > 0).mean() > 0.5  # most queries found neighbours
+    idx.set_scan_form("auto")
 
 
 def test_errors():

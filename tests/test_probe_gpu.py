@@ -56,12 +56,14 @@ def _recall(got, truth):
                           for i in range(len(truth))]))
 
 
-def test_learned_probe_parity_and_gain(orc):
+@pytest.mark.parametrize("form", ["auto", "lists"])
+def test_learned_probe_parity_and_gain(orc, form):
     x, cen, _ = gaussian_mixture(40_000, 32, 128, seed=21, aniso=(0.1, 0.5))
     train, _, _ = gaussian_mixture(20_000, 32, 128, seed=22, aniso=(0.1, 0.5), centres=cen)
     test, _, _ = gaussian_mixture(512, 32, 128, seed=23, aniso=(0.1, 0.5), centres=cen)
     nlist, k = 64, 10
     idx = _index(x, nlist, 4)
+    idx.set_scan_form(form)
     loss = idx.train_probe(train, label_k=k, hidden=128, epochs=6, batch=512, lr=2e-3, seed=3)
     assert np.isfinite(loss)
     truth, _, _ = orc.bf_search(x, test, k, orc.EUCLIDEAN)
