@@ -11,6 +11,7 @@
 #include "ivfpq.h"
 #include "pq_common.h"
 #include "profiler.h"
+#include "wave_heap.h"
 
 #pragma clang fp contract(off)
 
@@ -964,68 +965,10 @@ struct GHeap {
 
 constexpr int RP_THREADS = 256;
 
-// std BinaryHeap (heap.rs:21-68, 95-112) over packed (distance bits, item)
-// entries, position p stored at e[p + 1]: a sibling pair (2p+1, 2p+2) is one
-// aligned 16-byte read, so a level of sift_down_to_bottom is one LDS round
-// trip.  Every lane of the wave runs the same operations on the same words.
-struct PackedHeap {
-    uint2 *e;  // 16-byte aligned, capacity + 1 entries
-    uint32_t len;
-    __device__ static float val(uint2 x) { return __uint_as_float(x.y); }
-    __device__ void sift_up(uint32_t pos, uint2 x)
-    {
-        while (pos > 0) {
-            const uint32_t parent = (pos - 1) / 2;
-            const uint2 pe = e[parent + 1];
-            if (val(x) <= val(pe)) break;
-            e[pos + 1] = pe;
-            pos = parent;
-        }
-        e[pos + 1] = x;
-    }
-    __device__ void push(float d, uint32_t i)
-    {
-        const uint32_t pos = len++;
-        sift_up(pos, make_uint2(i, __float_as_uint(d)));
-    }
-    __device__ void pop(float *od, uint32_t *oi)
-    {
-        len -= 1;
-        const uint2 last = e[len + 1];
-        if (len == 0) {
-            *od = val(last);
-            *oi = last.x;
-            return;
-        }
-        const uint2 root = e[1];
-        *od = val(root);
-        *oi = root.x;
-        const uint32_t end = len;
-        uint32_t pos = 0, child = 1;
-        while (end >= 2 && child < end - 1) {
-            const uint4 pr = *reinterpret_cast<const uint4 *>(&e[child + 1]);
-            const uint2 a = make_uint2(pr.x, pr.y), b = make_uint2(pr.z, pr.w);
-            if (val(a) <= val(b)) {
-                child += 1;
-                e[pos + 1] = b;
-            } else {
-                e[pos + 1] = a;
-            }
-            pos = child;
-            child = 2 * pos + 1;
-        }
-        if (end >= 1 && child == end - 1) {
-            e[pos + 1] = e[child + 1];
-            pos = child;
-        }
-        sift_up(pos, last);
-    }
-};
-
-
 // Exact replay of IVFPQIndex::search_knn_adc + node_search_k
-// (pq_idx.rs:408-477) for tied queries, with the heaps in LDS, split three
-// ways so that a tied query's probed lists are simulated in parallel:
+// (pq_idx.rs:408-477) for tied queries, with the heaps in LDS (each run by
+// one wave as a PackedWaveHeap), split three ways so that a tied query's
+// probed lists are simulated in parallel:
 //   ivf_order_kernel: the coarse heap of pq_idx.rs:416-425 (or the given
 //     probes) -> order[t][p], np[t]; one wave per tied query;
 //   ivf_list_replay_kernel: one workgroup per (tied query, probed list):
@@ -1050,15 +993,18 @@ __global__ __launch_bounds__(64) void ivf_order_kernel(
         if (tid == 0) np_out[t] = np_q;
         return;
     }
-    PackedHeap coarse{reinterpret_cast<uint2 *>(io_smem), 0};  // [nlist + 1]
+    PackedWaveHeap coarse(reinterpret_cast<uint2 *>(io_smem), (uint32_t)tid);  // [nlist + 2]
     const float *drow = Dc + (size_t)q * ldd;
-    for (uint32_t i = 0; i < nlist; ++i) coarse.push(-drow[i], i);
+    for (uint32_t i0 = 0; i0 < nlist; i0 += 64) {  // 64 distances per load round
+        const float v = i0 + tid < nlist ? -drow[i0 + tid] : 0.0f;
+        const uint32_t cnt = min(64u, nlist - i0);
+        for (uint32_t u = 0; u < cnt; ++u)
+            coarse.push(make_uint2(i0 + u, (uint32_t)__builtin_amdgcn_readlane(__float_as_int(v), (int)u)));
+    }
     uint32_t p = 0;
     for (; p < nprobe && coarse.len > 0; ++p) {
-        float xd;
-        uint32_t l;
-        coarse.pop(&xd, &l);
-        if (tid == 0) ord[p] = l;
+        const uint2 x = coarse.pop();
+        if (tid == 0) ord[p] = x.x;
     }
     if (tid == 0) np_out[t] = p;
 }
@@ -1141,22 +1087,18 @@ __global__ __launch_bounds__(RP_THREADS) void ivf_list_replay_kernel(
                                 ids[base + (uint32_t)v]);
         }
     } else {
-        PackedHeap lh{he, 0};
-        for (uint32_t i = 0; i < L; ++i) {  // item i sits at position i until pushed
-            const uint2 x = he[i + 1];
-            lh.len = i + 1;
-            lh.sift_up(i, x);
+        // item i sits at position i until pushed, and a push writes only
+        // positions up to its own: 64 items are read per load round
+        PackedWaveHeap lh(he, (uint32_t)tid);
+        for (uint32_t i0 = 0; i0 < L; i0 += 64) {
+            const uint2 v = i0 + tid < L ? he[i0 + tid + 1] : make_uint2(0u, 0u);
+            const uint32_t cnt = min(64u, L - i0);
+            for (uint32_t u = 0; u < cnt; ++u) lh.push(wh_lane(v, u));
         }
-        while (lh.len > k) {
-            float a;
-            uint32_t b;
-            lh.pop(&a, &b);
-        }
+        while (lh.len > k) (void)lh.pop();
         for (uint32_t j = 0; lh.len > 0; ++j) {
-            float a;
-            uint32_t b;
-            lh.pop(&a, &b);
-            if (tid == 0) out[j] = make_uint2(__float_as_uint(a), ids[base + b]);
+            const uint2 x = lh.pop();
+            if (tid == 0) out[j] = make_uint2(x.y, ids[base + x.x]);
         }
     }
     if (tid == 0) seq_cnt[(size_t)t * ldo + p] = m;
@@ -1170,7 +1112,7 @@ __global__ __launch_bounds__(64) void ivf_merge_replay_kernel(
     extern __shared__ __align__(16) unsigned char mr_smem[];
     const uint32_t t = blockIdx.x, q = rows[t];
     const int tid = threadIdx.x;
-    PackedHeap top{reinterpret_cast<uint2 *>(mr_smem), 0};  // [k + 2]
+    PackedWaveHeap top(reinterpret_cast<uint2 *>(mr_smem), (uint32_t)tid);  // [k + 3]
     const uint32_t np_q = np[t];
     for (uint32_t p = 0; p < np_q; ++p) {
         const uint2 *sq = seq + ((size_t)t * ldo + p) * k;
@@ -1182,23 +1124,17 @@ __global__ __launch_bounds__(64) void ivf_merge_replay_kernel(
             for (uint32_t u = 0; u < cnt; ++u) {
                 const uint32_t xd = (uint32_t)__builtin_amdgcn_readlane((int)v.x, (int)u);
                 const uint32_t xi = (uint32_t)__builtin_amdgcn_readlane((int)v.y, (int)u);
-                top.push(__uint_as_float(xd), xi);
-                if (top.len > k) {
-                    float a;
-                    uint32_t b;
-                    top.pop(&a, &b);
-                }
+                top.push(make_uint2(xi, xd));
+                if (top.len > k) (void)top.pop();
             }
         }
     }
     const uint32_t cnt = top.len;
     for (uint32_t p = cnt; p > 0; --p) {
-        float a;
-        uint32_t b;
-        top.pop(&a, &b);
+        const uint2 x = top.pop();
         if (tid == 0) {
-            out_idx[(size_t)q * k + p - 1] = b;
-            out_dist[(size_t)q * k + p - 1] = a;
+            out_idx[(size_t)q * k + p - 1] = x.x;
+            out_dist[(size_t)q * k + p - 1] = __uint_as_float(x.y);
         }
     }
     const float inf = __int_as_float(0x7f800000);

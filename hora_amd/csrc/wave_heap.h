@@ -120,6 +120,96 @@ template <class S> struct WaveHeapT {
     }
 };
 
+// std BinaryHeap on packed (item, distance bits) entries in LDS, position p
+// at e[p + 1] (so a sibling pair is one aligned 16-byte word), run by one
+// whole wave.
+//   push: as WaveHeapT::push (all ancestors compared in one load round).
+//   pop: the reference's sift_down_to_bottom followed by sift_up leaves the
+//     same heap as a top-down sift of the last element that moves the larger
+//     child up (the right one when left <= right) while that child is not
+//     smaller than the element, and stops at a leaf: the children below the
+//     stop point move up on the way down and back on the way up.  Each round
+//     covers WH_R levels of that sift: lane l holds node l of the round's
+//     63-node subtree (l's parent is (l - 1) / 2) and loads its sibling pair
+//     of children.  Two ballots say where the sift stops and which way it
+//     turns; lane l is on the sift's path when none of its ancestors stops
+//     and each turns toward it (two mask compares against per-lane ancestor
+//     and direction masks), so the path's moves are one parallel store and
+//     the stopping lane stores the element.
+struct PackedWaveHeap {
+    uint2 *e;  // 16-byte aligned, capacity + 2 entries
+    uint32_t len;
+    uint32_t lane;
+    uint32_t lj, lo;    // level of node `lane` in a round, its offset in the level
+    uint64_t anc, dir;  // its ancestors, and those whose right child leads to it
+
+    __device__ PackedWaveHeap(uint2 *e_, uint32_t lane_) : e(e_), len(0), lane(lane_)
+    {
+        lj = 31u - (uint32_t)__clz(lane + 1);
+        lo = lane + 1 - (1u << lj);
+        anc = 0;
+        dir = 0;
+        for (uint32_t x = lane < 63 ? lane : 0; x > 0;) {
+            const uint32_t p = (x - 1) / 2;
+            anc |= 1ull << p;
+            if ((x & 1u) == 0) dir |= 1ull << p;
+            x = p;
+        }
+    }
+
+    __device__ static float val(uint2 x) { return __uint_as_float(x.y); }
+
+    // heap.rs:21-35.  Lane l compares with ancestor l + 1; lanes past the
+    // root read the unused slot e[0].
+    __device__ void push(uint2 x)
+    {
+        const uint32_t p = len++;
+        const uint32_t A = 31u - (uint32_t)__clz(p + 1);  // ancestors of p
+        const uint2 u = e[(p + 1) >> min(lane + 1, 31u)];
+        const uint64_t m = __ballot(lane < A && val(x) > val(u));
+        const uint32_t t = (uint32_t)__builtin_ctzll(~m);  // levels climbed
+        if (lane < t) e[(p + 1) >> lane] = u;               // ancestor lane + 1 moves down
+        if (lane == 0) e[(p + 1) >> t] = x;
+    }
+
+    // heap.rs:54-68, 95-112: remove and return the root (len >= 1).  The
+    // first round's loads go out with the reads of the root and the last
+    // element; every lane loads (a node without children reads the pair at
+    // the clamped index end & ~1, inside the array).
+    __device__ uint2 pop()
+    {
+        len -= 1;
+        const uint32_t end = len, ceven = end & ~1u;
+        uint32_t mypos = (1u << lj) - 1 + lo;  // round root 0
+        uint4 pr = *reinterpret_cast<const uint4 *>(&e[min(2 * mypos + 2, ceven)]);
+        const uint2 last = e[end + 1];
+        const uint2 root = e[1];
+        if (end == 0) return last;
+        const float lv = val(last);
+        for (;;) {
+            const bool ex = lane < 63 && 2 * mypos + 1 < end;
+            const uint2 a = make_uint2(pr.x, pr.y), b = make_uint2(pr.z, pr.w);
+            const bool right = ex && 2 * mypos + 2 < end && val(a) <= val(b);
+            const uint2 w = right ? b : a;
+            const bool st = !ex || lv > val(w);  // the sift stops at this node
+            const uint64_t ms = __ballot(st), mr = __ballot(right);
+            const bool onp = lane < 63 && (ms & anc) == 0 && (mr & anc) == dir;
+            if (onp && !st) e[mypos + 1] = w;
+            if (__ballot(onp && st)) {
+                if (onp && st) e[mypos + 1] = last;
+                break;
+            }
+            // no stop in this round: continue below the path's level-5 node
+            const uint32_t l5 = (uint32_t)__builtin_ctzll(__ballot(onp && lj == WH_R - 1));
+            const uint32_t child = 2 * mypos + 1 + (right ? 1u : 0u);
+            const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)child, (int)l5);
+            mypos = ((r0 + 1) << lj) - 1 + lo;
+            pr = *reinterpret_cast<const uint4 *>(&e[min(2 * mypos + 2, ceven)]);
+        }
+        return root;
+    }
+};
+
 // Storage: separate distance / item arrays (LDS or global).
 struct WhSplit {
     float *d;
