@@ -967,7 +967,7 @@ constexpr int RP_THREADS = 256;
 
 // Exact replay of IVFPQIndex::search_knn_adc + node_search_k
 // (pq_idx.rs:408-477) for tied queries, with the heaps in LDS (each run by
-// one wave as a PackedWaveHeap), split three ways so that a tied query's
+// one wave as a WaveHeap), split three ways so that a tied query's
 // probed lists are simulated in parallel:
 //   ivf_order_kernel: the coarse heap of pq_idx.rs:416-425 (or the given
 //     probes) -> order[t][p], np[t]; one wave per tied query;
@@ -993,7 +993,7 @@ __global__ __launch_bounds__(64) void ivf_order_kernel(
         if (tid == 0) np_out[t] = np_q;
         return;
     }
-    PackedWaveHeap coarse(reinterpret_cast<uint2 *>(io_smem), (uint32_t)tid);  // [nlist + 2]
+    WaveHeap<WhPackedLds> coarse({reinterpret_cast<uint2 *>(io_smem)}, (uint32_t)tid);  // [nlist + 2]
     const float *drow = Dc + (size_t)q * ldd;
     for (uint32_t i0 = 0; i0 < nlist; i0 += 64) {  // 64 distances per load round
         const float v = i0 + tid < nlist ? -drow[i0 + tid] : 0.0f;
@@ -1089,7 +1089,7 @@ __global__ __launch_bounds__(RP_THREADS) void ivf_list_replay_kernel(
     } else {
         // item i sits at position i until pushed, and a push writes only
         // positions up to its own: 64 items are read per load round
-        PackedWaveHeap lh(he, (uint32_t)tid);
+        WaveHeap<WhPackedLds> lh({he}, (uint32_t)tid);
         for (uint32_t i0 = 0; i0 < L; i0 += 64) {
             const uint2 v = i0 + tid < L ? he[i0 + tid + 1] : make_uint2(0u, 0u);
             const uint32_t cnt = min(64u, L - i0);
@@ -1112,7 +1112,7 @@ __global__ __launch_bounds__(64) void ivf_merge_replay_kernel(
     extern __shared__ __align__(16) unsigned char mr_smem[];
     const uint32_t t = blockIdx.x, q = rows[t];
     const int tid = threadIdx.x;
-    PackedWaveHeap top(reinterpret_cast<uint2 *>(mr_smem), (uint32_t)tid);  // [k + 3]
+    WaveHeap<WhPackedLds> top({reinterpret_cast<uint2 *>(mr_smem)}, (uint32_t)tid);  // [k + 3]
     const uint32_t np_q = np[t];
     for (uint32_t p = 0; p < np_q; ++p) {
         const uint2 *sq = seq + ((size_t)t * ldo + p) * k;

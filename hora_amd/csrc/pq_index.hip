@@ -572,6 +572,11 @@ template <bool HYB> struct WhHybrid {
         }
         lds[p] = v;
     }
+    __device__ uint4 pair(uint32_t first, uint32_t end) const
+    {
+        const uint2 a = ld(min(first, end)), b = ld(min(first + 1, end));
+        return make_uint4(a.x, a.y, b.x, b.y);
+    }
 };
 
 template <bool HYB>
@@ -587,13 +592,13 @@ __global__ __launch_bounds__(PR_THREADS) void pq_replay_kernel(
     for (uint32_t t = blockIdx.x; t < nrows; t += gridDim.x) {
         const uint32_t r = rows[t];
         const float *drow = D + (size_t)r * ldd;
-        WaveHeapT<WhHybrid<HYB>> h{{pr_heap, hg, H}, 0u, lane};
+        WaveHeap<WhHybrid<HYB>> h({pr_heap, hg, H}, lane);
         for (uint32_t i0 = 0; i0 < n; i0 += 64) {
             const float v = i0 + lane < n ? drow[i0 + lane] : 0.0f;
             const uint32_t cnt = min(64u, n - i0);
             for (uint32_t u = 0; u < cnt; ++u) {
                 const uint32_t bits = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(v), (int)u);
-                h.push(make_uint2(bits, i0 + u));
+                h.push(make_uint2(i0 + u, bits));
             }
         }
         while (h.len > k) (void)h.pop();
@@ -601,8 +606,8 @@ __global__ __launch_bounds__(PR_THREADS) void pq_replay_kernel(
         for (uint32_t p = m; p > 0; --p) {
             const uint2 x = h.pop();
             if (lane == 0) {
-                out_idx[(size_t)r * k + p - 1] = x.y;
-                out_dist[(size_t)r * k + p - 1] = __uint_as_float(x.x);
+                out_idx[(size_t)r * k + p - 1] = x.x;
+                out_dist[(size_t)r * k + p - 1] = __uint_as_float(x.y);
             }
         }
         __syncthreads();
