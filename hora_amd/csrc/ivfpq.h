@@ -59,8 +59,9 @@ void launch_ivf_scan_lists(const float *Qp, uint32_t dpad, const float *C,
 // the row's length.  Needs a codebook that fits in registers:
 // ivf_adc_rows_variant != 0.  D has stride ldd >= nprobe * (longest list).
 int ivf_adc_rows_variant(uint32_t M, uint32_t KS, uint32_t sdm);
-void launch_ivf_adc_rows(const float *Qp, uint32_t dpad, const float *C, const uint32_t *probes,
-                         uint32_t ldp, uint32_t nprobe, uint32_t rows, uint32_t nlist,
+void launch_ivf_adc_rows(const float *Qp, uint32_t dpad, uint32_t dim, const float *C,
+                         const uint32_t *probes, uint32_t ldp, uint32_t nprobe, uint32_t rows,
+                         uint32_t nlist,
                          const uint32_t *off, const uint8_t *codes, const float *codebooks,
                          const uint32_t *sub_b, const uint32_t *sub_e, uint32_t M, uint32_t KS,
                          uint32_t sdm, const uint32_t *nprobe_q, uint32_t *pair_cnt,

@@ -693,7 +693,7 @@ __global__ void ivf_seg_kernel(const uint32_t *__restrict__ probes, uint32_t ldp
 // Per pair: the residual z = q - c_l in LDS, the LUT entries (the same fold
 // as ivf_scan_kernel: -0.0 plus (z - w)^2 per dimension in order), then the
 // ADC of every item (0.0 plus the LUT values in sub-space order) into D.
-template <int R, int SD>
+template <int R, int SD, bool FULL>
 __global__ __launch_bounds__(LM_THREADS, 2) void ivf_adc_rows_kernel(
     const float *__restrict__ Qp, uint32_t dpad, const float *__restrict__ C,
     const uint32_t *__restrict__ pair_off, const uint32_t *__restrict__ unit_off, uint32_t nlist,
@@ -745,11 +745,27 @@ __global__ __launch_bounds__(LM_THREADS, 2) void ivf_adc_rows_kernel(
         for (int r = 0; r < R; ++r) {
             const uint32_t t = tid + (uint32_t)r * LM_THREADS;
             float acc = -0.0f;
+            if constexpr (FULL) {
+                // every sub-space SD wide (4-aligned): its residual slice
+                // as float4 reads, no per-dimension guard
+                const float4 *zp = reinterpret_cast<const float4 *>(z + eb[r]);
 #pragma unroll
-            for (int j = 0; j < SD; ++j) {
-                if ((uint32_t)j < esd[r]) {
-                    const float df = z[eb[r] + j] - w[r][j];
-                    acc = acc + df * df;
+                for (int j4 = 0; j4 < SD / 4; ++j4) {
+                    const float4 zv = zp[j4];
+                    const float zz[4] = {zv.x, zv.y, zv.z, zv.w};
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) {
+                        const float df = zz[h] - w[r][4 * j4 + h];
+                        acc = acc + df * df;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < SD; ++j) {
+                    if ((uint32_t)j < esd[r]) {
+                        const float df = z[eb[r] + j] - w[r][j];
+                        acc = acc + df * df;
+                    }
                 }
             }
             if (t < nent) lut[t] = acc;
@@ -1468,8 +1484,9 @@ int ivf_adc_rows_variant(uint32_t M, uint32_t KS, uint32_t sdm)
     return 0;
 }
 
-void launch_ivf_adc_rows(const float *Qp, uint32_t dpad, const float *C, const uint32_t *probes,
-                         uint32_t ldp, uint32_t nprobe, uint32_t rows, uint32_t nlist,
+void launch_ivf_adc_rows(const float *Qp, uint32_t dpad, uint32_t dim, const float *C,
+                         const uint32_t *probes, uint32_t ldp, uint32_t nprobe, uint32_t rows,
+                         uint32_t nlist,
                          const uint32_t *off, const uint8_t *codes, const float *codebooks,
                          const uint32_t *sub_b, const uint32_t *sub_e, uint32_t M, uint32_t KS,
                          uint32_t sdm, const uint32_t *nprobe_q, uint32_t *pair_cnt,
@@ -1480,6 +1497,9 @@ void launch_ivf_adc_rows(const float *Qp, uint32_t dpad, const float *C, const u
     if (rows == 0 || nprobe == 0) return;
     const int R = ivf_adc_rows_variant(M, KS, sdm);
     if (!R) fail(HM_ERR_INVALID_ARG, "row-form scan: codebook does not fit in registers");
+    // every sub-space exactly R-variant wide (d a multiple of M, width a
+    // multiple of 4): the guard-free LUT fold
+    const bool full = dim % M == 0 && dim / M == (uint32_t)(64 / R) && (dim / M) % 4 == 0;
     const uint64_t total = (uint64_t)rows * nprobe;
     if (total > 0x7FFFFFFFull) fail(HM_ERR_INVALID_ARG, "too many probe pairs");
     const size_t lds = (((size_t)M * KS + 3) & ~(size_t)3) * 4 + (size_t)dpad * 4;
@@ -1490,10 +1510,15 @@ void launch_ivf_adc_rows(const float *Qp, uint32_t dpad, const float *C, const u
     const uint32_t G = build_pairs(probes, ldp, nprobe, rows, nlist, nprobe_q, pair_cnt, pair_off,
                                    pair_fill, unit_off, pairs, nullptr, 0, s);
     const unsigned units = (unsigned)((total + G - 1) / G + nlist);  // >= sum of ceil(cnt_l / G)
-#define HM_ADC_ROWS(RR, SDD)                                                                    \
-    ivf_adc_rows_kernel<RR, SDD><<<units, LM_THREADS, lds, s>>>(                                \
-        Qp, dpad, C, pair_off, unit_off, nlist, G, pairs, nprobe, off, codes, codebooks, sub_b, \
-        sub_e, M, KS, sdm, seg, D, ldd)
+#define HM_ADC_ROWS(RR, SDD)                                                                     \
+    if (full)                                                                                    \
+        ivf_adc_rows_kernel<RR, SDD, true><<<units, LM_THREADS, lds, s>>>(                       \
+            Qp, dpad, C, pair_off, unit_off, nlist, G, pairs, nprobe, off, codes, codebooks,     \
+            sub_b, sub_e, M, KS, sdm, seg, D, ldd);                                              \
+    else                                                                                         \
+        ivf_adc_rows_kernel<RR, SDD, false><<<units, LM_THREADS, lds, s>>>(                      \
+            Qp, dpad, C, pair_off, unit_off, nlist, G, pairs, nprobe, off, codes, codebooks,     \
+            sub_b, sub_e, M, KS, sdm, seg, D, ldd)
     switch (R) {
     case 16: HM_ADC_ROWS(16, 4); break;
     case 8: HM_ADC_ROWS(8, 8); break;

@@ -284,7 +284,8 @@ class IVFPQ final : public Index {
         // ADC distances into D rows, radix row select), list-major with the
         // codebook in LDS and a per-list top-k1, and one workgroup per
         // (query, probe).
-        const bool rows_ok = ivf_adc_rows_variant(M_, KS_, sdm_) != 0 && k1 <= SELECT_MAX_K;
+        const bool rows_ok =
+            ivf_adc_rows_variant(M_, KS_, sdm_) != 0 && k1 <= SELECT_MAX_K && dpad_ < 65536;
         const bool lists_ok = ivf_scan_lists_pool(M_, KS_, sdm_, dpad_, k1) != 0;
         if ((scan_mode_ == 3 && !rows_ok) || (scan_mode_ == 2 && !lists_ok))
             fail(HM_ERR_INVALID_ARG, "the chosen scan form does not fit this index");
@@ -349,7 +350,7 @@ class IVFPQ final : public Index {
             }
             const uint32_t *npq_dev = learned ? nprobe_q_.as<uint32_t>() : nullptr;
             if (rows_form) {
-                launch_ivf_adc_rows(qpad_.as<float>(), dpad_, centers_.as<float>(),
+                launch_ivf_adc_rows(qpad_.as<float>(), dpad_, (uint32_t)dim_, centers_.as<float>(),
                                     probe_i_.as<uint32_t>(), Kc, np, rows, nlist_,
                                     off_.as<uint32_t>(), codes_.as<uint8_t>(),
                                     codebooks_.as<float>(), sub_b_.as<uint32_t>(),
