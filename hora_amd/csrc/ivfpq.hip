@@ -348,7 +348,7 @@ __global__ __launch_bounds__(256) void ivf_scan_kernel(
         for (uint32_t size = 2; size <= PP; size <<= 1)
             for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
                 for (uint32_t t = tid; t < PP / 2; t += 256) {
-                    uint32_t lo = 2 * stride * (t / stride) + (t % stride), hi = lo + stride;
+                    uint32_t lo = bitonic_lo(t, stride), hi = lo + stride;
                     bool asc = (lo & size) == 0;
                     unsigned long long a = keys[lo], bb = keys[hi];
                     if ((a > bb) == asc) {
@@ -564,7 +564,7 @@ __global__ __launch_bounds__(LM_THREADS) void ivf_scan_lists_kernel(
         for (uint32_t size = 2; size <= PP; size <<= 1)
             for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
                 for (uint32_t t = tid; t < PP / 2; t += LM_THREADS) {
-                    const uint32_t a0 = 2 * stride * (t / stride) + (t % stride), a1 = a0 + stride;
+                    const uint32_t a0 = bitonic_lo(t, stride), a1 = a0 + stride;
                     const bool asc = (a0 & size) == 0;
                     const unsigned long long a = keys[a0], bb = keys[a1];
                     if ((a > bb) == asc) {
@@ -881,7 +881,7 @@ __global__ __launch_bounds__(256) void ivf_merge_kernel(
     for (uint32_t size = 2; size <= P; size <<= 1)
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
             for (uint32_t t = tid; t < P / 2; t += 256) {
-                uint32_t lo = 2 * stride * (t / stride) + (t % stride), hi = lo + stride;
+                uint32_t lo = bitonic_lo(t, stride), hi = lo + stride;
                 bool asc = (lo & size) == 0;
                 unsigned long long a = keys[lo], bb = keys[hi];
                 if ((a > bb) == asc) {
@@ -1079,7 +1079,7 @@ __global__ __launch_bounds__(RP_THREADS) void ivf_list_replay_kernel(
     for (uint32_t size = 2; size <= P; size <<= 1)
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
             for (uint32_t u = tid; u < P / 2; u += RP_THREADS) {
-                const uint32_t a0 = 2 * stride * (u / stride) + (u % stride), a1 = a0 + stride;
+                const uint32_t a0 = bitonic_lo(u, stride), a1 = a0 + stride;
                 const bool asc = (a0 & size) == 0;
                 const unsigned long long a = sk[a0], bb = sk[a1];
                 if ((a > bb) == asc) {

@@ -6,6 +6,13 @@
 namespace hm {
 
 enum AccumKind { ACC_EUCLID = 0, ACC_MANHATTAN = 1, ACC_DOT = 2 };
+
+// Bitonic sort step: the lower index of compare-exchange pair t for a
+// power-of-two stride (2 * stride * (t / stride) + t % stride, by masks).
+__device__ __forceinline__ uint32_t bitonic_lo(uint32_t t, uint32_t stride)
+{
+    return ((t & ~(stride - 1)) << 1) | (t & (stride - 1));
+}
 enum EpilogueKind { EPI_NONE = 0, EPI_COSINE = 1, EPI_ANGULAR = 2 };
 
 // Row flags written by the selection kernels.

@@ -125,7 +125,7 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
     for (uint32_t size = 2; size <= P; size <<= 1) {
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
             for (uint32_t t = tid; t < P / 2; t += RR_THREADS) {
-                uint32_t lo = 2 * stride * (t / stride) + (t % stride);
+                uint32_t lo = bitonic_lo(t, stride);
                 uint32_t hi = lo + stride;
                 bool asc = (lo & size) == 0;
                 unsigned long long a = keys[lo], b = keys[hi];
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(LP_THREADS) void list_prepare_kernel(
     for (uint32_t size = 2; size <= P; size <<= 1) {
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
             for (uint32_t t = tid; t < P / 2; t += LP_THREADS) {
-                const uint32_t lo = 2 * stride * (t / stride) + (t % stride), hi = lo + stride;
+                const uint32_t lo = bitonic_lo(t, stride), hi = lo + stride;
                 const bool asc = (lo & size) == 0;
                 const unsigned long long a = lp_keys[lo], b = lp_keys[hi];
                 if ((a > b) == asc) {

@@ -17,7 +17,22 @@ namespace hm {
 namespace {
 
 constexpr int SEL_THREADS = 256;
-constexpr uint32_t SEL_CAP = 8192;  // candidate slots in LDS, power of two
+constexpr uint32_t SEL_CAP = 8192;  // most candidate slots in LDS, power of two
+
+// Keys up to the target bucket collected for the final sort at most.
+__host__ __device__ inline uint32_t sel_early_cap(uint32_t K)
+{
+    const uint32_t c = 2 * K > 1024u ? 2 * K : 1024u;
+    return c < SEL_CAP ? c : SEL_CAP;
+}
+
+// Candidate slots for K_max: a power of two >= max(K_max, early cap).
+inline uint32_t sel_slots(uint32_t K_max)
+{
+    uint32_t need = sel_early_cap(K_max) > K_max ? sel_early_cap(K_max) : K_max, c = 1;
+    while (c < need) c <<= 1;
+    return c;
+}
 
 // Order-preserving map of a float to uint32 (ascending float order ==
 // ascending key).  -0.0 is canonicalised to +0.0 first: the reference
@@ -98,14 +113,16 @@ __global__ __launch_bounds__(SEL_THREADS) void row_select_kernel(
     const float *__restrict__ D, uint64_t ldd, uint32_t n_max, uint32_t K_max,
     float *__restrict__ cand_d, uint32_t *__restrict__ cand_i, uint32_t ldc,
     uint32_t *__restrict__ flags, const uint32_t *__restrict__ dev_rows,
-    const uint32_t *__restrict__ row_n)
+    const uint32_t *__restrict__ row_n, uint32_t cap)
 {
     if (dev_rows && blockIdx.x >= *dev_rows) return;  // graph replays with fewer rows
     // ragged rows: this row's length and the candidates it can give
     const uint32_t n = row_n ? min(row_n[blockIdx.x], n_max) : n_max;
     const uint32_t K = min(K_max, n);
     __shared__ uint32_t hist[2048];
-    __shared__ unsigned long long cand[SEL_CAP];
+    // candidate slots: cap (a power of two >= max(K_max, early_cap)) in
+    // dynamic LDS, so small K leaves room for many rows per CU
+    extern __shared__ unsigned long long cand[];
     __shared__ uint32_t s_w[SEL_THREADS / 64];
     __shared__ uint32_t s_prefix, s_remaining, s_bucket, s_cnt, s_nan, s_tie;
 
@@ -121,8 +138,11 @@ __global__ __launch_bounds__(SEL_THREADS) void row_select_kernel(
     }
 
     // Radix select of the K-th smallest key: digits of 11, 11 and 10 bits.
-    // After each pass, when every key up to the target bucket fits in LDS,
-    // they are collected and sorted directly (usually after the first pass).
+    // After each pass, when the keys up to the target bucket are few (at
+    // most max(2K, 1024)), they are collected and sorted directly; a larger
+    // target bucket (a narrow value range) is refined by the next digit
+    // first, so the sort stays small.
+    const uint32_t early_cap = sel_early_cap(K);
     uint32_t mask = 0;
     bool early = false;
 #pragma unroll 1
@@ -160,7 +180,7 @@ __global__ __launch_bounds__(SEL_THREADS) void row_select_kernel(
         }
         __syncthreads();
         mask |= (nb - 1) << shift;
-        if ((K - s_remaining) + s_bucket <= SEL_CAP) {
+        if ((K - s_remaining) + s_bucket <= early_cap) {
             early = true;
             break;
         }
@@ -176,7 +196,7 @@ __global__ __launch_bounds__(SEL_THREADS) void row_select_kernel(
             const uint32_t key = fkey(f);
             if ((key & mask) <= T) {
                 const uint32_t pos = atomicAdd(&s_cnt, 1u);
-                if (pos < SEL_CAP) cand[pos] = ((unsigned long long)key << 32) | i;
+                if (pos < cap) cand[pos] = ((unsigned long long)key << 32) | i;
             }
         });
         __syncthreads();
@@ -190,7 +210,7 @@ __global__ __launch_bounds__(SEL_THREADS) void row_select_kernel(
             const uint32_t key = fkey(f);
             if (key < T || (all_eq && key == T)) {
                 const uint32_t pos = atomicAdd(&s_cnt, 1u);
-                if (pos < SEL_CAP) cand[pos] = ((unsigned long long)key << 32) | i;
+                if (pos < cap) cand[pos] = ((unsigned long long)key << 32) | i;
             }
         });
         __syncthreads();
@@ -220,7 +240,7 @@ __global__ __launch_bounds__(SEL_THREADS) void row_select_kernel(
     for (uint32_t size = 2; size <= P; size <<= 1) {
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
             for (uint32_t t = tid; t < P / 2; t += SEL_THREADS) {
-                uint32_t lo = 2 * stride * (t / stride) + (t % stride);
+                uint32_t lo = bitonic_lo(t, stride);
                 uint32_t hi = lo + stride;
                 bool asc = (lo & size) == 0;
                 unsigned long long a = cand[lo], b = cand[hi];
@@ -773,12 +793,14 @@ void launch_row_select(const float *D, uint64_t ldd, uint32_t rows, uint32_t n, 
     if (K > SEL_CAP || (!row_n && K > n)) fail(HM_ERR_INVALID_ARG, "row_select: K out of range");
     if (ldd < n) fail(HM_ERR_INVALID_ARG, "row_select: row stride below row length");
     ProfScope prof("row_select", s);
+    const uint32_t cap = sel_slots(K);
+    const size_t lds = (size_t)cap * 8;
     if (ldd % 4 == 0 && ((uintptr_t)D & 15) == 0)
-        row_select_kernel<true><<<rows, SEL_THREADS, 0, s>>>(D, ldd, n, K, cand_d, cand_i, ldc,
-                                                             flags, dev_rows, row_n);
+        row_select_kernel<true><<<rows, SEL_THREADS, lds, s>>>(D, ldd, n, K, cand_d, cand_i, ldc,
+                                                               flags, dev_rows, row_n, cap);
     else
-        row_select_kernel<false><<<rows, SEL_THREADS, 0, s>>>(D, ldd, n, K, cand_d, cand_i, ldc,
-                                                              flags, dev_rows, row_n);
+        row_select_kernel<false><<<rows, SEL_THREADS, lds, s>>>(D, ldd, n, K, cand_d, cand_i, ldc,
+                                                                flags, dev_rows, row_n, cap);
     HM_CHECK_LAUNCH();
 }
 
