@@ -33,6 +33,7 @@ FP32_PEAK = 157.3    # TFLOP/s, MI355X published FP32
 BF16_PEAK = 2500.0   # TFLOP/s dense
 FP8_PEAK = 5000.0
 HBM_PEAK = 8000.0    # GB/s
+LDS_PEAK = 256 * 128 * 2.4  # GB/s: 256 CUs x 128 B/clk x 2.4 GHz
 
 
 def parse():
@@ -376,7 +377,7 @@ def main_pq(args):
                                                   d_dist.data_ptr(), d_cnt.data_ptr(),
                                                   stream.cuda_stream),
                          args.steps, args.warmup, "pq")
-    km = kernel_ms(["pq_scan", "row_select", "pq_finalize", "pq_replay"])
+    km = kernel_ms(["pq_scan", "pq_list_select", "row_select", "pq_finalize", "pq_replay"])
     replayed = int(ix.search_stats()[0])
     got = d_idx.cpu().numpy().view(np.uint64)
     r10 = float(np.mean([len(set(got[i].tolist()) & set(truth[i].tolist())) / K
@@ -390,16 +391,23 @@ def main_pq(args):
     scan_ms, scan_n = km["pq_scan"]
     roofline = None
     if scan_n:
-        # bytes per scan launch: the codes once (N x 16 B) and the distance
-        # block written (rows x N x 4 B); the LUT lookups run in LDS
+        # The scan gathers LUT entries from LDS: one 16-byte read serves 4
+        # queries, so each query's lookup moves 4 bytes; per scan scope,
+        # rows x (N + 4,096 sampled items in the filter form) x 16 lookups.
+        # The distance block (rows x N x 4 B, whole-row form only) and the
+        # codes are the HBM side.
         rows = NQ * args.steps / scan_n
-        bytes_ = N * 16 + rows * N * 4
+        filt = km["row_select"][1] == 0
+        items = N + (4096 if filt else 0)
         t = scan_ms / scan_n / 1e3
-        ach = bytes_ / t / 1e9
-        roofline = {"bound": "hbm", "kernel": "pq_scan_kernel", "achieved": round(ach, 1),
-                    "peak": HBM_PEAK, "unit": "GB/s", "frac": round(ach / HBM_PEAK, 4),
-                    "traffic": None, "avg_launch_ms": round(scan_ms / scan_n, 3),
-                    "lds_lookups_per_s": round(rows * N * 16 / t, 1)}
+        lds_bytes = rows * items * 16 * 4
+        ach = lds_bytes / t / 1e9
+        roofline = {"bound": "lds", "kernel": "pq_scan_kernel" + (" (filter form)" if filt else ""),
+                    "achieved": round(ach, 1), "peak": LDS_PEAK, "unit": "GB/s",
+                    "frac": round(ach / LDS_PEAK, 4), "traffic": None,
+                    "avg_launch_ms": round(scan_ms / scan_n, 3),
+                    "lds_lookups_per_s": round(rows * items * 16 / t, 1),
+                    "hbm_bytes_per_launch": N * 16 + (0 if filt else rows * N * 4)}
     cpu = None
     if not args.no_cpu_baseline:
         ns = min(args.cpu_sample, NQ)

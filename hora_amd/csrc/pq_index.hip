@@ -397,14 +397,43 @@ __device__ __forceinline__ float pq_lut_entry(const float *__restrict__ z,
 // written by consecutive lanes.
 constexpr int PS_THREADS = 256;
 constexpr int PS_Q = 4;
+// Filter form of the search: sample size for the threshold, largest k + 1
+// (the threshold is a k-th smallest of at most 32), fewest items.
+constexpr uint32_t PQ_SAMPLE = 4096;
+constexpr uint32_t PQ_FILTER_MAX_K = 32;
+constexpr uint32_t PQ_FILTER_MIN_N = 16384;
 
-template <int MET>
+// Order-preserving float -> uint32 map (-0.0 as +0.0) and its inverse.
+__device__ __forceinline__ uint32_t pq_fkey(float f)
+{
+    f = f + 0.0f;
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float pq_unkey(uint32_t k)
+{
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+// Filter form of the scan's output: instead of the distance block, each
+// query keeps the items whose distance is <= tau[q] as (fkey(dist) << 32 |
+// item) in list[q][0..cap) (cnt[q] counts them, past cap too; zeroed by the
+// caller), and nanq[q] = 1 when one of its distances is NaN.
+struct PqFilter {
+    const float *tau;
+    unsigned long long *list;
+    uint32_t cap;
+    uint32_t *cnt;
+    uint32_t *nanq;
+};
+
+template <int MET, bool FILT>
 __global__ __launch_bounds__(PS_THREADS) void pq_scan_kernel(
     const float *__restrict__ Qp, uint32_t dpad, uint32_t rows,
     const uint32_t *__restrict__ sub_b, const uint32_t *__restrict__ sub_e, uint32_t M,
     uint32_t KS, uint32_t sdm, const float *__restrict__ cb, const float *__restrict__ cbn,
     const uint8_t *__restrict__ codes, uint32_t n, uint32_t per_split, float *__restrict__ D,
-    uint64_t ldd)
+    uint64_t ldd, PqFilter flt)
 {
     extern __shared__ __align__(16) float4 lut4[];
     float *lut = reinterpret_cast<float *>(lut4);
@@ -423,12 +452,44 @@ __global__ __launch_bounds__(PS_THREADS) void pq_scan_kernel(
     }
     __syncthreads();
     const uint32_t nq = min((uint32_t)PS_Q, rows - q0);
-    float *d0 = D + (size_t)q0 * ldd;
+    float tq[PS_Q];
+    bool nanf[PS_Q];
+#pragma unroll
+    for (int qq = 0; qq < PS_Q; ++qq) {
+        tq[qq] = (FILT && (uint32_t)qq < nq) ? flt.tau[q0 + qq] : 0.0f;
+        nanf[qq] = false;
+    }
+    const uint32_t lane = tid & 63;
+    const uint64_t below = (1ull << lane) - 1;
     auto store = [&](uint32_t i, float a0, float a1, float a2, float a3) {
-        d0[i] = a0;
-        if (nq > 1) d0[ldd + i] = a1;
-        if (nq > 2) d0[2 * ldd + i] = a2;
-        if (nq > 3) d0[3 * ldd + i] = a3;
+        if constexpr (!FILT) {
+            float *d0 = D + (size_t)q0 * ldd;
+            d0[i] = a0;
+            if (nq > 1) d0[ldd + i] = a1;
+            if (nq > 2) d0[2 * ldd + i] = a2;
+            if (nq > 3) d0[3 * ldd + i] = a3;
+        } else {
+            // one counter update per wave and query: the wave's hits take
+            // consecutive slots in lane order
+            const float av[PS_Q] = {a0, a1, a2, a3};
+#pragma unroll
+            for (int qq = 0; qq < PS_Q; ++qq) {
+                const bool live = (uint32_t)qq < nq;
+                if (live && av[qq] != av[qq]) nanf[qq] = true;
+                const bool hit = live && av[qq] <= tq[qq];
+                const uint64_t m = __ballot(hit);
+                if (m) {
+                    const uint32_t lead = (uint32_t)__builtin_ctzll(m);
+                    uint32_t base = 0;
+                    if (lane == lead) base = atomicAdd(&flt.cnt[q0 + qq], (uint32_t)__popcll(m));
+                    base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)lead);
+                    const uint32_t pos = base + (uint32_t)__popcll(m & below);
+                    if (hit && pos < flt.cap)
+                        flt.list[(size_t)(q0 + qq) * flt.cap + pos] =
+                            ((unsigned long long)pq_fkey(av[qq]) << 32) | i;
+                }
+            }
+        }
     };
     if ((M & 15u) == 0) {
         // 16 codes per 16-byte load, two items in flight per lane
@@ -476,6 +537,67 @@ __global__ __launch_bounds__(PS_THREADS) void pq_scan_kernel(
             store(i, a0, a1, a2, a3);
         }
     }
+    if constexpr (FILT) {
+#pragma unroll
+        for (int qq = 0; qq < PS_Q; ++qq)
+            if (nanf[qq]) flt.nanq[q0 + qq] = 1u;
+    }
+}
+
+// Strided sample of the codes for the filter threshold: sample j is item
+// floor(j * n / S).
+__global__ void pq_sample_codes_kernel(const uint8_t *__restrict__ codes, uint32_t M, uint32_t n,
+                                       uint32_t S, uint8_t *__restrict__ out)
+{
+    const uint64_t total = (uint64_t)S * M;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t j = e / M, c = e % M;
+        out[e] = codes[(j * n / S) * M + c];
+    }
+}
+
+// Filter lists -> the row select's candidate format: the first K entries
+// by (distance, item) as cand_d / cand_i, ROW_TIE when they are not
+// pairwise distinct, ROW_NAN from the scan, ROW_UNCERT when the list
+// overflowed (or holds fewer than K, which only NaN thresholds allow).
+__global__ __launch_bounds__(256) void pq_list_select_kernel(
+    const unsigned long long *__restrict__ list, uint32_t cap, const uint32_t *__restrict__ cnt,
+    const uint32_t *__restrict__ nanq, uint32_t K, float *__restrict__ cand_d,
+    uint32_t *__restrict__ cand_i, uint32_t *__restrict__ flags)
+{
+    extern __shared__ unsigned long long lk[];  // [P], P a power of two >= min(cnt, cap)
+    __shared__ uint32_t s_tie;
+    const uint32_t q = blockIdx.x, tid = threadIdx.x;
+    const uint32_t c = cnt[q], m = min(c, cap);
+    const bool bad = c > cap || c < K;
+    uint32_t P = 1;
+    while (P < m) P <<= 1;
+    if (tid == 0) s_tie = 0;
+    for (uint32_t j = tid; j < P; j += 256) lk[j] = j < m ? list[(size_t)q * cap + j] : ~0ull;
+    __syncthreads();
+    for (uint32_t size = 2; size <= P; size <<= 1)
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t t = tid; t < P / 2; t += 256) {
+                const uint32_t lo = bitonic_lo(t, stride), hi = lo + stride;
+                const bool asc = (lo & size) == 0;
+                const unsigned long long a = lk[lo], b = lk[hi];
+                if ((a > b) == asc) {
+                    lk[lo] = b;
+                    lk[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    for (uint32_t j = tid; j < K; j += 256) {
+        const unsigned long long v = j < m ? lk[j] : ~0ull;
+        cand_i[(size_t)q * K + j] = (uint32_t)v;
+        cand_d[(size_t)q * K + j] = pq_unkey((uint32_t)(v >> 32));
+        if (j + 1 < K && j + 1 < m && (uint32_t)(v >> 32) == (uint32_t)(lk[j + 1] >> 32)) s_tie = 1;
+    }
+    __syncthreads();
+    if (tid == 0)
+        flags[q] = (s_tie ? ROW_TIE : 0u) | (nanq[q] ? ROW_NAN : 0u) | (bad ? ROW_UNCERT : 0u);
 }
 
 // The LUTs of codebooks too large for LDS, in global memory:
@@ -743,6 +865,7 @@ class PQ final : public Index {
             codeword_norms(s);
         HM_HIP(hipStreamSynchronize(s));
         trained_ = true;
+        sample_for_ = 0;
         n_trained_ = n;
     }
     const char *name() const override { return "PQIndex"; }
@@ -808,9 +931,32 @@ class PQ final : public Index {
         err_.reserve(4);
         HM_HIP(hipMemsetAsync(err_.get(), 0, 4, s));
         launch_check_finite(dq, dim_, nq, (uint32_t)dim_, err_.as<uint32_t>(), s);
+        last_replayed_ = 0;
+        last_filtered_ = 0;
+        last_uncert_ = 0;
+        if (lut_qb() != 0 && K <= PQ_FILTER_MAX_K && n >= PQ_FILTER_MIN_N)
+            search_filter(dq, nq, k, d_idx, d_dist, d_count, s);
+        else
+            search_full(dq, nq, k, d_idx, d_dist, d_count, s);
+        uint32_t err = 0;
+        HM_HIP(hipMemcpyAsync(&err, err_.get(), 4, hipMemcpyDeviceToHost, s));
+        HM_HIP(hipStreamSynchronize(s));
+        if (err & 2u) fail(HM_ERR_NONFINITE_INPUT, "invalid float element");
+        if (err & 1u) fail(HM_ERR_NAN_DISTANCE, "NaN distance (partial_cmp of NaN)");
+    }
+
+    // Whole distance rows: scan, row select of the k+1 smallest, the
+    // tie-free rows' results, and the heap replay of tied rows.
+    void search_full(const float *dq, uint64_t nq, uint32_t k, uint64_t *d_idx, float *d_dist,
+                     uint32_t *d_count, hipStream_t s)
+    {
+        const uint32_t n = (uint32_t)n_trained_;
+        const uint32_t K = std::min<uint32_t>(k + 1, n);
         const uint64_t ldd = round_up(n, 4);
+        // distance rows of up to 4 GiB per chunk: the replay of a chunk's
+        // tied rows fills the GPU best when the chunk is large
         uint64_t qc = std::min<uint64_t>(nq, 65535);
-        qc = std::min<uint64_t>(qc, std::max<uint64_t>(1, ((size_t)1 << 30) / (ldd * 4)));
+        qc = std::min<uint64_t>(qc, std::max<uint64_t>(1, ((size_t)1 << 32) / (ldd * 4)));
         const int qb = lut_qb();
         qpad_.reserve(qc * dpad_ * 4);
         D_.reserve(qc * ldd * 4);
@@ -819,21 +965,17 @@ class PQ final : public Index {
         flags_.reserve(qc * 4);
         if (qb == 0) lutg_.reserve(qc * M_ * KS_ * 4);
         std::vector<uint32_t> fl(qc);
-        last_replayed_ = 0;
         for (uint64_t q0 = 0; q0 < nq; q0 += qc) {
             const uint32_t rows = (uint32_t)std::min<uint64_t>(qc, nq - q0);
             launch_pad_rows(dq + q0 * dim_, dim_, qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
                             dpad_, s);
-            scan(rows, qb, ldd, s);
+            {
+                ProfScope ps("pq_scan", s);
+                scan(rows, qb, codes_.as<uint8_t>(), n, D_.as<float>(), ldd, nullptr, s);
+            }
             launch_row_select(D_.as<float>(), ldd, rows, n, K, cand_d_.as<float>(),
                               cand_i_.as<uint32_t>(), K, flags_.as<uint32_t>(), s);
-            {
-                ProfScope ps("pq_finalize", s);
-                pq_finalize_kernel<<<rows, 64, 0, s>>>(
-                    cand_d_.as<float>(), cand_i_.as<uint32_t>(), K, flags_.as<uint32_t>(), rows, n,
-                    k, d_idx + q0 * k, d_dist + q0 * k, d_count + q0, err_.as<uint32_t>());
-                HM_CHECK_LAUNCH();
-            }
+            finalize(rows, n, k, K, d_idx + q0 * k, d_dist + q0 * k, d_count + q0, s);
             HM_HIP(hipMemcpyAsync(fl.data(), flags_.get(), rows * 4, hipMemcpyDeviceToHost, s));
             HM_HIP(hipStreamSynchronize(s));
             std::vector<uint32_t> tied;
@@ -842,18 +984,109 @@ class PQ final : public Index {
             last_replayed_ += tied.size();
             if (!tied.empty()) replay(tied, n, k, ldd, d_idx + q0 * k, d_dist + q0 * k, s);
         }
-        uint32_t err = 0;
-        HM_HIP(hipMemcpyAsync(&err, err_.get(), 4, hipMemcpyDeviceToHost, s));
-        HM_HIP(hipStreamSynchronize(s));
-        if (err & 2u) fail(HM_ERR_NONFINITE_INPUT, "invalid float element");
-        if (err & 1u) fail(HM_ERR_NAN_DISTANCE, "NaN distance (partial_cmp of NaN)");
+    }
+
+    void finalize(uint32_t rows, uint32_t n, uint32_t k, uint32_t K, uint64_t *d_idx,
+                  float *d_dist, uint32_t *d_count, hipStream_t s)
+    {
+        ProfScope ps("pq_finalize", s);
+        pq_finalize_kernel<<<rows, 64, 0, s>>>(cand_d_.as<float>(), cand_i_.as<uint32_t>(), K,
+                                               flags_.as<uint32_t>(), rows, n, k, d_idx, d_dist,
+                                               d_count, err_.as<uint32_t>());
+        HM_CHECK_LAUNCH();
+    }
+
+    // Filter form (k + 1 <= 32, the LUTs in LDS): tau[q] = the (k+1)-th
+    // smallest ADC distance over a strided sample of PQ_SAMPLE items, an
+    // upper bound of the row's (k+1)-th smallest; the scan keeps only the
+    // items at or below it, so the first k+1 of the row by (distance, item)
+    // are among them.  Rows whose list overflows, and tied rows (the heap
+    // replay needs the whole row), are computed again by search_full.
+    void search_filter(const float *dq, uint64_t nq, uint32_t k, uint64_t *d_idx,
+                       float *d_dist, uint32_t *d_count, hipStream_t s)
+    {
+        const uint32_t n = (uint32_t)n_trained_;
+        const uint32_t K = std::min<uint32_t>(k + 1, n);
+        const uint32_t S = PQ_SAMPLE;
+        if (sample_for_ != n) {
+            codes_s_.reserve((size_t)S * M_);
+            pq_sample_codes_kernel<<<std::min<uint32_t>(ceil_div(S * M_, 256), 1024), 256, 0, s>>>(
+                codes_.as<uint8_t>(), M_, n, S, codes_s_.as<uint8_t>());
+            HM_CHECK_LAUNCH();
+            sample_for_ = n;
+        }
+        // list capacity: 8x the expected K * n / S entries, a power of two
+        uint32_t cap = 1024;
+        while (cap < 8ull * K * n / S && cap < 8192) cap <<= 1;
+        const uint64_t qc = std::min<uint64_t>(nq, 65535);
+        qpad_.reserve(qc * dpad_ * 4);
+        Ds_.reserve(qc * S * 4);
+        tau_.reserve(qc * 4);
+        lists_.reserve(qc * cap * 8);
+        lcnt_.reserve(qc * 2 * 4);
+        cand_d_.reserve(qc * K * 4);
+        cand_i_.reserve(qc * K * 4);
+        flags_.reserve(qc * 4);
+        std::vector<uint32_t> fl(qc);
+        for (uint64_t q0 = 0; q0 < nq; q0 += qc) {
+            const uint32_t rows = (uint32_t)std::min<uint64_t>(qc, nq - q0);
+            launch_pad_rows(dq + q0 * dim_, dim_, qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
+                            dpad_, s);
+            uint32_t *cnt = lcnt_.as<uint32_t>(), *nanq = cnt + rows;
+            {
+                ProfScope ps("pq_scan", s);
+                scan(rows, 1, codes_s_.as<uint8_t>(), S, Ds_.as<float>(), S, nullptr, s);
+                launch_kth_smallest(Ds_.as<float>(), S, rows, S, K, tau_.as<float>(), s);
+                launch_fill_u32(cnt, 2 * rows, 0u, s);
+                const PqFilter f{tau_.as<float>(), lists_.as<unsigned long long>(), cap, cnt, nanq};
+                scan(rows, 1, codes_.as<uint8_t>(), n, nullptr, 0, &f, s);
+            }
+            {
+                ProfScope ps("pq_list_select", s);
+                uint32_t P = 1;
+                while (P < cap) P <<= 1;
+                pq_list_select_kernel<<<rows, 256, (size_t)P * 8, s>>>(
+                    lists_.as<unsigned long long>(), cap, cnt, nanq, K, cand_d_.as<float>(),
+                    cand_i_.as<uint32_t>(), flags_.as<uint32_t>());
+                HM_CHECK_LAUNCH();
+            }
+            finalize(rows, n, k, K, d_idx + q0 * k, d_dist + q0 * k, d_count + q0, s);
+            HM_HIP(hipMemcpyAsync(fl.data(), flags_.get(), rows * 4, hipMemcpyDeviceToHost, s));
+            HM_HIP(hipStreamSynchronize(s));
+            std::vector<uint32_t> redo;
+            for (uint32_t i = 0; i < rows; ++i)
+                if ((fl[i] & ROW_UNCERT) || ((fl[i] & ROW_TIE) && !(fl[i] & ROW_NAN)))
+                    redo.push_back((uint32_t)(q0 + i));
+            last_filtered_ += rows - redo.size();
+            for (uint32_t i = 0; i < rows; ++i) last_uncert_ += (fl[i] & ROW_UNCERT) ? 1u : 0u;
+            if (redo.empty()) continue;
+            // the rows again with whole distance rows, into compact outputs
+            const uint32_t m = (uint32_t)redo.size();
+            redo_rows_.reserve(m * 4);
+            redo_q_.reserve((size_t)m * dim_ * 4);
+            redo_i_.reserve((size_t)m * k * 8);
+            redo_d_.reserve((size_t)m * k * 4);
+            redo_c_.reserve((size_t)m * 4);
+            HM_HIP(hipMemcpyAsync(redo_rows_.get(), redo.data(), m * 4, hipMemcpyHostToDevice, s));
+            launch_gather_rows(dq, dim_, redo_rows_.as<uint32_t>(), m, (uint32_t)dim_,
+                               redo_q_.as<float>(), dim_, s);
+            search_full(redo_q_.as<float>(), m, k, redo_i_.as<uint64_t>(), redo_d_.as<float>(),
+                        redo_c_.as<uint32_t>(), s);
+            launch_scatter_results(redo_i_.as<uint64_t>(), redo_d_.as<float>(),
+                                   redo_c_.as<uint32_t>(), redo_rows_.as<uint32_t>(), m, k, d_idx,
+                                   d_dist, d_count, s);
+            HM_HIP(hipStreamSynchronize(s));
+        }
     }
 
     void search_stats(uint64_t *fallback, uint64_t *uncertified) const override
     {
         *fallback = last_replayed_;
-        *uncertified = 0;
+        *uncertified = last_uncert_;
     }
+
+    // Rows of the last search answered by the filter form.
+    uint64_t last_filtered() const { return last_filtered_; }
 
   private:
     void grow(uint64_t n_rows)
@@ -876,9 +1109,12 @@ class PQ final : public Index {
         return per * PS_Q <= lim ? 1 : 0;
     }
 
-    template <int MET> void scan_m(uint32_t rows, int qb, uint64_t ldd, hipStream_t s)
+    // ADC of `rows` padded queries (qpad_) against n coded items: the
+    // distance block D (stride ldd), or with flt the filter lists.
+    template <int MET>
+    void scan_m(uint32_t rows, int qb, const uint8_t *codes, uint32_t n, float *D, uint64_t ldd,
+                const PqFilter *flt, hipStream_t s)
     {
-        const uint32_t n = (uint32_t)n_trained_;
         if (qb == 0) {
             const uint32_t nsplit = std::max<uint32_t>(1, std::min<uint32_t>(
                 ceil_div(1024, rows), std::max<uint32_t>(1, n / 2048)));
@@ -888,8 +1124,7 @@ class PQ final : public Index {
                 sdm_, cb_.as<float>(), cbn_.as<float>(), lutg_.as<float>());
             HM_CHECK_LAUNCH();
             pq_scan_glut_kernel<<<dim3(ceil_div(n, per_split), rows), PS_THREADS, 0, s>>>(
-                rows, M_, KS_, codes_.as<uint8_t>(), n, per_split, lutg_.as<float>(),
-                D_.as<float>(), ldd);
+                rows, M_, KS_, codes, n, per_split, lutg_.as<float>(), D, ldd);
             HM_CHECK_LAUNCH();
             return;
         }
@@ -900,22 +1135,30 @@ class PQ final : public Index {
         const uint32_t per_split = (uint32_t)round_up(ceil_div(n, nsplit), 2 * PS_THREADS);
         const uint32_t ns = ceil_div(n, per_split);
         const size_t lds = (size_t)M_ * KS_ * PS_Q * 4;
-        pq_scan_kernel<MET><<<dim3(ns, groups), PS_THREADS, lds, s>>>(
-            qpad_.as<float>(), dpad_, rows, sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), M_, KS_,
-            sdm_, cb_.as<float>(), cbn_.as<float>(), codes_.as<uint8_t>(), n, per_split,
-            D_.as<float>(), ldd);
+        if (flt)
+            pq_scan_kernel<MET, true><<<dim3(ns, groups), PS_THREADS, lds, s>>>(
+                qpad_.as<float>(), dpad_, rows, sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), M_,
+                KS_, sdm_, cb_.as<float>(), cbn_.as<float>(), codes, n, per_split, nullptr, 0,
+                *flt);
+        else
+            pq_scan_kernel<MET, false><<<dim3(ns, groups), PS_THREADS, lds, s>>>(
+                qpad_.as<float>(), dpad_, rows, sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), M_,
+                KS_, sdm_, cb_.as<float>(), cbn_.as<float>(), codes, n, per_split, D, ldd,
+                PqFilter{});
         HM_CHECK_LAUNCH();
     }
 
-    void scan(uint32_t rows, int qb, uint64_t ldd, hipStream_t s)
+    void scan(uint32_t rows, int qb, const uint8_t *codes, uint32_t n, float *D, uint64_t ldd,
+              const PqFilter *flt, hipStream_t s)
     {
-        ProfScope ps("pq_scan", s);
         switch (metric_) {
-        case HM_METRIC_EUCLIDEAN: scan_m<PM_EUCLID>(rows, qb, ldd, s); break;
-        case HM_METRIC_MANHATTAN: scan_m<PM_MANHATTAN>(rows, qb, ldd, s); break;
-        case HM_METRIC_DOT_PRODUCT: scan_m<PM_DOT>(rows, qb, ldd, s); break;
-        case HM_METRIC_COSINE_SIMILARITY: scan_m<PM_COSINE>(rows, qb, ldd, s); break;
-        default: scan_m<PM_ANGULAR>(rows, qb, ldd, s); break;
+        case HM_METRIC_EUCLIDEAN: scan_m<PM_EUCLID>(rows, qb, codes, n, D, ldd, flt, s); break;
+        case HM_METRIC_MANHATTAN: scan_m<PM_MANHATTAN>(rows, qb, codes, n, D, ldd, flt, s); break;
+        case HM_METRIC_DOT_PRODUCT: scan_m<PM_DOT>(rows, qb, codes, n, D, ldd, flt, s); break;
+        case HM_METRIC_COSINE_SIMILARITY:
+            scan_m<PM_COSINE>(rows, qb, codes, n, D, ldd, flt, s);
+            break;
+        default: scan_m<PM_ANGULAR>(rows, qb, codes, n, D, ldd, flt, s); break;
         }
     }
 
@@ -1052,6 +1295,7 @@ class PQ final : public Index {
             codeword_norms(s);
         HM_HIP(hipStreamSynchronize(s));
         trained_ = true;
+        sample_for_ = 0;
         n_trained_ = n;
     }
 
@@ -1072,6 +1316,12 @@ class PQ final : public Index {
     DeviceBuffer base_, sub_b_, sub_e_, cb_, cbn_, asg_, counts_, codes_, rng_;
     DeviceBuffer ccnt_, coff_, cbase_, members_;  // update: items grouped by centre
     DeviceBuffer qpad_, D_, cand_d_, cand_i_, flags_, err_, lutg_, rp_rows_, rp_g_;
+    // filter form: sampled codes, sample distances, thresholds, lists, and
+    // the rows computed again with whole distance rows
+    DeviceBuffer codes_s_, Ds_, tau_, lists_, lcnt_, redo_rows_, redo_q_, redo_i_, redo_d_,
+        redo_c_;
+    uint32_t sample_for_ = 0;  // item count the sampled codes were taken for
+    uint64_t last_filtered_ = 0, last_uncert_ = 0;
 };
 
 }  // namespace

@@ -138,3 +138,31 @@ def test_edges_and_errors(orc):
     with pytest.raises(HoraError) as e:  # zero query: NaN LUT -> NaN distances
         cos.search_batch(np.zeros((1, 8), np.float32), 3)
     assert e.value.status == 8
+
+
+@pytest.mark.parametrize("mt", [Metric.Euclidean, Metric.Manhattan, Metric.DotProduct,
+                                Metric.CosineSimilarity, Metric.Angular],
+                         ids=lambda m: m.name)
+def test_filter_form_matches_oracle(orc, mt):
+    # 20,000 items and k + 1 <= 32: the threshold from a strided sample of
+    # 4,096 items, the filtered lists, and tied rows recomputed whole
+    x, cen, _ = gaussian_mixture(20_000, 32, 50, seed=75, sigma=0.6)
+    q, _, _ = gaussian_mixture(60, 32, 50, seed=76, sigma=0.6, centres=cen)
+    idx = _build(x, mt, 8, 8, 3)
+    for k in (1, 10, 31):
+        _search_check(orc, idx, q, k, mt)
+    assert idx.search_stats()[1] == 0  # no list overflowed
+
+
+def test_filter_overflow_rows_recomputed(orc):
+    # items 4j (the strided sample) far from every query, the rest near:
+    # the sample's threshold admits all 12,288 near items, more than the
+    # list holds, so every row is computed again with its whole distance row
+    rng = np.random.default_rng(77)
+    n, d = 16_384, 16
+    x = rng.standard_normal((n, d), dtype=np.float32)
+    x[::4] += np.float32(40.0)
+    q = rng.standard_normal((8, d), dtype=np.float32)
+    idx = _build(x, Metric.Euclidean, 4, 8, 2)
+    _search_check(orc, idx, q, 31, Metric.Euclidean)
+    assert idx.search_stats()[1] == 8
