@@ -505,74 +505,152 @@ struct WaveHeap {
 // maximum is pushed and popped again; that round trip permutes the layout
 // in a way that depends on the layout only, so when it is known to leave
 // the current layout unchanged (heap_noop_roundtrip, checked after every
-// change) such elements are skipped.  One workgroup of 64 lanes per row;
+// change) such elements are skipped.  One workgroup of 256 threads per
+// row (four waves find the groups worth visiting, one wave runs the heap);
 // rows without a tie copy their candidates.
-__global__ __launch_bounds__(64) void bf_finalize_wave_kernel(
+constexpr uint32_t BF_BLOCK = 256;       // values per lane-wide load (a float4 per lane)
+constexpr uint32_t BF_GMIN_MAX = 16384;  // group minima held in LDS
+
+// Wave-wide minimum (every lane gets it).
+__device__ __forceinline__ float wave_min(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// bpg: blocks of 256 values per group, so that the row has at most
+// BF_GMIN_MAX groups.
+__global__ __launch_bounds__(256) void bf_finalize_wave_kernel(
     const float *__restrict__ cand_d, const uint32_t *__restrict__ cand_i, uint32_t ldc,
     const uint32_t *__restrict__ flags, const float *__restrict__ D, uint64_t ldd, uint32_t n,
-    uint32_t k, uint64_t *__restrict__ out_idx, float *__restrict__ out_dist,
+    uint32_t k, uint32_t bpg, uint64_t *__restrict__ out_idx, float *__restrict__ out_dist,
     uint32_t *__restrict__ out_count, uint32_t *err)
 {
     const uint32_t r = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const float inf = __int_as_float(0x7f800000);
     uint64_t *oi = out_idx + (size_t)r * k;
     float *od = out_dist + (size_t)r * k;
     const uint32_t m = k < n ? k : n;
     const uint32_t f = flags[r];
     if ((f & (ROW_NAN | ROW_BAD)) && n >= 2) {
-        if (lane == 0) atomicOr(err, (f & ROW_BAD) ? 4u : 1u);
-        for (uint32_t j = lane; j < k; j += 64) {
+        if (tid == 0) atomicOr(err, (f & ROW_BAD) ? 4u : 1u);
+        for (uint32_t j = tid; j < k; j += 256) {
             oi[j] = ~0ull;
             od[j] = inf;
         }
-        if (lane == 0) out_count[r] = 0;
+        if (tid == 0) out_count[r] = 0;
         return;
     }
     if (!(f & ROW_TIE)) {
         const float *cd = cand_d + (size_t)r * ldc;
         const uint32_t *ci = cand_i + (size_t)r * ldc;
-        for (uint32_t j = lane; j < k; j += 64) {
+        for (uint32_t j = tid; j < k; j += 256) {
             oi[j] = j < m ? (uint64_t)ci[j] : ~0ull;
             od[j] = j < m ? cd[j] : inf;
         }
-        if (lane == 0) out_count[r] = m;
+        if (tid == 0) out_count[r] = m;
         return;
     }
-    WaveHeap h{0.0f, 0u, 0u, lane};
     const float *row = D + (size_t)r * ldd;
+    // A tied row.  Every element that is not greater than the root of a
+    // full heap with a stable layout is pushed; the others change nothing.
+    // The four waves first take the minimum key of every group of bpg
+    // blocks of 256 values (16 blocks per load round, all in flight
+    // together), so the heap wave visits only the groups that can hold such
+    // an element, with the next block loading while it works on the current
+    // one.  The row's NaN-free (tied rows only), so fkey order is value order.
+    const uint32_t nb = (n + BF_BLOCK - 1) / BF_BLOCK, ng = (nb + bpg - 1) / bpg;
+    extern __shared__ uint32_t gkey[];  // [ng]
+    for (uint32_t g = tid; g < ng; g += 256) gkey[g] = ~0u;
+    __syncthreads();
+    for (uint32_t b0 = (uint32_t)wave * 16; b0 < nb; b0 += 4 * 16) {
+        float4 v[16];
+#pragma unroll
+        for (uint32_t u = 0; u < 16; ++u) {
+            const uint64_t e = min((uint64_t)(b0 + u) * BF_BLOCK + 4 * (uint32_t)lane, ldd - 4);
+            v[u] = *reinterpret_cast<const float4 *>(row + e);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 16; ++u) {
+            const uint64_t e = (uint64_t)(b0 + u) * BF_BLOCK + 4 * (uint32_t)lane;
+            float mn = inf;
+            if (e < n) mn = v[u].x;
+            if (e + 1 < n) mn = fminf(mn, v[u].y);
+            if (e + 2 < n) mn = fminf(mn, v[u].z);
+            if (e + 3 < n) mn = fminf(mn, v[u].w);
+            mn = wave_min(mn);
+            if (lane == 0 && b0 + u < nb) atomicMin(&gkey[(b0 + u) / bpg], fkey(mn));
+        }
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    WaveHeap h{0.0f, 0u, 0u, lane};
     bool stable = false;
     float root = inf;
-    // While the heap is full and its layout is stable, a ballot finds the
-    // next element that is not greater than the root (the only ones that
-    // change anything), so the rejected majority costs one ballot per 64.
-    __shared__ __align__(16) float sv[4096];
-    // values above the root of a full, stable heap change nothing
-    auto threshold = [&]() { return (h.len == k && stable) ? root : inf; };
-    stream_row(row, n, ldd, lane, sv, threshold,
-               [&](uint32_t base, float v, const float *, uint32_t cnt) {
-        uint32_t j = 0;
-        while (j < cnt) {
-            if (h.len == k && stable) {
-                const unsigned long long live =
-                    __ballot((uint32_t)lane >= j && (uint32_t)lane < cnt && !(v > root));
-                if (live == 0) break;
-                j = (uint32_t)__builtin_ctzll(live);
+    auto skip = [&]() { return h.len == k && stable; };
+    // first block at or after b that can change the heap: inside a group
+    // being visited the next block; else the first block of the next group
+    // whose minimum is not above the root (any group while not skipping)
+    auto next_block = [&](uint32_t b) -> uint32_t {
+        if (b >= nb) return nb;
+        if (b % bpg != 0 || !skip()) return b;
+        const uint32_t rk = fkey(root);
+        for (uint32_t g0 = b / bpg; g0 < ng; g0 += 64) {
+            const uint32_t g = g0 + (uint32_t)lane;
+            const uint64_t live = __ballot(g < ng && !(gkey[g] > rk));
+            if (live) return min((g0 + (uint32_t)__builtin_ctzll(live)) * bpg, nb);
+        }
+        return nb;
+    };
+    auto load_block = [&](uint32_t b) -> float4 {
+        const uint64_t e = min((uint64_t)b * BF_BLOCK + 4 * (uint32_t)lane, ldd - 4);
+        return *reinterpret_cast<const float4 *>(row + e);
+    };
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t b = next_block(0);
+    float4 v = b < nb ? load_block(b) : zero4;
+    while (b < nb) {
+        const uint32_t nx = next_block(b + 1);  // under the current root
+        const float4 pf = nx < nb ? load_block(nx) : zero4;
+        const uint32_t base = b * BF_BLOCK, cnt = min(BF_BLOCK, n - base);
+        const float vc[4] = {v.x, v.y, v.z, v.w};
+        uint32_t e = 0;  // element 4 * lane + c of the block is vc[c] of that lane
+        while (e < cnt) {
+            if (skip()) {
+                uint32_t best = BF_BLOCK;
+#pragma unroll
+                for (uint32_t c = 0; c < 4; ++c) {
+                    const uint32_t idx = 4 * (uint32_t)lane + c;
+                    const uint64_t live = __ballot(idx >= e && idx < cnt && !(vc[c] > root));
+                    if (live) best = min(best, 4 * (uint32_t)__builtin_ctzll(live) + c);
+                }
+                if (best >= cnt) break;
+                e = best;
             }
-            const float x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)j));
-            h.push(x, base + j);
+            const uint32_t c = e & 3;
+            const float sel = c == 0 ? vc[0] : (c == 1 ? vc[1] : (c == 2 ? vc[2] : vc[3]));
+            const float x =
+                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sel), (int)(e >> 2)));
+            h.push(x, base + e);
             if (h.len > k) {
                 float a;
-                uint32_t b;
-                h.pop(&a, &b);
+                uint32_t bb;
+                h.pop(&a, &bb);
             }
             if (h.len == k) {
                 root = h.gd(0);
                 stable = root < inf && h.noop_roundtrip();
             }
-            ++j;
+            ++e;
         }
-    });
+        // While the layout stays stable the root only falls, so the next
+        // block is at or after nx; after a loss of stability it is b + 1.
+        const uint32_t b2 = next_block(b + 1);
+        v = b2 == nx ? pf : (b2 < nb ? load_block(b2) : v);
+        b = b2;
+    }
     const uint32_t cntk = h.len;
     for (uint32_t p = cntk; p > 0; --p) {
         float a;
@@ -816,8 +894,12 @@ void launch_bf_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ld
         fail(HM_ERR_INVALID_ARG, "bf_finalize: rows must be 16-byte aligned, stride a multiple of 4");
     ProfScope prof("bf_finalize", s);
     if (k + 1 <= 64) {
-        bf_finalize_wave_kernel<<<rows, 64, 0, s>>>(cand_d, cand_i, ldc, flags, D, ldd, n, k,
-                                                     out_idx, out_dist, out_count, err);
+        const uint32_t nb = (n + BF_BLOCK - 1) / BF_BLOCK;
+        uint32_t bpg = 1;
+        while ((nb + bpg - 1) / bpg > BF_GMIN_MAX) bpg <<= 1;
+        const size_t lds = (size_t)((nb + bpg - 1) / bpg) * 4;
+        bf_finalize_wave_kernel<<<rows, 256, lds, s>>>(cand_d, cand_i, ldc, flags, D, ldd, n, k,
+                                                       bpg, out_idx, out_dist, out_count, err);
     } else {
         const size_t lds = (size_t)(k + 1) * 8;  // <= 64 KiB
         bf_replay_lds_kernel<<<rows, 64, lds, s>>>(cand_d, cand_i, ldc, flags, D, ldd, n, k,
