@@ -11,6 +11,7 @@
 // reference carries in heap.rs:179-270) over the row in index order.
 #include "kernels.h"
 #include "profiler.h"
+#include "wave_heap.h"
 
 namespace hm {
 
@@ -341,9 +342,6 @@ struct LaneHeap {
     }
 };
 
-// ---- std BinaryHeap held across the lanes of one wave: lane j owns slot j.
-// Every operation runs with uniform control flow; a slot is read with
-// v_readlane (uniform index) and written by the owning lane.
 // Streams a distance row in batches of 4096 values (16 float4 loads per
 // lane, the next batch in flight while the current one is processed) and
 // hands over, in index order, the 64-value chunks that may hold a value not
@@ -409,99 +407,9 @@ __device__ void stream_row(const float *__restrict__ row, uint32_t n, uint64_t l
 #undef HM_ROW_LOAD
 }
 
-struct WaveHeap {
-    float d;     // this lane's slot
-    uint32_t i;  // this lane's id
-    uint32_t len;
-    int lane;
 
-    __device__ float gd(uint32_t p) const
-    {
-        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), (int)p));
-    }
-    __device__ uint32_t gi(uint32_t p) const
-    {
-        return (uint32_t)__builtin_amdgcn_readlane((int)i, (int)p);
-    }
-    __device__ void put(uint32_t p, float v, uint32_t id)
-    {
-        if ((uint32_t)lane == p) {
-            d = v;
-            i = id;
-        }
-    }
-    __device__ void sift_up(uint32_t start, uint32_t pos)
-    {
-        const float ed = gd(pos);
-        const uint32_t ei = gi(pos);
-        while (pos > start) {
-            const uint32_t parent = (pos - 1) / 2;
-            const float pd = gd(parent);
-            if (ed <= pd) break;
-            put(pos, pd, gi(parent));
-            pos = parent;
-        }
-        put(pos, ed, ei);
-    }
-    __device__ void sift_down_to_bottom(uint32_t pos)
-    {
-        const uint32_t end = len, start = pos;
-        const float ed = gd(pos);
-        const uint32_t ei = gi(pos);
-        uint32_t child = 2 * pos + 1;
-        while (end >= 2 && child < end - 1) {
-            child += (gd(child) <= gd(child + 1)) ? 1u : 0u;
-            put(pos, gd(child), gi(child));
-            pos = child;
-            child = 2 * pos + 1;
-        }
-        if (end >= 1 && child == end - 1) {
-            put(pos, gd(child), gi(child));
-            pos = child;
-        }
-        put(pos, ed, ei);
-        sift_up(start, pos);
-    }
-    __device__ void push(float x, uint32_t id)
-    {
-        put(len, x, id);
-        len += 1;
-        sift_up(0, len - 1);
-    }
-    // heap_noop_roundtrip (below) on the lane-resident heap
-    __device__ bool noop_roundtrip() const
-    {
-        const uint32_t L = len;
-        if (L == 0) return true;
-        uint32_t a = L, p = (a - 1) / 2;
-        if (a == 2 * p + 2 && !(gd(p) > gd(L - 1))) return false;
-        a = p;
-        while (a > 0) {
-            p = (a - 1) / 2;
-            if (a == 2 * p + 1 && a + 1 < L && !(gd(p) > gd(a + 1))) return false;
-            a = p;
-        }
-        return true;
-    }
-    __device__ void pop(float *od, uint32_t *oi)
-    {
-        const float ld = gd(len - 1);
-        const uint32_t li = gi(len - 1);
-        len -= 1;
-        if (len > 0) {
-            *od = gd(0);
-            *oi = gi(0);
-            put(0, ld, li);
-            sift_down_to_bottom(0);
-        } else {
-            *od = ld;
-            *oi = li;
-        }
-    }
-};
-
-// Replays bruteforce_idx.rs:47-73 over a tied row with the heap in the
-// lanes of one wave (k + 1 <= 64).  An element x greater than the heap's
+// Replays bruteforce_idx.rs:47-73 over a tied row (k + 1 <= 64) with the
+// heap in LDS, run by one wave (wave_heap.h).  An element x greater than the heap's
 // maximum is pushed and popped again; that round trip permutes the layout
 // in a way that depends on the layout only, so when it is known to leave
 // the current layout unchanged (heap_noop_roundtrip, checked after every
@@ -562,7 +470,9 @@ __global__ __launch_bounds__(256) void bf_finalize_wave_kernel(
     // an element, with the next block loading while it works on the current
     // one.  The row's NaN-free (tied rows only), so fkey order is value order.
     const uint32_t nb = (n + BF_BLOCK - 1) / BF_BLOCK, ng = (nb + bpg - 1) / bpg;
-    extern __shared__ uint32_t gkey[];  // [ng]
+    extern __shared__ __align__(16) uint2 bf_smem[];
+    uint2 *he = bf_smem;                                                 // heap, [k + 2]
+    uint32_t *gkey = reinterpret_cast<uint32_t *>(bf_smem + ((k + 3) & ~1u));  // [ng]
     for (uint32_t g = tid; g < ng; g += 256) gkey[g] = ~0u;
     __syncthreads();
     for (uint32_t b0 = (uint32_t)wave * 16; b0 < nb; b0 += 4 * 16) {
@@ -586,9 +496,28 @@ __global__ __launch_bounds__(256) void bf_finalize_wave_kernel(
     }
     __syncthreads();
     if (wave != 0) return;
-    WaveHeap h{0.0f, 0u, 0u, lane};
+    // the reference's heap (wave_heap.h) in LDS: (item, distance bits)
+    WaveHeap<WhPackedLds> h({he}, (uint32_t)lane);
     bool stable = false;
     float root = inf;
+    auto val = [&](uint32_t pos) { return __uint_as_float(he[pos + 1].y); };
+    // heap_noop_roundtrip on the LDS heap, all ancestor steps at once: lane
+    // j checks step j of the walk from position L = len to the root
+    auto noop = [&]() -> bool {
+        const uint32_t L = h.len;
+        if (L == 0) return true;
+        const uint32_t steps = 32u - (uint32_t)__clz(L + 1);  // ancestors of L, plus L
+        bool bad = false;
+        if ((uint32_t)lane < steps) {
+            const uint32_t a = ((L + 1) >> lane) - 1;
+            if (a > 0) {
+                const uint32_t p = (a - 1) / 2;
+                if (lane == 0) bad = a == 2 * p + 2 && !(val(p) > val(L - 1));
+                else bad = a == 2 * p + 1 && a + 1 < L && !(val(p) > val(a + 1));
+            }
+        }
+        return __ballot(bad) == 0;
+    };
     auto skip = [&]() { return h.len == k && stable; };
     // first block at or after b that can change the heap: inside a group
     // being visited the next block; else the first block of the next group
@@ -608,12 +537,13 @@ __global__ __launch_bounds__(256) void bf_finalize_wave_kernel(
         const uint64_t e = min((uint64_t)b * BF_BLOCK + 4 * (uint32_t)lane, ldd - 4);
         return *reinterpret_cast<const float4 *>(row + e);
     };
-    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    // past the end: the last block again (an in-bounds load, never used)
+    auto ld = [&](uint32_t b) { return load_block(min(b, nb - 1)); };
     uint32_t b = next_block(0);
-    float4 v = b < nb ? load_block(b) : zero4;
+    float4 v = ld(b);
     while (b < nb) {
         const uint32_t nx = next_block(b + 1);  // under the current root
-        const float4 pf = nx < nb ? load_block(nx) : zero4;
+        const float4 pf = ld(nx);
         const uint32_t base = b * BF_BLOCK, cnt = min(BF_BLOCK, n - base);
         const float vc[4] = {v.x, v.y, v.z, v.w};
         uint32_t e = 0;  // element 4 * lane + c of the block is vc[c] of that lane
@@ -633,32 +563,26 @@ __global__ __launch_bounds__(256) void bf_finalize_wave_kernel(
             const float sel = c == 0 ? vc[0] : (c == 1 ? vc[1] : (c == 2 ? vc[2] : vc[3]));
             const float x =
                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sel), (int)(e >> 2)));
-            h.push(x, base + e);
-            if (h.len > k) {
-                float a;
-                uint32_t bb;
-                h.pop(&a, &bb);
-            }
+            h.push(make_uint2(base + e, __float_as_uint(x)));
+            if (h.len > k) (void)h.pop();
             if (h.len == k) {
-                root = h.gd(0);
-                stable = root < inf && h.noop_roundtrip();
+                root = val(0);
+                stable = root < inf && noop();
             }
             ++e;
         }
         // While the layout stays stable the root only falls, so the next
         // block is at or after nx; after a loss of stability it is b + 1.
         const uint32_t b2 = next_block(b + 1);
-        v = b2 == nx ? pf : (b2 < nb ? load_block(b2) : v);
+        v = b2 == nx ? pf : ld(b2);
         b = b2;
     }
     const uint32_t cntk = h.len;
     for (uint32_t p = cntk; p > 0; --p) {
-        float a;
-        uint32_t b;
-        h.pop(&a, &b);
+        const uint2 x = h.pop();
         if (lane == 0) {
-            oi[p - 1] = b;
-            od[p - 1] = a;
+            oi[p - 1] = x.x;
+            od[p - 1] = __uint_as_float(x.y);
         }
     }
     for (uint32_t j = cntk + lane; j < k; j += 64) {
@@ -897,7 +821,7 @@ void launch_bf_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ld
         const uint32_t nb = (n + BF_BLOCK - 1) / BF_BLOCK;
         uint32_t bpg = 1;
         while ((nb + bpg - 1) / bpg > BF_GMIN_MAX) bpg <<= 1;
-        const size_t lds = (size_t)((nb + bpg - 1) / bpg) * 4;
+        const size_t lds = (size_t)((k + 3) & ~1u) * 8 + (size_t)((nb + bpg - 1) / bpg) * 4;
         bf_finalize_wave_kernel<<<rows, 256, lds, s>>>(cand_d, cand_i, ldc, flags, D, ldd, n, k,
                                                        bpg, out_idx, out_dist, out_count, err);
     } else {
