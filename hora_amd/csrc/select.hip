@@ -710,6 +710,11 @@ __global__ __launch_bounds__(64) void bf_replay_lds_kernel(
 // NaN values are never taken.
 constexpr uint32_t KTH_MAX = 32;
 
+// One wave per row: each lane keeps the KT smallest of its values (KT >= m)
+// as a sorted run; the row is read 4,096 values per round with every load
+// of the round in flight (16-byte loads when the row allows them), then m
+// rounds of a wave minimum over the lanes' heads give the m-th smallest.
+template <uint32_t KT, bool VEC>
 __global__ __launch_bounds__(256) void kth_smallest_kernel(const float *__restrict__ D,
                                                            uint64_t ldd, uint32_t rows, uint32_t n,
                                                            uint32_t m, float *__restrict__ out,
@@ -720,19 +725,48 @@ __global__ __launch_bounds__(256) void kth_smallest_kernel(const float *__restri
     if (r >= rows || (dev_rows && r >= *dev_rows)) return;
     const float inf = __int_as_float(0x7f800000);
     const float *row = D + (size_t)r * ldd;
-    float best[KTH_MAX];
+    float best[KT];
 #pragma unroll
-    for (uint32_t i = 0; i < KTH_MAX; ++i) best[i] = inf;
-    for (uint32_t i = lane; i < n; i += 64) {
-        float x = row[i];
-        if (!(x < best[KTH_MAX - 1])) continue;  // also drops NaN
-        // insert x into the sorted run best[0..KTH_MAX)
+    for (uint32_t i = 0; i < KT; ++i) best[i] = inf;
+    auto insert = [&](float x) {
+        if (!(x < best[KT - 1])) return;  // also drops NaN
 #pragma unroll
-        for (uint32_t t = 0; t < KTH_MAX; ++t) {
+        for (uint32_t t = 0; t < KT; ++t) {
             const float b = best[t];
             const bool lt = x < b;
             best[t] = lt ? x : b;
             x = lt ? b : x;
+        }
+    };
+    constexpr uint32_t U = 16;
+    for (uint64_t b0 = 0; b0 < n; b0 += (uint64_t)U * 256) {
+        if constexpr (VEC) {
+            float4 v[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint64_t e = min(b0 + u * 256 + 4 * (uint32_t)lane, ldd - 4);
+                v[u] = *reinterpret_cast<const float4 *>(row + e);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint64_t e = b0 + u * 256 + 4 * (uint32_t)lane;
+                if (e < n) insert(v[u].x);
+                if (e + 1 < n) insert(v[u].y);
+                if (e + 2 < n) insert(v[u].z);
+                if (e + 3 < n) insert(v[u].w);
+            }
+        } else {
+            // scalar loads: the round in two halves of 32 per lane
+#pragma unroll
+            for (uint32_t hh = 0; hh < 2; ++hh) {
+                float v[2 * U];
+#pragma unroll
+                for (uint32_t u = 0; u < 2 * U; ++u)
+                    v[u] = row[min(b0 + (hh * 2 * U + u) * 64 + (uint32_t)lane, (uint64_t)n - 1)];
+#pragma unroll
+                for (uint32_t u = 0; u < 2 * U; ++u)
+                    if (b0 + (hh * 2 * U + u) * 64 + (uint32_t)lane < n) insert(v[u]);
+            }
         }
     }
     // m rounds: the wave minimum of the lanes' heads; its lane drops its head
@@ -750,9 +784,9 @@ __global__ __launch_bounds__(256) void kth_smallest_kernel(const float *__restri
             // shift: the lane's next smallest becomes its head
             float nxt = inf;
 #pragma unroll
-            for (uint32_t t = 1; t < KTH_MAX; ++t)
+            for (uint32_t t = 1; t < KT; ++t)
                 if (t == used) nxt = best[t];
-            head = used < KTH_MAX ? nxt : inf;
+            head = used < KT ? nxt : inf;
         }
     }
     if (lane == 0) out[r] = kth;
@@ -838,7 +872,22 @@ void launch_kth_smallest(const float *D, uint64_t ldd, uint32_t rows, uint32_t n
     if (rows == 0) return;
     if (m == 0 || m > KTH_MAX || m > n) fail(HM_ERR_INVALID_ARG, "kth_smallest: m out of range");
     ProfScope prof("kth_smallest", s);
-    kth_smallest_kernel<<<ceil_div(rows, 4), 256, 0, s>>>(D, ldd, rows, n, m, out, dev_rows);
+    const bool vec = ldd % 4 == 0 && ((uintptr_t)D & 15) == 0 && ldd >= 4;
+#define HM_KTH(KT)                                                                             \
+    if (vec)                                                                                   \
+        kth_smallest_kernel<KT, true><<<ceil_div(rows, 4), 256, 0, s>>>(D, ldd, rows, n, m, out, \
+                                                                        dev_rows);             \
+    else                                                                                       \
+        kth_smallest_kernel<KT, false><<<ceil_div(rows, 4), 256, 0, s>>>(D, ldd, rows, n, m,     \
+                                                                         out, dev_rows)
+    if (m <= 8) {
+        HM_KTH(8);
+    } else if (m <= 16) {
+        HM_KTH(16);
+    } else {
+        HM_KTH(32);
+    }
+#undef HM_KTH
     HM_CHECK_LAUNCH();
 }
 
