@@ -244,20 +244,49 @@ __global__ __launch_bounds__(256) void exact_rows_kernel(
 }
 
 // s = sum_t x_t*x_t as a left fold from -0.0 (the reference's dot(x,x) up to
-// sign, calc.rs:13-18), one lane per row.
-__global__ void self_dot_kernel(const float *__restrict__ X, uint32_t ldx, uint32_t rows,
-                                uint32_t d, float *__restrict__ out,
-                                const uint32_t *__restrict__ dev_rows)
+// sign, calc.rs:13-18), one lane per row.  The workgroup's 256 rows pass
+// through LDS 16 dimensions at a time (coalesced 16-byte loads when VEC:
+// rows 16-byte aligned with a stride that is a multiple of 16 floats).
+template <bool VEC>
+__global__ __launch_bounds__(256) void self_dot_kernel(const float *__restrict__ X, uint32_t ldx,
+                                                       uint32_t rows, uint32_t d,
+                                                       float *__restrict__ out,
+                                                       const uint32_t *__restrict__ dev_rows)
 {
-    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows || (dev_rows && r >= *dev_rows)) return;
-    const float *x = X + (size_t)r * ldx;
+    __shared__ float sl[256][17];
+    const uint32_t nrows = dev_rows ? min(rows, *dev_rows) : rows;
+    const uint32_t r0 = blockIdx.x * 256, tid = threadIdx.x;
+    if (r0 >= nrows) return;
     float acc = -0.0f;
-    for (uint32_t t = 0; t < d; ++t) {
-        float p = x[t] * x[t];
-        acc = acc + p;
+    for (uint32_t t0 = 0; t0 < d; t0 += 16) {
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t rl = tid / 4 + 64 * j, c = (tid % 4) * 4;
+            const float *src = X + (size_t)min(r0 + rl, nrows - 1) * ldx + t0 + c;
+            float4 v;
+            if constexpr (VEC) {
+                v = *reinterpret_cast<const float4 *>(src);
+            } else {
+                v.x = t0 + c < d ? src[0] : 0.0f;
+                v.y = t0 + c + 1 < d ? src[1] : 0.0f;
+                v.z = t0 + c + 2 < d ? src[2] : 0.0f;
+                v.w = t0 + c + 3 < d ? src[3] : 0.0f;
+            }
+            sl[rl][c] = v.x;
+            sl[rl][c + 1] = v.y;
+            sl[rl][c + 2] = v.z;
+            sl[rl][c + 3] = v.w;
+        }
+        __syncthreads();
+        const uint32_t tn = min(16u, d - t0);
+        for (uint32_t t = 0; t < tn; ++t) {
+            const float x = sl[tid][t];
+            const float p = x * x;
+            acc = acc + p;
+        }
+        __syncthreads();
     }
-    out[r] = acc;
+    if (r0 + tid < nrows) out[r0 + tid] = acc;
 }
 
 // Batched distance service of the graph searches: one lane per (query,
@@ -415,7 +444,10 @@ void launch_self_dot(const float *X, uint32_t ldx, uint32_t rows, uint32_t d, fl
                      hipStream_t s, const uint32_t *dev_rows)
 {
     if (rows == 0) return;
-    self_dot_kernel<<<ceil_div(rows, 256), 256, 0, s>>>(X, ldx, rows, d, out, dev_rows);
+    if (ldx % 16 == 0 && ((uintptr_t)X & 15) == 0 && d <= ldx)
+        self_dot_kernel<true><<<ceil_div(rows, 256), 256, 0, s>>>(X, ldx, rows, d, out, dev_rows);
+    else
+        self_dot_kernel<false><<<ceil_div(rows, 256), 256, 0, s>>>(X, ldx, rows, d, out, dev_rows);
     HM_CHECK_LAUNCH();
 }
 
