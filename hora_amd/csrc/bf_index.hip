@@ -175,13 +175,18 @@ class BruteForce final : public Index {
         launch_check_finite(dq, dim_, nq, (uint32_t)dim_, err_.as<uint32_t>(), s);
         const bool shortlist_form = shortlist_usable(s);
         last_fallback_ = last_uncertified_ = 0;
+        herr_.reserve(sizeof(uint32_t));
+        bool err_read = false;  // the shortlist form may read the error word itself
         if (shortlist_form)
-            search_shortlist(dq, nq, k, d_idx, d_dist, d_count, s);
+            err_read = search_shortlist(dq, nq, k, d_idx, d_dist, d_count, s);
         else
             search_exact(dq, dim_, nq, k, d_idx, d_dist, d_count, s);
-        uint32_t err = 0;
-        HM_HIP(hipMemcpyAsync(&err, err_.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HM_HIP(hipStreamSynchronize(s));
+        if (!err_read) {
+            HM_HIP(hipMemcpyAsync(herr_.as<uint32_t>(), err_.get(), sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, s));
+            HM_HIP(hipStreamSynchronize(s));
+        }
+        const uint32_t err = *herr_.as<uint32_t>();
         if (err & 2u) fail(HM_ERR_NONFINITE_INPUT, "invalid float element");
         if (err & 4u) fail(HM_ERR_HIP, "internal: selection consistency check failed");
         if (err & 1u) fail(HM_ERR_NAN_DISTANCE, "NaN distance (partial_cmp of NaN)");
@@ -629,7 +634,9 @@ class BruteForce final : public Index {
 
     // MFMA shortlist (BF16 or FP8) + exact rerank; rows that are tied or not
     // certified are rerun on the exact path.
-    void search_shortlist(const float *dq, uint64_t nq, uint32_t k, uint64_t *d_idx,
+    // Returns true when no row needed the exact path: the error word was
+    // then read (into herr_) together with the row flags.
+    bool search_shortlist(const float *dq, uint64_t nq, uint32_t k, uint64_t *d_idx,
                           float *d_dist, uint32_t *d_count, hipStream_t s)
     {
         ensure_shortlist_data(s);
@@ -729,40 +736,51 @@ class BruteForce final : public Index {
                           d_dist + q0 * k, d_count + q0, rflags_.as<uint32_t>() + q0, s,
                           nullptr, nullptr, nullptr, s_q, key_s, pre);
         }
-        // Rows needing the exact path (ties: heap replay; uncertified).
-        std::vector<uint32_t> fl(nq);
-        HM_HIP(hipMemcpyAsync(fl.data(), rflags_.get(), nq * sizeof(uint32_t),
+        // Rows needing the exact path (ties: heap replay; uncertified): the
+        // flags and the error word come back in one pinned copy.
+        hflags_.reserve(nq * sizeof(uint32_t));
+        hrows_.reserve(nq * sizeof(uint32_t));
+        HM_HIP(hipMemcpyAsync(hflags_.as<uint32_t>(), rflags_.get(), nq * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, s));
+        HM_HIP(hipMemcpyAsync(herr_.as<uint32_t>(), err_.get(), sizeof(uint32_t),
                               hipMemcpyDeviceToHost, s));
         HM_HIP(hipStreamSynchronize(s));
+        const uint32_t *fl = hflags_.as<uint32_t>();
         // Certified rows with a tie only need the heap replay over their
-        // exact row; the others take the whole exact path.
-        std::vector<uint32_t> redo, tied;
+        // exact row; the others take the whole exact path.  Both lists go
+        // to the device from pinned memory: tied rows from the front, the
+        // others from the back.
+        uint32_t *rl = hrows_.as<uint32_t>();
+        uint32_t nt = 0, nr = 0;
         last_uncertified_ = 0;
         for (uint64_t i = 0; i < nq; ++i) {
             if (fl[i] == ROW_TIE)
-                tied.push_back((uint32_t)i);
+                rl[nt++] = (uint32_t)i;
             else if (fl[i] & (ROW_TIE | ROW_NAN | ROW_UNCERT | ROW_BAD))
-                redo.push_back((uint32_t)i);
+                rl[nq - 1 - nr++] = (uint32_t)i;
             last_uncertified_ += (fl[i] & ROW_UNCERT) ? 1 : 0;
         }
-        last_fallback_ = redo.size() + tied.size();
-        if (!tied.empty()) rerun_rows(dq, tied, k, d_idx, d_dist, d_count, true, s);
-        if (!redo.empty()) rerun_rows(dq, redo, k, d_idx, d_dist, d_count, false, s);
+        last_fallback_ = nt + nr;
+        if (nt) rerun_rows(dq, rl, nt, k, d_idx, d_dist, d_count, true, s);
+        if (nr) rerun_rows(dq, rl + (nq - nr), nr, k, d_idx, d_dist, d_count, false, s);
+        return nt + nr == 0;
     }
 
-    // Exact path for the listed query rows; results scattered back.
-    void rerun_rows(const float *dq, const std::vector<uint32_t> &redo, uint32_t k,
+    // Exact path for the listed query rows (m entries of `rows`, in pinned
+    // host memory that stays unchanged until the stream has run the copy);
+    // results scattered back.  No synchronisation: the caller reads the
+    // error word after the last launch.
+    void rerun_rows(const float *dq, const uint32_t *rows, uint32_t m, uint32_t k,
                     uint64_t *d_idx, float *d_dist, uint32_t *d_count, bool all_tied,
                     hipStream_t s)
     {
-        const uint32_t m = (uint32_t)redo.size();
         redo_rows_.reserve(m * sizeof(uint32_t));
         redo_q_.reserve((size_t)m * dim_ * sizeof(float));
         redo_i_.reserve((size_t)m * k * sizeof(uint64_t));
         redo_d_.reserve((size_t)m * k * sizeof(float));
         redo_c_.reserve((size_t)m * sizeof(uint32_t));
-        HM_HIP(hipMemcpyAsync(redo_rows_.get(), redo.data(), m * sizeof(uint32_t),
-                              hipMemcpyHostToDevice, s));
+        HM_HIP(hipMemcpyAsync(redo_rows_.get(), rows, m * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              s));
         launch_gather_rows(dq, dim_, redo_rows_.as<uint32_t>(), m, (uint32_t)dim_,
                            redo_q_.as<float>(), dim_, s);
         search_exact(redo_q_.as<float>(), dim_, m, k, redo_i_.as<uint64_t>(), redo_d_.as<float>(),
@@ -770,7 +788,6 @@ class BruteForce final : public Index {
         launch_scatter_results(redo_i_.as<uint64_t>(), redo_d_.as<float>(),
                                redo_c_.as<uint32_t>(), redo_rows_.as<uint32_t>(), m, k, d_idx,
                                d_dist, d_count, s);
-        HM_HIP(hipStreamSynchronize(s));
     }
 
   public:
@@ -940,6 +957,7 @@ class BruteForce final : public Index {
     DeviceBuffer bsamp_, bs_norm2_, bs_inv_, bs_scale_;
     DeviceBuffer Ds_, sel_d_, sel_i_, sel_f_, list_, lcnt_, sq_, ksq_, pref_;
     DeviceBuffer redo_rows_, redo_q_, redo_i_, redo_d_, redo_c_;
+    PinnedBuffer hflags_, hrows_, herr_;  // row flags, rerun rows, error word
     uint64_t last_fallback_ = 0, last_uncertified_ = 0;
     // ragged graph
     hipStream_t rg_stream_ = nullptr;

@@ -54,6 +54,23 @@ class DeviceBuffer {
     size_t bytes_ = 0;
 };
 
+// Owning pinned host allocation (hipHostMalloc), grown on demand: small
+// device-to-host status copies go straight to it by DMA.
+class PinnedBuffer {
+  public:
+    PinnedBuffer() = default;
+    ~PinnedBuffer();
+    PinnedBuffer(const PinnedBuffer &) = delete;
+    PinnedBuffer &operator=(const PinnedBuffer &) = delete;
+    // Ensure at least `bytes`; contents are discarded when grown.
+    void reserve(size_t bytes);
+    template <class T> T *as() const { return static_cast<T *>(ptr_); }
+
+  private:
+    void *ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
+
 inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 

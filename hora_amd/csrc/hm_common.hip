@@ -23,6 +23,29 @@ void DeviceBuffer::release()
     bytes_ = 0;
 }
 
+PinnedBuffer::~PinnedBuffer()
+{
+    if (ptr_) (void)hipHostFree(ptr_);
+}
+
+void PinnedBuffer::reserve(size_t bytes)
+{
+    if (bytes <= bytes_) return;
+    if (ptr_) (void)hipHostFree(ptr_);
+    ptr_ = nullptr;
+    bytes_ = 0;
+    const size_t want = std::max<size_t>(bytes, 256);
+    void *p = nullptr;
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(HM_ERR_OUT_OF_MEMORY, "hipHostMalloc of " + std::to_string(want) +
+                                       " bytes failed: " + hipGetErrorString(e));
+    }
+    ptr_ = p;
+    bytes_ = want;
+}
+
 void DeviceBuffer::reserve(size_t bytes)
 {
     if (bytes <= bytes_) return;
