@@ -260,13 +260,11 @@ class BruteForce final : public Index {
             launch_exact_tile(metric_, qpad_.as<float>(), dpad_, rows, base_.as<float>(), dpad_,
                               n, dpad_, qnorm_.as<float>(), bnorm_.as<float>(), D_.as<float>(),
                               ldd, s);
-            if (all_tied)
-                launch_fill_u32(flags_.as<uint32_t>(), rows, ROW_TIE, s);
-            else
+            if (!all_tied)
                 launch_row_select(D_.as<float>(), ldd, rows, n, K, cand_d_.as<float>(),
                                   cand_i_.as<uint32_t>(), K, flags_.as<uint32_t>(), s);
             launch_bf_finalize(cand_d_.as<float>(), cand_i_.as<uint32_t>(), K,
-                               flags_.as<uint32_t>(), D_.as<float>(), ldd, rows, n, k,
+                               all_tied ? nullptr : flags_.as<uint32_t>(), D_.as<float>(), ldd, rows, n, k,
                                d_idx + q0 * k, d_dist + q0 * k, d_count + q0,
                                err_.as<uint32_t>(), s);
         }

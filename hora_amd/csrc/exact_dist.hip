@@ -244,24 +244,24 @@ __global__ __launch_bounds__(256) void exact_rows_kernel(
 }
 
 // s = sum_t x_t*x_t as a left fold from -0.0 (the reference's dot(x,x) up to
-// sign, calc.rs:13-18), one lane per row.  The workgroup's 256 rows pass
+// sign, calc.rs:13-18), one lane per row.  The workgroup's 64 rows pass
 // through LDS 16 dimensions at a time (coalesced 16-byte loads when VEC:
 // rows 16-byte aligned with a stride that is a multiple of 16 floats).
 template <bool VEC>
-__global__ __launch_bounds__(256) void self_dot_kernel(const float *__restrict__ X, uint32_t ldx,
-                                                       uint32_t rows, uint32_t d,
-                                                       float *__restrict__ out,
-                                                       const uint32_t *__restrict__ dev_rows)
+__global__ __launch_bounds__(64) void self_dot_kernel(const float *__restrict__ X, uint32_t ldx,
+                                                      uint32_t rows, uint32_t d,
+                                                      float *__restrict__ out,
+                                                      const uint32_t *__restrict__ dev_rows)
 {
-    __shared__ float sl[256][17];
+    __shared__ float sl[64][17];
     const uint32_t nrows = dev_rows ? min(rows, *dev_rows) : rows;
-    const uint32_t r0 = blockIdx.x * 256, tid = threadIdx.x;
+    const uint32_t r0 = blockIdx.x * 64, tid = threadIdx.x;
     if (r0 >= nrows) return;
     float acc = -0.0f;
     for (uint32_t t0 = 0; t0 < d; t0 += 16) {
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
-            const uint32_t rl = tid / 4 + 64 * j, c = (tid % 4) * 4;
+            const uint32_t rl = tid / 4 + 16 * j, c = (tid % 4) * 4;
             const float *src = X + (size_t)min(r0 + rl, nrows - 1) * ldx + t0 + c;
             float4 v;
             if constexpr (VEC) {
@@ -445,9 +445,9 @@ void launch_self_dot(const float *X, uint32_t ldx, uint32_t rows, uint32_t d, fl
 {
     if (rows == 0) return;
     if (ldx % 16 == 0 && ((uintptr_t)X & 15) == 0 && d <= ldx)
-        self_dot_kernel<true><<<ceil_div(rows, 256), 256, 0, s>>>(X, ldx, rows, d, out, dev_rows);
+        self_dot_kernel<true><<<ceil_div(rows, 64), 64, 0, s>>>(X, ldx, rows, d, out, dev_rows);
     else
-        self_dot_kernel<false><<<ceil_div(rows, 256), 256, 0, s>>>(X, ldx, rows, d, out, dev_rows);
+        self_dot_kernel<false><<<ceil_div(rows, 64), 64, 0, s>>>(X, ldx, rows, d, out, dev_rows);
     HM_CHECK_LAUNCH();
 }
 

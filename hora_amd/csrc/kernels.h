@@ -64,8 +64,8 @@ void launch_kth_smallest(const float *D, uint64_t ldd, uint32_t rows, uint32_t n
 // Final result of node_search_k per row: rows without ROW_TIE copy the
 // first min(k, n) candidates; rows with ROW_TIE replay the reference's
 // std-BinaryHeap push/pop stream over the whole distance row.  A NaN row
-// sets bit 0 of *err (when k >= 1 and n >= 2).  Indices are offset by
-// idx_base.
+// sets bit 0 of *err (when k >= 1 and n >= 2).  flags may be null: every
+// row is then tied (and free of NaN), and the candidates are not read.
 void launch_bf_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ldc,
                         const uint32_t *flags, const float *D, uint64_t ldd, uint32_t rows,
                         uint32_t n, uint32_t k, uint64_t *out_idx, float *out_dist,

@@ -441,7 +441,7 @@ __global__ __launch_bounds__(256) void bf_finalize_wave_kernel(
     uint64_t *oi = out_idx + (size_t)r * k;
     float *od = out_dist + (size_t)r * k;
     const uint32_t m = k < n ? k : n;
-    const uint32_t f = flags[r];
+    const uint32_t f = flags ? flags[r] : ROW_TIE;  // no flags: every row tied
     if ((f & (ROW_NAN | ROW_BAD)) && n >= 2) {
         if (tid == 0) atomicOr(err, (f & ROW_BAD) ? 4u : 1u);
         for (uint32_t j = tid; j < k; j += 256) {
@@ -634,7 +634,7 @@ __global__ __launch_bounds__(64) void bf_replay_lds_kernel(
     uint64_t *oi = out_idx + (size_t)r * k;
     float *od = out_dist + (size_t)r * k;
     const uint32_t m = k < n ? k : n;
-    const uint32_t f = flags[r];
+    const uint32_t f = flags ? flags[r] : ROW_TIE;  // no flags: every row tied
     if ((f & (ROW_NAN | ROW_BAD)) && n >= 2) {
         if (lane == 0) atomicOr(err, (f & ROW_BAD) ? 4u : 1u);
         for (uint32_t j = lane; j < k; j += 64) {
