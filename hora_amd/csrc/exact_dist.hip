@@ -295,35 +295,68 @@ __global__ __launch_bounds__(64) void self_dot_kernel(const float *__restrict__ 
 // [off[q], off[q+1]) belong to query q.  A node index out of range writes
 // NaN and sets *err bit 2.
 template <int ACC, int EPI>
-__global__ __launch_bounds__(256) void pair_dist_kernel(
+__global__ __launch_bounds__(64) void pair_dist_kernel(
     const float *__restrict__ Qp, uint32_t dpad, uint32_t nq, const float *__restrict__ B,
     uint32_t ldb, uint32_t n, const uint32_t *__restrict__ ids, const uint64_t *__restrict__ off,
     uint64_t total, const float *__restrict__ qnorm, const float *__restrict__ bnorm,
     float *__restrict__ out, uint32_t *__restrict__ err)
 {
-    const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (p >= total) return;
-    uint32_t lo = 0, hi = nq;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (off[mid] <= p) lo = mid;
-        else hi = mid;
+    // One wave of 64 pairs.  Each lane folds its own pair in order; the 64
+    // node rows and query rows pass through LDS 16 dimensions at a time,
+    // loaded as coalesced 64-byte row segments (rows padded to 16 floats).
+    __shared__ float sb[64][17], sq[64][17];
+    __shared__ uint32_t s_id[64], s_q[64];
+    const uint32_t lane = threadIdx.x;
+    const uint64_t p = (uint64_t)blockIdx.x * 64 + lane;
+    const bool live = p < total;
+    if (n == 0) {  // no rows: every index is out of range, nothing to load
+        if (live) {
+            out[p] = __int_as_float(0x7fc00000);
+            atomicOr(err, 4u);
+        }
+        return;
     }
-    const uint32_t q = lo, id = ids[p];
-    if (id >= n) {
+    uint32_t q = 0, id = 0;
+    if (live) {
+        uint32_t lo = 0, hi = nq;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (off[mid] <= p) lo = mid;
+            else hi = mid;
+        }
+        q = lo;
+        id = ids[p];
+    }
+    const bool bad = live && id >= n;
+    s_id[lane] = (live && !bad) ? id : 0u;  // rows the loads may read
+    s_q[lane] = q;
+    __syncthreads();
+    float acc = -0.0f;
+    for (uint32_t t0 = 0; t0 < dpad; t0 += 16) {
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t r = lane / 4 + 16 * j, c = (lane % 4) * 4;
+            const float4 bv = *reinterpret_cast<const float4 *>(B + (size_t)s_id[r] * ldb + t0 + c);
+            const float4 qv = *reinterpret_cast<const float4 *>(Qp + (size_t)s_q[r] * dpad + t0 + c);
+            sb[r][c] = bv.x;
+            sb[r][c + 1] = bv.y;
+            sb[r][c + 2] = bv.z;
+            sb[r][c + 3] = bv.w;
+            sq[r][c] = qv.x;
+            sq[r][c + 1] = qv.y;
+            sq[r][c + 2] = qv.z;
+            sq[r][c + 3] = qv.w;
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t t = 0; t < 16; ++t) acc = acc_step<ACC>(acc, sb[lane][t], sq[lane][t]);
+        __syncthreads();
+    }
+    if (!live) return;
+    if (bad) {
         out[p] = __int_as_float(0x7fc00000);
         atomicOr(err, 4u);
         return;
-    }
-    const float4 *a = reinterpret_cast<const float4 *>(B + (size_t)id * ldb);
-    const float4 *b = reinterpret_cast<const float4 *>(Qp + (size_t)q * dpad);
-    float acc = -0.0f;
-    for (uint32_t t = 0; t < dpad / 4; ++t) {
-        const float4 x = a[t], y = b[t];
-        acc = acc_step<ACC>(acc, x.x, y.x);
-        acc = acc_step<ACC>(acc, x.y, y.y);
-        acc = acc_step<ACC>(acc, x.z, y.z);
-        acc = acc_step<ACC>(acc, x.w, y.w);
     }
     if constexpr (EPI == EPI_COSINE) acc = epi_cosine(acc, bnorm[id], qnorm[q]);
     else if constexpr (EPI == EPI_ANGULAR) acc = epi_angular(acc, bnorm[id], qnorm[q]);
@@ -422,12 +455,15 @@ void launch_pair_dist(int metric, const float *Qp, uint32_t dpad, uint32_t nq, c
                       uint32_t *err, hipStream_t s)
 {
     if (total == 0 || nq == 0) return;
-    if (dpad % 4 || ldb % 4) fail(HM_ERR_INVALID_ARG, "pair_dist: rows must be padded to 4");
+    if (dpad % 16 || ldb % 16 || ldb < dpad || ((uintptr_t)B & 15) || ((uintptr_t)Qp & 15))
+        fail(HM_ERR_INVALID_ARG, "pair_dist: rows must be 16-byte aligned and padded to 16");
     ProfScope prof("pair_dist", s);
-    const unsigned blocks = (unsigned)((total + 255) / 256);
+    const uint64_t nblk = (total + 63) / 64;
+    if (nblk > 0x7FFFFFFFull) fail(HM_ERR_INVALID_ARG, "pair_dist: too many pairs");
+    const unsigned blocks = (unsigned)nblk;
 #define HM_PD(ACC, EPI)                                                                        \
-    pair_dist_kernel<ACC, EPI><<<blocks, 256, 0, s>>>(Qp, dpad, nq, B, ldb, n, ids, off, total,  \
-                                                      qnorm, bnorm, out, err)
+    pair_dist_kernel<ACC, EPI><<<blocks, 64, 0, s>>>(Qp, dpad, nq, B, ldb, n, ids, off, total,   \
+                                                     qnorm, bnorm, out, err)
     switch (metric) {
     case HM_METRIC_EUCLIDEAN: HM_PD(ACC_EUCLID, EPI_NONE); break;
     case HM_METRIC_MANHATTAN: HM_PD(ACC_MANHATTAN, EPI_NONE); break;
