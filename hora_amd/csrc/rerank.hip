@@ -55,6 +55,8 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
     unsigned long long *keys = reinterpret_cast<unsigned long long *>(rr_smem);
     float *vals = reinterpret_cast<float *>(rr_smem + (size_t)P * 8);
     float *sq = reinterpret_cast<float *>(rr_smem + (size_t)P * 12);
+    float *sb = sq + dpad;  // [RR_THREADS][17]: a 16-dimension slab of the candidate rows
+    __shared__ uint32_t s_rows[RR_THREADS];
 
     const int tid = threadIdx.x;
     const uint32_t r = blockIdx.x;
@@ -68,34 +70,55 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
 
     const uint32_t *ci = cand_i + (size_t)r * ldc;
     const float qn2 = qnorm2[r];  // sum q_t^2 (left fold), needed by every bound
-    for (uint32_t j = tid; j < s; j += RR_THREADS) {
-        const uint32_t idx = ci[j];
-        if (idx >= n) {  // never expected: keep the read in range and flag the row
+    // Candidates in groups of RR_THREADS, one per thread; each folds its own
+    // row in reference order while the group's rows pass through LDS 16
+    // dimensions at a time, loaded as coalesced 64-byte segments.
+    for (uint32_t g0 = 0; g0 < s; g0 += RR_THREADS) {
+        const uint32_t R = min((uint32_t)RR_THREADS, s - g0);
+        const uint32_t j = g0 + tid;
+        uint32_t idx = 0;
+        if ((uint32_t)tid < R) idx = ci[j];
+        const bool bad = (uint32_t)tid < R && idx >= n;
+        s_rows[tid] = ((uint32_t)tid < R && !bad) ? idx : 0u;  // rows the loads may read
+        __syncthreads();
+        float acc = -0.0f;
+        for (uint32_t t0 = 0; t0 < dpad; t0 += 16) {
+            for (uint32_t i = tid; i < R * 4; i += RR_THREADS) {
+                const uint32_t row = i / 4, c = (i % 4) * 4;
+                const float4 v =
+                    *reinterpret_cast<const float4 *>(Bp + (size_t)s_rows[row] * dpad + t0 + c);
+                float *d = sb + row * 17 + c;
+                d[0] = v.x;
+                d[1] = v.y;
+                d[2] = v.z;
+                d[3] = v.w;
+            }
+            __syncthreads();
+            if ((uint32_t)tid < R) {
+#pragma unroll
+                for (uint32_t e = 0; e < 16; ++e) {
+                    const float q = sq[t0 + e], bq = sb[tid * 17 + e];
+                    if constexpr (ACC == ACC_EUCLID) {
+                        float df = q - bq;
+                        float sqr = df * df;
+                        acc = acc + sqr;
+                    } else if constexpr (ACC == ACC_MANHATTAN) {
+                        float df = q - bq;
+                        acc = acc + fabsf(df);
+                    } else {
+                        float p = q * bq;
+                        acc = acc + p;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if ((uint32_t)tid >= R) continue;
+        if (bad) {  // never expected: the read stayed in range; flag the row
             s_bad = 1;
             vals[j] = __int_as_float(0x7f800000);
             keys[j] = ~0ull;
             continue;
-        }
-        const float4 *b4 = reinterpret_cast<const float4 *>(Bp + (size_t)idx * dpad);
-        float acc = -0.0f;
-        for (uint32_t t4 = 0; t4 < dpad / 4; ++t4) {
-            const float4 bv = b4[t4];
-            const float bb[4] = {bv.x, bv.y, bv.z, bv.w};
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const float q = sq[t4 * 4 + e];
-                if constexpr (ACC == ACC_EUCLID) {
-                    float df = q - bb[e];
-                    float sqr = df * df;
-                    acc = acc + sqr;
-                } else if constexpr (ACC == ACC_MANHATTAN) {
-                    float df = q - bb[e];
-                    acc = acc + fabsf(df);
-                } else {
-                    float p = q * bb[e];
-                    acc = acc + p;
-                }
-            }
         }
         float E = acc;
         if constexpr (EPI == EPI_COSINE) {
@@ -347,7 +370,9 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
     ProfScope prof("rerank", st);
     uint32_t P = 1;
     while (P < s) P <<= 1;
-    const size_t lds = (size_t)P * 12 + (size_t)dpad * 4;
+    if (dpad % 16 || ((uintptr_t)Bp & 15))
+        fail(HM_ERR_INVALID_ARG, "rerank: rows must be 16-byte aligned and padded to 16");
+    const size_t lds = (size_t)P * 12 + (size_t)dpad * 4 + (size_t)RR_THREADS * 17 * 4;
 #define HM_RR(ACC, EPI)                                                                        \
     rerank_kernel<ACC, EPI><<<rows, RR_THREADS, lds, st>>>(                                     \
         Qp, dpad, Bp, cand_i, cand_key, ldc, s, qnorm2, bnorm2, n, k, bmax, gamma, c_in_exact, \
