@@ -88,10 +88,24 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
     if (qtile * GM >= nq) return;
     __shared__ __align__(16) uint8_t sA[2][GM * LDS_ROW];
     __shared__ __align__(16) uint8_t sB[2][GN * LDS_ROW];
+    // the epilogue's per-query (scale, threshold) and per-row (norm term,
+    // scale) operands, loaded before the K loop so their latency is hidden
+    __shared__ float s_rs[GM], s_rt[GM], s_cv[GN], s_cs[GN];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     const uint32_t q0 = qtile * GM, n0 = btile * GN;
+    if (tid < GM) {
+        const uint32_t q = q0 + tid, qc = q < nq ? q : nq - 1;
+        s_rs[tid] = (ELEM == 1) ? qscale[qc] : 1.0f;
+        s_rt[tid] = FILTER ? out.tau[(size_t)qc * out.tau_ld] : 0.0f;
+    } else {
+        const uint32_t c = n0 + (tid - GM), cc = c < n ? c : n - 1;
+        if constexpr (MET == HM_METRIC_EUCLIDEAN) s_cv[tid - GM] = bnorm2[cc];
+        else if constexpr (MET == HM_METRIC_DOT_PRODUCT) s_cv[tid - GM] = 0.0f;
+        else s_cv[tid - GM] = inv_bnorm[cc];
+        s_cs[tid - GM] = (ELEM == 1) ? bscale[cc] : 1.0f;
+    }
 
     floatx4 acc[4][4];
 #pragma unroll
@@ -227,24 +241,19 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
     uint32_t colc[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const uint32_t c = n0 + wc * 64 + j * 16 + lr;
-        const uint32_t cc = c < n ? c : n - 1;
-        colc[j] = c;
-        if constexpr (MET == HM_METRIC_EUCLIDEAN) colv[j] = bnorm2[cc];
-        else if constexpr (MET == HM_METRIC_DOT_PRODUCT) colv[j] = 0.0f;
-        else colv[j] = inv_bnorm[cc];
-        cols[j] = (ELEM == 1) ? bscale[cc] : 1.0f;
+        const uint32_t cl = wc * 64 + j * 16 + lr;
+        colc[j] = n0 + cl;
+        colv[j] = s_cv[cl];
+        cols[j] = s_cs[cl];
     }
     float rowt[4][4], rows_[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const uint32_t q = q0 + wr * 64 + i * 16 + lk * 4 + e;
-            const uint32_t qc = q < nq ? q : nq - 1;
-            rows_[i][e] = (ELEM == 1) ? qscale[qc] : 1.0f;
-            if constexpr (FILTER) rowt[i][e] = out.tau[(size_t)qc * out.tau_ld];
-            else rowt[i][e] = 0.0f;
+            const uint32_t ql = wr * 64 + i * 16 + lk * 4 + e;
+            rows_[i][e] = s_rs[ql];
+            rowt[i][e] = s_rt[ql];
         }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
