@@ -70,10 +70,9 @@ class BruteForce final : public Index {
             base_.grow_keep(nc * dpad_ * sizeof(float), n_ * dpad_ * sizeof(float), stream_);
             cap_ = nc;
         }
-        float *dst = base_.as<float>() + n_ * dpad_;
-        if (dpad_ != dim_) HM_HIP(hipMemsetAsync(dst, 0, n_rows * dpad_ * sizeof(float), stream_));
-        HM_HIP(hipMemcpy2DAsync(dst, dpad_ * sizeof(float), rows, dim_ * sizeof(float),
-                                dim_ * sizeof(float), n_rows, hipMemcpyHostToDevice, stream_));
+        DeviceBuffer stage;
+        upload_rows(rows, n_rows, (uint32_t)dim_, base_.as<float>() + n_ * dpad_, dpad_, stage,
+                    stream_);
         HM_HIP(hipStreamSynchronize(stream_));
         n_ = need;
         bnorm_valid_ = false;
@@ -105,9 +104,9 @@ class BruteForce final : public Index {
     {
         if (first + count > n_) fail(HM_ERR_INVALID_ARG, "row range out of bounds");
         if (count == 0) return;
-        HM_HIP(hipMemcpy2DAsync(out, dim_ * sizeof(float), base_.as<float>() + first * dpad_,
-                                dpad_ * sizeof(float), dim_ * sizeof(float), count,
-                                hipMemcpyDeviceToHost, stream_));
+        DeviceBuffer stage;
+        download_rows(base_.as<float>() + first * dpad_, dpad_, count, (uint32_t)dim_, out, stage,
+                      stream_);
         HM_HIP(hipStreamSynchronize(stream_));
     }
 

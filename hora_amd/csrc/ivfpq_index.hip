@@ -81,10 +81,9 @@ class IVFPQ final : public Index {
             base_.grow_keep(nc * dpad_ * 4, n_ * dpad_ * 4, stream_);
             cap_ = nc;
         }
-        float *dst = base_.as<float>() + n_ * dpad_;
-        if (dpad_ != dim_) HM_HIP(hipMemsetAsync(dst, 0, n_rows * dpad_ * 4, stream_));
-        HM_HIP(hipMemcpy2DAsync(dst, dpad_ * 4, rows, dim_ * 4, dim_ * 4, n_rows,
-                                hipMemcpyHostToDevice, stream_));
+        DeviceBuffer stage;
+        upload_rows(rows, n_rows, (uint32_t)dim_, base_.as<float>() + n_ * dpad_, dpad_, stage,
+                    stream_);
         HM_HIP(hipStreamSynchronize(stream_));
         n_ = need;
     }
@@ -121,8 +120,9 @@ class IVFPQ final : public Index {
     {
         if (first + count > n_) fail(HM_ERR_INVALID_ARG, "row range out of bounds");
         if (count == 0) return;
-        HM_HIP(hipMemcpy2DAsync(out, dim_ * 4, base_.as<float>() + first * dpad_, dpad_ * 4,
-                                dim_ * 4, count, hipMemcpyDeviceToHost, stream_));
+        DeviceBuffer stage;
+        download_rows(base_.as<float>() + first * dpad_, dpad_, count, (uint32_t)dim_, out, stage,
+                      stream_);
         HM_HIP(hipStreamSynchronize(stream_));
     }
 

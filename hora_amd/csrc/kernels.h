@@ -75,6 +75,15 @@ void launch_bf_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ld
 void launch_check_finite(const float *X, uint64_t ld, uint64_t rows, uint32_t d, uint32_t *err,
                          hipStream_t s);
 
+// Host rows [n][d] -> device rows of stride dpad (zero padded), and back:
+// contiguous copies through a device staging buffer in chunks of at most
+// 256 MiB, the padding done on the device (a pitched copy of small rows
+// from pageable memory runs row by row).
+void upload_rows(const float *h_rows, uint64_t n, uint32_t d, float *d_dst, uint32_t dpad,
+                 DeviceBuffer &stage, hipStream_t s);
+void download_rows(const float *d_src, uint32_t dpad, uint64_t n, uint32_t d, float *h_out,
+                   DeviceBuffer &stage, hipStream_t s);
+
 // dst[r][0..d) = src[r][0..d), dst[r][d..dpad) = 0.
 void launch_pad_rows(const float *src, uint64_t lds, float *dst, uint64_t ldd, uint64_t rows,
                      uint32_t d, uint32_t dpad, hipStream_t s, const uint32_t *dev_rows = nullptr);

@@ -804,10 +804,9 @@ class PQ final : public Index {
         if (!rows) fail(HM_ERR_INVALID_ARG, "rows is null");
         check_finite_host(rows, n_rows * dim_);
         grow(n_rows);
-        float *dst = base_.as<float>() + n_ * dpad_;
-        if (dpad_ != dim_) HM_HIP(hipMemsetAsync(dst, 0, n_rows * dpad_ * 4, stream_));
-        HM_HIP(hipMemcpy2DAsync(dst, dpad_ * 4, rows, dim_ * 4, dim_ * 4, n_rows,
-                                hipMemcpyHostToDevice, stream_));
+        DeviceBuffer stage;
+        upload_rows(rows, n_rows, (uint32_t)dim_, base_.as<float>() + n_ * dpad_, dpad_, stage,
+                    stream_);
         HM_HIP(hipStreamSynchronize(stream_));
         n_ += n_rows;
     }
@@ -840,8 +839,9 @@ class PQ final : public Index {
     {
         if (first + count > n_) fail(HM_ERR_INVALID_ARG, "row range out of bounds");
         if (count == 0) return;
-        HM_HIP(hipMemcpy2DAsync(out, dim_ * 4, base_.as<float>() + first * dpad_, dpad_ * 4,
-                                dim_ * 4, count, hipMemcpyDeviceToHost, stream_));
+        DeviceBuffer stage;
+        download_rows(base_.as<float>() + first * dpad_, dpad_, count, (uint32_t)dim_, out, stage,
+                      stream_);
         HM_HIP(hipStreamSynchronize(stream_));
     }
 

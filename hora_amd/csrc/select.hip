@@ -901,6 +901,42 @@ void launch_check_finite(const float *X, uint64_t ld, uint64_t rows, uint32_t d,
     HM_CHECK_LAUNCH();
 }
 
+void upload_rows(const float *h_rows, uint64_t n, uint32_t d, float *d_dst, uint32_t dpad,
+                 DeviceBuffer &stage, hipStream_t s)
+{
+    if (n == 0) return;
+    if (dpad == d) {
+        HM_HIP(hipMemcpyAsync(d_dst, h_rows, n * d * sizeof(float), hipMemcpyHostToDevice, s));
+        return;
+    }
+    const uint64_t chunk = std::max<uint64_t>(1, ((uint64_t)64 << 20) / d);  // 256 MiB of rows
+    stage.reserve(std::min(n, chunk) * d * sizeof(float));
+    for (uint64_t r0 = 0; r0 < n; r0 += chunk) {
+        const uint64_t m = std::min(chunk, n - r0);
+        HM_HIP(hipMemcpyAsync(stage.get(), h_rows + r0 * d, m * d * sizeof(float),
+                              hipMemcpyHostToDevice, s));
+        launch_pad_rows(stage.as<float>(), d, d_dst + r0 * dpad, dpad, m, d, dpad, s);
+    }
+}
+
+void download_rows(const float *d_src, uint32_t dpad, uint64_t n, uint32_t d, float *h_out,
+                   DeviceBuffer &stage, hipStream_t s)
+{
+    if (n == 0) return;
+    if (dpad == d) {
+        HM_HIP(hipMemcpyAsync(h_out, d_src, n * d * sizeof(float), hipMemcpyDeviceToHost, s));
+        return;
+    }
+    const uint64_t chunk = std::max<uint64_t>(1, ((uint64_t)64 << 20) / d);
+    stage.reserve(std::min(n, chunk) * d * sizeof(float));
+    for (uint64_t r0 = 0; r0 < n; r0 += chunk) {
+        const uint64_t m = std::min(chunk, n - r0);
+        launch_pad_rows(d_src + r0 * dpad, dpad, stage.as<float>(), d, m, d, d, s);
+        HM_HIP(hipMemcpyAsync(h_out + r0 * d, stage.get(), m * d * sizeof(float),
+                              hipMemcpyDeviceToHost, s));
+    }
+}
+
 void launch_pad_rows(const float *src, uint64_t lds, float *dst, uint64_t ldd, uint64_t rows,
                      uint32_t d, uint32_t dpad, hipStream_t s, const uint32_t *dev_rows)
 {

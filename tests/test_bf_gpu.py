@@ -71,8 +71,9 @@ def test_ties_replay_the_std_heap(orc, k):
 
 @pytest.mark.parametrize("k", [63, 64, 200])
 def test_ties_replay_long_rows(orc, k):
-    # 30,000 rows in equal triples: the replay streams many batches of the
-    # row; k = 63 keeps the heap in the lanes of a wave, 64 and 200 in LDS
+    # 30,000 rows in equal triples: many groups of the row to visit; k = 63
+    # runs the group-minimum replay (k + 1 <= 64), 64 and 200 the streamed
+    # LDS replay
     rng = np.random.default_rng(11)
     uniq = rng.integers(0, 6, (10_000, 6)).astype(np.float32)
     base = np.repeat(uniq, 3, axis=0)
@@ -80,6 +81,23 @@ def test_ties_replay_long_rows(orc, k):
     q = rng.integers(0, 6, (48, 6)).astype(np.float32)
     for mt in (Metric.Euclidean, Metric.Manhattan):
         _check(orc, base, q, k, mt)
+
+
+def test_ties_replay_rows_past_4m(orc):
+    # 4.3 million rows: more than 16,384 groups of 256 values, so each group
+    # minimum covers two blocks of the row.  Continuous values, except two
+    # copies of each query's nearest row far apart in the row: every query is
+    # tied at its head, and the replay visits few groups
+    rng = np.random.default_rng(13)
+    base = rng.standard_normal((4_300_000, 4), dtype=np.float32)
+    q = rng.standard_normal((3, 4), dtype=np.float32)
+    for r in range(3):
+        near = q[r] + np.float32(1e-3)
+        base[1000 + r] = near
+        base[4_200_000 + r] = near
+    gi, _ = _check(orc, base, q, 6, Metric.Euclidean)
+    assert [sorted(gi[r, :2].tolist()) for r in range(3)] == \
+        [[1000 + r, 4_200_000 + r] for r in range(3)]
 
 
 def test_hand_traced_tie_case(orc):
