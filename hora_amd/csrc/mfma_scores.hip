@@ -12,10 +12,11 @@
 //
 // Tiling: 256 threads = 4 waves in a 2x2 arrangement; the workgroup owns a
 // 128-query x 128-base tile, each wave 64x64 = 4x4 MFMA 16x16 tiles (64 fp32
-// accumulators per lane).  Each K step stages 128 bytes per row of both
-// operands in LDS (two LDS buffers, two register sets).  The filter form
-// writes no key matrix: keys below a per-query threshold are appended to
-// per-query candidate lists in the epilogue.
+// accumulators per lane).  Each K step brings 128 bytes per row of both
+// operands into LDS with direct global->LDS loads (two LDS buffers; no
+// register staging).  The filter form writes no key matrix: keys below a
+// per-query threshold are appended to per-query candidate lists in the
+// epilogue.
 #include "kernels.h"
 #include "profiler.h"
 
@@ -28,9 +29,14 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 
 constexpr int GM = 128, GN = 128;
-constexpr int ROWB = 128;           // bytes per row per K step (64 bf16 / 128 fp8)
-constexpr int LDS_ROW = ROWB + 16;  // padded LDS row stride (bytes)
-constexpr int CPT = GM * ROWB / 16 / 256;  // 16-byte chunks per thread per operand (4)
+constexpr int ROWB = 128;  // bytes per row per K step (64 bf16 / 128 fp8)
+
+// LDS layout of one K step of one operand: rows of 128 bytes, unpadded; the
+// 16-byte chunk c of row r sits at chunk position c ^ swz(r).  A wave's
+// fragment read touches 16 rows at one chunk index, and the swizzle spreads
+// those rows over all eight chunk positions of both row parities, so the
+// 16-byte reads hit distinct banks with 32 or 64 banks.
+__device__ __forceinline__ uint32_t swz(uint32_t r) { return (r ^ (r >> 3)) & 7u; }
 
 __device__ __forceinline__ uint32_t fkey(float f)
 {
@@ -69,9 +75,14 @@ __device__ __forceinline__ void tile_of(uint32_t L, uint32_t TX, uint32_t TY, ui
     *bt = g * G + w % gb;
 }
 
-// Software pipeline: the global loads of step kt+2 are issued before the
-// MFMAs of step kt into one of two register sets, and stored to LDS one
-// step later, so each load has two steps of matrix work to land.
+// Software pipeline: step kt reads its fragments from LDS buffer kt & 1,
+// then issues the direct loads of step kt+1 into the other buffer, then
+// runs its MFMAs, so each load has one step of matrix work to land.  A
+// global_load_lds_dwordx4 writes 16 bytes per lane at (wave-uniform LDS
+// base + 16 * lane): wave w's instruction c fills rows 8 (4w + c) .. +7,
+// lane l taking row 8 (4w + c) + l / 8 and chunk position l % 8, that is
+// global chunk (l % 8) ^ swz(row).  Rows past the end load a clamped valid
+// row; their products are finite and the epilogue masks them.
 // The metric is a template parameter: the epilogue has no run-time branch
 // on it (a uniform branch chain there compiled to a store whose address
 // register was left unset on the last arm).
@@ -86,8 +97,8 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
     tile_of(blockIdx.x, (n + GN - 1) / GN, TY, G, &btile, &qtile);
     if (dev_rows && *dev_rows < nq) nq = *dev_rows;
     if (qtile * GM >= nq) return;
-    __shared__ __align__(16) uint8_t sA[2][GM * LDS_ROW];
-    __shared__ __align__(16) uint8_t sB[2][GN * LDS_ROW];
+    __shared__ __align__(1024) uint8_t sA[2][GM * ROWB];
+    __shared__ __align__(1024) uint8_t sB[2][GN * ROWB];
     // the epilogue's per-query (scale, threshold) and per-row (norm term,
     // scale) operands, loaded before the K loop so their latency is hidden
     __shared__ float s_rs[GM], s_rt[GM], s_cv[GN], s_cs[GN];
@@ -113,121 +124,104 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    // Per thread: CPT chunks of each operand per step.  Rows past the end
-    // load from a clamped (valid) row and are zeroed by a mask.
-    const uint8_t *qp[CPT], *bp[CPT];
-    uint32_t mq[CPT], mb[CPT];
-    int soff[CPT];
+    const uint8_t *qp[4], *bp[4];
 #pragma unroll
-    for (int c = 0; c < CPT; ++c) {
-        const int id = tid + 256 * c, row = id >> 3, cc = (id & 7) * 16;
-        const bool qa = q0 + row < nq, ba = n0 + row < n;
-        qp[c] = Q + (size_t)(qa ? q0 + row : nq - 1) * row_bytes + cc;
-        bp[c] = B + (size_t)(ba ? n0 + row : n - 1) * row_bytes + cc;
-        mq[c] = qa ? ~0u : 0u;
-        mb[c] = ba ? ~0u : 0u;
-        soff[c] = row * LDS_ROW + cc;
+    for (int c = 0; c < 4; ++c) {
+        const uint32_t r = 8 * (4 * wave + c) + lane / 8;
+        const uint32_t g = (uint32_t)(lane % 8) ^ swz(r);
+        const uint32_t qa = q0 + r < nq ? q0 + r : nq - 1, ba = n0 + r < n ? n0 + r : n - 1;
+        qp[c] = Q + (size_t)qa * row_bytes + 16 * g;
+        bp[c] = B + (size_t)ba * row_bytes + 16 * g;
     }
-    uint4 ra[2][CPT], rb[2][CPT];
-#define HM_MASK4(v, m) make_uint4((v).x & (m), (v).y & (m), (v).z & (m), (v).w & (m))
-#define HM_GLOAD(set, kb)                                                              \
-    _Pragma("unroll") for (int c = 0; c < CPT; ++c)                                    \
-    {                                                                                  \
-        ra[set][c] = *reinterpret_cast<const uint4 *>(qp[c] + (kb));                   \
-        rb[set][c] = *reinterpret_cast<const uint4 *>(bp[c] + (kb));                   \
-    }
-#define HM_SSTORE(set, buf)                                                            \
-    _Pragma("unroll") for (int c = 0; c < CPT; ++c)                                    \
-    {                                                                                  \
-        *reinterpret_cast<uint4 *>(&sA[buf][soff[c]]) = HM_MASK4(ra[set][c], mq[c]);   \
-        *reinterpret_cast<uint4 *>(&sB[buf][soff[c]]) = HM_MASK4(rb[set][c], mb[c]);   \
-    }
+    auto issue = [&](int buf, uint32_t kb) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            __builtin_amdgcn_global_load_lds(qp[c] + kb, &sA[buf][(4 * wave + c) * 1024], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(bp[c] + kb, &sB[buf][(4 * wave + c) * 1024], 16, 0, 0);
+        }
+    };
+    // 16 bytes of row `row` at global chunk index ch
+    auto chunk = [&](const uint8_t *S, uint32_t row, uint32_t ch) {
+        return *reinterpret_cast<const uint4 *>(S + row * ROWB + 16 * (ch ^ swz(row)));
+    };
 
     const uint32_t nk = row_bytes / ROWB;
     const int lr = lane & 15, lk = lane >> 4;
-    // One K step: MFMAs over a 128-byte slab in LDS.
-    auto mma_step = [&](const uint8_t *As, const uint8_t *Bs) {
+    issue(0, 0u);
+    for (uint32_t kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's loads of step kt have landed
+        __syncthreads();                // ... and every wave's; buffer buf ^ 1 is free
+        const uint8_t *As = sA[buf], *Bs = sB[buf];
         if constexpr (ELEM == 2) {
-            // two MFMA K=32 per step; lane holds 8 bf16 at k = 32*kk + 8*lk
+            // two MFMA K=32 per step; lane holds 8 bf16 at k = 32*kk + 8*lk,
+            // bytes 64*kk + 16*lk: chunk 4*kk + lk
+            bf16x8 a[2][4], b[2][4];
 #pragma unroll
             for (int kk = 0; kk < 2; ++kk) {
-                bf16x8 a[4], b[4];
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    a[i] = *reinterpret_cast<const bf16x8 *>(
-                        As + (wr * 64 + i * 16 + lr) * LDS_ROW + kk * 64 + lk * 16);
+                for (int i = 0; i < 4; ++i) {
+                    const uint4 v = chunk(As, wr * 64 + i * 16 + lr, 4 * kk + lk);
+                    a[kk][i] = __builtin_bit_cast(bf16x8, v);
+                }
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    b[j] = *reinterpret_cast<const bf16x8 *>(
-                        Bs + (wc * 64 + j * 16 + lr) * LDS_ROW + kk * 64 + lk * 16);
+                for (int j = 0; j < 4; ++j) {
+                    const uint4 v = chunk(Bs, wc * 64 + j * 16 + lr, 4 * kk + lk);
+                    b[kk][j] = __builtin_bit_cast(bf16x8, v);
+                }
+            }
+            if (kt + 1 < nk) issue(buf ^ 1, (kt + 1) * ROWB);
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j],
-                                                                            0, 0, 0);
-            }
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kk][i], b[kk][j],
+                                                                            acc[i][j], 0, 0, 0);
         } else {
             // one MFMA K=128 per step (v_mfma_f32_16x16x128_f8f6f4, e4m3 x
-            // e4m3, no block scale); lane holds 32 fp8 at k = 32*lk.  A and
-            // B use the same k packing, so the products pair up for any
-            // order the hardware assigns within a lane.
-            {
-                i32x8 a[4], b[4];
+            // e4m3, no block scale); lane holds 32 fp8 at k = 32*lk, chunks
+            // 2*lk and 2*lk + 1.  A and B use the same k packing, so the
+            // products pair up for any order the hardware assigns within a
+            // lane.
+            i32x8 a[4], b[4];
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    a[i] = *reinterpret_cast<const i32x8 *>(
-                        As + (wr * 64 + i * 16 + lr) * LDS_ROW + lk * 32);
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t row = wr * 64 + i * 16 + lr;
+                const uint4 lo = chunk(As, row, 2 * lk), hi = chunk(As, row, 2 * lk + 1);
+                a[i] = i32x8{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w,
+                             (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t row = wc * 64 + j * 16 + lr;
+                const uint4 lo = chunk(Bs, row, 2 * lk), hi = chunk(Bs, row, 2 * lk + 1);
+                b[j] = i32x8{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w,
+                             (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
+            }
+            if (kt + 1 < nk) issue(buf ^ 1, (kt + 1) * ROWB);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    b[j] = *reinterpret_cast<const i32x8 *>(
-                        Bs + (wc * 64 + j * 16 + lr) * LDS_ROW + lk * 32);
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-                            a[i], b[j], acc[i][j], 0, 0, 0, 0, 0, 0);
-            }
+                    acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                        a[i], b[j], acc[i][j], 0, 0, 0, 0, 0, 0);
         }
-    };
-    // Step kt computes buffer kt&1 while register set kt&1 (free since its
-    // contents went to LDS one step earlier) loads step kt+2, then register
-    // set (kt+1)&1, loaded one step earlier, goes to buffer (kt+1)&1.  The
-    // loop is unrolled by two so that both sets are named statically; loads
-    // past the last step re-read the last step (clamped, unused) so that
-    // every step has the same shape and the waits are exact.
-    const uint32_t last = (nk - 1) * ROWB;
-    HM_GLOAD(0, 0u);
-    HM_GLOAD(1, min((uint32_t)ROWB, last));
-    HM_SSTORE(0, 0);
-    __syncthreads();
-    uint32_t kt = 0;
-    for (; kt + 1 < nk; kt += 2) {
-        HM_GLOAD(0, min((kt + 2) * ROWB, last));
-        __builtin_amdgcn_sched_barrier(0);  // issue the loads before the MFMAs
-        mma_step(sA[0], sB[0]);
-        HM_SSTORE(1, 1);
-        __syncthreads();
-        HM_GLOAD(1, min((kt + 3) * ROWB, last));
-        __builtin_amdgcn_sched_barrier(0);
-        mma_step(sA[1], sB[1]);
-        HM_SSTORE(0, 0);
-        __syncthreads();
     }
-    if (kt < nk) mma_step(sA[0], sB[0]);  // odd step count: the last step is in buffer 0
-#undef HM_GLOAD
-#undef HM_SSTORE
-#undef HM_MASK4
 
     // Filter hits are staged in LDS (the operand buffers are free once every
-    // wave is past its last MFMA step) and then appended to the global
-    // lists with one atomic per hit issued by different threads, so their
-    // round trips overlap instead of serialising inside each wave.
-    constexpr uint32_t STAGE_CAP = sizeof(sA) / 12;
+    // wave is past its last MFMA step): each thread has ST_PT slots of its
+    // own, filled without atomics, and hits past those take a slot of the
+    // shared overflow region through an LDS atomic.  Then each thread
+    // appends its own slots to the global lists, its atomics issued back to
+    // back, and the overflow region is appended by all threads.
+    constexpr uint32_t ST_PT = 8, STAGE_CAP = sizeof(sA) / 8;  // 4,096 entries
+    static_assert(sizeof(sB) / 4 >= STAGE_CAP, "stage_q fits in sB");
     uint64_t *stage_v = reinterpret_cast<uint64_t *>(&sA[0][0]);
-    uint32_t *stage_q = reinterpret_cast<uint32_t *>(&sA[0][0] + (size_t)STAGE_CAP * 8);
+    uint32_t *stage_q = reinterpret_cast<uint32_t *>(&sB[0][0]);
     __shared__ uint32_t s_nhit;
+    uint32_t myc = 0;  // this thread's hits
     if constexpr (FILTER) {
         __syncthreads();
         if (tid == 0) s_nhit = 0;
@@ -274,13 +268,15 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
                 const bool ok = qok && c < n;
                 if constexpr (FILTER) {
                     if (ok && key < rowt[i][e]) {  // rare: a few per query per tile row
-                        const uint32_t pos = atomicAdd(&s_nhit, 1u);
+                        const uint32_t pos = myc < ST_PT ? tid * ST_PT + myc
+                                                         : 256 * ST_PT + atomicAdd(&s_nhit, 1u);
                         if (pos < STAGE_CAP) {
                             stage_v[pos] = ((uint64_t)fkey(key) << 32) | c;
                             stage_q[pos] = q;
                         } else {
                             atomicAdd(out.cnt + q, out.cap + 1);  // mark the list overflowed
                         }
+                        ++myc;
                     }
                 } else {
                     if (ok) out.Kmat[(size_t)q * out.ldk + c] = key;
@@ -289,9 +285,18 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
         }
     }
     if constexpr (FILTER) {
+        uint32_t pq[ST_PT], pp[ST_PT];
+#pragma unroll
+        for (uint32_t k = 0; k < ST_PT; ++k) {
+            pq[k] = stage_q[tid * ST_PT + k];
+            pp[k] = k < myc ? atomicAdd(out.cnt + pq[k], 1u) : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < ST_PT; ++k)
+            if (pp[k] < out.cap) out.list[(size_t)pq[k] * out.cap + pp[k]] = stage_v[tid * ST_PT + k];
         __syncthreads();
-        const uint32_t nh = min(s_nhit, STAGE_CAP);
-        for (uint32_t h = tid; h < nh; h += 256) {
+        const uint32_t nh = min(256 * ST_PT + s_nhit, STAGE_CAP);
+        for (uint32_t h = 256 * ST_PT + tid; h < nh; h += 256) {
             const uint32_t q = stage_q[h];
             const uint32_t pos = atomicAdd(out.cnt + q, 1u);
             if (pos < out.cap) out.list[(size_t)q * out.cap + pos] = stage_v[h];
