@@ -733,12 +733,49 @@ class BruteForce final : public Index {
                           d_dist + q0 * k, d_count + q0, rflags_.as<uint32_t>() + q0, s,
                           nullptr, nullptr, nullptr, s_q, key_s, pre);
         }
-        // Rows needing the exact path (ties: heap replay; uncertified): the
-        // flags and the error word come back in one pinned copy.
+        // Rows needing the exact path (ties: heap replay; uncertified).  Up
+        // to RERUN_SLOTS rows that are only tied are rerun on the device
+        // without a host round trip: a plan kernel lists them and the exact
+        // row and heap replay kernels take their count from device memory.
+        // The flags, the plan and the error word then come back in one
+        // pinned copy; the host reruns the rows only when the plan left them.
+        const bool dev_rerun = exact_rows_dev_ok(dpad_, dpad_, base_.as<float>());
+        if (dev_rerun) {
+            const uint64_t ldd = round_up(n, 4);
+            plan_.reserve(RERUN_PLAN_WORDS * sizeof(uint32_t));
+            r8q_.reserve((size_t)RERUN_SLOTS * dim_ * sizeof(float));
+            r8p_.reserve((size_t)RERUN_SLOTS * dpad_ * sizeof(float));
+            r8n_.reserve(RERUN_SLOTS * sizeof(float));
+            r8D_.reserve((size_t)RERUN_SLOTS * ldd * sizeof(float));
+            r8i_.reserve((size_t)RERUN_SLOTS * k * sizeof(uint64_t));
+            r8d_.reserve((size_t)RERUN_SLOTS * k * sizeof(float));
+            r8c_.reserve(RERUN_SLOTS * sizeof(uint32_t));
+            uint32_t *pl = plan_.as<uint32_t>();
+            const uint32_t *cnt = pl + 2 * RERUN_SLOTS;
+            launch_rerun_plan(rflags_.as<uint32_t>(), (uint32_t)nq, pl, s);
+            launch_gather_rows(dq, dim_, pl, RERUN_SLOTS, (uint32_t)dim_, r8q_.as<float>(), dim_, s);
+            launch_pad_rows(r8q_.as<float>(), dim_, r8p_.as<float>(), dpad_, RERUN_SLOTS,
+                            (uint32_t)dim_, dpad_, s, cnt);
+            if (need_norm())
+                launch_self_dot(r8p_.as<float>(), dpad_, RERUN_SLOTS, (uint32_t)dim_,
+                                r8n_.as<float>(), s, cnt);
+            launch_exact_rows_dev(metric_, r8p_.as<float>(), dpad_, base_.as<float>(), dpad_, n,
+                                  dpad_, r8n_.as<float>(), bnorm_.as<float>(), r8D_.as<float>(),
+                                  ldd, cnt, s);
+            launch_bf_finalize(cand_d_.as<float>(), cand_i_.as<uint32_t>(), 1, pl + RERUN_SLOTS,
+                               r8D_.as<float>(), ldd, RERUN_SLOTS, n, k, r8i_.as<uint64_t>(),
+                               r8d_.as<float>(), r8c_.as<uint32_t>(), err_.as<uint32_t>(), s);
+            launch_scatter_results(r8i_.as<uint64_t>(), r8d_.as<float>(), r8c_.as<uint32_t>(), pl,
+                                   RERUN_SLOTS, k, d_idx, d_dist, d_count, s);
+        }
         hflags_.reserve(nq * sizeof(uint32_t));
         hrows_.reserve(nq * sizeof(uint32_t));
+        hplan_.reserve(RERUN_PLAN_WORDS * sizeof(uint32_t));
         HM_HIP(hipMemcpyAsync(hflags_.as<uint32_t>(), rflags_.get(), nq * sizeof(uint32_t),
                               hipMemcpyDeviceToHost, s));
+        if (dev_rerun)
+            HM_HIP(hipMemcpyAsync(hplan_.as<uint32_t>(), plan_.get(),
+                                  RERUN_PLAN_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HM_HIP(hipMemcpyAsync(herr_.as<uint32_t>(), err_.get(), sizeof(uint32_t),
                               hipMemcpyDeviceToHost, s));
         HM_HIP(hipStreamSynchronize(s));
@@ -758,6 +795,8 @@ class BruteForce final : public Index {
             last_uncertified_ += (fl[i] & ROW_UNCERT) ? 1 : 0;
         }
         last_fallback_ = nt + nr;
+        if (dev_rerun && hplan_.as<uint32_t>()[2 * RERUN_SLOTS + 1] == 0)
+            return true;  // the tied rows were rerun on the device before the copy
         if (nt) rerun_rows(dq, rl, nt, k, d_idx, d_dist, d_count, true, s);
         if (nr) rerun_rows(dq, rl + (nq - nr), nr, k, d_idx, d_dist, d_count, false, s);
         return nt + nr == 0;
@@ -954,7 +993,8 @@ class BruteForce final : public Index {
     DeviceBuffer bsamp_, bs_norm2_, bs_inv_, bs_scale_;
     DeviceBuffer Ds_, sel_d_, sel_i_, sel_f_, list_, lcnt_, sq_, ksq_, pref_;
     DeviceBuffer redo_rows_, redo_q_, redo_i_, redo_d_, redo_c_;
-    PinnedBuffer hflags_, hrows_, herr_;  // row flags, rerun rows, error word
+    PinnedBuffer hflags_, hrows_, herr_, hplan_;  // row flags, rerun rows, error word, rerun plan
+    DeviceBuffer plan_, r8q_, r8p_, r8n_, r8D_, r8i_, r8d_, r8c_;  // device rerun of tied rows
     uint64_t last_fallback_ = 0, last_uncertified_ = 0;
     // ragged graph
     hipStream_t rg_stream_ = nullptr;

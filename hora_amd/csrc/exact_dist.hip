@@ -206,8 +206,14 @@ template <int ACC, int EPI, int RQ>
 __global__ __launch_bounds__(256) void exact_rows_kernel(
     const float *__restrict__ Q, uint32_t ldq, uint32_t nq, const float *__restrict__ B,
     uint32_t ldb, uint32_t n, uint32_t dpad, const float *__restrict__ qnorm,
-    const float *__restrict__ bnorm, float *__restrict__ D, uint64_t ldd)
+    const float *__restrict__ bnorm, float *__restrict__ D, uint64_t ldd,
+    const uint32_t *__restrict__ dev_nq = nullptr, uint32_t lo = 0, uint32_t hi = 0)
 {
+    if (dev_nq) {  // run only when lo <= *dev_nq <= hi, for that many rows
+        const uint32_t c = *dev_nq;
+        if (c < lo || c > hi) return;
+        nq = min(nq, c);
+    }
     extern __shared__ __align__(16) float xr_q[];  // [RQ][dpad]
     const uint32_t tid = threadIdx.x;
     for (uint32_t e = tid; e < RQ * dpad; e += 256) {
@@ -439,6 +445,40 @@ void launch_exact_tile(int metric, const float *Q, uint32_t ldq, uint32_t nq, co
 {
     launch_exact_tile_impl<false>(metric, Q, ldq, nq, B, ldb, n, dpad, qnorm, bnorm, D, ldd,
                                   nullptr, s);
+}
+
+bool exact_rows_dev_ok(uint32_t dpad, uint32_t ldb, const float *B)
+{
+    return dpad % 4 == 0 && ldb % 4 == 0 && (size_t)8 * dpad * 4 <= 65536 &&
+           ((uintptr_t)B & 15) == 0;
+}
+
+void launch_exact_rows_dev(int metric, const float *Q, uint32_t ldq, const float *B, uint32_t ldb,
+                           uint32_t n, uint32_t dpad, const float *qnorm, const float *bnorm,
+                           float *D, uint64_t ldd, const uint32_t *dev_nq, hipStream_t s)
+{
+    if (n == 0) return;
+    if (!exact_rows_dev_ok(dpad, ldb, B)) fail(HM_ERR_INVALID_ARG, "exact_rows_dev: shape");
+    ProfScope prof("exact_rows", s);
+    const unsigned blocks = ceil_div(n, 256);
+    // one row (the usual case) by the one-query kernel, 2..8 by the eight-query one
+#define HM_ROWS_DEV(ACC, EPI)                                                                  \
+    do {                                                                                       \
+        exact_rows_kernel<ACC, EPI, 1><<<blocks, 256, (size_t)dpad * 4, s>>>(                  \
+            Q, ldq, 1, B, ldb, n, dpad, qnorm, bnorm, D, ldd, dev_nq, 1, 1);                   \
+        exact_rows_kernel<ACC, EPI, 8><<<blocks, 256, (size_t)8 * dpad * 4, s>>>(              \
+            Q, ldq, 8, B, ldb, n, dpad, qnorm, bnorm, D, ldd, dev_nq, 2, 8);                   \
+    } while (0)
+    switch (metric) {
+    case HM_METRIC_EUCLIDEAN: HM_ROWS_DEV(ACC_EUCLID, EPI_NONE); break;
+    case HM_METRIC_MANHATTAN: HM_ROWS_DEV(ACC_MANHATTAN, EPI_NONE); break;
+    case HM_METRIC_DOT_PRODUCT: HM_ROWS_DEV(ACC_DOT, EPI_NONE); break;
+    case HM_METRIC_COSINE_SIMILARITY: HM_ROWS_DEV(ACC_DOT, EPI_COSINE); break;
+    case HM_METRIC_ANGULAR: HM_ROWS_DEV(ACC_DOT, EPI_ANGULAR); break;
+    default: fail(HM_ERR_UNKNOWN_METRIC, "unknown method");
+    }
+#undef HM_ROWS_DEV
+    HM_CHECK_LAUNCH();
 }
 
 void launch_exact_argmin(int metric, const float *Q, uint32_t ldq, uint32_t nq, const float *B,

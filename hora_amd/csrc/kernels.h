@@ -20,6 +20,14 @@ constexpr uint32_t ROW_TIE = 1u;   // equal distances among the first min(k+1, n
 constexpr uint32_t ROW_NAN = 2u;   // a NaN distance in the row
 constexpr uint32_t ROW_UNCERT = 4u;  // shortlist could not be certified
 constexpr uint32_t ROW_BAD = 8u;     // internal consistency check failed
+constexpr uint32_t ROW_SKIP = 0x80000000u;  // bf_finalize: leave the row's outputs alone
+
+// Rows kept on the device for the exact rerun: the plan block of
+// RERUN_PLAN_WORDS words holds the rows [0, RERUN_SLOTS) (~0u when unused),
+// their finalize flags [RERUN_SLOTS, 2 RERUN_SLOTS), the count, and a word
+// that is 1 when the host must decide (more tied rows than slots, or rows
+// flagged for another reason; the count is then 0).
+constexpr uint32_t RERUN_SLOTS = 8, RERUN_PLAN_WORDS = 2 * RERUN_SLOTS + 2;
 
 // ---- exact_dist.hip
 // D[q][j] = metric(Q[q], B[j]) in the reference's summation order.
@@ -27,6 +35,13 @@ constexpr uint32_t ROW_BAD = 8u;     // internal consistency check failed
 void launch_exact_tile(int metric, const float *Q, uint32_t ldq, uint32_t nq, const float *B,
                        uint32_t ldb, uint32_t n, uint32_t dpad, const float *qnorm,
                        const float *bnorm, float *D, uint64_t ldd, hipStream_t s);
+// The exact rows of the first *dev_nq (<= 8) of 8 query rows; nothing when
+// *dev_nq is 0.  Needs dpad <= 2048 and 16-byte aligned B rows
+// (exact_rows_dev_ok).
+bool exact_rows_dev_ok(uint32_t dpad, uint32_t ldb, const float *B);
+void launch_exact_rows_dev(int metric, const float *Q, uint32_t ldq, const float *B, uint32_t ldb,
+                           uint32_t n, uint32_t dpad, const float *qnorm, const float *bnorm,
+                           float *D, uint64_t ldd, const uint32_t *dev_nq, hipStream_t s);
 // best[q] = min over j of (fkey(metric(Q[q], B[j])) << 32 | j); best must be
 // initialised to ~0 by the caller.
 void launch_exact_argmin(int metric, const float *Q, uint32_t ldq, uint32_t nq, const float *B,
@@ -145,6 +160,8 @@ void launch_scatter_ragged(const uint64_t *si, const float *sd, const uint32_t *
                            hipStream_t s);
 // Fill kernel (graph-safe replacement of hipMemsetAsync).
 void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s);
+void launch_rerun_plan(const uint32_t *flags, uint32_t nq, uint32_t *plan, hipStream_t s);
+// rows[i] == ~0u: the row is skipped (gather and scatter).
 void launch_gather_rows(const float *src, uint64_t lds, const uint32_t *rows, uint32_t nrows,
                         uint32_t d, float *dst, uint64_t ldd, hipStream_t s);
 void launch_scatter_results(const uint64_t *si, const float *sd, const uint32_t *sc,

@@ -287,7 +287,8 @@ __global__ void gather_rows_kernel(const float *__restrict__ src, uint64_t lds,
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
          e += (uint64_t)gridDim.x * blockDim.x) {
         uint64_t r = e / d, c = e % d;
-        dst[r * ldd + c] = src[(uint64_t)rows[r] * lds + c];
+        const uint32_t row = rows[r];
+        if (row != ~0u) dst[r * ldd + c] = src[(uint64_t)row * lds + c];
     }
 }
 
@@ -301,12 +302,14 @@ __global__ void scatter_results_kernel(const uint64_t *__restrict__ si, const fl
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
          e += (uint64_t)gridDim.x * blockDim.x) {
         uint64_t r = e / k, j = e % k;
-        di[(uint64_t)rows[r] * k + j] = si[e];
-        dd[(uint64_t)rows[r] * k + j] = sd[e];
+        const uint32_t row = rows[r];
+        if (row == ~0u) continue;
+        di[(uint64_t)row * k + j] = si[e];
+        dd[(uint64_t)row * k + j] = sd[e];
     }
     for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < nrows;
          r += (uint64_t)gridDim.x * blockDim.x)
-        dc[rows[r]] = sc[r];
+        if (rows[r] != ~0u) dc[rows[r]] = sc[r];
 }
 
 // Exact-path results ([m][kmax]) of rows[] copied into a ragged output:

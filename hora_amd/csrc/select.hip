@@ -442,6 +442,7 @@ __global__ __launch_bounds__(256) void bf_finalize_wave_kernel(
     float *od = out_dist + (size_t)r * k;
     const uint32_t m = k < n ? k : n;
     const uint32_t f = flags ? flags[r] : ROW_TIE;  // no flags: every row tied
+    if (f == ROW_SKIP) return;
     if ((f & (ROW_NAN | ROW_BAD)) && n >= 2) {
         if (tid == 0) atomicOr(err, (f & ROW_BAD) ? 4u : 1u);
         for (uint32_t j = tid; j < k; j += 256) {
@@ -635,6 +636,7 @@ __global__ __launch_bounds__(64) void bf_replay_lds_kernel(
     float *od = out_dist + (size_t)r * k;
     const uint32_t m = k < n ? k : n;
     const uint32_t f = flags ? flags[r] : ROW_TIE;  // no flags: every row tied
+    if (f == ROW_SKIP) return;
     if ((f & (ROW_NAN | ROW_BAD)) && n >= 2) {
         if (lane == 0) atomicOr(err, (f & ROW_BAD) ? 4u : 1u);
         for (uint32_t j = lane; j < k; j += 64) {
@@ -898,6 +900,46 @@ void launch_check_finite(const float *X, uint64_t ld, uint64_t rows, uint32_t d,
     uint64_t total = rows * d;
     unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 4096);
     check_finite_kernel<<<blocks, 256, 0, s>>>(X, ld, rows, d, err);
+    HM_CHECK_LAUNCH();
+}
+
+namespace {
+// One workgroup: the rows flagged ROW_TIE alone go to the plan's slots when
+// there are at most RERUN_SLOTS of them and no row has another flag;
+// otherwise the host takes over (the count stays 0).
+__global__ __launch_bounds__(256) void rerun_plan_kernel(const uint32_t *__restrict__ flags,
+                                                         uint32_t nq, uint32_t *__restrict__ plan)
+{
+    __shared__ uint32_t s_tie, s_other, s_rows[RERUN_SLOTS];
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) s_tie = s_other = 0;
+    __syncthreads();
+    for (uint32_t r = tid; r < nq; r += 256) {
+        const uint32_t f = flags[r];
+        if (f == ROW_TIE) {
+            const uint32_t p = atomicAdd(&s_tie, 1u);
+            if (p < RERUN_SLOTS) s_rows[p] = r;
+        } else if (f & (ROW_TIE | ROW_NAN | ROW_UNCERT | ROW_BAD)) {
+            atomicAdd(&s_other, 1u);
+        }
+    }
+    __syncthreads();
+    const bool dev = s_other == 0 && s_tie <= RERUN_SLOTS;
+    const uint32_t c = dev ? s_tie : 0u;
+    if (tid < RERUN_SLOTS) {
+        plan[tid] = tid < c ? s_rows[tid] : ~0u;
+        plan[RERUN_SLOTS + tid] = tid < c ? ROW_TIE : ROW_SKIP;
+    }
+    if (tid == 0) {
+        plan[2 * RERUN_SLOTS] = c;
+        plan[2 * RERUN_SLOTS + 1] = dev ? 0u : 1u;
+    }
+}
+}  // namespace
+
+void launch_rerun_plan(const uint32_t *flags, uint32_t nq, uint32_t *plan, hipStream_t s)
+{
+    rerun_plan_kernel<<<1, 256, 0, s>>>(flags, nq, plan);
     HM_CHECK_LAUNCH();
 }
 
