@@ -171,8 +171,9 @@ class BruteForce final : public Index {
         }
         err_.reserve(sizeof(uint32_t));
         HM_HIP(hipMemsetAsync(err_.get(), 0, sizeof(uint32_t), s));
-        launch_check_finite(dq, dim_, nq, (uint32_t)dim_, err_.as<uint32_t>(), s);
         const bool shortlist_form = shortlist_usable(s);
+        // the shortlist form checks the queries while it pads them
+        if (!shortlist_form) launch_check_finite(dq, dim_, nq, (uint32_t)dim_, err_.as<uint32_t>(), s);
         last_fallback_ = last_uncertified_ = 0;
         herr_.reserve(sizeof(uint32_t));
         bool err_read = false;  // the shortlist form may read the error word itself
@@ -679,7 +680,7 @@ class BruteForce final : public Index {
         for (uint64_t q0 = 0; q0 < nq; q0 += qc) {
             const uint32_t rows = (uint32_t)std::min<uint64_t>(qc, nq - q0);
             launch_pad_rows(dq + q0 * dim_, dim_, qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
-                            dpad_, s);
+                            dpad_, s, nullptr, err_.as<uint32_t>());
             launch_self_dot(qpad_.as<float>(), dpad_, rows, (uint32_t)dim_, qnorm_.as<float>(), s);
             if (bf16)
                 launch_to_bf16(qpad_.as<float>(), dpad_, rows, (uint32_t)dim_, dk_,
@@ -697,9 +698,9 @@ class BruteForce final : public Index {
                                   D_.as<float>(), ldk, s);
                 const float *tau;
                 uint32_t tau_ld;
-                if (fp.m <= 32) {
+                if (fp.m <= 32) {  // the select also zeroes the list counters
                     launch_kth_smallest(D_.as<float>(), ldk, rows, fp.ns, fp.m, sel_d_.as<float>(),
-                                        s);
+                                        s, nullptr, lcnt_.as<uint32_t>());
                     tau = sel_d_.as<float>();
                     tau_ld = 1;
                 } else {
@@ -707,8 +708,8 @@ class BruteForce final : public Index {
                                       sel_i_.as<uint32_t>(), fp.m, sel_f_.as<uint32_t>(), s);
                     tau = sel_d_.as<float>() + (fp.m - 1);
                     tau_ld = fp.m;
+                    launch_fill_u32(lcnt_.as<uint32_t>(), rows, 0, s);
                 }
-                launch_fill_u32(lcnt_.as<uint32_t>(), rows, 0, s);
                 launch_score_filter(bf16 ? 2 : 1, qlow_.get(), rows, blow_.get(), n, row_bytes,
                                     qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
                                     inv_bnorm_.as<float>(), metric_, tau, tau_ld,
@@ -771,12 +772,19 @@ class BruteForce final : public Index {
         hflags_.reserve(nq * sizeof(uint32_t));
         hrows_.reserve(nq * sizeof(uint32_t));
         hplan_.reserve(RERUN_PLAN_WORDS * sizeof(uint32_t));
-        HM_HIP(hipMemcpyAsync(hflags_.as<uint32_t>(), rflags_.get(), nq * sizeof(uint32_t),
+        HM_HIP(hipMemcpyAsync(herr_.as<uint32_t>(), err_.get(), sizeof(uint32_t),
                               hipMemcpyDeviceToHost, s));
-        if (dev_rerun)
+        if (dev_rerun) {
+            // the plan and the error word; the flags only if the host must act
             HM_HIP(hipMemcpyAsync(hplan_.as<uint32_t>(), plan_.get(),
                                   RERUN_PLAN_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HM_HIP(hipMemcpyAsync(herr_.as<uint32_t>(), err_.get(), sizeof(uint32_t),
+            HM_HIP(hipStreamSynchronize(s));
+            const uint32_t *hp = hplan_.as<uint32_t>();
+            last_fallback_ = hp[2 * RERUN_SLOTS + 2];
+            last_uncertified_ = hp[2 * RERUN_SLOTS + 3];
+            if (hp[2 * RERUN_SLOTS + 1] == 0) return true;  // tied rows rerun on the device
+        }
+        HM_HIP(hipMemcpyAsync(hflags_.as<uint32_t>(), rflags_.get(), nq * sizeof(uint32_t),
                               hipMemcpyDeviceToHost, s));
         HM_HIP(hipStreamSynchronize(s));
         const uint32_t *fl = hflags_.as<uint32_t>();
@@ -795,8 +803,6 @@ class BruteForce final : public Index {
             last_uncertified_ += (fl[i] & ROW_UNCERT) ? 1 : 0;
         }
         last_fallback_ = nt + nr;
-        if (dev_rerun && hplan_.as<uint32_t>()[2 * RERUN_SLOTS + 1] == 0)
-            return true;  // the tied rows were rerun on the device before the copy
         if (nt) rerun_rows(dq, rl, nt, k, d_idx, d_dist, d_count, true, s);
         if (nr) rerun_rows(dq, rl + (nq - nr), nr, k, d_idx, d_dist, d_count, false, s);
         return nt + nr == 0;

@@ -24,10 +24,11 @@ constexpr uint32_t ROW_SKIP = 0x80000000u;  // bf_finalize: leave the row's outp
 
 // Rows kept on the device for the exact rerun: the plan block of
 // RERUN_PLAN_WORDS words holds the rows [0, RERUN_SLOTS) (~0u when unused),
-// their finalize flags [RERUN_SLOTS, 2 RERUN_SLOTS), the count, and a word
-// that is 1 when the host must decide (more tied rows than slots, or rows
-// flagged for another reason; the count is then 0).
-constexpr uint32_t RERUN_SLOTS = 8, RERUN_PLAN_WORDS = 2 * RERUN_SLOTS + 2;
+// their finalize flags [RERUN_SLOTS, 2 RERUN_SLOTS), the count, a word that
+// is 1 when the host must decide (more tied rows than slots, or rows flagged
+// for another reason; the count is then 0), the number of flagged rows and
+// the number of uncertified rows.
+constexpr uint32_t RERUN_SLOTS = 8, RERUN_PLAN_WORDS = 2 * RERUN_SLOTS + 4;
 
 // ---- exact_dist.hip
 // D[q][j] = metric(Q[q], B[j]) in the reference's summation order.
@@ -73,8 +74,11 @@ void launch_row_select(const float *D, uint64_t ldd, uint32_t rows, uint32_t n, 
                        const uint32_t *row_n = nullptr);
 
 // out[r] = the m-th smallest value of D's row r (1 <= m <= 32; NaN skipped).
+// zero (optional): zero[r] = 0 for every row handled (a list counter the
+// next kernel appends to).
 void launch_kth_smallest(const float *D, uint64_t ldd, uint32_t rows, uint32_t n, uint32_t m,
-                         float *out, hipStream_t s, const uint32_t *dev_rows = nullptr);
+                         float *out, hipStream_t s, const uint32_t *dev_rows = nullptr,
+                         uint32_t *zero = nullptr);
 
 // Final result of node_search_k per row: rows without ROW_TIE copy the
 // first min(k, n) candidates; rows with ROW_TIE replay the reference's
@@ -100,8 +104,11 @@ void download_rows(const float *d_src, uint32_t dpad, uint64_t n, uint32_t d, fl
                    DeviceBuffer &stage, hipStream_t s);
 
 // dst[r][0..d) = src[r][0..d), dst[r][d..dpad) = 0.
+// err (optional): a non-finite source element sets bit 1 of *err, as
+// launch_check_finite does.
 void launch_pad_rows(const float *src, uint64_t lds, float *dst, uint64_t ldd, uint64_t rows,
-                     uint32_t d, uint32_t dpad, hipStream_t s, const uint32_t *dev_rows = nullptr);
+                     uint32_t d, uint32_t dpad, hipStream_t s, const uint32_t *dev_rows = nullptr,
+                     uint32_t *err = nullptr);
 
 // ---- mfma_scores.hip
 // Kmat[q][j] = approximate ranking key of base row j for query q (see file

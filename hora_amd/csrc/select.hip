@@ -720,11 +720,13 @@ template <uint32_t KT, bool VEC>
 __global__ __launch_bounds__(256) void kth_smallest_kernel(const float *__restrict__ D,
                                                            uint64_t ldd, uint32_t rows, uint32_t n,
                                                            uint32_t m, float *__restrict__ out,
-                                                           const uint32_t *__restrict__ dev_rows)
+                                                           const uint32_t *__restrict__ dev_rows,
+                                                           uint32_t *__restrict__ zero)
 {
     const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (r >= rows || (dev_rows && r >= *dev_rows)) return;
+    if (zero && lane == 0) zero[r] = 0;
     const float inf = __int_as_float(0x7f800000);
     const float *row = D + (size_t)r * ldd;
     float best[KT];
@@ -810,15 +812,19 @@ __global__ void check_finite_kernel(const float *__restrict__ X, uint64_t ld, ui
 
 __global__ void pad_rows_kernel(const float *__restrict__ src, uint64_t lds, float *__restrict__ dst,
                                 uint64_t ldd, uint64_t rows, uint32_t d, uint32_t dpad,
-                                const uint32_t *__restrict__ dev_rows)
+                                const uint32_t *__restrict__ dev_rows, uint32_t *err)
 {
     if (dev_rows && *dev_rows < rows) rows = *dev_rows;
     uint64_t total = rows * dpad;
+    bool bad = false;
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
          e += (uint64_t)gridDim.x * blockDim.x) {
         uint64_t r = e / dpad, c = e % dpad;
-        dst[r * ldd + c] = c < d ? src[r * lds + c] : 0.0f;
+        const float v = c < d ? src[r * lds + c] : 0.0f;
+        bad |= !isfinite(v);
+        dst[r * ldd + c] = v;
     }
+    if (err && bad) atomicOr(err, 2u);
 }
 
 }  // namespace
@@ -869,7 +875,7 @@ void launch_bf_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ld
 }
 
 void launch_kth_smallest(const float *D, uint64_t ldd, uint32_t rows, uint32_t n, uint32_t m,
-                         float *out, hipStream_t s, const uint32_t *dev_rows)
+                         float *out, hipStream_t s, const uint32_t *dev_rows, uint32_t *zero)
 {
     if (rows == 0) return;
     if (m == 0 || m > KTH_MAX || m > n) fail(HM_ERR_INVALID_ARG, "kth_smallest: m out of range");
@@ -878,10 +884,10 @@ void launch_kth_smallest(const float *D, uint64_t ldd, uint32_t rows, uint32_t n
 #define HM_KTH(KT)                                                                             \
     if (vec)                                                                                   \
         kth_smallest_kernel<KT, true><<<ceil_div(rows, 4), 256, 0, s>>>(D, ldd, rows, n, m, out, \
-                                                                        dev_rows);             \
+                                                                        dev_rows, zero);             \
     else                                                                                       \
         kth_smallest_kernel<KT, false><<<ceil_div(rows, 4), 256, 0, s>>>(D, ldd, rows, n, m,     \
-                                                                         out, dev_rows)
+                                                                         out, dev_rows, zero)
     if (m <= 8) {
         HM_KTH(8);
     } else if (m <= 16) {
@@ -906,23 +912,35 @@ void launch_check_finite(const float *X, uint64_t ld, uint64_t rows, uint32_t d,
 namespace {
 // One workgroup: the rows flagged ROW_TIE alone go to the plan's slots when
 // there are at most RERUN_SLOTS of them and no row has another flag;
-// otherwise the host takes over (the count stays 0).
-__global__ __launch_bounds__(256) void rerun_plan_kernel(const uint32_t *__restrict__ flags,
-                                                         uint32_t nq, uint32_t *__restrict__ plan)
+// otherwise the host takes over (the count stays 0).  Four flags per 16-byte
+// load when the array allows it.
+__global__ __launch_bounds__(1024) void rerun_plan_kernel(const uint32_t *__restrict__ flags,
+                                                          uint32_t nq, uint32_t *__restrict__ plan)
 {
-    __shared__ uint32_t s_tie, s_other, s_rows[RERUN_SLOTS];
+    __shared__ uint32_t s_tie, s_other, s_unc, s_rows[RERUN_SLOTS];
     const uint32_t tid = threadIdx.x;
-    if (tid == 0) s_tie = s_other = 0;
+    if (tid == 0) s_tie = s_other = s_unc = 0;
     __syncthreads();
-    for (uint32_t r = tid; r < nq; r += 256) {
-        const uint32_t f = flags[r];
+    auto visit = [&](uint32_t r, uint32_t f) {
         if (f == ROW_TIE) {
             const uint32_t p = atomicAdd(&s_tie, 1u);
             if (p < RERUN_SLOTS) s_rows[p] = r;
         } else if (f & (ROW_TIE | ROW_NAN | ROW_UNCERT | ROW_BAD)) {
             atomicAdd(&s_other, 1u);
         }
+        if (f & ROW_UNCERT) atomicAdd(&s_unc, 1u);
+    };
+    const uint32_t nv = ((uintptr_t)flags & 15) == 0 ? nq / 4 : 0;
+    for (uint32_t v = tid; v < nv; v += 1024) {
+        const uint4 f4 = reinterpret_cast<const uint4 *>(flags)[v];
+        if (f4.x | f4.y | f4.z | f4.w) {
+            visit(4 * v, f4.x);
+            visit(4 * v + 1, f4.y);
+            visit(4 * v + 2, f4.z);
+            visit(4 * v + 3, f4.w);
+        }
     }
+    for (uint32_t r = 4 * nv + tid; r < nq; r += 1024) visit(r, flags[r]);
     __syncthreads();
     const bool dev = s_other == 0 && s_tie <= RERUN_SLOTS;
     const uint32_t c = dev ? s_tie : 0u;
@@ -933,13 +951,15 @@ __global__ __launch_bounds__(256) void rerun_plan_kernel(const uint32_t *__restr
     if (tid == 0) {
         plan[2 * RERUN_SLOTS] = c;
         plan[2 * RERUN_SLOTS + 1] = dev ? 0u : 1u;
+        plan[2 * RERUN_SLOTS + 2] = s_tie + s_other;
+        plan[2 * RERUN_SLOTS + 3] = s_unc;
     }
 }
 }  // namespace
 
 void launch_rerun_plan(const uint32_t *flags, uint32_t nq, uint32_t *plan, hipStream_t s)
 {
-    rerun_plan_kernel<<<1, 256, 0, s>>>(flags, nq, plan);
+    rerun_plan_kernel<<<1, 1024, 0, s>>>(flags, nq, plan);
     HM_CHECK_LAUNCH();
 }
 
@@ -980,12 +1000,13 @@ void download_rows(const float *d_src, uint32_t dpad, uint64_t n, uint32_t d, fl
 }
 
 void launch_pad_rows(const float *src, uint64_t lds, float *dst, uint64_t ldd, uint64_t rows,
-                     uint32_t d, uint32_t dpad, hipStream_t s, const uint32_t *dev_rows)
+                     uint32_t d, uint32_t dpad, hipStream_t s, const uint32_t *dev_rows,
+                     uint32_t *err)
 {
     if (rows == 0) return;
     uint64_t total = rows * dpad;
     unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 8192);
-    pad_rows_kernel<<<blocks, 256, 0, s>>>(src, lds, dst, ldd, rows, d, dpad, dev_rows);
+    pad_rows_kernel<<<blocks, 256, 0, s>>>(src, lds, dst, ldd, rows, d, dpad, dev_rows, err);
     HM_CHECK_LAUNCH();
 }
 
