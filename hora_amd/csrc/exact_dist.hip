@@ -295,6 +295,63 @@ __global__ __launch_bounds__(64) void self_dot_kernel(const float *__restrict__ 
     if (r0 + tid < nrows) out[r0 + tid] = acc;
 }
 
+// The same fold for rows 16-byte aligned with a stride that is a multiple of
+// 16 floats: 256 threads per 64 rows load slabs of 128 dimensions (clamped
+// addresses, values past d zeroed) into LDS, the next slab in flight while
+// the first wave folds the current one, one lane per row in order.
+__global__ __launch_bounds__(256) void self_dot_wide_kernel(const float *__restrict__ X,
+                                                            uint32_t ldx, uint32_t rows, uint32_t d,
+                                                            float *__restrict__ out,
+                                                            const uint32_t *__restrict__ dev_rows)
+{
+    constexpr uint32_t SL = 128;
+    __shared__ float sl[2][64][SL + 1];
+    const uint32_t nrows = dev_rows ? min(rows, *dev_rows) : rows;
+    const uint32_t r0 = blockIdx.x * 64, tid = threadIdx.x;
+    if (r0 >= nrows) return;
+    // thread tid loads float4 number tid + 256 j (j < 8) of the 64 x 128 slab:
+    // row (tid + 256 j) / 32, column 4 ((tid + 256 j) % 32)
+    const uint32_t cb = 4 * (tid % 32);
+    const float *rp[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j)
+        rp[j] = X + (size_t)min(r0 + (tid + 256 * j) / 32, nrows - 1) * ldx;
+    float4 v[8];
+    auto load = [&](uint32_t t0) {
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) {
+            const uint32_t c = t0 + cb;
+            const float4 x = *reinterpret_cast<const float4 *>(rp[j] + min(c, ldx - 4));
+            const bool ok = c < d;  // d <= ldx, both multiples of 4 past c
+            v[j] = make_float4(ok ? x.x : 0.0f, c + 1 < d ? x.y : 0.0f, c + 2 < d ? x.z : 0.0f,
+                               c + 3 < d ? x.w : 0.0f);
+        }
+    };
+    load(0);
+    float acc = -0.0f;
+    for (uint32_t t0 = 0, b = 0; t0 < d; t0 += SL, b ^= 1) {
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) {
+            float *dst = &sl[b][(tid + 256 * j) / 32][cb];
+            dst[0] = v[j].x;
+            dst[1] = v[j].y;
+            dst[2] = v[j].z;
+            dst[3] = v[j].w;
+        }
+        __syncthreads();
+        if (t0 + SL < d) load(t0 + SL);
+        if (tid < 64) {
+            const uint32_t tn = min(SL, d - t0);
+            for (uint32_t t = 0; t < tn; ++t) {
+                const float x = sl[b][tid][t];
+                const float p = x * x;
+                acc = acc + p;
+            }
+        }
+    }
+    if (tid < 64 && r0 + tid < nrows) out[r0 + tid] = acc;
+}
+
 // Batched distance service of the graph searches: one lane per (query,
 // node) pair, metric(node, query) in the reference's fold order -- the
 // operand order of hnsw_idx.rs:243-254 and ssg_idx.rs:402-456.  Pairs p in
@@ -521,7 +578,7 @@ void launch_self_dot(const float *X, uint32_t ldx, uint32_t rows, uint32_t d, fl
 {
     if (rows == 0) return;
     if (ldx % 16 == 0 && ((uintptr_t)X & 15) == 0 && d <= ldx)
-        self_dot_kernel<true><<<ceil_div(rows, 64), 64, 0, s>>>(X, ldx, rows, d, out, dev_rows);
+        self_dot_wide_kernel<<<ceil_div(rows, 64), 256, 0, s>>>(X, ldx, rows, d, out, dev_rows);
     else
         self_dot_kernel<false><<<ceil_div(rows, 64), 64, 0, s>>>(X, ldx, rows, d, out, dev_rows);
     HM_CHECK_LAUNCH();
