@@ -743,6 +743,11 @@ __global__ __launch_bounds__(256) void kth_smallest_kernel(const float *__restri
         }
     };
     constexpr uint32_t U = 16;
+    // cut: an upper bound of the row's m-th smallest value, the m-th smallest
+    // of the lanes' minima over the first round (m distinct elements); only
+    // values not above it can be among the m smallest, so only they are
+    // inserted.  Until it is known (and in the scalar path) it is +inf.
+    float cut = inf;
     for (uint64_t b0 = 0; b0 < n; b0 += (uint64_t)U * 256) {
         if constexpr (VEC) {
             float4 v[U];
@@ -751,13 +756,36 @@ __global__ __launch_bounds__(256) void kth_smallest_kernel(const float *__restri
                 const uint64_t e = min(b0 + u * 256 + 4 * (uint32_t)lane, ldd - 4);
                 v[u] = *reinterpret_cast<const float4 *>(row + e);
             }
+            if (b0 == 0) {
+                float lm = inf;
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    const uint64_t e = u * 256 + 4 * (uint32_t)lane;
+                    if (e < n) lm = fminf(lm, v[u].x);
+                    if (e + 1 < n) lm = fminf(lm, v[u].y);
+                    if (e + 2 < n) lm = fminf(lm, v[u].z);
+                    if (e + 3 < n) lm = fminf(lm, v[u].w);
+                }
+                // m rounds of the wave minimum over the lanes' minima; the
+                // owner (lowest lane holding it) drops out each round
+                float mine = lm;
+                for (uint32_t round = 0; round < m; ++round) {
+                    float w = mine;
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) w = fminf(w, __shfl_xor(w, o, 64));
+                    cut = w;
+                    const unsigned long long owner = __ballot(mine == w);
+                    if ((uint32_t)lane == (uint32_t)__builtin_ctzll(owner)) mine = inf;
+                    if (owner == 0) break;  // fewer than m finite minima: cut stays +inf
+                }
+            }
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u) {
                 const uint64_t e = b0 + u * 256 + 4 * (uint32_t)lane;
-                if (e < n) insert(v[u].x);
-                if (e + 1 < n) insert(v[u].y);
-                if (e + 2 < n) insert(v[u].z);
-                if (e + 3 < n) insert(v[u].w);
+                if (e < n && v[u].x <= cut) insert(v[u].x);
+                if (e + 1 < n && v[u].y <= cut) insert(v[u].y);
+                if (e + 2 < n && v[u].z <= cut) insert(v[u].z);
+                if (e + 3 < n && v[u].w <= cut) insert(v[u].w);
             }
         } else {
             // scalar loads: the round in two halves of 32 per lane
