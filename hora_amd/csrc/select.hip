@@ -497,25 +497,27 @@ __global__ __launch_bounds__(256) void bf_finalize_wave_kernel(
     }
     __syncthreads();
     if (wave != 0) return;
-    // the reference's heap (wave_heap.h) in LDS: (item, distance bits)
-    WaveHeap<WhPackedLds> h({he}, (uint32_t)lane);
+    // the reference's heap (wave_heap.h), k + 1 <= 64 entries in the wave's
+    // registers: (item, distance bits)
+    WaveHeapReg h((uint32_t)lane);
     bool stable = false;
     float root = inf;
-    auto val = [&](uint32_t pos) { return __uint_as_float(he[pos + 1].y); };
-    // heap_noop_roundtrip on the LDS heap, all ancestor steps at once: lane
+    auto val = [&](uint32_t pos) { return h.val_lane(pos); };
+    // heap_noop_roundtrip on the heap, all ancestor steps at once: lane
     // j checks step j of the walk from position L = len to the root
     auto noop = [&]() -> bool {
         const uint32_t L = h.len;
         if (L == 0) return true;
         const uint32_t steps = 32u - (uint32_t)__clz(L + 1);  // ancestors of L, plus L
+        const bool act = (uint32_t)lane < steps;
+        const uint32_t a = act ? ((L + 1) >> lane) - 1 : 0u;
+        const uint32_t p = a > 0 ? (a - 1) / 2 : 0u;
+        // the cross-lane reads are made by every lane, outside the branches
+        const float vp = val(p), vq = val(lane == 0 ? L - 1 : a + 1);
         bool bad = false;
-        if ((uint32_t)lane < steps) {
-            const uint32_t a = ((L + 1) >> lane) - 1;
-            if (a > 0) {
-                const uint32_t p = (a - 1) / 2;
-                if (lane == 0) bad = a == 2 * p + 2 && !(val(p) > val(L - 1));
-                else bad = a == 2 * p + 1 && a + 1 < L && !(val(p) > val(a + 1));
-            }
+        if (act && a > 0) {
+            if (lane == 0) bad = a == 2 * p + 2 && !(vp > vq);
+            else bad = a == 2 * p + 1 && a + 1 < L && !(vp > vq);
         }
         return __ballot(bad) == 0;
     };
@@ -567,7 +569,7 @@ __global__ __launch_bounds__(256) void bf_finalize_wave_kernel(
             h.push(make_uint2(base + e, __float_as_uint(x)));
             if (h.len > k) (void)h.pop();
             if (h.len == k) {
-                root = val(0);
+                root = h.val_at(0);
                 stable = root < inf && noop();
             }
             ++e;

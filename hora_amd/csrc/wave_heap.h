@@ -114,6 +114,85 @@ template <class S> struct WaveHeap {
     }
 };
 
+// The same heap for at most 64 entries held in registers: lane p holds
+// position p, and the loads of the LDS form become cross-lane reads.  push
+// and pop make the same moves as WaveHeap's (a pop's sift ends inside one
+// 63-node round, since every position is below 64).
+struct WaveHeapReg {
+    uint32_t it, dv;  // this lane's entry: item, distance bits
+    uint32_t len;
+    uint32_t lane;
+    uint64_t anc, dir;  // as WaveHeap's, for node `lane` of the 63-node round
+
+    __device__ explicit WaveHeapReg(uint32_t lane_) : it(0), dv(0), len(0), lane(lane_)
+    {
+        anc = 0;
+        dir = 0;
+        for (uint32_t x = lane < 63 ? lane : 0; x > 0;) {
+            const uint32_t p = (x - 1) / 2;
+            anc |= 1ull << p;
+            if ((x & 1u) == 0) dir |= 1ull << p;
+            x = p;
+        }
+    }
+
+    __device__ static float fv(uint32_t b) { return __uint_as_float(b); }
+    __device__ static uint32_t xl(uint32_t v, uint32_t src)  // v of lane src (per lane)
+    {
+        return (uint32_t)__shfl((int)v, (int)src, 64);
+    }
+    // entry / value at a wave-uniform position, value at a per-lane one
+    __device__ uint2 at(uint32_t p) const { return wh_lane(make_uint2(it, dv), p); }
+    __device__ float val_at(uint32_t p) const
+    {
+        return fv((uint32_t)__builtin_amdgcn_readlane((int)dv, (int)p));
+    }
+    __device__ float val_lane(uint32_t p) const { return fv(xl(dv, min(p, 63u))); }
+
+    // heap.rs:21-35 (len < 64): lane l < A compares with ancestor l + 1; the
+    // position j steps above p takes its parent's entry for j < t and x at t
+    __device__ void push(uint2 x)
+    {
+        const uint32_t p = len++, P = p + 1;
+        const uint32_t A = 31u - (uint32_t)__clz(P);
+        const float u = fv(xl(dv, lane < A ? (P >> (lane + 1)) - 1 : 0u));
+        const uint64_t m = __ballot(lane < A && fv(x.y) > u);
+        const uint32_t t = (uint32_t)__builtin_ctzll(~m);
+        const uint32_t Q = lane + 1, bq = 32u - (uint32_t)__clz(Q), bp = 32u - (uint32_t)__clz(P);
+        const uint32_t j = bp - bq;
+        const bool on = bq <= bp && (P >> j) == Q && j <= t;
+        const uint32_t par = lane > 0 ? (lane - 1) / 2 : 0u;
+        const uint32_t pit = xl(it, par), pdv = xl(dv, par);
+        if (on) {
+            it = j < t ? pit : x.x;
+            dv = j < t ? pdv : x.y;
+        }
+    }
+
+    // heap.rs:54-68, 95-112 (len >= 1): the top-down sift of the last entry
+    __device__ uint2 pop()
+    {
+        len -= 1;
+        const uint32_t end = len;
+        const uint2 root = at(0), last = at(end);
+        if (end == 0) return last;
+        const float lv = fv(last.y);
+        const uint32_t c1 = min(2 * lane + 1, 63u), c2 = min(2 * lane + 2, 63u);
+        const uint32_t ait = xl(it, c1), adv = xl(dv, c1), bit = xl(it, c2), bdv = xl(dv, c2);
+        const bool ex = lane < 63 && 2 * lane + 1 < end;
+        const bool right = ex && 2 * lane + 2 < end && fv(adv) <= fv(bdv);
+        const uint32_t wit = right ? bit : ait, wdv = right ? bdv : adv;
+        const bool st = !ex || lv > fv(wdv);  // the sift stops at this node
+        const uint64_t ms = __ballot(st), mr = __ballot(right);
+        const bool onp = lane < 63 && (ms & anc) == 0 && (mr & anc) == dir;
+        if (onp) {
+            it = st ? last.x : wit;
+            dv = st ? last.y : wdv;
+        }
+        return root;
+    }
+};
+
 // Storage: the heap in LDS with position p at e[p + 1], so the children
 // (2q + 1, 2q + 2) of a node are one aligned 16-byte word.  e: 16-byte
 // aligned, capacity + 1 entries.
