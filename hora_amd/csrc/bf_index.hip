@@ -671,7 +671,8 @@ class BruteForce final : public Index {
             ksq_.reserve(qc * sizeof(float));
             pref_.reserve(qc * sizeof(uint32_t));
         }
-        HM_HIP(hipMemsetAsync(qinexact_.get(), 0, sizeof(uint32_t), s));
+        // the queries' BF16-inexact flag: read only by the certification (BF16 form)
+        if (bf16) HM_HIP(hipMemsetAsync(qinexact_.get(), 0, sizeof(uint32_t), s));
 
         const float u = 5.9604644775390625e-8f;
         const float gamma = 2.0f * (float)(std::max<uint32_t>(dk_, dpad_) + 8) * u;
