@@ -472,8 +472,7 @@ __global__ __launch_bounds__(256) void bf_finalize_wave_kernel(
     // one.  The row's NaN-free (tied rows only), so fkey order is value order.
     const uint32_t nb = (n + BF_BLOCK - 1) / BF_BLOCK, ng = (nb + bpg - 1) / bpg;
     extern __shared__ __align__(16) uint2 bf_smem[];
-    uint2 *he = bf_smem;                                                 // heap, [k + 2]
-    uint32_t *gkey = reinterpret_cast<uint32_t *>(bf_smem + ((k + 3) & ~1u));  // [ng]
+    uint32_t *gkey = reinterpret_cast<uint32_t *>(bf_smem);  // [ng] group minima (fkey)
     for (uint32_t g = tid; g < ng; g += 256) gkey[g] = ~0u;
     __syncthreads();
     for (uint32_t b0 = (uint32_t)wave * 16; b0 < nb; b0 += 4 * 16) {
@@ -893,7 +892,7 @@ void launch_bf_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ld
         const uint32_t nb = (n + BF_BLOCK - 1) / BF_BLOCK;
         uint32_t bpg = 1;
         while ((nb + bpg - 1) / bpg > BF_GMIN_MAX) bpg <<= 1;
-        const size_t lds = (size_t)((k + 3) & ~1u) * 8 + (size_t)((nb + bpg - 1) / bpg) * 4;
+        const size_t lds = (size_t)((nb + bpg - 1) / bpg) * 4;  // the group minima
         bf_finalize_wave_kernel<<<rows, 256, lds, s>>>(cand_d, cand_i, ldc, flags, D, ldd, n, k,
                                                        bpg, out_idx, out_dist, out_count, err);
     } else {
