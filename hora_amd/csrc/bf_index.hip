@@ -451,14 +451,21 @@ class BruteForce final : public Index {
         const uint32_t *s_q = nullptr, *pre = nullptr;
         const float *key_s = nullptr;
         if (filter) {
-            launch_score_gemm(bf16 ? 2 : 1, rg_qlow_.get(), QC, bsamp_.get(), fp.ns, row_bytes,
-                              rg_qscale_.as<float>(), bs_scale_.as<float>(), bs_norm2_.as<float>(),
-                              bs_inv_.as<float>(), metric_, rg_D_.as<float>(), ldk, g, dr);
+            if (fp.m <= 32)  // from the minima of the sample keys in fours
+                launch_score_min4(bf16 ? 2 : 1, rg_qlow_.get(), QC, bsamp_.get(), fp.ns, row_bytes,
+                                  rg_qscale_.as<float>(), bs_scale_.as<float>(),
+                                  bs_norm2_.as<float>(), bs_inv_.as<float>(), metric_,
+                                  rg_D_.as<float>(), ldk, g, dr);
+            else
+                launch_score_gemm(bf16 ? 2 : 1, rg_qlow_.get(), QC, bsamp_.get(), fp.ns, row_bytes,
+                                  rg_qscale_.as<float>(), bs_scale_.as<float>(),
+                                  bs_norm2_.as<float>(), bs_inv_.as<float>(), metric_,
+                                  rg_D_.as<float>(), ldk, g, dr);
             const float *tau;
             uint32_t tau_ld;
             if (fp.m <= 32) {
-                launch_kth_smallest(rg_D_.as<float>(), ldk, QC, fp.ns, fp.m, rg_seld_.as<float>(), g,
-                                    dr);
+                launch_kth_smallest(rg_D_.as<float>(), ldk, QC, fp.ns / 4, fp.m,
+                                    rg_seld_.as<float>(), g, dr);
                 tau = rg_seld_.as<float>();
                 tau_ld = 1;
             } else {
@@ -693,15 +700,23 @@ class BruteForce final : public Index {
             const float *key_s = nullptr;
             if (filter) {
                 // threshold from the sample, then the filtered full scan
-                launch_score_gemm(bf16 ? 2 : 1, qlow_.get(), rows, bsamp_.get(), fp.ns, row_bytes,
-                                  qscale_.as<float>(), bs_scale_.as<float>(),
-                                  bs_norm2_.as<float>(), bs_inv_.as<float>(), metric_,
-                                  D_.as<float>(), ldk, s);
+                // (m <= 32: from the minima of the sample keys in fours,
+                // a quarter of the key matrix; see launch_score_min4)
+                if (fp.m <= 32)
+                    launch_score_min4(bf16 ? 2 : 1, qlow_.get(), rows, bsamp_.get(), fp.ns,
+                                      row_bytes, qscale_.as<float>(), bs_scale_.as<float>(),
+                                      bs_norm2_.as<float>(), bs_inv_.as<float>(), metric_,
+                                      D_.as<float>(), ldk, s);
+                else
+                    launch_score_gemm(bf16 ? 2 : 1, qlow_.get(), rows, bsamp_.get(), fp.ns,
+                                      row_bytes, qscale_.as<float>(), bs_scale_.as<float>(),
+                                      bs_norm2_.as<float>(), bs_inv_.as<float>(), metric_,
+                                      D_.as<float>(), ldk, s);
                 const float *tau;
                 uint32_t tau_ld;
                 if (fp.m <= 32) {  // the select also zeroes the list counters
-                    launch_kth_smallest(D_.as<float>(), ldk, rows, fp.ns, fp.m, sel_d_.as<float>(),
-                                        s, nullptr, lcnt_.as<uint32_t>());
+                    launch_kth_smallest(D_.as<float>(), ldk, rows, fp.ns / 4, fp.m,
+                                        sel_d_.as<float>(), s, nullptr, lcnt_.as<uint32_t>());
                     tau = sel_d_.as<float>();
                     tau_ld = 1;
                 } else {

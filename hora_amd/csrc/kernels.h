@@ -118,6 +118,15 @@ void launch_score_gemm(int elem_bytes, const void *Q, uint32_t nq, const void *B
                        uint32_t row_bytes, const float *qscale, const float *bscale,
                        const float *bnorm2, const float *inv_bnorm, int metric, float *Kmat,
                        uint64_t ldk, hipStream_t s, const uint32_t *dev_rows = nullptr);
+// As launch_score_gemm, with each group of four keys of a row (one lane's
+// columns) reduced to its minimum: Kmat[q][ceil(n / 128) * 32], ldk at
+// least that.  The m-th smallest of these minima is at least the m-th
+// smallest key (they are m distinct keys): a threshold estimate from a
+// quarter of the data.
+void launch_score_min4(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
+                       uint32_t row_bytes, const float *qscale, const float *bscale,
+                       const float *bnorm2, const float *inv_bnorm, int metric, float *Kmat,
+                       uint64_t ldk, hipStream_t s, const uint32_t *dev_rows = nullptr);
 // Filter form: keys below tau[q * tau_ld] are appended to list[q][0..cap)
 // as fkey(key) << 32 | row, counted in cnt[q] (zeroed by the caller; the
 // count may exceed cap, the entries past cap are dropped).

@@ -86,7 +86,10 @@ __device__ __forceinline__ void tile_of(uint32_t L, uint32_t TX, uint32_t TY, ui
 // The metric is a template parameter: the epilogue has no run-time branch
 // on it (a uniform branch chain there compiled to a store whose address
 // register was left unset on the last arm).
-template <int ELEM, int MET, bool FILTER>  // ELEM: 2 = bf16, 1 = fp8
+// MIN4 (key matrix form only): each lane writes the minimum of its four keys
+// of a row (columns lr + 16 j of the wave's 64), at column
+// btile * 32 + wc * 16 + lr of a [nq][ceil(n / 128) * 32] matrix.
+template <int ELEM, int MET, bool FILTER, bool MIN4 = false>  // ELEM: 2 = bf16, 1 = fp8
 __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
     const uint8_t *__restrict__ Q, uint32_t nq, uint32_t TY, uint32_t G,
     const uint8_t *__restrict__ B, uint32_t n, uint32_t row_bytes, const float *__restrict__ qscale,
@@ -249,6 +252,29 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
             rows_[i][e] = s_rs[ql];
             rowt[i][e] = s_rt[ql];
         }
+    if constexpr (!FILTER && MIN4) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t q = q0 + wr * 64 + i * 16 + lk * 4 + e;
+                float mn = __int_as_float(0x7f800000);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    float s = acc[i][j][e];
+                    if (ELEM == 1) s = s * (rows_[i][e] * cols[j]);
+                    float key;
+                    if constexpr (MET == HM_METRIC_EUCLIDEAN) key = colv[j] - 2.0f * s;
+                    else if constexpr (MET == HM_METRIC_DOT_PRODUCT) key = s;
+                    else if constexpr (MET == HM_METRIC_COSINE_SIMILARITY) key = -s * colv[j];
+                    else key = s * colv[j];
+                    if (colc[j] < n) mn = fminf(mn, key);
+                }
+                if (q < nq) out.Kmat[(size_t)q * out.ldk + btile * 32 + wc * 16 + lr] = mn;
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -394,7 +420,7 @@ __global__ void norm_stats_kernel(const float *__restrict__ norm2, uint32_t rows
 
 }  // namespace
 
-template <bool FILTER>
+template <bool FILTER, bool MIN4 = false>
 static void score_launch(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
                          uint32_t row_bytes, const float *qscale, const float *bscale,
                          const float *bnorm2, const float *inv_bnorm, int metric,
@@ -411,7 +437,7 @@ static void score_launch(int elem_bytes, const void *Q, uint32_t nq, const void 
     const uint32_t G = (uint32_t)std::min<uint64_t>(
         32, std::max<uint64_t>(2, (2u << 20) / ((uint64_t)GN * row_bytes)));
 #define HM_SG(E, M)                                                                           \
-    score_gemm_kernel<E, M, FILTER><<<grid, 256, 0, s>>>(                                     \
+    score_gemm_kernel<E, M, FILTER, MIN4><<<grid, 256, 0, s>>>(                                     \
         (const uint8_t *)Q, nq, (uint32_t)TY, G, (const uint8_t *)B, n, row_bytes, qscale,     \
         bscale, bnorm2, inv_bnorm, out, dev_rows)
 #define HM_SG_METRIC(E)                                                                       \
@@ -441,6 +467,18 @@ void launch_score_gemm(int elem_bytes, const void *Q, uint32_t nq, const void *B
     ScoreOut out{Kmat, ldk, nullptr, 0, nullptr, 0, nullptr};
     score_launch<false>(elem_bytes, Q, nq, B, n, row_bytes, qscale, bscale, bnorm2, inv_bnorm,
                         metric, out, s, dev_rows);
+}
+
+void launch_score_min4(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
+                       uint32_t row_bytes, const float *qscale, const float *bscale,
+                       const float *bnorm2, const float *inv_bnorm, int metric, float *Kmat,
+                       uint64_t ldk, hipStream_t s, const uint32_t *dev_rows)
+{
+    if (ldk < (uint64_t)ceil_div(n, GN) * 32) fail(HM_ERR_INVALID_ARG, "score_min4: ldk too small");
+    ProfScope prof("score_gemm", s);
+    ScoreOut out{Kmat, ldk, nullptr, 0, nullptr, 0, nullptr};
+    score_launch<false, true>(elem_bytes, Q, nq, B, n, row_bytes, qscale, bscale, bnorm2,
+                              inv_bnorm, metric, out, s, dev_rows);
 }
 
 void launch_score_filter(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,

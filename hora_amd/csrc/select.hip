@@ -717,7 +717,7 @@ constexpr uint32_t KTH_MAX = 32;
 // as a sorted run; the row is read 4,096 values per round with every load
 // of the round in flight (16-byte loads when the row allows them), then m
 // rounds of a wave minimum over the lanes' heads give the m-th smallest.
-template <uint32_t KT, bool VEC>
+template <uint32_t KT, bool VEC, uint32_t U = 16>  // U float4 loads per lane per round
 __global__ __launch_bounds__(256) void kth_smallest_kernel(const float *__restrict__ D,
                                                            uint64_t ldd, uint32_t rows, uint32_t n,
                                                            uint32_t m, float *__restrict__ out,
@@ -743,7 +743,6 @@ __global__ __launch_bounds__(256) void kth_smallest_kernel(const float *__restri
             x = lt ? b : x;
         }
     };
-    constexpr uint32_t U = 16;
     // cut: an upper bound of the row's m-th smallest value, the m-th smallest
     // of the lanes' minima over the first round (m distinct elements); only
     // values not above it can be among the m smallest, so only they are
@@ -911,9 +910,12 @@ void launch_kth_smallest(const float *D, uint64_t ldd, uint32_t rows, uint32_t n
     ProfScope prof("kth_smallest", s);
     const bool vec = ldd % 4 == 0 && ((uintptr_t)D & 15) == 0 && ldd >= 4;
 #define HM_KTH(KT)                                                                             \
-    if (vec)                                                                                   \
+    if (vec && n <= 1024)                                                                      \
+        kth_smallest_kernel<KT, true, 4><<<ceil_div(rows, 4), 256, 0, s>>>(D, ldd, rows, n, m,   \
+                                                                           out, dev_rows, zero); \
+    else if (vec)                                                                              \
         kth_smallest_kernel<KT, true><<<ceil_div(rows, 4), 256, 0, s>>>(D, ldd, rows, n, m, out, \
-                                                                        dev_rows, zero);             \
+                                                                        dev_rows, zero);       \
     else                                                                                       \
         kth_smallest_kernel<KT, false><<<ceil_div(rows, 4), 256, 0, s>>>(D, ldd, rows, n, m,     \
                                                                          out, dev_rows, zero)
