@@ -250,10 +250,9 @@ __global__ __launch_bounds__(256) void exact_rows_kernel(
 }
 
 // s = sum_t x_t*x_t as a left fold from -0.0 (the reference's dot(x,x) up to
-// sign, calc.rs:13-18), one lane per row.  The workgroup's 64 rows pass
-// through LDS 16 dimensions at a time (coalesced 16-byte loads when VEC:
-// rows 16-byte aligned with a stride that is a multiple of 16 floats).
-template <bool VEC>
+// sign, calc.rs:13-18), one lane per row, for rows of any alignment and
+// stride (self_dot_wide_kernel takes the aligned ones).  The workgroup's 64
+// rows pass through LDS 16 dimensions at a time.
 __global__ __launch_bounds__(64) void self_dot_kernel(const float *__restrict__ X, uint32_t ldx,
                                                       uint32_t rows, uint32_t d,
                                                       float *__restrict__ out,
@@ -270,14 +269,10 @@ __global__ __launch_bounds__(64) void self_dot_kernel(const float *__restrict__ 
             const uint32_t rl = tid / 4 + 16 * j, c = (tid % 4) * 4;
             const float *src = X + (size_t)min(r0 + rl, nrows - 1) * ldx + t0 + c;
             float4 v;
-            if constexpr (VEC) {
-                v = *reinterpret_cast<const float4 *>(src);
-            } else {
-                v.x = t0 + c < d ? src[0] : 0.0f;
-                v.y = t0 + c + 1 < d ? src[1] : 0.0f;
-                v.z = t0 + c + 2 < d ? src[2] : 0.0f;
-                v.w = t0 + c + 3 < d ? src[3] : 0.0f;
-            }
+            v.x = t0 + c < d ? src[0] : 0.0f;
+            v.y = t0 + c + 1 < d ? src[1] : 0.0f;
+            v.z = t0 + c + 2 < d ? src[2] : 0.0f;
+            v.w = t0 + c + 3 < d ? src[3] : 0.0f;
             sl[rl][c] = v.x;
             sl[rl][c + 1] = v.y;
             sl[rl][c + 2] = v.z;
@@ -580,7 +575,7 @@ void launch_self_dot(const float *X, uint32_t ldx, uint32_t rows, uint32_t d, fl
     if (ldx % 16 == 0 && ((uintptr_t)X & 15) == 0 && d <= ldx)
         self_dot_wide_kernel<<<ceil_div(rows, 64), 256, 0, s>>>(X, ldx, rows, d, out, dev_rows);
     else
-        self_dot_kernel<false><<<ceil_div(rows, 64), 64, 0, s>>>(X, ldx, rows, d, out, dev_rows);
+        self_dot_kernel<<<ceil_div(rows, 64), 64, 0, s>>>(X, ldx, rows, d, out, dev_rows);
     HM_CHECK_LAUNCH();
 }
 
