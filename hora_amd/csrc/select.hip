@@ -417,6 +417,7 @@ __device__ void stream_row(const float *__restrict__ row, uint32_t n, uint64_t l
 // row (four waves find the groups worth visiting, one wave runs the heap);
 // rows without a tie copy their candidates.
 constexpr uint32_t BF_BLOCK = 256;       // values per lane-wide load (a float4 per lane)
+constexpr uint32_t BF_THREADS = 512;  // bf_finalize_wave_kernel: the group minima by 8 waves, the heap by one
 constexpr uint32_t BF_GMIN_MAX = 16384;  // group minima held in LDS
 
 // Wave-wide minimum (every lane gets it).
@@ -429,7 +430,7 @@ __device__ __forceinline__ float wave_min(float v)
 
 // bpg: blocks of 256 values per group, so that the row has at most
 // BF_GMIN_MAX groups.
-__global__ __launch_bounds__(256) void bf_finalize_wave_kernel(
+__global__ __launch_bounds__(BF_THREADS) void bf_finalize_wave_kernel(
     const float *__restrict__ cand_d, const uint32_t *__restrict__ cand_i, uint32_t ldc,
     const uint32_t *__restrict__ flags, const float *__restrict__ D, uint64_t ldd, uint32_t n,
     uint32_t k, uint32_t bpg, uint64_t *__restrict__ out_idx, float *__restrict__ out_dist,
@@ -445,7 +446,7 @@ __global__ __launch_bounds__(256) void bf_finalize_wave_kernel(
     if (f == ROW_SKIP) return;
     if ((f & (ROW_NAN | ROW_BAD)) && n >= 2) {
         if (tid == 0) atomicOr(err, (f & ROW_BAD) ? 4u : 1u);
-        for (uint32_t j = tid; j < k; j += 256) {
+        for (uint32_t j = tid; j < k; j += BF_THREADS) {
             oi[j] = ~0ull;
             od[j] = inf;
         }
@@ -455,7 +456,7 @@ __global__ __launch_bounds__(256) void bf_finalize_wave_kernel(
     if (!(f & ROW_TIE)) {
         const float *cd = cand_d + (size_t)r * ldc;
         const uint32_t *ci = cand_i + (size_t)r * ldc;
-        for (uint32_t j = tid; j < k; j += 256) {
+        for (uint32_t j = tid; j < k; j += BF_THREADS) {
             oi[j] = j < m ? (uint64_t)ci[j] : ~0ull;
             od[j] = j < m ? cd[j] : inf;
         }
@@ -465,7 +466,7 @@ __global__ __launch_bounds__(256) void bf_finalize_wave_kernel(
     const float *row = D + (size_t)r * ldd;
     // A tied row.  Every element that is not greater than the root of a
     // full heap with a stable layout is pushed; the others change nothing.
-    // The four waves first take the minimum key of every group of bpg
+    // The eight waves first take the minimum key of every group of bpg
     // blocks of 256 values (16 blocks per load round, all in flight
     // together), so the heap wave visits only the groups that can hold such
     // an element, with the next block loading while it works on the current
@@ -473,9 +474,9 @@ __global__ __launch_bounds__(256) void bf_finalize_wave_kernel(
     const uint32_t nb = (n + BF_BLOCK - 1) / BF_BLOCK, ng = (nb + bpg - 1) / bpg;
     extern __shared__ __align__(16) uint2 bf_smem[];
     uint32_t *gkey = reinterpret_cast<uint32_t *>(bf_smem);  // [ng] group minima (fkey)
-    for (uint32_t g = tid; g < ng; g += 256) gkey[g] = ~0u;
+    for (uint32_t g = tid; g < ng; g += BF_THREADS) gkey[g] = ~0u;
     __syncthreads();
-    for (uint32_t b0 = (uint32_t)wave * 16; b0 < nb; b0 += 4 * 16) {
+    for (uint32_t b0 = (uint32_t)wave * 16; b0 < nb; b0 += (BF_THREADS / 64) * 16) {
         float4 v[16];
 #pragma unroll
         for (uint32_t u = 0; u < 16; ++u) {
@@ -892,7 +893,7 @@ void launch_bf_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ld
         uint32_t bpg = 1;
         while ((nb + bpg - 1) / bpg > BF_GMIN_MAX) bpg <<= 1;
         const size_t lds = (size_t)((nb + bpg - 1) / bpg) * 4;  // the group minima
-        bf_finalize_wave_kernel<<<rows, 256, lds, s>>>(cand_d, cand_i, ldc, flags, D, ldd, n, k,
+        bf_finalize_wave_kernel<<<rows, BF_THREADS, lds, s>>>(cand_d, cand_i, ldc, flags, D, ldd, n, k,
                                                        bpg, out_idx, out_dist, out_count, err);
     } else {
         const size_t lds = (size_t)(k + 1) * 8;  // <= 64 KiB
