@@ -760,7 +760,6 @@ class BruteForce final : public Index {
         if (dev_rerun) {
             const uint64_t ldd = round_up(n, 4);
             plan_.reserve(RERUN_PLAN_WORDS * sizeof(uint32_t));
-            r8q_.reserve((size_t)RERUN_SLOTS * dim_ * sizeof(float));
             r8p_.reserve((size_t)RERUN_SLOTS * dpad_ * sizeof(float));
             r8n_.reserve(RERUN_SLOTS * sizeof(float));
             r8D_.reserve((size_t)RERUN_SLOTS * ldd * sizeof(float));
@@ -770,9 +769,8 @@ class BruteForce final : public Index {
             uint32_t *pl = plan_.as<uint32_t>();
             const uint32_t *cnt = pl + 2 * RERUN_SLOTS;
             launch_rerun_plan(rflags_.as<uint32_t>(), (uint32_t)nq, pl, s);
-            launch_gather_rows(dq, dim_, pl, RERUN_SLOTS, (uint32_t)dim_, r8q_.as<float>(), dim_, s);
-            launch_pad_rows(r8q_.as<float>(), dim_, r8p_.as<float>(), dpad_, RERUN_SLOTS,
-                            (uint32_t)dim_, dpad_, s, cnt);
+            launch_pad_rows(dq, dim_, r8p_.as<float>(), dpad_, RERUN_SLOTS, (uint32_t)dim_, dpad_,
+                            s, cnt, nullptr, pl);
             if (need_norm())
                 launch_self_dot(r8p_.as<float>(), dpad_, RERUN_SLOTS, (uint32_t)dim_,
                                 r8n_.as<float>(), s, cnt);
@@ -1016,7 +1014,7 @@ class BruteForce final : public Index {
     DeviceBuffer Ds_, sel_d_, sel_i_, sel_f_, list_, lcnt_, sq_, ksq_, pref_;
     DeviceBuffer redo_rows_, redo_q_, redo_i_, redo_d_, redo_c_;
     PinnedBuffer hflags_, hrows_, herr_, hplan_;  // row flags, rerun rows, error word, rerun plan
-    DeviceBuffer plan_, r8q_, r8p_, r8n_, r8D_, r8i_, r8d_, r8c_;  // device rerun of tied rows
+    DeviceBuffer plan_, r8p_, r8n_, r8D_, r8i_, r8d_, r8c_;  // device rerun of tied rows
     uint64_t last_fallback_ = 0, last_uncertified_ = 0;
     // ragged graph
     hipStream_t rg_stream_ = nullptr;

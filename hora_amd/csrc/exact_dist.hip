@@ -203,17 +203,11 @@ __global__ __launch_bounds__(256, 2) void exact_tile_kernel(
 // same left folds and epilogues as exact_tile_kernel, so the values are the
 // same bits; the 128-query tile would leave most of its work idle here.
 template <int ACC, int EPI, int RQ>
-__global__ __launch_bounds__(256) void exact_rows_kernel(
+__device__ __forceinline__ void exact_rows_body(
     const float *__restrict__ Q, uint32_t ldq, uint32_t nq, const float *__restrict__ B,
     uint32_t ldb, uint32_t n, uint32_t dpad, const float *__restrict__ qnorm,
-    const float *__restrict__ bnorm, float *__restrict__ D, uint64_t ldd,
-    const uint32_t *__restrict__ dev_nq = nullptr, uint32_t lo = 0, uint32_t hi = 0)
+    const float *__restrict__ bnorm, float *__restrict__ D, uint64_t ldd)
 {
-    if (dev_nq) {  // run only when lo <= *dev_nq <= hi, for that many rows
-        const uint32_t c = *dev_nq;
-        if (c < lo || c > hi) return;
-        nq = min(nq, c);
-    }
     extern __shared__ __align__(16) float xr_q[];  // [RQ][dpad]
     const uint32_t tid = threadIdx.x;
     for (uint32_t e = tid; e < RQ * dpad; e += 256) {
@@ -247,6 +241,29 @@ __global__ __launch_bounds__(256) void exact_rows_kernel(
         else if constexpr (EPI == EPI_ANGULAR) a = epi_angular(a, qnorm[r], bnorm[j]);
         D[(size_t)r * ldd + j] = a;
     }
+}
+
+template <int ACC, int EPI, int RQ>
+__global__ __launch_bounds__(256) void exact_rows_kernel(
+    const float *__restrict__ Q, uint32_t ldq, uint32_t nq, const float *__restrict__ B,
+    uint32_t ldb, uint32_t n, uint32_t dpad, const float *__restrict__ qnorm,
+    const float *__restrict__ bnorm, float *__restrict__ D, uint64_t ldd)
+{
+    exact_rows_body<ACC, EPI, RQ>(Q, ldq, nq, B, ldb, n, dpad, qnorm, bnorm, D, ldd);
+}
+
+// The first *dev_nq (<= 8) rows: nothing for 0, the one-query body for 1,
+// the eight-query body otherwise (LDS for eight query rows).
+template <int ACC, int EPI>
+__global__ __launch_bounds__(256) void exact_rows_dev_kernel(
+    const float *__restrict__ Q, uint32_t ldq, const float *__restrict__ B, uint32_t ldb,
+    uint32_t n, uint32_t dpad, const float *__restrict__ qnorm, const float *__restrict__ bnorm,
+    float *__restrict__ D, uint64_t ldd, const uint32_t *__restrict__ dev_nq)
+{
+    const uint32_t c = min(*dev_nq, 8u);
+    if (c == 0) return;
+    if (c == 1) exact_rows_body<ACC, EPI, 1>(Q, ldq, 1, B, ldb, n, dpad, qnorm, bnorm, D, ldd);
+    else exact_rows_body<ACC, EPI, 8>(Q, ldq, c, B, ldb, n, dpad, qnorm, bnorm, D, ldd);
 }
 
 // s = sum_t x_t*x_t as a left fold from -0.0 (the reference's dot(x,x) up to
@@ -513,14 +530,9 @@ void launch_exact_rows_dev(int metric, const float *Q, uint32_t ldq, const float
     if (!exact_rows_dev_ok(dpad, ldb, B)) fail(HM_ERR_INVALID_ARG, "exact_rows_dev: shape");
     ProfScope prof("exact_rows", s);
     const unsigned blocks = ceil_div(n, 256);
-    // one row (the usual case) by the one-query kernel, 2..8 by the eight-query one
 #define HM_ROWS_DEV(ACC, EPI)                                                                  \
-    do {                                                                                       \
-        exact_rows_kernel<ACC, EPI, 1><<<blocks, 256, (size_t)dpad * 4, s>>>(                  \
-            Q, ldq, 1, B, ldb, n, dpad, qnorm, bnorm, D, ldd, dev_nq, 1, 1);                   \
-        exact_rows_kernel<ACC, EPI, 8><<<blocks, 256, (size_t)8 * dpad * 4, s>>>(              \
-            Q, ldq, 8, B, ldb, n, dpad, qnorm, bnorm, D, ldd, dev_nq, 2, 8);                   \
-    } while (0)
+    exact_rows_dev_kernel<ACC, EPI><<<blocks, 256, (size_t)8 * dpad * 4, s>>>(                 \
+        Q, ldq, B, ldb, n, dpad, qnorm, bnorm, D, ldd, dev_nq)
     switch (metric) {
     case HM_METRIC_EUCLIDEAN: HM_ROWS_DEV(ACC_EUCLID, EPI_NONE); break;
     case HM_METRIC_MANHATTAN: HM_ROWS_DEV(ACC_MANHATTAN, EPI_NONE); break;

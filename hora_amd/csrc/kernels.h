@@ -105,10 +105,11 @@ void download_rows(const float *d_src, uint32_t dpad, uint64_t n, uint32_t d, fl
 
 // dst[r][0..d) = src[r][0..d), dst[r][d..dpad) = 0.
 // err (optional): a non-finite source element sets bit 1 of *err, as
-// launch_check_finite does.
+// launch_check_finite does.  src_rows (optional): dst row r is source row
+// src_rows[r] (rows past *dev_rows are not read).
 void launch_pad_rows(const float *src, uint64_t lds, float *dst, uint64_t ldd, uint64_t rows,
                      uint32_t d, uint32_t dpad, hipStream_t s, const uint32_t *dev_rows = nullptr,
-                     uint32_t *err = nullptr);
+                     uint32_t *err = nullptr, const uint32_t *src_rows = nullptr);
 
 // ---- mfma_scores.hip
 // Kmat[q][j] = approximate ranking key of base row j for query q (see file

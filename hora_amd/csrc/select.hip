@@ -841,7 +841,8 @@ __global__ void check_finite_kernel(const float *__restrict__ X, uint64_t ld, ui
 
 __global__ void pad_rows_kernel(const float *__restrict__ src, uint64_t lds, float *__restrict__ dst,
                                 uint64_t ldd, uint64_t rows, uint32_t d, uint32_t dpad,
-                                const uint32_t *__restrict__ dev_rows, uint32_t *err)
+                                const uint32_t *__restrict__ dev_rows, uint32_t *err,
+                                const uint32_t *__restrict__ src_rows)
 {
     if (dev_rows && *dev_rows < rows) rows = *dev_rows;
     uint64_t total = rows * dpad;
@@ -849,7 +850,8 @@ __global__ void pad_rows_kernel(const float *__restrict__ src, uint64_t lds, flo
     for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
          e += (uint64_t)gridDim.x * blockDim.x) {
         uint64_t r = e / dpad, c = e % dpad;
-        const float v = c < d ? src[r * lds + c] : 0.0f;
+        const uint64_t sr = src_rows ? src_rows[r] : r;
+        const float v = c < d ? src[sr * lds + c] : 0.0f;
         bad |= !isfinite(v);
         dst[r * ldd + c] = v;
     }
@@ -1033,12 +1035,13 @@ void download_rows(const float *d_src, uint32_t dpad, uint64_t n, uint32_t d, fl
 
 void launch_pad_rows(const float *src, uint64_t lds, float *dst, uint64_t ldd, uint64_t rows,
                      uint32_t d, uint32_t dpad, hipStream_t s, const uint32_t *dev_rows,
-                     uint32_t *err)
+                     uint32_t *err, const uint32_t *src_rows)
 {
     if (rows == 0) return;
     uint64_t total = rows * dpad;
     unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 8192);
-    pad_rows_kernel<<<blocks, 256, 0, s>>>(src, lds, dst, ldd, rows, d, dpad, dev_rows, err);
+    pad_rows_kernel<<<blocks, 256, 0, s>>>(src, lds, dst, ldd, rows, d, dpad, dev_rows, err,
+                                           src_rows);
     HM_CHECK_LAUNCH();
 }
 
