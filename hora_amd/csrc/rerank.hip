@@ -82,36 +82,80 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
         s_rows[tid] = ((uint32_t)tid < R && !bad) ? idx : 0u;  // rows the loads may read
         __syncthreads();
         float acc = -0.0f;
-        for (uint32_t t0 = 0; t0 < dpad; t0 += 16) {
-            for (uint32_t i = tid; i < R * 4; i += RR_THREADS) {
-                const uint32_t row = i / 4, c = (i % 4) * 4;
-                const float4 v =
-                    *reinterpret_cast<const float4 *>(Bp + (size_t)s_rows[row] * dpad + t0 + c);
-                float *d = sb + row * 17 + c;
-                d[0] = v.x;
-                d[1] = v.y;
-                d[2] = v.z;
-                d[3] = v.w;
-            }
-            __syncthreads();
-            if ((uint32_t)tid < R) {
+        if (R <= 64) {
+            // at most 64 rows: 64-dimension slabs ([64][65] floats, the same
+            // LDS as the [256][17] layout), four 16-byte loads per thread in
+            // flight per slab, so a query makes dpad / 64 round trips
+            float *sw = sb;
+            for (uint32_t t0 = 0; t0 < dpad; t0 += 64) {
 #pragma unroll
-                for (uint32_t e = 0; e < 16; ++e) {
-                    const float q = sq[t0 + e], bq = sb[tid * 17 + e];
-                    if constexpr (ACC == ACC_EUCLID) {
-                        float df = q - bq;
-                        float sqr = df * df;
-                        acc = acc + sqr;
-                    } else if constexpr (ACC == ACC_MANHATTAN) {
-                        float df = q - bq;
-                        acc = acc + fabsf(df);
-                    } else {
-                        float p = q * bq;
-                        acc = acc + p;
+                for (uint32_t u = 0; u < 4; ++u) {
+                    const uint32_t i = (uint32_t)tid + RR_THREADS * u;
+                    const uint32_t row = i / 16, c = (i % 16) * 4;
+                    if (row < R && t0 + c < dpad) {
+                        const float4 v = *reinterpret_cast<const float4 *>(
+                            Bp + (size_t)s_rows[row] * dpad + t0 + c);
+                        float *d = sw + row * 65 + c;
+                        d[0] = v.x;
+                        d[1] = v.y;
+                        d[2] = v.z;
+                        d[3] = v.w;
                     }
                 }
+                __syncthreads();
+                if ((uint32_t)tid < R) {
+                    const uint32_t tn = min(64u, dpad - t0);  // a multiple of 16
+                    for (uint32_t e0 = 0; e0 < tn; e0 += 16)
+#pragma unroll
+                    for (uint32_t e = e0; e < e0 + 16; ++e) {
+                        const float q = sq[t0 + e], bq = sw[tid * 65 + e];
+                        if constexpr (ACC == ACC_EUCLID) {
+                            float df = q - bq;
+                            float sqr = df * df;
+                            acc = acc + sqr;
+                        } else if constexpr (ACC == ACC_MANHATTAN) {
+                            float df = q - bq;
+                            acc = acc + fabsf(df);
+                        } else {
+                            float p = q * bq;
+                            acc = acc + p;
+                        }
+                    }
+                }
+                __syncthreads();
             }
-            __syncthreads();
+        } else {
+            for (uint32_t t0 = 0; t0 < dpad; t0 += 16) {
+                for (uint32_t i = tid; i < R * 4; i += RR_THREADS) {
+                    const uint32_t row = i / 4, c = (i % 4) * 4;
+                    const float4 v =
+                        *reinterpret_cast<const float4 *>(Bp + (size_t)s_rows[row] * dpad + t0 + c);
+                    float *d = sb + row * 17 + c;
+                    d[0] = v.x;
+                    d[1] = v.y;
+                    d[2] = v.z;
+                    d[3] = v.w;
+                }
+                __syncthreads();
+                if ((uint32_t)tid < R) {
+#pragma unroll
+                    for (uint32_t e = 0; e < 16; ++e) {
+                        const float q = sq[t0 + e], bq = sb[tid * 17 + e];
+                        if constexpr (ACC == ACC_EUCLID) {
+                            float df = q - bq;
+                            float sqr = df * df;
+                            acc = acc + sqr;
+                        } else if constexpr (ACC == ACC_MANHATTAN) {
+                            float df = q - bq;
+                            acc = acc + fabsf(df);
+                        } else {
+                            float p = q * bq;
+                            acc = acc + p;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
         }
         if ((uint32_t)tid >= R) continue;
         if (bad) {  // never expected: the read stayed in range; flag the row
