@@ -216,6 +216,57 @@ def test_kmeans_train_runs_and_converges(orc):
     assert np.array_equal(cent, cent2) and np.array_equal(assign, assign2)
 
 
+def _argmin_lowest(orc, x, cent):
+    # strict '<' scan from centre 0 (kmeans.rs:127-148) = numpy's first argmin
+    return np.argmin(orc.distance_rows(cent, x, orc.EUCLIDEAN), axis=1).astype(np.uint32)
+
+
+def test_kmeans_train_last_epoch_keeps_empty_centres_zero(orc):
+    """Kmeans::train (kmeans.rs:195-207): init, then per epoch search_data,
+    update_center and, for every epoch but the last, split_center.  With 4
+    centres the init has only two distinct ones (0 = 2, 1 = 3), so the
+    lowest-index argmin leaves 2 and 3 empty; update_center makes an empty
+    centre all zeros (kmeans.rs:115-122) and, one epoch only, no split runs
+    after it."""
+    rng = np.random.default_rng(80)
+    x = rng.normal(3.0, 1.0, (64, 5)).astype(np.float32)
+    init = orc.kmeans_init(x, 4)
+    cent0, assign0 = orc.kmeans_train(x, 4, 0, seed=5)
+    assert np.array_equal(cent0, init)  # no epoch: the init pattern itself
+    a = _argmin_lowest(orc, x, init)
+    assert set(a.tolist()) <= {0, 1}
+    expect, counts = orc.kmeans_update(x, a, 4)
+    assert counts[2] == 0 and counts[3] == 0
+    cent1, assign1 = orc.kmeans_train(x, 4, 1, seed=5)
+    assert np.array_equal(cent1, expect)
+    assert not cent1[2].any() and not cent1[3].any()
+    assert np.array_equal(assign1, _argmin_lowest(orc, x, cent1))
+
+
+def test_kmeans_train_epoch_loop_composes_search_update_split(orc):
+    """Three epochs: (search, update, split) twice with one random stream,
+    then search and update with no split; the final assignment is one more
+    search (pq_idx.rs:366-367)."""
+    import ctypes
+
+    L = orc.lib()
+    L.orc_stream_seed.restype = ctypes.c_uint64
+    L.orc_stream_seed.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    rng = np.random.default_rng(81)
+    x = rng.normal(0.0, 2.0, (120, 6)).astype(np.float32)
+    for n_center, seed in ((4, 9), (8, 10), (16, 11)):
+        state = L.orc_stream_seed(seed, 0)
+        cent = orc.kmeans_init(x, n_center)
+        for epoch in range(3):
+            a = _argmin_lowest(orc, x, cent)
+            cent, counts = orc.kmeans_update(x, a, n_center)
+            if epoch < 2:
+                cent, counts, state = orc.kmeans_split(cent, counts, x.shape[0], state)
+        got, assign = orc.kmeans_train(x, n_center, 3, seed=seed)
+        assert np.array_equal(got, cent), n_center
+        assert np.array_equal(assign, _argmin_lowest(orc, x, cent))
+
+
 # --------------------------------------------------------------------- PQ
 def test_pq_ranges_closed_form(orc):
     assert orc.pq_ranges(10, 4) == [(0, 2), (2, 4), (4, 6), (6, 8)]  # dims 8,9 dropped
@@ -266,6 +317,120 @@ def test_ivfpq_search_matches_bruteforce_over_probed_lists(orc):
         assert len({c[0] for c in cand[:8]}) == 8  # tie-free case
         assert ids[r].tolist() == [c[1] for c in cand[:7]]
         assert dist[r].tolist() == [c[0] for c in cand[:7]]
+
+
+def _heap_model_drain(orc, dists, ids, k):
+    """push every (dist, id), pop while more than k remain, then drain by
+    pops: the std-heap model's output in pop order (largest first)."""
+    n = len(dists)
+    m = min(k, n)
+    ops = [0] * n + [1] * (n - m) + [1] * m
+    _, popped = orc.heap_script(ops, np.concatenate([np.asarray(dists, np.float32),
+                                                     np.zeros(n, np.float32)]),
+                                np.concatenate([np.asarray(ids, np.uint64),
+                                                np.zeros(n, np.uint64)]))
+    return popped[n - m:]
+
+
+def _ivfpq_model(orc, q, cent, cb, offsets, order, codes, nprobe, k):
+    """IVFPQIndex::search_knn_adc + node_search_k (pq_idx.rs:408-477) composed
+    from the hand-traced heap model: the coarse heap of (-dist, list) popped
+    nprobe times; per probed list the push-all / pop-to-k heap of its ADC
+    distances, drained by pops and each entry remapped and pushed into the
+    global heap, which pops whenever it holds more than k; the final drain
+    reversed."""
+    nlist = cent.shape[0]
+    n_sub = cb.shape[1]
+    cd = np.array([orc.metric(q, cent[i], orc.EUCLIDEAN) for i in range(nlist)], np.float32)
+    _, popped = orc.heap_script([0] * nlist + [1] * nprobe,
+                                np.concatenate([-cd, np.zeros(nprobe, np.float32)]),
+                                np.concatenate([np.arange(nlist), np.zeros(nprobe)]))
+    probes = [i for _, i in popped]
+    gops, gd, gi, size = [], [], [], 0
+    for l in probes:
+        a, b = int(offsets[l]), int(offsets[l + 1])
+        adc = _adc_fold(orc.pq_lut(q, cent[l], n_sub, cb[l]), codes[a:b])
+        for dist, li in _heap_model_drain(orc, adc, np.arange(b - a), k):
+            gops.append(0), gd.append(dist), gi.append(int(order[a + li]))
+            size += 1
+            if size > k:
+                gops.append(1), gd.append(0.0), gi.append(0)
+                size -= 1
+    gops += [1] * size
+    gd += [0.0] * size
+    gi += [0] * size
+    _, popped = orc.heap_script(gops, np.array(gd, np.float32), np.array(gi, np.uint64))
+    tail = popped[len(popped) - size:][::-1]
+    return probes, [i for _, i in tail], [x for x, _ in tail]
+
+
+def _tied_ivfpq(orc, seed, nlist=5, n=90):
+    """Integer centres, codebooks and queries: equal ADC distances inside and
+    across lists, and centres at equal distance from a query."""
+    rng = np.random.default_rng(seed)
+    d, n_sub, K = 2, 2, 3
+    cent = rng.integers(-1, 2, (nlist, d)).astype(np.float32)
+    cent[1] = -cent[0]  # a mirrored pair: equidistant from the origin
+    x = rng.integers(-2, 3, (n, d)).astype(np.float32)
+    assign = orc.kmeans_assign(x, cent)
+    order = np.argsort(assign, kind="stable").astype(np.uint64)
+    offsets = np.concatenate([[0], np.cumsum(np.bincount(assign, minlength=nlist))])
+    cb = rng.integers(-1, 2, (nlist, n_sub, K, 1)).astype(np.float32)
+    codes = rng.integers(0, K, (n, n_sub)).astype(np.uint32)
+    q = rng.integers(-1, 2, (6, d)).astype(np.float32)
+    q[0] = 0.0
+    return q, cent, cb, offsets, order, codes
+
+
+@pytest.mark.parametrize("seed", [70, 71, 72])
+def test_ivfpq_search_ties_follow_the_composed_heap_model(orc, seed):
+    q, cent, cb, offsets, order, codes = _tied_ivfpq(orc, seed)
+    nlist = cent.shape[0]
+    for nprobe in (1, 2, 3, nlist):
+        for k in (1, 2, 5, 17, 200):
+            ids, dist, cnt, probes = orc.ivfpq_search(q, orc.EUCLIDEAN, cent, cb, offsets, order,
+                                                      codes, nprobe, k)
+            for r in range(q.shape[0]):
+                p, mi, md = _ivfpq_model(orc, q[r], cent, cb, offsets, order, codes, nprobe, k)
+                assert probes[r].tolist() == p, (seed, nprobe, k, r)
+                assert cnt[r] == len(mi)
+                assert ids[r, :cnt[r]].tolist() == mi, (seed, nprobe, k, r)
+                assert dist[r, :cnt[r]].tolist() == md
+    # the data really is tied: coarse ties for the origin query, ADC ties in lists
+    cd = [orc.metric(q[0], c, orc.EUCLIDEAN) for c in cent]
+    assert cd[0] == cd[1]
+
+
+def test_ivfpq_search_two_equal_lists_hand_traced(orc):
+    """Two lists of two items, every ADC distance 1, k=2, nprobe 2, and the
+    two centres equidistant from the query.  Traced by hand from heap.rs
+    (push: sift_up stops below an equal parent; pop: the last element takes
+    the root, sinks to the bottom choosing the right child when left <= right
+    or the lone left child, then sifts up):
+      coarse (-1,c0), (-1,c1): [c0,c1]; pops give c0, then c1.
+      list 0 (items 0, 2): [0,2]; drain pops 0 (2 takes the root), then 2.
+      global: push 0, 2 -> [0,2].
+      list 1 (items 1, 3): drains 1, then 3.
+      push 1 -> [0,2,1]; pop returns 0: 1 takes the root, sinks to the lone
+        left child's place (2 moves up) -> [2,1].
+      push 3 -> [2,1,3]; pop returns 2: 3 takes the root, 1 moves up -> [1,3].
+      final drain pops 1, then 3; reversed: ids [3, 1]."""
+    cent = np.array([[1.0], [-1.0]], np.float32)
+    offsets = np.array([0, 2, 4], np.uint64)
+    order = np.array([0, 2, 1, 3], np.uint64)
+    codes = np.array([[1], [1], [1], [1]], np.uint32)
+    # one sub-space of one dim; the query 0 has residual -1 in list 0 and +1
+    # in list 1, and codeword 1 sits at distance 1 from each residual
+    cb = np.zeros((2, 1, 2, 1), np.float32)
+    cb[0, 0, 1, 0] = -2.0
+    cb[1, 0, 1, 0] = 2.0
+    q = np.zeros((1, 1), np.float32)
+    ids, dist, cnt, probes = orc.ivfpq_search(q, orc.EUCLIDEAN, cent, cb, offsets, order, codes,
+                                              2, 2)
+    assert probes[0].tolist() == [0, 1]
+    assert cnt[0] == 2
+    assert dist[0].tolist() == [1.0, 1.0]
+    assert ids[0].tolist() == [3, 1]
 
 
 # ---------------------------------------------------------- standalone PQ
@@ -399,6 +564,63 @@ def test_bincode_layout_hand_assembled():
     got_pq = bc.pqindex(2, 1, 1, 5, [[[0.5, 1.5], [2.0, 3.0]]], np.array([[1]]), [[1.0, 2.0]],
                         [bc.option_usize(0)], 2)
     assert got_pq == want_pq
+
+
+def test_bincode_ivfpq_layout_hand_assembled():
+    """A trained two-list IVFPQIndex (pq_idx.rs:267-292) assembled by hand:
+    the top-level fields in declaration order, each list's PQIndex
+    (pq_idx.rs:21-45) with its residual set to the list centre
+    (pq_idx.rs:376-393; mt stays PQIndex::new's Euclidean, pq_idx.rs:71),
+    the rows in `_nodes` and again in `_nodes_tmp` (dump, pq_idx.rs:510-515),
+    and the top-level `_assigned_center` left empty (train never fills it)."""
+    import struct
+
+    from oracle import bincode as bc
+
+    q, i, f = (lambda v: struct.pack("<Q", v)), (lambda v: struct.pack("<I", v)), \
+        (lambda v: struct.pack("<f", v))
+    some = lambda v: b"\x01" + q(v)  # noqa: E731
+    none = b"\x00"
+
+    def node(a, b, tag):
+        return q(2) + f(a) + f(b) + tag
+
+    n0, n1, n2 = node(1.0, 2.0, some(10)), node(5.0, 4.0, some(11)), node(0.0, 1.0, none)
+
+    def pq(residual, cw, members, codes):
+        return (q(2) + q(1) + q(2)                         # _dimension, _n_sub, _sub_dimension
+                + q(1) + q(2) + q(0) + q(2)                # _dimension_range [[0, 2]]
+                + q(1) + q(1) + q(2) + q(1) + q(5)         # _sub_bits .. _train_epoch
+                + q(1) + q(2)                              # _centers: 1 sub-space, 2 codewords
+                + q(2) + f(cw[0][0]) + f(cw[0][1]) + q(2) + f(cw[1][0]) + f(cw[1][1])
+                + b"\x01" + b"\x01"                        # _is_trained, _has_residual
+                + q(2) + f(residual[0]) + f(residual[1])   # _residual = the list centre
+                + q(len(members)) + q(100000)              # _n_items, _max_item
+                + q(len(members)) + b"".join(members)      # _nodes
+                + q(1) + q(len(codes)) + b"".join(q(c) for c in codes)  # _assigned_center
+                + i(3)                                     # mt: Euclidean
+                + q(len(members)) + b"".join(members))     # _nodes_tmp
+    want = (q(2) + q(1) + q(2) + q(1) + q(1) + q(2) + q(1)  # _dimension .. _code_bytes
+            + q(5) + q(1) + q(2)                            # _train_epoch, _search_n_center, nlist
+            + q(2) + q(2) + f(0.5) + f(1.5) + q(2) + f(4.0) + f(4.0)   # _centers
+            + q(2) + q(2) + q(0) + q(2) + q(1) + q(1)       # _ivflist [[0, 2], [1]]
+            + q(2)                                          # _pq_list
+            + pq((0.5, 1.5), ((0.1, 0.2), (0.3, 0.4)), [n0, n2], [1, 0])
+            + pq((4.0, 4.0), ((1.0, 1.0), (2.0, 2.0)), [n1], [1])
+            + b"\x01"                                       # _is_trained
+            + q(3) + q(100000)                              # _n_items, _max_item
+            + q(3) + n0 + n1 + n2                           # _nodes
+            + q(0)                                          # _assigned_center: empty
+            + i(2)                                          # mt: DotProduct
+            + q(3) + n0 + n1 + n2)                          # _nodes_tmp
+    cb = np.zeros((2, 1, 2, 2), np.float32)
+    cb[0, 0] = [[0.1, 0.2], [0.3, 0.4]]
+    cb[1, 0] = [[1.0, 1.0], [2.0, 2.0]]
+    got = bc.ivfpq(2, 1, 1, 5, 1, 2, np.array([[0.5, 1.5], [4.0, 4.0]], np.float32),
+                   [0, 2, 3], [0, 2, 1], cb, np.array([[1], [0], [1]]),
+                   [[1.0, 2.0], [5.0, 4.0], [0.0, 1.0]],
+                   [bc.option_usize(10), bc.option_usize(11), bc.option_usize(None)], 2)
+    assert got == want
 
 
 # ------------------------------------------------------------- kNN graph
