@@ -250,25 +250,39 @@ class BruteForce final : public Index {
         cand_i_.reserve(qc * K * sizeof(uint32_t));
         flags_.reserve(qc * sizeof(uint32_t));
         qnorm_.reserve(qc * sizeof(float));
+        const ExactBufs b{qpad_.as<float>(), qnorm_.as<float>(), D_.as<float>(),
+                          cand_d_.as<float>(), cand_i_.as<uint32_t>(), flags_.as<uint32_t>(),
+                          err_.as<uint32_t>()};
         for (uint64_t q0 = 0; q0 < nq; q0 += qc) {
             const uint32_t rows = (uint32_t)std::min<uint64_t>(qc, nq - q0);
-            launch_pad_rows(dq + q0 * ldq, ldq, qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
-                            dpad_, s);
-            if (need_norm())
-                launch_self_dot(qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
-                                qnorm_.as<float>(), s);
-            launch_exact_tile(metric_, qpad_.as<float>(), dpad_, rows, base_.as<float>(), dpad_,
-                              n, dpad_, qnorm_.as<float>(), bnorm_.as<float>(), D_.as<float>(),
-                              ldd, s);
-            if (!all_tied)
-                launch_row_select(D_.as<float>(), ldd, rows, n, K, cand_d_.as<float>(),
-                                  cand_i_.as<uint32_t>(), K, flags_.as<uint32_t>(), s);
-            launch_bf_finalize(cand_d_.as<float>(), cand_i_.as<uint32_t>(), K,
-                               all_tied ? nullptr : flags_.as<uint32_t>(), D_.as<float>(), ldd, rows, n, k,
-                               d_idx + q0 * k, d_dist + q0 * k, d_count + q0,
-                               err_.as<uint32_t>(), s);
+            exact_pass(dq + q0 * ldq, ldq, rows, k, K, ldd, b, d_idx + q0 * k, d_dist + q0 * k,
+                       d_count + q0, s, all_tied);
         }
     }
+
+    // Working buffers of one exact pass over `rows` queries: padded queries
+    // [rows][dpad], their self-dots, the distance block [rows][ldd], the
+    // top-K candidates [rows][K] and row flags, and the error word.
+    struct ExactBufs {
+        float *qpad, *qnorm, *D, *cand_d;
+        uint32_t *cand_i, *flags, *err;
+    };
+
+    void exact_pass(const float *dq, uint64_t ldq, uint32_t rows, uint32_t k, uint32_t K,
+                    uint64_t ldd, const ExactBufs &b, uint64_t *d_idx, float *d_dist,
+                    uint32_t *d_count, hipStream_t s, bool all_tied)
+    {
+        const uint32_t n = (uint32_t)n_;
+        launch_pad_rows(dq, ldq, b.qpad, dpad_, rows, (uint32_t)dim_, dpad_, s);
+        if (need_norm()) launch_self_dot(b.qpad, dpad_, rows, (uint32_t)dim_, b.qnorm, s);
+        launch_exact_tile(metric_, b.qpad, dpad_, rows, base_.as<float>(), dpad_, n, dpad_,
+                          b.qnorm, bnorm_.as<float>(), b.D, ldd, s);
+        if (!all_tied)
+            launch_row_select(b.D, ldd, rows, n, K, b.cand_d, b.cand_i, K, b.flags, s);
+        launch_bf_finalize(b.cand_d, b.cand_i, K, all_tied ? nullptr : b.flags, b.D, ldd, rows, n,
+                           k, d_idx, d_dist, d_count, b.err, s);
+    }
+
 
     // naive_build_knn_graph (knn.rs:14-41): exact distances of a chunk of
     // rows against all rows, the row's own column excluded.
@@ -534,10 +548,6 @@ class BruteForce final : public Index {
         HM_HIP(hipMemcpyAsync(redo_rows_.get(), rows.data(), m * 4, hipMemcpyHostToDevice, s));
         launch_gather_rows(dq, dim_, redo_rows_.as<uint32_t>(), m, (uint32_t)dim_,
                            redo_q_.as<float>(), dim_, s);
-        if (!err_.get()) {
-            err_.reserve(4);
-            HM_HIP(hipMemsetAsync(err_.get(), 0, 4, s));
-        }
         search_exact(redo_q_.as<float>(), dim_, m, kmax, redo_i_.as<uint64_t>(),
                      redo_d_.as<float>(), redo_c_.as<uint32_t>(), s, all_tied);
         launch_scatter_ragged(redo_i_.as<uint64_t>(), redo_d_.as<float>(), redo_c_.as<uint32_t>(),
@@ -860,33 +870,51 @@ class BruteForce final : public Index {
         if (!s) s = stream_;
         if (metric_ == HM_METRIC_UNKNOWN) fail(HM_ERR_UNKNOWN_METRIC, "unknown method");
         if (nb == 0) return;
-        const uint64_t Q = h_off[nb];
-        if (Q == 0) return;
-        std::vector<uint32_t> kq(Q);
-        std::vector<uint64_t> off(Q + 1);
+        if (h_off[0] != 0) fail(HM_ERR_INVALID_ARG, "batch_offsets[0] must be 0");
+        uint64_t total = 0;
         uint32_t kmax = 0;
         for (uint32_t b = 0; b < nb; ++b) {
             if (h_off[b + 1] < h_off[b]) fail(HM_ERR_INVALID_ARG, "batch offsets not ascending");
-            for (uint64_t q = h_off[b]; q < h_off[b + 1]; ++q) kq[q] = h_k[b];
+            total += (uint64_t)(h_off[b + 1] - h_off[b]) * h_k[b];
             kmax = std::max(kmax, h_k[b]);
         }
+        const uint64_t Q = h_off[nb];
+        if (Q == 0) return;
+        if (!dq || !d_count || (total && (!d_idx || !d_dist)))
+            fail(HM_ERR_INVALID_ARG, "null pointer");
+        if (Q > 0x7FFFFFFFull) fail(HM_ERR_INVALID_ARG, "too many queries");
+        err_.reserve(4);
+        HM_HIP(hipMemsetAsync(err_.get(), 0, 4, s));
+        launch_check_finite(dq, dim_, Q, (uint32_t)dim_, err_.as<uint32_t>(), s);
+        const bool shortlist_form = kmax > 0 && shortlist_usable(s);
+        if (!shortlist_form) {
+            // exact form: each batch is an exact search with its own k,
+            // written where the packed layout puts it
+            uint64_t o = 0;
+            last_fallback_ = last_uncertified_ = 0;
+            for (uint32_t b = 0; b < nb; ++b) {
+                const uint64_t q0 = h_off[b], m = h_off[b + 1] - q0;
+                if (m == 0) continue;
+                if (h_k[b] == 0 || n_ == 0)
+                    launch_fill_results(d_idx + o, d_dist + o, d_count + q0, m, h_k[b], s);
+                else
+                    search_exact(dq + q0 * dim_, dim_, m, h_k[b], d_idx + o, d_dist + o,
+                                 d_count + q0, s);
+                o += m * h_k[b];
+            }
+            read_error(s);
+            return;
+        }
+        std::vector<uint32_t> kq(Q);
+        std::vector<uint64_t> off(Q + 1);
+        for (uint32_t b = 0; b < nb; ++b)
+            for (uint64_t q = h_off[b]; q < h_off[b + 1]; ++q) kq[q] = h_k[b];
         off[0] = 0;
         for (uint64_t q = 0; q < Q; ++q) off[q + 1] = off[q] + kq[q];
-        const bool shortlist_form = kmax > 0 && shortlist_usable(s);
         rg_kq_.reserve(Q * 4);
         rg_off_.reserve((Q + 1) * 8);
         HM_HIP(hipMemcpyAsync(rg_kq_.get(), kq.data(), Q * 4, hipMemcpyHostToDevice, s));
         HM_HIP(hipMemcpyAsync(rg_off_.get(), off.data(), (Q + 1) * 8, hipMemcpyHostToDevice, s));
-        if (!shortlist_form) {
-            // exact form: one exact search with kmax, packed per query
-            std::vector<uint32_t> all(Q);
-            for (uint64_t q = 0; q < Q; ++q) all[q] = (uint32_t)q;
-            ragged_exact(dq, all, kmax, d_idx, d_dist, d_count, s);
-            return;
-        }
-        err_.reserve(4);
-        HM_HIP(hipMemsetAsync(err_.get(), 0, 4, s));
-        launch_check_finite(dq, dim_, Q, (uint32_t)dim_, err_.as<uint32_t>(), s);
         ensure_shortlist_data(s);
         const uint32_t S = shortlist_size(kmax);
         const uint32_t QC = RAG_CHUNK;
@@ -933,12 +961,34 @@ class BruteForce final : public Index {
             last_uncertified_ += (fl[q] & ROW_UNCERT) ? 1 : 0;
         }
         last_fallback_ = redo.size() + tied.size();
-        if (!tied.empty()) ragged_exact(dq, tied, kmax, d_idx, d_dist, d_count, s, true);
-        if (!redo.empty()) ragged_exact(dq, redo, kmax, d_idx, d_dist, d_count, s);
+        // a tied row's result depends on its own k (the heap's capacity),
+        // so the rows rerun in groups of equal k
+        auto rerun_by_k = [&](const std::vector<uint32_t> &rows, bool all_tied) {
+            std::vector<uint32_t> order(rows);
+            std::stable_sort(order.begin(), order.end(),
+                             [&](uint32_t a, uint32_t b) { return kq[a] < kq[b]; });
+            for (size_t i = 0; i < order.size();) {
+                size_t j = i;
+                while (j < order.size() && kq[order[j]] == kq[order[i]]) ++j;
+                ragged_exact(dq, std::vector<uint32_t>(order.begin() + i, order.begin() + j),
+                             kq[order[i]], d_idx, d_dist, d_count, s, all_tied);
+                i = j;
+            }
+        };
+        if (!tied.empty()) rerun_by_k(tied, true);
+        if (!redo.empty()) rerun_by_k(redo, false);
+        read_error(s);
+    }
+
+    // The error word of the last search: non-finite input, a NaN distance,
+    // an internal consistency failure.
+    void read_error(hipStream_t s)
+    {
         uint32_t err = 0;
         HM_HIP(hipMemcpyAsync(&err, err_.get(), 4, hipMemcpyDeviceToHost, s));
         HM_HIP(hipStreamSynchronize(s));
         if (err & 2u) fail(HM_ERR_NONFINITE_INPUT, "invalid float element");
+        if (err & 4u) fail(HM_ERR_HIP, "internal: selection consistency check failed");
         if (err & 1u) fail(HM_ERR_NAN_DISTANCE, "NaN distance (partial_cmp of NaN)");
     }
 
@@ -954,9 +1004,23 @@ class BruteForce final : public Index {
         od.reserve(nq * std::max(k, 1u) * 4);
         oc.reserve(nq * 4);
         HM_HIP(hipMemcpyAsync(q.get(), h_queries, nq * dim_ * 4, hipMemcpyHostToDevice, s));
-        hipEvent_t e0, e1;
-        HM_HIP(hipEventCreate(&e0));
-        HM_HIP(hipEventCreate(&e1));
+        // events destroyed and the caller's depth restored on every exit,
+        // also when a trial search throws (NaN distance, out of memory)
+        struct Sweep {
+            uint32_t &depth;
+            const uint32_t saved;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            bool done = false;
+            ~Sweep()
+            {
+                if (e0) (void)hipEventDestroy(e0);
+                if (e1) (void)hipEventDestroy(e1);
+                if (!done) depth = saved;
+            }
+        } sw{shortlist_, shortlist_};
+        HM_HIP(hipEventCreate(&sw.e0));
+        HM_HIP(hipEventCreate(&sw.e1));
+        hipEvent_t e0 = sw.e0, e1 = sw.e1;
         float best_ms = 1e30f;
         uint32_t best = 0;
         const uint32_t saved = shortlist_;
@@ -977,9 +1041,8 @@ class BruteForce final : public Index {
                 best = S;
             }
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         shortlist_ = best ? best : saved;
+        sw.done = true;
         return shortlist_;
     }
 

@@ -85,10 +85,13 @@ void launch_kth_smallest(const float *D, uint64_t ldd, uint32_t rows, uint32_t n
 // std-BinaryHeap push/pop stream over the whole distance row.  A NaN row
 // sets bit 0 of *err (when k >= 1 and n >= 2).  flags may be null: every
 // row is then tied (and free of NaN), and the candidates are not read.
+// k is the output stride; row_k (optional) gives row r its own k, read at
+// row_k[row_k_map[r]] (or row_k[r]) and capped at the stride.
 void launch_bf_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ldc,
                         const uint32_t *flags, const float *D, uint64_t ldd, uint32_t rows,
                         uint32_t n, uint32_t k, uint64_t *out_idx, float *out_dist,
-                        uint32_t *out_count, uint32_t *err, hipStream_t s);
+                        uint32_t *out_count, uint32_t *err, hipStream_t s,
+                        const uint32_t *row_k = nullptr, const uint32_t *row_k_map = nullptr);
 
 // Non-finite check of a device matrix (rows x d, stride ld): sets *err bit 1.
 void launch_check_finite(const float *X, uint64_t ld, uint64_t rows, uint32_t d, uint32_t *err,

@@ -434,14 +434,18 @@ __global__ __launch_bounds__(BF_THREADS) void bf_finalize_wave_kernel(
     const float *__restrict__ cand_d, const uint32_t *__restrict__ cand_i, uint32_t ldc,
     const uint32_t *__restrict__ flags, const float *__restrict__ D, uint64_t ldd, uint32_t n,
     uint32_t k, uint32_t bpg, uint64_t *__restrict__ out_idx, float *__restrict__ out_dist,
-    uint32_t *__restrict__ out_count, uint32_t *err)
+    uint32_t *__restrict__ out_count, uint32_t *err, const uint32_t *__restrict__ row_k,
+    const uint32_t *__restrict__ row_k_map)
 {
     const uint32_t r = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const float inf = __int_as_float(0x7f800000);
-    uint64_t *oi = out_idx + (size_t)r * k;
+    uint64_t *oi = out_idx + (size_t)r * k;  // k: the output stride
     float *od = out_dist + (size_t)r * k;
-    const uint32_t m = k < n ? k : n;
+    // this row's k (at most the stride), and the heap's capacity: a heap
+    // over n <= k elements never pops, so min(k_r, n) replays the same steps
+    const uint32_t kr = row_k ? min(row_k[row_k_map ? row_k_map[r] : r], k) : k;
+    const uint32_t m = kr < n ? kr : n;
     const uint32_t f = flags ? flags[r] : ROW_TIE;  // no flags: every row tied
     if (f == ROW_SKIP) return;
     if ((f & (ROW_NAN | ROW_BAD)) && n >= 2) {
@@ -521,7 +525,7 @@ __global__ __launch_bounds__(BF_THREADS) void bf_finalize_wave_kernel(
         }
         return __ballot(bad) == 0;
     };
-    auto skip = [&]() { return h.len == k && stable; };
+    auto skip = [&]() { return h.len == m && stable; };
     // first block at or after b that can change the heap: inside a group
     // being visited the next block; else the first block of the next group
     // whose minimum is not above the root (any group while not skipping)
@@ -567,8 +571,8 @@ __global__ __launch_bounds__(BF_THREADS) void bf_finalize_wave_kernel(
             const float x =
                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sel), (int)(e >> 2)));
             h.push(make_uint2(base + e, __float_as_uint(x)));
-            if (h.len > k) (void)h.pop();
-            if (h.len == k) {
+            if (h.len > m) (void)h.pop();
+            if (h.len == m) {
                 root = h.val_at(0);
                 stable = root < inf && noop();
             }
@@ -628,15 +632,18 @@ __global__ __launch_bounds__(64) void bf_replay_lds_kernel(
     const float *__restrict__ cand_d, const uint32_t *__restrict__ cand_i, uint32_t ldc,
     const uint32_t *__restrict__ flags, const float *__restrict__ D, uint64_t ldd, uint32_t n,
     uint32_t k, uint64_t *__restrict__ out_idx, float *__restrict__ out_dist,
-    uint32_t *__restrict__ out_count, uint32_t *err)
+    uint32_t *__restrict__ out_count, uint32_t *err, const uint32_t *__restrict__ row_k,
+    const uint32_t *__restrict__ row_k_map)
 {
     extern __shared__ __align__(16) unsigned char rp_smem[];
     const uint32_t r = blockIdx.x;
     const int lane = threadIdx.x;
     const float inf = __int_as_float(0x7f800000);
-    uint64_t *oi = out_idx + (size_t)r * k;
+    uint64_t *oi = out_idx + (size_t)r * k;  // k: the output stride
     float *od = out_dist + (size_t)r * k;
-    const uint32_t m = k < n ? k : n;
+    const uint32_t kr = row_k ? min(row_k[row_k_map ? row_k_map[r] : r], k) : k;
+    const uint32_t m = kr < n ? kr : n;  // heap capacity (see the wave kernel)
+    const uint32_t cap = k < n ? k : n;  // the LDS layout's capacity, every row
     const uint32_t f = flags ? flags[r] : ROW_TIE;  // no flags: every row tied
     if (f == ROW_SKIP) return;
     if ((f & (ROW_NAN | ROW_BAD)) && n >= 2) {
@@ -660,18 +667,18 @@ __global__ __launch_bounds__(64) void bf_replay_lds_kernel(
     }
     LaneHeap h;
     h.d = reinterpret_cast<float *>(rp_smem);
-    h.ix = reinterpret_cast<uint32_t *>(rp_smem + (size_t)(k + 1) * 4);
+    h.ix = reinterpret_cast<uint32_t *>(rp_smem + (size_t)(cap + 1) * 4);
     h.len = 0;
     const float *row = D + (size_t)r * ldd;
     bool stable = false;
     float root = inf;
     __shared__ __align__(16) float sv[4096];
-    auto threshold = [&]() { return (h.len == k && stable) ? root : inf; };
+    auto threshold = [&]() { return (h.len == m && stable) ? root : inf; };
     stream_row(row, n, ldd, lane, sv, threshold,
                [&](uint32_t base, float v, const float *cv, uint32_t cnt) {
         uint32_t j = 0;
         while (j < cnt) {
-            if (h.len == k && stable) {
+            if (h.len == m && stable) {
                 const unsigned long long live =
                     __ballot((uint32_t)lane >= j && (uint32_t)lane < cnt && !(v > root));
                 if (live == 0) break;
@@ -679,12 +686,12 @@ __global__ __launch_bounds__(64) void bf_replay_lds_kernel(
             }
             const float x = cv[j];
             h.push(x, base + j);
-            if (h.len > k) {
+            if (h.len > m) {
                 float a;
                 uint32_t b;
                 h.pop(&a, &b);
             }
-            if (h.len == k) {
+            if (h.len == m) {
                 root = h.d[0];
                 stable = root < inf && heap_noop_roundtrip(h.d, h.len);
             }
@@ -882,25 +889,30 @@ void launch_row_select(const float *D, uint64_t ldd, uint32_t rows, uint32_t n, 
 void launch_bf_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ldc,
                         const uint32_t *flags, const float *D, uint64_t ldd, uint32_t rows,
                         uint32_t n, uint32_t k, uint64_t *out_idx, float *out_dist,
-                        uint32_t *out_count, uint32_t *err, hipStream_t s)
+                        uint32_t *out_count, uint32_t *err, hipStream_t s,
+                        const uint32_t *row_k, const uint32_t *row_k_map)
 {
     if (rows == 0) return;
     if (k == 0) fail(HM_ERR_INVALID_ARG, "bf_finalize: k is zero");
-    if (k + 1 > SELECT_MAX_K) fail(HM_ERR_INVALID_ARG, "bf_finalize: k too large");
+    // the heap never holds more than min(k, n) + 1 entries
+    const uint32_t cap = k < n ? k : n;
+    if (cap + 1 > SELECT_MAX_K) fail(HM_ERR_INVALID_ARG, "bf_finalize: k too large");
     if (ldd % 4 || ldd < n || ((uintptr_t)D & 15))
         fail(HM_ERR_INVALID_ARG, "bf_finalize: rows must be 16-byte aligned, stride a multiple of 4");
     ProfScope prof("bf_finalize", s);
-    if (k + 1 <= 64) {
+    if (cap + 1 <= 64) {
         const uint32_t nb = (n + BF_BLOCK - 1) / BF_BLOCK;
         uint32_t bpg = 1;
         while ((nb + bpg - 1) / bpg > BF_GMIN_MAX) bpg <<= 1;
         const size_t lds = (size_t)((nb + bpg - 1) / bpg) * 4;  // the group minima
         bf_finalize_wave_kernel<<<rows, BF_THREADS, lds, s>>>(cand_d, cand_i, ldc, flags, D, ldd, n, k,
-                                                       bpg, out_idx, out_dist, out_count, err);
+                                                       bpg, out_idx, out_dist, out_count, err,
+                                                       row_k, row_k_map);
     } else {
-        const size_t lds = (size_t)(k + 1) * 8;  // <= 64 KiB
+        const size_t lds = (size_t)(cap + 1) * 8;  // <= 64 KiB
         bf_replay_lds_kernel<<<rows, 64, lds, s>>>(cand_d, cand_i, ldc, flags, D, ldd, n, k,
-                                                   out_idx, out_dist, out_count, err);
+                                                   out_idx, out_dist, out_count, err, row_k,
+                                                   row_k_map);
     }
     HM_CHECK_LAUNCH();
 }

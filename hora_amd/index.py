@@ -396,6 +396,8 @@ class BruteForceIndex(_Index):
         """Ragged batches with per-batch k on device buffers (see the C header)."""
         off = np.ascontiguousarray(batch_offsets, dtype=np.uint32)
         ks = np.ascontiguousarray(batch_k, dtype=np.uint32)
+        if off.size != ks.size + 1:
+            raise HoraError(1, "batch_offsets needs one entry more than batch_k")
         check(lib().hm_index_search_ragged_device(self._h, q_ptr, ks.size, off.ctypes.data,
                                                   ks.ctypes.data, idx_ptr, dist_ptr, count_ptr,
                                                   stream or None))

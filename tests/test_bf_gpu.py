@@ -200,3 +200,17 @@ def test_add_after_build_and_clear(orc):
     _check(orc, base, q, 5, Metric.CosineSimilarity, idx=idx)
     idx.clear()
     assert idx.nodes_size() == 0
+
+
+@pytest.mark.parametrize("mode", [hora_amd.HM_SEARCH_EXACT, hora_amd.HM_SEARCH_BF16_RERANK])
+def test_k_beyond_select_limit_on_small_index(orc, mode):
+    # bruteforce_idx.rs:47-73 returns min(k, n) results for any k
+    base = bf_normal(100, 8, seed=96)
+    q = bf_normal(3, 8, seed=97)
+    idx = _index(base, Metric.Euclidean)
+    idx.set_search_mode(mode, 0)
+    gi, gd, gc = idx.search_batch(q, 9000)
+    ri, rd, rc = orc.bf_search(base, q, 9000, orc.EUCLIDEAN)
+    assert np.array_equal(gc, rc) and (gc == 100).all()
+    assert np.array_equal(gi[:, :100], ri[:, :100]) and np.array_equal(gd[:, :100], rd[:, :100])
+    assert (gi[:, 100:] == np.iinfo(np.uint64).max).all() and np.isinf(gd[:, 100:]).all()

@@ -79,3 +79,57 @@ def test_autotune_keeps_results_exact(orc):
     gi, gd, gc = idx.search_batch(q, 10)
     ri, rd, rc = orc.bf_search(base, q, 10, orc.DOT)
     assert np.array_equal(gi, ri) and np.array_equal(gd, rd)
+
+
+def _tied_base(n, d, seed):
+    # small integers in few dims: many equal distances
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, 4, (n, d)).astype(np.float32)
+
+
+@pytest.mark.parametrize("mode,mt", [(hora_amd.HM_SEARCH_EXACT, Metric.Manhattan),
+                                     (hora_amd.HM_SEARCH_BF16_RERANK, Metric.Euclidean)])
+def test_ragged_tied_rows_use_their_own_k(orc, mode, mt):
+    # a tied row's result depends on k (the heap's capacity), so batches
+    # with k 1, 10 and 100 over the same tied data must each match the
+    # oracle run with that batch's k
+    base = _tied_base(6_000, 6, seed=91)
+    q = _tied_base(300, 6, seed=92)
+    idx = BruteForceIndex(6)
+    idx.add_batch(base)
+    idx.build(mt)
+    idx.set_search_mode(mode, 0)
+    sizes, ks = [100, 100, 100], [100, 1, 10]
+    gi, gd, gc, off = _ragged(idx, q, sizes, ks)
+    _check(orc, base, q, sizes, ks, mt, gi, gd, gc, off)
+
+
+@pytest.mark.parametrize("mode", [hora_amd.HM_SEARCH_EXACT, hora_amd.HM_SEARCH_BF16_RERANK])
+def test_ragged_errors_are_statuses(mode):
+    from hora_amd import HoraError
+
+    base = bf_normal(2_000, 16, seed=93)
+    idx = BruteForceIndex(16)
+    idx.add_batch(base)
+    idx.build(Metric.Euclidean)
+    idx.set_search_mode(mode, 0)
+    q = bf_normal(40, 16, seed=94)
+    q[27, 5] = np.nan
+    with pytest.raises(HoraError) as e:
+        _ragged(idx, q, [20, 20], [10, 1])
+    assert e.value.status == 3
+    with pytest.raises(HoraError) as e:  # the wrapper checks the offsets' length
+        idx.search_ragged_device(1, [0, 20], [10, 1], 1, 1, 1)
+    assert e.value.status == 1
+    with pytest.raises(HoraError) as e:  # batch_offsets[0] must be 0
+        idx.search_ragged_device(1, [5, 20], [10], 1, 1, 1)
+    assert e.value.status == 1
+    cos = BruteForceIndex(16)
+    zb = base.copy()
+    zb[11] = 0.0  # a zero-norm base row: the Cosine distance is NaN
+    cos.add_batch(zb)
+    cos.build(Metric.CosineSimilarity)
+    cos.set_search_mode(mode, 0)
+    with pytest.raises(HoraError) as e:
+        _ragged(cos, bf_normal(30, 16, seed=95), [10, 20], [5, 3])
+    assert e.value.status == 8
