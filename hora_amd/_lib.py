@@ -76,6 +76,7 @@ def lib():
     L.hm_index_built.argtypes = [vp, C.POINTER(i32)]
     L.hm_index_search.argtypes = [vp, vp, u64, u32, vp, vp, vp]
     L.hm_index_search_device.argtypes = [vp, vp, u64, u32, vp, vp, vp, vp]
+    L.hm_index_search_device_noalloc.argtypes = [vp, vp, u64, u32, vp, vp, vp, vp]
     L.hm_index_set_search_mode.argtypes = [vp, i32, u32]
     L.hm_index_search_ragged_device.argtypes = [vp, vp, u32, vp, vp, vp, vp, vp, vp]
     L.hm_index_autotune.argtypes = [vp, vp, u64, u32, C.POINTER(u32)]

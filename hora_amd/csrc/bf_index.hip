@@ -78,6 +78,7 @@ class BruteForce final : public Index {
         bnorm_valid_ = false;
         sl_valid_ = false;
         samp_ns_ = 0;
+        prepare_norms();
     }
 
     void add_device(const float *d_rows, uint64_t n_rows, uint64_t ld) override
@@ -96,6 +97,7 @@ class BruteForce final : public Index {
         bnorm_valid_ = false;
         sl_valid_ = false;
         samp_ns_ = 0;
+        prepare_norms();
     }
 
     int32_t metric() const override { return metric_; }
@@ -115,6 +117,16 @@ class BruteForce final : public Index {
         if (metric < HM_METRIC_UNKNOWN || metric > HM_METRIC_ANGULAR)
             fail(HM_ERR_INVALID_ARG, "metric out of range");
         metric_ = metric;  // bruteforce_idx.rs:36-39 stores mt and nothing else
+        prepare_norms();
+    }
+
+    // Cosine / Angular read the base rows' self-dots; computing them here
+    // (and after every add) keeps the no-allocation search free of it.
+    void prepare_norms()
+    {
+        if (!need_norm() || n_ == 0 || bnorm_valid_) return;
+        ensure_bnorm(stream_);
+        HM_HIP(hipStreamSynchronize(stream_));
     }
 
     bool built() const override { return true; }  // bruteforce_idx.rs:44-46
@@ -283,6 +295,71 @@ class BruteForce final : public Index {
                            k, d_idx, d_dist, d_count, b.err, s);
     }
 
+  public:
+    // search_device in the exact form with every working buffer carved
+    // from `scratch` (no allocation; see the header of noalloc.hip).
+    void search_device_scratch(const float *dq, uint64_t nq, uint32_t k, uint64_t *d_idx,
+                               float *d_dist, uint32_t *d_count, hipStream_t s, void *scratch,
+                               size_t scratch_bytes) override
+    {
+        if (!s) s = stream_;
+        if (metric_ == HM_METRIC_UNKNOWN) fail(HM_ERR_UNKNOWN_METRIC, "unknown method");
+        if (nq == 0) return;
+        if (!dq || !d_count || (k && (!d_idx || !d_dist)))
+            fail(HM_ERR_INVALID_ARG, "null pointer");
+        if (nq > 0x7FFFFFFFull) fail(HM_ERR_INVALID_ARG, "too many queries");
+        if (k == 0 || n_ == 0) {
+            launch_fill_results(d_idx, d_dist, d_count, nq, k, s);
+            HM_HIP(hipStreamSynchronize(s));
+            return;
+        }
+        const uint32_t n = (uint32_t)n_;
+        const uint32_t K = (uint32_t)std::min<uint64_t>((uint64_t)k + 1, n);
+        if (K > SELECT_MAX_K) fail(HM_ERR_INVALID_ARG, "k too large (max 8191)");
+        if (need_norm() && !bnorm_valid_)
+            fail(HM_ERR_INVALID_ARG, "base norms not prepared: build the index after adding rows");
+        const uint64_t ldd = round_up(n, 4);
+        // per query: padded row, self-dot, distance row, K candidates, flag;
+        // each array of the carve starts on a 256-byte boundary
+        const uint64_t per_q = (uint64_t)dpad_ * 4 + 4 + ldd * 4 + (uint64_t)K * 8 + 4;
+        const uint64_t slack = 256 * 7;
+        if (scratch_bytes < slack + per_q)
+            fail(HM_ERR_CAPACITY, "index too large for the no-allocation scratch (" +
+                                      std::to_string(scratch_bytes >> 20) + " MiB holds no " +
+                                      std::to_string(per_q) + "-byte query)");
+        const uint64_t qc = std::min<uint64_t>(nq, (scratch_bytes - slack) / per_q);
+        unsigned char *p = static_cast<unsigned char *>(scratch);
+        auto carve = [&](uint64_t bytes) {
+            void *r = p;
+            p += round_up(bytes, 256);
+            return r;
+        };
+        ExactBufs b;
+        b.err = static_cast<uint32_t *>(carve(4));
+        b.qpad = static_cast<float *>(carve(qc * dpad_ * 4));
+        b.qnorm = static_cast<float *>(carve(qc * 4));
+        b.D = static_cast<float *>(carve(qc * ldd * 4));
+        b.cand_d = static_cast<float *>(carve(qc * K * 4));
+        b.cand_i = static_cast<uint32_t *>(carve(qc * K * 4));
+        b.flags = static_cast<uint32_t *>(carve(qc * 4));
+        // the array keeps the previous call's contents: clear the error word
+        // with a kernel (no memset node, no allocation)
+        launch_fill_u32(b.err, 1, 0, s);
+        launch_check_finite(dq, dim_, nq, (uint32_t)dim_, b.err, s);
+        for (uint64_t q0 = 0; q0 < nq; q0 += qc) {
+            const uint32_t rows = (uint32_t)std::min<uint64_t>(qc, nq - q0);
+            exact_pass(dq + q0 * dim_, dim_, rows, k, K, ldd, b, d_idx + q0 * k, d_dist + q0 * k,
+                       d_count + q0, s, false);
+        }
+        uint32_t err = 0;  // pageable stack word: no pinned allocation
+        HM_HIP(hipMemcpyAsync(&err, b.err, 4, hipMemcpyDeviceToHost, s));
+        HM_HIP(hipStreamSynchronize(s));
+        if (err & 2u) fail(HM_ERR_NONFINITE_INPUT, "invalid float element");
+        if (err & 4u) fail(HM_ERR_HIP, "internal: selection consistency check failed");
+        if (err & 1u) fail(HM_ERR_NAN_DISTANCE, "NaN distance (partial_cmp of NaN)");
+    }
+
+  private:
 
     // naive_build_knn_graph (knn.rs:14-41): exact distances of a chunk of
     // rows against all rows, the row's own column excluded.

@@ -24,6 +24,17 @@ class Index {
     // Queries and outputs in device memory; outputs are [nq][k] / [nq].
     virtual void search_device(const float *d_queries, uint64_t nq, uint32_t k, uint64_t *d_idx,
                                float *d_dist, uint32_t *d_count, hipStream_t s) = 0;
+    // The exact form of search_device with every working buffer carved from
+    // the caller's scratch (scratch_bytes at a 256-byte aligned device
+    // address): no device allocation.  Queries run in chunks sized to fit.
+    virtual void search_device_scratch(const float *d_queries, uint64_t nq, uint32_t k,
+                                       uint64_t *d_idx, float *d_dist, uint32_t *d_count,
+                                       hipStream_t s, void *scratch, size_t scratch_bytes)
+    {
+        (void)d_queries, (void)nq, (void)k, (void)d_idx, (void)d_dist, (void)d_count, (void)s,
+            (void)scratch, (void)scratch_bytes;
+        fail(HM_ERR_NOT_IMPLEMENTED, "no-allocation search not supported by this index");
+    }
     virtual const char *name() const = 0;
     virtual uint64_t nodes_size() const = 0;
     // The metric of the last build (HM_METRIC_UNKNOWN before one).

@@ -379,6 +379,13 @@ class _Index:
         check(lib().hm_index_search_device(self._h, q_ptr, nq, k, idx_ptr, dist_ptr, count_ptr,
                                            stream or None))
 
+    def search_device_noalloc(self, q_ptr: int, nq: int, k: int, idx_ptr: int, dist_ptr: int,
+                              count_ptr: int, stream: int = 0) -> None:
+        """search_device in the exact form without any allocation (the
+        library's fixed per-GPU scratch array; see the C header)."""
+        check(lib().hm_index_search_device_noalloc(self._h, q_ptr, nq, k, idx_ptr, dist_ptr,
+                                                   count_ptr, stream or None))
+
 
 class BruteForceIndex(_Index):
     """bruteforce_idx.rs:15-106."""

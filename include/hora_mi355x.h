@@ -135,6 +135,26 @@ HM_API int32_t hm_index_search_device(hm_index *h, const float *d_queries, uint6
                                       uint32_t k, uint64_t *d_idx, float *d_dist,
                                       uint32_t *d_count, void *stream);
 
+/* Size of the device array that hm_index_search_device_noalloc works in. */
+#define HM_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+
+/* hm_index_search_device with no allocation of any kind: no hipMalloc,
+ * hipFree or allocating library call.  No counterpart in the reference; it
+ * is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  BruteForceIndex only.
+ * It runs the exact form whatever hm_index_set_search_mode chose (ids,
+ * distances and counts equal hm_index_search_device's in HM_SEARCH_EXACT
+ * mode) in a file-scope device array of HM_NOALLOC_SCRATCH_BYTES per GPU,
+ * queries in chunks that fit it.  HM_ERR_CAPACITY when one query's buffers
+ * (padded row, distance row of nodes_size floats, min(k+1, n) candidates)
+ * exceed it, i.e. an index of more than about 8.3 million rows.  Cosine and
+ * Angular read base norms that build and add compute.  Calls on one GPU
+ * share the array: they are serialised, and each returns after its stream
+ * has drained. */
+HM_API int32_t hm_index_search_device_noalloc(hm_index *h, const float *d_queries, uint64_t nq,
+                                              uint32_t k, uint64_t *d_idx, float *d_dist,
+                                              uint32_t *d_count, void *stream);
+
 /* A set of query batches of uneven size, each with its own k: batch b is
  * queries [batch_offsets[b], batch_offsets[b+1]) (host array, nb+1 entries)
  * searched with k = batch_k[b] (host array).  d_queries holds all queries
