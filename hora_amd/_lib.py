@@ -102,6 +102,7 @@ def lib():
                                        C.POINTER(C.c_double)]
     L.hm_ivfpq_set_probe_policy.argtypes = [vp, i32, C.c_float, u32, u32]
     L.hm_ivfpq_set_scan_form.argtypes = [vp, i32]
+    L.hm_ivfpq_set_rerank.argtypes = [vp, u32]
     L.hm_ivfpq_mean_nprobe.argtypes = [vp, C.POINTER(C.c_double)]
     L.hm_ivfpq_last_probes.argtypes = [vp, u64, u32, vp, vp]
     L.hm_mlp_loss_grad.argtypes = [u32, u32, u32, u32] + [vp] * 12

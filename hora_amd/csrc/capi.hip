@@ -503,6 +503,15 @@ HM_API int32_t hm_ivfpq_set_scan_form(hm_index *h, int32_t form)
     });
 }
 
+HM_API int32_t hm_ivfpq_set_rerank(hm_index *h, uint32_t depth)
+{
+    return guard([&] {
+        need(h, "index");
+        std::lock_guard<std::mutex> lk(h->mu);
+        ivfpq_set_rerank(h->impl.get(), depth);
+    });
+}
+
 HM_API int32_t hm_ivfpq_mean_nprobe(hm_index *h, double *out)
 {
     return guard([&] {

@@ -133,6 +133,7 @@ int32_t ivfpq_export(Index *ix, float *centers, uint64_t *offsets, uint64_t *ids
 int32_t ivfpq_set_nprobe(Index *ix, uint32_t nprobe);
 int32_t ivfpq_set_probe_policy(Index *ix, int32_t mode, float mass, uint32_t pmin, uint32_t pmax);
 int32_t ivfpq_set_scan_form(Index *ix, int32_t form);
+int32_t ivfpq_set_rerank(Index *ix, uint32_t depth);
 int32_t ivfpq_train_probe(Index *ix, const float *q, uint64_t n, uint32_t label_k,
                           uint32_t hidden, uint32_t epochs, uint32_t batch, float lr,
                           uint64_t seed, double *loss);

@@ -188,6 +188,7 @@ class IVFPQ final : public Index {
     {
         base_.release();
         n_ = cap_ = 0;
+        rr_bnorm_n_ = 0;
         trained_ = false;
         n_trained_ = 0;
     }
@@ -251,6 +252,61 @@ class IVFPQ final : public Index {
         if (nq == 0) return;
         if (!dq || !d_count || (k && (!d_idx || !d_dist)))
             fail(HM_ERR_INVALID_ARG, "null pointer");
+        if (rerank_ == 0 || k == 0 || nprobe_ == 0 || n_trained_ == 0) {
+            search_adc(dq, nq, k, d_idx, d_dist, d_count, s);
+            return;
+        }
+        // Exact rerank (no reference counterpart; the reference returns ADC
+        // distances, pq_idx.rs:181-191): the reference's ADC search for the
+        // R = max(rerank, k) nearest, then each candidate's exact distance
+        // (the reference's metric fold), sorted by (exact distance, ADC
+        // rank); the first k with their exact distances are the result.
+        const uint32_t R = std::max(rerank_, k);
+        if (dpad_ > 4096) fail(HM_ERR_INVALID_ARG, "exact rerank: dimension above 4096");
+        rr_i_.reserve(nq * R * 8);
+        rr_d_.reserve(nq * R * 4);
+        rr_c_.reserve(nq * 4);
+        search_adc(dq, nq, R, rr_i_.as<uint64_t>(), rr_d_.as<float>(), rr_c_.as<uint32_t>(), s);
+        const bool norms = metric_ == HM_METRIC_COSINE_SIMILARITY || metric_ == HM_METRIC_ANGULAR;
+        if (norms && rr_bnorm_n_ != n_) {
+            rr_bnorm_.reserve(n_ * 4);
+            launch_self_dot(base_.as<float>(), dpad_, (uint32_t)n_, (uint32_t)dim_,
+                            rr_bnorm_.as<float>(), s);
+            rr_bnorm_n_ = n_;
+        }
+        rr_ci_.reserve(nq * R * 4);
+        rr_qpad_.reserve(nq * dpad_ * 4);
+        rr_qn_.reserve(nq * 4);
+        rr_f_.reserve(nq * 4);
+        launch_u64_to_u32(rr_i_.as<uint64_t>(), rr_ci_.as<uint32_t>(), nq * R, s);
+        launch_pad_rows(dq, dim_, rr_qpad_.as<float>(), dpad_, nq, (uint32_t)dim_, dpad_, s);
+        launch_self_dot(rr_qpad_.as<float>(), dpad_, (uint32_t)nq, (uint32_t)dim_,
+                        rr_qn_.as<float>(), s);
+        launch_rerank(metric_, rr_qpad_.as<float>(), dpad_, base_.as<float>(),
+                      rr_ci_.as<uint32_t>(), rr_d_.as<float>(), R, R, rr_qn_.as<float>(),
+                      rr_bnorm_.as<float>(), (uint32_t)nq, (uint32_t)n_, k, 0.0f, 0.0f, 0.0f, 0.0f,
+                      nullptr, 0, d_idx, d_dist, d_count, rr_f_.as<uint32_t>(), s, nullptr,
+                      nullptr, nullptr, rr_c_.as<uint32_t>());
+        std::vector<uint32_t> fl(nq);
+        HM_HIP(hipMemcpyAsync(fl.data(), rr_f_.get(), nq * 4, hipMemcpyDeviceToHost, s));
+        HM_HIP(hipStreamSynchronize(s));
+        for (uint64_t q = 0; q < nq; ++q) {
+            if (fl[q] & ROW_BAD) fail(HM_ERR_HIP, "internal: rerank candidate out of range");
+            if (fl[q] & ROW_NAN) fail(HM_ERR_NAN_DISTANCE, "NaN distance (partial_cmp of NaN)");
+        }
+    }
+
+    // The top-R depth of the exact rerank (0: the reference's ADC result).
+    void set_rerank(uint32_t r)
+    {
+        if (r > 4095) fail(HM_ERR_INVALID_ARG, "rerank depth above 4095");
+        rerank_ = r;
+    }
+
+    // IVFPQIndex::search_knn_adc + node_search_k (pq_idx.rs:408-477).
+    void search_adc(const float *dq, uint64_t nq, uint32_t k, uint64_t *d_idx, float *d_dist,
+                    uint32_t *d_count, hipStream_t s)
+    {
         // pq_idx.rs:424-425: popping more centres than exist unwraps None
         if (nprobe_ > nlist_) fail(HM_ERR_INVALID_ARG, "search_n_center larger than n_kmeans_center");
         if (k == 0 || nprobe_ == 0 || n_trained_ == 0) {
@@ -668,6 +724,10 @@ class IVFPQ final : public Index {
     int scan_mode_ = 0;
     DeviceBuffer Dr_, seg_, row_n_, rcand_d_, rcand_i_, rflags_;  // row-form scan
     DeviceBuffer rp_rows_, rp_f_, rp_u_, rp_order_, rp_np_, rp_seq_, rp_scnt_;
+    // exact rerank of the ADC top-R
+    uint32_t rerank_ = 0;
+    uint64_t rr_bnorm_n_ = 0;
+    DeviceBuffer rr_i_, rr_d_, rr_c_, rr_ci_, rr_qpad_, rr_qn_, rr_f_, rr_bnorm_;
     // learned probe
     std::unique_ptr<ProbeMLP> mlp_;
     int32_t probe_mode_ = 0;
@@ -716,6 +776,14 @@ int32_t ivfpq_export(Index *ix, float *centers, uint64_t *offsets, uint64_t *ids
     auto *p = dynamic_cast<IVFPQ *>(ix);
     if (!p) fail(HM_ERR_INVALID_ARG, "not an IVFPQIndex");
     p->export_trained(centers, offsets, ids, codebooks, codes);
+    return HM_OK;
+}
+
+int32_t ivfpq_set_rerank(Index *ix, uint32_t depth)
+{
+    auto *p = dynamic_cast<IVFPQ *>(ix);
+    if (!p) fail(HM_ERR_INVALID_ARG, "not an IVFPQIndex");
+    p->set_rerank(depth);
     return HM_OK;
 }
 

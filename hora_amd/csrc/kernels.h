@@ -178,6 +178,8 @@ void launch_scatter_ragged(const uint64_t *si, const float *sd, const uint32_t *
                            const uint32_t *rows, uint32_t nrows, uint32_t kmax, const uint32_t *kq,
                            const uint64_t *out_off, uint64_t *di, float *dd, uint32_t *dc,
                            hipStream_t s);
+// dst[i] = (uint32_t)src[i] (row indices below 2^32).
+void launch_u64_to_u32(const uint64_t *src, uint32_t *dst, uint64_t n, hipStream_t s);
 // Fill kernel (graph-safe replacement of hipMemsetAsync).
 void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s);
 void launch_rerun_plan(const uint32_t *flags, uint32_t nq, uint32_t *plan, hipStream_t s);

@@ -376,6 +376,14 @@ __global__ void scatter_ragged_kernel(const uint64_t *__restrict__ si, const flo
     if (threadIdx.x == 0) dc[q] = sc[r] < kr ? sc[r] : kr;
 }
 
+__global__ void u64_to_u32_kernel(const uint64_t *__restrict__ src, uint32_t *__restrict__ dst,
+                                  uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = (uint32_t)src[i];
+}
+
 __global__ void fill_u32_kernel(uint32_t *p, uint32_t n, uint32_t v)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -426,6 +434,7 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
         c_in_inexact, q_inexact, certify, out_idx, out_dist, out_count, flags, dev_rows, kq,    \
         out_off, s_q, key_s_q, pre_flags)
     switch (metric) {
+    case HM_METRIC_MANHATTAN: HM_RR(ACC_MANHATTAN, EPI_NONE); break;  // no certification bound
     case HM_METRIC_EUCLIDEAN: HM_RR(ACC_EUCLID, EPI_NONE); break;
     case HM_METRIC_DOT_PRODUCT: HM_RR(ACC_DOT, EPI_NONE); break;
     case HM_METRIC_COSINE_SIMILARITY: HM_RR(ACC_DOT, EPI_COSINE); break;
@@ -433,6 +442,14 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
     default: fail(HM_ERR_INVALID_ARG, "rerank: metric has no shortlist form");
     }
 #undef HM_RR
+    HM_CHECK_LAUNCH();
+}
+
+void launch_u64_to_u32(const uint64_t *src, uint32_t *dst, uint64_t n, hipStream_t s)
+{
+    if (n == 0) return;
+    const unsigned blocks = (unsigned)std::min<uint64_t>((n + 255) / 256, 4096);
+    u64_to_u32_kernel<<<blocks, 256, 0, s>>>(src, dst, n);
     HM_CHECK_LAUNCH();
 }
 

@@ -8,7 +8,10 @@
   gaussian_mixture -- centres ~ Normal(0, I); each point picks a centre
                     uniformly and adds Normal(0, sigma^2) noise (isotropic,
                     or per-centre diagonal sigma_j ~ U(lo, hi) when
-                    `aniso=(lo, hi)`).
+                    `aniso=(lo, hi)`).  The per-centre sigmas are drawn from
+                    a generator seeded by the centres themselves, so points
+                    drawn in separate calls from the same centres (a base
+                    and its queries) share them.
   unit_normal    -- Normal(0, 1) rows L2-normalised.
 """
 from __future__ import annotations
@@ -43,10 +46,19 @@ def gaussian_mixture(n: int, d: int, n_centres: int, seed: int, sigma: float = 0
     if aniso is None:
         noise *= np.float32(sigma)
     else:
-        sig = r.uniform(aniso[0], aniso[1], size=(n_centres, d)).astype(np.float32)
+        sig = mixture_sigmas(centres, aniso)
         noise *= sig[which]
     x = centres[which] + noise
     return x.astype(np.float32), centres, which
+
+
+def mixture_sigmas(centres: np.ndarray, aniso: tuple) -> np.ndarray:
+    """Per-centre diagonal sigmas ~ U(lo, hi), a function of the centres."""
+    import hashlib
+
+    c = np.ascontiguousarray(centres, dtype=np.float32)
+    seed = int.from_bytes(hashlib.sha256(c.tobytes()).digest()[:8], "little")
+    return _rng(seed).uniform(aniso[0], aniso[1], size=c.shape).astype(np.float32)
 
 
 def unit_normal(n: int, d: int, seed: int) -> np.ndarray:

@@ -481,6 +481,11 @@ class IVFPQIndex(_Index):
         "per_probe", "lists" or "rows"."""
         check(lib().hm_ivfpq_set_scan_form(self._h, self.SCAN_FORMS[form]))
 
+    def set_rerank(self, depth: int) -> None:
+        """Exact rerank of the ADC top-max(depth, k) (0: the reference's ADC
+        result; see the C header)."""
+        check(lib().hm_ivfpq_set_rerank(self._h, depth))
+
     def set_probe_policy(self, mode: int, mass: float = 0.9, pmin: int = 1,
                          pmax: int = 64) -> None:
         """0: fixed search_n_center (reference); 1: learned per-query nprobe."""

@@ -304,6 +304,15 @@ HM_API int32_t hm_ivfpq_set_probe_policy(hm_index *h, int32_t mode, float mass, 
  * tuning and for testing every form. */
 HM_API int32_t hm_ivfpq_set_scan_form(hm_index *h, int32_t form);
 
+/* Exact rerank of the IVFPQ result (0, the default, keeps the reference's
+ * ADC result, pq_idx.rs:181-191, 429-436): the reference's ADC search for
+ * the R = max(depth, k) nearest, each candidate rescored with the exact
+ * metric (the reference's f32 fold, metrics.rs:22-34), sorted by (exact
+ * distance, ADC rank); the first k come back with their exact distances.
+ * No counterpart in the reference: it wins back the accuracy the PQ codes
+ * lose.  depth <= 4095; nprobe * (R + 1) <= 16384. */
+HM_API int32_t hm_ivfpq_set_rerank(hm_index *h, uint32_t depth);
+
 /* Mean number of lists probed per query by the last search. */
 HM_API int32_t hm_ivfpq_mean_nprobe(hm_index *h, double *out);
 

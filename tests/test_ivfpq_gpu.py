@@ -220,3 +220,50 @@ def test_errors():
     with pytest.raises(HoraError) as e:
         IVFPQIndex(8, IVFPQParams().n_sub(2).sub_bits(9))
     assert e.value.status == 10
+
+
+def _rerank_oracle(orc, x, q, mt, ex, nprobe, k, depth):
+    """The exact rerank: the reference's ADC search for R = max(depth, k),
+    each candidate's exact distance by the oracle metric, a stable sort by
+    that distance (equal distances keep the ADC rank), the first k."""
+    R = max(depth, k)
+    ri, rd, rc, _ = orc.ivfpq_search(q, int(mt), ex["centers"], ex["codebooks"],
+                                     ex["list_offsets"], ex["list_ids"], ex["codes"], nprobe, R)
+    ids, dist, cnt = [], [], []
+    for r in range(q.shape[0]):
+        cand = ri[r, :rc[r]].astype(np.int64)
+        ed = np.array([orc.metric(q[r], x[c], int(mt)) for c in cand], np.float32)
+        order = np.argsort(ed, kind="stable")[:k]
+        ids.append(cand[order])
+        dist.append(ed[order])
+        cnt.append(min(k, int(rc[r])))
+    return ids, dist, np.array(cnt, np.uint32)
+
+
+@pytest.mark.parametrize("mt,tied", [(Metric.Euclidean, False), (Metric.Euclidean, True),
+                                     (Metric.CosineSimilarity, False), (Metric.Manhattan, True),
+                                     (Metric.DotProduct, False)])
+def test_exact_rerank_matches_oracle(orc, mt, tied):
+    x, cen, _ = gaussian_mixture(6000, 24, 32, seed=61, sigma=0.4)
+    if tied:  # duplicated rows: equal ADC and equal exact distances
+        x = np.round(np.repeat(x[:3000], 2, axis=0) * 2.0).astype(np.float32)
+    idx = _build(x, mt, 32, 6, 8, 4, 62, fp64=True, nprobe=4)
+    q, _, _ = gaussian_mixture(60, 24, 32, seed=63, sigma=0.4, centres=cen)
+    if tied:
+        q = np.round(q * 2.0).astype(np.float32)
+    ex = idx.export()
+    for depth, k, nprobe in ((50, 10, 4), (10, 10, 2), (200, 25, 6), (5, 1, 1)):
+        idx.set_search_n_center(nprobe)
+        idx.set_rerank(depth)
+        gi, gd, gc = idx.search_batch(q, k)
+        ri, rd, rc = _rerank_oracle(orc, x, q, mt, ex, nprobe, k, depth)
+        assert np.array_equal(gc, rc), (depth, k, nprobe)
+        for r in range(q.shape[0]):
+            assert np.array_equal(gi[r, :gc[r]], ri[r]), (depth, k, nprobe, r)
+            assert np.array_equal(gd[r, :gc[r]].view(np.uint32), rd[r].view(np.uint32))
+    idx.set_rerank(0)  # back to the reference's ADC result
+    ai, ad, ac = idx.search_batch(q, 10)
+    ri, rd, rc, _ = orc.ivfpq_search(q, int(mt), ex["centers"], ex["codebooks"],
+                                     ex["list_offsets"], ex["list_ids"], ex["codes"], 1, 10)
+    assert np.array_equal(ac, rc)
+    _same_valid(ai, ad, ri, rd, rc)
