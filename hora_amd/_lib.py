@@ -108,6 +108,7 @@ def lib():
     L.hm_mlp_loss_grad.argtypes = [u32, u32, u32, u32] + [vp] * 12
     L.hm_profile_enable.argtypes = [i32]
     L.hm_profile_reset.argtypes = []
+    L.hm_profile_only.argtypes = [C.c_char_p]
     L.hm_profile_query.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)]
     L.hm_last_error.argtypes = []
     L.hm_last_error.restype = C.c_char_p
@@ -127,6 +128,11 @@ def check(status: int) -> None:
 
 def profile_enable(on: bool = True) -> None:
     check(lib().hm_profile_enable(1 if on else 0))
+
+
+def profile_only(kernel=None) -> None:
+    """Time only this kernel family (None: every family)."""
+    check(lib().hm_profile_only(kernel.encode() if kernel else None))
 
 
 def profile_reset() -> None:

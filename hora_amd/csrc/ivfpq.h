@@ -70,6 +70,19 @@ void launch_ivf_adc_rows(const float *Qp, uint32_t dpad, uint32_t dim, const flo
                          hipStream_t s);
 // The row select's candidates (columns) -> ids, counts and ROW_TIE flags,
 // as launch_ivf_merge.
+// Narrows the coarse select's ROW_TIE to ties that can change the result
+// (ties among empty lists cannot) and marks edge ties ROW_TIE_EDGE.
+void launch_coarse_ties(const float *D, uint64_t ldd, uint32_t nlist, const float *pd,
+                        const uint32_t *pi, uint32_t Kc, uint32_t nprobe, const uint32_t *off,
+                        uint32_t *flags, uint32_t rows, hipStream_t s);
+// Row form: makes the selected first k the k smallest by (distance, row
+// index) when the k-th and (k+1)-th are equal (the select broke ties by
+// column); cand_i holds columns, rows of row_n[q] values in Dr.
+void launch_ivf_edge_fixup(const float *Dr, uint64_t ldr, const uint32_t *row_n,
+                           const float *cand_d, uint32_t *cand_i, uint32_t ldc, uint32_t k,
+                           const uint32_t *probes, uint32_t ldp, uint32_t nprobe,
+                           const uint32_t *nprobe_q, const uint32_t *seg, const uint32_t *off,
+                           const uint32_t *ids, uint32_t rows, hipStream_t s);
 void launch_ivf_rows_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ldc,
                               const uint32_t *sel_flags, const uint32_t *coarse_flags,
                               const uint32_t *probes, uint32_t ldp, uint32_t nprobe,
@@ -105,7 +118,7 @@ bool launch_ivf_replay_lds(const uint32_t *rows, uint32_t nrows, const float *Dc
 // Learned-probe helpers.
 void launch_negate(const float *x, float *y, uint64_t n, hipStream_t s);
 void launch_probe_count(const float *negp, uint32_t ld, uint32_t rows, uint32_t pmax, float mass,
-                        uint32_t pmin, uint32_t *nprobe_q, hipStream_t s);
+                        uint32_t pmin, uint32_t *nprobe_q, hipStream_t s, const float *lse = nullptr);
 void launch_dense_targets(const uint32_t *lab, uint32_t k, uint32_t rows, uint32_t nlist,
                           float *T, hipStream_t s);
 void launch_ids_to_lists(const uint64_t *ids, uint64_t n, const uint32_t *assign, uint64_t n_items,

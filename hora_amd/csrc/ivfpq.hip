@@ -811,6 +811,99 @@ __global__ __launch_bounds__(LM_THREADS, 2) void ivf_adc_rows_kernel(
 // (columns, ascending by (distance, column)): the first min(k, row_n) as
 // item ids, ROW_TIE when the K are not pairwise distinct (the select's flag)
 // or the coarse probe set was tied.  One wave per query.
+// Canonical candidate set of the row form (the exact-rerank form's rule:
+// the k smallest ADC distances, ties broken by row index).  The radix
+// select broke ties by column (probe rank, then list position); when the
+// k-th and (k+1)-th selected distances are equal, the entries of that
+// distance in the set are replaced by the ones with the smallest indices:
+// a bisection on the index threshold T with need = k - (entries below the
+// distance) finds the smallest T such that `need` entries of the row with
+// that distance have index <= T, and those entries fill the set's tail.
+// One workgroup per row; rows without such a tie return at once.
+__global__ __launch_bounds__(256) void ivf_edge_fixup_kernel(
+    const float *__restrict__ Dr, uint64_t ldr, const uint32_t *__restrict__ row_n,
+    const float *__restrict__ cand_d, uint32_t *__restrict__ cand_i, uint32_t ldc, uint32_t k,
+    const uint32_t *__restrict__ probes, uint32_t ldp, uint32_t nprobe,
+    const uint32_t *__restrict__ nprobe_q, const uint32_t *__restrict__ seg,
+    const uint32_t *__restrict__ off, const uint32_t *__restrict__ ids)
+{
+    const uint32_t q = blockIdx.x, tid = threadIdx.x;
+    const uint32_t n = row_n[q];
+    const float *cd = cand_d + (size_t)q * ldc;
+    if (n <= k || k == 0 || !(cd[k - 1] == cd[k])) return;
+    const float v = cd[k - 1];
+    __shared__ uint32_t s_cnt, s_below;
+    if (tid == 0) {
+        s_cnt = 0;
+        s_below = 0;
+    }
+    __syncthreads();
+    uint32_t below = 0;
+    for (uint32_t j = tid; j < k; j += 256) below += cd[j] < v ? 1u : 0u;
+    atomicAdd(&s_below, below);
+    const uint32_t np_q = nprobe_q ? nprobe_q[q] : nprobe;
+    const uint32_t *sg = seg + (size_t)q * nprobe;
+    const float *row = Dr + (size_t)q * ldr;
+    auto id_of = [&](uint32_t col) -> uint32_t {
+        uint32_t a = 0, b = np_q;  // the last probe p with sg[p] <= col
+        while (b - a > 1) {
+            const uint32_t mid = (a + b) / 2;
+            if (sg[mid] <= col) a = mid;
+            else b = mid;
+        }
+        const uint32_t l = probes[(size_t)q * ldp + a];
+        return ids[off[l] + (col - sg[a])];
+    };
+    // the entries of that distance, with their indices, while they fit in
+    // LDS: each takes its rank by index among them
+    constexpr uint32_t CAP = 1024;
+    __shared__ uint32_t s_col[CAP], s_id[CAP], s_m;
+    if (tid == 0) s_m = 0;
+    __syncthreads();
+    for (uint32_t col = tid; col < n; col += 256)
+        if (row[col] == v) {
+            const uint32_t j = atomicAdd(&s_m, 1u);
+            if (j < CAP) {
+                s_col[j] = col;
+                s_id[j] = id_of(col);
+            }
+        }
+    __syncthreads();
+    const uint32_t need = k - s_below, mcount = s_m;
+    uint32_t *ci = cand_i + (size_t)q * ldc;
+    if (mcount <= CAP) {
+        for (uint32_t j = tid; j < mcount; j += 256) {
+            uint32_t rank = 0;
+            for (uint32_t t = 0; t < mcount; ++t) rank += s_id[t] < s_id[j] ? 1u : 0u;
+            if (rank < need) ci[s_below + rank] = s_col[j];
+        }
+        return;
+    }
+    // more than CAP entries share the distance: bisection on the index
+    uint32_t lo = 0, hi = 0xFFFFFFFFu;
+    while (lo < hi) {  // uniform: every thread sees the same counts
+        const uint32_t mid = lo + (hi - lo) / 2;
+        uint32_t c = 0;
+        for (uint32_t col = tid; col < n; col += 256)
+            if (row[col] == v && id_of(col) <= mid) ++c;
+        if (tid == 0) s_cnt = 0;
+        __syncthreads();
+        atomicAdd(&s_cnt, c);
+        __syncthreads();
+        const uint32_t total = s_cnt;
+        __syncthreads();
+        if (total >= need) hi = mid;
+        else lo = mid + 1;
+    }
+    if (tid == 0) s_cnt = 0;
+    __syncthreads();
+    for (uint32_t col = tid; col < n; col += 256)
+        if (row[col] == v && id_of(col) <= lo) {
+            const uint32_t slot = atomicAdd(&s_cnt, 1u);
+            if (slot < need) ci[s_below + slot] = col;
+        }
+}
+
 __global__ __launch_bounds__(64) void ivf_rows_finalize_kernel(
     const float *__restrict__ cand_d, const uint32_t *__restrict__ cand_i, uint32_t ldc,
     const uint32_t *__restrict__ sel_flags, const uint32_t *__restrict__ coarse_flags,
@@ -845,8 +938,60 @@ __global__ __launch_bounds__(64) void ivf_rows_finalize_kernel(
     if (tid == 0) {
         out_count[q] = m;
         const uint32_t f = sel_flags[q];
-        flags[q] = ((f & (ROW_TIE | ROW_BAD)) || (coarse_flags[q] & ROW_TIE)) ? ROW_TIE : 0u;
+        const uint32_t cf = coarse_flags[q];
+        // the set of the first k is tie-order dependent only at its edge
+        const bool edge = n > k && cand_d[(size_t)q * ldc + k - 1] == cand_d[(size_t)q * ldc + k];
+        flags[q] = (((f & (ROW_TIE | ROW_BAD)) || (cf & ROW_TIE)) ? ROW_TIE : 0u) |
+                   ((edge || (cf & ROW_TIE_EDGE) || (f & ROW_BAD)) ? ROW_TIE_EDGE : 0u);
     }
+}
+
+// Which ties among a query's coarse distances can change its search
+// result (pq_idx.rs:413-426 pops the coarse heap nprobe times; a list's
+// place in the probe order matters only through the items it holds, so an
+// empty list is interchangeable with any other empty list).  From the
+// Kc = nprobe + 1 smallest (distance, list) pairs and the full distance row:
+//  * a run of equal distances wholly inside the first nprobe: the order of
+//    its non-empty lists is heap-dependent -> ROW_TIE when it holds >= 2;
+//  * the run at the edge (the nprobe-th and (nprobe+1)-th are equal): which
+//    of its lists are probed is heap-dependent -> ROW_TIE | ROW_TIE_EDGE
+//    when any list of that distance in the whole row is non-empty.
+// A NaN keeps its flag.  One wave per query.
+__global__ __launch_bounds__(64) void coarse_ties_kernel(
+    const float *__restrict__ D, uint64_t ldd, uint32_t nlist, const float *__restrict__ pd,
+    const uint32_t *__restrict__ pi, uint32_t Kc, uint32_t nprobe,
+    const uint32_t *__restrict__ off, uint32_t *__restrict__ flags)
+{
+    const uint32_t q = blockIdx.x, lane = threadIdx.x;
+    const uint32_t f = flags[q];
+    if (!(f & ROW_TIE) || (f & ROW_NAN)) return;  // no tie among the Kc, or a NaN row
+    const float *d = pd + (size_t)q * Kc;
+    const uint32_t *li = pi + (size_t)q * Kc;
+    const uint32_t np = nprobe < Kc ? nprobe : Kc;
+    auto nonempty = [&](uint32_t l) { return off[l + 1] > off[l]; };
+    uint32_t out = 0;
+    // runs inside the first np (each lane walks the runs that start at its
+    // positions; Kc is small)
+    for (uint32_t a = lane; a < np; a += 64) {
+        if (a > 0 && d[a - 1] == d[a]) continue;  // not a run start
+        uint32_t b = a + 1, ne = nonempty(li[a]) ? 1u : 0u;
+        while (b < Kc && d[b] == d[a]) {
+            ne += nonempty(li[b]) ? 1u : 0u;
+            ++b;
+        }
+        if (b <= np && ne >= 2) out |= ROW_TIE;
+    }
+    // the edge run: every list of the row at that distance
+    if (np < Kc && np > 0 && d[np - 1] == d[np]) {
+        const float v = d[np];
+        const float *row = D + (size_t)q * ldd;
+        bool any = false;
+        for (uint32_t l = lane; l < nlist; l += 64) any |= row[l] == v && nonempty(l);
+        if (__ballot(any)) out |= ROW_TIE | ROW_TIE_EDGE;
+    }
+    // combine the lanes' findings
+    const uint64_t t = __ballot((out & ROW_TIE) != 0), e = __ballot((out & ROW_TIE_EDGE) != 0);
+    if (lane == 0) flags[q] = (f & ~ROW_TIE) | (t ? ROW_TIE : 0u) | (e ? ROW_TIE_EDGE : 0u);
 }
 
 // Per query: the nprobe*k1 candidates -> first k ascending; ROW_TIE when
@@ -909,7 +1054,10 @@ __global__ __launch_bounds__(256) void ivf_merge_kernel(
     __syncthreads();
     if (tid == 0) {
         out_count[q] = m;
-        flags[q] = (s_tie || (coarse_flags[q] & ROW_TIE)) ? ROW_TIE : 0u;
+        const uint32_t cf = coarse_flags[q];
+        const bool edge = nv > k && (keys[k - 1] >> 32) == (keys[k] >> 32);
+        flags[q] = ((s_tie || (cf & ROW_TIE)) ? ROW_TIE : 0u) |
+                   ((edge || (cf & ROW_TIE_EDGE)) ? ROW_TIE_EDGE : 0u);
     }
 }
 
@@ -1266,14 +1414,16 @@ __global__ void negate_kernel(const float *__restrict__ x, float *__restrict__ y
 // clamped to [pmin, pmax]; the candidates hold -p sorted ascending.
 __global__ void probe_count_kernel(const float *__restrict__ negp, uint32_t ld, uint32_t rows,
                                    uint32_t pmax, float mass, uint32_t pmin,
-                                   uint32_t *__restrict__ nprobe_q)
+                                   uint32_t *__restrict__ nprobe_q, const float *__restrict__ lse)
 {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= rows) return;
     float acc = 0.f;
     uint32_t m = pmax;
     for (uint32_t j = 0; j < pmax; ++j) {
-        acc += -negp[(size_t)q * ld + j];
+        // -p, or the negated logit z with p = exp(z - lse)
+        const float v = negp[(size_t)q * ld + j];
+        acc += lse ? expf(-v - lse[q]) : -v;
         if (acc >= mass) {
             m = j + 1;
             break;
@@ -1529,6 +1679,28 @@ void launch_ivf_adc_rows(const float *Qp, uint32_t dpad, uint32_t dim, const flo
     HM_CHECK_LAUNCH();
 }
 
+void launch_ivf_edge_fixup(const float *Dr, uint64_t ldr, const uint32_t *row_n,
+                           const float *cand_d, uint32_t *cand_i, uint32_t ldc, uint32_t k,
+                           const uint32_t *probes, uint32_t ldp, uint32_t nprobe,
+                           const uint32_t *nprobe_q, const uint32_t *seg, const uint32_t *off,
+                           const uint32_t *ids, uint32_t rows, hipStream_t s)
+{
+    if (rows == 0) return;
+    ProfScope prof("ivf_merge", s);
+    ivf_edge_fixup_kernel<<<rows, 256, 0, s>>>(Dr, ldr, row_n, cand_d, cand_i, ldc, k, probes, ldp,
+                                               nprobe, nprobe_q, seg, off, ids);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_coarse_ties(const float *D, uint64_t ldd, uint32_t nlist, const float *pd,
+                        const uint32_t *pi, uint32_t Kc, uint32_t nprobe, const uint32_t *off,
+                        uint32_t *flags, uint32_t rows, hipStream_t s)
+{
+    if (rows == 0) return;
+    coarse_ties_kernel<<<rows, 64, 0, s>>>(D, ldd, nlist, pd, pi, Kc, nprobe, off, flags);
+    HM_CHECK_LAUNCH();
+}
+
 void launch_ivf_rows_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ldc,
                               const uint32_t *sel_flags, const uint32_t *coarse_flags,
                               const uint32_t *probes, uint32_t ldp, uint32_t nprobe,
@@ -1647,11 +1819,11 @@ void launch_negate(const float *x, float *y, uint64_t n, hipStream_t s)
 }
 
 void launch_probe_count(const float *negp, uint32_t ld, uint32_t rows, uint32_t pmax, float mass,
-                        uint32_t pmin, uint32_t *nprobe_q, hipStream_t s)
+                        uint32_t pmin, uint32_t *nprobe_q, hipStream_t s, const float *lse)
 {
     if (rows == 0) return;
     probe_count_kernel<<<ceil_div(rows, 256), 256, 0, s>>>(negp, ld, rows, pmax, mass, pmin,
-                                                            nprobe_q);
+                                                            nprobe_q, lse);
     HM_CHECK_LAUNCH();
 }
 

@@ -257,16 +257,20 @@ class IVFPQ final : public Index {
             return;
         }
         // Exact rerank (no reference counterpart; the reference returns ADC
-        // distances, pq_idx.rs:181-191): the reference's ADC search for the
-        // R = max(rerank, k) nearest, then each candidate's exact distance
-        // (the reference's metric fold), sorted by (exact distance, ADC
-        // rank); the first k with their exact distances are the result.
+        // distances, pq_idx.rs:181-191): the nprobe lists nearest by (coarse
+        // distance, list), or the probe network's lists; among their items
+        // the R = max(rerank, k) smallest ADC distances by (distance, row
+        // index); each candidate's exact distance (the reference's metric
+        // fold), sorted by (exact distance, row index); the first k with
+        // their exact distances are the result.  Every step is a total
+        // order, so no heap is replayed.
         const uint32_t R = std::max(rerank_, k);
         if (dpad_ > 4096) fail(HM_ERR_INVALID_ARG, "exact rerank: dimension above 4096");
         rr_i_.reserve(nq * R * 8);
         rr_d_.reserve(nq * R * 4);
         rr_c_.reserve(nq * 4);
-        search_adc(dq, nq, R, rr_i_.as<uint64_t>(), rr_d_.as<float>(), rr_c_.as<uint32_t>(), s);
+        search_adc(dq, nq, R, rr_i_.as<uint64_t>(), rr_d_.as<float>(), rr_c_.as<uint32_t>(), s,
+                   true);
         const bool norms = metric_ == HM_METRIC_COSINE_SIMILARITY || metric_ == HM_METRIC_ANGULAR;
         if (norms && rr_bnorm_n_ != n_) {
             rr_bnorm_.reserve(n_ * 4);
@@ -286,7 +290,7 @@ class IVFPQ final : public Index {
                       rr_ci_.as<uint32_t>(), rr_d_.as<float>(), R, R, rr_qn_.as<float>(),
                       rr_bnorm_.as<float>(), (uint32_t)nq, (uint32_t)n_, k, 0.0f, 0.0f, 0.0f, 0.0f,
                       nullptr, 0, d_idx, d_dist, d_count, rr_f_.as<uint32_t>(), s, nullptr,
-                      nullptr, nullptr, rr_c_.as<uint32_t>());
+                      nullptr, nullptr, rr_c_.as<uint32_t>(), nullptr, nullptr, true);
         std::vector<uint32_t> fl(nq);
         HM_HIP(hipMemcpyAsync(fl.data(), rr_f_.get(), nq * 4, hipMemcpyDeviceToHost, s));
         HM_HIP(hipStreamSynchronize(s));
@@ -304,8 +308,13 @@ class IVFPQ final : public Index {
     }
 
     // IVFPQIndex::search_knn_adc + node_search_k (pq_idx.rs:408-477).
+    // canonical: no heap replay; the probed lists are the nprobe nearest by
+    // (distance, list) and the result is the set of the k smallest ADC
+    // distances by (distance, row index), in no particular order among
+    // equal distances (the exact rerank re-sorts it).  Otherwise the
+    // reference's search (pq_idx.rs:408-477), tied rows replayed.
     void search_adc(const float *dq, uint64_t nq, uint32_t k, uint64_t *d_idx, float *d_dist,
-                    uint32_t *d_count, hipStream_t s)
+                    uint32_t *d_count, hipStream_t s, bool canonical = false)
     {
         // pq_idx.rs:424-425: popping more centres than exist unwraps None
         if (nprobe_ > nlist_) fail(HM_ERR_INVALID_ARG, "search_n_center larger than n_kmeans_center");
@@ -367,9 +376,8 @@ class IVFPQ final : public Index {
             pairs_.reserve(qc * np * 4);
         }
         if (learned) {
-            xq16_.reserve(qc * dim_ * 2);
-            Zq_.reserve(qc * nlist_ * 4);
-            Pq_.reserve(qc * nlist_ * 4);
+            xq16_.reserve(qc * (size_t)mlp_->din_padded() * 2);
+            Pq_.reserve(qc * 4);  // the rows' log-partition of the logits
             nprobe_q_.reserve(qc * 4);
         }
         std::vector<uint32_t> fl(qc);
@@ -381,17 +389,18 @@ class IVFPQ final : public Index {
             launch_pad_rows(dq + q0 * dim_, dim_, qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
                             dpad_, s);
             if (learned) {
-                // scores of every list from the probe network, then the
-                // smallest prefix of the best lists holding `mass` of them
-                f32_to_bf16(dq + q0 * dim_, xq16_.as<uint16_t>(), (long)rows * dim_, s);
-                mlp_->forward(xq16_.as<uint16_t>(), rows, Zq_.as<float>());
-                softmax_xent(Zq_.as<float>(), nullptr, rows, nlist_, 1.0f, nullptr, nullptr,
-                             nullptr, Pq_.as<float>(), s);
-                launch_negate(Pq_.as<float>(), D_.as<float>(), (uint64_t)rows * nlist_, s);
-                launch_row_select(D_.as<float>(), nlist_, rows, nlist_, Kc, probe_d_.as<float>(),
+                // scores of every list from the probe network (negated
+                // logits, by (score, list) in the select), then the smallest
+                // prefix of the best lists holding `mass` of the softmax
+                const int dinp = mlp_->din_padded();
+                rows_to_bf16(dq + q0 * dim_, (long)dim_, rows, (int)dim_, xq16_.as<uint16_t>(),
+                             dinp, s);
+                mlp_->scores(xq16_.as<uint16_t>(), rows, D_.as<float>(), (long)ldd,
+                             Pq_.as<float>());
+                launch_row_select(D_.as<float>(), ldd, rows, nlist_, Kc, probe_d_.as<float>(),
                                   probe_i_.as<uint32_t>(), Kc, cflags_.as<uint32_t>(), s);
                 launch_probe_count(probe_d_.as<float>(), Kc, rows, pmax_, mass_, pmin_,
-                                   nprobe_q_.as<uint32_t>(), s);
+                                   nprobe_q_.as<uint32_t>(), s, Pq_.as<float>());
                 HM_HIP(hipMemsetAsync(cflags_.get(), 0, rows * 4, s));
             } else {
                 if (norms)
@@ -403,6 +412,9 @@ class IVFPQ final : public Index {
                                   D_.as<float>(), ldd, s);
                 launch_row_select(D_.as<float>(), ldd, rows, nlist_, Kc, probe_d_.as<float>(),
                                   probe_i_.as<uint32_t>(), Kc, cflags_.as<uint32_t>(), s);
+                launch_coarse_ties(D_.as<float>(), ldd, nlist_, probe_d_.as<float>(),
+                                   probe_i_.as<uint32_t>(), Kc, nprobe_, off_.as<uint32_t>(),
+                                   cflags_.as<uint32_t>(), rows, s);
             }
             const uint32_t *npq_dev = learned ? nprobe_q_.as<uint32_t>() : nullptr;
             if (rows_form) {
@@ -418,6 +430,12 @@ class IVFPQ final : public Index {
                 launch_row_select(Dr_.as<float>(), ldr, rows, (uint32_t)ldr, k1,
                                   rcand_d_.as<float>(), rcand_i_.as<uint32_t>(), k1,
                                   rflags_.as<uint32_t>(), s, nullptr, row_n_.as<uint32_t>());
+                if (canonical)
+                    launch_ivf_edge_fixup(Dr_.as<float>(), ldr, row_n_.as<uint32_t>(),
+                                          rcand_d_.as<float>(), rcand_i_.as<uint32_t>(), k1, k,
+                                          probe_i_.as<uint32_t>(), Kc, np, npq_dev,
+                                          seg_.as<uint32_t>(), off_.as<uint32_t>(),
+                                          ids_.as<uint32_t>(), rows, s);
                 launch_ivf_rows_finalize(rcand_d_.as<float>(), rcand_i_.as<uint32_t>(), k1,
                                          rflags_.as<uint32_t>(), cflags_.as<uint32_t>(),
                                          probe_i_.as<uint32_t>(), Kc, np, npq_dev,
@@ -456,7 +474,7 @@ class IVFPQ final : public Index {
             else
                 probes_used += (uint64_t)rows * np;
             std::vector<uint32_t> tied;
-            for (uint32_t i = 0; i < rows; ++i)
+            for (uint32_t i = 0; i < rows && !canonical; ++i)
                 if (fl[i] & ROW_TIE) tied.push_back(i);
             last_replayed_ += tied.size();
             if (!tied.empty())

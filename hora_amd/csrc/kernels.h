@@ -20,6 +20,7 @@ constexpr uint32_t ROW_TIE = 1u;   // equal distances among the first min(k+1, n
 constexpr uint32_t ROW_NAN = 2u;   // a NaN distance in the row
 constexpr uint32_t ROW_UNCERT = 4u;  // shortlist could not be certified
 constexpr uint32_t ROW_BAD = 8u;     // internal consistency check failed
+constexpr uint32_t ROW_TIE_EDGE = 16u;  // the k-th and (k+1)-th distances are equal (IVFPQ)
 constexpr uint32_t ROW_SKIP = 0x80000000u;  // bf_finalize: leave the row's outputs alone
 
 // Rows kept on the device for the exact rerun: the plan block of
@@ -162,7 +163,8 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
                    uint32_t *out_count, uint32_t *flags, hipStream_t st,
                    const uint32_t *dev_rows = nullptr, const uint32_t *kq = nullptr,
                    const uint64_t *out_off = nullptr, const uint32_t *s_q = nullptr,
-                   const float *key_s_q = nullptr, const uint32_t *pre_flags = nullptr);
+                   const float *key_s_q = nullptr, const uint32_t *pre_flags = nullptr,
+                   int by_id = 0);
 // Candidate lists of the filter form -> the shortlist format of rerank:
 // per query the first min(cnt, S) entries by (key, row) into cand_i /
 // cand_key (stride ldc), s_q = that count, key_s_q = a lower bound of the

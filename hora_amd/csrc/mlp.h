@@ -10,6 +10,16 @@ namespace hm {
 void gemm_bf16(const uint16_t *A, long sam, long sak, const uint16_t *B, long sbk, long sbn,
                int M, int N, int K, const float *bias, bool relu, float *C32, long ldc32,
                uint16_t *C16, long ldc16, hipStream_t s);
+// C(m,n) = sum_k A[m][k] B[n][k] (both k-contiguous bf16; K % 32 == 0,
+// lda and ldb multiples of 8, 16-byte aligned); epilogue + bias[n],
+// optional ReLU, optional negation, f32 and/or bf16 stores.
+void gemm_nt_bf16(const uint16_t *A, long lda, const uint16_t *B, long ldb, int M, int N, int K,
+                  const float *bias, bool relu, bool negate, float *C32, long ldc32,
+                  uint16_t *C16, long ldc16, hipStream_t s);
+// lse[r] = log sum_j exp(-negz[r][j]).
+void row_lse(const float *negz, long ld, int rows, int cols, float *lse, hipStream_t s);
+// bf16 rows [rows][dpad] from f32 rows [rows][ld], zero padded past d.
+void rows_to_bf16(const float *x, long ld, long rows, int d, uint16_t *y, int dpad, hipStream_t s);
 void f32_to_bf16(const float *x, uint16_t *y, long n, hipStream_t s);
 // Row softmax; with T: G = (p - T) * scale and loss[row]; P optional.
 void softmax_xent(const float *Z, const float *T, int rows, int cols, float scale, uint16_t *G16,
@@ -35,6 +45,10 @@ class ProbeMLP {
 
     // Z[n][dout] logits for bf16 inputs Xb[n][din].
     void forward(const uint16_t *Xb, int n, float *Z);
+    // Inference: negated logits negz[n][ldz] and each row's log-partition
+    // lse[n] for bf16 inputs Xp[n][din_padded()] (zero padded).
+    void scores(const uint16_t *Xp, int n, float *negz, long ldz, float *lse);
+    int din_padded() const { return (din_ + 31) / 32 * 32; }
     // Loss and gradients (into the internal gradient buffers) for a batch;
     // T[n][dout] are target distributions.  Returns nothing: the loss per
     // row is left in loss_buf().
@@ -52,6 +66,9 @@ class ProbeMLP {
     DeviceBuffer gW1_, gb1_, gW2_, gb2_;
     DeviceBuffer mW1_, vW1_, mb1_, vb1_, mW2_, vW2_, mb2_, vb2_;
     DeviceBuffer Hb_, Z_, G16_, dH16_, loss_;
+    DeviceBuffer W1t_, W2t_, b1p_, Hinf_;  // inference copies, k contiguous, padded
+    bool inf_valid_ = false;
+    int hinf_cap_ = 0;
 };
 
 }  // namespace hm

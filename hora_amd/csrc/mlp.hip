@@ -90,6 +90,161 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
             }
 }
 
+// C(m,n) = sum_k A[m][k] B[n][k] for row-major bf16 A [M][lda] and B
+// [N][ldb] (k contiguous in both; K, lda, ldb multiples of 32, 8, 8).  The
+// inference GEMMs of the network (B is a transposed weight copy).  256
+// threads, a 128x128 tile of four 64x64 wave tiles (4x4 MFMA 16x16x32
+// blocks), K stages of 64 (two MFMA steps) staged through two LDS buffers
+// with the next stage's 16-byte global loads in flight during the current
+// stage's 32 MFMAs per wave.  Rows of the LDS tiles are 72 elements (144
+// bytes) apart.  Epilogue: + bias[n], optional ReLU, optional negation, f32
+// and/or bf16 stores.
+constexpr int NT_B = 128, NT_BK = 64, NT_LDK = NT_BK + 8;
+
+__global__ __launch_bounds__(256) void gemm_nt_bf16_kernel(
+    const uint16_t *__restrict__ A, long lda, const uint16_t *__restrict__ B, long ldb, int M,
+    int N, int K, const float *__restrict__ bias, int relu, int negate, float *__restrict__ C32,
+    long ldc32, uint16_t *__restrict__ C16, long ldc16)
+{
+    extern __shared__ __align__(16) uint16_t nt_smem[];
+    uint16_t *sA = nt_smem, *sB = nt_smem + 2 * NT_B * NT_LDK;  // [2][128][72] each
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int m0 = blockIdx.y * NT_B, n0 = blockIdx.x * NT_B;
+    const int lr = lane & 15, lk = lane >> 4;
+    floatx4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // chunk c of a 128 x 64 operand tile: row c / 8, k offset (c % 8) * 8;
+    // a stage of K = 32 (the last one when K % 64 == 32) loads chunks with
+    // (c % 8) < 4 only
+    uint4 ra[4], rb[4];
+    const uint4 zero = make_uint4(0, 0, 0, 0);
+    auto load = [&](int k0) {
+        const int kw = min(NT_BK, K - k0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = tid + 256 * u, r = c >> 3, kc = (c & 7) * 8;
+            const int gm = m0 + r, gn = n0 + r;
+            const bool kin = kc < kw;
+            ra[u] = (gm < M && kin) ? *reinterpret_cast<const uint4 *>(A + (long)gm * lda + k0 + kc)
+                                    : zero;
+            rb[u] = (gn < N && kin) ? *reinterpret_cast<const uint4 *>(B + (long)gn * ldb + k0 + kc)
+                                    : zero;
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = tid + 256 * u, r = c >> 3, kc = (c & 7) * 8;
+            *reinterpret_cast<uint4 *>(&sA[(buf * NT_B + r) * NT_LDK + kc]) = ra[u];
+            *reinterpret_cast<uint4 *>(&sB[(buf * NT_B + r) * NT_LDK + kc]) = rb[u];
+        }
+    };
+    load(0);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < K; k0 += NT_BK) {
+        const bool more = k0 + NT_BK < K;
+        if (more) load(k0 + NT_BK);
+        const int steps = min(NT_BK, K - k0) / 32;
+        for (int ks = 0; ks < steps; ++ks) {
+            bf16x8 a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                a[i] = *reinterpret_cast<const bf16x8 *>(
+                    &sA[(buf * NT_B + wr * 64 + i * 16 + lr) * NT_LDK + ks * 32 + lk * 8]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                b[j] = *reinterpret_cast<const bf16x8 *>(
+                    &sB[(buf * NT_B + wc * 64 + j * 16 + lr) * NT_LDK + ks * 32 + lk * 8]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] =
+                        __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int m = m0 + wr * 64 + i * 16 + lk * 4 + e;
+                const int n = n0 + wc * 64 + j * 16 + lr;
+                if (m >= M || n >= N) continue;
+                float v = acc[i][j][e];
+                if (bias) v += bias[n];
+                if (relu) v = fmaxf(v, 0.0f);
+                if (negate) v = -v;
+                if (C32) C32[(long)m * ldc32 + n] = v;
+                if (C16) C16[(long)m * ldc16 + n] = f2bf(v);
+            }
+}
+
+// dst[c][r] = src[r][c] (bf16 [rows][cols] -> [cols][ldd]; columns of dst
+// past `rows` are zero).
+__global__ void transpose_bf16_kernel(const uint16_t *__restrict__ src, int rows, int cols,
+                                      uint16_t *__restrict__ dst, int ldd)
+{
+    const long total = (long)cols * ldd;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(e / ldd), r = (int)(e % ldd);
+        dst[e] = r < rows ? src[(long)r * cols + c] : (uint16_t)0;
+    }
+}
+
+// bf16 rows [rows][dpad] from f32 rows [rows][ld] (d valid, zero padded).
+__global__ void rows_to_bf16_kernel(const float *__restrict__ x, long ld, long rows, int d,
+                                    uint16_t *__restrict__ y, int dpad)
+{
+    const long total = rows * dpad;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const long r = e / dpad;
+        const int c = (int)(e % dpad);
+        y[e] = c < d ? f2bf(x[r * ld + c]) : (uint16_t)0;
+    }
+}
+
+// lse[r] = log sum_j exp(-negz[r][j]) (the log-partition of the logits
+// z = -negz).  One workgroup per row.
+__global__ __launch_bounds__(256) void row_lse_kernel(const float *__restrict__ negz, long ld,
+                                                      int cols, float *__restrict__ lse)
+{
+    __shared__ float red[256];
+    const int r = blockIdx.x, tid = threadIdx.x;
+    const float *z = negz + (long)r * ld;
+    float mx = -3.0e38f;
+    for (int j = tid; j < cols; j += 256) mx = fmaxf(mx, -z[j]);
+    red[tid] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) red[tid] = fmaxf(red[tid], red[tid + s]);
+        __syncthreads();
+    }
+    mx = red[0];
+    __syncthreads();
+    float sum = 0.f;
+    for (int j = tid; j < cols; j += 256) sum += expf(-z[j] - mx);
+    red[tid] = sum;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) lse[r] = logf(red[0]) + mx;
+}
+
 __global__ void f32_to_bf16_kernel(const float *__restrict__ x, uint16_t *__restrict__ y, long n)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -201,6 +356,35 @@ void gemm_bf16(const uint16_t *A, long sam, long sak, const uint16_t *B, long sb
     HM_CHECK_LAUNCH();
 }
 
+void gemm_nt_bf16(const uint16_t *A, long lda, const uint16_t *B, long ldb, int M, int N, int K,
+                  const float *bias, bool relu, bool negate, float *C32, long ldc32,
+                  uint16_t *C16, long ldc16, hipStream_t s)
+{
+    if (M <= 0 || N <= 0) return;
+    if (K % 32 || lda % 8 || ldb % 8 || ((uintptr_t)A & 15) || ((uintptr_t)B & 15))
+        fail(HM_ERR_INVALID_ARG, "gemm_nt_bf16: K, strides or alignment");
+    dim3 grid((N + NT_B - 1) / NT_B, (M + NT_B - 1) / NT_B);
+    ProfScope prof("mlp_gemm", s);
+    const size_t lds = (size_t)4 * NT_B * NT_LDK * sizeof(uint16_t);  // 72 KiB
+    gemm_nt_bf16_kernel<<<grid, 256, lds, s>>>(A, lda, B, ldb, M, N, K, bias, relu ? 1 : 0,
+                                               negate ? 1 : 0, C32, ldc32, C16, ldc16);
+    HM_CHECK_LAUNCH();
+}
+
+void row_lse(const float *negz, long ld, int rows, int cols, float *lse, hipStream_t s)
+{
+    if (rows <= 0) return;
+    row_lse_kernel<<<rows, 256, 0, s>>>(negz, ld, cols, lse);
+    HM_CHECK_LAUNCH();
+}
+
+void rows_to_bf16(const float *x, long ld, long rows, int d, uint16_t *y, int dpad, hipStream_t s)
+{
+    if (rows <= 0) return;
+    rows_to_bf16_kernel<<<grid1(rows * dpad), 256, 0, s>>>(x, ld, rows, d, y, dpad);
+    HM_CHECK_LAUNCH();
+}
+
 void f32_to_bf16(const float *x, uint16_t *y, long n, hipStream_t s)
 {
     if (n <= 0) return;
@@ -284,6 +468,40 @@ void ProbeMLP::set_weights(const float *W1, const float *b1, const float *W2, co
     f32_to_bf16(W1_.as<float>(), W1b_.as<uint16_t>(), (long)n1, s_);
     f32_to_bf16(W2_.as<float>(), W2b_.as<uint16_t>(), (long)n2, s_);
     HM_HIP(hipStreamSynchronize(s_));
+    inf_valid_ = false;
+}
+
+void ProbeMLP::scores(const uint16_t *Xp, int n, float *negz, long ldz, float *lse)
+{
+    const int dinp = din_padded(), hidp = (hid_ + 31) / 32 * 32;
+    if (!inf_valid_) {
+        // k-contiguous bf16 copies of the weights, zero padded to multiples
+        // of 32: W1t [hidp][dinp], W2t [dout][hidp]; the bias b1 padded with
+        // zeros, so the padding units of H are relu(0) = 0
+        W1t_.reserve((size_t)hidp * dinp * 2);
+        W2t_.reserve((size_t)dout_ * hidp * 2);
+        b1p_.reserve((size_t)hidp * 4);
+        HM_HIP(hipMemsetAsync(W1t_.get(), 0, (size_t)hidp * dinp * 2, s_));
+        HM_HIP(hipMemsetAsync(b1p_.get(), 0, (size_t)hidp * 4, s_));
+        HM_HIP(hipMemcpyAsync(b1p_.get(), b1_.get(), (size_t)hid_ * 4, hipMemcpyDeviceToDevice, s_));
+        transpose_bf16_kernel<<<grid1((long)hid_ * dinp), 256, 0, s_>>>(
+            W1b_.as<uint16_t>(), din_, hid_, W1t_.as<uint16_t>(), dinp);
+        HM_CHECK_LAUNCH();
+        transpose_bf16_kernel<<<grid1((long)dout_ * hidp), 256, 0, s_>>>(
+            W2b_.as<uint16_t>(), hid_, dout_, W2t_.as<uint16_t>(), hidp);
+        HM_CHECK_LAUNCH();
+        inf_valid_ = true;
+    }
+    if (n > hinf_cap_) {
+        Hinf_.reserve((size_t)n * hidp * 2);
+        hinf_cap_ = n;
+    }
+    // H = relu(X W1 + b1) (bf16), then -(H W2 + b2) and the rows' log-partition
+    gemm_nt_bf16(Xp, dinp, W1t_.as<uint16_t>(), dinp, n, hidp, dinp, b1p_.as<float>(), true,
+                 false, nullptr, 0, Hinf_.as<uint16_t>(), hidp, s_);
+    gemm_nt_bf16(Hinf_.as<uint16_t>(), hidp, W2t_.as<uint16_t>(), hidp, n, dout_, hidp,
+                 b2_.as<float>(), false, true, negz, ldz, nullptr, 0, s_);
+    row_lse(negz, ldz, n, dout_, lse, s_);
 }
 
 void ProbeMLP::get_weights(float *W1, float *b1, float *W2, float *b2) const
@@ -351,6 +569,7 @@ void ProbeMLP::loss_grad(const uint16_t *Xb, const float *T, int n)
 void ProbeMLP::adam(float lr, float wd)
 {
     ++step_;
+    inf_valid_ = false;
     const long n1 = (long)din_ * hid_, n2 = (long)hid_ * dout_;
     adam_step(W1_.as<float>(), W1b_.as<uint16_t>(), gW1_.as<float>(), mW1_.as<float>(),
               vW1_.as<float>(), n1, lr, step_, wd, s_);

@@ -22,6 +22,7 @@ struct Total {
     uint64_t n = 0;
 };
 std::map<std::string, Total> g_totals;
+std::string g_only;  // when set, only this family is timed
 
 void drain_locked()
 {
@@ -44,6 +45,10 @@ bool profiling_enabled() { return g_on.load(std::memory_order_relaxed); }
 ProfScope::ProfScope(const char *name, hipStream_t s) : name_(name), s_(s)
 {
     if (!profiling_enabled()) return;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (!g_only.empty() && g_only != name) return;
+    }
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
     if (hipEventCreate(&start_) != hipSuccess || hipEventCreate(&stop_) != hipSuccess) {
@@ -89,6 +94,12 @@ extern "C" {
 HM_API int32_t hm_profile_enable(int32_t on)
 {
     hm::g_on.store(on != 0);
+    return HM_OK;
+}
+HM_API int32_t hm_profile_only(const char *kernel)
+{
+    std::lock_guard<std::mutex> lk(hm::g_mu);
+    hm::g_only = kernel ? kernel : "";
     return HM_OK;
 }
 HM_API int32_t hm_profile_reset(void)

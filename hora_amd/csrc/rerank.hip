@@ -43,7 +43,7 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
     float *__restrict__ out_dist, uint32_t *__restrict__ out_count, uint32_t *__restrict__ flags,
     const uint32_t *__restrict__ dev_rows, const uint32_t *__restrict__ kq,
     const uint64_t *__restrict__ out_off, const uint32_t *__restrict__ s_q,
-    const float *__restrict__ key_s_q, const uint32_t *__restrict__ pre_flags)
+    const float *__restrict__ key_s_q, const uint32_t *__restrict__ pre_flags, int by_id)
 {
     if (dev_rows && blockIdx.x >= *dev_rows) return;  // graph replays with fewer rows
     // dynamic LDS: keys[P] (u64) | vals[P] | sq[dpad]; P covers the largest s
@@ -213,7 +213,34 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
     uint64_t *oi = out_idx + obase;
     float *od = out_dist + obase;
     const float inf = __int_as_float(0x7f800000);
-    for (uint32_t j = tid; j < k; j += RR_THREADS) {
+    if (by_id) {
+        // order by (distance, row index): each sorted entry moves to its
+        // rank by index inside its run of equal distances
+        const uint32_t mm = m < s ? m : s;
+        for (uint32_t j = tid; j < s; j += RR_THREADS) {
+            const uint32_t kj = (uint32_t)(keys[j] >> 32);
+            uint32_t a = j, b = j + 1;
+            while (a > 0 && (uint32_t)(keys[a - 1] >> 32) == kj) --a;
+            while (b < s && (uint32_t)(keys[b] >> 32) == kj) ++b;
+            const uint32_t pos = (uint32_t)keys[j];
+            if (a >= mm || pos >= s) continue;  // past the result, or a flagged bad row
+            const uint32_t id = ci[pos];
+            uint32_t slot = a;
+            for (uint32_t t = a; t < b; ++t) {
+                const uint32_t pt = (uint32_t)keys[t];
+                slot += pt < s && ci[pt] < id;
+            }
+            if (slot < mm) {
+                oi[slot] = id;
+                od[slot] = vals[pos];
+            }
+        }
+        for (uint32_t j = mm + tid; j < k; j += RR_THREADS) {
+            oi[j] = ~0ull;
+            od[j] = inf;
+        }
+    }
+    for (uint32_t j = tid; j < k && !by_id; j += RR_THREADS) {
         if (j < m && j < s) {
             const uint32_t pos = (uint32_t)keys[j];
             oi[j] = ci[pos];
@@ -417,7 +444,7 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
                    const uint32_t *q_inexact, int certify, uint64_t *out_idx, float *out_dist,
                    uint32_t *out_count, uint32_t *flags, hipStream_t st,
                    const uint32_t *dev_rows, const uint32_t *kq, const uint64_t *out_off,
-                   const uint32_t *s_q, const float *key_s_q, const uint32_t *pre_flags)
+                   const uint32_t *s_q, const float *key_s_q, const uint32_t *pre_flags, int by_id)
 {
     if (rows == 0) return;
     if (s == 0 || s > RR_CAP) fail(HM_ERR_INVALID_ARG, "rerank: shortlist size out of range");
@@ -432,7 +459,7 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
     rerank_kernel<ACC, EPI><<<rows, RR_THREADS, lds, st>>>(                                     \
         Qp, dpad, Bp, cand_i, cand_key, ldc, s, qnorm2, bnorm2, n, k, bmax, gamma, c_in_exact, \
         c_in_inexact, q_inexact, certify, out_idx, out_dist, out_count, flags, dev_rows, kq,    \
-        out_off, s_q, key_s_q, pre_flags)
+        out_off, s_q, key_s_q, pre_flags, by_id)
     switch (metric) {
     case HM_METRIC_MANHATTAN: HM_RR(ACC_MANHATTAN, EPI_NONE); break;  // no certification bound
     case HM_METRIC_EUCLIDEAN: HM_RR(ACC_EUCLID, EPI_NONE); break;

@@ -305,12 +305,15 @@ HM_API int32_t hm_ivfpq_set_probe_policy(hm_index *h, int32_t mode, float mass, 
 HM_API int32_t hm_ivfpq_set_scan_form(hm_index *h, int32_t form);
 
 /* Exact rerank of the IVFPQ result (0, the default, keeps the reference's
- * ADC result, pq_idx.rs:181-191, 429-436): the reference's ADC search for
- * the R = max(depth, k) nearest, each candidate rescored with the exact
- * metric (the reference's f32 fold, metrics.rs:22-34), sorted by (exact
- * distance, ADC rank); the first k come back with their exact distances.
- * No counterpart in the reference: it wins back the accuracy the PQ codes
- * lose.  depth <= 4095; nprobe * (R + 1) <= 16384. */
+ * ADC result, pq_idx.rs:181-191, 429-436).  With depth > 0: the nprobe
+ * lists nearest by (coarse distance, list index) -- or the probe network's
+ * lists -- are scanned; the R = max(depth, k) smallest ADC distances by
+ * (distance, row index) are rescored with the exact metric (the reference's
+ * f32 fold, metrics.rs:22-34) and sorted by (exact distance, row index); the
+ * first k come back with their exact distances.  Every step is a total
+ * order, so no heap is replayed.  No counterpart in the reference: it wins
+ * back the accuracy the PQ codes lose.  depth <= 4095;
+ * nprobe * (R + 1) <= 16384. */
 HM_API int32_t hm_ivfpq_set_rerank(hm_index *h, uint32_t depth);
 
 /* Mean number of lists probed per query by the last search. */
@@ -337,6 +340,9 @@ HM_API int32_t hm_mlp_loss_grad(uint32_t din, uint32_t hidden, uint32_t dout, ui
  * hm_profile_query synchronises and returns the accumulated milliseconds
  * and launch count of a kernel family ("exact_tile", "row_select", ...). */
 HM_API int32_t hm_profile_enable(int32_t on);
+/* Restricts the timing to one family (NULL or "" times every family), so a
+ * timed region carries only the events of the kernel it measures. */
+HM_API int32_t hm_profile_only(const char *kernel);
 HM_API int32_t hm_profile_reset(void);
 HM_API int32_t hm_profile_query(const char *kernel, double *total_ms, uint64_t *launches);
 

@@ -11,6 +11,7 @@ import pytest
 
 from hora_amd import HoraError, IVFPQIndex, IVFPQParams, Metric
 from hora_amd.datasets import gaussian_mixture
+from ivf_rerank_oracle import rerank_oracle as _rerank_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -222,24 +223,6 @@ def test_errors():
     assert e.value.status == 10
 
 
-def _rerank_oracle(orc, x, q, mt, ex, nprobe, k, depth):
-    """The exact rerank: the reference's ADC search for R = max(depth, k),
-    each candidate's exact distance by the oracle metric, a stable sort by
-    that distance (equal distances keep the ADC rank), the first k."""
-    R = max(depth, k)
-    ri, rd, rc, _ = orc.ivfpq_search(q, int(mt), ex["centers"], ex["codebooks"],
-                                     ex["list_offsets"], ex["list_ids"], ex["codes"], nprobe, R)
-    ids, dist, cnt = [], [], []
-    for r in range(q.shape[0]):
-        cand = ri[r, :rc[r]].astype(np.int64)
-        ed = np.array([orc.metric(q[r], x[c], int(mt)) for c in cand], np.float32)
-        order = np.argsort(ed, kind="stable")[:k]
-        ids.append(cand[order])
-        dist.append(ed[order])
-        cnt.append(min(k, int(rc[r])))
-    return ids, dist, np.array(cnt, np.uint32)
-
-
 @pytest.mark.parametrize("mt,tied", [(Metric.Euclidean, False), (Metric.Euclidean, True),
                                      (Metric.CosineSimilarity, False), (Metric.Manhattan, True),
                                      (Metric.DotProduct, False)])
@@ -267,3 +250,33 @@ def test_exact_rerank_matches_oracle(orc, mt, tied):
                                      ex["list_offsets"], ex["list_ids"], ex["codes"], 1, 10)
     assert np.array_equal(ac, rc)
     _same_valid(ai, ad, ri, rd, rc)
+
+
+@pytest.mark.parametrize("form", ["per_probe", "lists", "rows"])
+def test_ties_among_empty_lists_need_no_replay(orc, form):
+    # 4 tight clusters far from the origin, 32 lists and one epoch: the
+    # init has two distinct centres, so 30 lists end empty with an all-zero
+    # centroid (kmeans.rs:115-122) and the queries see runs of equal
+    # coarse distances; those runs only hold empty lists, which cannot
+    # change the result (the tie check skips them)
+    x, cen, _ = gaussian_mixture(3000, 16, 4, seed=71, sigma=0.05)
+    x = (x + 3.0).astype(np.float32)
+    idx = _build(x, Metric.Euclidean, 32, 4, 4, 1, 72, fp64=True)
+    idx.set_scan_form(form)
+    ex = idx.export()
+    sizes = np.diff(ex["list_offsets"].astype(np.int64))
+    zero = int((np.abs(ex["centers"]).sum(1) == 0).sum())
+    assert (sizes == 0).sum() >= 20 and zero >= 6
+    q, _, _ = gaussian_mixture(50, 16, 4, seed=73, sigma=0.05, centres=cen)
+    q = (q + 3.0).astype(np.float32)
+    for nprobe in (2, 5, 16):
+        _check_conditional(orc, idx, x, Metric.Euclidean, q, nprobe, 10)
+    cd = np.sort(orc.distance_rows(ex["centers"], q, orc.EUCLIDEAN), axis=1)
+    assert (cd[:, 15] == cd[:, 16]).all()  # the 16th and 17th are zero centroids
+    idx.set_rerank(30)
+    idx.set_search_n_center(16)
+    gi, gd, gc = idx.search_batch(q, 10)
+    ri, rd, rc = _rerank_oracle(orc, x, q, Metric.Euclidean, ex, 16, 10, 30)
+    assert np.array_equal(gc, rc)
+    for r in range(q.shape[0]):
+        assert np.array_equal(gi[r, :gc[r]], ri[r])

@@ -7,6 +7,7 @@ import pytest
 from hora_amd import IVFPQIndex, IVFPQParams, Metric
 from hora_amd.datasets import gaussian_mixture
 from hora_amd.index import mlp_loss_grad
+from ivf_rerank_oracle import rerank_oracle
 from oracle import mlp
 
 pytestmark = pytest.mark.gpu
@@ -100,8 +101,17 @@ def test_learned_probe_gain_over_fixed_nprobe(orc):
         fixed[p] = _recall(idx.search_batch(test, k)[0], truth)
     assert fixed[1] < 0.97  # one probe is not enough on this data
     idx.set_probe_policy(1, mass=0.8, pmin=1, pmax=16)
-    learned = _recall(idx.search_batch(test, k)[0], truth)
+    gi, gd, gc = idx.search_batch(test, k)
+    learned = _recall(gi, truth)
     mean_np = idx.mean_nprobe()
+    # the learned + rerank form against its definition over the chosen lists
+    probes, counts = idx.last_probes(len(test), 16)
+    ri, rd, rc = rerank_oracle(orc, x, test[:100], Metric.Euclidean, idx.export(), None, k, 100,
+                               given=(probes[:100], counts[:100]))
+    assert np.array_equal(gc[:100], rc)
+    for r in range(100):
+        assert np.array_equal(gi[r, :gc[r]], ri[r])
+        assert np.array_equal(gd[r, :gc[r]].view(np.uint32), rd[r].view(np.uint32))
     lo = int(np.floor(mean_np))
     interp = fixed[lo] + (mean_np - lo) * (fixed[lo + 1] - fixed[lo])
     print(f"fixed nprobe recall@10 {fixed}; learned: {learned:.4f} at mean nprobe "
