@@ -128,3 +128,28 @@ def test_nan_distances_fail_like_the_exact_form():
         with pytest.raises(HoraError) as e:
             idx.search_batch(q, 5)
         assert e.value.status == 8
+
+
+@pytest.mark.parametrize("mt", [Metric.Euclidean, Metric.DotProduct, Metric.CosineSimilarity],
+                         ids=lambda m: Metric(m).name)
+@pytest.mark.parametrize("mode", [hora_amd.HM_SEARCH_BF16_RERANK, hora_amd.HM_SEARCH_FP8_RERANK],
+                         ids=["bf16", "fp8"])
+def test_filtered_scan_on_a_large_batch(orc, mt, mode):
+    # 2,500 queries (a partial last query tile), 6,000 rows (a partial last
+    # base tile), the filter form
+    base = bf_sparse_u8(6_000, 784, seed=411)
+    q = bf_sparse_u8(2_500, 784, seed=412)
+    idx = _index(base, mt, mode)
+    gi, gd, gc = idx.search_batch(q, 10)
+    ri, rd, rc = orc.bf_search(base, q, 10, int(mt))
+    assert np.array_equal(gc, rc)
+    if mode == hora_amd.HM_SEARCH_BF16_RERANK:  # certified: bit-exact
+        assert np.array_equal(gi, ri) and np.array_equal(gd, rd)
+    else:
+        hits = np.mean([len(set(gi[r].tolist()) & set(ri[r].tolist())) / 10
+                        for r in range(q.shape[0])])
+        assert hits >= 0.95, hits
+        for r in range(0, q.shape[0], 97):  # every returned distance is the exact value
+            for j in range(10):
+                e = orc.metric(q[r], base[int(gi[r, j])], int(mt))
+                assert np.float32(e).view(np.uint32) == gd[r, j].view(np.uint32)
