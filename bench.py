@@ -658,35 +658,45 @@ def main_ragged(args):
     d_cnt = torch.empty(Q, dtype=torch.int32, device=dev)
     timer = Timer(torch, stream)
     out = {}
+    # the configuration's shortlist of 4,096 rows, and the automatic depth
+    # (max(64, 4 (k + 1)) per query) for comparison
+    S_cfg = int(os.environ.get("HM_SHORTLIST", "4096"))
     for mt in (Metric.DotProduct, Metric.CosineSimilarity):
         for form, code in (("bf16", hora_amd.HM_SEARCH_BF16_RERANK),
                            ("fp8", hora_amd.HM_SEARCH_FP8_RERANK)):
-            ix = BruteForceIndex(D)
-            ix.add_device(xb.data_ptr(), N)
-            ix.build(mt)
-            ix.set_search_mode(code, int(os.environ.get("HM_SHORTLIST", "0")))
-            wall, _ = timer.run(lambda: ix.search_ragged_device(
-                xq.data_ptr(), off, ks, d_idx.data_ptr(), d_dist.data_ptr(), d_cnt.data_ptr(),
-                stream.cuda_stream), args.steps, args.warmup, f"{mt.name}/{form}",
-                only="score_filter")
-            km = kernel_ms(["score_gemm", "score_filter", "list_prepare", "row_select", "rerank",
-                            "exact_tile"], timer)
-            fb = ix.search_stats()
-            out[f"{mt.name}/{form}"] = {
-                "qps": round(args.steps * Q / wall, 1),
-                "exact_fallback_rows": {"rerun": fb[0], "uncertified": fb[1]},
-                "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in km.items()
-                                       if v[1]}}
-            log(f"{mt.name}/{form}: {out[f'{mt.name}/{form}']['qps']} QPS")
-            del ix
-    best = max(out, key=lambda k_: out[k_]["qps"])
+            for S in (S_cfg, 0):
+                ix = BruteForceIndex(D)
+                ix.add_device(xb.data_ptr(), N)
+                ix.build(mt)
+                ix.set_search_mode(code, S)
+                key = f"{mt.name}/{form}/shortlist_{S or 'auto'}"
+                # one captured graph per call (every chunk and the device
+                # rerun of tied rows), so the library's per-kernel events,
+                # which are off under capture, time only the host reruns
+                wall, _ = timer.run(lambda: ix.search_ragged_device(
+                    xq.data_ptr(), off, ks, d_idx.data_ptr(), d_dist.data_ptr(),
+                    d_cnt.data_ptr(), stream.cuda_stream), args.steps, args.warmup, key)
+                km = kernel_ms(["exact_tile", "row_select", "bf_finalize"], timer)
+                fb = ix.search_stats()
+                out[key] = {
+                    "qps": round(args.steps * Q / wall, 1),
+                    "ms_per_call": round(wall / args.steps * 1e3, 2),
+                    "exact_fallback_rows": {"rerun": fb[0], "uncertified": fb[1]},
+                    "host_rerun_kernel_ms_per_call": {k: round(v[0] / args.steps, 3)
+                                                      for k, v in km.items() if v[1]}}
+                log(f"{key}: {out[key]['qps']} QPS")
+                del ix
+    cfg = [k_ for k_ in out if k_.endswith(f"shortlist_{S_cfg}")]
+    best = max(cfg, key=lambda k_: out[k_]["qps"])
     print(json.dumps({"metric": METRIC, "value": out[best]["qps"], "unit": "QPS", "n_gpus": 1,
                       "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
-                      "scaling": "weak", "vs_baseline": None, "dtype": best.split("/")[1],
+                      "scaling": "weak", "vs_baseline": None,
+                      "dtype": best.split("/")[1] + " shortlist + f32 rerank",
                       "data": "synthetic N(0,1) rows L2-normalised (torch, seeded)",
                       "config": {"workload": "BruteForceIndex 1,000,000x768, 64 ragged batches "
                                              "of 1-4096 queries, k in {1,10,100}",
-                                 "queries": Q},
+                                 "queries": Q, "shortlist": S_cfg, "metric_form": best,
+                                 "graph_launches_per_call": 1},
                       "results": out}), flush=True)
 
 

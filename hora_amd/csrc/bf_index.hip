@@ -10,6 +10,7 @@
 // picks the min(k+1, n) smallest by (distance, index) and flags ties, and
 // finalize emits the first k or replays the heap for a tied row.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -52,7 +53,7 @@ class BruteForce final : public Index {
     }
     ~BruteForce() override
     {
-        if (rg_exec_) (void)hipGraphExecDestroy(rg_exec_);
+        drop_ragged_graphs();
         if (rg_stream_) (void)hipStreamDestroy(rg_stream_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
@@ -478,130 +479,220 @@ class BruteForce final : public Index {
 
     static constexpr uint32_t RAG_CHUNK = 4096;
 
-    void capture_ragged_graph(uint32_t QC, uint32_t kmax, uint32_t S)
+    // The captured ragged graphs, by call shape: nch chunks of RAG_CHUNK
+    // queries each run the shortlist pipeline, then one tail reruns up to
+    // RERUN_SLOTS rows that are only tied, with their own k, on the device.
+    struct RaggedGraph {
+        uint32_t nch, kmax, S, ns;
+        int32_t mode, metric;
+        uint64_t n;
+        std::array<const void *, 9> data;  // the index buffers it reads
+        hipGraphExec_t exec;
+    };
+    static constexpr size_t RAG_GRAPHS = 4;
+
+    void drop_ragged_graphs()
     {
+        for (auto &g : rg_graphs_) (void)hipGraphExecDestroy(g.exec);
+        rg_graphs_.clear();
+    }
+
+    // Grows a buffer the captured graphs use; a growth moves it, so every
+    // cached graph is dropped.
+    void rg_reserve(DeviceBuffer &b, size_t bytes)
+    {
+        if (bytes <= b.bytes()) return;
+        drop_ragged_graphs();
+        b.reserve(bytes);
+    }
+
+    hipGraphExec_t ragged_graph(uint32_t nch, uint32_t kmax, uint32_t S, bool dev_rerun)
+    {
+        const uint32_t QC = RAG_CHUNK;
         const FilterPlan fp = filter_plan(S);
         if (fp.ns) ensure_sample(fp.ns, stream_);
-        if (rg_exec_ && rg_qc_ == QC && rg_kmax_ == kmax && rg_s_ == S && rg_mode_ == mode_ &&
-            rg_metric_ == metric_ && rg_n_ == n_ && rg_ns_ == fp.ns)
-            return;
-        if (rg_exec_) {
-            (void)hipGraphExecDestroy(rg_exec_);
-            rg_exec_ = nullptr;
-        }
-        HM_HIP(hipStreamSynchronize(stream_));  // sample data ready before capture
-        if (!rg_stream_) HM_HIP(hipStreamCreateWithFlags(&rg_stream_, hipStreamNonBlocking));
         const uint32_t n = (uint32_t)n_;
         const bool bf16 = mode_ == HM_SEARCH_BF16_RERANK;
         const bool filter = fp.ns != 0;
         const uint32_t row_bytes = bf16 ? dk_ * 2 : dk_;
         const uint64_t ldk = round_up(filter ? fp.ns : n, 4);
-        rg_rows_.reserve(4);
-        rg_q_.reserve((size_t)QC * dim_ * 4);
-        rg_ck_.reserve((size_t)QC * 4);
-        rg_coff_.reserve((size_t)QC * 8);
-        rg_oi_.reserve((size_t)QC * kmax * 8);
-        rg_od_.reserve((size_t)QC * kmax * 4);
-        rg_oc_.reserve((size_t)QC * 4);
-        rg_qpad_.reserve((size_t)QC * dpad_ * 4);
-        rg_qnorm_.reserve((size_t)QC * 4);
-        rg_qlow_.reserve((size_t)QC * row_bytes);
-        rg_qscale_.reserve((size_t)QC * 4);
-        rg_D_.reserve((size_t)QC * ldk * 4);
-        rg_cd_.reserve((size_t)QC * S * 4);
-        rg_ci_.reserve((size_t)QC * S * 4);
-        rg_sf_.reserve((size_t)QC * 4);
-        rg_rf_.reserve((size_t)QC * 4);
-        rg_qinex_.reserve(4);
+        const uint64_t ldd = round_up(n, 4);
+        const size_t nq_all = (size_t)nch * QC;
+        rg_reserve(rg_call_, sizeof(RaggedCall));
+        rg_reserve(rg_kq_, nq_all * 4);
+        rg_reserve(rg_off_, (nq_all + 1) * 8);
+        rg_reserve(rg_flags_all_, nq_all * 4);
+        rg_reserve(rg_rows_, 4);
+        rg_reserve(rg_err_, 4);
+        rg_reserve(rg_ck_, (size_t)QC * 4);
+        rg_reserve(rg_coff_, (size_t)QC * 8);
+        rg_reserve(rg_oi_, (size_t)QC * kmax * 8);
+        rg_reserve(rg_od_, (size_t)QC * kmax * 4);
+        rg_reserve(rg_oc_, (size_t)QC * 4);
+        rg_reserve(rg_qpad_, (size_t)QC * dpad_ * 4);
+        rg_reserve(rg_qnorm_, (size_t)QC * 4);
+        rg_reserve(rg_qlow_, (size_t)QC * row_bytes);
+        rg_reserve(rg_qscale_, (size_t)QC * 4);
+        rg_reserve(rg_D_, (size_t)QC * ldk * 4);
+        rg_reserve(rg_cd_, (size_t)QC * S * 4);
+        rg_reserve(rg_ci_, (size_t)QC * S * 4);
+        rg_reserve(rg_sf_, (size_t)QC * 4);
+        rg_reserve(rg_rf_, (size_t)QC * 4);
+        rg_reserve(rg_qinex_, 4);
+        rg_reserve(rg_plan_, RERUN_PLAN_WORDS * 4);
         if (filter) {
-            rg_seld_.reserve((size_t)QC * fp.m * 4);
-            rg_seli_.reserve((size_t)QC * fp.m * 4);
-            rg_list_.reserve((size_t)QC * fp.cap * 8);
-            rg_lcnt_.reserve((size_t)QC * 4);
-            rg_sq_.reserve((size_t)QC * 4);
-            rg_ksq_.reserve((size_t)QC * 4);
-            rg_pref_.reserve((size_t)QC * 4);
+            rg_reserve(rg_seld_, (size_t)QC * fp.m * 4);
+            rg_reserve(rg_seli_, (size_t)QC * fp.m * 4);
+            rg_reserve(rg_list_, (size_t)QC * fp.cap * 8);
+            rg_reserve(rg_lcnt_, (size_t)QC * 4);
+            rg_reserve(rg_sq_, (size_t)QC * 4);
+            rg_reserve(rg_ksq_, (size_t)QC * 4);
+            rg_reserve(rg_pref_, (size_t)QC * 4);
         }
+        if (dev_rerun) {
+            rg_reserve(rg_r8p_, (size_t)RERUN_SLOTS * dpad_ * 4);
+            rg_reserve(rg_r8n_, RERUN_SLOTS * 4);
+            rg_reserve(rg_r8D_, (size_t)RERUN_SLOTS * ldd * 4);
+            rg_reserve(rg_r8i_, (size_t)RERUN_SLOTS * kmax * 8);
+            rg_reserve(rg_r8d_, (size_t)RERUN_SLOTS * kmax * 4);
+            rg_reserve(rg_r8c_, RERUN_SLOTS * 4);
+        }
+        // a graph reads the index's buffers where they were at capture: one
+        // captured before any of them moved (add, clear, mode or sample
+        // change) is dropped, never launched
+        const std::array<const void *, 9> data{base_.get(),   blow_.get(),     bscale_.get(),
+                                               bnorm_.get(),  inv_bnorm_.get(), bsamp_.get(),
+                                               bs_norm2_.get(), bs_inv_.get(),  bs_scale_.get()};
+        for (size_t i = 0; i < rg_graphs_.size();) {
+            if (rg_graphs_[i].data != data) {
+                (void)hipGraphExecDestroy(rg_graphs_[i].exec);
+                rg_graphs_.erase(rg_graphs_.begin() + i);
+            } else {
+                ++i;
+            }
+        }
+        for (auto &g : rg_graphs_)
+            if (g.nch == nch && g.kmax == kmax && g.S == S && g.ns == fp.ns && g.mode == mode_ &&
+                g.metric == metric_ && g.n == n_)
+                return g.exec;
+        if (rg_graphs_.size() >= RAG_GRAPHS) {
+            (void)hipGraphExecDestroy(rg_graphs_.front().exec);
+            rg_graphs_.erase(rg_graphs_.begin());
+        }
+        HM_HIP(hipStreamSynchronize(stream_));  // sample data ready before capture
+        if (!rg_stream_) HM_HIP(hipStreamCreateWithFlags(&rg_stream_, hipStreamNonBlocking));
         const float u = 5.9604644775390625e-8f;
         const float gamma = 2.0f * (float)(std::max<uint32_t>(dk_, dpad_) + 8) * u;
         const float c_bf16 = 1.0f / 128.0f + 1.0f / 65536.0f;
         const float c_exact = base_inexact_ ? c_bf16 : 0.0f;
         const uint32_t *dr = rg_rows_.as<uint32_t>();
+        const RaggedCall *call = rg_call_.as<RaggedCall>();
+        uint32_t *err = rg_err_.as<uint32_t>();
         hipStream_t g = rg_stream_;
         HM_HIP(hipStreamBeginCapture(g, hipStreamCaptureModeThreadLocal));
-        launch_fill_u32(rg_qinex_.as<uint32_t>(), 1, 0, g);
-        launch_pad_rows(rg_q_.as<float>(), dim_, rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_,
-                        dpad_, g, dr);
-        launch_self_dot(rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_, rg_qnorm_.as<float>(), g, dr);
-        if (bf16)
-            launch_to_bf16(rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_, dk_, rg_qlow_.as<uint16_t>(),
-                           rg_qinex_.as<uint32_t>(), g, dr);
-        else
-            launch_to_fp8(rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_, dk_, rg_qlow_.as<uint8_t>(),
-                          rg_qscale_.as<float>(), g, dr);
-        const uint32_t *s_q = nullptr, *pre = nullptr;
-        const float *key_s = nullptr;
-        if (filter) {
-            if (fp.m <= 32)  // from the minima of the sample keys in fours
-                launch_score_min4(bf16 ? 2 : 1, rg_qlow_.get(), QC, bsamp_.get(), fp.ns, row_bytes,
-                                  rg_qscale_.as<float>(), bs_scale_.as<float>(),
-                                  bs_norm2_.as<float>(), bs_inv_.as<float>(), metric_,
-                                  rg_D_.as<float>(), ldk, g, dr);
+        launch_fill_u32(err, 1, 0, g);
+        for (uint32_t c = 0; c < nch; ++c) {
+            launch_ragged_chunk_begin(call, c, QC, (uint32_t)dim_, dpad_, rg_kq_.as<uint32_t>(),
+                                      rg_off_.as<uint64_t>(), rg_rows_.as<uint32_t>(),
+                                      rg_ck_.as<uint32_t>(), rg_coff_.as<uint64_t>(),
+                                      rg_qpad_.as<float>(), err, g);
+            launch_fill_u32(rg_qinex_.as<uint32_t>(), 1, 0, g);
+            launch_self_dot(rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_, rg_qnorm_.as<float>(),
+                            g, dr);
+            if (bf16)
+                launch_to_bf16(rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_, dk_,
+                               rg_qlow_.as<uint16_t>(), rg_qinex_.as<uint32_t>(), g, dr);
             else
-                launch_score_gemm(bf16 ? 2 : 1, rg_qlow_.get(), QC, bsamp_.get(), fp.ns, row_bytes,
-                                  rg_qscale_.as<float>(), bs_scale_.as<float>(),
-                                  bs_norm2_.as<float>(), bs_inv_.as<float>(), metric_,
-                                  rg_D_.as<float>(), ldk, g, dr);
-            const float *tau;
-            uint32_t tau_ld;
-            if (fp.m <= 32) {
-                launch_kth_smallest(rg_D_.as<float>(), ldk, QC, fp.ns / 4, fp.m,
-                                    rg_seld_.as<float>(), g, dr);
-                tau = rg_seld_.as<float>();
-                tau_ld = 1;
+                launch_to_fp8(rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_, dk_,
+                              rg_qlow_.as<uint8_t>(), rg_qscale_.as<float>(), g, dr);
+            const uint32_t *s_q = nullptr, *pre = nullptr;
+            const float *key_s = nullptr;
+            if (filter) {
+                if (fp.m <= 32)  // from the minima of the sample keys in fours
+                    launch_score_min4(bf16 ? 2 : 1, rg_qlow_.get(), QC, bsamp_.get(), fp.ns,
+                                      row_bytes, rg_qscale_.as<float>(), bs_scale_.as<float>(),
+                                      bs_norm2_.as<float>(), bs_inv_.as<float>(), metric_,
+                                      rg_D_.as<float>(), ldk, g, dr);
+                else
+                    launch_score_gemm(bf16 ? 2 : 1, rg_qlow_.get(), QC, bsamp_.get(), fp.ns,
+                                      row_bytes, rg_qscale_.as<float>(), bs_scale_.as<float>(),
+                                      bs_norm2_.as<float>(), bs_inv_.as<float>(), metric_,
+                                      rg_D_.as<float>(), ldk, g, dr);
+                const float *tau;
+                uint32_t tau_ld;
+                if (fp.m <= 32) {
+                    launch_kth_smallest(rg_D_.as<float>(), ldk, QC, fp.ns / 4, fp.m,
+                                        rg_seld_.as<float>(), g, dr);
+                    tau = rg_seld_.as<float>();
+                    tau_ld = 1;
+                } else {
+                    launch_row_select(rg_D_.as<float>(), ldk, QC, fp.ns, fp.m,
+                                      rg_seld_.as<float>(), rg_seli_.as<uint32_t>(), fp.m,
+                                      rg_sf_.as<uint32_t>(), g, dr);
+                    tau = rg_seld_.as<float>() + (fp.m - 1);
+                    tau_ld = fp.m;
+                }
+                launch_fill_u32(rg_lcnt_.as<uint32_t>(), QC, 0, g);
+                launch_score_filter(bf16 ? 2 : 1, rg_qlow_.get(), QC, blow_.get(), n, row_bytes,
+                                    rg_qscale_.as<float>(), bscale_.as<float>(),
+                                    bnorm_.as<float>(), inv_bnorm_.as<float>(), metric_, tau,
+                                    tau_ld, rg_list_.as<uint64_t>(), fp.cap,
+                                    rg_lcnt_.as<uint32_t>(), g, dr);
+                launch_list_prepare(rg_list_.as<uint64_t>(), fp.cap, rg_lcnt_.as<uint32_t>(), tau,
+                                    tau_ld, QC, S, rg_ci_.as<uint32_t>(), rg_cd_.as<float>(), S,
+                                    rg_sq_.as<uint32_t>(), rg_ksq_.as<float>(),
+                                    rg_pref_.as<uint32_t>(), g, dr,
+                                    shortlist_ ? nullptr : rg_ck_.as<uint32_t>());
+                s_q = rg_sq_.as<uint32_t>();
+                key_s = rg_ksq_.as<float>();
+                pre = rg_pref_.as<uint32_t>();
             } else {
-                launch_row_select(rg_D_.as<float>(), ldk, QC, fp.ns, fp.m, rg_seld_.as<float>(),
-                                  rg_seli_.as<uint32_t>(), fp.m, rg_sf_.as<uint32_t>(), g, dr);
-                tau = rg_seld_.as<float>() + (fp.m - 1);
-                tau_ld = fp.m;
+                launch_score_gemm(bf16 ? 2 : 1, rg_qlow_.get(), QC, blow_.get(), n, row_bytes,
+                                  rg_qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
+                                  inv_bnorm_.as<float>(), metric_, rg_D_.as<float>(), ldk, g, dr);
+                launch_row_select(rg_D_.as<float>(), ldk, QC, n, S, rg_cd_.as<float>(),
+                                  rg_ci_.as<uint32_t>(), S, rg_sf_.as<uint32_t>(), g, dr);
             }
-            launch_fill_u32(rg_lcnt_.as<uint32_t>(), QC, 0, g);
-            launch_score_filter(bf16 ? 2 : 1, rg_qlow_.get(), QC, blow_.get(), n, row_bytes,
-                                rg_qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
-                                inv_bnorm_.as<float>(), metric_, tau, tau_ld, rg_list_.as<uint64_t>(),
-                                fp.cap, rg_lcnt_.as<uint32_t>(), g, dr);
-            launch_list_prepare(rg_list_.as<uint64_t>(), fp.cap, rg_lcnt_.as<uint32_t>(), tau, tau_ld,
-                                QC, S, rg_ci_.as<uint32_t>(), rg_cd_.as<float>(), S,
-                                rg_sq_.as<uint32_t>(), rg_ksq_.as<float>(), rg_pref_.as<uint32_t>(),
-                                g, dr, shortlist_ ? nullptr : rg_ck_.as<uint32_t>());
-            s_q = rg_sq_.as<uint32_t>();
-            key_s = rg_ksq_.as<float>();
-            pre = rg_pref_.as<uint32_t>();
-        } else {
-            launch_score_gemm(bf16 ? 2 : 1, rg_qlow_.get(), QC, blow_.get(), n, row_bytes,
-                              rg_qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
-                              inv_bnorm_.as<float>(), metric_, rg_D_.as<float>(), ldk, g, dr);
-            launch_row_select(rg_D_.as<float>(), ldk, QC, n, S, rg_cd_.as<float>(),
-                              rg_ci_.as<uint32_t>(), S, rg_sf_.as<uint32_t>(), g, dr);
+            launch_rerank(metric_, rg_qpad_.as<float>(), dpad_, base_.as<float>(),
+                          rg_ci_.as<uint32_t>(), rg_cd_.as<float>(), S, S, rg_qnorm_.as<float>(),
+                          bnorm_.as<float>(), QC, n, kmax, bmax_, gamma, c_exact, c_bf16,
+                          rg_qinex_.as<uint32_t>(), bf16 ? certify_ : 0, rg_oi_.as<uint64_t>(),
+                          rg_od_.as<float>(), rg_oc_.as<uint32_t>(), rg_rf_.as<uint32_t>(), g, dr,
+                          rg_ck_.as<uint32_t>(), rg_coff_.as<uint64_t>(), s_q, key_s, pre);
+            launch_ragged_chunk_end(call, c, QC, rg_rows_.as<uint32_t>(), rg_off_.as<uint64_t>(),
+                                    rg_oi_.as<uint64_t>(), rg_od_.as<float>(),
+                                    rg_oc_.as<uint32_t>(), rg_rf_.as<uint32_t>(),
+                                    rg_flags_all_.as<uint32_t>(), g);
         }
-        launch_rerank(metric_, rg_qpad_.as<float>(), dpad_, base_.as<float>(), rg_ci_.as<uint32_t>(),
-                      rg_cd_.as<float>(), S, S, rg_qnorm_.as<float>(), bnorm_.as<float>(), QC, n,
-                      kmax, bmax_, gamma, c_exact, c_bf16, rg_qinex_.as<uint32_t>(),
-                      bf16 ? certify_ : 0, rg_oi_.as<uint64_t>(), rg_od_.as<float>(),
-                      rg_oc_.as<uint32_t>(), rg_rf_.as<uint32_t>(), g, dr, rg_ck_.as<uint32_t>(),
-                      rg_coff_.as<uint64_t>(), s_q, key_s, pre);
+        // the rows that are only tied: their heap replay over the exact row,
+        // each with its own k (the heap's capacity)
+        uint32_t *pl = rg_plan_.as<uint32_t>();
+        launch_rerun_plan(rg_flags_all_.as<uint32_t>(), (uint32_t)nq_all, pl, g);
+        if (dev_rerun) {
+            const uint32_t *cnt = pl + 2 * RERUN_SLOTS;
+            launch_ragged_rerun_pad(call, pl, (uint32_t)dim_, dpad_, rg_r8p_.as<float>(), g);
+            if (need_norm())
+                launch_self_dot(rg_r8p_.as<float>(), dpad_, RERUN_SLOTS, (uint32_t)dim_,
+                                rg_r8n_.as<float>(), g, cnt);
+            launch_exact_rows_dev(metric_, rg_r8p_.as<float>(), dpad_, base_.as<float>(), dpad_, n,
+                                  dpad_, rg_r8n_.as<float>(), bnorm_.as<float>(),
+                                  rg_r8D_.as<float>(), ldd, cnt, g);
+            launch_bf_finalize(rg_cd_.as<float>(), rg_ci_.as<uint32_t>(), 1, pl + RERUN_SLOTS,
+                               rg_r8D_.as<float>(), ldd, RERUN_SLOTS, n, kmax,
+                               rg_r8i_.as<uint64_t>(), rg_r8d_.as<float>(),
+                               rg_r8c_.as<uint32_t>(), err, g, rg_kq_.as<uint32_t>(), pl);
+            launch_ragged_rerun_scatter(call, pl, rg_r8i_.as<uint64_t>(), rg_r8d_.as<float>(),
+                                        rg_r8c_.as<uint32_t>(), kmax, rg_kq_.as<uint32_t>(),
+                                        rg_off_.as<uint64_t>(), g);
+        }
         hipGraph_t graph = nullptr;
         HM_HIP(hipStreamEndCapture(g, &graph));
-        hipError_t e = hipGraphInstantiate(&rg_exec_, graph, nullptr, nullptr, 0);
+        hipGraphExec_t exec = nullptr;
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         HM_HIP(e);
-        rg_qc_ = QC;
-        rg_kmax_ = kmax;
-        rg_s_ = S;
-        rg_mode_ = mode_;
-        rg_metric_ = metric_;
-        rg_n_ = n_;
-        rg_ns_ = fp.ns;
+        rg_graphs_.push_back({nch, kmax, S, fp.ns, mode_, metric_, n_, data, exec});
+        return exec;
     }
 
     // Exact path for the listed queries with kmax, packed into the ragged
@@ -988,43 +1079,36 @@ class BruteForce final : public Index {
             for (uint64_t q = h_off[b]; q < h_off[b + 1]; ++q) kq[q] = h_k[b];
         off[0] = 0;
         for (uint64_t q = 0; q < Q; ++q) off[q + 1] = off[q] + kq[q];
-        rg_kq_.reserve(Q * 4);
-        rg_off_.reserve((Q + 1) * 8);
-        HM_HIP(hipMemcpyAsync(rg_kq_.get(), kq.data(), Q * 4, hipMemcpyHostToDevice, s));
-        HM_HIP(hipMemcpyAsync(rg_off_.get(), off.data(), (Q + 1) * 8, hipMemcpyHostToDevice, s));
         ensure_shortlist_data(s);
         const uint32_t S = shortlist_size(kmax);
         const uint32_t QC = RAG_CHUNK;
-        capture_ragged_graph(QC, kmax, S);
-        rg_flags_all_.reserve(Q * 4);
-        uint32_t rows_h = 0;
-        for (uint64_t q0 = 0; q0 < Q; q0 += QC) {
-            rows_h = (uint32_t)std::min<uint64_t>(QC, Q - q0);
-            const uint64_t o0 = off[q0], o1 = off[q0 + rows_h];
-            HM_HIP(hipMemcpyAsync(rg_rows_.get(), &rows_h, 4, hipMemcpyHostToDevice, s));
-            HM_HIP(hipMemcpyAsync(rg_q_.get(), dq + q0 * dim_, (size_t)rows_h * dim_ * 4,
-                                  hipMemcpyDeviceToDevice, s));
-            HM_HIP(hipMemcpyAsync(rg_ck_.get(), kq.data() + q0, (size_t)rows_h * 4,
-                                  hipMemcpyHostToDevice, s));
-            std::vector<uint64_t> loc(rows_h);
-            for (uint32_t i = 0; i < rows_h; ++i) loc[i] = off[q0 + i] - o0;
-            HM_HIP(hipMemcpyAsync(rg_coff_.get(), loc.data(), (size_t)rows_h * 8,
-                                  hipMemcpyHostToDevice, s));
-            HM_HIP(hipStreamSynchronize(s));  // inputs staged before the graph's stream runs
-            HM_HIP(hipGraphLaunch(rg_exec_, rg_stream_));
-            HM_HIP(hipStreamSynchronize(rg_stream_));
-            if (o1 > o0) {
-                HM_HIP(hipMemcpyAsync(d_idx + o0, rg_oi_.get(), (o1 - o0) * 8,
-                                      hipMemcpyDeviceToDevice, s));
-                HM_HIP(hipMemcpyAsync(d_dist + o0, rg_od_.get(), (o1 - o0) * 4,
-                                      hipMemcpyDeviceToDevice, s));
-            }
-            HM_HIP(hipMemcpyAsync(d_count + q0, rg_oc_.get(), (size_t)rows_h * 4,
-                                  hipMemcpyDeviceToDevice, s));
-            HM_HIP(hipMemcpyAsync(rg_flags_all_.as<uint32_t>() + q0, rg_rf_.get(),
-                                  (size_t)rows_h * 4, hipMemcpyDeviceToDevice, s));
-            ++graph_launches_;
+        const uint32_t nch = (uint32_t)((Q + QC - 1) / QC);
+        const bool dev_rerun = exact_rows_dev_ok(dpad_, dpad_, base_.as<float>());
+        hipGraphExec_t exec = ragged_graph(nch, kmax, S, dev_rerun);
+        // the call's inputs, staged once: per-query k and output offsets,
+        // and the pointers the graph reads
+        rg_host_.reserve(sizeof(RaggedCall) + (RERUN_PLAN_WORDS + 1) * 4);
+        RaggedCall *hc = rg_host_.as<RaggedCall>();
+        *hc = RaggedCall{dq, d_idx, d_dist, d_count, Q};
+        HM_HIP(hipMemcpyAsync(rg_kq_.get(), kq.data(), Q * 4, hipMemcpyHostToDevice, s));
+        HM_HIP(hipMemcpyAsync(rg_off_.get(), off.data(), (Q + 1) * 8, hipMemcpyHostToDevice, s));
+        HM_HIP(hipMemcpyAsync(rg_call_.get(), hc, sizeof(RaggedCall), hipMemcpyHostToDevice, s));
+        HM_HIP(hipGraphLaunch(exec, s));
+        ++graph_launches_;
+        uint32_t *hp = reinterpret_cast<uint32_t *>(rg_host_.as<unsigned char>() + sizeof(RaggedCall));
+        HM_HIP(hipMemcpyAsync(hp, rg_plan_.get(), RERUN_PLAN_WORDS * 4, hipMemcpyDeviceToHost, s));
+        HM_HIP(hipMemcpyAsync(hp + RERUN_PLAN_WORDS, rg_err_.get(), 4, hipMemcpyDeviceToHost, s));
+        HM_HIP(hipStreamSynchronize(s));
+        HM_HIP(hipMemcpyAsync(err_.get(), rg_err_.get(), 4, hipMemcpyDeviceToDevice, s));
+        last_fallback_ = hp[2 * RERUN_SLOTS + 2];
+        last_uncertified_ = hp[2 * RERUN_SLOTS + 3];
+        const bool host_acts = hp[2 * RERUN_SLOTS + 1] != 0 || (!dev_rerun && last_fallback_ > 0);
+        if (!host_acts) {
+            read_error(s);
+            return;
         }
+        // rows the device plan left (more than RERUN_SLOTS tied rows, or
+        // uncertified / NaN rows): the host reruns them
         std::vector<uint32_t> fl(Q);
         HM_HIP(hipMemcpyAsync(fl.data(), rg_flags_all_.get(), Q * 4, hipMemcpyDeviceToHost, s));
         HM_HIP(hipStreamSynchronize(s));
@@ -1156,13 +1240,13 @@ class BruteForce final : public Index {
     PinnedBuffer hflags_, hrows_, herr_, hplan_;  // row flags, rerun rows, error word, rerun plan
     DeviceBuffer plan_, r8p_, r8n_, r8D_, r8i_, r8d_, r8c_;  // device rerun of tied rows
     uint64_t last_fallback_ = 0, last_uncertified_ = 0;
-    // ragged graph
+    // ragged graphs
     hipStream_t rg_stream_ = nullptr;
-    hipGraphExec_t rg_exec_ = nullptr;
-    uint32_t rg_qc_ = 0, rg_kmax_ = 0, rg_s_ = 0, rg_ns_ = 0;
-    int32_t rg_mode_ = -1, rg_metric_ = -1;
-    uint64_t rg_n_ = 0, graph_launches_ = 0;
-    DeviceBuffer rg_rows_, rg_q_, rg_ck_, rg_coff_, rg_oi_, rg_od_, rg_oc_, rg_kq_, rg_off_,
+    std::vector<RaggedGraph> rg_graphs_;
+    uint64_t graph_launches_ = 0;
+    PinnedBuffer rg_host_;  // the call block and the plan / error read-back
+    DeviceBuffer rg_call_, rg_err_, rg_plan_, rg_r8p_, rg_r8n_, rg_r8D_, rg_r8i_, rg_r8d_, rg_r8c_;
+    DeviceBuffer rg_rows_, rg_ck_, rg_coff_, rg_oi_, rg_od_, rg_oc_, rg_kq_, rg_off_,
         rg_flags_all_;
     // buffers owned by the captured graph (never reallocated by other paths)
     DeviceBuffer rg_qpad_, rg_qnorm_, rg_qlow_, rg_qscale_, rg_D_, rg_cd_, rg_ci_, rg_sf_, rg_rf_,

@@ -182,6 +182,37 @@ void launch_scatter_ragged(const uint64_t *si, const float *sd, const uint32_t *
                            hipStream_t s);
 // dst[i] = (uint32_t)src[i] (row indices below 2^32).
 void launch_u64_to_u32(const uint64_t *src, uint32_t *dst, uint64_t n, hipStream_t s);
+// One ragged-batch call, in device memory: the captured graph's kernels
+// read the call's query and output pointers from it, so one graph serves
+// every call with the same shape.
+struct RaggedCall {
+    const float *dq;  // [Q][dim] queries
+    uint64_t *d_idx;  // packed outputs, k_b per query
+    float *d_dist;
+    uint32_t *d_count;  // [Q]
+    uint64_t Q;
+};
+// Chunk c (rows [c QC, min((c+1) QC, Q))): *rows = its row count; the rows
+// padded to dpad into qpad (non-finite values set *err bit 1), ck[r] =
+// kq[c QC + r], coff[r] = off[c QC + r] - off[c QC].
+void launch_ragged_chunk_begin(const RaggedCall *call, uint32_t c, uint32_t QC, uint32_t dim,
+                               uint32_t dpad, const uint32_t *kq, const uint64_t *off,
+                               uint32_t *rows, uint32_t *ck, uint64_t *coff, float *qpad,
+                               uint32_t *err, hipStream_t s);
+// The chunk's packed results (oi / od / oc, rf row flags) into the call's
+// outputs; flags_all[c QC + r] = rf[r] (0 past the chunk's rows).
+void launch_ragged_chunk_end(const RaggedCall *call, uint32_t c, uint32_t QC,
+                             const uint32_t *rows, const uint64_t *off, const uint64_t *oi,
+                             const float *od, const uint32_t *oc, const uint32_t *rf,
+                             uint32_t *flags_all, hipStream_t s);
+// The rerun plan's rows (plan[0 .. count)) padded from the call's queries.
+void launch_ragged_rerun_pad(const RaggedCall *call, const uint32_t *plan, uint32_t dim,
+                             uint32_t dpad, float *dst, hipStream_t s);
+// The rerun rows' results ([RERUN_SLOTS][kmax]) into the call's packed
+// outputs, query q keeping its first kq[q].
+void launch_ragged_rerun_scatter(const RaggedCall *call, const uint32_t *plan, const uint64_t *si,
+                                 const float *sd, const uint32_t *sc, uint32_t kmax,
+                                 const uint32_t *kq, const uint64_t *off, hipStream_t s);
 // Fill kernel (graph-safe replacement of hipMemsetAsync).
 void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s);
 void launch_rerun_plan(const uint32_t *flags, uint32_t nq, uint32_t *plan, hipStream_t s);

@@ -411,6 +411,88 @@ __global__ void u64_to_u32_kernel(const uint64_t *__restrict__ src, uint32_t *__
         dst[i] = (uint32_t)src[i];
 }
 
+// one workgroup of 64 threads per chunk row
+__global__ __launch_bounds__(64) void ragged_chunk_begin_kernel(
+    const RaggedCall *__restrict__ call, uint32_t c, uint32_t QC, uint32_t dim, uint32_t dpad,
+    const uint32_t *__restrict__ kq, const uint64_t *__restrict__ off, uint32_t *__restrict__ rows,
+    uint32_t *__restrict__ ck, uint64_t *__restrict__ coff, float *__restrict__ qpad,
+    uint32_t *err)
+{
+    const uint64_t Q = call->Q, q0 = (uint64_t)c * QC;
+    const uint32_t nr = Q > q0 ? (uint32_t)min<uint64_t>(QC, Q - q0) : 0u;
+    const uint32_t r = blockIdx.x, tid = threadIdx.x;
+    if (r == 0 && tid == 0) *rows = nr;
+    if (r >= nr) return;
+    const float *src = call->dq + (q0 + r) * dim;
+    float *dst = qpad + (size_t)r * dpad;
+    bool bad = false;
+    for (uint32_t t = tid; t < dpad; t += 64) {
+        const float v = t < dim ? src[t] : 0.0f;
+        bad |= !isfinite(v);
+        dst[t] = v;
+    }
+    if (bad) atomicOr(err, 2u);
+    if (tid == 0) {
+        ck[r] = kq[q0 + r];
+        coff[r] = off[q0 + r] - off[q0];
+    }
+}
+
+__global__ void ragged_chunk_end_kernel(const RaggedCall *__restrict__ call, uint32_t c,
+                                        uint32_t QC, const uint32_t *__restrict__ rows,
+                                        const uint64_t *__restrict__ off,
+                                        const uint64_t *__restrict__ oi,
+                                        const float *__restrict__ od,
+                                        const uint32_t *__restrict__ oc,
+                                        const uint32_t *__restrict__ rf,
+                                        uint32_t *__restrict__ flags_all)
+{
+    const uint32_t nr = *rows;
+    const uint64_t q0 = (uint64_t)c * QC;
+    const uint64_t o0 = nr ? off[q0] : 0, tot = nr ? off[q0 + nr] - o0 : 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    for (uint64_t e = t0; e < tot; e += stride) {
+        call->d_idx[o0 + e] = oi[e];
+        call->d_dist[o0 + e] = od[e];
+    }
+    for (uint64_t r = t0; r < QC; r += stride) {
+        if (r < nr) call->d_count[q0 + r] = oc[r];
+        flags_all[q0 + r] = r < nr ? rf[r] : 0u;
+    }
+}
+
+__global__ void ragged_rerun_pad_kernel(const RaggedCall *__restrict__ call,
+                                        const uint32_t *__restrict__ plan, uint32_t dim,
+                                        uint32_t dpad, float *__restrict__ dst)
+{
+    const uint32_t cnt = plan[2 * RERUN_SLOTS];
+    const uint32_t slot = blockIdx.x;
+    if (slot >= cnt) return;
+    const float *src = call->dq + (uint64_t)plan[slot] * dim;
+    for (uint32_t t = threadIdx.x; t < dpad; t += blockDim.x)
+        dst[(size_t)slot * dpad + t] = t < dim ? src[t] : 0.0f;
+}
+
+__global__ void ragged_rerun_scatter_kernel(const RaggedCall *__restrict__ call,
+                                            const uint32_t *__restrict__ plan,
+                                            const uint64_t *__restrict__ si,
+                                            const float *__restrict__ sd,
+                                            const uint32_t *__restrict__ sc, uint32_t kmax,
+                                            const uint32_t *__restrict__ kq,
+                                            const uint64_t *__restrict__ off)
+{
+    const uint32_t cnt = plan[2 * RERUN_SLOTS];
+    const uint32_t slot = blockIdx.x;
+    if (slot >= cnt) return;
+    const uint32_t q = plan[slot], kr = kq[q];
+    for (uint32_t j = threadIdx.x; j < kr; j += blockDim.x) {
+        call->d_idx[off[q] + j] = si[(size_t)slot * kmax + j];
+        call->d_dist[off[q] + j] = sd[(size_t)slot * kmax + j];
+    }
+    if (threadIdx.x == 0) call->d_count[q] = sc[slot] < kr ? sc[slot] : kr;
+}
+
 __global__ void fill_u32_kernel(uint32_t *p, uint32_t n, uint32_t v)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -469,6 +551,41 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
     default: fail(HM_ERR_INVALID_ARG, "rerank: metric has no shortlist form");
     }
 #undef HM_RR
+    HM_CHECK_LAUNCH();
+}
+
+void launch_ragged_chunk_begin(const RaggedCall *call, uint32_t c, uint32_t QC, uint32_t dim,
+                               uint32_t dpad, const uint32_t *kq, const uint64_t *off,
+                               uint32_t *rows, uint32_t *ck, uint64_t *coff, float *qpad,
+                               uint32_t *err, hipStream_t s)
+{
+    ragged_chunk_begin_kernel<<<QC, 64, 0, s>>>(call, c, QC, dim, dpad, kq, off, rows, ck, coff,
+                                               qpad, err);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_ragged_chunk_end(const RaggedCall *call, uint32_t c, uint32_t QC,
+                             const uint32_t *rows, const uint64_t *off, const uint64_t *oi,
+                             const float *od, const uint32_t *oc, const uint32_t *rf,
+                             uint32_t *flags_all, hipStream_t s)
+{
+    ragged_chunk_end_kernel<<<1024, 256, 0, s>>>(call, c, QC, rows, off, oi, od, oc, rf,
+                                                 flags_all);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_ragged_rerun_pad(const RaggedCall *call, const uint32_t *plan, uint32_t dim,
+                             uint32_t dpad, float *dst, hipStream_t s)
+{
+    ragged_rerun_pad_kernel<<<RERUN_SLOTS, 256, 0, s>>>(call, plan, dim, dpad, dst);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_ragged_rerun_scatter(const RaggedCall *call, const uint32_t *plan, const uint64_t *si,
+                                 const float *sd, const uint32_t *sc, uint32_t kmax,
+                                 const uint32_t *kq, const uint64_t *off, hipStream_t s)
+{
+    ragged_rerun_scatter_kernel<<<RERUN_SLOTS, 256, 0, s>>>(call, plan, si, sd, sc, kmax, kq, off);
     HM_CHECK_LAUNCH();
 }
 

@@ -442,12 +442,12 @@ __global__ __launch_bounds__(BF_THREADS) void bf_finalize_wave_kernel(
     const float inf = __int_as_float(0x7f800000);
     uint64_t *oi = out_idx + (size_t)r * k;  // k: the output stride
     float *od = out_dist + (size_t)r * k;
+    const uint32_t f = flags ? flags[r] : ROW_TIE;  // no flags: every row tied
+    if (f == ROW_SKIP) return;  // (a skipped row's map entry may be unset)
     // this row's k (at most the stride), and the heap's capacity: a heap
     // over n <= k elements never pops, so min(k_r, n) replays the same steps
     const uint32_t kr = row_k ? min(row_k[row_k_map ? row_k_map[r] : r], k) : k;
     const uint32_t m = kr < n ? kr : n;
-    const uint32_t f = flags ? flags[r] : ROW_TIE;  // no flags: every row tied
-    if (f == ROW_SKIP) return;
     if ((f & (ROW_NAN | ROW_BAD)) && n >= 2) {
         if (tid == 0) atomicOr(err, (f & ROW_BAD) ? 4u : 1u);
         for (uint32_t j = tid; j < k; j += BF_THREADS) {
@@ -641,11 +641,11 @@ __global__ __launch_bounds__(64) void bf_replay_lds_kernel(
     const float inf = __int_as_float(0x7f800000);
     uint64_t *oi = out_idx + (size_t)r * k;  // k: the output stride
     float *od = out_dist + (size_t)r * k;
+    const uint32_t f = flags ? flags[r] : ROW_TIE;  // no flags: every row tied
+    if (f == ROW_SKIP) return;
     const uint32_t kr = row_k ? min(row_k[row_k_map ? row_k_map[r] : r], k) : k;
     const uint32_t m = kr < n ? kr : n;  // heap capacity (see the wave kernel)
     const uint32_t cap = k < n ? k : n;  // the LDS layout's capacity, every row
-    const uint32_t f = flags ? flags[r] : ROW_TIE;  // no flags: every row tied
-    if (f == ROW_SKIP) return;
     if ((f & (ROW_NAN | ROW_BAD)) && n >= 2) {
         if (lane == 0) atomicOr(err, (f & ROW_BAD) ? 4u : 1u);
         for (uint32_t j = lane; j < k; j += 64) {

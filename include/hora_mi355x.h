@@ -159,10 +159,12 @@ HM_API int32_t hm_index_search_device_noalloc(hm_index *h, const float *d_querie
  * queries [batch_offsets[b], batch_offsets[b+1]) (host array, nb+1 entries)
  * searched with k = batch_k[b] (host array).  d_queries holds all queries
  * back to back; results are packed batch after batch, k_b entries per query
- * (d_idx / d_dist hold sum_b n_b*k_b entries), d_count one per query.  The
- * shortlist pipeline runs as one captured hipGraph per chunk of <= 4096
- * queries.  No reference counterpart (the reference searches one query at a
- * time, ann_index.rs:91-97). */
+ * (d_idx / d_dist hold sum_b n_b*k_b entries), d_count one per query.  In
+ * the shortlist forms one captured hipGraph per call runs every chunk of
+ * <= 4096 queries and the device rerun of up to 8 tied rows (each with its
+ * own k); the host reruns rows only when that plan leaves some.  The exact
+ * form searches each batch with its own k.  No reference counterpart (the
+ * reference searches one query at a time, ann_index.rs:91-97). */
 HM_API int32_t hm_index_search_ragged_device(hm_index *h, const float *d_queries,
                                              uint32_t n_batches, const uint32_t *batch_offsets,
                                              const uint32_t *batch_k, uint64_t *d_idx,
