@@ -5,10 +5,15 @@
 
 namespace hm {
 
+// Working memory of the fp64 centre steps (grown on demand).
+struct SegScratch {
+    DeviceBuffer part, nch, cstart, tmp;
+};
 // kmeans.rs:50-86 for the coarse quantizer: column means of X (f32 left fold
-// in row order, or fp64) and the +-1 initial centres, written [nc][ldc].
+// in row order, or fp64 sums over fixed row chunks) and the +-1 initial
+// centres, written [nc][ldc].
 void launch_coarse_init(bool fp64, const float *X, uint32_t ldx, uint64_t n, uint32_t d,
-                        uint32_t nc, float *C, uint32_t ldc, hipStream_t s);
+                        uint32_t nc, float *C, uint32_t ldc, SegScratch &ws, hipStream_t s);
 void launch_unpack_assign(const unsigned long long *best, uint64_t n, uint32_t *assign,
                           hipStream_t s);
 // Stable grouping of rows by list: ids = row indices sorted by (assign, row),
@@ -16,19 +21,24 @@ void launch_unpack_assign(const unsigned long long *best, uint64_t n, uint32_t *
 void build_lists(const uint32_t *assign, uint64_t n, uint32_t nlist, uint32_t *sorted_keys,
                  uint32_t *ids, uint32_t *off, DeviceBuffer &iota, DeviceBuffer &tmp,
                  hipStream_t s);
-// kmeans.rs:88-125: centre = mean of members (point order), empty -> 0.
+// kmeans.rs:88-125: centre = mean of members, empty -> 0 (f32: point-order
+// fold; fp64: sums over fixed chunks of SEG_ROWS members).  n = rows in ids.
 void launch_update(bool fp64, const float *X, uint32_t ldx, const uint32_t *off,
                    const uint32_t *ids, uint32_t nlist, uint32_t d, float *C, uint32_t ldc,
-                   hipStream_t s);
+                   uint64_t n, SegScratch &ws, hipStream_t s);
 // R[r] = X[ids[r]] - C[lists[r]] for rows in list order.
 void launch_residuals(const float *X, uint32_t ldx, const uint32_t *ids, const uint32_t *lists,
                       const float *C, uint64_t n, uint32_t dpad, float *R, hipStream_t s);
 // Per (list, sub-space) k-means on the residuals: codebooks
-// [nlist][M][KS][sdm] and codes [n][M] (rows in list order).
+// [nlist][M][KS][sdm] and codes [n][M] (rows in list order).  lorder[nlist]
+// is the order in which workgroups take the lists (longest first);
+// asg_scratch [M][n] u16 and ord_scratch [M][n] u32 hold the assignments and
+// the member order of lists too long to keep in LDS.
 void launch_pq_train(bool fp64, const float *R, uint32_t ldr, const uint32_t *off,
-                     uint32_t nlist, uint32_t M, const uint32_t *sub_b, const uint32_t *sub_e,
-                     uint32_t KS, uint32_t sd_max, uint32_t sdm, uint32_t epochs, uint64_t seed,
-                     uint32_t max_list, uint64_t n, uint16_t *asg_scratch, float *codebooks,
+                     const uint32_t *lorder, uint32_t nlist, uint32_t M, const uint32_t *sub_b,
+                     const uint32_t *sub_e, uint32_t KS, uint32_t sd_max, uint32_t sdm,
+                     uint32_t epochs, uint64_t seed, uint32_t max_list, uint64_t n,
+                     uint16_t *asg_scratch, uint32_t *ord_scratch, float *codebooks,
                      uint8_t *codes, hipStream_t s);
 // Candidate slots per (query, probe) workgroup.
 uint32_t ivf_scan_pool(uint32_t k1);

@@ -126,6 +126,119 @@ __global__ void update_kernel(const float *__restrict__ X, uint32_t ldx,
     C[(size_t)c * ldc + j] = v;
 }
 
+// FP64 column sums with a fixed split (the fp64 accumulation mode of the
+// centre steps, kmeans.rs:50-86 and 88-125 with f64 sums).  The positions of
+// segment c (off[c]..off[c+1] of ids; rows 0..n when off is null) are cut
+// into chunks of SEG_ROWS; seg_partial_kernel folds one chunk per workgroup
+// (four row groups, each a contiguous quarter of the chunk, combined in
+// group order) and seg_mean_kernel folds a segment's chunk sums in chunk
+// order.  The split depends only on the segment sizes, so the sums are the
+// same on every run; they differ from a point-order f64 fold only in the
+// rounding of the f64 additions.
+constexpr uint32_t SEG_ROWS = 512;
+
+// nch[c] = chunks of segment c; nch[nseg] = 0 (the exclusive scan of nch
+// then gives each segment's first chunk and, at nseg, the total).
+__global__ void seg_chunk_count_kernel(const uint32_t *__restrict__ off, uint32_t nseg,
+                                       uint32_t *__restrict__ nch)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > nseg) return;
+    nch[c] = c < nseg ? (off[c + 1] - off[c] + SEG_ROWS - 1) / SEG_ROWS : 0u;
+}
+
+__global__ __launch_bounds__(256) void seg_partial_kernel(
+    const float *__restrict__ X, uint32_t ldx, const uint32_t *__restrict__ ids,
+    const uint32_t *__restrict__ off, const uint32_t *__restrict__ cstart, uint32_t nseg,
+    uint64_t n, uint32_t d, double *__restrict__ part)
+{
+    __shared__ double red[4][64];
+    const uint32_t q = blockIdx.x;
+    uint64_t a, b;
+    if (cstart) {
+        if (q >= cstart[nseg]) return;
+        uint32_t lo = 0, hi = nseg - 1;  // the segment c with cstart[c] <= q < cstart[c+1]
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (cstart[mid + 1] <= q) lo = mid + 1;
+            else hi = mid;
+        }
+        a = off[lo] + (uint64_t)(q - cstart[lo]) * SEG_ROWS;
+        b = min((uint64_t)off[lo + 1], a + SEG_ROWS);
+    } else {
+        a = (uint64_t)q * SEG_ROWS;
+        if (a >= n) return;
+        b = min(n, a + SEG_ROWS);
+    }
+    const uint32_t lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const uint32_t j = blockIdx.y * 64 + lane;
+    const uint64_t quarter = (b - a + 3) / 4;
+    const uint64_t ga = min(b, a + g * quarter), gb = min(b, ga + quarter);
+    double acc = 0.0;
+    if (j < d) {
+        uint64_t r = ga;
+        for (; r + 8 <= gb; r += 8) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint64_t row = ids ? ids[r + u] : r + u;
+                v[u] = X[row * ldx + j];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = acc + (double)v[u];
+        }
+        for (; r < gb; ++r) acc = acc + (double)X[(ids ? ids[r] : r) * ldx + j];
+    }
+    red[g][lane] = acc;
+    __syncthreads();
+    if (g == 0 && j < d)
+        part[(size_t)q * d + j] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+}
+
+// Segment c's sum, its chunk sums folded in chunk order (four groups over
+// contiguous quarters, combined in group order), divided by its size.
+// nc_init == 0: C[c] = the mean (zero for an empty segment).  nc_init > 0
+// (one segment, the whole of X): C[i] = mean +- 1 by init_offset(i, j) for
+// i < nc_init.  Columns d..ldc are zero.
+__global__ __launch_bounds__(256) void seg_mean_kernel(
+    const double *__restrict__ part, const uint32_t *__restrict__ cstart,
+    const uint32_t *__restrict__ off, uint64_t n, uint32_t d, uint32_t nc_init,
+    float *__restrict__ C, uint32_t ldc)
+{
+    __shared__ double red[4][64];
+    const uint32_t c = blockIdx.x;
+    const uint32_t lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const uint32_t j = blockIdx.y * 64 + lane;
+    const uint32_t q0 = cstart ? cstart[c] : 0u;
+    const uint32_t q1 = cstart ? cstart[c + 1] : (uint32_t)((n + SEG_ROWS - 1) / SEG_ROWS);
+    const uint64_t cnt = off ? (uint64_t)(off[c + 1] - off[c]) : n;
+    const uint32_t quarter = (q1 - q0 + 3) / 4;
+    const uint32_t ga = min(q1, q0 + g * quarter), gb = min(q1, ga + quarter);
+    double acc = 0.0;
+    if (j < d) {
+        uint32_t q = ga;
+        for (; q + 4 <= gb; q += 4) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = part[(size_t)(q + u) * d + j];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc = acc + v[u];
+        }
+        for (; q < gb; ++q) acc = acc + part[(size_t)q * d + j];
+    }
+    red[g][lane] = acc;
+    __syncthreads();
+    if (g != 0 || j >= ldc) return;
+    const double sum = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+    if (nc_init == 0) {
+        C[(size_t)c * ldc + j] = (j < d && cnt != 0) ? (float)(sum / (double)cnt) : 0.0f;
+        return;
+    }
+    const float mean = (float)(sum / (double)cnt);
+    for (uint32_t i = 0; i < nc_init; ++i)
+        C[(size_t)i * ldc + j] = j < d ? (init_offset(i, j) > 0 ? mean + 1.0f : mean - 1.0f) : 0.0f;
+}
+
 // R[r] = X[ids[r]] - C[list(r)] (f32), rows in list order.
 __global__ void residual_kernel(const float *__restrict__ X, uint32_t ldx,
                                 const uint32_t *__restrict__ ids,
@@ -145,44 +258,83 @@ __global__ void residual_kernel(const float *__restrict__ X, uint32_t ldx,
 // One workgroup trains the k-means of sub-space s of list l on the list's
 // residuals (pq_idx.rs:122-149 -> kmeans.rs:195-207) and writes its
 // codebook and the final assignment (the codes, pq_idx.rs:143-146).
+//
+// Centres live in LDS at a stride of SDM floats with zero padding past sd:
+// a padded term adds (0 - 0)^2 = +0 to a fold that is already >= +0, so the
+// distances equal the sd-term fold.  update_center (kmeans.rs:88-125) sums
+// each centre's members in point order: a stable counting sort of the items
+// by assignment (LDS histogram, block scan, then a per-wave rank from
+// ballots of the assignment bits) lists every centre's members in item
+// order, and one lane per (centre, dimension) folds its run.  This is the
+// same sequence of additions as scanning the items for each centre, in
+// O(L) work instead of O(L * KS).  A list of at most stage_cap items keeps
+// its residuals, assignments and the sorted order in LDS; a longer one
+// reads R and keeps both arrays in global scratch laid out [M][n_rows].
+// Workgroups take lists in the order lorder gives (longest first), so the
+// long lists do not start last.
 constexpr int PQ_THREADS = 256;
 constexpr int PQ_MAXSD = 32;
 
-template <typename ACC_T>
+// Exclusive prefix sum of v over the workgroup (all threads call it).
+__device__ __forceinline__ uint32_t pq_block_excl_scan(uint32_t v, uint32_t *wtot)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wtot[w] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (int u = 0; u < w; ++u) pre += wtot[u];
+    __syncthreads();
+    return pre + x - v;
+}
+
+template <typename ACC_T, int SDM>
 __global__ __launch_bounds__(PQ_THREADS) void pq_train_kernel(
-    const float *__restrict__ R, uint32_t ldr, const uint32_t *__restrict__ off, uint32_t M,
-    const uint32_t *__restrict__ sub_b, const uint32_t *__restrict__ sub_e, uint32_t KS,
-    uint32_t sdm, uint32_t epochs, uint64_t seed, int staged, uint64_t n_rows,
-    uint16_t *__restrict__ asg_g, float *__restrict__ codebooks, uint8_t *__restrict__ codes)
+    const float *__restrict__ R, uint32_t ldr, const uint32_t *__restrict__ off,
+    const uint32_t *__restrict__ lorder, uint32_t M, const uint32_t *__restrict__ sub_b,
+    const uint32_t *__restrict__ sub_e, uint32_t KS, uint32_t sdm, uint32_t epochs,
+    uint64_t seed, uint32_t stage_cap, uint64_t n_rows, uint16_t *__restrict__ asg_g,
+    uint32_t *__restrict__ ord_g, float *__restrict__ codebooks, uint8_t *__restrict__ codes)
 {
     extern __shared__ __align__(16) unsigned char pq_smem[];
-    const uint32_t inst = blockIdx.x;
-    const uint32_t l = inst / M, s = inst % M;
+    __shared__ uint32_t wtot[PQ_THREADS / 64];
+    constexpr int NW = PQ_THREADS / 64;
+    const uint32_t l = lorder[blockIdx.x / M], s = blockIdx.x % M;
     const uint32_t base = off[l], L = off[l + 1] - base;
     const uint32_t b = sub_b[s], sd = sub_e[s] - sub_b[s];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     float *cb = codebooks + ((size_t)l * M + s) * KS * sdm;
     if (L == 0) {
         for (uint32_t e = tid; e < KS * sdm; e += PQ_THREADS) cb[e] = 0.0f;
         return;
     }
-    float *cent = reinterpret_cast<float *>(pq_smem);          // KS * sd
-    uint32_t *cnt = reinterpret_cast<uint32_t *>(cent + KS * sd);  // KS
-    float *zs = reinterpret_cast<float *>(cnt + KS);              // L * sd when staged
-    uint16_t *as = reinterpret_cast<uint16_t *>(zs + (staged ? (size_t)L * sd : 0));
-    // assignments: LDS when staged, else the global scratch laid out [M][n_rows]
-    uint16_t *asg = staged ? as : asg_g + (size_t)s * n_rows + base;
+    float *cent = reinterpret_cast<float *>(pq_smem);        // KS * SDM
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(cent + KS * SDM);  // KS
+    uint32_t *start = cnt + KS;                              // KS
+    uint32_t *run = start + KS;                              // KS
+    uint16_t *wcnt = reinterpret_cast<uint16_t *>(run + KS);  // NW * KS
+    float *zs = reinterpret_cast<float *>(wcnt + NW * KS);    // L * sd when staged
+    const bool staged = L <= stage_cap;
+    uint32_t *ord = staged ? reinterpret_cast<uint32_t *>(zs + (size_t)L * sd)
+                           : ord_g + (size_t)s * n_rows + base;
+    uint16_t *asg = staged ? reinterpret_cast<uint16_t *>(ord + L)
+                           : asg_g + (size_t)s * n_rows + base;
 
     auto Z = [&](uint32_t i, uint32_t j) -> float {
         return staged ? zs[i * sd + j] : R[(size_t)(base + i) * ldr + b + j];
     };
-    if (staged) {
+    if (staged)
         for (uint32_t e = tid; e < L * sd; e += PQ_THREADS) {
             uint32_t i = e / sd, j = e % sd;
             zs[e] = R[(size_t)(base + i) * ldr + b + j];
         }
-        __syncthreads();
-    }
+    for (uint32_t e = tid; e < KS * SDM; e += PQ_THREADS) cent[e] = 0.0f;
+    __syncthreads();
 
     // init_center (kmeans.rs:50-86)
     if (tid < (int)sd) {
@@ -191,10 +343,8 @@ __global__ __launch_bounds__(PQ_THREADS) void pq_train_kernel(
         float mean;
         if constexpr (sizeof(ACC_T) == 4) mean = acc / (float)L;
         else mean = (float)(acc / (double)L);
-        for (uint32_t c = 0; c < KS; ++c) {
-            float v = init_offset(c, tid) > 0 ? mean + 1.0f : mean - 1.0f;
-            cent[c * sd + tid] = v;
-        }
+        for (uint32_t c = 0; c < KS; ++c)
+            cent[c * SDM + tid] = init_offset(c, tid) > 0 ? mean + 1.0f : mean - 1.0f;
     }
     __syncthreads();
 
@@ -203,21 +353,20 @@ __global__ __launch_bounds__(PQ_THREADS) void pq_train_kernel(
     for (uint32_t ep = 0; ep <= epochs; ++ep) {
         // search_data (kmeans.rs:127-148): argmin, strict '<'
         for (uint32_t i = tid; i < L; i += PQ_THREADS) {
-            float z[PQ_MAXSD];
-            for (uint32_t j = 0; j < sd; ++j) z[j] = Z(i, j);
-            float bd = -0.0f;
-            for (uint32_t j = 0; j < sd; ++j) {
-                float df = z[j] - cent[j];
-                bd = bd + df * df;
-            }
+            float z[SDM];
+#pragma unroll
+            for (int j = 0; j < SDM; ++j) z[j] = (uint32_t)j < sd ? Z(i, j) : 0.0f;
+            float bd = 0.0f;
             uint32_t best = 0;
-            for (uint32_t c = 1; c < KS; ++c) {
+            for (uint32_t c = 0; c < KS; ++c) {
+                const float *cc = cent + c * SDM;
                 float dc = -0.0f;
-                for (uint32_t j = 0; j < sd; ++j) {
-                    float df = z[j] - cent[c * sd + j];
+#pragma unroll
+                for (int j = 0; j < SDM; ++j) {
+                    const float df = z[j] - cc[j];
                     dc = dc + df * df;
                 }
-                if (dc < bd) {
+                if (c == 0 || dc < bd) {
                     bd = dc;
                     best = c;
                 }
@@ -226,57 +375,111 @@ __global__ __launch_bounds__(PQ_THREADS) void pq_train_kernel(
             else asg[i] = (uint16_t)best;
         }
         if (ep == epochs) break;  // the search after train (pq_idx.rs:143-144)
-        __syncthreads();
-        // update_center (kmeans.rs:88-125)
-        // KS <= 256 = PQ_THREADS: each lane owns at most one centre
-        float nv[PQ_MAXSD];
-        uint32_t mycnt = 0;
+        // update_center (kmeans.rs:88-125), KS <= PQ_THREADS
         if ((uint32_t)tid < KS) {
-            const uint32_t c = tid;
-            ACC_T acc[PQ_MAXSD];
-            for (uint32_t j = 0; j < sd; ++j) acc[j] = 0;
-            uint32_t n_c = 0;
-            for (uint32_t i = 0; i < L; ++i) {
-                if (asg[i] == c) {
-                    ++n_c;
-                    for (uint32_t j = 0; j < sd; ++j) acc[j] = acc[j] + (ACC_T)Z(i, j);
+            cnt[tid] = 0;
+#pragma unroll
+            for (int u = 0; u < NW; ++u) wcnt[u * KS + tid] = 0;
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < L; i += PQ_THREADS) atomicAdd(&cnt[asg[i]], 1u);
+        __syncthreads();
+        {
+            const uint32_t st = pq_block_excl_scan((uint32_t)tid < KS ? cnt[tid] : 0u, wtot);
+            if ((uint32_t)tid < KS) {
+                start[tid] = st;
+                run[tid] = st;
+            }
+        }
+        __syncthreads();
+        // stable scatter: ord[start[c] ..] = members of c in item order
+        for (uint32_t i0 = 0; i0 < L; i0 += PQ_THREADS) {
+            const uint32_t i = i0 + tid;
+            const bool valid = i < L;
+            const uint32_t v = valid ? asg[i] : 0u;
+            uint64_t eq = __ballot(valid);
+            for (uint32_t bit = 1; bit < KS; bit <<= 1) {
+                const uint64_t bb = __ballot((v & bit) != 0);
+                eq &= (v & bit) ? bb : ~bb;
+            }
+            const uint32_t rk = __popcll(eq & ((1ull << lane) - 1ull));
+            if (valid && rk == 0) wcnt[w * KS + v] = (uint16_t)__popcll(eq);
+            __syncthreads();
+            if (valid) {
+                uint32_t p = run[v] + rk;
+                for (int u = 0; u < w; ++u) p += wcnt[u * KS + v];
+                ord[p] = i;
+            }
+            __syncthreads();
+            if ((uint32_t)tid < KS) {
+                uint32_t add = 0;
+#pragma unroll
+                for (int u = 0; u < NW; ++u) {
+                    add += wcnt[u * KS + tid];
+                    wcnt[u * KS + tid] = 0;
                 }
+                run[tid] += add;
             }
-            for (uint32_t j = 0; j < sd; ++j) {
-                if (n_c == 0) nv[j] = 0.0f;
-                else if constexpr (sizeof(ACC_T) == 4) nv[j] = acc[j] / (float)n_c;
-                else nv[j] = (float)(acc[j] / (double)n_c);
-            }
-            mycnt = n_c;
+            __syncthreads();
+        }
+        // new centre = its members' point-order sum / count; empty -> 0
+        for (uint32_t e = tid; e < KS * sd; e += PQ_THREADS) {
+            const uint32_t c = e / sd, j = e % sd;
+            const uint32_t p0 = start[c], n_c = cnt[c];
+            ACC_T acc = 0;
+            for (uint32_t p = p0; p < p0 + n_c; ++p) acc = acc + (ACC_T)Z(ord[p], j);
+            float nv;
+            if (n_c == 0) nv = 0.0f;
+            else if constexpr (sizeof(ACC_T) == 4) nv = acc / (float)n_c;
+            else nv = (float)(acc / (double)n_c);
+            cent[c * SDM + j] = nv;
         }
         __syncthreads();
-        if ((uint32_t)tid < KS) {
-            for (uint32_t j = 0; j < sd; ++j) cent[tid * sd + j] = nv[j];
-            cnt[tid] = mycnt;
-        }
-        __syncthreads();
-        // split_center (kmeans.rs:150-193), all epochs but the last
-        if (ep + 1 < epochs && tid == 0) {
+        // split_center (kmeans.rs:150-193), all epochs but the last, by wave
+        // 0.  The walk for an empty centre i draws once per step: step t
+        // looks at centre (i + 1 + t) % KS with the generator's (t + 1)-th
+        // next draw and stops at the first u < count / L.  Each lane takes
+        // one step of a block of 64, a ballot finds the first hit, and the
+        // generator advances by the steps taken -- the same draws and the
+        // same picks as the step-by-step walk.
+        if (ep + 1 < epochs && w == 0) {
             for (uint32_t i = 0; i < KS; ++i) {
                 if (cnt[i] != 0) continue;
-                uint32_t sc = (i + 1) % KS;
-                for (;;) {
-                    double pick = (double)cnt[sc] / (double)L;
-                    if (splitmix_unit(&rng) < pick) break;
-                    sc = (sc + 1) % KS;
-                }
-                for (uint32_t j = 0; j < sd; ++j) {
-                    float src = cent[sc * sd + j];
-                    if (j % 2 == 0) {
-                        cent[i * sd + j] = src * (1.0f - EPS);
-                        cent[sc * sd + j] = src * (1.0f + EPS);
-                    } else {
-                        cent[i * sd + j] = src * (1.0f + EPS);
-                        cent[sc * sd + j] = src * (1.0f - EPS);
+                uint32_t steps = 0;
+                for (uint32_t t0 = 0;; t0 += 64) {
+                    const uint32_t t = t0 + lane;
+                    const uint32_t c = (i + 1 + t) % KS;
+                    const uint32_t n_c = cnt[c];
+                    const bool hit =
+                        n_c != 0 && splitmix_unit_at(rng, t) < (double)n_c / (double)L;
+                    const uint64_t m = __ballot(hit);
+                    if (m) {
+                        steps = t0 + (uint32_t)__builtin_ctzll(m);
+                        break;
                     }
                 }
-                cnt[i] = cnt[sc] / 2;
-                cnt[sc] -= cnt[i];
+                const uint32_t sc = (i + 1 + steps) % KS;
+                rng += ((uint64_t)steps + 1) * 0x9E3779B97F4A7C15ull;
+                if ((uint32_t)lane < sd) {
+                    const uint32_t j = lane;
+                    const float src = cent[sc * SDM + j];
+                    if (j % 2 == 0) {
+                        cent[i * SDM + j] = src * (1.0f - EPS);
+                        cent[sc * SDM + j] = src * (1.0f + EPS);
+                    } else {
+                        cent[i * SDM + j] = src * (1.0f + EPS);
+                        cent[sc * SDM + j] = src * (1.0f - EPS);
+                    }
+                }
+                if (lane == 0) {
+                    const uint32_t half = cnt[sc] / 2;
+                    cnt[i] = half;
+                    cnt[sc] -= half;
+                }
+                // the next walk reads these counts from other lanes
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
         }
         __syncthreads();
@@ -284,7 +487,7 @@ __global__ __launch_bounds__(PQ_THREADS) void pq_train_kernel(
     __syncthreads();
     for (uint32_t e = tid; e < KS * sdm; e += PQ_THREADS) {
         uint32_t c = e / sdm, j = e % sdm;
-        cb[e] = j < sd ? cent[c * sd + j] : 0.0f;
+        cb[e] = j < sd ? cent[c * SDM + j] : 0.0f;
     }
 }
 
@@ -1465,13 +1668,25 @@ inline unsigned grid_for(uint64_t work, unsigned cap = 8192)
 }  // namespace
 
 void launch_coarse_init(bool fp64, const float *X, uint32_t ldx, uint64_t n, uint32_t d,
-                        uint32_t nc, float *C, uint32_t ldc, hipStream_t s)
+                        uint32_t nc, float *C, uint32_t ldc, SegScratch &ws, hipStream_t s)
 {
     ProfScope prof("kmeans_init", s);
-    if (fp64)
-        coarse_init_kernel<double><<<ceil_div(ldc, 64), 64, 0, s>>>(X, ldx, n, d, nc, C, ldc);
-    else
+    if (!fp64) {
         coarse_init_kernel<float><<<ceil_div(ldc, 64), 64, 0, s>>>(X, ldx, n, d, nc, C, ldc);
+        HM_CHECK_LAUNCH();
+        return;
+    }
+    if (n == 0) fail(HM_ERR_EMPTY, "no rows to average");
+    const uint64_t nch = (n + SEG_ROWS - 1) / SEG_ROWS;
+    if (nch > 0x7FFFFFFFull) fail(HM_ERR_CAPACITY, "too many rows");
+    ws.part.reserve((size_t)nch * std::max(d, 1u) * 8);
+    if (d > 0) {
+        seg_partial_kernel<<<dim3((unsigned)nch, ceil_div(d, 64)), 256, 0, s>>>(
+            X, ldx, nullptr, nullptr, nullptr, 1, n, d, ws.part.as<double>());
+        HM_CHECK_LAUNCH();
+    }
+    seg_mean_kernel<<<dim3(1, ceil_div(ldc, 64)), 256, 0, s>>>(ws.part.as<double>(), nullptr,
+                                                               nullptr, n, d, nc, C, ldc);
     HM_CHECK_LAUNCH();
 }
 
@@ -1504,14 +1719,38 @@ void build_lists(const uint32_t *assign, uint64_t n, uint32_t nlist, uint32_t *s
 
 void launch_update(bool fp64, const float *X, uint32_t ldx, const uint32_t *off,
                    const uint32_t *ids, uint32_t nlist, uint32_t d, float *C, uint32_t ldc,
-                   hipStream_t s)
+                   uint64_t n, SegScratch &ws, hipStream_t s)
 {
     ProfScope prof("kmeans_update", s);
-    dim3 grid(nlist, ceil_div(ldc, 64));
-    if (fp64)
-        update_kernel<double><<<grid, 64, 0, s>>>(X, ldx, off, ids, d, C, ldc);
-    else
-        update_kernel<float><<<grid, 64, 0, s>>>(X, ldx, off, ids, d, C, ldc);
+    if (!fp64) {
+        update_kernel<float><<<dim3(nlist, ceil_div(ldc, 64)), 64, 0, s>>>(X, ldx, off, ids, d,
+                                                                           C, ldc);
+        HM_CHECK_LAUNCH();
+        return;
+    }
+    // chunk table: cstart = exclusive scan of the per-list chunk counts
+    ws.nch.reserve((size_t)(nlist + 1) * 4);
+    ws.cstart.reserve((size_t)(nlist + 1) * 4);
+    seg_chunk_count_kernel<<<ceil_div(nlist + 1, 256), 256, 0, s>>>(off, nlist,
+                                                                      ws.nch.as<uint32_t>());
+    HM_CHECK_LAUNCH();
+    size_t tb = 0;
+    HM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, ws.nch.as<uint32_t>(),
+                                            ws.cstart.as<uint32_t>(), (int)(nlist + 1), s));
+    ws.tmp.reserve(tb);
+    HM_HIP(hipcub::DeviceScan::ExclusiveSum(ws.tmp.get(), tb, ws.nch.as<uint32_t>(),
+                                            ws.cstart.as<uint32_t>(), (int)(nlist + 1), s));
+    // every list holds at most one partial chunk, so the total is bounded by
+    const uint64_t max_chunks = n / SEG_ROWS + nlist;
+    if (max_chunks > 0x7FFFFFFFull) fail(HM_ERR_CAPACITY, "too many rows");
+    if (d > 0 && max_chunks > 0) {
+        ws.part.reserve((size_t)max_chunks * d * 8);
+        seg_partial_kernel<<<dim3((unsigned)max_chunks, ceil_div(d, 64)), 256, 0, s>>>(
+            X, ldx, ids, off, ws.cstart.as<uint32_t>(), nlist, n, d, ws.part.as<double>());
+        HM_CHECK_LAUNCH();
+    }
+    seg_mean_kernel<<<dim3(nlist, ceil_div(ldc, 64)), 256, 0, s>>>(
+        ws.part.as<double>(), ws.cstart.as<uint32_t>(), off, n, d, 0, C, ldc);
     HM_CHECK_LAUNCH();
 }
 
@@ -1522,30 +1761,66 @@ void launch_residuals(const float *X, uint32_t ldx, const uint32_t *ids, const u
     HM_CHECK_LAUNCH();
 }
 
-void launch_pq_train(bool fp64, const float *R, uint32_t ldr, const uint32_t *off,
+namespace {
+
+template <typename ACC_T, int SDM>
+void pq_train_launch(const float *R, uint32_t ldr, const uint32_t *off, const uint32_t *lorder,
                      uint32_t nlist, uint32_t M, const uint32_t *sub_b, const uint32_t *sub_e,
                      uint32_t KS, uint32_t sd_max, uint32_t sdm, uint32_t epochs, uint64_t seed,
-                     uint32_t max_list, uint64_t n, uint16_t *asg_scratch, float *codebooks,
+                     uint32_t max_list, uint64_t n, uint16_t *asg, uint32_t *ord,
+                     float *codebooks, uint8_t *codes, hipStream_t s)
+{
+    constexpr size_t LDS_BUDGET = 72 * 1024;  // two workgroups per CU
+    const size_t fixed = (size_t)KS * SDM * 4 + (size_t)KS * 3 * 4 +
+                         (size_t)(PQ_THREADS / 64) * KS * 2;
+    const size_t per_item = (size_t)sd_max * 4 + 4 + 2;
+    uint32_t cap = fixed + 16 < LDS_BUDGET ? (uint32_t)((LDS_BUDGET - fixed - 16) / per_item) : 0;
+    cap = std::min(cap, max_list);
+    const size_t lds = fixed + (size_t)cap * per_item + 16;
+    pq_train_kernel<ACC_T, SDM><<<dim3(nlist * M), PQ_THREADS, lds, s>>>(
+        R, ldr, off, lorder, M, sub_b, sub_e, KS, sdm, epochs, seed, cap, n, asg, ord, codebooks,
+        codes);
+    HM_CHECK_LAUNCH();
+}
+
+template <typename ACC_T>
+void pq_train_dispatch(uint32_t sd_max, const float *R, uint32_t ldr, const uint32_t *off,
+                       const uint32_t *lorder, uint32_t nlist, uint32_t M, const uint32_t *sub_b,
+                       const uint32_t *sub_e, uint32_t KS, uint32_t sdm, uint32_t epochs,
+                       uint64_t seed, uint32_t max_list, uint64_t n, uint16_t *asg, uint32_t *ord,
+                       float *codebooks, uint8_t *codes, hipStream_t s)
+{
+#define HM_PQ_TRAIN(SDM)                                                                      \
+    pq_train_launch<ACC_T, SDM>(R, ldr, off, lorder, nlist, M, sub_b, sub_e, KS, sd_max, sdm, \
+                                epochs, seed, max_list, n, asg, ord, codebooks, codes, s)
+    if (sd_max <= 4) HM_PQ_TRAIN(4);
+    else if (sd_max <= 8) HM_PQ_TRAIN(8);
+    else if (sd_max <= 16) HM_PQ_TRAIN(16);
+    else HM_PQ_TRAIN(32);
+#undef HM_PQ_TRAIN
+}
+
+}  // namespace
+
+void launch_pq_train(bool fp64, const float *R, uint32_t ldr, const uint32_t *off,
+                     const uint32_t *lorder, uint32_t nlist, uint32_t M, const uint32_t *sub_b,
+                     const uint32_t *sub_e, uint32_t KS, uint32_t sd_max, uint32_t sdm,
+                     uint32_t epochs, uint64_t seed, uint32_t max_list, uint64_t n,
+                     uint16_t *asg_scratch, uint32_t *ord_scratch, float *codebooks,
                      uint8_t *codes, hipStream_t s)
 {
     if (sd_max > (uint32_t)PQ_MAXSD) fail(HM_ERR_NOT_IMPLEMENTED, "sub-space dimension above 32");
     if (KS > 256) fail(HM_ERR_NOT_IMPLEMENTED, "sub_bits above 8");
-    const size_t fixed = (size_t)KS * sd_max * 4 + (size_t)KS * 4;
-    const size_t stage = (size_t)max_list * sd_max * 4 + (size_t)max_list * 2 + 16;
-    const int staged = fixed + stage <= 56 * 1024 ? 1 : 0;
-    const size_t lds = fixed + (staged ? stage : 0);
     if (n > 0xFFFFFFFFull) fail(HM_ERR_CAPACITY, "too many rows");
-    const dim3 grid(nlist * M);
     ProfScope prof("pq_train", s);
     if (fp64)
-        pq_train_kernel<double><<<grid, PQ_THREADS, lds, s>>>(R, ldr, off, M, sub_b, sub_e, KS,
-                                                              sdm, epochs, seed, staged, n,
-                                                              asg_scratch, codebooks, codes);
+        pq_train_dispatch<double>(sd_max, R, ldr, off, lorder, nlist, M, sub_b, sub_e, KS, sdm,
+                                  epochs, seed, max_list, n, asg_scratch, ord_scratch, codebooks,
+                                  codes, s);
     else
-        pq_train_kernel<float><<<grid, PQ_THREADS, lds, s>>>(R, ldr, off, M, sub_b, sub_e, KS,
-                                                             sdm, epochs, seed, staged, n,
-                                                             asg_scratch, codebooks, codes);
-    HM_CHECK_LAUNCH();
+        pq_train_dispatch<float>(sd_max, R, ldr, off, lorder, nlist, M, sub_b, sub_e, KS, sdm,
+                                 epochs, seed, max_list, n, asg_scratch, ord_scratch, codebooks,
+                                 codes, s);
 }
 
 uint32_t ivf_scan_pool(uint32_t k1) { return k1 <= 1024 ? 4096u : 8192u; }
@@ -1848,14 +2123,13 @@ void split_centers_host(float *C, uint32_t ldc, uint32_t nc, uint32_t dim, uint6
 {
     if (n == 0) fail(HM_ERR_EMPTY, "None to assigned impossible split center");
     const float EPS = 1.0f / 1024.0f;
+    std::vector<uint64_t> words((nc + 63) / 64, 0);
+    NonzeroSet nz{words.data(), nc};
+    for (uint32_t c = 0; c < nc; ++c)
+        if (counts[c] != 0) nz.put(c, true);
     for (uint32_t i = 0; i < nc; ++i) {
         if (counts[i] != 0) continue;
-        uint32_t sc = (i + 1) % nc;
-        for (;;) {
-            double pick = (double)counts[sc] / (double)n;
-            if (splitmix_unit(rng) < pick) break;
-            sc = (sc + 1) % nc;
-        }
+        const uint32_t sc = split_pick(i, nc, n, [&](uint32_t c) { return counts[c]; }, nz, rng);
         for (uint32_t j = 0; j < dim; ++j) {
             float src = C[(size_t)sc * ldc + j];
             if (j % 2 == 0) {
@@ -1868,6 +2142,8 @@ void split_centers_host(float *C, uint32_t ldc, uint32_t nc, uint32_t dim, uint6
         }
         counts[i] = counts[sc] / 2;
         counts[sc] -= counts[i];
+        nz.put(i, counts[i] != 0);
+        nz.put(sc, counts[sc] != 0);
     }
 }
 

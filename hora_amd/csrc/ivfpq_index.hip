@@ -678,7 +678,7 @@ class IVFPQ final : public Index {
         }
         // Kmeans::train (kmeans.rs:195-207)
         launch_coarse_init(fp64, base_.as<float>(), dpad_, n, (uint32_t)dim_, nlist_,
-                           centers_.as<float>(), dpad_, s);
+                           centers_.as<float>(), dpad_, seg_ws_, s);
         uint64_t rng = host_stream_seed(p_.seed, 0);
         std::vector<float> hc((size_t)nlist_ * dpad_);
         std::vector<uint32_t> hoff(nlist_ + 1);
@@ -686,7 +686,7 @@ class IVFPQ final : public Index {
         for (uint32_t ep = 0; ep < epochs; ++ep) {
             assign_all(s);
             launch_update(fp64, base_.as<float>(), dpad_, off_.as<uint32_t>(), ids_.as<uint32_t>(),
-                          nlist_, (uint32_t)dim_, centers_.as<float>(), dpad_, s);
+                          nlist_, (uint32_t)dim_, centers_.as<float>(), dpad_, n, seg_ws_, s);
             if (ep + 1 < epochs) {
                 HM_HIP(hipMemcpyAsync(hoff.data(), off_.get(), (nlist_ + 1) * 4,
                                       hipMemcpyDeviceToHost, s));
@@ -705,18 +705,28 @@ class IVFPQ final : public Index {
         HM_HIP(hipStreamSynchronize(s));
         max_list_ = 0;
         for (uint32_t c = 0; c < nlist_; ++c) max_list_ = std::max(max_list_, hoff[c + 1] - hoff[c]);
-        // per-list residual PQ (pq_idx.rs:376-393)
-        DeviceBuffer R, asg;
+        // per-list residual PQ (pq_idx.rs:376-393); workgroups take the
+        // lists longest first
+        std::vector<uint32_t> lorder(nlist_);
+        for (uint32_t c = 0; c < nlist_; ++c) lorder[c] = c;
+        std::stable_sort(lorder.begin(), lorder.end(), [&](uint32_t a, uint32_t b) {
+            return hoff[a + 1] - hoff[a] > hoff[b + 1] - hoff[b];
+        });
+        DeviceBuffer R, asg, ord, dlorder;
         R.reserve(n * dpad_ * 4);
         launch_residuals(base_.as<float>(), dpad_, ids_.as<uint32_t>(), lists_.as<uint32_t>(),
                          centers_.as<float>(), n, dpad_, R.as<float>(), s);
         codebooks_.reserve((size_t)nlist_ * M_ * KS_ * sdm_ * 4);
         codes_.reserve(n * M_);
         asg.reserve((size_t)M_ * n * 2);
-        launch_pq_train(fp64, R.as<float>(), dpad_, off_.as<uint32_t>(), nlist_, M_,
-                        sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), KS_, sd_max_, sdm_, epochs,
-                        p_.seed, max_list_, n, asg.as<uint16_t>(), codebooks_.as<float>(),
-                        codes_.as<uint8_t>(), s);
+        ord.reserve((size_t)M_ * n * 4);
+        dlorder.reserve((size_t)nlist_ * 4);
+        HM_HIP(hipMemcpyAsync(dlorder.get(), lorder.data(), (size_t)nlist_ * 4,
+                              hipMemcpyHostToDevice, s));
+        launch_pq_train(fp64, R.as<float>(), dpad_, off_.as<uint32_t>(), dlorder.as<uint32_t>(),
+                        nlist_, M_, sub_b_.as<uint32_t>(), sub_e_.as<uint32_t>(), KS_, sd_max_,
+                        sdm_, epochs, p_.seed, max_list_, n, asg.as<uint16_t>(),
+                        ord.as<uint32_t>(), codebooks_.as<float>(), codes_.as<uint8_t>(), s);
         HM_HIP(hipStreamSynchronize(s));
         if (norms)
             launch_self_dot(centers_.as<float>(), dpad_, nlist_, (uint32_t)dim_, cnorm_.as<float>(),
@@ -756,6 +766,7 @@ class IVFPQ final : public Index {
     uint32_t last_chunks_ = 0, last_kc_ = 0;
     bool last_learned_ = false;
     DeviceBuffer xq16_, Zq_, Pq_, nprobe_q_;
+    SegScratch seg_ws_;  // fp64 centre steps
 };
 
 }  // namespace

@@ -335,14 +335,13 @@ __global__ void pq_split_kernel(float *__restrict__ cb, uint32_t M, uint32_t KS,
     uint32_t *cnt = counts + (size_t)s * KS;
     uint64_t st = rng[s];
     const float EPS = 1.0f / 1024.0f;
+    uint64_t words[4] = {0, 0, 0, 0};  // KS <= 256
+    NonzeroSet nz{words, KS};
+    for (uint32_t c = 0; c < KS; ++c)
+        if (cnt[c] != 0) nz.put(c, true);
     for (uint32_t i = 0; i < KS; ++i) {
         if (cnt[i] != 0) continue;
-        uint32_t sc = (i + 1) % KS;
-        for (;;) {
-            double pick = (double)cnt[sc] / (double)n;
-            if (splitmix_unit(&st) < pick) break;
-            sc = (sc + 1) % KS;
-        }
+        const uint32_t sc = split_pick(i, KS, n, [&](uint32_t c) { return cnt[c]; }, nz, &st);
         for (uint32_t j = 0; j < sd; ++j) {
             float src = cent[(size_t)sc * sdm + j];
             if (j % 2 == 0) {
@@ -355,6 +354,8 @@ __global__ void pq_split_kernel(float *__restrict__ cb, uint32_t M, uint32_t KS,
         }
         cnt[i] = cnt[sc] / 2;
         cnt[sc] -= cnt[i];
+        nz.put(i, cnt[i] != 0);
+        nz.put(sc, cnt[sc] != 0);
     }
     rng[s] = st;
 }
