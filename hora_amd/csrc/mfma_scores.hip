@@ -275,37 +275,82 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
         }
         return;
     }
+    if constexpr (FILTER) {
+        // Validity is folded into the operands instead of a mask per key:
+        // a row past nq compares against -inf, and a column past n gets a
+        // key of +inf (Euclidean: cv = +inf; other metrics: a select).  The
+        // four keys of a row are tested through their minimum, so the common
+        // case costs one branch per four keys.  Euclidean FP8 computes
+        // cv - (2 rs cs) acc, which equals cv - 2 (acc (rs cs)): scaling by 2
+        // is exact.
+        constexpr float PINF = __builtin_huge_valf();
+        float cvv[4];
+        bool cok[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+        for (int j = 0; j < 4; ++j) {
+            cok[j] = colc[j] < n;
+            cvv[j] = (MET == HM_METRIC_EUCLIDEAN && !cok[j]) ? PINF : colv[j];
+        }
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const uint32_t q = q0 + wr * 64 + i * 16 + lk * 4 + e;
-            const bool qok = q < nq;
+        for (int i = 0; i < 4; ++i) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                float s = acc[i][j][e];
-                if (ELEM == 1) s = s * (rows_[i][e] * cols[j]);
-                float key;
-                if constexpr (MET == HM_METRIC_EUCLIDEAN) key = colv[j] - 2.0f * s;
-                else if constexpr (MET == HM_METRIC_DOT_PRODUCT) key = s;
-                else if constexpr (MET == HM_METRIC_COSINE_SIMILARITY) key = -s * colv[j];
-                else key = s * colv[j];
-                const uint32_t c = colc[j];
-                const bool ok = qok && c < n;
-                if constexpr (FILTER) {
-                    if (ok && key < rowt[i][e]) {  // rare: a few per query per tile row
-                        const uint32_t pos = myc < ST_PT ? tid * ST_PT + myc
-                                                         : 256 * ST_PT + atomicAdd(&s_nhit, 1u);
-                        if (pos < STAGE_CAP) {
-                            stage_v[pos] = ((uint64_t)fkey(key) << 32) | c;
-                            stage_q[pos] = q;
-                        } else {
-                            atomicAdd(out.cnt + q, out.cap + 1);  // mark the list overflowed
-                        }
-                        ++myc;
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t q = q0 + wr * 64 + i * 16 + lk * 4 + e;
+                const float rt = q < nq ? rowt[i][e] : -PINF;
+                float key[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float a = acc[i][j][e];
+                    if constexpr (MET == HM_METRIC_EUCLIDEAN) {
+                        if constexpr (ELEM == 1) key[j] = cvv[j] - ((2.0f * rows_[i][e]) * cols[j]) * a;
+                        else key[j] = cvv[j] - 2.0f * a;
+                    } else {
+                        const float s = ELEM == 1 ? a * (rows_[i][e] * cols[j]) : a;
+                        float k;
+                        if constexpr (MET == HM_METRIC_DOT_PRODUCT) k = s;
+                        else if constexpr (MET == HM_METRIC_COSINE_SIMILARITY) k = -s * cvv[j];
+                        else k = s * cvv[j];
+                        key[j] = cok[j] ? k : PINF;
                     }
-                } else {
-                    if (ok) out.Kmat[(size_t)q * out.ldk + c] = key;
+                }
+                // fminf drops a NaN key, which never compares below rt anyway
+                const float mn = fminf(fminf(key[0], key[1]), fminf(key[2], key[3]));
+                if (mn < rt) {  // rare: a few per query per tile row
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (key[j] < rt) {
+                            const uint32_t pos = myc < ST_PT ? tid * ST_PT + myc
+                                                             : 256 * ST_PT + atomicAdd(&s_nhit, 1u);
+                            if (pos < STAGE_CAP) {
+                                stage_v[pos] = ((uint64_t)fkey(key[j]) << 32) | colc[j];
+                                stage_q[pos] = q;
+                            } else {
+                                atomicAdd(out.cnt + q, out.cap + 1);  // mark the list overflowed
+                            }
+                            ++myc;
+                        }
+                    }
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t q = q0 + wr * 64 + i * 16 + lk * 4 + e;
+                const bool qok = q < nq;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    float s = acc[i][j][e];
+                    if (ELEM == 1) s = s * (rows_[i][e] * cols[j]);
+                    float key;
+                    if constexpr (MET == HM_METRIC_EUCLIDEAN) key = colv[j] - 2.0f * s;
+                    else if constexpr (MET == HM_METRIC_DOT_PRODUCT) key = s;
+                    else if constexpr (MET == HM_METRIC_COSINE_SIMILARITY) key = -s * colv[j];
+                    else key = s * colv[j];
+                    const uint32_t c = colc[j];
+                    if (qok && c < n) out.Kmat[(size_t)q * out.ldk + c] = key;
                 }
             }
         }
