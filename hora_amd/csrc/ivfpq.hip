@@ -313,7 +313,8 @@ __global__ __launch_bounds__(PQ_THREADS) void pq_train_kernel(
         for (uint32_t e = tid; e < KS * sdm; e += PQ_THREADS) cb[e] = 0.0f;
         return;
     }
-    float *cent = reinterpret_cast<float *>(pq_smem);        // KS * SDM
+    double *pk = reinterpret_cast<double *>(pq_smem);         // KS (split)
+    float *cent = reinterpret_cast<float *>(pk + KS);         // KS * SDM
     uint32_t *cnt = reinterpret_cast<uint32_t *>(cent + KS * SDM);  // KS
     uint32_t *start = cnt + KS;                              // KS
     uint32_t *run = start + KS;                              // KS
@@ -443,15 +444,20 @@ __global__ __launch_bounds__(PQ_THREADS) void pq_train_kernel(
         // generator advances by the steps taken -- the same draws and the
         // same picks as the step-by-step walk.
         if (ep + 1 < epochs && w == 0) {
+            // pk[c] = count / L, the walk's pick probability, updated as
+            // the counts change
+            for (uint32_t c = lane; c < KS; c += 64) pk[c] = (double)cnt[c] / (double)L;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             for (uint32_t i = 0; i < KS; ++i) {
                 if (cnt[i] != 0) continue;
                 uint32_t steps = 0;
                 for (uint32_t t0 = 0;; t0 += 64) {
                     const uint32_t t = t0 + lane;
                     const uint32_t c = (i + 1 + t) % KS;
-                    const uint32_t n_c = cnt[c];
-                    const bool hit =
-                        n_c != 0 && splitmix_unit_at(rng, t) < (double)n_c / (double)L;
+                    const double pc = pk[c];
+                    const bool hit = pc != 0.0 && splitmix_unit_at(rng, t) < pc;
                     const uint64_t m = __ballot(hit);
                     if (m) {
                         steps = t0 + (uint32_t)__builtin_ctzll(m);
@@ -475,6 +481,8 @@ __global__ __launch_bounds__(PQ_THREADS) void pq_train_kernel(
                     const uint32_t half = cnt[sc] / 2;
                     cnt[i] = half;
                     cnt[sc] -= half;
+                    pk[i] = (double)half / (double)L;
+                    pk[sc] = (double)cnt[sc] / (double)L;
                 }
                 // the next walk reads these counts from other lanes
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1770,8 +1778,8 @@ void pq_train_launch(const float *R, uint32_t ldr, const uint32_t *off, const ui
                      uint32_t max_list, uint64_t n, uint16_t *asg, uint32_t *ord,
                      float *codebooks, uint8_t *codes, hipStream_t s)
 {
-    constexpr size_t LDS_BUDGET = 72 * 1024;  // two workgroups per CU
-    const size_t fixed = (size_t)KS * SDM * 4 + (size_t)KS * 3 * 4 +
+    constexpr size_t LDS_BUDGET = 52 * 1024;  // three workgroups per CU
+    const size_t fixed = (size_t)KS * 8 + (size_t)KS * SDM * 4 + (size_t)KS * 3 * 4 +
                          (size_t)(PQ_THREADS / 64) * KS * 2;
     const size_t per_item = (size_t)sd_max * 4 + 4 + 2;
     uint32_t cap = fixed + 16 < LDS_BUDGET ? (uint32_t)((LDS_BUDGET - fixed - 16) / per_item) : 0;
