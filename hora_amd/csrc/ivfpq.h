@@ -133,9 +133,11 @@ void launch_dense_targets(const uint32_t *lab, uint32_t k, uint32_t rows, uint32
                           float *T, hipStream_t s);
 void launch_ids_to_lists(const uint64_t *ids, uint64_t n, const uint32_t *assign, uint64_t n_items,
                          uint32_t *out, hipStream_t s);
-// kmeans.rs:150-193 on host memory, seeded SplitMix64 (state *rng).
-void split_centers_host(float *C, uint32_t ldc, uint32_t nc, uint32_t dim, uint64_t n,
-                        uint64_t *counts, uint64_t *rng);
+// kmeans.rs:150-193 on the device: counts from the list offsets, the
+// seeded walk for each empty centre in order, the 1 +- eps copy; *rng (device)
+// carries the generator between epochs.  cnt_ws [nc] u32, pk_ws [nc] f64.
+void launch_coarse_split(float *C, uint32_t ldc, uint32_t nc, uint32_t dim, const uint32_t *off,
+                         uint64_t n, uint64_t *rng, uint32_t *cnt_ws, double *pk_ws, hipStream_t s);
 uint64_t host_stream_seed(uint64_t seed, uint64_t instance);
 
 }  // namespace hm

@@ -1668,6 +1668,79 @@ __global__ void ids_to_lists_kernel(const uint64_t *__restrict__ ids, uint64_t n
     }
 }
 
+// split_center (kmeans.rs:150-193) of the coarse k-means, on the device,
+// one 1,024-thread workgroup.  counts[c] = list sizes from off, pk[c] =
+// counts[c] / n (the walk's pick probability), both in global scratch since
+// nc may exceed LDS.  The empty centres are visited in order; the walk for
+// centre i draws once per step, step t looking at centre (i + 1 + t) % nc
+// with the generator's (t + 1)-th next draw, and stops at the first
+// u < pk.  Each round every thread takes one step, the waves' first hits
+// meet in LDS, and the generator then advances by the steps taken: the same
+// draws and picks as the step-by-step walk.  *rng carries the generator
+// from epoch to epoch.
+constexpr int CS_THREADS = 1024;
+
+__global__ __launch_bounds__(CS_THREADS) void coarse_split_kernel(
+    float *__restrict__ C, uint32_t ldc, uint32_t nc, uint32_t dim,
+    const uint32_t *__restrict__ off, uint64_t n, uint64_t *__restrict__ rng_state,
+    uint32_t *__restrict__ cnt, double *__restrict__ pk)
+{
+    __shared__ uint32_t s_first[2][CS_THREADS / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr uint32_t NW = CS_THREADS / 64;
+    for (uint32_t c = tid; c < nc; c += CS_THREADS) {
+        const uint32_t v = off[c + 1] - off[c];
+        cnt[c] = v;
+        pk[c] = (double)v / (double)n;
+    }
+    __syncthreads();
+    uint64_t rng = *rng_state;
+    const float EPS = 1.0f / 1024.0f;
+    uint32_t par = 0;
+    for (uint32_t i = 0; i < nc; ++i) {
+        if (cnt[i] != 0) continue;
+        uint32_t steps = 0;
+        for (uint32_t t0 = 0;; t0 += CS_THREADS) {
+            const uint32_t t = t0 + tid;
+            const uint32_t c = (uint32_t)(((uint64_t)i + 1 + t) % nc);
+            const double pc = pk[c];
+            const bool hit = pc != 0.0 && splitmix_unit_at(rng, t) < pc;
+            const uint64_t m = __ballot(hit);
+            if (lane == 0) s_first[par][wave] = m ? t0 + wave * 64 + (uint32_t)__builtin_ctzll(m) : ~0u;
+            __syncthreads();
+            uint32_t first = ~0u;
+#pragma unroll
+            for (uint32_t u = 0; u < NW; ++u) first = min(first, s_first[par][u]);
+            par ^= 1;  // the next round writes the other buffer
+            if (first != ~0u) {
+                steps = first;
+                break;
+            }
+        }
+        const uint32_t sc = (uint32_t)(((uint64_t)i + 1 + steps) % nc);
+        rng += ((uint64_t)steps + 1) * 0x9E3779B97F4A7C15ull;
+        for (uint32_t j = tid; j < dim; j += CS_THREADS) {
+            const float src = C[(size_t)sc * ldc + j];
+            if (j % 2 == 0) {
+                C[(size_t)i * ldc + j] = src * (1.0f - EPS);
+                C[(size_t)sc * ldc + j] = src * (1.0f + EPS);
+            } else {
+                C[(size_t)i * ldc + j] = src * (1.0f + EPS);
+                C[(size_t)sc * ldc + j] = src * (1.0f - EPS);
+            }
+        }
+        if (tid == 0) {
+            const uint32_t half = cnt[sc] / 2;
+            cnt[i] = half;
+            cnt[sc] -= half;
+            pk[i] = (double)half / (double)n;
+            pk[sc] = (double)cnt[sc] / (double)n;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *rng_state = rng;
+}
+
 inline unsigned grid_for(uint64_t work, unsigned cap = 8192)
 {
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((work + 255) / 256, cap));
@@ -2126,33 +2199,14 @@ void launch_ids_to_lists(const uint64_t *ids, uint64_t n, const uint32_t *assign
     HM_CHECK_LAUNCH();
 }
 
-void split_centers_host(float *C, uint32_t ldc, uint32_t nc, uint32_t dim, uint64_t n,
-                        uint64_t *counts, uint64_t *rng)
+void launch_coarse_split(float *C, uint32_t ldc, uint32_t nc, uint32_t dim, const uint32_t *off,
+                         uint64_t n, uint64_t *rng, uint32_t *cnt_ws, double *pk_ws, hipStream_t s)
 {
     if (n == 0) fail(HM_ERR_EMPTY, "None to assigned impossible split center");
-    const float EPS = 1.0f / 1024.0f;
-    std::vector<uint64_t> words((nc + 63) / 64, 0);
-    NonzeroSet nz{words.data(), nc};
-    for (uint32_t c = 0; c < nc; ++c)
-        if (counts[c] != 0) nz.put(c, true);
-    for (uint32_t i = 0; i < nc; ++i) {
-        if (counts[i] != 0) continue;
-        const uint32_t sc = split_pick(i, nc, n, [&](uint32_t c) { return counts[c]; }, nz, rng);
-        for (uint32_t j = 0; j < dim; ++j) {
-            float src = C[(size_t)sc * ldc + j];
-            if (j % 2 == 0) {
-                C[(size_t)i * ldc + j] = src * (1.0f - EPS);
-                C[(size_t)sc * ldc + j] = src * (1.0f + EPS);
-            } else {
-                C[(size_t)i * ldc + j] = src * (1.0f + EPS);
-                C[(size_t)sc * ldc + j] = src * (1.0f - EPS);
-            }
-        }
-        counts[i] = counts[sc] / 2;
-        counts[sc] -= counts[i];
-        nz.put(i, counts[i] != 0);
-        nz.put(sc, counts[sc] != 0);
-    }
+    if (nc == 0) return;
+    ProfScope prof("kmeans_split", s);
+    coarse_split_kernel<<<1, CS_THREADS, 0, s>>>(C, ldc, nc, dim, off, n, rng, cnt_ws, pk_ws);
+    HM_CHECK_LAUNCH();
 }
 
 uint64_t host_stream_seed(uint64_t seed, uint64_t instance) { return stream_seed(seed, instance); }

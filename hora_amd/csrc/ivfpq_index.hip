@@ -679,26 +679,22 @@ class IVFPQ final : public Index {
         // Kmeans::train (kmeans.rs:195-207)
         launch_coarse_init(fp64, base_.as<float>(), dpad_, n, (uint32_t)dim_, nlist_,
                            centers_.as<float>(), dpad_, seg_ws_, s);
-        uint64_t rng = host_stream_seed(p_.seed, 0);
-        std::vector<float> hc((size_t)nlist_ * dpad_);
-        std::vector<uint32_t> hoff(nlist_ + 1);
-        std::vector<uint64_t> cnt(nlist_);
+        // the split's generator lives on the device across epochs
+        const uint64_t rng0 = host_stream_seed(p_.seed, 0);
+        km_rng_.reserve(8);
+        km_cnt_.reserve((size_t)nlist_ * 4);
+        km_pk_.reserve((size_t)nlist_ * 8);
+        HM_HIP(hipMemcpyAsync(km_rng_.get(), &rng0, 8, hipMemcpyHostToDevice, s));
         for (uint32_t ep = 0; ep < epochs; ++ep) {
             assign_all(s);
             launch_update(fp64, base_.as<float>(), dpad_, off_.as<uint32_t>(), ids_.as<uint32_t>(),
                           nlist_, (uint32_t)dim_, centers_.as<float>(), dpad_, n, seg_ws_, s);
-            if (ep + 1 < epochs) {
-                HM_HIP(hipMemcpyAsync(hoff.data(), off_.get(), (nlist_ + 1) * 4,
-                                      hipMemcpyDeviceToHost, s));
-                HM_HIP(hipMemcpyAsync(hc.data(), centers_.get(), hc.size() * 4,
-                                      hipMemcpyDeviceToHost, s));
-                HM_HIP(hipStreamSynchronize(s));
-                for (uint32_t c = 0; c < nlist_; ++c) cnt[c] = hoff[c + 1] - hoff[c];
-                split_centers_host(hc.data(), dpad_, nlist_, (uint32_t)dim_, n, cnt.data(), &rng);
-                HM_HIP(hipMemcpyAsync(centers_.get(), hc.data(), hc.size() * 4,
-                                      hipMemcpyHostToDevice, s));
-            }
+            if (ep + 1 < epochs)
+                launch_coarse_split(centers_.as<float>(), dpad_, nlist_, (uint32_t)dim_,
+                                    off_.as<uint32_t>(), n, km_rng_.as<uint64_t>(),
+                                    km_cnt_.as<uint32_t>(), km_pk_.as<double>(), s);
         }
+        std::vector<uint32_t> hoff(nlist_ + 1);
         // search_data after train -> _ivflist (pq_idx.rs:369-375)
         assign_all(s);
         HM_HIP(hipMemcpyAsync(hoff.data(), off_.get(), (nlist_ + 1) * 4, hipMemcpyDeviceToHost, s));
@@ -766,7 +762,8 @@ class IVFPQ final : public Index {
     uint32_t last_chunks_ = 0, last_kc_ = 0;
     bool last_learned_ = false;
     DeviceBuffer xq16_, Zq_, Pq_, nprobe_q_;
-    SegScratch seg_ws_;  // fp64 centre steps
+    SegScratch seg_ws_;                 // fp64 centre steps
+    DeviceBuffer km_rng_, km_cnt_, km_pk_;  // coarse split: generator, counts, picks
 };
 
 }  // namespace
