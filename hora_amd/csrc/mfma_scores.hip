@@ -17,6 +17,8 @@
 // register staging).  The filter form writes no key matrix: keys below a
 // per-query threshold are appended to per-query candidate lists in the
 // epilogue.
+#include <type_traits>
+
 #include "kernels.h"
 #include "profiler.h"
 
@@ -89,7 +91,7 @@ __device__ __forceinline__ void tile_of(uint32_t L, uint32_t TX, uint32_t TY, ui
 // MIN4 (key matrix form only): each lane writes the minimum of its four keys
 // of a row (columns lr + 16 j of the wave's 64), at column
 // btile * 32 + wc * 16 + lr of a [nq][ceil(n / 128) * 32] matrix.
-template <int ELEM, int MET, bool FILTER, bool MIN4 = false>  // ELEM: 2 = bf16, 1 = fp8
+template <int ELEM, int MET, bool FILTER, bool MIN4 = false, bool OFF32 = true>  // ELEM: 2 = bf16, 1 = fp8
 __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
     const uint8_t *__restrict__ Q, uint32_t nq, uint32_t TY, uint32_t G,
     const uint8_t *__restrict__ B, uint32_t n, uint32_t row_bytes, const float *__restrict__ qscale,
@@ -127,83 +129,97 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
+    // Loop-invariant addressing, so that a K step spends no VALU on it:
+    //  - each lane's four load rows as 32-bit byte offsets from the operand
+    //    base (OFF32: the launcher checks that both operands span < 4 GiB),
+    //    the step's k offset added to the wave-uniform base;
+    //  - the LDS destination of each load from the wave index in an SGPR;
+    //  - each lane's fragment read offsets within a buffer; the K loop is
+    //    unrolled by two, so the buffer is a constant in each half.
+    const uint32_t wave_s = (uint32_t)__builtin_amdgcn_readfirstlane(wave);
+    uint32_t qo[4], bo[4];
     const uint8_t *qp[4], *bp[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         const uint32_t r = 8 * (4 * wave + c) + lane / 8;
         const uint32_t g = (uint32_t)(lane % 8) ^ swz(r);
         const uint32_t qa = q0 + r < nq ? q0 + r : nq - 1, ba = n0 + r < n ? n0 + r : n - 1;
-        qp[c] = Q + (size_t)qa * row_bytes + 16 * g;
-        bp[c] = B + (size_t)ba * row_bytes + 16 * g;
+        if constexpr (OFF32) {
+            qo[c] = qa * row_bytes + 16 * g;
+            bo[c] = ba * row_bytes + 16 * g;
+        } else {
+            qp[c] = Q + (size_t)qa * row_bytes + 16 * g;
+            bp[c] = B + (size_t)ba * row_bytes + 16 * g;
+        }
     }
     auto issue = [&](int buf, uint32_t kb) {
+        const uint8_t *Qk = Q + kb, *Bk = B + kb;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            __builtin_amdgcn_global_load_lds(qp[c] + kb, &sA[buf][(4 * wave + c) * 1024], 16, 0, 0);
-            __builtin_amdgcn_global_load_lds(bp[c] + kb, &sB[buf][(4 * wave + c) * 1024], 16, 0, 0);
+            const uint8_t *src_q = OFF32 ? Qk + qo[c] : qp[c] + kb;
+            const uint8_t *src_b = OFF32 ? Bk + bo[c] : bp[c] + kb;
+            __builtin_amdgcn_global_load_lds(src_q, &sA[buf][(4 * wave_s + c) * 1024], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(src_b, &sB[buf][(4 * wave_s + c) * 1024], 16, 0, 0);
         }
     };
-    // 16 bytes of row `row` at global chunk index ch
-    auto chunk = [&](const uint8_t *S, uint32_t row, uint32_t ch) {
-        return *reinterpret_cast<const uint4 *>(S + row * ROWB + 16 * (ch ^ swz(row)));
-    };
+    const int lr = lane & 15, lk = lane >> 4;
+    // fragment offsets: h = 0, 1 are the lane's two 16-byte chunks of a row
+    // (FP8: chunks 2 lk, 2 lk + 1; BF16: chunks lk and 4 + lk)
+    uint32_t fa[4][2], fb[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t ch = ELEM == 1 ? 2 * lk + h : 4 * h + lk;
+            const uint32_t ra = wr * 64 + i * 16 + lr, rb = wc * 64 + i * 16 + lr;
+            fa[i][h] = ra * ROWB + 16 * (ch ^ swz(ra));
+            fb[i][h] = rb * ROWB + 16 * (ch ^ swz(rb));
+        }
 
     const uint32_t nk = row_bytes / ROWB;
-    const int lr = lane & 15, lk = lane >> 4;
-    issue(0, 0u);
-    for (uint32_t kt = 0; kt < nk; ++kt) {
-        const int buf = kt & 1;
+    // Software pipeline: step kt reads its fragments from LDS buffer kt & 1,
+    // then issues the direct loads of step kt+1 into the other buffer, then
+    // runs its MFMAs, so each load has one step of matrix work to land.
+    auto step = [&](auto bufc, uint32_t kt) {
+        constexpr int buf = decltype(bufc)::value;
         __builtin_amdgcn_s_waitcnt(0);  // this wave's loads of step kt have landed
         __syncthreads();                // ... and every wave's; buffer buf ^ 1 is free
         const uint8_t *As = sA[buf], *Bs = sB[buf];
-        if constexpr (ELEM == 2) {
-            // two MFMA K=32 per step; lane holds 8 bf16 at k = 32*kk + 8*lk,
-            // bytes 64*kk + 16*lk: chunk 4*kk + lk
-            bf16x8 a[2][4], b[2][4];
+        uint4 av[4][2], bv[4][2];
 #pragma unroll
-            for (int kk = 0; kk < 2; ++kk) {
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const uint4 v = chunk(As, wr * 64 + i * 16 + lr, 4 * kk + lk);
-                    a[kk][i] = __builtin_bit_cast(bf16x8, v);
-                }
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const uint4 v = chunk(Bs, wc * 64 + j * 16 + lr, 4 * kk + lk);
-                    b[kk][j] = __builtin_bit_cast(bf16x8, v);
-                }
+            for (int h = 0; h < 2; ++h) {
+                av[i][h] = *reinterpret_cast<const uint4 *>(As + fa[i][h]);
+                bv[i][h] = *reinterpret_cast<const uint4 *>(Bs + fb[i][h]);
             }
-            if (kt + 1 < nk) issue(buf ^ 1, (kt + 1) * ROWB);
+        if (kt + 1 < nk) issue(buf ^ 1, (kt + 1) * ROWB);
+        if constexpr (ELEM == 2) {
+            // two MFMA K=32 per step; lane holds 8 bf16 at k = 32*kk + 8*lk
 #pragma unroll
             for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kk][i], b[kk][j],
-                                                                            acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf16x8, av[i][kk]),
+                            __builtin_bit_cast(bf16x8, bv[j][kk]), acc[i][j], 0, 0, 0);
         } else {
             // one MFMA K=128 per step (v_mfma_f32_16x16x128_f8f6f4, e4m3 x
-            // e4m3, no block scale); lane holds 32 fp8 at k = 32*lk, chunks
-            // 2*lk and 2*lk + 1.  A and B use the same k packing, so the
-            // products pair up for any order the hardware assigns within a
-            // lane.
+            // e4m3, no block scale); lane holds 32 fp8 at k = 32*lk.  A and B
+            // use the same k packing, so the products pair up for any order
+            // the hardware assigns within a lane.
             i32x8 a[4], b[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const uint32_t row = wr * 64 + i * 16 + lr;
-                const uint4 lo = chunk(As, row, 2 * lk), hi = chunk(As, row, 2 * lk + 1);
+                const uint4 lo = av[i][0], hi = av[i][1];
                 a[i] = i32x8{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w,
                              (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
+                const uint4 lb = bv[i][0], hb = bv[i][1];
+                b[i] = i32x8{(int)lb.x, (int)lb.y, (int)lb.z, (int)lb.w,
+                             (int)hb.x, (int)hb.y, (int)hb.z, (int)hb.w};
             }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const uint32_t row = wc * 64 + j * 16 + lr;
-                const uint4 lo = chunk(Bs, row, 2 * lk), hi = chunk(Bs, row, 2 * lk + 1);
-                b[j] = i32x8{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w,
-                             (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
-            }
-            if (kt + 1 < nk) issue(buf ^ 1, (kt + 1) * ROWB);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -211,6 +227,11 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
                     acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
                         a[i], b[j], acc[i][j], 0, 0, 0, 0, 0, 0);
         }
+    };
+    issue(0, 0u);
+    for (uint32_t kt = 0; kt < nk; kt += 2) {
+        step(std::integral_constant<int, 0>{}, kt);
+        if (kt + 1 < nk) step(std::integral_constant<int, 1>{}, kt + 1);
     }
 
     // Filter hits are staged in LDS (the operand buffers are free once every
@@ -481,10 +502,16 @@ static void score_launch(int elem_bytes, const void *Q, uint32_t nq, const void 
     // base tiles per group: about 2 MiB of base rows per XCD's L2
     const uint32_t G = (uint32_t)std::min<uint64_t>(
         32, std::max<uint64_t>(2, (2u << 20) / ((uint64_t)GN * row_bytes)));
-#define HM_SG(E, M)                                                                           \
-    score_gemm_kernel<E, M, FILTER, MIN4><<<grid, 256, 0, s>>>(                                     \
+    // 32-bit per-lane load offsets when both operands span less than 4 GiB
+    const bool off32 = (uint64_t)nq * row_bytes <= 0xFFFFFFFFull &&
+                       (uint64_t)n * row_bytes <= 0xFFFFFFFFull;
+#define HM_SG2(E, M, O)                                                                       \
+    score_gemm_kernel<E, M, FILTER, MIN4, O><<<grid, 256, 0, s>>>(                            \
         (const uint8_t *)Q, nq, (uint32_t)TY, G, (const uint8_t *)B, n, row_bytes, qscale,     \
         bscale, bnorm2, inv_bnorm, out, dev_rows)
+#define HM_SG(E, M)                                                                           \
+    if (off32) HM_SG2(E, M, true);                                                            \
+    else HM_SG2(E, M, false)
 #define HM_SG_METRIC(E)                                                                       \
     switch (metric) {                                                                         \
     case HM_METRIC_EUCLIDEAN: HM_SG(E, HM_METRIC_EUCLIDEAN); break;                           \
@@ -500,6 +527,7 @@ static void score_launch(int elem_bytes, const void *Q, uint32_t nq, const void 
     }
 #undef HM_SG_METRIC
 #undef HM_SG
+#undef HM_SG2
     HM_CHECK_LAUNCH();
 }
 
