@@ -461,17 +461,19 @@ def main_pq(args):
     cpu = None
     if not args.no_cpu_baseline:
         ns = min(args.cpu_sample, NQ)
+        hc = host_cpu()
         t0 = time.perf_counter()
         oracle.pqindex_search(q[:ns], oracle.EUCLIDEAN, ex["codebook"], ex["codes"], K,
-                              nthreads=16)
+                              nthreads=hc["threads"])
         t_all = time.perf_counter() - t0
         t0 = time.perf_counter()
         oracle.pqindex_search(q[:10], oracle.EUCLIDEAN, ex["codebook"], ex["codes"], K,
                               nthreads=1)
         t_one = time.perf_counter() - t0
-        cpu = {"value": round(ns / t_all, 1), "unit": "QPS", "cores": 16, "kind": "port",
-               "sample": f"first {ns} queries on 16 threads (single thread: "
-                         f"{round(10 / t_one, 1)} QPS on 10 queries)"}
+        cpu = {"value": round(ns / t_all, 1), "unit": "QPS", "cores": hc["threads"],
+               "kind": "port",
+               "sample": f"first {ns} queries on {hc['threads']} threads (single thread: "
+                         f"{round(10 / t_one, 1)} QPS on 10 queries)", "host": hc}
     print(json.dumps({"metric": "QPS, 1 MI355X, PQIndex ADC", "value": round(args.steps * NQ / wall, 1),
                       "unit": "QPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                       "ms_per_step": round(ev * 1e3 / args.steps, 3), "higher_is_better": True,
@@ -536,14 +538,16 @@ def main_knn_graph(args):
     cpu = None
     if not args.no_cpu_baseline:
         ns = 3000
+        hc = host_cpu()
         t0 = time.perf_counter()
-        oracle.knn_graph(x[:ns], K, oracle.EUCLIDEAN, nthreads=16)
+        oracle.knn_graph(x[:ns], K, oracle.EUCLIDEAN, nthreads=hc["threads"])
         t = time.perf_counter() - t0
         pairs_per_s = ns * (ns - 1) / t
-        cpu = {"value": round(pairs_per_s / (N - 1), 1), "unit": "graph rows/s", "cores": 16,
-               "kind": "port",
-               "sample": f"graph of the first {ns} rows on 16 threads ({pairs_per_s:.3g} "
-                         f"pairs/s), scaled to rows of {N - 1} pairs"}
+        cpu = {"value": round(pairs_per_s / (N - 1), 1), "unit": "graph rows/s",
+               "cores": hc["threads"], "kind": "port",
+               "sample": f"graph of the first {ns} rows on {hc['threads']} threads "
+                         f"({pairs_per_s:.3g} pairs/s), scaled to rows of {N - 1} pairs",
+               "host": hc}
     print(json.dumps({"metric": "graph rows/s, 1 MI355X, naive_build_knn_graph",
                       "value": round(N / wall, 1), "unit": "graph rows/s", "n_gpus": 1,
                       "steps": args.steps, "warmup": args.warmup,
@@ -608,11 +612,14 @@ def main_distances(args):
     cpu = None
     if not args.no_cpu_baseline:
         ns = 400
+        hc = host_cpu()
         t0 = time.perf_counter()
-        oracle.distance_rows(x, q[:ns], oracle.EUCLIDEAN, nthreads=16)
+        oracle.distance_rows(x, q[:ns], oracle.EUCLIDEAN, nthreads=hc["threads"])
         tc = time.perf_counter() - t0
-        cpu = {"value": round(ns * N / tc, 1), "unit": "distance pairs/s", "cores": 16,
-               "kind": "port", "sample": f"{ns} queries x all {N} nodes on 16 threads"}
+        cpu = {"value": round(ns * N / tc, 1), "unit": "distance pairs/s",
+               "cores": hc["threads"], "kind": "port",
+               "sample": f"{ns} queries x all {N} nodes on {hc['threads']} threads",
+               "host": hc}
     print(json.dumps({"metric": "distance pairs/s, 1 MI355X, graph-search distance service",
                       "value": round(args.steps * pairs / wall, 1), "unit": "distance pairs/s",
                       "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -769,8 +769,9 @@ class BruteForce final : public Index {
     // candidate list of every row whose key is below a per-query threshold
     // tau, so no [queries x n] key matrix is written or scanned.  tau is the
     // m-th smallest key over a strided sample of ns base rows, sized so that
-    // about 2S rows fall below it (m ~ 8 sample rows, a relative spread of
-    // about 1/sqrt(8) in the list length).  A list shorter than k+1 or
+    // about T = min(2S, cap / 2) rows fall below it (m = clamp(T / 128, 8,
+    // 32) sample rows, a relative spread of about 1/sqrt(m) in the list
+    // length).  A list shorter than k+1 or
     // longer than its capacity cannot be certified and its query reruns on
     // the exact path, so the estimate affects speed only.
     struct FilterPlan {
@@ -780,13 +781,24 @@ class BruteForce final : public Index {
     {
         FilterPlan p;
         if (!use_filter_ || n_ == 0) return p;
-        const uint64_t T = 2ull * S;
-        uint64_t ns = round_up((8 * n_ + T - 1) / T, 128);
+        const uint32_t cap =
+            (uint32_t)std::min<uint64_t>(8192, std::max<uint64_t>(1024, round_up(8ull * S, 256)));
+        // about 2S hits, but at most half the capacity, so that the spread of
+        // the estimate rarely overflows a list (a list needs the top k+1
+        // under tau for certification, not S entries)
+        const uint64_t T = std::min<uint64_t>(2ull * S, cap / 2);
+        // m sample rows below tau: the list length spreads by about 1/sqrt(m);
+        // the sample costs m/T of the scan, so a long list affords more
+        const uint64_t mt = std::min<uint64_t>(32, std::max<uint64_t>(8, T / 128));
+        uint64_t ns = round_up((mt * n_ + T - 1) / T, 128);
         ns = std::max<uint64_t>(ns, 1024);
         if (ns * 4 > n_) return p;  // a sample this large does not pay
         p.ns = (uint32_t)ns;
-        p.m = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((T * ns + n_ - 1) / n_, 1), ns);
-        p.cap = (uint32_t)std::min<uint64_t>(8192, std::max<uint64_t>(1024, round_up(8ull * S, 256)));
+        // at most 32, so that the threshold can come from the minima of the
+        // sample keys in fours
+        p.m = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((T * ns + n_ - 1) / n_, 1),
+                                           std::min<uint64_t>(ns, 32));
+        p.cap = cap;
         return p;
     }
 
