@@ -86,10 +86,13 @@ def lib():
     L.hm_index_name.argtypes = [vp]
     L.hm_index_name.restype = C.c_char_p
     L.hm_index_clear.argtypes = [vp]
+    L.hm_index_rebuild.argtypes = [vp, i32]
+    L.hm_index_idx_info.argtypes = [vp, C.c_char_p, u64, C.POINTER(u64)]
     L.hm_index_get_rows.argtypes = [vp, vp, u64, vp]
     L.hm_index_set_nprobe.argtypes = [vp, u32]
     L.hm_ivfpq_info.argtypes = [vp, vp]
     L.hm_ivfpq_export.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.hm_ivfpq_import.argtypes = [vp, i32, vp, vp, vp, vp, vp]
     L.hm_index_dump.argtypes = [vp, C.c_char_p, vp, vp]
     L.hm_index_load.argtypes = [i32, C.c_char_p, i32, C.POINTER(vp)]
     L.hm_index_loaded_ids.argtypes = [vp, vp, vp, C.POINTER(u64)]
@@ -98,6 +101,7 @@ def lib():
     L.hm_index_distances_device.argtypes = [vp, vp, u64, vp, vp, u64, vp, vp]
     L.hm_pq_info.argtypes = [vp, vp]
     L.hm_pq_export.argtypes = [vp, vp, vp]
+    L.hm_pq_import.argtypes = [vp, i32, vp, vp, u64]
     L.hm_ivfpq_train_probe.argtypes = [vp, vp, u64, u32, u32, u32, u32, C.c_float, u64,
                                        C.POINTER(C.c_double)]
     L.hm_ivfpq_set_probe_policy.argtypes = [vp, i32, C.c_float, u32, u32]

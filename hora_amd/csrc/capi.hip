@@ -408,6 +408,31 @@ HM_API int32_t hm_index_clear(hm_index *h)
     });
 }
 
+HM_API int32_t hm_index_rebuild(hm_index *h, int32_t metric)
+{
+    (void)metric;
+    return guard([&] {
+        need(h, "index");
+        fail(HM_ERR_NOT_IMPLEMENTED, "not implement");  // ann_index.rs:70-72
+    });
+}
+
+HM_API int32_t hm_index_idx_info(const hm_index *h, char *buf, uint64_t cap, uint64_t *len)
+{
+    return guard([&] {
+        need(h, "index");
+        if (cap > 0) need(buf, "buf");
+        static const char text[] = "not implement";  // ann_index.rs:113-115
+        const uint64_t n = sizeof(text) - 1;
+        if (len) *len = n;
+        if (cap > 0) {
+            const uint64_t c = n < cap - 1 ? n : cap - 1;
+            std::memcpy(buf, text, c);
+            buf[c] = '\0';
+        }
+    });
+}
+
 HM_API int32_t hm_index_get_rows(const hm_index *h, const uint64_t *idx, uint64_t n, float *out)
 {
     return guard([&] {
@@ -450,6 +475,21 @@ HM_API int32_t hm_ivfpq_export(hm_index *h, float *centers, uint64_t *list_offse
     });
 }
 
+HM_API int32_t hm_ivfpq_import(hm_index *h, int32_t metric, const float *centers,
+                               const uint64_t *list_offsets, const uint64_t *list_ids,
+                               const float *codebooks, const uint8_t *codes)
+{
+    return guard([&] {
+        need(h, "index");
+        need(centers, "centers");
+        need(list_offsets, "list_offsets");
+        need(codebooks, "codebooks");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        ivfpq_import(h->impl.get(), metric, centers, list_offsets, list_ids, codebooks, codes);
+    });
+}
+
 HM_API int32_t hm_pq_info(hm_index *h, uint32_t *out8)
 {
     return guard([&] {
@@ -467,6 +507,19 @@ HM_API int32_t hm_pq_export(hm_index *h, float *codebook, uint8_t *codes)
         std::lock_guard<std::mutex> lk(h->mu);
         DeviceScope ds(h->device);
         pq_export(h->impl.get(), codebook, codes);
+    });
+}
+
+HM_API int32_t hm_pq_import(hm_index *h, int32_t metric, const float *codebook,
+                            const uint8_t *codes, uint64_t n_coded)
+{
+    return guard([&] {
+        need(h, "index");
+        need(codebook, "codebook");
+        if (n_coded > 0) need(codes, "codes");
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceScope ds(h->device);
+        pq_import(h->impl.get(), metric, codebook, codes, n_coded);
     });
 }
 

@@ -135,6 +135,7 @@ class IVFPQ final : public Index {
             fail(HM_ERR_UNKNOWN_METRIC, "unknown method");
         const uint64_t n = offsets[nlist_];
         if (n != n_) fail(HM_ERR_INVALID_ARG, "inverted lists do not cover the rows");
+        if (n > 0 && (!ids || !codes)) fail(HM_ERR_INVALID_ARG, "list ids or codes are null");
         std::vector<uint32_t> off(nlist_ + 1), id(n), asg(n, 0xFFFFFFFFu), lst(n);
         uint32_t maxl = 0;
         for (uint32_t l = 0; l < nlist_; ++l) {

@@ -270,8 +270,9 @@ class _Index:
         return bool(out.value)
 
     def rebuild(self, mt: Metric) -> None:
-        """ann_index.rs:69-71: the default implementation."""
-        raise HoraError(10, "not implement")
+        """ann_index.rs:69-71: the trait default, which none of the index
+        types overrides (raises HoraError "not implement")."""
+        check(lib().hm_index_rebuild(self._h, int(mt)))
 
     def search_batch(self, queries, k: int) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
         """node_search_k for a batch: (positions u64 [nq,k], dist f32 [nq,k],
@@ -335,8 +336,12 @@ class _Index:
         self._ids = []
 
     def idx_info(self) -> str:
-        """ann_index.rs:113-115."""
-        return "not implement"
+        """ann_index.rs:113-115: the trait default text."""
+        n = C.c_uint64()
+        check(lib().hm_index_idx_info(self._h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(int(n.value) + 1)
+        check(lib().hm_index_idx_info(self._h, buf, int(n.value) + 1, None))
+        return buf.value.decode()
 
     def dimension(self) -> int:
         out = C.c_uint64()
@@ -526,6 +531,24 @@ class IVFPQIndex(_Index):
         return {"centers": cen, "list_offsets": off, "list_ids": ids[:n], "codebooks": cb,
                 "codes": codes[:n]}
 
+    def import_trained(self, state: dict, mt: Metric) -> None:
+        """Install a trained state (the arrays of export(); codes in list
+        order) on this index, which must hold the same rows; it is then
+        built for mt (see hm_ivfpq_import)."""
+        cen = np.ascontiguousarray(state["centers"], dtype=np.float32)
+        off = np.ascontiguousarray(state["list_offsets"], dtype=np.uint64)
+        ids = np.ascontiguousarray(state["list_ids"], dtype=np.uint64)
+        cb = np.ascontiguousarray(state["codebooks"], dtype=np.float32)
+        codes = np.ascontiguousarray(state["codes"], dtype=np.uint8)
+        i = self.info()
+        nl, m, ks, sdm = i["nlist"], i["n_sub"], i["n_sub_center"], i["sub_dim_max"]
+        if cen.shape != (nl, self._dim) or off.shape != (nl + 1,) or \
+                cb.shape != (nl, m, ks, sdm) or codes.shape != (ids.shape[0], m):
+            raise HoraError(1, "trained state does not match the index shape")
+        check(lib().hm_ivfpq_import(self._h, int(mt), cen.ctypes.data, off.ctypes.data,
+                                    ids.ctypes.data if ids.size else None, cb.ctypes.data,
+                                    codes.ctypes.data if codes.size else None))
+
 
 PQIVFIndex = IVFPQIndex  # README.md:57 name
 
@@ -555,6 +578,19 @@ class PQIndex(_Index):
         codes = np.zeros((max(n, 1), m), np.uint8)
         check(lib().hm_pq_export(self._h, cb.ctypes.data, codes.ctypes.data))
         return {"codebook": cb, "codes": codes[:n]}
+
+    def import_trained(self, state: dict, mt: Metric) -> None:
+        """Install a codebook and the codes of the first len(codes) rows (the
+        arrays of export()); the index is then built for mt (see
+        hm_pq_import)."""
+        cb = np.ascontiguousarray(state["codebook"], dtype=np.float32)
+        codes = np.ascontiguousarray(state["codes"], dtype=np.uint8)
+        i = self.info()
+        m, ks, sdm = i["n_sub"], i["n_sub_center"], i["sub_dim_max"]
+        if cb.shape != (m, ks, sdm) or codes.ndim != 2 or codes.shape[1] != m:
+            raise HoraError(1, "trained state does not match the index shape")
+        check(lib().hm_pq_import(self._h, int(mt), cb.ctypes.data,
+                                 codes.ctypes.data if codes.size else None, codes.shape[0]))
 
 
 def naive_build_knn_graph(rows, mt: Metric, k: int) -> List[List[Tuple[int, float]]]:

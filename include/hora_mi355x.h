@@ -244,6 +244,18 @@ HM_API const char *hm_index_name(const hm_index *h);
 /* ANNIndex::clear (ann_index.rs:110): drops all rows and built state. */
 HM_API int32_t hm_index_clear(hm_index *h);
 
+/* ANNIndex::rebuild (ann_index.rs:66-72).  None of the three index types
+ * overrides the trait default, so this returns HM_ERR_NOT_IMPLEMENTED with
+ * the default's message "not implement" and leaves the index unchanged. */
+HM_API int32_t hm_index_rebuild(hm_index *h, int32_t metric);
+
+/* ANNIndex::idx_info (ann_index.rs:113-115).  None of the three index
+ * types overrides the trait default, so the text is the default's "not
+ * implement".  Writes at most cap bytes including the terminating NUL into
+ * buf (buf may be NULL when cap is 0); *len (optional) = the text's length
+ * without the NUL. */
+HM_API int32_t hm_index_idx_info(const hm_index *h, char *buf, uint64_t cap, uint64_t *len);
+
 /* Copies rows by index into `out` (n x dimension floats): what
  * search_nodes returns with each Node (ann_index.rs:81-84). */
 HM_API int32_t hm_index_get_rows(const hm_index *h, const uint64_t *idx, uint64_t n,
@@ -267,6 +279,16 @@ HM_API int32_t hm_ivfpq_info(hm_index *h, uint32_t *out8);
 HM_API int32_t hm_ivfpq_export(hm_index *h, float *centers, uint64_t *list_offsets,
                                uint64_t *list_ids, float *codebooks, uint8_t *codes);
 
+/* The inverse of hm_ivfpq_export: installs a trained state (the same arrays
+ * and layouts; codes in list order) on an index that holds the rows, and
+ * marks it built for `metric`.  The lists must hold rows 0..n-1 of the
+ * index exactly once.  The reference takes such state only through load
+ * (pq_idx.rs:488-523); this entry lets a caller bring centroids and
+ * codebooks trained elsewhere, as conditional parity does. */
+HM_API int32_t hm_ivfpq_import(hm_index *h, int32_t metric, const float *centers,
+                               const uint64_t *list_offsets, const uint64_t *list_ids,
+                               const float *codebooks, const uint8_t *codes);
+
 /* Standalone PQIndex shape after build: out8 = {n_sub, 2^sub_bits,
  * sub_dim_max, rows trained, trained flag, padded dimension, metric, 0}
  * (private fields of pq_idx.rs:22-45). */
@@ -277,6 +299,12 @@ HM_API int32_t hm_pq_info(hm_index *h, uint32_t *out8);
  * sub-space with fewer dims is zero padded), codes [n][n_sub] u8 (row i is
  * item i).  Either pointer may be NULL. */
 HM_API int32_t hm_pq_export(hm_index *h, float *codebook, uint8_t *codes);
+
+/* The inverse of hm_pq_export: installs the codebook and the codes of the
+ * first n_coded rows (n_coded <= rows added) and marks the index built for
+ * `metric` (the reference: load, pq_idx.rs:243-265). */
+HM_API int32_t hm_pq_import(hm_index *h, int32_t metric, const float *codebook,
+                            const uint8_t *codes, uint64_t n_coded);
 
 /* Learned probe (no reference counterpart: the reference probes a fixed
  * search_n_center lists, pq_idx.rs:424).  Trains, on the GPU, a network
