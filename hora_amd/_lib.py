@@ -80,6 +80,7 @@ def lib():
     L.hm_index_set_search_mode.argtypes = [vp, i32, u32]
     L.hm_index_search_ragged_device.argtypes = [vp, vp, u32, vp, vp, vp, vp, vp, vp]
     L.hm_index_autotune.argtypes = [vp, vp, u64, u32, C.POINTER(u32)]
+    L.hm_index_tuning.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]
     L.hm_index_search_stats.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
     L.hm_index_nodes_size.argtypes = [vp, C.POINTER(u64)]
     L.hm_index_dimension.argtypes = [vp, C.POINTER(u64)]

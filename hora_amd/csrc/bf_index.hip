@@ -612,12 +612,12 @@ class BruteForce final : public Index {
                     launch_score_min4(bf16 ? 2 : 1, rg_qlow_.get(), QC, bsamp_.get(), fp.ns,
                                       row_bytes, rg_qscale_.as<float>(), bs_scale_.as<float>(),
                                       bs_norm2_.as<float>(), bs_inv_.as<float>(), metric_,
-                                      rg_D_.as<float>(), ldk, g, dr);
+                                      rg_D_.as<float>(), ldk, g, dr, group_kib_);
                 else
                     launch_score_gemm(bf16 ? 2 : 1, rg_qlow_.get(), QC, bsamp_.get(), fp.ns,
                                       row_bytes, rg_qscale_.as<float>(), bs_scale_.as<float>(),
                                       bs_norm2_.as<float>(), bs_inv_.as<float>(), metric_,
-                                      rg_D_.as<float>(), ldk, g, dr);
+                                      rg_D_.as<float>(), ldk, g, dr, group_kib_);
                 const float *tau;
                 uint32_t tau_ld;
                 if (fp.m <= 32) {
@@ -637,7 +637,7 @@ class BruteForce final : public Index {
                                     rg_qscale_.as<float>(), bscale_.as<float>(),
                                     bnorm_.as<float>(), inv_bnorm_.as<float>(), metric_, tau,
                                     tau_ld, rg_list_.as<uint64_t>(), fp.cap,
-                                    rg_lcnt_.as<uint32_t>(), g, dr);
+                                    rg_lcnt_.as<uint32_t>(), g, dr, group_kib_);
                 launch_list_prepare(rg_list_.as<uint64_t>(), fp.cap, rg_lcnt_.as<uint32_t>(), tau,
                                     tau_ld, QC, S, rg_ci_.as<uint32_t>(), rg_cd_.as<float>(), S,
                                     rg_sq_.as<uint32_t>(), rg_ksq_.as<float>(),
@@ -649,7 +649,8 @@ class BruteForce final : public Index {
             } else {
                 launch_score_gemm(bf16 ? 2 : 1, rg_qlow_.get(), QC, blow_.get(), n, row_bytes,
                                   rg_qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
-                                  inv_bnorm_.as<float>(), metric_, rg_D_.as<float>(), ldk, g, dr);
+                                  inv_bnorm_.as<float>(), metric_, rg_D_.as<float>(), ldk, g, dr,
+                                  group_kib_);
                 launch_row_select(rg_D_.as<float>(), ldk, QC, n, S, rg_cd_.as<float>(),
                                   rg_ci_.as<uint32_t>(), S, rg_sf_.as<uint32_t>(), g, dr);
             }
@@ -896,12 +897,12 @@ class BruteForce final : public Index {
                     launch_score_min4(bf16 ? 2 : 1, qlow_.get(), rows, bsamp_.get(), fp.ns,
                                       row_bytes, qscale_.as<float>(), bs_scale_.as<float>(),
                                       bs_norm2_.as<float>(), bs_inv_.as<float>(), metric_,
-                                      D_.as<float>(), ldk, s);
+                                      D_.as<float>(), ldk, s, nullptr, group_kib_);
                 else
                     launch_score_gemm(bf16 ? 2 : 1, qlow_.get(), rows, bsamp_.get(), fp.ns,
                                       row_bytes, qscale_.as<float>(), bs_scale_.as<float>(),
                                       bs_norm2_.as<float>(), bs_inv_.as<float>(), metric_,
-                                      D_.as<float>(), ldk, s);
+                                      D_.as<float>(), ldk, s, nullptr, group_kib_);
                 const float *tau;
                 uint32_t tau_ld;
                 if (fp.m <= 32) {  // the select also zeroes the list counters
@@ -919,7 +920,8 @@ class BruteForce final : public Index {
                 launch_score_filter(bf16 ? 2 : 1, qlow_.get(), rows, blow_.get(), n, row_bytes,
                                     qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
                                     inv_bnorm_.as<float>(), metric_, tau, tau_ld,
-                                    list_.as<uint64_t>(), fp.cap, lcnt_.as<uint32_t>(), s);
+                                    list_.as<uint64_t>(), fp.cap, lcnt_.as<uint32_t>(), s, nullptr,
+                                    group_kib_);
                 launch_list_prepare(list_.as<uint64_t>(), fp.cap, lcnt_.as<uint32_t>(), tau, tau_ld,
                                     rows, S, cand_i_.as<uint32_t>(), cand_d_.as<float>(), S,
                                     sq_.as<uint32_t>(), ksq_.as<float>(), pref_.as<uint32_t>(), s);
@@ -929,7 +931,8 @@ class BruteForce final : public Index {
             } else {
                 launch_score_gemm(bf16 ? 2 : 1, qlow_.get(), rows, blow_.get(), n, row_bytes,
                                   qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
-                                  inv_bnorm_.as<float>(), metric_, D_.as<float>(), ldk, s);
+                                  inv_bnorm_.as<float>(), metric_, D_.as<float>(), ldk, s, nullptr,
+                                  group_kib_);
                 launch_row_select(D_.as<float>(), ldk, rows, n, S, cand_d_.as<float>(),
                                   cand_i_.as<uint32_t>(), S, flags_.as<uint32_t>(), s);
             }
@@ -1182,15 +1185,20 @@ class BruteForce final : public Index {
         struct Sweep {
             uint32_t &depth;
             const uint32_t saved;
+            uint32_t &group;
+            const uint32_t saved_group;
             hipEvent_t e0 = nullptr, e1 = nullptr;
             bool done = false;
             ~Sweep()
             {
                 if (e0) (void)hipEventDestroy(e0);
                 if (e1) (void)hipEventDestroy(e1);
-                if (!done) depth = saved;
+                if (!done) {
+                    depth = saved;
+                    group = saved_group;
+                }
             }
-        } sw{shortlist_, shortlist_};
+        } sw{shortlist_, shortlist_, group_kib_, group_kib_};
         HM_HIP(hipEventCreate(&sw.e0));
         HM_HIP(hipEventCreate(&sw.e1));
         hipEvent_t e0 = sw.e0, e1 = sw.e1;
@@ -1215,8 +1223,41 @@ class BruteForce final : public Index {
             }
         }
         shortlist_ = best ? best : saved;
+        // then the score tile order at that depth: base tiles per group as
+        // KiB of base rows per XCD's L2 (each timed three times, the best kept)
+        if (best) {
+            const uint32_t saved_g = group_kib_;
+            uint32_t best_g = saved_g;
+            float best_g_ms = 1e30f;
+            for (uint32_t gk : {512u, 1024u, 2048u, 3072u}) {
+                group_kib_ = gk;
+                float t = 1e30f;
+                for (int rep = 0; rep < 3; ++rep) {
+                    HM_HIP(hipEventRecord(e0, s));
+                    search_device(q.as<float>(), nq, k, oi.as<uint64_t>(), od.as<float>(),
+                                  oc.as<uint32_t>(), s);
+                    HM_HIP(hipEventRecord(e1, s));
+                    HM_HIP(hipEventSynchronize(e1));
+                    float ms = 0;
+                    HM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    t = std::min(t, ms);
+                }
+                if (t < best_g_ms) {
+                    best_g_ms = t;
+                    best_g = gk;
+                }
+            }
+            group_kib_ = best_g;
+            if (group_kib_ != saved_g) drop_ragged_graphs();  // captured with the old order
+        }
         sw.done = true;
         return shortlist_;
+    }
+
+    void tuning(uint32_t *shortlist, uint32_t *group_kib) const override
+    {
+        *shortlist = shortlist_;
+        *group_kib = group_kib_ ? group_kib_ : 2048u;
     }
 
     void search_stats(uint64_t *fallback, uint64_t *uncertified) const override
@@ -1228,6 +1269,7 @@ class BruteForce final : public Index {
   private:
     uint64_t n_ = 0, cap_ = 0;
     uint32_t dpad_ = 0;
+    uint32_t group_kib_ = 0;  // score tile order (0: the default 2,048 KiB)
     int32_t metric_ = HM_METRIC_UNKNOWN;
     DeviceBuffer sv_q_, sv_qn_;  // distance service: padded queries, their norms
     int32_t mode_ = HM_SEARCH_EXACT;

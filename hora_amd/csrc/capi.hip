@@ -259,6 +259,20 @@ HM_API int32_t hm_index_autotune(hm_index *h, const float *queries, uint64_t nq,
     });
 }
 
+HM_API int32_t hm_index_tuning(const hm_index *h, uint32_t *shortlist, uint32_t *score_group_kib)
+{
+    return guard([&] {
+        need(h, "index");
+        uint32_t sl = 0, g = 0;
+        {
+            std::lock_guard<std::mutex> lk(h->mu);
+            h->impl->tuning(&sl, &g);
+        }
+        if (shortlist) *shortlist = sl;
+        if (score_group_kib) *score_group_kib = g;
+    });
+}
+
 HM_API int32_t hm_index_set_search_mode(hm_index *h, int32_t mode, uint32_t shortlist)
 {
     return guard([&] {

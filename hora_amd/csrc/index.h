@@ -75,6 +75,13 @@ class Index {
         (void)h_queries, (void)nq, (void)k;
         return 0;
     }
+    // The shortlist depth and score tile group (KiB) searches use; 0, 0 for
+    // an index without shortlist forms.
+    virtual void tuning(uint32_t *shortlist, uint32_t *group_kib) const
+    {
+        *shortlist = 0;
+        *group_kib = 0;
+    }
     // Appends rows already in device memory (n rows of dimension floats,
     // row stride ld floats).
     virtual void add_device(const float *d_rows, uint64_t n_rows, uint64_t ld)
@@ -161,7 +168,7 @@ void launch_fill_results(uint64_t *idx, float *dist, uint32_t *count, uint64_t n
 
 struct hm_index {
     std::unique_ptr<hm::Index> impl;
-    std::mutex mu;
+    mutable std::mutex mu;  // every entry that touches the index state holds it
     int device = 0;
     // Host-entry staging buffers.
     hm::DeviceBuffer q_stage, idx_stage, dist_stage, count_stage, err_stage;

@@ -122,7 +122,8 @@ void launch_pad_rows(const float *src, uint64_t lds, float *dst, uint64_t ldd, u
 void launch_score_gemm(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
                        uint32_t row_bytes, const float *qscale, const float *bscale,
                        const float *bnorm2, const float *inv_bnorm, int metric, float *Kmat,
-                       uint64_t ldk, hipStream_t s, const uint32_t *dev_rows = nullptr);
+                       uint64_t ldk, hipStream_t s, const uint32_t *dev_rows = nullptr,
+                       uint32_t group_kib = 0);
 // As launch_score_gemm, with each group of four keys of a row (one lane's
 // columns) reduced to its minimum: Kmat[q][ceil(n / 128) * 32], ldk at
 // least that.  The m-th smallest of these minima is at least the m-th
@@ -131,7 +132,8 @@ void launch_score_gemm(int elem_bytes, const void *Q, uint32_t nq, const void *B
 void launch_score_min4(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
                        uint32_t row_bytes, const float *qscale, const float *bscale,
                        const float *bnorm2, const float *inv_bnorm, int metric, float *Kmat,
-                       uint64_t ldk, hipStream_t s, const uint32_t *dev_rows = nullptr);
+                       uint64_t ldk, hipStream_t s, const uint32_t *dev_rows = nullptr,
+                       uint32_t group_kib = 0);
 // Filter form: keys below tau[q * tau_ld] are appended to list[q][0..cap)
 // as fkey(key) << 32 | row, counted in cnt[q] (zeroed by the caller; the
 // count may exceed cap, the entries past cap are dropped).
@@ -139,7 +141,10 @@ void launch_score_filter(int elem_bytes, const void *Q, uint32_t nq, const void 
                          uint32_t row_bytes, const float *qscale, const float *bscale,
                          const float *bnorm2, const float *inv_bnorm, int metric,
                          const float *tau, uint32_t tau_ld, uint64_t *list, uint32_t cap,
-                         uint32_t *cnt, hipStream_t s, const uint32_t *dev_rows = nullptr);
+                         uint32_t *cnt, hipStream_t s, const uint32_t *dev_rows = nullptr,
+                         uint32_t group_kib = 0);
+// Score tile order: base tiles are taken in groups of about group_kib KiB of
+// base rows per XCD's L2, the query tiles inside a group (0: 2,048 KiB).
 // Strided sample of the base: dst row i = src row floor(i * n / ns).
 void launch_sample_rows(const void *src, uint32_t row_bytes, uint32_t n, uint32_t ns, void *dst,
                         hipStream_t s);

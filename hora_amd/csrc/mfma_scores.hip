@@ -490,7 +490,8 @@ template <bool FILTER, bool MIN4 = false>
 static void score_launch(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
                          uint32_t row_bytes, const float *qscale, const float *bscale,
                          const float *bnorm2, const float *inv_bnorm, int metric,
-                         const ScoreOut &out, hipStream_t s, const uint32_t *dev_rows)
+                         const ScoreOut &out, hipStream_t s, const uint32_t *dev_rows,
+                         uint32_t group_kib)
 {
     if (nq == 0 || n == 0) return;
     if (row_bytes == 0 || row_bytes % ROWB)
@@ -500,8 +501,9 @@ static void score_launch(int elem_bytes, const void *Q, uint32_t nq, const void 
     if (TX * TY > 0x7FFFFFull) fail(HM_ERR_INVALID_ARG, "score_gemm: too many tiles");
     const dim3 grid((unsigned)(TX * TY));
     // base tiles per group: about 2 MiB of base rows per XCD's L2
+    const uint64_t group_bytes = (uint64_t)(group_kib ? group_kib : 2048) << 10;
     const uint32_t G = (uint32_t)std::min<uint64_t>(
-        32, std::max<uint64_t>(2, (2u << 20) / ((uint64_t)GN * row_bytes)));
+        32, std::max<uint64_t>(2, group_bytes / ((uint64_t)GN * row_bytes)));
     // 32-bit per-lane load offsets when both operands span less than 4 GiB
     const bool off32 = (uint64_t)nq * row_bytes <= 0xFFFFFFFFull &&
                        (uint64_t)n * row_bytes <= 0xFFFFFFFFull;
@@ -534,37 +536,40 @@ static void score_launch(int elem_bytes, const void *Q, uint32_t nq, const void 
 void launch_score_gemm(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
                        uint32_t row_bytes, const float *qscale, const float *bscale,
                        const float *bnorm2, const float *inv_bnorm, int metric, float *Kmat,
-                       uint64_t ldk, hipStream_t s, const uint32_t *dev_rows)
+                       uint64_t ldk, hipStream_t s, const uint32_t *dev_rows,
+                       uint32_t group_kib)
 {
     ProfScope prof("score_gemm", s);
     ScoreOut out{Kmat, ldk, nullptr, 0, nullptr, 0, nullptr};
     score_launch<false>(elem_bytes, Q, nq, B, n, row_bytes, qscale, bscale, bnorm2, inv_bnorm,
-                        metric, out, s, dev_rows);
+                        metric, out, s, dev_rows, group_kib);
 }
 
 void launch_score_min4(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
                        uint32_t row_bytes, const float *qscale, const float *bscale,
                        const float *bnorm2, const float *inv_bnorm, int metric, float *Kmat,
-                       uint64_t ldk, hipStream_t s, const uint32_t *dev_rows)
+                       uint64_t ldk, hipStream_t s, const uint32_t *dev_rows,
+                       uint32_t group_kib)
 {
     if (ldk < (uint64_t)ceil_div(n, GN) * 32) fail(HM_ERR_INVALID_ARG, "score_min4: ldk too small");
     ProfScope prof("score_gemm", s);
     ScoreOut out{Kmat, ldk, nullptr, 0, nullptr, 0, nullptr};
     score_launch<false, true>(elem_bytes, Q, nq, B, n, row_bytes, qscale, bscale, bnorm2,
-                              inv_bnorm, metric, out, s, dev_rows);
+                              inv_bnorm, metric, out, s, dev_rows, group_kib);
 }
 
 void launch_score_filter(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
                          uint32_t row_bytes, const float *qscale, const float *bscale,
                          const float *bnorm2, const float *inv_bnorm, int metric,
                          const float *tau, uint32_t tau_ld, uint64_t *list, uint32_t cap,
-                         uint32_t *cnt, hipStream_t s, const uint32_t *dev_rows)
+                         uint32_t *cnt, hipStream_t s, const uint32_t *dev_rows,
+                         uint32_t group_kib)
 {
     if (!tau || !list || !cnt || cap == 0) fail(HM_ERR_INVALID_ARG, "score_filter: null output");
     ProfScope prof("score_filter", s);
     ScoreOut out{nullptr, 0, tau, tau_ld, list, cap, cnt};
     score_launch<true>(elem_bytes, Q, nq, B, n, row_bytes, qscale, bscale, bnorm2, inv_bnorm,
-                       metric, out, s, dev_rows);
+                       metric, out, s, dev_rows, group_kib);
 }
 
 void launch_sample_rows(const void *src, uint32_t row_bytes, uint32_t n, uint32_t ns, void *dst,

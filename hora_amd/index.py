@@ -451,10 +451,18 @@ class BruteForceIndex(_Index):
         return idx[:n, :k], dist[:n, :k], cnt[:n]
 
     def autotune(self, queries, k: int) -> int:
+        """Times the shortlist depths, then the score tile group sizes, on
+        these queries and keeps the fastest; returns the depth."""
         q = _f32_rows(queries, self._dim)
         out = C.c_uint32()
         check(lib().hm_index_autotune(self._h, q.ctypes.data, q.shape[0], k, C.byref(out)))
         return int(out.value)
+
+    def tuning(self) -> dict:
+        """The shortlist depth (0: automatic) and score tile group (KiB) in use."""
+        sl, g = C.c_uint32(), C.c_uint32()
+        check(lib().hm_index_tuning(self._h, C.byref(sl), C.byref(g)))
+        return {"shortlist": int(sl.value), "score_group_kib": int(g.value)}
 
 
 

@@ -171,9 +171,18 @@ HM_API int32_t hm_index_search_ragged_device(hm_index *h, const float *d_queries
                                              float *d_dist, uint32_t *d_count, void *stream);
 
 /* Times the shortlist forms on a host query sample for shortlist depths
- * 64..4096 and keeps the fastest (written to *shortlist). */
+ * 64..4096 and keeps the fastest (written to *shortlist); then, at that
+ * depth, the score GEMM's tile order: base tiles are taken in groups of
+ * 512, 1,024, 2,048 or 3,072 KiB of base rows per XCD's L2 (the query tiles
+ * inside a group), each timed three times, and the fastest is kept (read it
+ * with hm_index_tuning).  No reference counterpart. */
 HM_API int32_t hm_index_autotune(hm_index *h, const float *queries, uint64_t nq, uint32_t k,
                                  uint32_t *shortlist);
+
+/* The shortlist depth (0: automatic) and the score tile group size in KiB
+ * (2,048 by default) that the index's shortlist searches use; 0 and 0 for an
+ * index without shortlist forms.  Either pointer may be NULL. */
+HM_API int32_t hm_index_tuning(const hm_index *h, uint32_t *shortlist, uint32_t *score_group_kib);
 
 /* Selects the BruteForceIndex search form and its shortlist depth
  * (0 = automatic).  No reference counterpart (the reference has one form). */

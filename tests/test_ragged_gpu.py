@@ -74,11 +74,18 @@ def test_autotune_keeps_results_exact(orc):
     idx.add_batch(base)
     idx.build(Metric.DotProduct)
     idx.set_search_mode(hora_amd.HM_SEARCH_BF16_RERANK, 0)
+    sizes, ks = [100, 156], [10, 10]
+    gi0, gd0, gc0, off0 = _ragged(idx, q, sizes, ks)  # captures a graph before tuning
     s = idx.autotune(q, 10)
     assert 64 <= s <= 4096
+    t = idx.tuning()
+    assert t["shortlist"] == s and t["score_group_kib"] in (512, 1024, 2048, 3072)
     gi, gd, gc = idx.search_batch(q, 10)
     ri, rd, rc = orc.bf_search(base, q, 10, orc.DOT)
     assert np.array_equal(gi, ri) and np.array_equal(gd, rd)
+    # the ragged graphs are rebuilt for the tuned tile order, same results
+    gi1, gd1, gc1, off1 = _ragged(idx, q, sizes, ks)
+    _check(orc, base, q, sizes, ks, Metric.DotProduct, gi1, gd1, gc1, off1)
 
 
 def _tied_base(n, d, seed):
