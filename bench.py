@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-metrics", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="bf: run hm_index_autotune on 2,000 of the queries before timing")
     return ap.parse_args()
 
 
@@ -167,6 +169,12 @@ def main_bf(args):
     ids_by_form = {}
     for form in args.forms.split(","):
         ix = make(Metric.Euclidean, form)
+        tuning = None
+        if args.autotune and form != "exact":
+            # hm_index_autotune on the first 2,000 queries: depth, then tile order
+            ix.autotune(queries[:2000], K)
+            tuning = ix.tuning()
+            log(f"Euclidean/{form}: tuned {tuning}")
         wall, ev = timer.run(step_fn(ix), args.steps, args.warmup, f"Euclidean/{form}",
                              only="exact_tile" if form == "exact" else "score_filter")
         km = kernel_ms(["exact_tile", "exact_rows", "row_select", "kth_smallest", "bf_finalize",
@@ -181,6 +189,8 @@ def main_bf(args):
                                               for k, v in km.items() if v[1]},
                        "exact_fallback_rows": {"rerun": fb[0], "uncertified": fb[1]},
                        "_km": km}
+        if tuning is not None:
+            forms[form]["tuning"] = tuning
         log(f"Euclidean/{form}: {forms[form]['qps']} QPS")
         del ix
 
