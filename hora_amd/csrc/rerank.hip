@@ -124,23 +124,24 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
                 }
                 __syncthreads();
             }
-        } else {
+        } else if ((uint32_t)tid < R) {
+            // more than 64 rows: each thread streams its own row in 16-byte
+            // loads, 16 dimensions a step with the next steps' loads in
+            // flight; no LDS staging and no barriers, so the gathers of a
+            // deep shortlist keep many loads outstanding per CU
+            const float *row = Bp + (size_t)s_rows[tid] * dpad;
+#pragma unroll 2
             for (uint32_t t0 = 0; t0 < dpad; t0 += 16) {
-                for (uint32_t i = tid; i < R * 4; i += RR_THREADS) {
-                    const uint32_t row = i / 4, c = (i % 4) * 4;
-                    const float4 v =
-                        *reinterpret_cast<const float4 *>(Bp + (size_t)s_rows[row] * dpad + t0 + c);
-                    float *d = sb + row * 17 + c;
-                    d[0] = v.x;
-                    d[1] = v.y;
-                    d[2] = v.z;
-                    d[3] = v.w;
-                }
-                __syncthreads();
-                if ((uint32_t)tid < R) {
+                float4 v[4];
 #pragma unroll
-                    for (uint32_t e = 0; e < 16; ++e) {
-                        const float q = sq[t0 + e], bq = sb[tid * 17 + e];
+                for (uint32_t u = 0; u < 4; ++u)
+                    v[u] = *reinterpret_cast<const float4 *>(row + t0 + 4 * u);
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) {
+                    const float bv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                    for (uint32_t e = 0; e < 4; ++e) {
+                        const float q = sq[t0 + 4 * u + e], bq = bv[e];
                         if constexpr (ACC == ACC_EUCLID) {
                             float df = q - bq;
                             float sqr = df * df;
@@ -154,7 +155,6 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
                         }
                     }
                 }
-                __syncthreads();
             }
         }
         if ((uint32_t)tid >= R) continue;
