@@ -142,6 +142,8 @@ class BruteForce final : public Index {
         n_ = cap_ = 0;
         bnorm_valid_ = false;
         sl_valid_ = false;
+        b8_valid_ = false;
+        b8_.release();
         samp_ns_ = 0;
     }
 
@@ -486,7 +488,7 @@ class BruteForce final : public Index {
         uint32_t nch, kmax, S, ns;
         int32_t mode, metric;
         uint64_t n;
-        std::array<const void *, 9> data;  // the index buffers it reads
+        std::array<const void *, 10> data;  // the index buffers it reads
         hipGraphExec_t exec;
     };
     static constexpr size_t RAG_GRAPHS = 4;
@@ -560,9 +562,10 @@ class BruteForce final : public Index {
         // a graph reads the index's buffers where they were at capture: one
         // captured before any of them moved (add, clear, mode or sample
         // change) is dropped, never launched
-        const std::array<const void *, 9> data{base_.get(),   blow_.get(),     bscale_.get(),
-                                               bnorm_.get(),  inv_bnorm_.get(), bsamp_.get(),
-                                               bs_norm2_.get(), bs_inv_.get(),  bs_scale_.get()};
+        const std::array<const void *, 10> data{
+            base_.get(),     blow_.get(),   bscale_.get(),   bnorm_.get(),
+            inv_bnorm_.get(), bsamp_.get(), bs_norm2_.get(), bs_inv_.get(),
+            bs_scale_.get(), b8_valid_ ? b8_.get() : nullptr};
         for (size_t i = 0; i < rg_graphs_.size();) {
             if (rg_graphs_[i].data != data) {
                 (void)hipGraphExecDestroy(rg_graphs_[i].exec);
@@ -659,7 +662,8 @@ class BruteForce final : public Index {
                           bnorm_.as<float>(), QC, n, kmax, bmax_, gamma, c_exact, c_bf16,
                           rg_qinex_.as<uint32_t>(), bf16 ? certify_ : 0, rg_oi_.as<uint64_t>(),
                           rg_od_.as<float>(), rg_oc_.as<uint32_t>(), rg_rf_.as<uint32_t>(), g, dr,
-                          rg_ck_.as<uint32_t>(), rg_coff_.as<uint64_t>(), s_q, key_s, pre);
+                          rg_ck_.as<uint32_t>(), rg_coff_.as<uint64_t>(), s_q, key_s, pre, 0,
+                          b8_valid_ ? b8_.as<uint8_t>() : nullptr);
             launch_ragged_chunk_end(call, c, QC, rg_rows_.as<uint32_t>(), rg_off_.as<uint64_t>(),
                                     rg_oi_.as<uint64_t>(), rg_od_.as<float>(),
                                     rg_oc_.as<uint32_t>(), rg_rf_.as<uint32_t>(),
@@ -733,6 +737,7 @@ class BruteForce final : public Index {
         const uint32_t n = (uint32_t)n_;
         inv_bnorm_.reserve(n_ * sizeof(float));
         stat_.reserve(3 * sizeof(uint32_t));
+        b8_stat_.reserve(sizeof(uint32_t));
         HM_HIP(hipMemsetAsync(stat_.get(), 0, 3 * sizeof(uint32_t), s));
         launch_norm_stats(bnorm_.as<float>(), n, inv_bnorm_.as<float>(), stat_.as<uint32_t>(), s);
         if (mode_ == HM_SEARCH_BF16_RERANK) {
@@ -747,14 +752,24 @@ class BruteForce final : public Index {
             launch_to_fp8(base_.as<float>(), dpad_, n, (uint32_t)dim_, dk_, blow_.as<uint8_t>(),
                           bscale_.as<float>(), s);
         }
-        uint32_t st[3];
-        HM_HIP(hipMemcpyAsync(st, stat_.get(), sizeof(st), hipMemcpyDeviceToHost, s));
+        // a u8 copy of the base when every value is one (integer data such
+        // as 0..255 pixels): the rerank then gathers a quarter of the bytes
+        // and converts them back to the same f32 values
+        b8_.reserve(n_ * dpad_ + 16);
+        HM_HIP(hipMemsetAsync(b8_stat_.get(), 0, sizeof(uint32_t), s));
+        launch_to_u8_exact(base_.as<float>(), dpad_, n_, b8_.as<uint8_t>(),
+                           b8_stat_.as<uint32_t>(), s);
+        uint32_t st[4];
+        HM_HIP(hipMemcpyAsync(st, stat_.get(), 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HM_HIP(hipMemcpyAsync(st + 3, b8_stat_.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HM_HIP(hipStreamSynchronize(s));
         float bm;
         std::memcpy(&bm, &st[0], 4);
         bmax_ = bm;
         base_inexact_ = st[1] != 0;
         zero_norm_ = st[2] != 0;
+        b8_valid_ = st[3] == 0;
+        if (!b8_valid_) b8_.release();
         sl_valid_ = true;
     }
 
@@ -941,7 +956,8 @@ class BruteForce final : public Index {
                           bnorm_.as<float>(), rows, n, k, bmax_, gamma, c_exact, c_bf16,
                           qinexact_.as<uint32_t>(), bf16 ? certify_ : 0, d_idx + q0 * k,
                           d_dist + q0 * k, d_count + q0, rflags_.as<uint32_t>() + q0, s,
-                          nullptr, nullptr, nullptr, s_q, key_s, pre);
+                          nullptr, nullptr, nullptr, s_q, key_s, pre, 0,
+                          b8_valid_ ? b8_.as<uint8_t>() : nullptr);
         }
         // Rows needing the exact path (ties: heap replay; uncertified).  Up
         // to RERUN_SLOTS rows that are only tied are rerun on the device
@@ -1280,6 +1296,8 @@ class BruteForce final : public Index {
     DeviceBuffer qpad_, qnorm_, D_, cand_d_, cand_i_, flags_, err_;
     // shortlist forms
     bool sl_valid_ = false, base_inexact_ = true, zero_norm_ = false;
+    bool b8_valid_ = false;              // b8_ holds the base exactly
+    DeviceBuffer b8_, b8_stat_;          // u8 copy of the base rows [n][dpad]
     int certify_ = 1;
     uint32_t dk_ = 0;
     float bmax_ = 0;

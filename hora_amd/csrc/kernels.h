@@ -154,6 +154,11 @@ void launch_to_bf16(const float *src, uint64_t lds, uint64_t rows, uint32_t d, u
                     const uint32_t *dev_rows = nullptr);
 void launch_to_fp8(const float *src, uint64_t lds, uint32_t rows, uint32_t d, uint32_t dk,
                    uint8_t *dst, float *dequant, hipStream_t s, const uint32_t *dev_rows = nullptr);
+// dst[r][j] = (uint8_t)src[r][j] over [rows][dpad]; *inexact |= 1 when some
+// value is not bit-identical to the f32 of its u8 (non-integers, values
+// outside 0..255, -0.0): with *inexact == 0 the copy holds the same values.
+void launch_to_u8_exact(const float *src, uint32_t dpad, uint64_t rows, uint8_t *dst,
+                        uint32_t *inexact, hipStream_t s);
 // inv[r] = 1/sqrt(norm2[r]); stats[0] |= max norm (float bits, atomicMax),
 // stats[2] |= 1 when a norm is zero.  stats[0..2] zeroed by the caller.
 void launch_norm_stats(const float *norm2, uint32_t rows, float *inv, uint32_t *stats,
@@ -169,7 +174,7 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
                    const uint32_t *dev_rows = nullptr, const uint32_t *kq = nullptr,
                    const uint64_t *out_off = nullptr, const uint32_t *s_q = nullptr,
                    const float *key_s_q = nullptr, const uint32_t *pre_flags = nullptr,
-                   int by_id = 0);
+                   int by_id = 0, const uint8_t *Bp8 = nullptr);
 // Candidate lists of the filter form -> the shortlist format of rerank:
 // per query the first min(cnt, S) entries by (key, row) into cand_i /
 // cand_key (stride ldc), s_q = that count, key_s_q = a lower bound of the

@@ -466,6 +466,20 @@ __global__ void to_fp8_kernel(const float *__restrict__ src, uint64_t lds, uint3
 
 // inv[r] = 1 / sqrt(norm2[r]); stats[0] = max_r sqrt(norm2[r]) (as float
 // bits); stats[2] = 1 if some row has norm zero.
+__global__ void to_u8_exact_kernel(const float *__restrict__ src, uint64_t total,
+                                   uint8_t *__restrict__ dst, uint32_t *__restrict__ inexact)
+{
+    bool bad = false;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const float v = src[i];
+        const uint8_t u = (v >= 0.0f && v <= 255.0f) ? (uint8_t)v : (uint8_t)0;
+        bad |= __float_as_uint(v) != __float_as_uint((float)u);
+        dst[i] = u;
+    }
+    if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(inexact, 1u);
+}
+
 __global__ void norm_stats_kernel(const float *__restrict__ norm2, uint32_t rows,
                                   float *__restrict__ inv, uint32_t *stats)
 {
@@ -605,6 +619,16 @@ void launch_to_fp8(const float *src, uint64_t lds, uint32_t rows, uint32_t d, ui
 {
     if (rows == 0) return;
     to_fp8_kernel<<<ceil_div(rows, 4), 256, 0, s>>>(src, lds, rows, d, dk, dst, dequant, dev_rows);
+    HM_CHECK_LAUNCH();
+}
+
+void launch_to_u8_exact(const float *src, uint32_t dpad, uint64_t rows, uint8_t *dst,
+                        uint32_t *inexact, hipStream_t s)
+{
+    const uint64_t total = rows * dpad;
+    if (total == 0) return;
+    const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 8192);
+    to_u8_exact_kernel<<<blocks, 256, 0, s>>>(src, total, dst, inexact);
     HM_CHECK_LAUNCH();
 }
 

@@ -33,7 +33,10 @@ __device__ __forceinline__ uint32_t fkey(float f)
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
-template <int ACC, int EPI>
+// Candidate rows come from Bp (f32) or, U8, from Bp8: a u8 copy of the base
+// whose values convert to the same f32 bits (checked when it is made), a
+// quarter of the bytes to gather.
+template <int ACC, int EPI, bool U8>
 __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
     const float *__restrict__ Qp, uint32_t dpad, const float *__restrict__ Bp,
     const uint32_t *__restrict__ cand_i, const float *__restrict__ cand_key, uint32_t ldc,
@@ -43,7 +46,8 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
     float *__restrict__ out_dist, uint32_t *__restrict__ out_count, uint32_t *__restrict__ flags,
     const uint32_t *__restrict__ dev_rows, const uint32_t *__restrict__ kq,
     const uint64_t *__restrict__ out_off, const uint32_t *__restrict__ s_q,
-    const float *__restrict__ key_s_q, const uint32_t *__restrict__ pre_flags, int by_id)
+    const float *__restrict__ key_s_q, const uint32_t *__restrict__ pre_flags, int by_id,
+    const uint8_t *__restrict__ Bp8)
 {
     if (dev_rows && blockIdx.x >= *dev_rows) return;  // graph replays with fewer rows
     // dynamic LDS: keys[P] (u64) | vals[P] | sq[dpad]; P covers the largest s
@@ -88,18 +92,32 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
             // flight per slab, so a query makes dpad / 64 round trips
             float *sw = sb;
             for (uint32_t t0 = 0; t0 < dpad; t0 += 64) {
-#pragma unroll
-                for (uint32_t u = 0; u < 4; ++u) {
-                    const uint32_t i = (uint32_t)tid + RR_THREADS * u;
-                    const uint32_t row = i / 16, c = (i % 16) * 4;
+                if constexpr (U8) {
+                    // one 16-byte load per thread: 16 dimensions of one row
+                    const uint32_t row = (uint32_t)tid / 4, c = ((uint32_t)tid % 4) * 16;
                     if (row < R && t0 + c < dpad) {
-                        const float4 v = *reinterpret_cast<const float4 *>(
-                            Bp + (size_t)s_rows[row] * dpad + t0 + c);
+                        const uint4 v = *reinterpret_cast<const uint4 *>(
+                            Bp8 + (size_t)s_rows[row] * dpad + t0 + c);
+                        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
                         float *d = sw + row * 65 + c;
-                        d[0] = v.x;
-                        d[1] = v.y;
-                        d[2] = v.z;
-                        d[3] = v.w;
+#pragma unroll
+                        for (uint32_t e = 0; e < 16; ++e)
+                            d[e] = (float)((w[e / 4] >> (8 * (e % 4))) & 0xFFu);
+                    }
+                } else {
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) {
+                        const uint32_t i = (uint32_t)tid + RR_THREADS * u;
+                        const uint32_t row = i / 16, c = (i % 16) * 4;
+                        if (row < R && t0 + c < dpad) {
+                            const float4 v = *reinterpret_cast<const float4 *>(
+                                Bp + (size_t)s_rows[row] * dpad + t0 + c);
+                            float *d = sw + row * 65 + c;
+                            d[0] = v.x;
+                            d[1] = v.y;
+                            d[2] = v.z;
+                            d[3] = v.w;
+                        }
                     }
                 }
                 __syncthreads();
@@ -130,12 +148,22 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
             // flight; no LDS staging and no barriers, so the gathers of a
             // deep shortlist keep many loads outstanding per CU
             const float *row = Bp + (size_t)s_rows[tid] * dpad;
+            const uint8_t *row8 = U8 ? Bp8 + (size_t)s_rows[tid] * dpad : nullptr;
 #pragma unroll 2
             for (uint32_t t0 = 0; t0 < dpad; t0 += 16) {
                 float4 v[4];
+                if constexpr (U8) {
+                    const uint4 w = *reinterpret_cast<const uint4 *>(row8 + t0);
+                    const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-                for (uint32_t u = 0; u < 4; ++u)
-                    v[u] = *reinterpret_cast<const float4 *>(row + t0 + 4 * u);
+                    for (uint32_t u = 0; u < 4; ++u)
+                        v[u] = make_float4((float)(ww[u] & 0xFFu), (float)((ww[u] >> 8) & 0xFFu),
+                                           (float)((ww[u] >> 16) & 0xFFu), (float)(ww[u] >> 24));
+                } else {
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u)
+                        v[u] = *reinterpret_cast<const float4 *>(row + t0 + 4 * u);
+                }
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u) {
                     const float bv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
@@ -526,7 +554,8 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
                    const uint32_t *q_inexact, int certify, uint64_t *out_idx, float *out_dist,
                    uint32_t *out_count, uint32_t *flags, hipStream_t st,
                    const uint32_t *dev_rows, const uint32_t *kq, const uint64_t *out_off,
-                   const uint32_t *s_q, const float *key_s_q, const uint32_t *pre_flags, int by_id)
+                   const uint32_t *s_q, const float *key_s_q, const uint32_t *pre_flags, int by_id,
+                   const uint8_t *Bp8)
 {
     if (rows == 0) return;
     if (s == 0 || s > RR_CAP) fail(HM_ERR_INVALID_ARG, "rerank: shortlist size out of range");
@@ -534,14 +563,17 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
     ProfScope prof("rerank", st);
     uint32_t P = 1;
     while (P < s) P <<= 1;
-    if (dpad % 16 || ((uintptr_t)Bp & 15))
+    if (dpad % 16 || ((uintptr_t)Bp & 15) || ((uintptr_t)Bp8 & 15))
         fail(HM_ERR_INVALID_ARG, "rerank: rows must be 16-byte aligned and padded to 16");
     const size_t lds = (size_t)P * 12 + (size_t)dpad * 4 + (size_t)RR_THREADS * 17 * 4;
-#define HM_RR(ACC, EPI)                                                                        \
-    rerank_kernel<ACC, EPI><<<rows, RR_THREADS, lds, st>>>(                                     \
+#define HM_RR2(ACC, EPI, U8)                                                                   \
+    rerank_kernel<ACC, EPI, U8><<<rows, RR_THREADS, lds, st>>>(                                 \
         Qp, dpad, Bp, cand_i, cand_key, ldc, s, qnorm2, bnorm2, n, k, bmax, gamma, c_in_exact, \
         c_in_inexact, q_inexact, certify, out_idx, out_dist, out_count, flags, dev_rows, kq,    \
-        out_off, s_q, key_s_q, pre_flags, by_id)
+        out_off, s_q, key_s_q, pre_flags, by_id, Bp8)
+#define HM_RR(ACC, EPI)                                                                        \
+    if (Bp8) HM_RR2(ACC, EPI, true);                                                           \
+    else HM_RR2(ACC, EPI, false)
     switch (metric) {
     case HM_METRIC_MANHATTAN: HM_RR(ACC_MANHATTAN, EPI_NONE); break;  // no certification bound
     case HM_METRIC_EUCLIDEAN: HM_RR(ACC_EUCLID, EPI_NONE); break;
@@ -551,6 +583,7 @@ void launch_rerank(int metric, const float *Qp, uint32_t dpad, const float *Bp,
     default: fail(HM_ERR_INVALID_ARG, "rerank: metric has no shortlist form");
     }
 #undef HM_RR
+#undef HM_RR2
     HM_CHECK_LAUNCH();
 }
 

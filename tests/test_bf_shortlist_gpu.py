@@ -153,3 +153,24 @@ def test_filtered_scan_on_a_large_batch(orc, mt, mode):
             for j in range(10):
                 e = orc.metric(q[r], base[int(gi[r, j])], int(mt))
                 assert np.float32(e).view(np.uint32) == gd[r, j].view(np.uint32)
+
+
+@pytest.mark.parametrize("shortlist", [64, 512])
+@pytest.mark.parametrize("spoil", [None, 0.5, -0.0, 256.0])
+def test_u8_copy_of_integer_base_keeps_results_exact(orc, shortlist, spoil):
+    # 0..255 integer rows: the rerank gathers a u8 copy of the base (slab path
+    # at 64 candidates, direct loads at 512).  One value that a u8 cannot
+    # hold bit for bit (0.5, -0.0, 256) switches the copy off for the whole
+    # base.  Either way the ids and distances are the reference's.
+    base = bf_sparse_u8(12_000, 784, seed=431)
+    if spoil is not None:
+        base[7, 100] = np.float32(spoil)
+    q = bf_sparse_u8(200, 784, seed=432)
+    idx = _index(base, Metric.Euclidean, hora_amd.HM_SEARCH_BF16_RERANK, shortlist=shortlist)
+    _exact_check(orc, idx, base, q, 10, Metric.Euclidean)
+    idx.set_search_mode(hora_amd.HM_SEARCH_FP8_RERANK, shortlist)
+    gi, gd, gc = idx.search_batch(q, 10)
+    ref = orc.distance_rows(base, q, orc.EUCLIDEAN)
+    for r in range(q.shape[0]):  # every returned distance is the reference's value
+        ids = gi[r, :gc[r]].astype(np.int64)
+        assert np.array_equal(gd[r, :gc[r]].view(np.uint32), ref[r, ids].view(np.uint32))
