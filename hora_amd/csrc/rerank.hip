@@ -86,38 +86,73 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(
         s_rows[tid] = ((uint32_t)tid < R && !bad) ? idx : 0u;  // rows the loads may read
         __syncthreads();
         float acc = -0.0f;
-        if (R <= 64) {
+        if (U8 && R <= 64) {
+            // at most 64 rows of u8: 256-dimension slabs of 256 bytes per
+            // row at a stride of 260 bytes (a lane's word w of its row is in
+            // bank (lane + w) % 64: conflict-free), four 16-byte loads per
+            // thread in flight per slab, so a query makes dpad / 256 round
+            // trips; the same LDS as the [256][17] layout
+            uint32_t *sw = reinterpret_cast<uint32_t *>(sb);
+            for (uint32_t t0 = 0; t0 < dpad; t0 += 256) {
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) {
+                    const uint32_t i = (uint32_t)tid + RR_THREADS * u;
+                    const uint32_t row = i / 16, c = (i % 16) * 16;
+                    if (row < R && t0 + c < dpad) {
+                        const uint4 v = *reinterpret_cast<const uint4 *>(
+                            Bp8 + (size_t)s_rows[row] * dpad + t0 + c);
+                        uint32_t *d = sw + row * 65 + c / 4;
+                        d[0] = v.x;
+                        d[1] = v.y;
+                        d[2] = v.z;
+                        d[3] = v.w;
+                    }
+                }
+                __syncthreads();
+                if ((uint32_t)tid < R) {
+                    const uint32_t tn = min(256u, dpad - t0);  // a multiple of 16
+                    for (uint32_t w0 = 0; w0 < tn / 4; w0 += 4)
+#pragma unroll
+                    for (uint32_t w = w0; w < w0 + 4; ++w) {
+                        const uint32_t word = sw[tid * 65 + w];
+#pragma unroll
+                        for (uint32_t b = 0; b < 4; ++b) {
+                            const float q = sq[t0 + 4 * w + b];
+                            const float bq = (float)((word >> (8 * b)) & 0xFFu);
+                            if constexpr (ACC == ACC_EUCLID) {
+                                float df = q - bq;
+                                float sqr = df * df;
+                                acc = acc + sqr;
+                            } else if constexpr (ACC == ACC_MANHATTAN) {
+                                float df = q - bq;
+                                acc = acc + fabsf(df);
+                            } else {
+                                float p = q * bq;
+                                acc = acc + p;
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        } else if (R <= 64) {
             // at most 64 rows: 64-dimension slabs ([64][65] floats, the same
             // LDS as the [256][17] layout), four 16-byte loads per thread in
             // flight per slab, so a query makes dpad / 64 round trips
             float *sw = sb;
             for (uint32_t t0 = 0; t0 < dpad; t0 += 64) {
-                if constexpr (U8) {
-                    // one 16-byte load per thread: 16 dimensions of one row
-                    const uint32_t row = (uint32_t)tid / 4, c = ((uint32_t)tid % 4) * 16;
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) {
+                    const uint32_t i = (uint32_t)tid + RR_THREADS * u;
+                    const uint32_t row = i / 16, c = (i % 16) * 4;
                     if (row < R && t0 + c < dpad) {
-                        const uint4 v = *reinterpret_cast<const uint4 *>(
-                            Bp8 + (size_t)s_rows[row] * dpad + t0 + c);
-                        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+                        const float4 v = *reinterpret_cast<const float4 *>(
+                            Bp + (size_t)s_rows[row] * dpad + t0 + c);
                         float *d = sw + row * 65 + c;
-#pragma unroll
-                        for (uint32_t e = 0; e < 16; ++e)
-                            d[e] = (float)((w[e / 4] >> (8 * (e % 4))) & 0xFFu);
-                    }
-                } else {
-#pragma unroll
-                    for (uint32_t u = 0; u < 4; ++u) {
-                        const uint32_t i = (uint32_t)tid + RR_THREADS * u;
-                        const uint32_t row = i / 16, c = (i % 16) * 4;
-                        if (row < R && t0 + c < dpad) {
-                            const float4 v = *reinterpret_cast<const float4 *>(
-                                Bp + (size_t)s_rows[row] * dpad + t0 + c);
-                            float *d = sw + row * 65 + c;
-                            d[0] = v.x;
-                            d[1] = v.y;
-                            d[2] = v.z;
-                            d[3] = v.w;
-                        }
+                        d[0] = v.x;
+                        d[1] = v.y;
+                        d[2] = v.z;
+                        d[3] = v.w;
                     }
                 }
                 __syncthreads();
