@@ -600,14 +600,22 @@ class BruteForce final : public Index {
                                       rg_ck_.as<uint32_t>(), rg_coff_.as<uint64_t>(),
                                       rg_qpad_.as<float>(), err, g);
             launch_fill_u32(rg_qinex_.as<uint32_t>(), 1, 0, g);
-            launch_self_dot(rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_, rg_qnorm_.as<float>(),
-                            g, dr);
-            if (bf16)
-                launch_to_bf16(rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_, dk_,
-                               rg_qlow_.as<uint16_t>(), rg_qinex_.as<uint32_t>(), g, dr);
-            else
-                launch_to_fp8(rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_, dk_,
-                              rg_qlow_.as<uint8_t>(), rg_qscale_.as<float>(), g, dr);
+            if (query_prep_ok(rg_qpad_.as<float>(), dpad_, (uint32_t)dim_, nullptr, 0)) {
+                // the padded rows were checked for finiteness as they were copied
+                launch_query_prep(bf16 ? 2 : 1, rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_,
+                                  dk_, rg_qlow_.get(), rg_qscale_.as<float>(),
+                                  rg_qinex_.as<uint32_t>(), rg_qnorm_.as<float>(), nullptr, 0,
+                                  nullptr, g, dr);
+            } else {
+                launch_self_dot(rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_,
+                                rg_qnorm_.as<float>(), g, dr);
+                if (bf16)
+                    launch_to_bf16(rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_, dk_,
+                                   rg_qlow_.as<uint16_t>(), rg_qinex_.as<uint32_t>(), g, dr);
+                else
+                    launch_to_fp8(rg_qpad_.as<float>(), dpad_, QC, (uint32_t)dim_, dk_,
+                                  rg_qlow_.as<uint8_t>(), rg_qscale_.as<float>(), g, dr);
+            }
             const uint32_t *s_q = nullptr, *pre = nullptr;
             const float *key_s = nullptr;
             if (filter) {
@@ -891,17 +899,31 @@ class BruteForce final : public Index {
         const float gamma = 2.0f * (float)(std::max<uint32_t>(dk_, dpad_) + 8) * u;
         const float c_bf16 = 1.0f / 128.0f + 1.0f / 65536.0f;
         const float c_exact = base_inexact_ ? c_bf16 : 0.0f;
+        // Rows that fit the one-pass preparation are read once; when they
+        // are already dpad floats apart and aligned, the rerank reads them
+        // in place instead of a padded copy.
+        const bool prep1 = query_prep_ok(dq, dim_, (uint32_t)dim_, qpad_.as<float>(), dpad_);
+        const bool in_place = prep1 && dim_ == dpad_;
         for (uint64_t q0 = 0; q0 < nq; q0 += qc) {
             const uint32_t rows = (uint32_t)std::min<uint64_t>(qc, nq - q0);
-            launch_pad_rows(dq + q0 * dim_, dim_, qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
-                            dpad_, s, nullptr, err_.as<uint32_t>());
-            launch_self_dot(qpad_.as<float>(), dpad_, rows, (uint32_t)dim_, qnorm_.as<float>(), s);
-            if (bf16)
-                launch_to_bf16(qpad_.as<float>(), dpad_, rows, (uint32_t)dim_, dk_,
-                               qlow_.as<uint16_t>(), qinexact_.as<uint32_t>(), s);
-            else
-                launch_to_fp8(qpad_.as<float>(), dpad_, rows, (uint32_t)dim_, dk_,
-                              qlow_.as<uint8_t>(), qscale_.as<float>(), s);
+            const float *qp = in_place ? dq + q0 * dim_ : qpad_.as<float>();
+            if (prep1) {
+                launch_query_prep(bf16 ? 2 : 1, dq + q0 * dim_, dim_, rows, (uint32_t)dim_, dk_,
+                                  qlow_.get(), qscale_.as<float>(), qinexact_.as<uint32_t>(),
+                                  qnorm_.as<float>(), in_place ? nullptr : qpad_.as<float>(),
+                                  dpad_, err_.as<uint32_t>(), s);
+            } else {
+                launch_pad_rows(dq + q0 * dim_, dim_, qpad_.as<float>(), dpad_, rows,
+                                (uint32_t)dim_, dpad_, s, nullptr, err_.as<uint32_t>());
+                launch_self_dot(qpad_.as<float>(), dpad_, rows, (uint32_t)dim_,
+                                qnorm_.as<float>(), s);
+                if (bf16)
+                    launch_to_bf16(qpad_.as<float>(), dpad_, rows, (uint32_t)dim_, dk_,
+                                   qlow_.as<uint16_t>(), qinexact_.as<uint32_t>(), s);
+                else
+                    launch_to_fp8(qpad_.as<float>(), dpad_, rows, (uint32_t)dim_, dk_,
+                                  qlow_.as<uint8_t>(), qscale_.as<float>(), s);
+            }
             const uint32_t *s_q = nullptr, *pre = nullptr;
             const float *key_s = nullptr;
             if (filter) {
@@ -951,7 +973,7 @@ class BruteForce final : public Index {
                 launch_row_select(D_.as<float>(), ldk, rows, n, S, cand_d_.as<float>(),
                                   cand_i_.as<uint32_t>(), S, flags_.as<uint32_t>(), s);
             }
-            launch_rerank(metric_, qpad_.as<float>(), dpad_, base_.as<float>(),
+            launch_rerank(metric_, qp, dpad_, base_.as<float>(),
                           cand_i_.as<uint32_t>(), cand_d_.as<float>(), S, S, qnorm_.as<float>(),
                           bnorm_.as<float>(), rows, n, k, bmax_, gamma, c_exact, c_bf16,
                           qinexact_.as<uint32_t>(), bf16 ? certify_ : 0, d_idx + q0 * k,

@@ -157,6 +157,16 @@ void launch_to_fp8(const float *src, uint64_t lds, uint32_t rows, uint32_t d, ui
 // dst[r][j] = (uint8_t)src[r][j] over [rows][dpad]; *inexact |= 1 when some
 // value is not bit-identical to the f32 of its u8 (non-integers, values
 // outside 0..255, -0.0): with *inexact == 0 the copy holds the same values.
+// The query rows of a shortlist form in one pass (one wave per row): the
+// non-finite check (err |= 2), the left-fold norms, the FP8 (elem 1, with
+// dequant) or BF16 (elem 2, with inexact) rows of dk columns and, when pad
+// is given, the rows zero-padded to dpad floats.  query_prep_ok tells
+// whether the shape fits (d % 4 == 0, d <= 1024, 16-byte aligned rows).
+bool query_prep_ok(const float *src, uint64_t lds, uint32_t d, const float *pad, uint32_t dpad);
+void launch_query_prep(int elem, const float *src, uint64_t lds, uint32_t rows, uint32_t d,
+                       uint32_t dk, void *dst, float *dequant, uint32_t *inexact, float *norm2,
+                       float *pad, uint32_t dpad, uint32_t *err, hipStream_t s,
+                       const uint32_t *dev_rows = nullptr);
 void launch_to_u8_exact(const float *src, uint32_t dpad, uint64_t rows, uint8_t *dst,
                         uint32_t *inexact, hipStream_t s);
 // inv[r] = 1/sqrt(norm2[r]); stats[0] |= max norm (float bits, atomicMax),

@@ -464,8 +464,110 @@ __global__ void to_fp8_kernel(const float *__restrict__ src, uint64_t lds, uint3
     }
 }
 
-// inv[r] = 1 / sqrt(norm2[r]); stats[0] = max_r sqrt(norm2[r]) (as float
-// bits); stats[2] = 1 if some row has norm zero.
+// The query side of the shortlist forms in one pass, one wave per row, for
+// rows 16-byte aligned with d % 4 == 0 and d <= 1024: lane l holds the
+// float4s at columns 4 (l + 64 j), j < 4, and from them
+//  - err |= 2 if a value is NaN or infinite;
+//  - norm2[r] = sum_t x_t*x_t as a left fold from -0.0 (lane 0 of the wave
+//    over the row staged in LDS; the same bits as self_dot_kernel);
+//  - MODE 1: the scaled FP8-e4m3 row of to_fp8_kernel (scale 224 / max|x|,
+//    dequant[r] = max|x| / 224), four bytes per lane store;
+//    MODE 2: the RNE BF16 row of to_bf16_kernel, *inexact set if a value
+//    does not survive;
+//  - pad (when given): the row zero-padded to dpad f32 columns.
+// Columns past d are zero in every output, up to dk (dpad for pad).
+template <int MODE>
+__global__ __launch_bounds__(256) void query_prep_kernel(
+    const float *__restrict__ src, uint64_t lds, uint32_t rows, uint32_t d, uint32_t dk,
+    void *__restrict__ dst, float *__restrict__ dequant, uint32_t *__restrict__ inexact,
+    float *__restrict__ norm2, float *__restrict__ pad, uint32_t dpad, uint32_t *__restrict__ err,
+    const uint32_t *__restrict__ dev_rows)
+{
+    __shared__ float4 srow[4][256];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (dev_rows && *dev_rows < rows) rows = *dev_rows;
+    const uint32_t r = blockIdx.x * 4 + wave;
+    const bool live = r < rows;  // no early return: the workgroup barrier below
+    const float *x = src + (size_t)(live ? r : 0) * lds;
+    float4 v[4];
+    float m = 0.0f;
+    bool bad = false;
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t c = 4 * (lane + 64 * j);
+        v[j] = live && c < d ? *reinterpret_cast<const float4 *>(x + c) : make_float4(0, 0, 0, 0);
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(v[j].x), fabsf(v[j].y)), fmaxf(fabsf(v[j].z), fabsf(v[j].w))));
+        bad |= !(isfinite(v[j].x) && isfinite(v[j].y) && isfinite(v[j].z) && isfinite(v[j].w));
+        srow[wave][lane + 64 * j] = v[j];
+    }
+    if (err && __ballot(bad) && lane == 0) atomicOr(err, 2u);
+    __syncthreads();
+    if (!live) return;
+    if (lane == 0) {
+        float acc = -0.0f;
+        const float4 *p = srow[wave];
+        for (uint32_t i = 0; i < d / 4; ++i) {
+            const float4 t = p[i];
+            float q = t.x * t.x;
+            acc = acc + q;
+            q = t.y * t.y;
+            acc = acc + q;
+            q = t.z * t.z;
+            acc = acc + q;
+            q = t.w * t.w;
+            acc = acc + q;
+        }
+        norm2[r] = acc;
+    }
+    if (pad) {
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t c = 4 * (lane + 64 * j);
+            if (c < dpad) *reinterpret_cast<float4 *>(pad + (size_t)r * dpad + c) = v[j];
+        }
+    }
+    if constexpr (MODE == 1) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        const float scale = m > 0.0f ? 224.0f / m : 1.0f;
+        if (lane == 0) dequant[r] = m > 0.0f ? m / 224.0f : 0.0f;
+        uint32_t *o = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(dst) + (size_t)r * dk);
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t c = 4 * (lane + 64 * j);
+            if (c < dk) {
+                const int lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[j].x * scale, v[j].y * scale, 0,
+                                                               false);
+                o[c / 4] = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(v[j].z * scale,
+                                                                     v[j].w * scale, lo, true);
+            }
+        }
+    } else {
+        uint32_t ib = 0;
+        uint16_t *o = static_cast<uint16_t *>(dst) + (size_t)r * dk;
+        auto rne = [&](float f) {
+            const uint32_t u = __float_as_uint(f);
+            const uint32_t h = (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+            ib |= __uint_as_float(h << 16) != f ? 1u : 0u;
+            return h;
+        };
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t c = 4 * (lane + 64 * j);
+            if (c < dk) {
+                uint2 w;
+                w.x = rne(v[j].x) | (rne(v[j].y) << 16);
+                w.y = rne(v[j].z) | (rne(v[j].w) << 16);
+                *reinterpret_cast<uint2 *>(o + c) = w;
+            }
+        }
+        if (inexact && __ballot(ib != 0) && lane == 0) atomicOr(inexact, 1u);
+    }
+}
+
+// dst = the u8 of each value; *inexact |= 1 if some value's bits differ
+// from those of the f32 of its u8 (a fraction, a value outside 0..255,
+// -0.0 or NaN): then the copy does not hold the base.
 __global__ void to_u8_exact_kernel(const float *__restrict__ src, uint64_t total,
                                    uint8_t *__restrict__ dst, uint32_t *__restrict__ inexact)
 {
@@ -480,6 +582,8 @@ __global__ void to_u8_exact_kernel(const float *__restrict__ src, uint64_t total
     if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(inexact, 1u);
 }
 
+// inv[r] = 1 / sqrt(norm2[r]); stats[0] = max_r sqrt(norm2[r]) (as float
+// bits); stats[2] = 1 if some row has norm zero.
 __global__ void norm_stats_kernel(const float *__restrict__ norm2, uint32_t rows,
                                   float *__restrict__ inv, uint32_t *stats)
 {
@@ -619,6 +723,33 @@ void launch_to_fp8(const float *src, uint64_t lds, uint32_t rows, uint32_t d, ui
 {
     if (rows == 0) return;
     to_fp8_kernel<<<ceil_div(rows, 4), 256, 0, s>>>(src, lds, rows, d, dk, dst, dequant, dev_rows);
+    HM_CHECK_LAUNCH();
+}
+
+bool query_prep_ok(const float *src, uint64_t lds, uint32_t d, const float *pad, uint32_t dpad)
+{
+    return d > 0 && d % 4 == 0 && d <= 1024 && lds % 4 == 0 && ((uintptr_t)src & 15) == 0 &&
+           (!pad || (dpad >= d && dpad % 4 == 0 && dpad <= 1024 && ((uintptr_t)pad & 15) == 0));
+}
+
+void launch_query_prep(int elem, const float *src, uint64_t lds, uint32_t rows, uint32_t d,
+                       uint32_t dk, void *dst, float *dequant, uint32_t *inexact, float *norm2,
+                       float *pad, uint32_t dpad, uint32_t *err, hipStream_t s,
+                       const uint32_t *dev_rows)
+{
+    if (rows == 0) return;
+    if (!query_prep_ok(src, lds, d, pad, dpad) || dk < d || dk > 1024 || dk % 4 != 0 ||
+        ((uintptr_t)dst & 7) != 0)
+        fail(HM_ERR_INVALID_ARG, "query_prep: unsupported row shape or alignment");
+    ProfScope prof("query_prep", s);
+    if (elem == 1)
+        query_prep_kernel<1><<<ceil_div(rows, 4), 256, 0, s>>>(src, lds, rows, d, dk, dst, dequant,
+                                                               inexact, norm2, pad, dpad, err,
+                                                               dev_rows);
+    else
+        query_prep_kernel<2><<<ceil_div(rows, 4), 256, 0, s>>>(src, lds, rows, d, dk, dst, dequant,
+                                                               inexact, norm2, pad, dpad, err,
+                                                               dev_rows);
     HM_CHECK_LAUNCH();
 }
 

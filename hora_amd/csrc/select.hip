@@ -835,13 +835,19 @@ __global__ __launch_bounds__(256) void kth_smallest_kernel(const float *__restri
 __global__ void check_finite_kernel(const float *__restrict__ X, uint64_t ld, uint64_t rows,
                                     uint32_t d, uint32_t *err)
 {
-    uint64_t total = rows * d;
-    uint32_t bad = 0;
-    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
-         e += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t r = e / d, c = e % d;
-        float v = X[r * ld + c];
-        if (!isfinite(v)) bad = 1;
+    bool bad = false;
+    if (ld == d && ld % 4 == 0 && ((uintptr_t)X & 15) == 0) {
+        // contiguous rows: the whole block as float4s
+        const float4 *X4 = reinterpret_cast<const float4 *>(X);
+        const uint64_t total = rows * d / 4;
+        for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+             e += (uint64_t)gridDim.x * blockDim.x) {
+            const float4 v = X4[e];
+            bad |= !(isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w));
+        }
+    } else {
+        for (uint64_t r = blockIdx.x; r < rows; r += gridDim.x)
+            for (uint32_t c = threadIdx.x; c < d; c += blockDim.x) bad |= !isfinite(X[r * ld + c]);
     }
     if (bad) atomicOr(err, 2u);
 }
@@ -852,15 +858,16 @@ __global__ void pad_rows_kernel(const float *__restrict__ src, uint64_t lds, flo
                                 const uint32_t *__restrict__ src_rows)
 {
     if (dev_rows && *dev_rows < rows) rows = *dev_rows;
-    uint64_t total = rows * dpad;
     bool bad = false;
-    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
-         e += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t r = e / dpad, c = e % dpad;
+    for (uint64_t r = blockIdx.x; r < rows; r += gridDim.x) {
         const uint64_t sr = src_rows ? src_rows[r] : r;
-        const float v = c < d ? src[sr * lds + c] : 0.0f;
-        bad |= !isfinite(v);
-        dst[r * ldd + c] = v;
+        const float *x = src + sr * lds;
+        float *y = dst + r * ldd;
+        for (uint32_t c = threadIdx.x; c < dpad; c += blockDim.x) {
+            const float v = c < d ? x[c] : 0.0f;
+            bad |= !isfinite(v);
+            y[c] = v;
+        }
     }
     if (err && bad) atomicOr(err, 2u);
 }
@@ -1050,8 +1057,7 @@ void launch_pad_rows(const float *src, uint64_t lds, float *dst, uint64_t ldd, u
                      uint32_t *err, const uint32_t *src_rows)
 {
     if (rows == 0) return;
-    uint64_t total = rows * dpad;
-    unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 8192);
+    const unsigned blocks = (unsigned)std::min<uint64_t>(rows, 8192);
     pad_rows_kernel<<<blocks, 256, 0, s>>>(src, lds, dst, ldd, rows, d, dpad, dev_rows, err,
                                            src_rows);
     HM_CHECK_LAUNCH();
