@@ -178,7 +178,7 @@ def main_bf(args):
         wall, ev = timer.run(step_fn(ix), args.steps, args.warmup, f"Euclidean/{form}",
                              only="exact_tile" if form == "exact" else "score_filter")
         km = kernel_ms(["exact_tile", "exact_rows", "row_select", "kth_smallest", "bf_finalize",
-                        "score_gemm", "score_filter", "list_prepare", "rerank"], timer)
+                        "score_gemm", "score_filter", "list_gather", "list_prepare", "rerank"], timer)
         fb = ix.search_stats()
         ids_by_form[form] = (d_idx.cpu().numpy().view(np.uint64).copy(),
                              d_dist.cpu().numpy().copy(),

@@ -547,6 +547,7 @@ class BruteForce final : public Index {
             rg_reserve(rg_seli_, (size_t)QC * fp.m * 4);
             rg_reserve(rg_list_, (size_t)QC * fp.cap * 8);
             rg_reserve(rg_lcnt_, (size_t)QC * 4);
+            rg_reserve(rg_fscr_, score_filter_scratch_bytes(QC, fp.cap));
             rg_reserve(rg_sq_, (size_t)QC * 4);
             rg_reserve(rg_ksq_, (size_t)QC * 4);
             rg_reserve(rg_pref_, (size_t)QC * 4);
@@ -648,7 +649,8 @@ class BruteForce final : public Index {
                                     rg_qscale_.as<float>(), bscale_.as<float>(),
                                     bnorm_.as<float>(), inv_bnorm_.as<float>(), metric_, tau,
                                     tau_ld, rg_list_.as<uint64_t>(), fp.cap,
-                                    rg_lcnt_.as<uint32_t>(), g, dr, group_kib_);
+                                    rg_lcnt_.as<uint32_t>(), rg_fscr_.get(), rg_fscr_.bytes(), g,
+                                    dr, group_kib_);
                 launch_list_prepare(rg_list_.as<uint64_t>(), fp.cap, rg_lcnt_.as<uint32_t>(), tau,
                                     tau_ld, QC, S, rg_ci_.as<uint32_t>(), rg_cd_.as<float>(), S,
                                     rg_sq_.as<uint32_t>(), rg_ksq_.as<float>(),
@@ -867,7 +869,8 @@ class BruteForce final : public Index {
         const bool filter = fp.ns != 0;
         if (filter) ensure_sample(fp.ns, s);
         const uint64_t ldk = round_up(filter ? fp.ns : n, 4);
-        const uint64_t per_q = ldk * sizeof(float) + (filter ? (uint64_t)fp.cap * 8 : 0);
+        // per query: its key row, and (filter form) its list and scan records
+        const uint64_t per_q = ldk * sizeof(float) + (filter ? (uint64_t)fp.cap * (8 + 24) : 0);
         uint64_t qc = std::max<uint64_t>(1, d_budget_ / per_q);
         qc = std::min<uint64_t>(qc, nq);
         qc = std::min<uint64_t>(qc, 65535ull * 128);
@@ -888,6 +891,7 @@ class BruteForce final : public Index {
             sel_f_.reserve(qc * sizeof(uint32_t));
             list_.reserve(qc * fp.cap * sizeof(uint64_t));
             lcnt_.reserve(qc * sizeof(uint32_t));
+            fscr_.reserve(score_filter_scratch_bytes((uint32_t)qc, fp.cap));
             sq_.reserve(qc * sizeof(uint32_t));
             ksq_.reserve(qc * sizeof(float));
             pref_.reserve(qc * sizeof(uint32_t));
@@ -957,8 +961,8 @@ class BruteForce final : public Index {
                 launch_score_filter(bf16 ? 2 : 1, qlow_.get(), rows, blow_.get(), n, row_bytes,
                                     qscale_.as<float>(), bscale_.as<float>(), bnorm_.as<float>(),
                                     inv_bnorm_.as<float>(), metric_, tau, tau_ld,
-                                    list_.as<uint64_t>(), fp.cap, lcnt_.as<uint32_t>(), s, nullptr,
-                                    group_kib_);
+                                    list_.as<uint64_t>(), fp.cap, lcnt_.as<uint32_t>(),
+                                    fscr_.get(), fscr_.bytes(), s, nullptr, group_kib_);
                 launch_list_prepare(list_.as<uint64_t>(), fp.cap, lcnt_.as<uint32_t>(), tau, tau_ld,
                                     rows, S, cand_i_.as<uint32_t>(), cand_d_.as<float>(), S,
                                     sq_.as<uint32_t>(), ksq_.as<float>(), pref_.as<uint32_t>(), s);
@@ -1329,7 +1333,7 @@ class BruteForce final : public Index {
     uint32_t samp_ns_ = 0;
     bool use_filter_ = true;
     DeviceBuffer bsamp_, bs_norm2_, bs_inv_, bs_scale_;
-    DeviceBuffer Ds_, sel_d_, sel_i_, sel_f_, list_, lcnt_, sq_, ksq_, pref_;
+    DeviceBuffer Ds_, sel_d_, sel_i_, sel_f_, list_, lcnt_, fscr_, sq_, ksq_, pref_;
     DeviceBuffer redo_rows_, redo_q_, redo_i_, redo_d_, redo_c_;
     PinnedBuffer hflags_, hrows_, herr_, hplan_;  // row flags, rerun rows, error word, rerun plan
     DeviceBuffer plan_, r8p_, r8n_, r8D_, r8i_, r8d_, r8c_;  // device rerun of tied rows
@@ -1345,7 +1349,7 @@ class BruteForce final : public Index {
     // buffers owned by the captured graph (never reallocated by other paths)
     DeviceBuffer rg_qpad_, rg_qnorm_, rg_qlow_, rg_qscale_, rg_D_, rg_cd_, rg_ci_, rg_sf_, rg_rf_,
         rg_qinex_;
-    DeviceBuffer rg_seld_, rg_seli_, rg_list_, rg_lcnt_, rg_sq_, rg_ksq_, rg_pref_;
+    DeviceBuffer rg_seld_, rg_seli_, rg_list_, rg_lcnt_, rg_fscr_, rg_sq_, rg_ksq_, rg_pref_;
 };
 
 }  // namespace

@@ -136,13 +136,18 @@ void launch_score_min4(int elem_bytes, const void *Q, uint32_t nq, const void *B
                        uint32_t group_kib = 0);
 // Filter form: keys below tau[q * tau_ld] are appended to list[q][0..cap)
 // as fkey(key) << 32 | row, counted in cnt[q] (zeroed by the caller; the
-// count may exceed cap, the entries past cap are dropped).
+// count may exceed cap, the entries past cap are dropped).  A persistent
+// scan writes the rows with keys below their threshold to per-wave record
+// regions in `scratch` (at least score_filter_scratch_bytes(nq, cap) bytes
+// for a full-size region; a smaller one only lets more lists overflow), and
+// a gather appends their keys to the lists.
+size_t score_filter_scratch_bytes(uint32_t nq, uint32_t cap);
 void launch_score_filter(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
                          uint32_t row_bytes, const float *qscale, const float *bscale,
                          const float *bnorm2, const float *inv_bnorm, int metric,
                          const float *tau, uint32_t tau_ld, uint64_t *list, uint32_t cap,
-                         uint32_t *cnt, hipStream_t s, const uint32_t *dev_rows = nullptr,
-                         uint32_t group_kib = 0);
+                         uint32_t *cnt, void *scratch, size_t scratch_bytes, hipStream_t s,
+                         const uint32_t *dev_rows = nullptr, uint32_t group_kib = 0);
 // Score tile order: base tiles are taken in groups of about group_kib KiB of
 // base rows per XCD's L2, the query tiles inside a group (0: 2,048 KiB).
 // Strided sample of the base: dst row i = src row floor(i * n / ns).

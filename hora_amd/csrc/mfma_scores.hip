@@ -47,16 +47,10 @@ __device__ __forceinline__ uint32_t fkey(float f)
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
-// Output of a score launch: either the full key matrix, or (FILTER) the
-// keys below a per-query threshold appended to per-query lists.
+// Output of a key-matrix launch.
 struct ScoreOut {
     float *Kmat;  // [nq][ldk]
     uint64_t ldk;
-    const float *tau;  // FILTER: threshold of query q at tau[q * tau_ld]
-    uint32_t tau_ld;
-    uint64_t *list;  // FILTER: [nq][cap] of fkey(key) << 32 | row
-    uint32_t cap;
-    uint32_t *cnt;  // FILTER: appended per query (may exceed cap)
 };
 
 // Tile order.  The workgroups are dispatched round robin over the 8 XCDs
@@ -91,7 +85,7 @@ __device__ __forceinline__ void tile_of(uint32_t L, uint32_t TX, uint32_t TY, ui
 // MIN4 (key matrix form only): each lane writes the minimum of its four keys
 // of a row (columns lr + 16 j of the wave's 64), at column
 // btile * 32 + wc * 16 + lr of a [nq][ceil(n / 128) * 32] matrix.
-template <int ELEM, int MET, bool FILTER, bool MIN4 = false, bool OFF32 = true>  // ELEM: 2 = bf16, 1 = fp8
+template <int ELEM, int MET, bool MIN4 = false, bool OFF32 = true>  // ELEM: 2 = bf16, 1 = fp8
 __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
     const uint8_t *__restrict__ Q, uint32_t nq, uint32_t TY, uint32_t G,
     const uint8_t *__restrict__ B, uint32_t n, uint32_t row_bytes, const float *__restrict__ qscale,
@@ -104,9 +98,9 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
     if (qtile * GM >= nq) return;
     __shared__ __align__(1024) uint8_t sA[2][GM * ROWB];
     __shared__ __align__(1024) uint8_t sB[2][GN * ROWB];
-    // the epilogue's per-query (scale, threshold) and per-row (norm term,
-    // scale) operands, loaded before the K loop so their latency is hidden
-    __shared__ float s_rs[GM], s_rt[GM], s_cv[GN], s_cs[GN];
+    // the epilogue's per-query scale and per-row (norm term, scale)
+    // operands, loaded before the K loop so their latency is hidden
+    __shared__ float s_rs[GM], s_cv[GN], s_cs[GN];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
@@ -114,7 +108,6 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
     if (tid < GM) {
         const uint32_t q = q0 + tid, qc = q < nq ? q : nq - 1;
         s_rs[tid] = (ELEM == 1) ? qscale[qc] : 1.0f;
-        s_rt[tid] = FILTER ? out.tau[(size_t)qc * out.tau_ld] : 0.0f;
     } else {
         const uint32_t c = n0 + (tid - GM), cc = c < n ? c : n - 1;
         if constexpr (MET == HM_METRIC_EUCLIDEAN) s_cv[tid - GM] = bnorm2[cc];
@@ -234,23 +227,6 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
         if (kt + 1 < nk) step(std::integral_constant<int, 1>{}, kt + 1);
     }
 
-    // Filter hits are staged in LDS (the operand buffers are free once every
-    // wave is past its last MFMA step): each thread has ST_PT slots of its
-    // own, filled without atomics, and hits past those take a slot of the
-    // shared overflow region through an LDS atomic.  Then each thread
-    // appends its own slots to the global lists, its atomics issued back to
-    // back, and the overflow region is appended by all threads.
-    constexpr uint32_t ST_PT = 8, STAGE_CAP = sizeof(sA) / 8;  // 4,096 entries
-    static_assert(sizeof(sB) / 4 >= STAGE_CAP, "stage_q fits in sB");
-    uint64_t *stage_v = reinterpret_cast<uint64_t *>(&sA[0][0]);
-    uint32_t *stage_q = reinterpret_cast<uint32_t *>(&sB[0][0]);
-    __shared__ uint32_t s_nhit;
-    uint32_t myc = 0;  // this thread's hits
-    if constexpr (FILTER) {
-        __syncthreads();
-        if (tid == 0) s_nhit = 0;
-        __syncthreads();
-    }
     // Epilogue: lane holds D[4*lk + e][lr] of each 16x16 tile.  Every
     // per-column and per-row operand is loaded up front (clamped indices,
     // no branches), so the loads overlap instead of each waiting inside a
@@ -264,16 +240,12 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
         colv[j] = s_cv[cl];
         cols[j] = s_cs[cl];
     }
-    float rowt[4][4], rows_[4][4];
+    float rows_[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const uint32_t ql = wr * 64 + i * 16 + lk * 4 + e;
-            rows_[i][e] = s_rs[ql];
-            rowt[i][e] = s_rt[ql];
-        }
-    if constexpr (!FILTER && MIN4) {
+        for (int e = 0; e < 4; ++e) rows_[i][e] = s_rs[wr * 64 + i * 16 + lk * 4 + e];
+    if constexpr (MIN4) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -296,65 +268,7 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
         }
         return;
     }
-    if constexpr (FILTER) {
-        // Validity is folded into the operands instead of a mask per key:
-        // a row past nq compares against -inf, and a column past n gets a
-        // key of +inf (Euclidean: cv = +inf; other metrics: a select).  The
-        // four keys of a row are tested through their minimum, so the common
-        // case costs one branch per four keys.  Euclidean FP8 computes
-        // cv - (2 rs cs) acc, which equals cv - 2 (acc (rs cs)): scaling by 2
-        // is exact.
-        constexpr float PINF = __builtin_huge_valf();
-        float cvv[4];
-        bool cok[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            cok[j] = colc[j] < n;
-            cvv[j] = (MET == HM_METRIC_EUCLIDEAN && !cok[j]) ? PINF : colv[j];
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const uint32_t q = q0 + wr * 64 + i * 16 + lk * 4 + e;
-                const float rt = q < nq ? rowt[i][e] : -PINF;
-                float key[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const float a = acc[i][j][e];
-                    if constexpr (MET == HM_METRIC_EUCLIDEAN) {
-                        if constexpr (ELEM == 1) key[j] = cvv[j] - ((2.0f * rows_[i][e]) * cols[j]) * a;
-                        else key[j] = cvv[j] - 2.0f * a;
-                    } else {
-                        const float s = ELEM == 1 ? a * (rows_[i][e] * cols[j]) : a;
-                        float k;
-                        if constexpr (MET == HM_METRIC_DOT_PRODUCT) k = s;
-                        else if constexpr (MET == HM_METRIC_COSINE_SIMILARITY) k = -s * cvv[j];
-                        else k = s * cvv[j];
-                        key[j] = cok[j] ? k : PINF;
-                    }
-                }
-                // fminf drops a NaN key, which never compares below rt anyway
-                const float mn = fminf(fminf(key[0], key[1]), fminf(key[2], key[3]));
-                if (mn < rt) {  // rare: a few per query per tile row
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        if (key[j] < rt) {
-                            const uint32_t pos = myc < ST_PT ? tid * ST_PT + myc
-                                                             : 256 * ST_PT + atomicAdd(&s_nhit, 1u);
-                            if (pos < STAGE_CAP) {
-                                stage_v[pos] = ((uint64_t)fkey(key[j]) << 32) | colc[j];
-                                stage_q[pos] = q;
-                            } else {
-                                atomicAdd(out.cnt + q, out.cap + 1);  // mark the list overflowed
-                            }
-                            ++myc;
-                        }
-                    }
-                }
-            }
-        }
-    } else {
+    {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -376,22 +290,360 @@ __global__ __launch_bounds__(256, 2) void score_gemm_kernel(
             }
         }
     }
-    if constexpr (FILTER) {
-        uint32_t pq[ST_PT], pp[ST_PT];
+}
+
+// ---- The filtered scan --------------------------------------------------
+//
+// A persistent grid of one 512-thread workgroup per CU walks 256-query x
+// 256-row tiles: 8 waves in a 2 x 4 arrangement, each wave a 128 x 64 tile
+// of 8 x 4 MFMA 16x16 blocks (128 fp32 accumulators per lane).  A K step
+// brings 128 bytes of each of the 512 rows into LDS (64 KiB; two buffers):
+// half the bytes per MFMA of a 128 x 128 tile, within what the direct loads
+// sustain from L2.  Workgroup w takes tiles w, w + P, w + 2P, ... of the
+// XCD-aware order (P a multiple of 8: each workgroup stays on its XCD's
+// tile range), and the direct loads run one step ahead across tile
+// boundaries: the next tile's first step and its epilogue operands are in
+// flight while the current tile finishes.  The step barrier is a bare
+// s_barrier after an explicit wait: __syncthreads' fence would also wait
+// for the record stores of the epilogue.
+//
+// The epilogue writes no list: a lane whose four keys of a row (columns
+// col0 + 16 j) include one below the row's threshold writes a record (the
+// four keys and {query row, col0}) to its wave's own region at a position
+// counted by ballots, with no atomic and no LDS staging.  A second kernel
+// (scan_gather_kernel) appends the keys below the threshold to the lists.
+// A record past the region's end marks its query's list as overflowed
+// (count past cap: the list is not trusted).
+constexpr int SM = 256, SN = 256, SCAN_WAVES = 8;
+
+struct ScanRecords {
+    float4 *keys;     // [waves][R]
+    uint2 *meta;      // [waves][R]: {query row, base row of key 0}
+    uint32_t *count;  // [waves]: records of the wave (past R: dropped)
+    uint32_t R;
+};
+
+__device__ __forceinline__ void scan_barrier()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+template <int ELEM, int MET>
+__global__ __launch_bounds__(512, 1) void score_scan_kernel(
+    const uint8_t *__restrict__ Q, uint32_t nq, uint32_t TX, uint32_t TY, uint32_t G,
+    const uint8_t *__restrict__ B, uint32_t n, uint32_t row_bytes, const float *__restrict__ qscale,
+    const float *__restrict__ bscale, const float *__restrict__ bnorm2,
+    const float *__restrict__ inv_bnorm, const float *__restrict__ tau, uint32_t tau_ld,
+    uint32_t *__restrict__ cnt, uint32_t cap, ScanRecords rec, const uint32_t *__restrict__ dev_rows)
+{
+    if (dev_rows && *dev_rows < nq) nq = *dev_rows;
+    const uint32_t TYe = (nq + SM - 1) / SM;  // query tiles holding rows
+    const uint32_t T = TX * TY, P = gridDim.x;
+    // the two buffers are separate objects: the compiler then knows that a
+    // direct load into one does not write the other, and reads of the
+    // current buffer need not wait for the loads in flight
+    __shared__ __align__(1024) uint8_t sA0[SM * ROWB], sA1[SM * ROWB];
+    __shared__ __align__(1024) uint8_t sB0[SN * ROWB], sB1[SN * ROWB];
+    // epilogue operands of the current and the next tile: per query row
+    // (scale, threshold) pairs, per base row (norm term, scale)
+    __shared__ __align__(16) float2 s_rq[2][SM];
+    __shared__ float s_cv[2][SN], s_cs[2][SN];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 2, wc = wave & 3;
+    const int lr = lane & 15, lk = lane >> 4;
+    const uint32_t wave_s = (uint32_t)__builtin_amdgcn_readfirstlane(wave);
+    const uint32_t nk = row_bytes / ROWB;
+    const uint32_t W = blockIdx.x * SCAN_WAVES + wave_s;
+    float4 *rk = rec.keys + (size_t)W * rec.R;
+    uint2 *rm = rec.meta + (size_t)W * rec.R;
+    uint32_t rec_n = 0;
+
+    auto next_tile = [&](uint32_t L, uint32_t &bt, uint32_t &qt) __attribute__((always_inline)) -> uint32_t {
+        for (; L < T; L += P) {
+            tile_of(L, TX, TY, G, &bt, &qt);
+            if (qt < TYe) return L;
+        }
+        return T;
+    };
+    uint32_t bt, qt;
+    uint32_t L = next_tile(blockIdx.x, bt, qt);
+    if (L >= T) {
+        if (lane == 0) rec.count[W] = 0;
+        return;
+    }
+
+    // each wave's direct loads fill rows 8 (4 w + c) + l / 8 of both
+    // operands, from per-lane 64-bit row pointers
+    const uint8_t *qp[4], *bp[4];
+    auto set_rows = [&](uint32_t bt_, uint32_t qt_) __attribute__((always_inline)) {
+        const uint32_t q0 = qt_ * SM, n0 = bt_ * SN;
 #pragma unroll
-        for (uint32_t k = 0; k < ST_PT; ++k) {
-            pq[k] = stage_q[tid * ST_PT + k];
-            pp[k] = k < myc ? atomicAdd(out.cnt + pq[k], 1u) : 0xFFFFFFFFu;
+        for (int c = 0; c < 4; ++c) {
+            const uint32_t r = 8 * (4 * wave + c) + lane / 8;
+            const uint32_t g = (uint32_t)(lane % 8) ^ swz(r);
+            const uint32_t qa = q0 + r < nq ? q0 + r : nq - 1, ba = n0 + r < n ? n0 + r : n - 1;
+            qp[c] = Q + (size_t)qa * row_bytes + 16 * g;
+            bp[c] = B + (size_t)ba * row_bytes + 16 * g;
+        }
+    };
+    auto issue = [&](auto bufc, uint32_t kb) __attribute__((always_inline)) {
+        constexpr int buf = decltype(bufc)::value;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const uint8_t *src_q = qp[c] + kb, *src_b = bp[c] + kb;
+            __builtin_amdgcn_global_load_lds(src_q, &(buf ? sA1 : sA0)[(4 * wave_s + c) * 1024], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(src_b, &(buf ? sB1 : sB0)[(4 * wave_s + c) * 1024], 16, 0, 0);
+        }
+    };
+    // tid through an empty asm: the per-tile helpers below recompute their
+    // addresses where they run instead of keeping them live across the K loop
+    auto opaque_tid = [&]() __attribute__((always_inline)) {
+        uint32_t t = (uint32_t)tid;
+        asm volatile("" : "+v"(t));
+        return t;
+    };
+    auto load_ops = [&](uint32_t bt_, uint32_t qt_, float &o0, float &o1) __attribute__((always_inline)) {
+        const uint32_t tid = opaque_tid();
+        if (tid < SM) {
+            const uint32_t q = qt_ * SM + tid, qc = q < nq ? q : nq - 1;
+            o0 = ELEM == 1 ? qscale[qc] : 1.0f;
+            o1 = tau[(size_t)qc * tau_ld];
+        } else {
+            const uint32_t c = bt_ * SN + (tid - SM), cc = c < n ? c : n - 1;
+            if constexpr (MET == HM_METRIC_EUCLIDEAN) o0 = bnorm2[cc];
+            else if constexpr (MET == HM_METRIC_DOT_PRODUCT) o0 = 0.0f;
+            else o0 = inv_bnorm[cc];
+            o1 = ELEM == 1 ? bscale[cc] : 1.0f;
+        }
+    };
+    auto store_ops = [&](int par, float o0, float o1) __attribute__((always_inline)) {
+        const uint32_t tid = opaque_tid();
+        if (tid < SM) {
+            s_rq[par][tid] = make_float2(o0, o1);
+        } else {
+            s_cv[par][tid - SM] = o0;
+            s_cs[par][tid - SM] = o1;
+        }
+    };
+    // fragment offsets within a buffer: h = 0, 1 are the lane's two 16-byte
+    // chunks of a row (FP8: chunks 2 lk, 2 lk + 1; BF16: chunks lk, 4 + lk).
+    // Row blocks i and i + 4 are 64 rows apart, which leaves the swizzle
+    // unchanged: block i + 4 reads at fa[i] + 64 rows (an immediate).
+    uint32_t fa[4][2], fb[4][2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t ch_ = ELEM == 1 ? 2 * lk + h : 4 * h + lk;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t ra = wr * 128 + i * 16 + lr;
+            fa[i][h] = ra * ROWB + 16 * (ch_ ^ swz(ra));
         }
 #pragma unroll
-        for (uint32_t k = 0; k < ST_PT; ++k)
-            if (pp[k] < out.cap) out.list[(size_t)pq[k] * out.cap + pp[k]] = stage_v[tid * ST_PT + k];
-        __syncthreads();
-        const uint32_t nh = min(256 * ST_PT + s_nhit, STAGE_CAP);
-        for (uint32_t h = 256 * ST_PT + tid; h < nh; h += 256) {
-            const uint32_t q = stage_q[h];
-            const uint32_t pos = atomicAdd(out.cnt + q, 1u);
-            if (pos < out.cap) out.list[(size_t)q * out.cap + pos] = stage_v[h];
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t rb = wc * 64 + j * 16 + lr;
+            fb[j][h] = rb * ROWB + 16 * (ch_ ^ swz(rb));
+        }
+    }
+
+    floatx4 acc[8][4];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    auto epilogue = [&](uint32_t bt_, uint32_t qt_, int par) __attribute__((always_inline)) {
+        constexpr float PINF = __builtin_huge_valf();
+        const uint32_t lane = opaque_tid() & 63u;
+        const uint32_t lr = lane & 15u, lk = lane >> 4;
+        const uint32_t q0 = qt_ * SM, n0 = bt_ * SN;
+        const uint32_t col0 = n0 + wc * 64 + lr;
+        float cvv[4], cols[4];
+        bool cok[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t cl = wc * 64 + j * 16 + lr;
+            cok[j] = n0 + cl < n;
+            cols[j] = s_cs[par][cl];
+            cvv[j] = (MET == HM_METRIC_EUCLIDEAN && !cok[j]) ? PINF : s_cv[par][cl];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            // the (scale, threshold) pairs of the lane's four rows of block i,
+            // read per block (two 16-byte reads) to keep registers for acc
+            const float4 *rq4 = reinterpret_cast<const float4 *>(&s_rq[par][wr * 128 + i * 16 + lk * 4]);
+            const float4 rqa = rq4[0], rqb = rq4[1];
+            const float rsv[4] = {rqa.x, rqa.z, rqb.x, rqb.z}, rtv[4] = {rqa.y, rqa.w, rqb.y, rqb.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t ql = wr * 128 + i * 16 + lk * 4 + e;
+                const uint32_t q = q0 + ql;
+                const float rs = rsv[e];
+                const float rt = q < nq ? rtv[e] : -PINF;
+                float key[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float a = acc[i][j][e];
+                    if constexpr (MET == HM_METRIC_EUCLIDEAN) {
+                        // cv - (2 rs cs) acc: the bits of cv - 2 (acc (rs cs))
+                        if constexpr (ELEM == 1) key[j] = cvv[j] - ((2.0f * rs) * cols[j]) * a;
+                        else key[j] = cvv[j] - 2.0f * a;
+                    } else {
+                        const float sc = ELEM == 1 ? a * (rs * cols[j]) : a;
+                        float kv;
+                        if constexpr (MET == HM_METRIC_DOT_PRODUCT) kv = sc;
+                        else if constexpr (MET == HM_METRIC_COSINE_SIMILARITY) kv = -sc * cvv[j];
+                        else kv = sc * cvv[j];
+                        key[j] = cok[j] ? kv : PINF;
+                    }
+                }
+                // fminf drops a NaN key, which never compares below rt anyway
+                const float mn = fminf(fminf(key[0], key[1]), fminf(key[2], key[3]));
+                const bool hit = mn < rt;
+                const uint64_t bal = __ballot(hit);
+                if (bal) {
+                    if (hit) {
+                        const uint32_t pos =
+                            rec_n + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                        if (pos < rec.R) {
+                            rk[pos] = make_float4(key[0], key[1], key[2], key[3]);
+                            rm[pos] = make_uint2(q, col0);
+                        } else {
+                            atomicAdd(cnt + q, cap + 1);  // mark the list overflowed
+                        }
+                    }
+                    rec_n += (uint32_t)__popcll(bal);
+                }
+            }
+            asm volatile("" ::: "memory");  // block i's operand reads stay in block i
+        }
+    };
+
+    set_rows(bt, qt);
+    issue(std::integral_constant<int, 0>{}, 0u);
+    {
+        float o0, o1;
+        load_ops(bt, qt, o0, o1);
+        store_ops(0, o0, o1);
+    }
+    uint32_t nbt = 0, nqt = 0;
+    uint32_t NL = next_tile(L + P, nbt, nqt);
+    float p0 = 0.0f, p1 = 0.0f;  // the next tile's epilogue operands
+    int par = 0;
+    uint32_t k = 0;
+    // one K step from buffer buf; true when the workgroup has no tile left
+    auto step = [&](auto bufc) __attribute__((always_inline)) -> bool {
+        constexpr int buf = decltype(bufc)::value;
+        using Other = std::integral_constant<int, buf ^ 1>;
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's loads of the step have landed
+        scan_barrier();                 // ... and every wave's; buffer buf ^ 1 is free
+        if (k + 1 < nk) {
+            issue(Other{}, (k + 1) * ROWB);
+        } else if (NL < T) {
+            set_rows(nbt, nqt);
+            issue(Other{}, 0u);
+        }
+        if (k == 0 && NL < T) load_ops(nbt, nqt, p0, p1);
+        const uint8_t *As = buf ? sA1 : sA0, *Bs = buf ? sB1 : sB0;
+        uint4 bv[4][2];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) bv[j][h] = *reinterpret_cast<const uint4 *>(Bs + fb[j][h]);
+        // query fragments: block i + 1 is read while block i's MFMAs issue
+        uint4 avb[2][2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) avb[0][h] = *reinterpret_cast<const uint4 *>(As + fa[0][h]);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (i + 1 < 8) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    avb[(i + 1) & 1][h] = *reinterpret_cast<const uint4 *>(
+                        As + fa[(i + 1) & 3][h] + ((i + 1) >> 2) * 64 * ROWB);
+            }
+            const uint4 *av = avb[i & 1];
+            if constexpr (ELEM == 2) {
+                // two MFMA K=32 per step; lane holds 8 bf16 at k = 32*kk + 8*lk
+#pragma unroll
+                for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf16x8, av[kk]), __builtin_bit_cast(bf16x8, bv[j][kk]),
+                            acc[i][j], 0, 0, 0);
+            } else {
+                // one MFMA K=128 (e4m3 x e4m3, no block scale); lane holds 32
+                // fp8 at k = 32*lk in A and B alike
+                const i32x8 a = i32x8{(int)av[0].x, (int)av[0].y, (int)av[0].z, (int)av[0].w,
+                                      (int)av[1].x, (int)av[1].y, (int)av[1].z, (int)av[1].w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const i32x8 b = i32x8{(int)bv[j][0].x, (int)bv[j][0].y, (int)bv[j][0].z,
+                                          (int)bv[j][0].w, (int)bv[j][1].x, (int)bv[j][1].y,
+                                          (int)bv[j][1].z, (int)bv[j][1].w};
+                    acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                        a, b, acc[i][j], 0, 0, 0, 0, 0, 0);
+                }
+            }
+        }
+        if (++k < nk) return false;
+        epilogue(bt, qt, par);
+        if (NL >= T) return true;
+        // the next tile: its operands go to the other half of the operand
+        // arrays (the previous tile's epilogue, which read them, is done:
+        // every step of this tile had a barrier)
+        store_ops(par ^ 1, p0, p1);
+        par ^= 1;
+        bt = nbt;
+        qt = nqt;
+        L = NL;
+        NL = next_tile(L + P, nbt, nqt);
+        k = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        return false;
+    };
+    for (;;) {
+        if (step(std::integral_constant<int, 0>{})) break;
+        if (step(std::integral_constant<int, 1>{})) break;
+    }
+    if (lane == 0) rec.count[W] = rec_n;
+}
+
+// Appends the keys below the threshold of every record of scan region
+// blockIdx.x to its query's list: one atomic per record reserves its hits.
+__global__ __launch_bounds__(256) void scan_gather_kernel(ScanRecords rec,
+                                                          const float *__restrict__ tau,
+                                                          uint32_t tau_ld, uint64_t *__restrict__ list,
+                                                          uint32_t cap, uint32_t *__restrict__ cnt)
+{
+    const uint32_t W = blockIdx.x;
+    const uint32_t c0 = rec.count[W], c = c0 < rec.R ? c0 : rec.R;
+    const float4 *rk = rec.keys + (size_t)W * rec.R;
+    const uint2 *rm = rec.meta + (size_t)W * rec.R;
+    for (uint32_t r = threadIdx.x; r < c; r += 256) {
+        const float4 kv = rk[r];
+        const uint2 m = rm[r];
+        const float t = tau[(size_t)m.x * tau_ld];
+        const float kk[4] = {kv.x, kv.y, kv.z, kv.w};
+        uint32_t hits = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) hits |= (kk[j] < t ? 1u : 0u) << j;
+        if (!hits) continue;
+        uint32_t pos = atomicAdd(cnt + m.x, (uint32_t)__builtin_popcount(hits));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if ((hits >> j) & 1u) {
+                if (pos < cap)
+                    list[(size_t)m.x * cap + pos] = ((uint64_t)fkey(kk[j]) << 32) | (m.y + 16u * j);
+                ++pos;
+            }
         }
     }
 }
@@ -604,7 +856,7 @@ __global__ void norm_stats_kernel(const float *__restrict__ norm2, uint32_t rows
 
 }  // namespace
 
-template <bool FILTER, bool MIN4 = false>
+template <bool MIN4>
 static void score_launch(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
                          uint32_t row_bytes, const float *qscale, const float *bscale,
                          const float *bnorm2, const float *inv_bnorm, int metric,
@@ -626,7 +878,7 @@ static void score_launch(int elem_bytes, const void *Q, uint32_t nq, const void 
     const bool off32 = (uint64_t)nq * row_bytes <= 0xFFFFFFFFull &&
                        (uint64_t)n * row_bytes <= 0xFFFFFFFFull;
 #define HM_SG2(E, M, O)                                                                       \
-    score_gemm_kernel<E, M, FILTER, MIN4, O><<<grid, 256, 0, s>>>(                            \
+    score_gemm_kernel<E, M, MIN4, O><<<grid, 256, 0, s>>>(                            \
         (const uint8_t *)Q, nq, (uint32_t)TY, G, (const uint8_t *)B, n, row_bytes, qscale,     \
         bscale, bnorm2, inv_bnorm, out, dev_rows)
 #define HM_SG(E, M)                                                                           \
@@ -658,7 +910,7 @@ void launch_score_gemm(int elem_bytes, const void *Q, uint32_t nq, const void *B
                        uint32_t group_kib)
 {
     ProfScope prof("score_gemm", s);
-    ScoreOut out{Kmat, ldk, nullptr, 0, nullptr, 0, nullptr};
+    ScoreOut out{Kmat, ldk};
     score_launch<false>(elem_bytes, Q, nq, B, n, row_bytes, qscale, bscale, bnorm2, inv_bnorm,
                         metric, out, s, dev_rows, group_kib);
 }
@@ -671,23 +923,87 @@ void launch_score_min4(int elem_bytes, const void *Q, uint32_t nq, const void *B
 {
     if (ldk < (uint64_t)ceil_div(n, GN) * 32) fail(HM_ERR_INVALID_ARG, "score_min4: ldk too small");
     ProfScope prof("score_gemm", s);
-    ScoreOut out{Kmat, ldk, nullptr, 0, nullptr, 0, nullptr};
-    score_launch<false, true>(elem_bytes, Q, nq, B, n, row_bytes, qscale, bscale, bnorm2,
+    ScoreOut out{Kmat, ldk};
+    score_launch<true>(elem_bytes, Q, nq, B, n, row_bytes, qscale, bscale, bnorm2,
                               inv_bnorm, metric, out, s, dev_rows, group_kib);
+}
+
+// Workgroups of the persistent scan: one per CU, a multiple of 8 (the XCDs).
+static uint32_t scan_workgroups()
+{
+    int dev = 0, cus = 0;
+    HM_HIP(hipGetDevice(&dev));
+    HM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return (uint32_t)std::max(8, cus / 8 * 8);
+}
+
+// Scratch of the filtered scan: the record count of each of its waves, then
+// their record regions (16-byte keys and 8-byte meta each) of
+// max(1024, nq * cap / waves) records.
+static uint64_t scan_count_bytes(uint64_t waves) { return (waves * 4 + 255) / 256 * 256; }
+
+size_t score_filter_scratch_bytes(uint32_t nq, uint32_t cap)
+{
+    const uint64_t waves = (uint64_t)scan_workgroups() * SCAN_WAVES;
+    const uint64_t R = std::max<uint64_t>(1024, ceil_div((uint64_t)nq * cap, waves));
+    return (size_t)(scan_count_bytes(waves) + waves * R * 24);
 }
 
 void launch_score_filter(int elem_bytes, const void *Q, uint32_t nq, const void *B, uint32_t n,
                          uint32_t row_bytes, const float *qscale, const float *bscale,
                          const float *bnorm2, const float *inv_bnorm, int metric,
                          const float *tau, uint32_t tau_ld, uint64_t *list, uint32_t cap,
-                         uint32_t *cnt, hipStream_t s, const uint32_t *dev_rows,
-                         uint32_t group_kib)
+                         uint32_t *cnt, void *scratch, size_t scratch_bytes, hipStream_t s,
+                         const uint32_t *dev_rows, uint32_t group_kib)
 {
-    if (!tau || !list || !cnt || cap == 0) fail(HM_ERR_INVALID_ARG, "score_filter: null output");
-    ProfScope prof("score_filter", s);
-    ScoreOut out{nullptr, 0, tau, tau_ld, list, cap, cnt};
-    score_launch<true>(elem_bytes, Q, nq, B, n, row_bytes, qscale, bscale, bnorm2, inv_bnorm,
-                       metric, out, s, dev_rows, group_kib);
+    if (!tau || !list || !cnt || cap == 0 || !scratch)
+        fail(HM_ERR_INVALID_ARG, "score_filter: null output");
+    if (nq == 0 || n == 0) return;
+    if (row_bytes == 0 || row_bytes % ROWB)
+        fail(HM_ERR_INVALID_ARG, "score_filter: row bytes not a multiple of 128");
+    if (elem_bytes != 1 && elem_bytes != 2) fail(HM_ERR_INVALID_ARG, "score_filter: element size");
+    const uint64_t TX = ceil_div(n, SN), TY = ceil_div(nq, SM);
+    if (TX * TY > 0xFFFFFFFFull - 4096) fail(HM_ERR_INVALID_ARG, "score_filter: too many tiles");
+    const uint32_t P = scan_workgroups();
+    const uint64_t waves = (uint64_t)P * SCAN_WAVES, cb = scan_count_bytes(waves);
+    if (scratch_bytes < cb + waves * 24) fail(HM_ERR_INVALID_ARG, "score_filter: scratch too small");
+    const uint64_t R = std::min<uint64_t>((scratch_bytes - cb) / (waves * 24), 0xFFFFFFFFull);
+    uint8_t *base = static_cast<uint8_t *>(scratch);
+    ScanRecords rec{reinterpret_cast<float4 *>(base + cb),
+                    reinterpret_cast<uint2 *>(base + cb + waves * R * 16),
+                    reinterpret_cast<uint32_t *>(base), (uint32_t)R};
+    // base tiles per group: about 2 MiB of base rows per XCD's L2
+    const uint64_t group_bytes = (uint64_t)(group_kib ? group_kib : 2048) << 10;
+    const uint32_t G = (uint32_t)std::min<uint64_t>(
+        32, std::max<uint64_t>(2, group_bytes / ((uint64_t)SN * row_bytes)));
+    {
+        ProfScope prof("score_filter", s);
+#define HM_SC(E, M)                                                                           \
+    score_scan_kernel<E, M><<<P, 64 * SCAN_WAVES, 0, s>>>(                                    \
+        (const uint8_t *)Q, nq, (uint32_t)TX, (uint32_t)TY, G, (const uint8_t *)B, n,         \
+        row_bytes, qscale, bscale, bnorm2, inv_bnorm, tau, tau_ld, cnt, cap, rec, dev_rows)
+#define HM_SC_METRIC(E)                                                                       \
+    switch (metric) {                                                                         \
+    case HM_METRIC_EUCLIDEAN: HM_SC(E, HM_METRIC_EUCLIDEAN); break;                           \
+    case HM_METRIC_DOT_PRODUCT: HM_SC(E, HM_METRIC_DOT_PRODUCT); break;                       \
+    case HM_METRIC_COSINE_SIMILARITY: HM_SC(E, HM_METRIC_COSINE_SIMILARITY); break;           \
+    case HM_METRIC_ANGULAR: HM_SC(E, HM_METRIC_ANGULAR); break;                               \
+    default: fail(HM_ERR_INVALID_ARG, "score_filter: metric has no shortlist form");          \
+    }
+        if (elem_bytes == 2) {
+            HM_SC_METRIC(2)
+        } else {
+            HM_SC_METRIC(1)
+        }
+#undef HM_SC_METRIC
+#undef HM_SC
+        HM_CHECK_LAUNCH();
+    }
+    {
+        ProfScope prof("list_gather", s);
+        scan_gather_kernel<<<(unsigned)waves, 256, 0, s>>>(rec, tau, tau_ld, list, cap, cnt);
+        HM_CHECK_LAUNCH();
+    }
 }
 
 void launch_sample_rows(const void *src, uint32_t row_bytes, uint32_t n, uint32_t ns, void *dst,
