@@ -40,6 +40,16 @@ constexpr int ROWB = 128;  // bytes per row per K step (64 bf16 / 128 fp8)
 // 16-byte reads hit distinct banks with 32 or 64 banks.
 __device__ __forceinline__ uint32_t swz(uint32_t r) { return (r ^ (r >> 3)) & 7u; }
 
+// f(integral_constant<int, I>) for I = B .. E-1, in order
+template <int I, int E, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (I < E) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, E>(f);
+    }
+}
+
 __device__ __forceinline__ uint32_t fkey(float f)
 {
     f = f + 0.0f;
@@ -347,8 +357,8 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
     __shared__ __align__(1024) uint8_t sA0[SM * ROWB], sA1[SM * ROWB];
     __shared__ __align__(1024) uint8_t sB0[SN * ROWB], sB1[SN * ROWB];
     // epilogue operands of the current and the next tile: per query row
-    // (scale, threshold) pairs, per base row (norm term, scale)
-    __shared__ __align__(16) float2 s_rq[2][SM];
+    // scale and threshold, per base row norm term and scale
+    __shared__ __align__(16) float s_rs[2][SM], s_rt[2][SM];
     __shared__ float s_cv[2][SN], s_cs[2][SN];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -422,7 +432,8 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
     auto store_ops = [&](int par, float o0, float o1) __attribute__((always_inline)) {
         const uint32_t tid = opaque_tid();
         if (tid < SM) {
-            s_rq[par][tid] = make_float2(o0, o1);
+            s_rs[par][tid] = o0;
+            s_rt[par][tid] = o1;
         } else {
             s_cv[par][tid - SM] = o0;
             s_cs[par][tid - SM] = o1;
@@ -448,11 +459,34 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
         }
     }
 
+    // FP8: the MFMA applies the rows' power-of-two scales, so acc holds the
+    // dot product s itself.  Euclidean keys cv - 2 s are tested as halves:
+    // t = s - cv/2 (cv/2 = +inf for a column past n), key = -2 t exactly,
+    // and key < rt becomes t > -rt/2 (scaling by -2 and halving are exact),
+    // one max over a lane's four columns and one compare per row.
+    constexpr bool HALFKEY = ELEM == 1 && MET == HM_METRIC_EUCLIDEAN;
     floatx4 acc[8][4];
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // E8M0 scale bytes of the lane's rows: query blocks i (byte i & 3 of
+    // sa_pk[i >> 2]) and base blocks j (byte j of sb_pk)
+    uint32_t sa_pk[2] = {0u, 0u}, sb_pk = 0u;
+    auto tile_start = [&](int par) __attribute__((always_inline)) {
+        const uint32_t lane = opaque_tid() & 63u;
+        const uint32_t lr = lane & 15u;
+        if constexpr (ELEM == 1) {
+            sa_pk[0] = sa_pk[1] = sb_pk = 0u;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                sa_pk[i >> 2] |= ((__float_as_uint(s_rs[par][wr * 128 + i * 16 + lr]) >> 23) & 0xFFu)
+                                 << (8 * (i & 3));
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                sb_pk |= ((__float_as_uint(s_cs[par][wc * 64 + j * 16 + lr]) >> 23) & 0xFFu) << (8 * j);
+        }
+    };
 
     auto epilogue = [&](uint32_t bt_, uint32_t qt_, int par) __attribute__((always_inline)) {
         constexpr float PINF = __builtin_huge_valf();
@@ -468,42 +502,52 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
             cok[j] = n0 + cl < n;
             cols[j] = s_cs[par][cl];
             cvv[j] = (MET == HM_METRIC_EUCLIDEAN && !cok[j]) ? PINF : s_cv[par][cl];
+            if constexpr (HALFKEY) cvv[j] = 0.5f * cvv[j];
         }
+        // the thresholds of the lane's 32 rows, one 16-byte read per block
+        float4 rt4[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            rt4[i] = *reinterpret_cast<const float4 *>(&s_rt[par][wr * 128 + i * 16 + lk * 4]);
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            // the (scale, threshold) pairs of the lane's four rows of block i,
-            // read per block (two 16-byte reads) to keep registers for acc
-            const float4 *rq4 = reinterpret_cast<const float4 *>(&s_rq[par][wr * 128 + i * 16 + lk * 4]);
-            const float4 rqa = rq4[0], rqb = rq4[1];
-            const float rsv[4] = {rqa.x, rqa.z, rqb.x, rqb.z}, rtv[4] = {rqa.y, rqa.w, rqb.y, rqb.w};
+            const float rtv[4] = {rt4[i].x, rt4[i].y, rt4[i].z, rt4[i].w};
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const uint32_t ql = wr * 128 + i * 16 + lk * 4 + e;
                 const uint32_t q = q0 + ql;
-                const float rs = rsv[e];
                 const float rt = q < nq ? rtv[e] : -PINF;
                 float key[4];
+                bool hit;
+                if constexpr (HALFKEY) {
+                    // key < rt  <=>  t > -rt/2; fmaxf drops a NaN, whose key
+                    // never compares below rt anyway
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const float a = acc[i][j][e];
-                    if constexpr (MET == HM_METRIC_EUCLIDEAN) {
-                        // cv - (2 rs cs) acc: the bits of cv - 2 (acc (rs cs))
-                        if constexpr (ELEM == 1) key[j] = cvv[j] - ((2.0f * rs) * cols[j]) * a;
-                        else key[j] = cvv[j] - 2.0f * a;
-                    } else {
-                        const float sc = ELEM == 1 ? a * (rs * cols[j]) : a;
-                        float kv;
-                        if constexpr (MET == HM_METRIC_DOT_PRODUCT) kv = sc;
-                        else if constexpr (MET == HM_METRIC_COSINE_SIMILARITY) kv = -sc * cvv[j];
-                        else kv = sc * cvv[j];
-                        key[j] = cok[j] ? kv : PINF;
+                    for (int j = 0; j < 4; ++j) key[j] = acc[i][j][e] - cvv[j];  // t (cvv = cv/2)
+                    hit = fmaxf(fmaxf(key[0], key[1]), fmaxf(key[2], key[3])) > -0.5f * rt;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const float a = acc[i][j][e];
+                        if constexpr (MET == HM_METRIC_EUCLIDEAN) {  // BF16: cv - 2 s
+                            key[j] = cvv[j] - 2.0f * a;
+                        } else {  // FP8: acc is s itself (scales applied by the MFMA)
+                            float kv;
+                            if constexpr (MET == HM_METRIC_DOT_PRODUCT) kv = a;
+                            else if constexpr (MET == HM_METRIC_COSINE_SIMILARITY) kv = -a * cvv[j];
+                            else kv = a * cvv[j];
+                            key[j] = cok[j] ? kv : PINF;
+                        }
                     }
+                    // fminf drops a NaN key, which never compares below rt anyway
+                    hit = fminf(fminf(key[0], key[1]), fminf(key[2], key[3])) < rt;
                 }
-                // fminf drops a NaN key, which never compares below rt anyway
-                const float mn = fminf(fminf(key[0], key[1]), fminf(key[2], key[3]));
-                const bool hit = mn < rt;
                 const uint64_t bal = __ballot(hit);
                 if (bal) {
+                    if constexpr (HALFKEY) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) key[j] = -2.0f * key[j];  // cv - 2 s, exactly
+                    }
                     if (hit) {
                         const uint32_t pos =
                             rec_n + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
@@ -518,7 +562,6 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
                     rec_n += (uint32_t)__popcll(bal);
                 }
             }
-            asm volatile("" ::: "memory");  // block i's operand reads stay in block i
         }
     };
 
@@ -546,7 +589,10 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
             set_rows(nbt, nqt);
             issue(Other{}, 0u);
         }
-        if (k == 0 && NL < T) load_ops(nbt, nqt, p0, p1);
+        if (k == 0) {
+            tile_start(par);
+            if (NL < T) load_ops(nbt, nqt, p0, p1);
+        }
         const uint8_t *As = buf ? sA1 : sA0, *Bs = buf ? sB1 : sB0;
         uint4 bv[4][2];
 #pragma unroll
@@ -554,11 +600,13 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
 #pragma unroll
             for (int h = 0; h < 2; ++h) bv[j][h] = *reinterpret_cast<const uint4 *>(Bs + fb[j][h]);
         // query fragments: block i + 1 is read while block i's MFMAs issue
+        // (i and j are compile-time indices: the MFMA's scale-byte selects
+        // must be constants)
         uint4 avb[2][2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) avb[0][h] = *reinterpret_cast<const uint4 *>(As + fa[0][h]);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        static_for<0, 8>([&](auto ic) __attribute__((always_inline)) {
+            constexpr int i = decltype(ic)::value;
             if (i + 1 < 8) {
 #pragma unroll
                 for (int h = 0; h < 2; ++h)
@@ -576,20 +624,22 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
                             __builtin_bit_cast(bf16x8, av[kk]), __builtin_bit_cast(bf16x8, bv[j][kk]),
                             acc[i][j], 0, 0, 0);
             } else {
-                // one MFMA K=128 (e4m3 x e4m3, no block scale); lane holds 32
-                // fp8 at k = 32*lk in A and B alike
+                // one MFMA K=128 (e4m3 x e4m3); lane holds 32 fp8 at k = 32*lk
+                // in A and B alike
                 const i32x8 a = i32x8{(int)av[0].x, (int)av[0].y, (int)av[0].z, (int)av[0].w,
                                       (int)av[1].x, (int)av[1].y, (int)av[1].z, (int)av[1].w};
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                static_for<0, 4>([&](auto jc) __attribute__((always_inline)) {
+                    constexpr int j = decltype(jc)::value;
                     const i32x8 b = i32x8{(int)bv[j][0].x, (int)bv[j][0].y, (int)bv[j][0].z,
                                           (int)bv[j][0].w, (int)bv[j][1].x, (int)bv[j][1].y,
                                           (int)bv[j][1].z, (int)bv[j][1].w};
+                    // the rows' scales applied by the matrix core (E8M0 bytes
+                    // i & 3 of sa_pk[i >> 2] and j of sb_pk)
                     acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-                        a, b, acc[i][j], 0, 0, 0, 0, 0, 0);
-                }
+                        a, b, acc[i][j], 0, 0, i & 3, (int)sa_pk[i >> 2], j, (int)sb_pk);
+                });
             }
-        }
+        });
         if (++k < nk) return false;
         epilogue(bt, qt, par);
         if (NL >= T) return true;
@@ -618,7 +668,10 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
 
 // Appends the keys below the threshold of every record of scan region
 // blockIdx.x to its query's list: one atomic per record reserves its hits.
-__global__ __launch_bounds__(256) void scan_gather_kernel(ScanRecords rec,
+// 1,024 threads per region: a region's records are usually one round, all
+// of their atomics in flight together.
+constexpr int GATHER_THREADS = 1024;
+__global__ __launch_bounds__(GATHER_THREADS) void scan_gather_kernel(ScanRecords rec,
                                                           const float *__restrict__ tau,
                                                           uint32_t tau_ld, uint64_t *__restrict__ list,
                                                           uint32_t cap, uint32_t *__restrict__ cnt)
@@ -627,7 +680,7 @@ __global__ __launch_bounds__(256) void scan_gather_kernel(ScanRecords rec,
     const uint32_t c0 = rec.count[W], c = c0 < rec.R ? c0 : rec.R;
     const float4 *rk = rec.keys + (size_t)W * rec.R;
     const uint2 *rm = rec.meta + (size_t)W * rec.R;
-    for (uint32_t r = threadIdx.x; r < c; r += 256) {
+    for (uint32_t r = threadIdx.x; r < c; r += GATHER_THREADS) {
         const float4 kv = rk[r];
         const uint2 m = rm[r];
         const float t = tau[(size_t)m.x * tau_ld];
@@ -690,8 +743,13 @@ __global__ void to_bf16_kernel(const float *__restrict__ src, uint64_t lds, uint
     if (bad && inexact) atomicOr(inexact, 1u);
 }
 
-// Per-row scaled FP8-e4m3: x * (224 / max|x|) rounded by the hardware
-// converter; dequant[r] = max|x| / 224.  One wave per row.
+// Per-row scaled FP8-e4m3 with a power-of-two scale: x * mult rounded by
+// the hardware converter, mult = 2^(8 - e) for max|x| = f 2^e, f in
+// [0.5, 1), so that max|x| * mult lies in [128, 256) (below e4m3's 448);
+// dequant[r] = 1 / mult exactly (0 for a zero row).  The scan applies it
+// as the row's E8M0 block scale (fp8_scale_bits).  One wave per row.
+__device__ __forceinline__ float fp8_mult(float m) { int e = 0; frexpf(m, &e); return m > 0.0f ? ldexpf(1.0f, 8 - e) : 1.0f; }
+__device__ __forceinline__ float fp8_dequant(float m) { int e = 0; frexpf(m, &e); return m > 0.0f ? ldexpf(1.0f, e - 8) : 0.0f; }
 __global__ void to_fp8_kernel(const float *__restrict__ src, uint64_t lds, uint32_t rows,
                               uint32_t d, uint32_t dk, uint8_t *__restrict__ dst,
                               float *__restrict__ dequant, const uint32_t *__restrict__ dev_rows)
@@ -704,8 +762,8 @@ __global__ void to_fp8_kernel(const float *__restrict__ src, uint64_t lds, uint3
     for (uint32_t c = lane; c < d; c += 64) m = fmaxf(m, fabsf(x[c]));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-    const float scale = m > 0.0f ? 224.0f / m : 1.0f;
-    if (lane == 0) dequant[r] = m > 0.0f ? m / 224.0f : 0.0f;
+    const float scale = fp8_mult(m);
+    if (lane == 0) dequant[r] = fp8_dequant(m);
     uint8_t *o = dst + (size_t)r * dk;
     for (uint32_t c = 2 * lane; c < dk; c += 128) {
         float a = c < d ? x[c] * scale : 0.0f;
@@ -722,8 +780,8 @@ __global__ void to_fp8_kernel(const float *__restrict__ src, uint64_t lds, uint3
 //  - err |= 2 if a value is NaN or infinite;
 //  - norm2[r] = sum_t x_t*x_t as a left fold from -0.0 (lane 0 of the wave
 //    over the row staged in LDS; the same bits as self_dot_kernel);
-//  - MODE 1: the scaled FP8-e4m3 row of to_fp8_kernel (scale 224 / max|x|,
-//    dequant[r] = max|x| / 224), four bytes per lane store;
+//  - MODE 1: the scaled FP8-e4m3 row of to_fp8_kernel (power-of-two scale,
+//    dequant[r] its inverse), four bytes per lane store;
 //    MODE 2: the RNE BF16 row of to_bf16_kernel, *inexact set if a value
 //    does not survive;
 //  - pad (when given): the row zero-padded to dpad f32 columns.
@@ -781,8 +839,8 @@ __global__ __launch_bounds__(256) void query_prep_kernel(
     if constexpr (MODE == 1) {
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-        const float scale = m > 0.0f ? 224.0f / m : 1.0f;
-        if (lane == 0) dequant[r] = m > 0.0f ? m / 224.0f : 0.0f;
+        const float scale = fp8_mult(m);
+        if (lane == 0) dequant[r] = fp8_dequant(m);
         uint32_t *o = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(dst) + (size_t)r * dk);
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
@@ -1001,7 +1059,7 @@ void launch_score_filter(int elem_bytes, const void *Q, uint32_t nq, const void 
     }
     {
         ProfScope prof("list_gather", s);
-        scan_gather_kernel<<<(unsigned)waves, 256, 0, s>>>(rec, tau, tau_ld, list, cap, cnt);
+        scan_gather_kernel<<<(unsigned)waves, GATHER_THREADS, 0, s>>>(rec, tau, tau_ld, list, cap, cnt);
         HM_CHECK_LAUNCH();
     }
 }
