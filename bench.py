@@ -254,7 +254,7 @@ def main_bf(args):
 
     per_metric = {"Euclidean": b["qps"]}
     if not args.no_metrics:
-        for mt in (Metric.Manhattan, Metric.DotProduct, Metric.CosineSimilarity):
+        for mt in (Metric.Manhattan, Metric.DotProduct, Metric.CosineSimilarity, Metric.Angular):
             ixm = make(mt, best)
             wall, _ = timer.run(step_fn(ixm), 1, 1, f"{mt.name}/{best}")
             per_metric[mt.name] = round(NQ / wall, 1)

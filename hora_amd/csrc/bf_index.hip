@@ -209,14 +209,10 @@ class BruteForce final : public Index {
 
   private:
     // Metrics with an MFMA shortlist form.  Manhattan has no GEMM form.
-    // Angular's shortlist form is switched off: its score epilogue faulted
-    // on the device (a store through an unset address register on that
-    // branch arm), and the rebuilt kernel has not yet run there; Angular
-    // requests in a shortlist mode run the exact form.
     bool shortlist_metric() const
     {
         return metric_ == HM_METRIC_EUCLIDEAN || metric_ == HM_METRIC_DOT_PRODUCT ||
-               metric_ == HM_METRIC_COSINE_SIMILARITY;
+               metric_ == HM_METRIC_COSINE_SIMILARITY || metric_ == HM_METRIC_ANGULAR;
     }
 
     // A zero-norm base row has no Cosine/Angular key (0 * inf), so the

@@ -14,15 +14,9 @@ from hora_amd.datasets import bf_normal, bf_sparse_u8, unit_normal
 
 pytestmark = pytest.mark.gpu
 
-# Angular's shortlist form faulted the device twice (a store through an
-# address register left unset on the Angular arm of the score epilogue; the
-# metric is now a template parameter).  The rebuilt kernel has not run on a
-# device yet, so the form stays switched off (Angular runs the exact form)
-# and its cases are skipped until it is verified.
-_ANGULAR_SKIP = pytest.mark.skip(reason="Angular shortlist form switched off: device fault in "
-                                 "its score epilogue, fixed but not yet re-verified")
-GEMM_METRICS = [Metric.Euclidean, Metric.DotProduct, Metric.CosineSimilarity,
-                pytest.param(Metric.Angular, marks=_ANGULAR_SKIP)]
+# Angular runs the same score kernels as Cosine: its instantiations are the
+# Cosine machine code with the products' signs flipped.
+GEMM_METRICS = [Metric.Euclidean, Metric.DotProduct, Metric.CosineSimilarity, Metric.Angular]
 
 
 def _index(base, mt, mode, shortlist=0):
@@ -92,7 +86,7 @@ def test_bf16_large_k_and_small_n(orc):
         _exact_check(orc, idx, base, q, k, Metric.Euclidean)
 
 
-@pytest.mark.parametrize("mt", [Metric.DotProduct, Metric.CosineSimilarity],
+@pytest.mark.parametrize("mt", [Metric.DotProduct, Metric.CosineSimilarity, Metric.Angular],
                          ids=lambda m: m.name)
 def test_fp8_recall_and_exact_distances(orc, mt):
     base = unit_normal(20_000, 256, seed=441)
@@ -130,8 +124,7 @@ def test_nan_distances_fail_like_the_exact_form():
         assert e.value.status == 8
 
 
-@pytest.mark.parametrize("mt", [Metric.Euclidean, Metric.DotProduct, Metric.CosineSimilarity],
-                         ids=lambda m: Metric(m).name)
+@pytest.mark.parametrize("mt", GEMM_METRICS, ids=lambda m: Metric(m).name)
 @pytest.mark.parametrize("mode", [hora_amd.HM_SEARCH_BF16_RERANK, hora_amd.HM_SEARCH_FP8_RERANK],
                          ids=["bf16", "fp8"])
 def test_filtered_scan_on_a_large_batch(orc, mt, mode):
