@@ -71,6 +71,9 @@ template <class S> struct WaveHeap {
         const uint32_t p = len++;
         const uint32_t A = 31u - (uint32_t)__clz(p + 1);  // ancestors of p
         const uint2 u = s.ld(lane < A ? ((p + 1) >> (lane + 1)) - 1 : 0u);
+        // both words in one load: left to itself the compiler loads the item
+        // under the move branch, a second LDS round trip
+        asm volatile("" ::"v"(u.x), "v"(u.y));
         const uint64_t m = __ballot(lane < A && val(x) > val(u));
         const uint32_t t = (uint32_t)__builtin_ctzll(~m);  // levels climbed
         if (lane < t) s.st(((p + 1) >> lane) - 1, u);       // ancestor lane + 1 moves down

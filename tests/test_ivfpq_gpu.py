@@ -168,6 +168,20 @@ def test_ties_replay(orc, form):
     assert idx.search_stats()[0] > 0  # rows went through the replay
 
 
+@pytest.mark.parametrize("form", ["lists", "rows"])
+def test_deep_list_heaps_with_ties(orc, form):
+    # every row twice in lists of ~3,000 items and k=100: the per-list heap
+    # replays run through levels 7-11 (subtree pushes and their events,
+    # pop tasks two to a lane) for every tied query
+    x, _, _ = gaussian_mixture(3_000, 32, 2, seed=61, sigma=0.3)
+    x = np.repeat(x, 2, axis=0)
+    idx = _build(x, Metric.Euclidean, 2, 4, 8, 3, 63, nprobe=2)
+    idx.set_scan_form(form)
+    q, _, _ = gaussian_mixture(24, 32, 2, seed=62, sigma=0.3)
+    _check_conditional(orc, idx, x, Metric.Euclidean, q, 2, 100)
+    assert idx.search_stats()[0] > 0
+
+
 @pytest.mark.parametrize("shape", [(64, 16, 8, 24), (32, 4, 4, 40), (48, 6, 5, 12),
                                    (96, 3, 8, 6)],
                          ids=["16x256x4", "4x16x8", "6x32x8", "3x256x32"])
