@@ -687,7 +687,8 @@ class BruteForce final : public Index {
                                 rg_r8n_.as<float>(), g, cnt);
             launch_exact_rows_dev(metric_, rg_r8p_.as<float>(), dpad_, base_.as<float>(), dpad_, n,
                                   dpad_, rg_r8n_.as<float>(), bnorm_.as<float>(),
-                                  rg_r8D_.as<float>(), ldd, cnt, g);
+                                  rg_r8D_.as<float>(), ldd, cnt, g,
+                                  b8_valid_ ? b8_.as<uint8_t>() : nullptr);
             launch_bf_finalize(rg_cd_.as<float>(), rg_ci_.as<uint32_t>(), 1, pl + RERUN_SLOTS,
                                rg_r8D_.as<float>(), ldd, RERUN_SLOTS, n, kmax,
                                rg_r8i_.as<uint64_t>(), rg_r8d_.as<float>(),
@@ -1007,7 +1008,7 @@ class BruteForce final : public Index {
                                 r8n_.as<float>(), s, cnt);
             launch_exact_rows_dev(metric_, r8p_.as<float>(), dpad_, base_.as<float>(), dpad_, n,
                                   dpad_, r8n_.as<float>(), bnorm_.as<float>(), r8D_.as<float>(),
-                                  ldd, cnt, s);
+                                  ldd, cnt, s, b8_valid_ ? b8_.as<uint8_t>() : nullptr);
             launch_bf_finalize(cand_d_.as<float>(), cand_i_.as<uint32_t>(), 1, pl + RERUN_SLOTS,
                                r8D_.as<float>(), ldd, RERUN_SLOTS, n, k, r8i_.as<uint64_t>(),
                                r8d_.as<float>(), r8c_.as<uint32_t>(), err_.as<uint32_t>(), s);

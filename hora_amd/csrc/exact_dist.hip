@@ -243,6 +243,56 @@ __device__ __forceinline__ void exact_rows_body(
     }
 }
 
+// The same rows from the u8 copy of an integer base (every value is the f32
+// of its byte, so the converted values, the folds and the bits are those of
+// the f32 rows): a quarter of the bytes, 16 dimensions per 16-byte load.
+template <int ACC, int EPI, int RQ>
+__device__ __forceinline__ void exact_rows_body_u8(
+    const float *__restrict__ Q, uint32_t ldq, uint32_t nq, const uint8_t *__restrict__ B8,
+    uint32_t ld8, uint32_t n, uint32_t dpad, const float *__restrict__ qnorm,
+    const float *__restrict__ bnorm, float *__restrict__ D, uint64_t ldd)
+{
+    extern __shared__ __align__(16) float xr_q[];  // [RQ][dpad]
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t e = tid; e < RQ * dpad; e += 256) {
+        const uint32_t r = e / dpad, t = e % dpad;
+        xr_q[e] = r < nq ? Q[(size_t)r * ldq + t] : 0.0f;
+    }
+    __syncthreads();
+    const uint32_t j = blockIdx.x * 256 + tid;
+    const uint8_t *b = B8 + (size_t)(j < n ? j : n - 1) * ld8;
+    float acc[RQ];
+#pragma unroll
+    for (int r = 0; r < RQ; ++r) acc[r] = -0.0f;
+#pragma unroll 2
+    for (uint32_t t = 0; t < dpad; t += 16) {
+        const uint4 raw = *reinterpret_cast<const uint4 *>(b + t);
+        const uint32_t w[4] = {raw.x, raw.y, raw.z, raw.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float b0 = (float)(w[u] & 0xFFu), b1 = (float)((w[u] >> 8) & 0xFFu);
+            const float b2 = (float)((w[u] >> 16) & 0xFFu), b3 = (float)(w[u] >> 24);
+#pragma unroll
+            for (int r = 0; r < RQ; ++r) {
+                const float4 qv = *reinterpret_cast<const float4 *>(xr_q + r * dpad + t + 4 * u);
+                acc[r] = acc_step<ACC>(acc[r], qv.x, b0);
+                acc[r] = acc_step<ACC>(acc[r], qv.y, b1);
+                acc[r] = acc_step<ACC>(acc[r], qv.z, b2);
+                acc[r] = acc_step<ACC>(acc[r], qv.w, b3);
+            }
+        }
+    }
+    if (j >= n) return;
+#pragma unroll
+    for (int r = 0; r < RQ; ++r) {
+        if ((uint32_t)r >= nq) break;
+        float a = acc[r];
+        if constexpr (EPI == EPI_COSINE) a = epi_cosine(a, qnorm[r], bnorm[j]);
+        else if constexpr (EPI == EPI_ANGULAR) a = epi_angular(a, qnorm[r], bnorm[j]);
+        D[(size_t)r * ldd + j] = a;
+    }
+}
+
 template <int ACC, int EPI, int RQ>
 __global__ __launch_bounds__(256) void exact_rows_kernel(
     const float *__restrict__ Q, uint32_t ldq, uint32_t nq, const float *__restrict__ B,
@@ -254,14 +304,21 @@ __global__ __launch_bounds__(256) void exact_rows_kernel(
 
 // The first *dev_nq (<= 8) rows: nothing for 0, the one-query body for 1,
 // the eight-query body otherwise (LDS for eight query rows).
+// (from the u8 copy B8, rows dpad bytes apart, when it is given)
 template <int ACC, int EPI>
 __global__ __launch_bounds__(256) void exact_rows_dev_kernel(
     const float *__restrict__ Q, uint32_t ldq, const float *__restrict__ B, uint32_t ldb,
     uint32_t n, uint32_t dpad, const float *__restrict__ qnorm, const float *__restrict__ bnorm,
-    float *__restrict__ D, uint64_t ldd, const uint32_t *__restrict__ dev_nq)
+    float *__restrict__ D, uint64_t ldd, const uint32_t *__restrict__ dev_nq,
+    const uint8_t *__restrict__ B8)
 {
     const uint32_t c = min(*dev_nq, 8u);
     if (c == 0) return;
+    if (B8) {
+        if (c == 1) exact_rows_body_u8<ACC, EPI, 1>(Q, ldq, 1, B8, dpad, n, dpad, qnorm, bnorm, D, ldd);
+        else exact_rows_body_u8<ACC, EPI, 8>(Q, ldq, c, B8, dpad, n, dpad, qnorm, bnorm, D, ldd);
+        return;
+    }
     if (c == 1) exact_rows_body<ACC, EPI, 1>(Q, ldq, 1, B, ldb, n, dpad, qnorm, bnorm, D, ldd);
     else exact_rows_body<ACC, EPI, 8>(Q, ldq, c, B, ldb, n, dpad, qnorm, bnorm, D, ldd);
 }
@@ -524,15 +581,18 @@ bool exact_rows_dev_ok(uint32_t dpad, uint32_t ldb, const float *B)
 
 void launch_exact_rows_dev(int metric, const float *Q, uint32_t ldq, const float *B, uint32_t ldb,
                            uint32_t n, uint32_t dpad, const float *qnorm, const float *bnorm,
-                           float *D, uint64_t ldd, const uint32_t *dev_nq, hipStream_t s)
+                           float *D, uint64_t ldd, const uint32_t *dev_nq, hipStream_t s,
+                           const uint8_t *B8)
 {
     if (n == 0) return;
     if (!exact_rows_dev_ok(dpad, ldb, B)) fail(HM_ERR_INVALID_ARG, "exact_rows_dev: shape");
+    // the u8 rows need 16-byte aligned rows of dpad bytes
+    if (B8 && (dpad % 16 != 0 || ((uintptr_t)B8 & 15) != 0)) B8 = nullptr;
     ProfScope prof("exact_rows", s);
     const unsigned blocks = ceil_div(n, 256);
 #define HM_ROWS_DEV(ACC, EPI)                                                                  \
     exact_rows_dev_kernel<ACC, EPI><<<blocks, 256, (size_t)8 * dpad * 4, s>>>(                 \
-        Q, ldq, B, ldb, n, dpad, qnorm, bnorm, D, ldd, dev_nq)
+        Q, ldq, B, ldb, n, dpad, qnorm, bnorm, D, ldd, dev_nq, B8)
     switch (metric) {
     case HM_METRIC_EUCLIDEAN: HM_ROWS_DEV(ACC_EUCLID, EPI_NONE); break;
     case HM_METRIC_MANHATTAN: HM_ROWS_DEV(ACC_MANHATTAN, EPI_NONE); break;

@@ -39,11 +39,13 @@ void launch_exact_tile(int metric, const float *Q, uint32_t ldq, uint32_t nq, co
                        const float *bnorm, float *D, uint64_t ldd, hipStream_t s);
 // The exact rows of the first *dev_nq (<= 8) of 8 query rows; nothing when
 // *dev_nq is 0.  Needs dpad <= 2048 and 16-byte aligned B rows
-// (exact_rows_dev_ok).
+// (exact_rows_dev_ok).  B8, when given: a u8 copy of B ([n][dpad] bytes)
+// holding every value exactly, read instead of B (same bits).
 bool exact_rows_dev_ok(uint32_t dpad, uint32_t ldb, const float *B);
 void launch_exact_rows_dev(int metric, const float *Q, uint32_t ldq, const float *B, uint32_t ldb,
                            uint32_t n, uint32_t dpad, const float *qnorm, const float *bnorm,
-                           float *D, uint64_t ldd, const uint32_t *dev_nq, hipStream_t s);
+                           float *D, uint64_t ldd, const uint32_t *dev_nq, hipStream_t s,
+                           const uint8_t *B8 = nullptr);
 // best[q] = min over j of (fkey(metric(Q[q], B[j])) << 32 | j); best must be
 // initialised to ~0 by the caller.
 void launch_exact_argmin(int metric, const float *Q, uint32_t ldq, uint32_t nq, const float *B,

@@ -167,3 +167,19 @@ def test_u8_copy_of_integer_base_keeps_results_exact(orc, shortlist, spoil):
     for r in range(q.shape[0]):  # every returned distance is the reference's value
         ids = gi[r, :gc[r]].astype(np.int64)
         assert np.array_equal(gd[r, :gc[r]].view(np.uint32), ref[r, ids].view(np.uint32))
+
+
+@pytest.mark.parametrize("mt", [Metric.Euclidean, Metric.DotProduct, Metric.CosineSimilarity],
+                         ids=lambda m: Metric(m).name)
+@pytest.mark.parametrize("mode", [hora_amd.HM_SEARCH_BF16_RERANK, hora_amd.HM_SEARCH_FP8_RERANK],
+                         ids=["bf16", "fp8"])
+def test_tied_rows_rerun_from_the_u8_copy(orc, mt, mode):
+    # every integer row twice, six queries: each query's top-(k+1) holds equal
+    # distances, so every row is rerun on the device (exact row from the u8
+    # copy of the base, then the heap replay) and must be the reference's
+    half = bf_sparse_u8(3_000, 784, seed=471)
+    base = np.concatenate([half, half])
+    q = bf_sparse_u8(6, 784, seed=472)
+    idx = _index(base, mt, mode)
+    rerun, _ = _exact_check(orc, idx, base, q, 10, mt)
+    assert rerun == 6
