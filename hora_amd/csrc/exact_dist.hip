@@ -302,9 +302,10 @@ __global__ __launch_bounds__(256) void exact_rows_kernel(
     exact_rows_body<ACC, EPI, RQ>(Q, ldq, nq, B, ldb, n, dpad, qnorm, bnorm, D, ldd);
 }
 
-// The first *dev_nq (<= 8) rows: nothing for 0, the one-query body for 1,
-// the eight-query body otherwise (LDS for eight query rows).
-// (from the u8 copy B8, rows dpad bytes apart, when it is given)
+// The first *dev_nq (<= RERUN_SLOTS) rows, eight per workgroup row of the
+// grid (blockIdx.y): nothing past the count, the one-query body for a group
+// of 1, the eight-query body otherwise (LDS for eight query rows); from the
+// u8 copy B8 (rows dpad bytes apart) when it is given.
 template <int ACC, int EPI>
 __global__ __launch_bounds__(256) void exact_rows_dev_kernel(
     const float *__restrict__ Q, uint32_t ldq, const float *__restrict__ B, uint32_t ldb,
@@ -312,8 +313,12 @@ __global__ __launch_bounds__(256) void exact_rows_dev_kernel(
     float *__restrict__ D, uint64_t ldd, const uint32_t *__restrict__ dev_nq,
     const uint8_t *__restrict__ B8)
 {
-    const uint32_t c = min(*dev_nq, 8u);
-    if (c == 0) return;
+    const uint32_t g0 = blockIdx.y * 8u, all = min(*dev_nq, RERUN_SLOTS);
+    if (g0 >= all) return;
+    const uint32_t c = min(all - g0, 8u);
+    Q += (size_t)g0 * ldq;
+    if (qnorm) qnorm += g0;
+    D += (size_t)g0 * ldd;
     if (B8) {
         if (c == 1) exact_rows_body_u8<ACC, EPI, 1>(Q, ldq, 1, B8, dpad, n, dpad, qnorm, bnorm, D, ldd);
         else exact_rows_body_u8<ACC, EPI, 8>(Q, ldq, c, B8, dpad, n, dpad, qnorm, bnorm, D, ldd);
@@ -591,7 +596,7 @@ void launch_exact_rows_dev(int metric, const float *Q, uint32_t ldq, const float
     ProfScope prof("exact_rows", s);
     const unsigned blocks = ceil_div(n, 256);
 #define HM_ROWS_DEV(ACC, EPI)                                                                  \
-    exact_rows_dev_kernel<ACC, EPI><<<blocks, 256, (size_t)8 * dpad * 4, s>>>(                 \
+    exact_rows_dev_kernel<ACC, EPI><<<dim3(blocks, RERUN_SLOTS / 8), 256, (size_t)8 * dpad * 4, s>>>( \
         Q, ldq, B, ldb, n, dpad, qnorm, bnorm, D, ldd, dev_nq, B8)
     switch (metric) {
     case HM_METRIC_EUCLIDEAN: HM_ROWS_DEV(ACC_EUCLID, EPI_NONE); break;

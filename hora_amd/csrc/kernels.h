@@ -29,7 +29,7 @@ constexpr uint32_t ROW_SKIP = 0x80000000u;  // bf_finalize: leave the row's outp
 // is 1 when the host must decide (more tied rows than slots, or rows flagged
 // for another reason; the count is then 0), the number of flagged rows and
 // the number of uncertified rows.
-constexpr uint32_t RERUN_SLOTS = 8, RERUN_PLAN_WORDS = 2 * RERUN_SLOTS + 4;
+constexpr uint32_t RERUN_SLOTS = 32, RERUN_PLAN_WORDS = 2 * RERUN_SLOTS + 4;
 
 // ---- exact_dist.hip
 // D[q][j] = metric(Q[q], B[j]) in the reference's summation order.
@@ -37,9 +37,9 @@ constexpr uint32_t RERUN_SLOTS = 8, RERUN_PLAN_WORDS = 2 * RERUN_SLOTS + 4;
 void launch_exact_tile(int metric, const float *Q, uint32_t ldq, uint32_t nq, const float *B,
                        uint32_t ldb, uint32_t n, uint32_t dpad, const float *qnorm,
                        const float *bnorm, float *D, uint64_t ldd, hipStream_t s);
-// The exact rows of the first *dev_nq (<= 8) of 8 query rows; nothing when
-// *dev_nq is 0.  Needs dpad <= 2048 and 16-byte aligned B rows
-// (exact_rows_dev_ok).  B8, when given: a u8 copy of B ([n][dpad] bytes)
+// The exact rows of the first *dev_nq (<= RERUN_SLOTS) of RERUN_SLOTS query
+// rows, eight per workgroup row of the grid; nothing when *dev_nq is 0.
+// Needs dpad <= 2048 and 16-byte aligned B rows (exact_rows_dev_ok).  B8, when given: a u8 copy of B ([n][dpad] bytes)
 // holding every value exactly, read instead of B (same bits).
 bool exact_rows_dev_ok(uint32_t dpad, uint32_t ldb, const float *B);
 void launch_exact_rows_dev(int metric, const float *Q, uint32_t ldq, const float *B, uint32_t ldb,

@@ -525,10 +525,49 @@ __global__ __launch_bounds__(BF_THREADS) void bf_finalize_wave_kernel(
         }
         return __ballot(bad) == 0;
     };
-    auto skip = [&]() { return h.len == m && stable; };
+    // Once the heap is full, an element above the root climbs to the root
+    // and is popped again: that round trip is a map F of the heap's layout
+    // that does not depend on the element's value (it beats every entry).
+    // A run of j such elements applies F^j; F is the identity while the
+    // layout is stable (noop), and otherwise the layouts F, F^2, ... either
+    // become stable or repeat with some period p, so F^j is F^(j mod p)
+    // after the first repeat.  Up to RT_HIST layouts are compared.
+    constexpr uint32_t RT_HIST = 8;
+    auto round_trip = [&]() {  // F: push an element above every entry, pop it
+        h.push(make_uint2(~0u, 0x7f800000u));  // +inf beats every (finite) entry
+        (void)h.pop();
+    };
+    auto apply_runs = [&](uint64_t j) {
+        if (j == 0 || stable) return;
+        uint32_t hist[RT_HIST];
+        hist[0] = h.it;
+        uint64_t t = 0;
+        while (t < j) {
+            round_trip();
+            ++t;
+            if (noop()) {  // stable from here on: the rest are no-ops
+                stable = true;
+                return;
+            }
+            const uint32_t seen = t < RT_HIST ? (uint32_t)t : RT_HIST;
+            uint32_t prev = RT_HIST;  // an earlier layout equal to this one
+            for (uint32_t q = 0; q < seen; ++q)
+                if (prev == RT_HIST && __ballot((uint32_t)lane < h.len && hist[q] != h.it) == 0)
+                    prev = q;
+            if (prev != RT_HIST) {  // period t - prev
+                const uint64_t left = (j - t) % (t - prev);
+                for (uint64_t u = 0; u < left; ++u) round_trip();
+                stable = noop();
+                return;
+            }
+            if (t < RT_HIST) hist[t] = h.it;
+        }
+    };
+    auto skip = [&]() { return h.len == m; };
     // first block at or after b that can change the heap: inside a group
     // being visited the next block; else the first block of the next group
-    // whose minimum is not above the root (any group while not skipping)
+    // whose minimum is not above the root (any group while the heap is not
+    // full); the caller applies the skipped elements' round trips
     auto next_block = [&](uint32_t b) -> uint32_t {
         if (b >= nb) return nb;
         if (b % bpg != 0 || !skip()) return b;
@@ -563,6 +602,7 @@ __global__ __launch_bounds__(BF_THREADS) void bf_finalize_wave_kernel(
                     const uint64_t live = __ballot(idx >= e && idx < cnt && !(vc[c] > root));
                     if (live) best = min(best, 4 * (uint32_t)__builtin_ctzll(live) + c);
                 }
+                apply_runs((best < cnt ? best : cnt) - e);  // the elements above the root
                 if (best >= cnt) break;
                 e = best;
             }
@@ -578,9 +618,10 @@ __global__ __launch_bounds__(BF_THREADS) void bf_finalize_wave_kernel(
             }
             ++e;
         }
-        // While the layout stays stable the root only falls, so the next
-        // block is at or after nx; after a loss of stability it is b + 1.
+        // The root only falls while the heap is full, so the next block is
+        // at or after nx; the whole blocks skipped hold elements above it.
         const uint32_t b2 = next_block(b + 1);
+        if (b2 > b + 1) apply_runs((uint64_t)min(b2 * BF_BLOCK, n) - (uint64_t)(b + 1) * BF_BLOCK);
         v = b2 == nx ? pf : ld(b2);
         b = b2;
     }

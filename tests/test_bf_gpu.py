@@ -100,6 +100,26 @@ def test_ties_replay_rows_past_4m(orc):
         [[1000 + r, 4_200_000 + r] for r in range(3)]
 
 
+@pytest.mark.parametrize("mt", [Metric.Angular, Metric.CosineSimilarity], ids=lambda m: m.name)
+@pytest.mark.parametrize("k", [10, 40, 63])
+def test_ties_inside_the_heap(orc, mt, k):
+    # every direction four times (x, 2x, 4x, 8x: the same cosine to the
+    # last bit), so a full heap holds equal distances and its layout is not
+    # stable: runs of elements above the root take the round-trip map F^j
+    # (until stable, or by its period), and the result must still be the
+    # reference's, in the exact form and in the FP8 form's rerun
+    half = bf_sparse_u8(3_000, 784, seed=481)
+    base = np.concatenate([half, 2 * half, 4 * half, 8 * half]).astype(np.float32)
+    q = bf_sparse_u8(20, 784, seed=482)
+    _, rd1, _ = orc.bf_search(base, q, k + 1, int(mt))
+    assert np.all(np.any(rd1[:, 1:] == rd1[:, :-1], axis=1))  # every row tied
+    idx = _index(base, mt)
+    _check(orc, base, q, k, mt, idx)
+    idx.set_search_mode(hora_amd.HM_SEARCH_FP8_RERANK, 0)
+    _check(orc, base, q, k, mt, idx)
+    assert idx.search_stats()[0] == q.shape[0]
+
+
 def test_hand_traced_tie_case(orc):
     # distances [1,1,1], k=2: the reference returns ids [2, 1]
     base = np.array([[1.0], [1.0], [1.0]], np.float32)

@@ -173,13 +173,15 @@ def test_u8_copy_of_integer_base_keeps_results_exact(orc, shortlist, spoil):
                          ids=lambda m: Metric(m).name)
 @pytest.mark.parametrize("mode", [hora_amd.HM_SEARCH_BF16_RERANK, hora_amd.HM_SEARCH_FP8_RERANK],
                          ids=["bf16", "fp8"])
-def test_tied_rows_rerun_from_the_u8_copy(orc, mt, mode):
-    # every integer row twice, six queries: each query's top-(k+1) holds equal
-    # distances, so every row is rerun on the device (exact row from the u8
-    # copy of the base, then the heap replay) and must be the reference's
+@pytest.mark.parametrize("nq", [6, 30, 40])
+def test_tied_rows_rerun_from_the_u8_copy(orc, mt, mode, nq):
+    # every integer row twice: each query's top-(k+1) holds equal distances,
+    # so every row is rerun -- on the device for up to 32 rows (exact rows
+    # from the u8 copy of the base, eight per group, then the heap replay),
+    # on the exact path past that -- and must be the reference's
     half = bf_sparse_u8(3_000, 784, seed=471)
     base = np.concatenate([half, half])
-    q = bf_sparse_u8(6, 784, seed=472)
+    q = bf_sparse_u8(nq, 784, seed=472)
     idx = _index(base, mt, mode)
     rerun, _ = _exact_check(orc, idx, base, q, 10, mt)
-    assert rerun == 6
+    assert rerun == nq
