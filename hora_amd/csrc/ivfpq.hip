@@ -1488,12 +1488,89 @@ __global__ __launch_bounds__(64) void ivf_merge_replay_kernel(
     const uint32_t t = blockIdx.x, q = rows[t];
     const int tid = threadIdx.x;
     WaveHeap<WhPackedLds> top({reinterpret_cast<uint2 *>(mr_smem)}, (uint32_t)tid);  // [k + 3]
+    const uint32_t lane = (uint32_t)tid;
+    // Once the heap holds k entries, an element above the root climbs to the
+    // root and is popped again: a map F of the layout that does not depend
+    // on the element's value.  A list's sequence is in pop order (largest
+    // first), so it starts with a run of j such elements and every later
+    // element is at most the root.  F^j: nothing while the layout is stable
+    // (the round trip retraces the push's path), else F, F^2, ... until the
+    // layout is stable or repeats one of the last RT_HIST, then the rest by
+    // the period.
+    constexpr uint32_t RT_HIST = 8;
+    auto round_trip = [&]() {
+        top.push(make_uint2(~0u, 0x7f800000u));  // +inf beats every finite entry
+        (void)top.pop();
+    };
+    // heap_noop_roundtrip for the full heap, lane j checking step j of the
+    // walk from position L = len to the root
+    auto noop = [&]() -> bool {
+        const uint32_t L = top.len;
+        if (L == 0) return true;
+        const uint32_t steps = 32u - (uint32_t)__clz(L + 1);
+        const bool act = lane < steps;
+        const uint32_t a = act ? ((L + 1) >> lane) - 1 : 0u;
+        const uint32_t pp = a > 0 ? (a - 1) / 2 : 0u;
+        const float vp = __uint_as_float(top.s.ld(pp).y);
+        const float vq = __uint_as_float(top.s.ld(lane == 0 ? L - 1 : min(a + 1, L - 1)).y);
+        bool bad = false;
+        if (act && a > 0) {
+            if (lane == 0) bad = a == 2 * pp + 2 && !(vp > vq);
+            else bad = a == 2 * pp + 1 && a + 1 < L && !(vp > vq);
+        }
+        return __ballot(bad) == 0;
+    };
+    auto layout = [&](uint32_t pos) { return pos < top.len ? top.s.ld(pos).x : 0u; };
+    auto apply_runs = [&](uint32_t j) {
+        if (j == 0 || noop()) return;
+        if (top.len > 128) {  // the layouts are compared for heaps of at most 128
+            for (uint32_t t2 = 0; t2 < j && !noop(); ++t2) round_trip();
+            return;
+        }
+        uint32_t h0[RT_HIST], h1[RT_HIST];  // items at positions lane, lane + 64
+        h0[0] = layout(lane);
+        h1[0] = layout(lane + 64);
+        for (uint32_t t2 = 1; t2 <= j; ++t2) {
+            round_trip();
+            if (noop()) return;
+            const uint32_t c0 = layout(lane), c1 = layout(lane + 64);
+            const uint32_t seen = t2 < RT_HIST ? t2 : RT_HIST;
+            uint32_t prev = RT_HIST;
+            for (uint32_t q2 = 0; q2 < seen; ++q2)
+                if (prev == RT_HIST && __ballot(h0[q2] != c0 || h1[q2] != c1) == 0) prev = q2;
+            if (prev != RT_HIST) {
+                const uint32_t left = (j - t2) % (t2 - prev);
+                for (uint32_t u = 0; u < left; ++u) round_trip();
+                return;
+            }
+            if (t2 < RT_HIST) {
+                h0[t2] = c0;
+                h1[t2] = c1;
+            }
+        }
+    };
     const uint32_t np_q = np[t];
     for (uint32_t p = 0; p < np_q; ++p) {
         const uint2 *sq = seq + ((size_t)t * ldo + p) * k;
         const uint32_t m = seq_cnt[(size_t)t * ldo + p];
+        uint32_t first = 0;
+        if (k > 0 && top.len == k && m > 0) {  // the run above the root
+            const float root = __uint_as_float(top.s.ld(0).y);
+            uint32_t j = m;
+            for (uint32_t c0 = 0; c0 < m; c0 += 64) {
+                const bool in = c0 + lane < m;
+                const uint64_t live =
+                    __ballot(in && !(__uint_as_float(sq[min(c0 + lane, m - 1)].x) > root));
+                if (live) {
+                    j = c0 + (uint32_t)__builtin_ctzll(live);
+                    break;
+                }
+            }
+            apply_runs(j);
+            first = j;
+        }
         // 64 entries per load round, handed out through register reads
-        for (uint32_t j0 = 0; j0 < m; j0 += 64) {
+        for (uint32_t j0 = first; j0 < m; j0 += 64) {
             const uint2 v = j0 + tid < m ? sq[j0 + tid] : make_uint2(0u, 0u);
             const uint32_t cnt = min(64u, m - j0);
             for (uint32_t u = 0; u < cnt; ++u) {
