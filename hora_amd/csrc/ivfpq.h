@@ -115,7 +115,10 @@ void launch_ivf_replay(const uint32_t *rows, uint32_t nrows, const float *Dc, ui
 // simulated in parallel: order [nrows][nprobe] and np [nrows] hold the probe
 // order, seq [nrows][nprobe][k] / seq_cnt [nrows][nprobe] each list's
 // drained survivors.  Returns false (launching nothing) when the lists are
-// too long for LDS.
+// too long for LDS.  probes [rows][ldp]: the probe order of every row (the
+// learned policy, with nprobe_q), or, with coarse_flags, the coarse
+// select's sorted candidates, used for the rows without ROW_TIE there (the
+// others replay the coarse heap).
 bool launch_ivf_replay_lds(const uint32_t *rows, uint32_t nrows, const float *Dc, uint32_t ldd,
                            uint32_t nlist, uint32_t nprobe, const float *Qp, uint32_t dpad,
                            const float *C, const uint32_t *off, const uint32_t *ids,
@@ -124,7 +127,8 @@ bool launch_ivf_replay_lds(const uint32_t *rows, uint32_t nrows, const float *Dc
                            uint32_t k, uint32_t lmax, const uint32_t *probes, uint32_t ldp,
                            const uint32_t *nprobe_q, uint32_t *order, uint32_t *np,
                            uint2 *seq, uint32_t *seq_cnt, uint64_t *out_idx, float *out_dist,
-                           uint32_t *out_count, hipStream_t s);
+                           uint32_t *out_count, hipStream_t s,
+                           const uint32_t *coarse_flags = nullptr);
 // Learned-probe helpers.
 void launch_negate(const float *x, float *y, uint64_t n, hipStream_t s);
 void launch_probe_count(const float *negp, uint32_t ld, uint32_t rows, uint32_t pmax, float mass,

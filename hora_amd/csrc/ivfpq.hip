@@ -1356,14 +1356,17 @@ __global__ __launch_bounds__(64) void ivf_order_kernel(
     const uint32_t *__restrict__ rows, const float *__restrict__ Dc, uint32_t ldd, uint32_t nlist,
     uint32_t nprobe, const uint32_t *__restrict__ probes, uint32_t ldp,
     const uint32_t *__restrict__ nprobe_q, uint32_t ldo, uint32_t *__restrict__ order,
-    uint32_t *__restrict__ np_out)
+    uint32_t *__restrict__ np_out, const uint32_t *__restrict__ coarse_flags)
 {
     extern __shared__ __align__(16) unsigned char io_smem[];
     const uint32_t t = blockIdx.x, q = rows[t];
     const int tid = threadIdx.x;
     uint32_t *ord = order + (size_t)t * ldo;
-    if (probes) {
-        const uint32_t np_q = nprobe_q[q];
+    // The given probes (learned policy), or the coarse select's sorted
+    // candidates when the coarse check found no tie that can change the
+    // result (an empty list may swap places with another: it adds nothing)
+    if (probes && (!coarse_flags || !(coarse_flags[q] & ROW_TIE))) {
+        const uint32_t np_q = nprobe_q ? nprobe_q[q] : (nprobe < nlist ? nprobe : nlist);
         for (uint32_t p = tid; p < np_q; p += 64) ord[p] = probes[(size_t)q * ldp + p];
         if (tid == 0) np_out[t] = np_q;
         return;
@@ -2216,7 +2219,7 @@ bool launch_ivf_replay_lds(const uint32_t *rows, uint32_t nrows, const float *Dc
                            uint32_t k, uint32_t lmax, const uint32_t *probes, uint32_t ldp,
                            const uint32_t *nprobe_q, uint32_t *order, uint32_t *np,
                            uint2 *seq, uint32_t *seq_cnt, uint64_t *out_idx, float *out_dist,
-                           uint32_t *out_count, hipStream_t s)
+                           uint32_t *out_count, hipStream_t s, const uint32_t *coarse_flags)
 {
     if (nrows == 0) return true;
     if (nrows > 65535) return false;
@@ -2226,14 +2229,14 @@ bool launch_ivf_replay_lds(const uint32_t *rows, uint32_t nrows, const float *Dc
     hipFuncAttributes attr{};
     HM_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(ivf_list_replay_kernel)));
     const size_t lds = (size_t)((Pcap + 1) & ~1u) * 8 + ((size_t)Lcap + 2) * 8 + (size_t)M * KS * 4;
-    const size_t lds_order = probes ? 0 : ((size_t)nlist + 2) * 8;
+    const size_t lds_order = probes && !coarse_flags ? 0 : ((size_t)nlist + 2) * 8;
     if (lds + attr.sharedSizeBytes > 160 * 1024 || lds_order > 160 * 1024 ||
         (size_t)(k + 3) * 8 > 160 * 1024)
         return false;  // the caller uses the global-memory form
     ProfScope prof("ivf_replay", s);
     const uint32_t ldo = nprobe;
     ivf_order_kernel<<<nrows, 64, lds_order, s>>>(rows, Dc, ldd, nlist, nprobe, probes, ldp,
-                                                  nprobe_q, ldo, order, np);
+                                                  nprobe_q, ldo, order, np, coarse_flags);
     HM_CHECK_LAUNCH();
     ivf_list_replay_kernel<<<dim3(nprobe, nrows), RP_THREADS, lds, s>>>(
         rows, Qp, dpad, C, off, ids, codes, codebooks, sub_b, sub_e, M, KS, sdm, k, Lcap, Pcap,

@@ -618,11 +618,11 @@ class IVFPQ final : public Index {
                                   off_.as<uint32_t>(), ids_.as<uint32_t>(), codes_.as<uint8_t>(),
                                   codebooks_.as<float>(), sub_b_.as<uint32_t>(),
                                   sub_e_.as<uint32_t>(), M_, KS_, sdm_, k, max_list_,
-                                  learned ? probe_i_.as<uint32_t>() : nullptr, ldp,
+                                  probe_i_.as<uint32_t>(), ldp,
                                   learned ? nprobe_q_.as<uint32_t>() : nullptr,
                                   rp_order_.as<uint32_t>(), rp_np_.as<uint32_t>(),
                                   rp_seq_.as<uint2>(), rp_scnt_.as<uint32_t>(), d_idx, d_dist,
-                                  d_count, s)) {
+                                  d_count, s, learned ? nullptr : cflags_.as<uint32_t>())) {
             HM_HIP(hipStreamSynchronize(s));
             return;
         }
