@@ -346,13 +346,14 @@ struct LaneHeap {
 // lane, the next batch in flight while the current one is processed) and
 // hands over, in index order, the 64-value chunks that may hold a value not
 // above threshold(): on_chunk(first_index, this_lane's_value,
-// values_in_lds, count).  Chunks are filtered by register ballots per group
+// values_in_lds, count); the values passed over go to on_skip(count), in
+// the same order.  Chunks are filtered by register ballots per group
 // of 256 values (rows with a NaN never reach a replay).  Rows are 16-byte
 // aligned with a stride ldd >= n that is a multiple of 4, so the vector
 // loads (clamped to the last full vector of the row) stay inside the row.
-template <class Threshold, class OnChunk>
+template <class Threshold, class OnChunk, class OnSkip>
 __device__ void stream_row(const float *__restrict__ row, uint32_t n, uint64_t ldd, int lane,
-                           float *sv, Threshold &&threshold, OnChunk &&on_chunk)
+                           float *sv, Threshold &&threshold, OnChunk &&on_chunk, OnSkip &&on_skip)
 {
     constexpr uint32_t FB4 = 16, BATCH = FB4 * 256;
     const float inf = __int_as_float(0x7f800000);
@@ -370,9 +371,8 @@ __device__ void stream_row(const float *__restrict__ row, uint32_t n, uint64_t l
     HM_ROW_LOAD(0u)
     for (uint32_t b0 = 0; b0 < n; b0 += BATCH) {
         // Which 256-value groups hold a value not above the threshold (the
-        // root of a full, stable heap; +inf otherwise)?  The root only
-        // falls while the heap stays stable, so a group with none can be
-        // passed over later as long as the heap is still stable.
+        // root of a full heap; +inf otherwise)?  The root only falls while
+        // the heap is full, so a group with none can be passed over later.
         const float thr = threshold();
         uint32_t gmask = 0;
 #pragma clang loop unroll(full)
@@ -396,12 +396,19 @@ __device__ void stream_row(const float *__restrict__ row, uint32_t n, uint64_t l
         if (n - b0 > BATCH) {
             HM_ROW_LOAD(b0 + BATCH)
         }
-        if (!gmask) continue;
+        if (!gmask) {
+            on_skip(n - b0 < BATCH ? n - b0 : BATCH);
+            continue;
+        }
         for (uint32_t c = 0; c < BATCH / 64; ++c) {
             const uint32_t base = b0 + c * 64;
             if (base >= n) break;
-            if (!((gmask >> (c >> 2)) & 1u) && threshold() < inf) continue;
-            on_chunk(base, sv[c * 64 + lane], sv + c * 64, n - base < 64 ? n - base : 64u);
+            const uint32_t cnt = n - base < 64 ? n - base : 64u;
+            if (!((gmask >> (c >> 2)) & 1u) && threshold() < inf) {
+                on_skip(cnt);
+                continue;
+            }
+            on_chunk(base, sv[c * 64 + lane], sv + c * 64, cnt);
         }
     }
 #undef HM_ROW_LOAD
@@ -714,16 +721,59 @@ __global__ __launch_bounds__(64) void bf_replay_lds_kernel(
     bool stable = false;
     float root = inf;
     __shared__ __align__(16) float sv[4096];
-    auto threshold = [&]() { return (h.len == m && stable) ? root : inf; };
+    // Runs of elements above the root of the full heap: one layout map F
+    // applied j times (see bf_finalize_wave_kernel), until the layout is
+    // stable or repeats one of the last RT_HIST (kept for heaps of at most
+    // RT_LEN entries), then by the period.
+    constexpr uint32_t RT_HIST = 8, RT_LEN = 512;
+    __shared__ uint32_t rt_hist[RT_HIST][RT_LEN];
+    auto round_trip = [&]() {
+        h.push(inf, ~0u);  // above every (finite) entry
+        float a;
+        uint32_t b;
+        h.pop(&a, &b);
+    };
+    auto save = [&](uint32_t q) {
+        for (uint32_t p = lane; p < h.len; p += 64) rt_hist[q][p] = h.ix[p];
+    };
+    auto same_as = [&](uint32_t q) {
+        bool diff = false;
+        for (uint32_t p = lane; p < h.len; p += 64) diff = diff || rt_hist[q][p] != h.ix[p];
+        return __ballot(diff) == 0;
+    };
+    auto apply_runs = [&](uint64_t j) {
+        if (j == 0 || stable) return;
+        const bool track = h.len <= RT_LEN;
+        if (track) save(0);
+        for (uint64_t t = 1; t <= j; ++t) {
+            round_trip();
+            if (heap_noop_roundtrip(h.d, h.len)) {
+                stable = true;
+                return;
+            }
+            if (!track) continue;
+            const uint32_t seen = t < RT_HIST ? (uint32_t)t : RT_HIST;
+            for (uint32_t q = 0; q < seen; ++q)
+                if (same_as(q)) {
+                    const uint64_t left = (j - t) % (t - q);
+                    for (uint64_t u = 0; u < left; ++u) round_trip();
+                    return;
+                }
+            if (t < RT_HIST) save((uint32_t)t);
+        }
+    };
+    auto threshold = [&]() { return h.len == m ? root : inf; };
     stream_row(row, n, ldd, lane, sv, threshold,
                [&](uint32_t base, float v, const float *cv, uint32_t cnt) {
         uint32_t j = 0;
         while (j < cnt) {
-            if (h.len == m && stable) {
+            if (h.len == m) {
                 const unsigned long long live =
                     __ballot((uint32_t)lane >= j && (uint32_t)lane < cnt && !(v > root));
+                const uint32_t nxt = live ? (uint32_t)__builtin_ctzll(live) : cnt;
+                apply_runs(nxt - j);
                 if (live == 0) break;
-                j = (uint32_t)__builtin_ctzll(live);
+                j = nxt;
             }
             const float x = cv[j];
             h.push(x, base + j);
@@ -738,7 +788,7 @@ __global__ __launch_bounds__(64) void bf_replay_lds_kernel(
             }
             ++j;
         }
-    });
+    }, [&](uint32_t cnt) { apply_runs(cnt); });
     const uint32_t cntk = h.len;
     for (uint32_t p = cntk; p > 0; --p) {
         float a;

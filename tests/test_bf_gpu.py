@@ -101,7 +101,7 @@ def test_ties_replay_rows_past_4m(orc):
 
 
 @pytest.mark.parametrize("mt", [Metric.Angular, Metric.CosineSimilarity], ids=lambda m: m.name)
-@pytest.mark.parametrize("k", [10, 40, 63])
+@pytest.mark.parametrize("k", [10, 40, 63, 100, 200, 600])  # register heap to 63, LDS heap above
 def test_ties_inside_the_heap(orc, mt, k):
     # every direction four times (x, 2x, 4x, 8x: the same cosine to the
     # last bit), so a full heap holds equal distances and its layout is not
