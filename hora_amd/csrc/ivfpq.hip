@@ -1526,7 +1526,9 @@ __global__ __launch_bounds__(64) void ivf_merge_replay_kernel(
     auto layout = [&](uint32_t pos) { return pos < top.len ? top.s.ld(pos).x : 0u; };
     auto apply_runs = [&](uint32_t j) {
         if (j == 0 || noop()) return;
-        if (top.len > 128) {  // the layouts are compared for heaps of at most 128
+        // heaps of more than 128 (the layouts are compared up to 128), and
+        // short runs, whose round trips cost less than the layouts
+        if (top.len > 128 || j <= 2 * RT_HIST) {
             for (uint32_t t2 = 0; t2 < j && !noop(); ++t2) round_trip();
             return;
         }

@@ -426,6 +426,7 @@ __device__ void stream_row(const float *__restrict__ row, uint32_t n, uint64_t l
 constexpr uint32_t BF_BLOCK = 256;       // values per lane-wide load (a float4 per lane)
 constexpr uint32_t BF_THREADS = 512;  // bf_finalize_wave_kernel: the group minima by 8 waves, the heap by one
 constexpr uint32_t BF_GMIN_MAX = 16384;  // group minima held in LDS
+constexpr uint32_t BF_RT_LEN = 512;      // bf_replay_lds_kernel: heaps whose layouts are compared
 
 // Wave-wide minimum (every lane gets it).
 __device__ __forceinline__ float wave_min(float v)
@@ -546,6 +547,16 @@ __global__ __launch_bounds__(BF_THREADS) void bf_finalize_wave_kernel(
     };
     auto apply_runs = [&](uint64_t j) {
         if (j == 0 || stable) return;
+        if (j <= 2 * RT_HIST) {  // short runs: the round trips cost less than the layouts
+            for (uint64_t t = 0; t < j; ++t) {
+                round_trip();
+                if (noop()) {
+                    stable = true;
+                    return;
+                }
+            }
+            return;
+        }
         uint32_t hist[RT_HIST];
         hist[0] = h.it;
         uint64_t t = 0;
@@ -725,8 +736,10 @@ __global__ __launch_bounds__(64) void bf_replay_lds_kernel(
     // applied j times (see bf_finalize_wave_kernel), until the layout is
     // stable or repeats one of the last RT_HIST (kept for heaps of at most
     // RT_LEN entries), then by the period.
-    constexpr uint32_t RT_HIST = 8, RT_LEN = 512;
-    __shared__ uint32_t rt_hist[RT_HIST][RT_LEN];
+    constexpr uint32_t RT_HIST = 8;
+    const uint32_t RT_LEN = cap < BF_RT_LEN ? cap : BF_RT_LEN;
+    // [RT_HIST][RT_LEN] after the heap's arrays
+    uint32_t *rt = reinterpret_cast<uint32_t *>(rp_smem + (size_t)(cap + 1) * 8);
     auto round_trip = [&]() {
         h.push(inf, ~0u);  // above every (finite) entry
         float a;
@@ -734,16 +747,17 @@ __global__ __launch_bounds__(64) void bf_replay_lds_kernel(
         h.pop(&a, &b);
     };
     auto save = [&](uint32_t q) {
-        for (uint32_t p = lane; p < h.len; p += 64) rt_hist[q][p] = h.ix[p];
+        for (uint32_t p = lane; p < h.len; p += 64) rt[q * RT_LEN + p] = h.ix[p];
     };
     auto same_as = [&](uint32_t q) {
         bool diff = false;
-        for (uint32_t p = lane; p < h.len; p += 64) diff = diff || rt_hist[q][p] != h.ix[p];
+        for (uint32_t p = lane; p < h.len; p += 64) diff = diff || rt[q * RT_LEN + p] != h.ix[p];
         return __ballot(diff) == 0;
     };
     auto apply_runs = [&](uint64_t j) {
         if (j == 0 || stable) return;
-        const bool track = h.len <= RT_LEN;
+        // short runs: the round trips themselves cost less than the layouts
+        const bool track = h.len <= RT_LEN && j > 2 * RT_HIST;
         if (track) save(0);
         for (uint64_t t = 1; t <= j; ++t) {
             round_trip();
@@ -1007,7 +1021,8 @@ void launch_bf_finalize(const float *cand_d, const uint32_t *cand_i, uint32_t ld
                                                        bpg, out_idx, out_dist, out_count, err,
                                                        row_k, row_k_map);
     } else {
-        const size_t lds = (size_t)(cap + 1) * 8;  // <= 64 KiB
+        // the heap (<= 64 KiB) and the layouts its runs compare
+        const size_t lds = (size_t)(cap + 1) * 8 + (size_t)8 * std::min(cap, BF_RT_LEN) * 4;
         bf_replay_lds_kernel<<<rows, 64, lds, s>>>(cand_d, cand_i, ldc, flags, D, ldd, n, k,
                                                    out_idx, out_dist, out_count, err, row_k,
                                                    row_k_map);
