@@ -386,26 +386,30 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
     }
 
     // each wave's direct loads fill rows 8 (4 w + c) + l / 8 of both
-    // operands, from per-lane 64-bit row pointers
-    const uint8_t *qp[4], *bp[4];
+    // operands: per-lane 32-bit byte offsets from the tile's first rows,
+    // whose addresses are wave-uniform (rows past the end load the last row)
+    uint32_t qo[4], bo[4];
+    const uint8_t *Qt = Q, *Bt = B;
     auto set_rows = [&](uint32_t bt_, uint32_t qt_) __attribute__((always_inline)) {
         const uint32_t q0 = qt_ * SM, n0 = bt_ * SN;
+        Qt = Q + (size_t)q0 * row_bytes;
+        Bt = B + (size_t)n0 * row_bytes;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const uint32_t r = 8 * (4 * wave + c) + lane / 8;
             const uint32_t g = (uint32_t)(lane % 8) ^ swz(r);
-            const uint32_t qa = q0 + r < nq ? q0 + r : nq - 1, ba = n0 + r < n ? n0 + r : n - 1;
-            qp[c] = Q + (size_t)qa * row_bytes + 16 * g;
-            bp[c] = B + (size_t)ba * row_bytes + 16 * g;
+            const uint32_t qa = q0 + r < nq ? r : nq - 1 - q0, ba = n0 + r < n ? r : n - 1 - n0;
+            qo[c] = qa * row_bytes + 16 * g;
+            bo[c] = ba * row_bytes + 16 * g;
         }
     };
     auto issue = [&](auto bufc, uint32_t kb) __attribute__((always_inline)) {
         constexpr int buf = decltype(bufc)::value;
+        const uint8_t *Qk = Qt + kb, *Bk = Bt + kb;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const uint8_t *src_q = qp[c] + kb, *src_b = bp[c] + kb;
-            __builtin_amdgcn_global_load_lds(src_q, &(buf ? sA1 : sA0)[(4 * wave_s + c) * 1024], 16, 0, 0);
-            __builtin_amdgcn_global_load_lds(src_b, &(buf ? sB1 : sB0)[(4 * wave_s + c) * 1024], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(Qk + qo[c], &(buf ? sA1 : sA0)[(4 * wave_s + c) * 1024], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(Bk + bo[c], &(buf ? sB1 : sB0)[(4 * wave_s + c) * 1024], 16, 0, 0);
         }
     };
     // tid through an empty asm: the per-tile helpers below recompute their
@@ -415,12 +419,18 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
         asm volatile("" : "+v"(t));
         return t;
     };
+    // FP8 Euclidean keys are tested as halves (below): their threshold is
+    // stored as -rt/2.  A row past nq gets a threshold no key passes.
+    constexpr bool HALFKEY = ELEM == 1 && MET == HM_METRIC_EUCLIDEAN;
     auto load_ops = [&](uint32_t bt_, uint32_t qt_, float &o0, float &o1) __attribute__((always_inline)) {
         const uint32_t tid = opaque_tid();
         if (tid < SM) {
+            constexpr float PINF = __builtin_huge_valf();
             const uint32_t q = qt_ * SM + tid, qc = q < nq ? q : nq - 1;
             o0 = ELEM == 1 ? qscale[qc] : 1.0f;
-            o1 = tau[(size_t)qc * tau_ld];
+            const float rt = tau[(size_t)qc * tau_ld];
+            if constexpr (HALFKEY) o1 = q < nq ? -0.5f * rt : PINF;
+            else o1 = q < nq ? rt : -PINF;
         } else {
             const uint32_t c = bt_ * SN + (tid - SM), cc = c < n ? c : n - 1;
             if constexpr (MET == HM_METRIC_EUCLIDEAN) o0 = bnorm2[cc];
@@ -464,7 +474,6 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
     // t = s - cv/2 (cv/2 = +inf for a column past n), key = -2 t exactly,
     // and key < rt becomes t > -rt/2 (scaling by -2 and halving are exact),
     // one max over a lane's four columns and one compare per row.
-    constexpr bool HALFKEY = ELEM == 1 && MET == HM_METRIC_EUCLIDEAN;
     floatx4 acc[8][4];
 #pragma unroll
     for (int i = 0; i < 8; ++i)
@@ -488,79 +497,92 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
         }
     };
 
-    auto epilogue = [&](uint32_t bt_, uint32_t qt_, int par) __attribute__((always_inline)) {
+    // The epilogue runs inside the tile's last K step: the rows of query
+    // block i are tested while the MFMAs of block i + 1 are in the matrix
+    // core, so its VALU work overlaps the matrix work of both waves of the
+    // SIMD instead of leaving the matrix core idle.
+    // epi_cols: the column operands of the lane's four columns
+    auto epi_cols = [&](uint32_t bt_, int par, float (&cvv)[4], bool (&cok)[4]) __attribute__((always_inline)) {
         constexpr float PINF = __builtin_huge_valf();
         const uint32_t lane = opaque_tid() & 63u;
-        const uint32_t lr = lane & 15u, lk = lane >> 4;
-        const uint32_t q0 = qt_ * SM, n0 = bt_ * SN;
-        const uint32_t col0 = n0 + wc * 64 + lr;
-        float cvv[4], cols[4];
-        bool cok[4];
+        const uint32_t lr = lane & 15u;
+        const uint32_t n0 = bt_ * SN;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const uint32_t cl = wc * 64 + j * 16 + lr;
             cok[j] = n0 + cl < n;
-            cols[j] = s_cs[par][cl];
             cvv[j] = (MET == HM_METRIC_EUCLIDEAN && !cok[j]) ? PINF : s_cv[par][cl];
             if constexpr (HALFKEY) cvv[j] = 0.5f * cvv[j];
         }
-        // the thresholds of the lane's 32 rows, one 16-byte read per block
-        float4 rt4[8];
+    };
+    // row_keys: the lane's four keys of row e of query block i against the
+    // row's stored threshold rt; true when one passes
+    auto row_keys = [&](auto ic, int e, float rt, const float (&cvv)[4], const bool (&cok)[4],
+                        float (&key)[4]) __attribute__((always_inline)) -> bool {
+        constexpr int i = decltype(ic)::value;
+        constexpr float PINF = __builtin_huge_valf();
+        if constexpr (HALFKEY) {
+            // key < rt  <=>  t > -rt/2 (the stored threshold); fmaxf drops a
+            // NaN, whose key never compares below rt anyway
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
-            rt4[i] = *reinterpret_cast<const float4 *>(&s_rt[par][wr * 128 + i * 16 + lk * 4]);
+            for (int j = 0; j < 4; ++j) key[j] = acc[i][j][e] - cvv[j];  // t (cvv = cv/2)
+            return fmaxf(fmaxf(key[0], key[1]), fmaxf(key[2], key[3])) > rt;
+        } else {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const float rtv[4] = {rt4[i].x, rt4[i].y, rt4[i].z, rt4[i].w};
+            for (int j = 0; j < 4; ++j) {
+                const float a = acc[i][j][e];
+                if constexpr (MET == HM_METRIC_EUCLIDEAN) {  // BF16: cv - 2 s
+                    key[j] = cvv[j] - 2.0f * a;
+                } else {  // FP8: acc is s itself (scales applied by the MFMA)
+                    float kv;
+                    if constexpr (MET == HM_METRIC_DOT_PRODUCT) kv = a;
+                    else if constexpr (MET == HM_METRIC_COSINE_SIMILARITY) kv = -a * cvv[j];
+                    else kv = a * cvv[j];
+                    key[j] = cok[j] ? kv : PINF;
+                }
+            }
+            // fminf drops a NaN key, which never compares below rt anyway
+            return fminf(fminf(key[0], key[1]), fminf(key[2], key[3])) < rt;
+        }
+    };
+    // the four rows' thresholds of query block i held by the lane (rows past
+    // nq: no key passes), one 16-byte read
+    auto block_rt = [&](auto ic, int par, uint32_t lk) __attribute__((always_inline)) -> float4 {
+        constexpr int i = decltype(ic)::value;
+        return *reinterpret_cast<const float4 *>(&s_rt[par][wr * 128 + i * 16 + lk * 4]);
+    };
+    // epi_block: the records of the four rows of query block i held by the
+    // lane
+    auto epi_block = [&](auto ic, uint32_t bt_, uint32_t qt_, int par, const float (&cvv)[4],
+                         const bool (&cok)[4]) __attribute__((always_inline)) {
+        constexpr int i = decltype(ic)::value;
+        const uint32_t lane = opaque_tid() & 63u;
+        const uint32_t lr = lane & 15u, lk = lane >> 4;
+        const uint32_t col0 = bt_ * SN + wc * 64 + lr;
+        const float4 t4 = block_rt(ic, par, lk);
+        const float rtv[4] = {t4.x, t4.y, t4.z, t4.w};
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const uint32_t ql = wr * 128 + i * 16 + lk * 4 + e;
-                const uint32_t q = q0 + ql;
-                const float rt = q < nq ? rtv[e] : -PINF;
-                float key[4];
-                bool hit;
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t q = qt_ * SM + wr * 128 + i * 16 + lk * 4 + e;
+            float key[4];
+            const bool hit = row_keys(ic, e, rtv[e], cvv, cok, key);
+            const uint64_t bal = __ballot(hit);
+            if (bal) {
                 if constexpr (HALFKEY) {
-                    // key < rt  <=>  t > -rt/2; fmaxf drops a NaN, whose key
-                    // never compares below rt anyway
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) key[j] = acc[i][j][e] - cvv[j];  // t (cvv = cv/2)
-                    hit = fmaxf(fmaxf(key[0], key[1]), fmaxf(key[2], key[3])) > -0.5f * rt;
-                } else {
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const float a = acc[i][j][e];
-                        if constexpr (MET == HM_METRIC_EUCLIDEAN) {  // BF16: cv - 2 s
-                            key[j] = cvv[j] - 2.0f * a;
-                        } else {  // FP8: acc is s itself (scales applied by the MFMA)
-                            float kv;
-                            if constexpr (MET == HM_METRIC_DOT_PRODUCT) kv = a;
-                            else if constexpr (MET == HM_METRIC_COSINE_SIMILARITY) kv = -a * cvv[j];
-                            else kv = a * cvv[j];
-                            key[j] = cok[j] ? kv : PINF;
-                        }
-                    }
-                    // fminf drops a NaN key, which never compares below rt anyway
-                    hit = fminf(fminf(key[0], key[1]), fminf(key[2], key[3])) < rt;
+                    for (int j = 0; j < 4; ++j) key[j] = -2.0f * key[j];  // cv - 2 s, exactly
                 }
-                const uint64_t bal = __ballot(hit);
-                if (bal) {
-                    if constexpr (HALFKEY) {
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) key[j] = -2.0f * key[j];  // cv - 2 s, exactly
+                if (hit) {
+                    const uint32_t pos = rec_n + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                    if (pos < rec.R) {
+                        rk[pos] = make_float4(key[0], key[1], key[2], key[3]);
+                        rm[pos] = make_uint2(q, col0);
+                    } else {
+                        atomicAdd(cnt + q, cap + 1);  // mark the list overflowed
                     }
-                    if (hit) {
-                        const uint32_t pos =
-                            rec_n + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
-                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                        if (pos < rec.R) {
-                            rk[pos] = make_float4(key[0], key[1], key[2], key[3]);
-                            rm[pos] = make_uint2(q, col0);
-                        } else {
-                            atomicAdd(cnt + q, cap + 1);  // mark the list overflowed
-                        }
-                    }
-                    rec_n += (uint32_t)__popcll(bal);
                 }
+                rec_n += (uint32_t)__popcll(bal);
             }
         }
     };
@@ -605,43 +627,54 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
         uint4 avb[2][2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) avb[0][h] = *reinterpret_cast<const uint4 *>(As + fa[0][h]);
-        static_for<0, 8>([&](auto ic) __attribute__((always_inline)) {
-            constexpr int i = decltype(ic)::value;
-            if (i + 1 < 8) {
+        // the MFMAs of the step, block by block; in the tile's last step each
+        // block's rows are tested after the next block's MFMAs
+        const bool last = k + 1 == nk;
+        float cvv[4] = {0.f, 0.f, 0.f, 0.f};
+        bool cok[4] = {false, false, false, false};
+        if (last) epi_cols(bt, par, cvv, cok);
+        {
+            static_for<0, 8>([&](auto ic) __attribute__((always_inline)) {
+                constexpr int i = decltype(ic)::value;
+                if (i + 1 < 8) {
 #pragma unroll
-                for (int h = 0; h < 2; ++h)
-                    avb[(i + 1) & 1][h] = *reinterpret_cast<const uint4 *>(
-                        As + fa[(i + 1) & 3][h] + ((i + 1) >> 2) * 64 * ROWB);
-            }
-            const uint4 *av = avb[i & 1];
-            if constexpr (ELEM == 2) {
-                // two MFMA K=32 per step; lane holds 8 bf16 at k = 32*kk + 8*lk
+                    for (int h = 0; h < 2; ++h)
+                        avb[(i + 1) & 1][h] = *reinterpret_cast<const uint4 *>(
+                            As + fa[(i + 1) & 3][h] + ((i + 1) >> 2) * 64 * ROWB);
+                }
+                const uint4 *av = avb[i & 1];
+                if constexpr (ELEM == 2) {
+                    // two MFMA K=32 per step; lane holds 8 bf16 at k = 32*kk + 8*lk
 #pragma unroll
-                for (int kk = 0; kk < 2; ++kk)
+                    for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            __builtin_bit_cast(bf16x8, av[kk]), __builtin_bit_cast(bf16x8, bv[j][kk]),
-                            acc[i][j], 0, 0, 0);
-            } else {
-                // one MFMA K=128 (e4m3 x e4m3); lane holds 32 fp8 at k = 32*lk
-                // in A and B alike
-                const i32x8 a = i32x8{(int)av[0].x, (int)av[0].y, (int)av[0].z, (int)av[0].w,
-                                      (int)av[1].x, (int)av[1].y, (int)av[1].z, (int)av[1].w};
-                static_for<0, 4>([&](auto jc) __attribute__((always_inline)) {
-                    constexpr int j = decltype(jc)::value;
-                    const i32x8 b = i32x8{(int)bv[j][0].x, (int)bv[j][0].y, (int)bv[j][0].z,
-                                          (int)bv[j][0].w, (int)bv[j][1].x, (int)bv[j][1].y,
-                                          (int)bv[j][1].z, (int)bv[j][1].w};
-                    // the rows' scales applied by the matrix core (E8M0 bytes
-                    // i & 3 of sa_pk[i >> 2] and j of sb_pk)
-                    acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-                        a, b, acc[i][j], 0, 0, i & 3, (int)sa_pk[i >> 2], j, (int)sb_pk);
-                });
-            }
-        });
+                        for (int j = 0; j < 4; ++j)
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                __builtin_bit_cast(bf16x8, av[kk]), __builtin_bit_cast(bf16x8, bv[j][kk]),
+                                acc[i][j], 0, 0, 0);
+                } else {
+                    // one MFMA K=128 (e4m3 x e4m3); lane holds 32 fp8 at k = 32*lk
+                    // in A and B alike
+                    const i32x8 a = i32x8{(int)av[0].x, (int)av[0].y, (int)av[0].z, (int)av[0].w,
+                                          (int)av[1].x, (int)av[1].y, (int)av[1].z, (int)av[1].w};
+                    static_for<0, 4>([&](auto jc) __attribute__((always_inline)) {
+                        constexpr int j = decltype(jc)::value;
+                        const i32x8 b = i32x8{(int)bv[j][0].x, (int)bv[j][0].y, (int)bv[j][0].z,
+                                              (int)bv[j][0].w, (int)bv[j][1].x, (int)bv[j][1].y,
+                                              (int)bv[j][1].z, (int)bv[j][1].w};
+                        // the rows' scales applied by the matrix core (E8M0 bytes
+                        // i & 3 of sa_pk[i >> 2] and j of sb_pk)
+                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                            a, b, acc[i][j], 0, 0, i & 3, (int)sa_pk[i >> 2], j, (int)sb_pk);
+                    });
+                }
+                // the records of the previous block, whose MFMAs are done
+                if constexpr (i > 0)
+                    if (last) epi_block(std::integral_constant<int, i - 1>{}, bt, qt, par, cvv, cok);
+            });
+            if (last) epi_block(std::integral_constant<int, 7>{}, bt, qt, par, cvv, cok);
+        }
         if (++k < nk) return false;
-        epilogue(bt, qt, par);
         if (NL >= T) return true;
         // the next tile: its operands go to the other half of the operand
         // arrays (the previous tile's epilogue, which read them, is done:
