@@ -425,12 +425,9 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
     auto load_ops = [&](uint32_t bt_, uint32_t qt_, float &o0, float &o1) __attribute__((always_inline)) {
         const uint32_t tid = opaque_tid();
         if (tid < SM) {
-            constexpr float PINF = __builtin_huge_valf();
             const uint32_t q = qt_ * SM + tid, qc = q < nq ? q : nq - 1;
             o0 = ELEM == 1 ? qscale[qc] : 1.0f;
-            const float rt = tau[(size_t)qc * tau_ld];
-            if constexpr (HALFKEY) o1 = q < nq ? -0.5f * rt : PINF;
-            else o1 = q < nq ? rt : -PINF;
+            o1 = tau[(size_t)qc * tau_ld];  // transformed when stored: its load is not waited on here
         } else {
             const uint32_t c = bt_ * SN + (tid - SM), cc = c < n ? c : n - 1;
             if constexpr (MET == HM_METRIC_EUCLIDEAN) o0 = bnorm2[cc];
@@ -439,11 +436,14 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
             o1 = ELEM == 1 ? bscale[cc] : 1.0f;
         }
     };
-    auto store_ops = [&](int par, float o0, float o1) __attribute__((always_inline)) {
+    auto store_ops = [&](int par, uint32_t qt_, float o0, float o1) __attribute__((always_inline)) {
         const uint32_t tid = opaque_tid();
         if (tid < SM) {
+            constexpr float PINF = __builtin_huge_valf();
+            const bool qok = qt_ * SM + tid < nq;
             s_rs[par][tid] = o0;
-            s_rt[par][tid] = o1;
+            if constexpr (HALFKEY) s_rt[par][tid] = qok ? -0.5f * o1 : PINF;
+            else s_rt[par][tid] = qok ? o1 : -PINF;
         } else {
             s_cv[par][tid - SM] = o0;
             s_cs[par][tid - SM] = o1;
@@ -592,7 +592,7 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
     {
         float o0, o1;
         load_ops(bt, qt, o0, o1);
-        store_ops(0, o0, o1);
+        store_ops(0, qt, o0, o1);
     }
     uint32_t nbt = 0, nqt = 0;
     uint32_t NL = next_tile(L + P, nbt, nqt);
@@ -679,7 +679,7 @@ __global__ __launch_bounds__(512, 1) void score_scan_kernel(
         // the next tile: its operands go to the other half of the operand
         // arrays (the previous tile's epilogue, which read them, is done:
         // every step of this tile had a barrier)
-        store_ops(par ^ 1, p0, p1);
+        store_ops(par ^ 1, nqt, p0, p1);
         par ^= 1;
         bt = nbt;
         qt = nqt;
