@@ -890,8 +890,8 @@ def main_copy(args):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (sharding is "
-              "described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (each GPU would "
+              "hold its own shard of the base); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch  # noqa: F401  (load torch's HIP runtime before the engine's library)
 
